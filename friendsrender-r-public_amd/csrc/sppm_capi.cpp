@@ -1,0 +1,882 @@
+// sppm_capi.cpp -- C ABI of the SPPM pass core (include/sppm_amd.h).
+//
+// Host side of the drop-in boundary.  Scene setup mirrors Scene::CreateScene
+// (Scene.cpp:13-218) after the OBJ has been read: facet records, light
+// registration in facet order (Scene.cpp:145-154), textured-light tables
+// (LightSampler.cpp:30-66) and the light pick table (LightSampler.cpp:6-28)
+// are built here; the closest-hit structure is built on the GPU.  All per-pass
+// work runs in hand-written HIP kernels; this file only allocates, uploads and
+// launches.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sppm_amd.h"
+#include "sppm_host_math.h"
+#include "sppm_kernels.h"
+#include "sppm_pass.h"
+
+using namespace sppm;
+namespace H = sppm_host;
+
+static thread_local std::string g_err;
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPOK(x)                                                                                     \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(SPPM_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_));  \
+    } while (0)
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * n);
+    if (e != hipSuccess) return fail(SPPM_E_NOMEM, "hipMalloc(%zu bytes): %s", sizeof(T) * n, hipGetErrorString(e));
+    return 0;
+}
+template <typename T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+struct SceneBuffers {
+    bool ready = false;
+    BvhResult bvh;
+    float* facet_nrm = nullptr;
+    float2* facet_uv = nullptr;
+    int* facet_attr = nullptr;
+    DevAttr* attrs = nullptr;
+    TexDesc* tex = nullptr;
+    float4* texels = nullptr;
+    float* lpick_prob = nullptr;
+    int* lpick_left = nullptr;
+    int* lpick_right = nullptr;
+    DevLight* lights = nullptr;
+    float4* ltab_uvw = nullptr;
+    float4* ltab_L = nullptr;
+    float* ltab_prob = nullptr;
+    int* ltab_left = nullptr;
+    int* ltab_right = nullptr;
+    SceneDev dev{};
+    float bmin[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+    int stack_cap = 0;
+    void release() {
+        free_bvh(&bvh);
+        dfree(facet_nrm); dfree(facet_uv); dfree(facet_attr); dfree(attrs); dfree(tex); dfree(texels);
+        dfree(lpick_prob); dfree(lpick_left); dfree(lpick_right); dfree(lights);
+        dfree(ltab_uvw); dfree(ltab_L); dfree(ltab_prob); dfree(ltab_left); dfree(ltab_right);
+        ready = false;
+    }
+};
+
+struct RenderBuffers {
+    bool ready = false;
+    int N = 0, W = 0, H = 0, np = 0;
+    SlotsDev slots{};
+    int* off = nullptr;
+    int* nstored = nullptr;   // device scalar
+    int* scan_tmp = nullptr;
+    int cell_cap = 0;
+    int* cell_count = nullptr;
+    int* cell_start = nullptr;
+    int* cell_scan_tmp = nullptr;
+    int2* keyrank = nullptr;
+    float4 *gpos = nullptr, *gflux = nullptr, *gdir = nullptr;
+    int photon_cap = 0;
+    GridParams* gp = nullptr;
+    HitDev hits{};
+    SharedDev shared{};
+    int* counts = nullptr;
+    int* list_off = nullptr;
+    int* lists = nullptr;
+    long long lists_cap = 0;
+    uint32_t* words = nullptr;
+    uint32_t* cam_mt = nullptr;
+    int* cam_pos = nullptr;
+    uint32_t* rmax_bits = nullptr;
+    uint32_t* flags = nullptr;
+    float4* image = nullptr;
+    bool have_lists = false;
+    void release() {
+        dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
+        dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_count); dfree(cell_start); dfree(cell_scan_tmp);
+        dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
+        dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
+        dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(lists);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rmax_bits); dfree(flags); dfree(image);
+        ready = false;
+    }
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms = 0.0;
+    long long n = 0;
+};
+
+struct sppm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    SceneBuffers scene;
+    RenderBuffers rb;
+    sppm_camera_desc cam{};
+    bool have_cam = false;
+    CamDev camdev{};
+    sppm_params params{};
+    bool have_params = false;
+    int pass = 0;
+    Timer t_photons, t_build, t_eye, t_gather;
+};
+
+extern "C" {
+
+int sppm_abi_version(void) { return SPPM_ABI_VERSION; }
+const char* sppm_last_error(void) { return g_err.c_str(); }
+
+int sppm_create(sppm_ctx** out, int device, void* stream) {
+    if (!out) return fail(SPPM_E_ARG, "out is NULL");
+    int ndev = 0;
+    HIPOK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SPPM_E_ARG, "device %d out of range (%d devices)", device, ndev);
+    HIPOK(hipSetDevice(device));
+    sppm_ctx* c = new sppm_ctx();
+    c->device = device;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return fail(SPPM_E_HIP, "hipStreamCreate failed");
+        }
+        c->own_stream = true;
+    }
+    HIPOK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPOK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPOK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) {
+        HIPOK(hipEventCreate(&t->a));
+        HIPOK(hipEventCreate(&t->b));
+    }
+    c->params.alpha = 0.9f;
+    c->params.first_radius = 0.015f;
+    c->params.photons_per_pass = 0;
+    c->params.bvh_width = 8;
+    *out = c;
+    return SPPM_OK;
+}
+
+int sppm_destroy(sppm_ctx* c) {
+    if (!c) return SPPM_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
+    c->rb.release();
+    c->scene.release();
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) {
+        if (t->a) (void)hipEventDestroy(t->a);
+        if (t->b) (void)hipEventDestroy(t->b);
+    }
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_join);
+    (void)hipStreamDestroy(c->stream2);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scene upload
+// ---------------------------------------------------------------------------
+int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width) {
+    if (!c || !d) return fail(SPPM_E_ARG, "NULL argument");
+    if (bvh_width != 2 && bvh_width != 4 && bvh_width != 8) return fail(SPPM_E_ARG, "bvh_width must be 2, 4 or 8");
+    const int nf = d->nfacets;
+    if (nf <= 0 || !d->pos || !d->nrm || !d->uv || !d->attr) return fail(SPPM_E_SCENE, "empty or incomplete facet arrays");
+    if (nf >= (1 << 27)) return fail(SPPM_E_SCENE, "too many facets (%d)", nf);
+    if (d->nattrs <= 0 || !d->attrs) return fail(SPPM_E_SCENE, "no attributes");
+    for (int f = 0; f < nf; ++f)
+        if (d->attr[f] < 0 || d->attr[f] >= d->nattrs)
+            return fail(SPPM_E_SCENE, "facet %d has attribute %d outside [0,%d)", f, d->attr[f], d->nattrs);
+    for (int a = 0; a < d->nattrs; ++a) {
+        const sppm_attr& A = d->attrs[a];
+        if (A.kind < SPPM_MAT_DIFFUSE || A.kind > SPPM_MAT_GLASS) return fail(SPPM_E_SCENE, "attribute %d: unknown kind %d", a, A.kind);
+        for (int t : {A.tex_albedo, A.tex_roughness, A.tex_bump})
+            if (t < -1 || t >= d->ntextures) return fail(SPPM_E_SCENE, "attribute %d: texture id %d out of range", a, t);
+    }
+    for (int t = 0; t < d->ntextures; ++t)
+        if (d->textures[t].width <= 0 || d->textures[t].height <= 0 || !d->textures[t].rgba)
+            return fail(SPPM_E_SCENE, "texture %d is empty", t);
+    HIPOK(hipSetDevice(c->device));
+    HIPOK(hipStreamSynchronize(c->stream));
+    c->scene.release();
+    SceneBuffers& S = c->scene;
+    const hipStream_t st = c->stream;
+
+    // ---- lights, in facet order (Scene.cpp:145-154) ----
+    std::vector<DevLight> lights;
+    std::vector<float4> tuvw, tL;
+    std::vector<float> tprob;
+    std::vector<int> tleft, tright;
+    for (int f = 0; f < nf; ++f) {
+        const sppm_attr& A = d->attrs[d->attr[f]];
+        H::V3 rad = H::ld(A.radiance);
+        if (!(rad.x + rad.y + rad.z > 0.00001f)) continue;
+        const float* P = d->pos + 9 * (size_t)f;
+        H::V3 p0 = H::ld(P), p1 = H::ld(P + 3), p2 = H::ld(P + 6);
+        float area = 0.5f * H::norm(H::cross(p1 - p0, p2 - p0));
+        const float* Nn = d->nrm + 9 * (size_t)f;
+        H::V3 nrm = H::normalized((H::ld(Nn) + H::ld(Nn + 3)) + H::ld(Nn + 6));
+        DevLight L{};
+        L.facet = f;
+        L.area = area;
+        L.normal[0] = nrm.x; L.normal[1] = nrm.y; L.normal[2] = nrm.z;
+        memcpy(L.p, P, sizeof(float) * 9);
+        H::V3 flux;
+        if (A.tex_albedo >= 0) {  // LightSampler::CreateAreaLight, textured branch
+            const sppm_texture& T = d->textures[A.tex_albedo];
+            const float* UV = d->uv + 9 * (size_t)f;
+            const int NS = 1000;
+            std::vector<H::V3> uvw(NS), Lv(NS);
+            H::V3 total = H::v3(0, 0, 0);
+            for (int i = 0; i < NS; ++i) {
+                float w0 = (float)i / (float)NS, w1 = H::halton(2, i), w2 = H::halton(3, i);
+                if (w1 > w0) { w0 *= 0.5f; w1 -= w0; }
+                else { w1 *= 0.5f; w0 -= w1; }
+                H::V3 tuv = (H::ld(UV) * w0 + H::ld(UV + 3) * w1) + H::ld(UV + 6) * (1.0f - w0 - w1);
+                float col[4];
+                H::bilinear(T.rgba, T.width, T.height, tuv.x, tuv.y, col);
+                H::V3 l = H::v3(col[0], col[1], col[2]) * rad;
+                if (H::norm2(l) < 0.01f) l = H::v3(0, 0, 0);
+                uvw[i] = H::v3(w0, w1, w2);
+                Lv[i] = l;
+                total = total + l;
+            }
+            if ((double)H::norm2(total) < 1e-6) {
+                Lv[0] = H::v3(1, 1, 1) * (float)1e-6;
+                uvw[0] = H::v3(0, 0, 0);
+            } else {
+                total = total * (2.0f * H::PI * area / (float)NS);
+            }
+            flux = total;
+            std::vector<float> vals(NS), prob;
+            std::vector<int> left, right;
+            for (int i = 0; i < NS; ++i) vals[i] = H::norm(Lv[i]);
+            float avg = 0.0f;
+            H::build_alias(vals, prob, left, right, avg);
+            L.textured = 1;
+            L.tab_off = (int)tprob.size();
+            L.tab_avg = avg;
+            for (int i = 0; i < NS; ++i) {
+                tuvw.push_back(make_float4(uvw[i].x, uvw[i].y, uvw[i].z, vals[i]));
+                tL.push_back(make_float4(Lv[i].x, Lv[i].y, Lv[i].z, 0.0f));
+                tprob.push_back(prob[i]);
+                tleft.push_back(left[i]);
+                tright.push_back(right[i]);
+            }
+        } else {
+            flux = rad * (2.0f * H::PI * area);
+            L.textured = 0;
+            L.tab_off = 0;
+            L.tab_avg = 1.0f;
+        }
+        L.flux[0] = flux.x; L.flux[1] = flux.y; L.flux[2] = flux.z;
+        L.flux_norm = H::norm(flux);
+        lights.push_back(L);
+    }
+    if (lights.empty()) return fail(SPPM_E_SCENE, "scene has no emitting facet (Radiance sum > 1e-5)");
+    std::vector<float> lvals(lights.size()), lprob;
+    std::vector<int> lleft, lright;
+    for (size_t i = 0; i < lights.size(); ++i) lvals[i] = lights[i].flux_norm;
+    float lavg = 0.0f;
+    H::build_alias(lvals, lprob, lleft, lright, lavg);
+
+    // ---- attributes ----
+    std::vector<DevAttr> attrs(d->nattrs);
+    for (int a = 0; a < d->nattrs; ++a) {
+        const sppm_attr& A = d->attrs[a];
+        DevAttr& D = attrs[a];
+        D.kind = A.kind; D.index = A.index; D.opaque = A.if_opaque ? 1 : 0; D.is_sppm = A.is_sppm ? 1 : 0;
+        for (int k = 0; k < 3; ++k) { D.albedo[k] = A.albedo[k]; D.radiance[k] = A.radiance[k]; }
+        D.rough = A.roughness; D.refl = A.reflectance; D.absorb = A.extinction;
+        D.tex_albedo = A.tex_albedo; D.tex_rough = A.tex_roughness; D.tex_bump = A.tex_bump;
+        D.radiance_norm2 = H::norm2(H::ld(A.radiance));
+    }
+    // ---- textures ----
+    std::vector<TexDesc> texd(std::max(1, d->ntextures));
+    long long ntexel = 0;
+    for (int t = 0; t < d->ntextures; ++t) {
+        texd[t].width = d->textures[t].width;
+        texd[t].height = d->textures[t].height;
+        texd[t].offset = ntexel;
+        ntexel += (long long)d->textures[t].width * d->textures[t].height;
+    }
+    // ---- facets ----
+    std::vector<float2> uv2(3 * (size_t)nf);
+    for (size_t f = 0; f < (size_t)nf; ++f)
+        for (int k = 0; k < 3; ++k) uv2[3 * f + k] = make_float2(d->uv[9 * f + 3 * k], d->uv[9 * f + 3 * k + 1]);
+    // scene box for the photon grid
+    float bmin[3] = {3e38f, 3e38f, 3e38f}, bmax[3] = {-3e38f, -3e38f, -3e38f};
+    for (size_t i = 0; i < (size_t)nf * 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float x = d->pos[3 * i + k];
+            bmin[k] = std::min(bmin[k], x);
+            bmax[k] = std::max(bmax[k], x);
+        }
+    for (int k = 0; k < 3; ++k) {
+        S.bmin[k] = bmin[k] - 0.01f;
+        S.ext[k] = (bmax[k] - bmin[k]) + 0.02f;
+    }
+
+    int rc = 0;
+    float* dpos = nullptr;
+    if ((rc = dalloc(&dpos, 9 * (size_t)nf))) return rc;
+    if ((rc = dalloc(&S.facet_nrm, 9 * (size_t)nf)) || (rc = dalloc(&S.facet_uv, 3 * (size_t)nf)) ||
+        (rc = dalloc(&S.facet_attr, nf)) || (rc = dalloc(&S.attrs, attrs.size())) || (rc = dalloc(&S.tex, texd.size())) ||
+        (rc = dalloc(&S.texels, (size_t)std::max(1LL, ntexel))) || (rc = dalloc(&S.lpick_prob, lprob.size())) ||
+        (rc = dalloc(&S.lpick_left, lleft.size())) || (rc = dalloc(&S.lpick_right, lright.size())) ||
+        (rc = dalloc(&S.lights, lights.size())) || (rc = dalloc(&S.ltab_uvw, std::max<size_t>(1, tuvw.size()))) ||
+        (rc = dalloc(&S.ltab_L, std::max<size_t>(1, tL.size()))) || (rc = dalloc(&S.ltab_prob, std::max<size_t>(1, tprob.size()))) ||
+        (rc = dalloc(&S.ltab_left, std::max<size_t>(1, tleft.size()))) || (rc = dalloc(&S.ltab_right, std::max<size_t>(1, tright.size())))) {
+        dfree(dpos);
+        S.release();
+        return rc;
+    }
+    HIPOK(hipMemcpyAsync(dpos, d->pos, sizeof(float) * 9 * nf, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.facet_nrm, d->nrm, sizeof(float) * 9 * nf, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.facet_uv, uv2.data(), sizeof(float2) * 3 * nf, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.facet_attr, d->attr, sizeof(int) * nf, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.attrs, attrs.data(), sizeof(DevAttr) * attrs.size(), hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.tex, texd.data(), sizeof(TexDesc) * texd.size(), hipMemcpyHostToDevice, st));
+    for (int t = 0; t < d->ntextures; ++t)
+        HIPOK(hipMemcpyAsync(S.texels + texd[t].offset, d->textures[t].rgba,
+                             sizeof(float4) * (size_t)texd[t].width * texd[t].height, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.lpick_prob, lprob.data(), sizeof(float) * lprob.size(), hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.lpick_left, lleft.data(), sizeof(int) * lleft.size(), hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.lpick_right, lright.data(), sizeof(int) * lright.size(), hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
+    if (!tuvw.empty()) {
+        HIPOK(hipMemcpyAsync(S.ltab_uvw, tuvw.data(), sizeof(float4) * tuvw.size(), hipMemcpyHostToDevice, st));
+        HIPOK(hipMemcpyAsync(S.ltab_L, tL.data(), sizeof(float4) * tL.size(), hipMemcpyHostToDevice, st));
+        HIPOK(hipMemcpyAsync(S.ltab_prob, tprob.data(), sizeof(float) * tprob.size(), hipMemcpyHostToDevice, st));
+        HIPOK(hipMemcpyAsync(S.ltab_left, tleft.data(), sizeof(int) * tleft.size(), hipMemcpyHostToDevice, st));
+        HIPOK(hipMemcpyAsync(S.ltab_right, tright.data(), sizeof(int) * tright.size(), hipMemcpyHostToDevice, st));
+    }
+    char err[256] = {0};
+    if (build_bvh(dpos, nf, bvh_width, st, &S.bvh, err, sizeof err) != 0) {
+        dfree(dpos);
+        S.release();
+        return fail(SPPM_E_HIP, "BVH build failed: %s", err);
+    }
+    HIPOK(hipStreamSynchronize(st));
+    dfree(dpos);
+
+    SceneDev& D = S.dev;
+    D.width = bvh_width;
+    D.nnodes = S.bvh.nnodes;
+    D.nodes = S.bvh.nodes;
+    D.ntri = nf;
+    D.tri = S.bvh.tri;
+    D.nfacets = nf;
+    D.facet_nrm = S.facet_nrm;
+    D.facet_uv = S.facet_uv;
+    D.facet_attr = S.facet_attr;
+    D.attrs = S.attrs;
+    D.tex = S.tex;
+    D.texels = S.texels;
+    D.nlights = (int)lights.size();
+    D.lpick_avg = lavg;
+    D.lpick_prob = S.lpick_prob;
+    D.lpick_left = S.lpick_left;
+    D.lpick_right = S.lpick_right;
+    D.lights = S.lights;
+    D.ltab_uvw = S.ltab_uvw;
+    D.ltab_L = S.ltab_L;
+    D.ltab_prob = S.ltab_prob;
+    D.ltab_left = S.ltab_left;
+    D.ltab_right = S.ltab_right;
+    // traversal stack bound: each wide level pushes at most W entries
+    S.stack_cap = bvh_width * (S.bvh.depth + 1) + 4;
+    S.ready = true;
+    return SPPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Camera (iCamera ctor, Camera.h:34-53)
+// ---------------------------------------------------------------------------
+int sppm_camera_set(sppm_ctx* c, const sppm_camera_desc* cd) {
+    if (!c || !cd) return fail(SPPM_E_ARG, "NULL argument");
+    if (cd->type != SPPM_CAM_PINHOLE && cd->type != SPPM_CAM_THINLENS) return fail(SPPM_E_ARG, "unknown camera type");
+    if (cd->width <= 0 || cd->height <= 0) return fail(SPPM_E_ARG, "bad resolution %dx%d", cd->width, cd->height);
+    if ((long long)cd->width * cd->height >= (1LL << 28)) return fail(SPPM_E_ARG, "resolution too large");
+    if (cd->sampler < SPPM_SAMPLER_UNIFORM || cd->sampler > SPPM_SAMPLER_HAMMERSLEY) return fail(SPPM_E_ARG, "unknown sampler");
+    if (cd->sampler == SPPM_SAMPLER_HAMMERSLEY && cd->sampler_nsample <= 0) return fail(SPPM_E_ARG, "Hammersley NumSample must be > 0");
+    c->cam = *cd;
+    CamDev& C = c->camdev;
+    C.type = cd->type;
+    C.W = cd->width;
+    C.H = cd->height;
+    H::V3 pos = H::ld(cd->pos);
+    H::V3 dir = H::normalized(H::ld(cd->target) - pos);
+    float fov = H::PI * cd->fov_deg / 360.0f;
+    C.PW = (float)(2.0 * (double)cd->dist_to_lens * tan((double)fov) * (double)cd->aspect);
+    C.PH = (float)(2.0 * (double)cd->dist_to_lens * tan((double)fov));
+    C.LW = C.PW;
+    C.LH = C.PH;
+    H::V3 e1, e2;
+    if ((double)dir.y > 0.99) {
+        e1 = H::v3(1, 0, 0);
+        e2 = H::v3(0, 0, 1);
+    } else {
+        e1 = H::normalized(H::cross(H::v3(0, 1, 0), dir));
+        e2 = H::normalized(H::cross(dir, e1));
+    }
+    C.pos[0] = pos.x; C.pos[1] = pos.y; C.pos[2] = pos.z;
+    C.dir[0] = dir.x; C.dir[1] = dir.y; C.dir[2] = dir.z;
+    C.e1[0] = e1.x; C.e1[1] = e1.y; C.e1[2] = e1.z;
+    C.e2[0] = e2.x; C.e2[1] = e2.y; C.e2[2] = e2.z;
+    C.F = cd->F;
+    C.D = cd->dist_to_lens;
+    C.sampler = cd->sampler;
+    c->have_cam = true;
+    c->rb.ready = false;  // resolution may have changed
+    return SPPM_OK;
+}
+
+int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
+    if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
+    if (p->photons_per_pass <= 0) return fail(SPPM_E_ARG, "photons_per_pass must be > 0");
+    if (!(p->alpha > 0.0f && p->alpha <= 1.0f)) return fail(SPPM_E_ARG, "alpha must be in (0, 1]");
+    if (!(p->first_radius > 0.0f)) return fail(SPPM_E_ARG, "first_radius must be > 0");
+    c->params = *p;
+    c->have_params = true;
+    c->rb.ready = false;
+    return SPPM_OK;
+}
+
+// initializeSharedPoints (SPPM.cpp:356-373) + buffers of the pass loop
+int sppm_begin(sppm_ctx* c) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (!c->have_cam) return fail(SPPM_E_STATE, "no camera set");
+    if (!c->have_params) return fail(SPPM_E_STATE, "no params set");
+    HIPOK(hipSetDevice(c->device));
+    HIPOK(hipStreamSynchronize(c->stream));
+    RenderBuffers& R = c->rb;
+    R.release();
+    const int N = c->params.photons_per_pass;
+    const int W = c->cam.width, Hh = c->cam.height, np = W * Hh;
+    R.N = N; R.W = W; R.H = Hh; R.np = np;
+    R.photon_cap = (int)std::min<long long>((long long)N * (NSLOT - 1), (1LL << 31) - 1);
+    R.cell_cap = 1 << 22;
+    const size_t NS = (size_t)NSLOT * N;
+    int rc = 0;
+    R.slots.npaths = N;
+    if ((rc = dalloc(&R.slots.pos, NS)) || (rc = dalloc(&R.slots.flux, NS)) || (rc = dalloc(&R.slots.dir, NS)) ||
+        (rc = dalloc(&R.slots.valid, NS)) || (rc = dalloc(&R.slots.count, N)) || (rc = dalloc(&R.off, N)) ||
+        (rc = dalloc(&R.nstored, 1)) || (rc = dalloc(&R.scan_tmp, scan_tmp_entries(N))) ||
+        (rc = dalloc(&R.cell_count, R.cell_cap + 1)) || (rc = dalloc(&R.cell_start, R.cell_cap + 1)) ||
+        (rc = dalloc(&R.cell_scan_tmp, scan_tmp_entries(R.cell_cap + 1))) || (rc = dalloc(&R.keyrank, R.photon_cap)) ||
+        (rc = dalloc(&R.gpos, R.photon_cap)) || (rc = dalloc(&R.gflux, R.photon_cap)) ||
+        (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
+        (rc = dalloc(&R.hits.nrm, np)) || (rc = dalloc(&R.hits.out, np)) || (rc = dalloc(&R.hits.weight, np)) ||
+        (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
+        (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np)) || (rc = dalloc(&R.list_off, np + 1)) ||
+        (rc = dalloc(&R.words, 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
+        (rc = dalloc(&R.rmax_bits, 1)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.image, np))) {
+        R.release();
+        return rc;
+    }
+    R.hits.np = np;
+    launch_init_slots(R.slots, c->stream);
+    launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
+    // UniformSample(ID): mt19937(ID) state, position 624 (first draw twists)
+    if (c->cam.sampler == SPPM_SAMPLER_UNIFORM) {
+        uint32_t mt[624];
+        mt[0] = (uint32_t)c->cam.sampler_id;
+        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+        int pos = 624;
+        HIPOK(hipMemcpyAsync(R.cam_mt, mt, sizeof mt, hipMemcpyHostToDevice, c->stream));
+        HIPOK(hipMemcpyAsync(R.cam_pos, &pos, sizeof pos, hipMemcpyHostToDevice, c->stream));
+    }
+    float r0 = c->params.first_radius;
+    uint32_t rb;
+    memcpy(&rb, &r0, 4);
+    launch_set_u32(R.rmax_bits, rb, c->stream);
+    launch_set_u32(R.flags, 0u, c->stream);
+    HIPOK(hipGetLastError());
+    HIPOK(hipStreamSynchronize(c->stream));
+    c->pass = 0;
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) { t->total_ms = 0; t->n = 0; }
+    R.ready = true;
+    R.have_lists = false;
+    return SPPM_OK;
+}
+
+static int check_ready(sppm_ctx* c) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (!c->rb.ready) return fail(SPPM_E_STATE, "sppm_begin has not been called since the last setup change");
+    return SPPM_OK;
+}
+
+static void fill_fixed_sample(sppm_ctx* c, int step) {
+    CamDev& C = c->camdev;
+    if (c->cam.sampler == SPPM_SAMPLER_HALTON) {
+        C.fixed_smp[0] = H::halton(2, step);
+        C.fixed_smp[1] = H::halton(3, step);
+        C.fixed_smp[2] = H::halton(5, step);
+    } else if (c->cam.sampler == SPPM_SAMPLER_HAMMERSLEY) {
+        C.fixed_smp[0] = (float)step / (float)c->cam.sampler_nsample;
+        C.fixed_smp[1] = H::halton(2, step);
+        C.fixed_smp[2] = H::halton(3, step);
+    }
+}
+
+static GridDev grid_dev(sppm_ctx* c) {
+    GridDev G{};
+    G.cell_start = c->rb.cell_start;
+    G.pos = c->rb.gpos;
+    G.flux = c->rb.gflux;
+    G.dir = c->rb.gdir;
+    return G;
+}
+
+// ---- split steps --------------------------------------------------------
+static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
+    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.stack_cap, c->rb.flags, st);
+}
+static void do_build(sppm_ctx* c, hipStream_t st) {
+    RenderBuffers& R = c->rb;
+    exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
+    launch_build_grid(R.slots, R.off, R.nstored, R.gp, R.rmax_bits, c->scene.bmin, c->scene.ext, R.cell_cap,
+                      R.cell_count, R.cell_start, R.cell_scan_tmp, R.keyrank, R.gpos, R.gflux, R.gdir, st);
+}
+static void do_camera_words(sppm_ctx* c, hipStream_t st) {
+    if (c->cam.sampler == SPPM_SAMPLER_UNIFORM)
+        launch_camera_words(c->rb.cam_mt, c->rb.cam_pos, c->rb.words, 6LL * c->rb.np, st);
+}
+static void do_eye(sppm_ctx* c, int step, hipStream_t st) {
+    fill_fixed_sample(c, step);
+    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.stack_cap, c->rb.words, c->rb.flags, st);
+}
+static int do_gather(sppm_ctx* c, hipStream_t st) {
+    RenderBuffers& R = c->rb;
+    GridDev G = grid_dev(c);
+    const float alpha = c->params.alpha;
+    if (c->params.record_neighbours) {
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rmax_bits, R.flags, st);
+        int total = 0;
+        exclusive_scan(R.counts, R.list_off, R.np + 1, nullptr, R.scan_tmp, st);
+        HIPOK(hipMemcpyAsync(&total, R.list_off + R.np, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        if ((long long)total > R.lists_cap) {
+            dfree(R.lists);
+            int rc = dalloc(&R.lists, (size_t)total + 1);
+            if (rc) return rc;
+            R.lists_cap = total;
+        }
+        launch_set_u32(R.rmax_bits, 0u, st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 2, R.counts, R.list_off, R.lists, R.rmax_bits,
+                      R.flags, st);
+        R.have_lists = true;
+    } else {
+        launch_set_u32(R.rmax_bits, 0u, st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 0, R.counts, nullptr, nullptr, R.rmax_bits, R.flags,
+                      st);
+    }
+    return SPPM_OK;
+}
+
+int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    do_photons(c, step, c->stream);
+    HIPOK(hipGetLastError());
+    if (emitted) *emitted = c->rb.N;
+    return SPPM_OK;
+}
+int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    do_build(c, c->stream);
+    HIPOK(hipGetLastError());
+    if (stored) {
+        HIPOK(hipMemcpyAsync(stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPOK(hipStreamSynchronize(c->stream));
+    }
+    return SPPM_OK;
+}
+int sppm_pass_eye(sppm_ctx* c, int step) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    do_camera_words(c, c->stream);
+    do_eye(c, step, c->stream);
+    HIPOK(hipGetLastError());
+    return SPPM_OK;
+}
+int sppm_pass_gather(sppm_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    rc = do_gather(c, c->stream);
+    if (rc) return rc;
+    HIPOK(hipGetLastError());
+    return SPPM_OK;
+}
+
+static void tstart(Timer& t, hipStream_t st) { (void)hipEventRecord(t.a, st); }
+static void tstop(Timer& t, hipStream_t st) { (void)hipEventRecord(t.b, st); }
+static void tacc(Timer& t) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+        t.total_ms += ms;
+        t.n += 1;
+    }
+}
+
+int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const int step = c->pass + 1;
+    hipStream_t st = c->stream;
+    const bool timing = stats != nullptr;
+    // camera stream words on the side stream, overlapping the photon pass
+    HIPOK(hipEventRecord(c->ev_fork, st));
+    HIPOK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    do_camera_words(c, c->stream2);
+    HIPOK(hipEventRecord(c->ev_join, c->stream2));
+    if (timing) tstart(c->t_photons, st);
+    do_photons(c, step, st);
+    if (timing) { tstop(c->t_photons, st); tstart(c->t_build, st); }
+    do_build(c, st);
+    if (timing) tstop(c->t_build, st);
+    HIPOK(hipStreamWaitEvent(st, c->ev_join, 0));
+    if (timing) tstart(c->t_eye, st);
+    do_eye(c, step, st);
+    if (timing) { tstop(c->t_eye, st); tstart(c->t_gather, st); }
+    rc = do_gather(c, st);
+    if (rc) return rc;
+    if (timing) tstop(c->t_gather, st);
+    HIPOK(hipGetLastError());
+    c->pass += 1;
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        int stored = 0;
+        uint32_t flags = 0, rmax = 0;
+        HIPOK(hipMemcpyAsync(&stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPOK(hipMemcpyAsync(&flags, c->rb.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPOK(hipMemcpyAsync(&rmax, c->rb.rmax_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        tacc(c->t_photons); tacc(c->t_build); tacc(c->t_eye); tacc(c->t_gather);
+        float fl = 0.0f, r;
+        memcpy(&r, &rmax, 4);
+        (void)fl;
+        stats->photons_emitted = c->rb.N;
+        stats->photons_stored = stored;
+        stats->max_radius = r;
+        stats->flags = (int)flags;
+        stats->ms_photons = (float)(c->t_photons.n ? c->t_photons.total_ms / c->t_photons.n : 0);
+        stats->ms_build = (float)(c->t_build.n ? c->t_build.total_ms / c->t_build.n : 0);
+        stats->ms_eye = (float)(c->t_eye.n ? c->t_eye.total_ms / c->t_eye.n : 0);
+        stats->ms_gather = (float)(c->t_gather.n ? c->t_gather.total_ms / c->t_gather.n : 0);
+        if (c->params.check_numeric && (flags & 1u)) return fail(SPPM_E_NUMERIC, "NaN/Inf radiance after pass %d", c->pass);
+    }
+    return SPPM_OK;
+}
+
+int sppm_sync(sppm_ctx* c) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    HIPOK(hipStreamSynchronize(c->stream2));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return SPPM_OK;
+}
+
+int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
+    if (!c || !name || !ms) return fail(SPPM_E_ARG, "NULL argument");
+    Timer* t = nullptr;
+    if (!strcmp(name, "photons")) t = &c->t_photons;
+    else if (!strcmp(name, "build")) t = &c->t_build;
+    else if (!strcmp(name, "eye")) t = &c->t_eye;
+    else if (!strcmp(name, "gather")) t = &c->t_gather;
+    if (!t) return fail(SPPM_E_ARG, "unknown kernel name '%s'", name);
+    *ms = (float)(t->n ? t->total_ms / t->n : 0.0);
+    return SPPM_OK;
+}
+
+int sppm_reconstruct(sppm_ctx* c, int32_t total_photons, int32_t passes, float* rgba) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!rgba) return fail(SPPM_E_ARG, "rgba is NULL");
+    launch_reconstruct(c->rb.shared, c->rb.np, total_photons, passes, c->rb.image, c->stream);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(rgba, c->rb.image, sizeof(float4) * c->rb.np, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return SPPM_OK;
+}
+
+// ---- debug readback ---------------------------------------------------------
+int sppm_read_photons(sppm_ctx* c, int32_t cap, float* pos, float* flux, float* indir, float* index, float* pdf,
+                      int32_t* count) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    RenderBuffers& R = c->rb;
+    int n = 0;
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&n, R.nstored, sizeof(int), hipMemcpyDeviceToHost));
+    if (count) *count = n;
+    if (!pos) return SPPM_OK;
+    if (n > cap) return fail(SPPM_E_ARG, "capacity %d < %d photons", cap, n);
+    float4 *fp = nullptr, *ff = nullptr, *fd = nullptr;
+    if ((rc = dalloc(&fp, n)) || (rc = dalloc(&ff, n)) || (rc = dalloc(&fd, n))) {
+        dfree(fp); dfree(ff); dfree(fd);
+        return rc;
+    }
+    launch_photon_flat(R.slots, R.off, fp, ff, fd, c->stream);
+    std::vector<float4> hp(n), hf(n), hd(n);
+    HIPOK(hipMemcpyAsync(hp.data(), fp, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipMemcpyAsync(hf.data(), ff, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipMemcpyAsync(hd.data(), fd, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    dfree(fp); dfree(ff); dfree(fd);
+    for (int i = 0; i < n; ++i) {
+        pos[3 * i] = hp[i].x; pos[3 * i + 1] = hp[i].y; pos[3 * i + 2] = hp[i].z;
+        flux[3 * i] = hf[i].x; flux[3 * i + 1] = hf[i].y; flux[3 * i + 2] = hf[i].z;
+        indir[3 * i] = hd[i].x; indir[3 * i + 1] = hd[i].y; indir[3 * i + 2] = hd[i].z;
+        index[i] = hp[i].w;
+        pdf[i] = hf[i].w;
+    }
+    return SPPM_OK;
+}
+
+int sppm_read_hitpoints(sppm_ctx* c, int32_t* type, int32_t* facet, float* pos, float* weight, float* outdir,
+                        float* normal) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    RenderBuffers& R = c->rb;
+    const int np = R.np;
+    std::vector<float4> p(np), w(np), o(np), n(np);
+    std::vector<int2> m(np);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(p.data(), R.hits.pos, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(w.data(), R.hits.weight, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(o.data(), R.hits.out, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(n.data(), R.hits.nrm, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(m.data(), R.hits.mat, sizeof(int2) * np, hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) {
+        if (type) type[i] = (int)p[i].w;
+        if (facet) facet[i] = m[i].y;
+        if (pos) { pos[3 * i] = p[i].x; pos[3 * i + 1] = p[i].y; pos[3 * i + 2] = p[i].z; }
+        if (weight) { weight[3 * i] = w[i].x; weight[3 * i + 1] = w[i].y; weight[3 * i + 2] = w[i].z; }
+        if (outdir) { outdir[3 * i] = o[i].x; outdir[3 * i + 1] = o[i].y; outdir[3 * i + 2] = o[i].z; }
+        if (normal) { normal[3 * i] = n[i].x; normal[3 * i + 1] = n[i].y; normal[3 * i + 2] = n[i].z; }
+    }
+    return SPPM_OK;
+}
+
+int sppm_read_shared(sppm_ctx* c, float* radiance, float* radius, int32_t* num_photon, int32_t* num_gather) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    RenderBuffers& R = c->rb;
+    const int np = R.np;
+    std::vector<float4> r(np);
+    std::vector<int2> k(np);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(r.data(), R.shared.rad, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(k.data(), R.shared.cnt, sizeof(int2) * np, hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) {
+        if (radiance) { radiance[3 * i] = r[i].x; radiance[3 * i + 1] = r[i].y; radiance[3 * i + 2] = r[i].z; }
+        if (radius) radius[i] = r[i].w;
+        if (num_photon) num_photon[i] = k[i].x;
+        if (num_gather) num_gather[i] = k[i].y;
+    }
+    return SPPM_OK;
+}
+
+int sppm_read_counts(sppm_ctx* c, int32_t* m) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!m) return fail(SPPM_E_ARG, "NULL output");
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(m, c->rb.counts, sizeof(int) * c->rb.np, hipMemcpyDeviceToHost));
+    return SPPM_OK;
+}
+
+int sppm_read_neighbours(sppm_ctx* c, int64_t cap, int64_t* offsets, int32_t* idx, int64_t* total) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    RenderBuffers& R = c->rb;
+    if (!R.have_lists) return fail(SPPM_E_STATE, "no neighbour lists (set params.record_neighbours and run a gather)");
+    const int np = R.np;
+    std::vector<int> cnt(np);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(cnt.data(), R.counts, sizeof(int) * np, hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int i = 0; i < np; ++i) tot += cnt[i];
+    if (total) *total = tot;
+    if (!offsets) return SPPM_OK;
+    if (tot > cap) return fail(SPPM_E_ARG, "capacity %lld < %lld neighbours", (long long)cap, (long long)tot);
+    offsets[0] = 0;
+    for (int i = 0; i < np; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+    if (tot > 0) HIPOK(hipMemcpy(idx, R.lists, sizeof(int) * tot, hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) std::sort(idx + offsets[i], idx + offsets[i + 1]);
+    return SPPM_OK;
+}
+
+int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, float tmax,
+                       int32_t* facet, float* t, float* u, float* v) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (n <= 0) return SPPM_OK;
+    if (!org || !dir || !facet) return fail(SPPM_E_ARG, "NULL argument");
+    HIPOK(hipSetDevice(c->device));
+    float *dorg = nullptr, *ddir = nullptr, *dtuv = nullptr;
+    int* dfac = nullptr;
+    uint32_t* dfl = nullptr;
+    int rc = 0;
+    if ((rc = dalloc(&dorg, 3 * (size_t)n)) || (rc = dalloc(&ddir, 3 * (size_t)n)) || (rc = dalloc(&dtuv, 3 * (size_t)n)) ||
+        (rc = dalloc(&dfac, n)) || (rc = dalloc(&dfl, 1))) {
+        dfree(dorg); dfree(ddir); dfree(dtuv); dfree(dfac); dfree(dfl);
+        return rc;
+    }
+    hipStream_t st = c->stream;
+    HIPOK(hipMemcpyAsync(dorg, org, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(ddir, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    launch_set_u32(dfl, 0u, st);
+    launch_trace(c->scene.dev, n, dorg, ddir, tmin, c->scene.stack_cap, dfac, dtuv, dfl, st);
+    HIPOK(hipGetLastError());
+    std::vector<float> tuv(3 * (size_t)n);
+    uint32_t fl = 0;
+    HIPOK(hipMemcpyAsync(facet, dfac, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(tuv.data(), dtuv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(&fl, dfl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    dfree(dorg); dfree(ddir); dfree(dtuv); dfree(dfac); dfree(dfl);
+    for (int i = 0; i < n; ++i) {
+        // hits are accepted only below tmax (MaxT); FLT_MAX means "no limit"
+        if (facet[i] >= 0 && !(tuv[3 * i] < tmax)) facet[i] = -1;
+        if (t) t[i] = facet[i] >= 0 ? tuv[3 * i] : 3.40282347e38f;
+        if (u) u[i] = facet[i] >= 0 ? tuv[3 * i + 1] : 0.0f;
+        if (v) v[i] = facet[i] >= 0 ? tuv[3 * i + 2] : 0.0f;
+    }
+    if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow");
+    return SPPM_OK;
+}
+
+}  // extern "C"

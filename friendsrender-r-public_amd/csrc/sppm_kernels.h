@@ -1,0 +1,34 @@
+// sppm_kernels.h -- host-side launchers shared between the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sppm_layout.h"
+
+namespace sppm {
+
+// ---- sort / scan (sppm_sort.hip) ------------------------------------------
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* hist, int n,
+                      int bits, hipStream_t st);
+size_t radix_hist_entries(int n);
+__global__ void k_tile_total_prep(int* tmp, int nt);
+__global__ void k_scan_small(uint32_t* data, int n);
+void exclusive_scan(const int* in, int* out, int n, int* total, int* tmp, hipStream_t st);
+size_t scan_tmp_entries(int n);
+
+// ---- BVH build (sppm_bvh.hip) ---------------------------------------------
+struct BvhResult {
+    float* nodes = nullptr;  // device, nnodes * node_floats(width)
+    float4* tri = nullptr;   // device, 3 * ntri
+    int nnodes = 0;
+    int depth = 0;           // wide levels
+    int width = 0;
+};
+// pos: device array of 9 floats per triangle. Returns 0 on success.
+int build_bvh(const float* pos, int ntri, int width, hipStream_t st, BvhResult* out, char* err, size_t errlen);
+void free_bvh(BvhResult* r);
+
+// ---- pass kernels (sppm_pass.hip) -----------------------------------------
+struct PassBuffers;  // defined in sppm_pass.hip
+
+}  // namespace sppm

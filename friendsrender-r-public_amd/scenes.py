@@ -1,0 +1,303 @@
+"""Seeded procedural scenes for the SPPM workloads.
+
+The reference only loads OBJ/MTL files (Scene.cpp:13-218).  These generators
+produce the same facet records that loader would (positions, per-vertex
+normals, UVs, material id per facet) plus the attribute table a
+SceneMaterialManager subclass would fill (main.cpp:14-114 pattern), and
+RGBA32F textures exactly as TextureUtility::LoadTextureFromFile stores 8-bit
+images (byte / 255.0f, alpha 1 for RGB images; TextureUtility.cpp:12-35).
+
+Workloads:
+  box_scene()      Cornell box, 5 walls x 2 triangles, diffuse + perfect
+                   specular sphere (40x20 UV spheres), 0.25^2 ceiling light
+                   with emission 15, untextured.
+  full_scene()     the same box plus a 200,000-triangle geodesic sphere
+                   displaced by 3-octave value noise (amplitude 0.05), GGX
+                   conductor and GGX dielectric spheres, a light with a
+                   256x256 RGB8 8x8 checker (5/20) emission texture and
+                   512x512 albedo / roughness / normal noise textures.
+"""
+import numpy as np
+
+from sppm_amd import (CAM_PINHOLE, CAM_THINLENS, MAT_DIFFUSE, MAT_GGX_GLASS, MAT_METAL, MAT_ROUGH_CONDUCTOR,
+                      SAMPLER_UNIFORM)
+
+
+class Scene:
+    def __init__(self, pos, nrm, uv, attr, attrs, textures):
+        self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
+        self.nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 9)
+        self.uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 9)
+        self.attr = np.ascontiguousarray(attr, np.int32)
+        self.attrs = attrs
+        self.textures = [np.ascontiguousarray(t, np.float32) for t in textures]
+
+    @property
+    def nfacets(self):
+        return self.pos.shape[0]
+
+    def diagonal(self):
+        p = self.pos.reshape(-1, 3)
+        return float(np.linalg.norm(p.max(0) - p.min(0)))
+
+
+class _Builder:
+    def __init__(self):
+        self.pos, self.nrm, self.uv, self.attr = [], [], [], []
+
+    def tri(self, p, n, t, a):
+        self.pos.append(np.asarray(p, np.float32).reshape(9))
+        self.nrm.append(np.asarray(n, np.float32).reshape(9))
+        self.uv.append(np.asarray(t, np.float32).reshape(9))
+        self.attr.append(a)
+
+    def quad(self, p0, p1, p2, p3, n, a, uvs=((0, 0), (1, 0), (1, 1), (0, 1))):
+        """Two triangles (p0,p1,p2), (p0,p2,p3) with a constant normal."""
+        t = [(u, v, 0.0) for u, v in uvs]
+        self.tri([p0, p1, p2], [n, n, n], [t[0], t[1], t[2]], a)
+        self.tri([p0, p2, p3], [n, n, n], [t[0], t[2], t[3]], a)
+
+    def mesh(self, V, N, T, F, a):
+        F = np.asarray(F)
+        self.pos.extend(V[F].reshape(-1, 9))
+        self.nrm.extend(N[F].reshape(-1, 9))
+        self.uv.extend(T[F].reshape(-1, 9))
+        self.attr.extend([a] * len(F))
+
+    def scene(self, attrs, textures=()):
+        return Scene(np.array(self.pos), np.array(self.nrm), np.array(self.uv), np.array(self.attr, np.int32),
+                     attrs, list(textures))
+
+
+def uv_sphere(center, r, nseg=40, nring=20):
+    """UV sphere: nseg segments, nring rings, fans at the poles (no degenerate triangles)."""
+    c = np.asarray(center, np.float64)
+    V, N, T = [], [], []
+    for j in range(nring + 1):
+        th = np.pi * j / nring
+        for i in range(nseg + 1):
+            ph = 2 * np.pi * i / nseg
+            n = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+            V.append(c + r * n)
+            N.append(n)
+            T.append((i / nseg, j / nring, 0.0))
+    V, N, T = np.array(V, np.float32), np.array(N, np.float32), np.array(T, np.float32)
+    F = []
+    row = nseg + 1
+    for j in range(nring):
+        for i in range(nseg):
+            a, b = j * row + i, j * row + i + 1
+            c_, d = (j + 1) * row + i, (j + 1) * row + i + 1
+            if j != 0:
+                F.append((a, c_, b))
+            if j != nring - 1:
+                F.append((b, c_, d))
+    return V, N, T, np.array(F)
+
+
+def box_walls(b, white, red, green):
+    """Box [0,1]^3 open at z=1; inward normals (Scene.cpp keeps OBJ normals)."""
+    b.quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), white)      # floor y=0
+    b.quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), white)     # ceiling y=1
+    b.quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), white)      # back z=0
+    b.quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), red)        # left x=0
+    b.quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), green)     # right x=1
+
+
+def light_quad(b, a, size=0.25, y=0.999, cx=0.5, cz=0.5):
+    h = size / 2
+    b.quad((cx - h, y, cz - h), (cx - h, y, cz + h), (cx + h, y, cz + h), (cx + h, y, cz - h), (0, -1, 0), a)
+
+
+def box_scene(with_spheres=True):
+    """Plain SPPM box workload (diffuse + specular spheres)."""
+    attrs = [
+        dict(kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),                     # 0 white
+        dict(kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),                     # 1 red
+        dict(kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),                     # 2 green
+        dict(kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+             if_opaque=1),                                                                            # 3 light
+        dict(kind=MAT_DIFFUSE, albedo=(0.75, 0.75, 0.75), is_sppm=1, if_opaque=1),                     # 4 sphere
+        dict(kind=MAT_METAL, albedo=(0.95, 0.95, 0.95), if_delta=1, is_sppm=0, if_opaque=1),           # 5 mirror
+    ]
+    b = _Builder()
+    box_walls(b, 0, 1, 2)
+    light_quad(b, 3)
+    if with_spheres:
+        b.mesh(*uv_sphere((0.3, 0.2, 0.4), 0.2), a=4)
+        b.mesh(*uv_sphere((0.72, 0.2, 0.62), 0.2), a=5)
+    return b.scene(attrs)
+
+
+def pinhole_camera(width, height, sampler=SAMPLER_UNIFORM, sampler_id=0):
+    return dict(type=CAM_PINHOLE, width=width, height=height, pos=(0.5, 0.5, 2.2), target=(0.5, 0.5, 0.5),
+                fov_deg=45.0, F=1.0, dist_to_lens=1.0, aspect=width / height, sampler=sampler,
+                sampler_id=sampler_id, sampler_nsample=1)
+
+
+def thinlens_camera(width, height, aperture=0.02, focus=2.0, fov_deg=45.0, sampler_id=0):
+    """ThinLensCamera: lens size == image-plane size (Camera.h:40-43), so the
+    aperture fixes DistanceToLens; F from the focus distance B = A*F/(A-F)."""
+    half = np.float32(np.pi) * np.float32(fov_deg) / np.float32(360.0)
+    A = float(aperture / (2.0 * np.tan(float(half)) * (width / height)))
+    F = A * focus / (A + focus)
+    return dict(type=CAM_THINLENS, width=width, height=height, pos=(0.5, 0.5, 2.5), target=(0.5, 0.5, 0.5),
+                fov_deg=fov_deg, F=F, dist_to_lens=A, aspect=width / height, sampler=SAMPLER_UNIFORM,
+                sampler_id=sampler_id, sampler_nsample=1)
+
+
+# ---- procedural textures ----------------------------------------------------
+def _value_noise(n, seed, octaves=3, base=8):
+    rng = np.random.default_rng(seed)
+    out = np.zeros((n, n), np.float64)
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        g = base * (2 ** o)
+        lat = rng.random((g + 1, g + 1))
+        x = np.linspace(0, g, n, endpoint=False)
+        i0 = np.floor(x).astype(int)
+        f = x - i0
+        f = f * f * (3 - 2 * f)
+        a = lat[i0][:, i0]
+        b_ = lat[i0][:, i0 + 1]
+        c = lat[i0 + 1][:, i0]
+        d = lat[i0 + 1][:, i0 + 1]
+        fx, fy = f[None, :], f[:, None]
+        out += amp * ((a * (1 - fx) + b_ * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy)
+        tot += amp
+        amp *= 0.5
+    return out / tot
+
+
+def rgb8_to_rgba32f(img8):
+    """Texture2D storage of an 8-bit RGB image: byte / 255.0f, alpha 1."""
+    h, w = img8.shape[:2]
+    out = np.ones((h, w, 4), np.float32)
+    out[..., :3] = img8.astype(np.float32) / np.float32(255.0)
+    return out
+
+
+def checker_texture(n=256, cells=8, lo=5, hi=20):
+    idx = (np.arange(n) * cells // n)
+    chk = ((idx[:, None] + idx[None, :]) % 2).astype(np.uint8)
+    v = np.where(chk == 1, hi, lo).astype(np.uint8)
+    return rgb8_to_rgba32f(np.repeat(v[..., None], 3, axis=2))
+
+
+def noise_textures(n=512, seed=7):
+    alb = _value_noise(n, seed)
+    albedo8 = np.clip(np.stack([0.35 + 0.5 * alb, 0.3 + 0.45 * alb, 0.25 + 0.4 * alb], -1) * 255, 0, 255).astype(np.uint8)
+    rough8 = np.clip(_value_noise(n, seed + 1) * 255, 0, 255).astype(np.uint8)
+    hgt = _value_noise(n, seed + 2, base=16)
+    gy, gx = np.gradient(hgt * 8.0)
+    nrm = np.stack([-gx, -gy, np.ones_like(gx)], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    normal8 = np.clip((nrm * 0.5 + 0.5) * 255, 0, 255).astype(np.uint8)
+    return (rgb8_to_rgba32f(albedo8), rgb8_to_rgba32f(np.repeat(rough8[..., None], 3, axis=2)),
+            rgb8_to_rgba32f(normal8))
+
+
+def geodesic_sphere(center, r, freq, seed, amp=0.05):
+    """Icosahedron with each face split into freq^2 triangles (20*freq^2
+    triangles), projected to the sphere and displaced radially by 3-octave
+    value noise on the sphere (amplitude amp)."""
+    t = (1 + 5 ** 0.5) / 2
+    iv = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                   (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], np.float64)
+    iv /= np.linalg.norm(iv, axis=1, keepdims=True)
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    rng = np.random.default_rng(seed)
+    octs = [(rng.normal(size=(3,)) * 0.0 + rng.random(3) * 10, 2.0 ** o) for o in range(3)]
+    lat = rng.random((3, 16, 16, 16))
+
+    def noise(p):
+        val = np.zeros(len(p))
+        amp_, tot = 1.0, 0.0
+        for o, (off, fr) in enumerate(octs):
+            q = (p * fr * 2.0 + off) % 16
+            i0 = np.floor(q).astype(int)
+            f = q - i0
+            f = f * f * (3 - 2 * f)
+            i1 = (i0 + 1) % 16
+            L = lat[o]
+            acc = 0
+            for dx in (0, 1):
+                for dy in (0, 1):
+                    for dz in (0, 1):
+                        ix = i1[:, 0] if dx else i0[:, 0]
+                        iy = i1[:, 1] if dy else i0[:, 1]
+                        iz = i1[:, 2] if dz else i0[:, 2]
+                        w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * \
+                            (f[:, 2] if dz else 1 - f[:, 2])
+                        acc = acc + w * L[ix, iy, iz]
+            val += amp_ * acc
+            tot += amp_
+            amp_ *= 0.5
+        return val / tot - 0.5
+
+    tris = []
+    for (a, b, c) in faces:
+        A, B, Cc = iv[a], iv[b], iv[c]
+        grid = {}
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                p = A + (B - A) * (i / freq) + (Cc - A) * (j / freq)
+                grid[(i, j)] = p / np.linalg.norm(p)
+        for i in range(freq):
+            for j in range(freq - i):
+                tris.append((grid[(i, j)], grid[(i + 1, j)], grid[(i, j + 1)]))
+                if j < freq - i - 1:
+                    tris.append((grid[(i + 1, j)], grid[(i + 1, j + 1)], grid[(i, j + 1)]))
+    P = np.array(tris, np.float64)  # (T, 3, 3) unit directions
+    flat = P.reshape(-1, 3)
+    disp = 1.0 + amp / r * noise(flat) * 2.0
+    Vd = flat * (r * disp)[:, None]
+    V = (Vd + np.asarray(center)).reshape(-1, 3, 3)
+    # face normals used as vertex normals of each facet (Scene.cpp:124 rule)
+    e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+    fn = np.cross(e1, e2)
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True) + 1e-30
+    Nn = np.repeat(fn[:, None, :], 3, axis=1)
+    uv = np.zeros_like(V)
+    uv[..., 0] = 0.5 + np.arctan2(flat[:, 2], flat[:, 0]).reshape(-1, 3) / (2 * np.pi)
+    uv[..., 1] = 0.5 - np.arcsin(np.clip(flat[:, 1], -1, 1)).reshape(-1, 3) / np.pi
+    return V.astype(np.float32), Nn.astype(np.float32), uv.astype(np.float32)
+
+
+def full_scene(freq=100, seed=1234, tex_size=512):
+    """Full-material workload: 200k-triangle displaced mesh (freq=100)."""
+    albedo, rough, normal = noise_textures(tex_size, seed)
+    checker = checker_texture()
+    textures = [albedo, rough, normal, checker]
+    attrs = [
+        dict(kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),                            # 0
+        dict(kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),                            # 1
+        dict(kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),                            # 2
+        dict(kind=MAT_DIFFUSE, albedo=(1.0, 1.0, 1.0), radiance=(255.0, 255.0, 255.0), is_sppm=1,
+             if_opaque=1, tex_albedo=3),                                                                      # 3 light
+        # GGX conductor: albedo = F0, scalar complex IOR (eta=1, k=20) so F(0) ~ 0.99
+        dict(kind=MAT_ROUGH_CONDUCTOR, albedo=(0.95, 0.64, 0.54), roughness=0.3, index=1.0, extinction=20.0,
+             is_sppm=1, if_opaque=1),                                                                         # 4
+        dict(kind=MAT_GGX_GLASS, albedo=(0.98, 0.98, 0.98), roughness=0.1, index=1.5, is_sppm=0,
+             if_opaque=0),                                                                                    # 5
+        dict(kind=MAT_DIFFUSE, albedo=(0.8, 0.8, 0.8), is_sppm=1, if_opaque=1, tex_albedo=0, tex_bump=2),     # 6 floor
+        dict(kind=MAT_ROUGH_CONDUCTOR, albedo=(0.9, 0.9, 0.9), roughness=0.4, index=1.0, extinction=20.0,
+             is_sppm=1, if_opaque=1, tex_roughness=1),                                                        # 7 mesh
+    ]
+    b = _Builder()
+    b.quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), 6)
+    b.quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), 0)
+    b.quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), 0)
+    b.quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), 1)
+    b.quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 2)
+    light_quad(b, 3)
+    b.mesh(*uv_sphere((0.27, 0.2, 0.35), 0.2), a=4)
+    b.mesh(*uv_sphere((0.73, 0.2, 0.65), 0.2), a=5)
+    V, N, T = geodesic_sphere((0.5, 0.6, 0.3), 0.15, freq, seed)
+    b.pos.extend(V.reshape(-1, 9))
+    b.nrm.extend(N.reshape(-1, 9))
+    b.uv.extend(T.reshape(-1, 9))
+    b.attr.extend([7] * len(V))
+    return b.scene(attrs, textures)

@@ -1,0 +1,294 @@
+"""ctypes binding of libsppm_amd.so (include/sppm_amd.h).
+
+This is plumbing for tests and the benchmark: every call goes straight to the
+C ABI, whose passes run as hand-written HIP kernels.  There is no CPU path:
+if the shared library is missing or fails to load, import raises.
+
+Mirrors the reference's per-pass structure (SPPM/SPPM/SPPM.cpp:770-854):
+``Renderer.pass_()`` is one loop body, ``reconstruct`` is
+``reconstructImageFromSharedPoints``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsppm_amd.so")
+
+MAT_DIFFUSE, MAT_METAL, MAT_ROUGH_CONDUCTOR, MAT_GGX_GLASS, MAT_GLASS = range(5)
+CAM_PINHOLE, CAM_THINLENS = 0, 1
+SAMPLER_UNIFORM, SAMPLER_HALTON, SAMPLER_HAMMERSLEY = 0, 1, 2
+
+EXPORTED = [
+    "sppm_abi_version", "sppm_last_error", "sppm_create", "sppm_destroy", "sppm_scene_upload",
+    "sppm_camera_set", "sppm_params_set", "sppm_begin", "sppm_pass", "sppm_pass_photons",
+    "sppm_pass_build", "sppm_pass_eye", "sppm_pass_gather", "sppm_sync", "sppm_reconstruct",
+    "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
+    "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms",
+]
+
+
+class Attr(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_float), ("if_delta", C.c_int32),
+                ("if_opaque", C.c_int32), ("albedo", C.c_float * 3), ("roughness", C.c_float),
+                ("radiance", C.c_float * 3), ("reflectance", C.c_float), ("extinction", C.c_float),
+                ("is_sppm", C.c_int32), ("tex_albedo", C.c_int32), ("tex_roughness", C.c_int32),
+                ("tex_bump", C.c_int32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba", C.POINTER(C.c_float))]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("nfacets", C.c_int32), ("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)),
+                ("uv", C.POINTER(C.c_float)), ("attr", C.POINTER(C.c_int32)), ("nattrs", C.c_int32),
+                ("attrs", C.POINTER(Attr)), ("ntextures", C.c_int32), ("textures", C.POINTER(Texture))]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("pos", C.c_float * 3), ("target", C.c_float * 3), ("fov_deg", C.c_float),
+                ("F", C.c_float), ("dist_to_lens", C.c_float), ("aspect", C.c_float),
+                ("sampler", C.c_int32), ("sampler_id", C.c_int32), ("sampler_nsample", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("first_radius", C.c_float), ("photons_per_pass", C.c_int32),
+                ("bvh_width", C.c_int32), ("block_size", C.c_int32), ("check_numeric", C.c_int32),
+                ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("photons_emitted", C.c_int32), ("photons_stored", C.c_int32), ("hit_points", C.c_int32),
+                ("neighbours", C.c_int64), ("max_radius", C.c_float), ("flags", C.c_int32),
+                ("ms_photons", C.c_float), ("ms_build", C.c_float), ("ms_eye", C.c_float),
+                ("ms_gather", C.c_float)]
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.sppm_last_error.restype = C.c_char_p
+    lib.sppm_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_void_p]
+    lib.sppm_destroy.argtypes = [C.c_void_p]
+    lib.sppm_scene_upload.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int]
+    lib.sppm_camera_set.argtypes = [C.c_void_p, C.POINTER(CameraDesc)]
+    lib.sppm_params_set.argtypes = [C.c_void_p, C.POINTER(Params)]
+    lib.sppm_begin.argtypes = [C.c_void_p]
+    lib.sppm_pass.argtypes = [C.c_void_p, C.POINTER(PassStats)]
+    lib.sppm_pass_photons.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32)]
+    lib.sppm_pass_build.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.sppm_pass_eye.argtypes = [C.c_void_p, C.c_int]
+    lib.sppm_pass_gather.argtypes = [C.c_void_p]
+    lib.sppm_sync.argtypes = [C.c_void_p]
+    lib.sppm_reconstruct.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+    lib.sppm_read_photons.argtypes = [C.c_void_p, C.c_int32] + [C.POINTER(C.c_float)] * 5 + [C.POINTER(C.c_int32)]
+    lib.sppm_read_hitpoints.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 4
+    lib.sppm_read_shared.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]
+    lib.sppm_read_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.sppm_read_neighbours.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int64)]
+    lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                       C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
+    lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+class SppmError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise SppmError(f"sppm error {rc}: {lib().sppm_last_error().decode()}")
+
+
+def scene_desc(scene):
+    """Builds a SceneDesc from a scenes.Scene; returns (desc, keepalive)."""
+    attrs = (Attr * len(scene.attrs))()
+    for i, a in enumerate(scene.attrs):
+        attrs[i].kind = a["kind"]
+        attrs[i].index = a.get("index", 1.0)
+        attrs[i].if_delta = int(a.get("if_delta", 0))
+        attrs[i].if_opaque = int(a.get("if_opaque", 1))
+        attrs[i].albedo[:] = list(a.get("albedo", (0.0, 0.0, 0.0)))
+        attrs[i].roughness = a.get("roughness", 1.0)
+        attrs[i].radiance[:] = list(a.get("radiance", (0.0, 0.0, 0.0)))
+        attrs[i].reflectance = a.get("reflectance", 1.0)
+        attrs[i].extinction = a.get("extinction", 1.0)
+        attrs[i].is_sppm = int(a.get("is_sppm", 0))
+        attrs[i].tex_albedo = a.get("tex_albedo", -1)
+        attrs[i].tex_roughness = a.get("tex_roughness", -1)
+        attrs[i].tex_bump = a.get("tex_bump", -1)
+    texs = (Texture * max(1, len(scene.textures)))()
+    for i, t in enumerate(scene.textures):
+        texs[i].height, texs[i].width = t.shape[0], t.shape[1]
+        texs[i].rgba = _fp(t)
+    d = SceneDesc()
+    d.nfacets = scene.pos.shape[0]
+    d.pos, d.nrm, d.uv, d.attr = _fp(scene.pos), _fp(scene.nrm), _fp(scene.uv), _ip(scene.attr)
+    d.nattrs = len(scene.attrs)
+    d.attrs = C.cast(attrs, C.POINTER(Attr))
+    d.ntextures = len(scene.textures)
+    d.textures = C.cast(texs, C.POINTER(Texture))
+    return d, (attrs, texs, scene)
+
+
+def camera_desc(cam):
+    c = CameraDesc()
+    c.type = cam["type"]
+    c.width, c.height = cam["width"], cam["height"]
+    c.pos[:] = list(cam["pos"])
+    c.target[:] = list(cam["target"])
+    c.fov_deg, c.F, c.dist_to_lens, c.aspect = cam["fov_deg"], cam["F"], cam["dist_to_lens"], cam["aspect"]
+    c.sampler = cam.get("sampler", SAMPLER_UNIFORM)
+    c.sampler_id = cam.get("sampler_id", 0)
+    c.sampler_nsample = cam.get("sampler_nsample", 1)
+    return c
+
+
+def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=True):
+    p = Params()
+    p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
+    p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)
+    p.record_neighbours, p.max_neighbours = int(record), 0
+    return p
+
+
+class Renderer:
+    """One SPPM render on one GPU: scene + camera + params, then passes."""
+
+    def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0):
+        self.lib = lib()
+        h = C.c_void_p()
+        _check(self.lib.sppm_create(C.byref(h), device, None))
+        self.h = h
+        self.W, self.H = cam["width"], cam["height"]
+        self.N = photons
+        d, self._keep = scene_desc(scene)
+        _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
+        self._cam = camera_desc(cam)
+        _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
+        self._params = params_struct(alpha, first_radius, photons, bvh_width, record)
+        _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
+        _check(self.lib.sppm_begin(self.h))
+        self.passes = 0
+        self.total_photons = 0
+
+    def close(self):
+        if self.h:
+            self.lib.sppm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pass_(self, stats=True):
+        st = PassStats() if stats else None
+        _check(self.lib.sppm_pass(self.h, C.byref(st) if stats else None))
+        self.passes += 1
+        self.total_photons += self.N
+        return st
+
+    def photons(self, step):
+        e = C.c_int32()
+        _check(self.lib.sppm_pass_photons(self.h, step, C.byref(e)))
+        return e.value
+
+    def build(self):
+        s = C.c_int32()
+        _check(self.lib.sppm_pass_build(self.h, C.byref(s)))
+        return s.value
+
+    def eye(self, step):
+        _check(self.lib.sppm_pass_eye(self.h, step))
+
+    def gather(self):
+        _check(self.lib.sppm_pass_gather(self.h))
+
+    def sync(self):
+        _check(self.lib.sppm_sync(self.h))
+
+    def kernel_ms(self, name):
+        v = C.c_float()
+        _check(self.lib.sppm_kernel_ms(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    def read_photons(self):
+        n = C.c_int32()
+        _check(self.lib.sppm_read_photons(self.h, 0, None, None, None, None, None, C.byref(n)))
+        n = n.value
+        pos, flux, indir = (np.zeros((n, 3), np.float32) for _ in range(3))
+        index, pdf = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(self.lib.sppm_read_photons(self.h, n, _fp(pos), _fp(flux), _fp(indir), _fp(index), _fp(pdf),
+                                          C.byref(C.c_int32())))
+        return dict(pos=pos, flux=flux, indir=indir, index=index, pdf=pdf)
+
+    def read_hitpoints(self):
+        np_ = self.W * self.H
+        t, f = np.zeros(np_, np.int32), np.zeros(np_, np.int32)
+        pos, w, o, n = (np.zeros((np_, 3), np.float32) for _ in range(4))
+        _check(self.lib.sppm_read_hitpoints(self.h, _ip(t), _ip(f), _fp(pos), _fp(w), _fp(o), _fp(n)))
+        return dict(type=t, facet=f, pos=pos, weight=w, outdir=o, normal=n)
+
+    def read_shared(self):
+        np_ = self.W * self.H
+        rad, r = np.zeros((np_, 3), np.float32), np.zeros(np_, np.float32)
+        n, g = np.zeros(np_, np.int32), np.zeros(np_, np.int32)
+        _check(self.lib.sppm_read_shared(self.h, _fp(rad), _fp(r), _ip(n), _ip(g)))
+        return dict(radiance=rad, radius=r, num_photon=n, num_gather=g)
+
+    def read_counts(self):
+        m = np.zeros(self.W * self.H, np.int32)
+        _check(self.lib.sppm_read_counts(self.h, _ip(m)))
+        return m
+
+    def read_neighbours(self):
+        tot = C.c_int64()
+        _check(self.lib.sppm_read_neighbours(self.h, 0, None, None, C.byref(tot)))
+        off = np.zeros(self.W * self.H + 1, np.int64)
+        idx = np.zeros(max(1, tot.value), np.int32)
+        _check(self.lib.sppm_read_neighbours(self.h, tot.value, off.ctypes.data_as(C.POINTER(C.c_int64)), _ip(idx),
+                                             C.byref(tot)))
+        return off, idx[: tot.value]
+
+    def reconstruct(self, total_photons=None, passes=None):
+        total = self.total_photons if total_photons is None else total_photons
+        p = self.passes if passes is None else passes
+        img = np.zeros((self.H, self.W, 4), np.float32)
+        _check(self.lib.sppm_reconstruct(self.h, total, p, _fp(img)))
+        return img
+
+    def trace(self, org, dirs, tmin=1e-4, tmax=3.4028234663852886e38):
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        f = np.zeros(n, np.int32)
+        t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+        _check(self.lib.sppm_trace_closest(self.h, n, _fp(org), _fp(dirs), tmin, tmax, _ip(f), _fp(t), _fp(u), _fp(v)))
+        return f, t, u, v

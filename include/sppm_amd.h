@@ -1,0 +1,207 @@
+/*
+ * sppm_amd.h -- C ABI of the MI355X-native SPPM pass core.
+ *
+ * This is the drop-in boundary for the per-pass loop of FriendsRender R
+ * (SPPM/SPPM/SPPM.cpp:770-854).  Every entry point is plain C: opaque
+ * context handle, plain pointers and sizes, int status (0 = ok, <0 = error,
+ * message in sppm_last_error()).  Device memory is owned by the context;
+ * host buffers are owned by the caller.  A context is bound to one HIP
+ * stream and is not thread-safe; distinct contexts are independent.
+ *
+ * Reference interfaces replaced (file:line under SPPM/SPPM/):
+ *   Scene::CreateScene ............... Scene.cpp:13-218   -> sppm_scene_upload
+ *   ThinLensCamera / PinholeCamera ... Camera.h:72-160    -> sppm_camera_set
+ *   UniformSample/Halton/Hammersley .. CameraSampler.h:19-55 (sampler kind)
+ *   constants ppm_Alpha, FirstRadius . SPPM.cpp:20-21     -> sppm_params
+ *   initializeSharedPoints ........... SPPM.cpp:356-373   -> sppm_begin
+ *   loop body resets ................. SPPM.cpp:771-779   -> sppm_pass
+ *   createPhotonMap .................. SPPM.cpp:495-680   -> sppm_pass_photons
+ *   photon flatten + kdtree::CreateTree SPPM.cpp:812-828, kdtree.h:417-431
+ *                                                          -> sppm_pass_build
+ *   createHitpoints .................. SPPM.cpp:308-353   -> sppm_pass_eye
+ *   applyHitPointInfo ................ SPPM.cpp:395-451   -> sppm_pass_gather
+ *   reconstructImageFromSharedPoints . SPPM.cpp:473-493   -> sppm_reconstruct
+ *   N_BVH<...>::ClosestHit ........... bvh8.h:709-712     -> sppm_trace_closest
+ */
+#ifndef SPPM_AMD_H
+#define SPPM_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPPM_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------- */
+#define SPPM_OK 0
+#define SPPM_E_ARG -1        /* bad argument / size */
+#define SPPM_E_HIP -2        /* a HIP runtime call failed */
+#define SPPM_E_STATE -3      /* call out of order (no scene, no camera, ...) */
+#define SPPM_E_NOMEM -4      /* device allocation failed */
+#define SPPM_E_SCENE -5      /* scene validation failed */
+#define SPPM_E_NUMERIC -6    /* NaN/Inf detected in per-pass flux */
+#define SPPM_E_OVERFLOW -7   /* a fixed-capacity device structure overflowed */
+
+/* ---- material kinds (subclasses of PTUtility::Material::Surface) ------- */
+#define SPPM_MAT_DIFFUSE 0          /* DiffuseSurface   material.h:163-177 */
+#define SPPM_MAT_METAL 1            /* MetalSurface     material.h:396-425 */
+#define SPPM_MAT_ROUGH_CONDUCTOR 2  /* Roughconductor   material.h:222-278 */
+#define SPPM_MAT_GGX_GLASS 3        /* GGXGlass         material.h:280-394 */
+#define SPPM_MAT_GLASS 4            /* GlassSurface     material.h:465-532 */
+
+/* Attribute = SURFACE_PARAM (material.h:35-90) + surface kind + texture
+ * roles map_Kd / map_d / map_bump (Scene.cpp:72-75). */
+typedef struct sppm_attr {
+    int32_t kind;          /* SPPM_MAT_* */
+    float index;           /* SURFACE_PARAM::Index (IOR) */
+    int32_t if_delta;      /* IfDeltaSurface (informational) */
+    int32_t if_opaque;     /* IfOpeque */
+    float albedo[3];       /* Albedo */
+    float roughness;       /* Roughness */
+    float radiance[3];     /* Radiance (emission, Ke) */
+    float reflectance;     /* Reflectance */
+    float extinction;      /* Extinction (conductor k) */
+    int32_t is_sppm;       /* isSPPMSurface: hit points / photons stored here */
+    int32_t tex_albedo;    /* texture id or -1 (map_Kd) */
+    int32_t tex_roughness; /* texture id or -1 (map_d) */
+    int32_t tex_bump;      /* texture id or -1 (map_bump, normal map) */
+} sppm_attr;
+
+/* RGBA32F texture, row-major, texel (x,y) at rgba[4*(x + y*width)]
+ * (Texture2D, Texture.h:10-42; loader convention TextureUtility.cpp:12-35). */
+typedef struct sppm_texture {
+    int32_t width, height;
+    const float* rgba;
+} sppm_texture;
+
+/* Flattened facet list (inoutFacet, Normal_Facet.h:16-59).  Facet i is
+ * triangle i of the closest-hit structure and its FacetID (Scene.cpp:141-142).
+ * pos/nrm/uv: 9 floats per facet (vertex 0,1,2 xyz; uv.z is 0). */
+typedef struct sppm_scene_desc {
+    int32_t nfacets;
+    const float* pos;
+    const float* nrm;
+    const float* uv;
+    const int32_t* attr;    /* attribute id per facet, 0 <= id < nattrs */
+    int32_t nattrs;
+    const sppm_attr* attrs;
+    int32_t ntextures;
+    const sppm_texture* textures;
+} sppm_scene_desc;
+
+/* ---- camera ------------------------------------------------------------ */
+#define SPPM_CAM_PINHOLE 0    /* PinholeCamera  Camera.h:72-87 */
+#define SPPM_CAM_THINLENS 1   /* ThinLensCamera Camera.h:89-160 (no diaphragm) */
+
+#define SPPM_SAMPLER_UNIFORM 0    /* UniformSample(ID): one shared mt19937 stream */
+#define SPPM_SAMPLER_HALTON 1     /* HaltonSample */
+#define SPPM_SAMPLER_HAMMERSLEY 2 /* HammersleySample(ID, NumSample) */
+
+typedef struct sppm_camera_desc {
+    int32_t type;              /* SPPM_CAM_* */
+    int32_t width, height;     /* ScreenWidth, ScreenHeight */
+    float pos[3];              /* CameraPos */
+    float target[3];           /* Target */
+    float fov_deg;             /* Fov (full angle in degrees; half angle = PI*Fov/360) */
+    float F;                   /* focal length F */
+    float dist_to_lens;        /* DistanceToLens */
+    float aspect;              /* AspectRatio */
+    int32_t sampler;           /* SPPM_SAMPLER_* */
+    int32_t sampler_id;        /* UniformSample seed ID */
+    int32_t sampler_nsample;   /* Hammersley NumSample */
+} sppm_camera_desc;
+
+/* ---- parameters (SPPM.cpp:20-21 constants + run shape) ------------------ */
+typedef struct sppm_params {
+    float alpha;               /* ppm_Alpha; reference 0.9 */
+    float first_radius;        /* FirstRadius; reference 0.015 */
+    int32_t photons_per_pass;  /* NumPhotonsPerStep */
+    int32_t bvh_width;         /* 2, 4 or 8 (closest hit is width-independent) */
+    int32_t block_size;        /* threads per block for path kernels (0 = default) */
+    int32_t check_numeric;     /* 1: fail a pass with SPPM_E_NUMERIC on NaN/Inf flux */
+    int32_t record_neighbours; /* 1: keep per-hit-point neighbour lists (debug/parity) */
+    int32_t max_neighbours;    /* capacity of the neighbour list buffer (entries) */
+} sppm_params;
+
+/* Per-pass statistics. */
+typedef struct sppm_pass_stats {
+    int32_t photons_emitted;   /* createPhotonMap return value */
+    int32_t photons_stored;    /* flattened photon count (SPPM.cpp:812-822) */
+    int32_t hit_points;        /* hit points of type HIT */
+    int64_t neighbours;        /* sum over hit points of in-radius photons (M) */
+    float max_radius;          /* max shared radius after the update */
+    int32_t flags;             /* bit0: NaN/Inf seen; bit1: rng cursor overflow */
+    float ms_photons, ms_build, ms_eye, ms_gather;  /* phase times (hipEvent) */
+} sppm_pass_stats;
+
+typedef struct sppm_ctx sppm_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+int sppm_abi_version(void);
+const char* sppm_last_error(void);
+/* device: HIP device ordinal; stream: hipStream_t or NULL for a private one */
+int sppm_create(sppm_ctx** out, int device, void* stream);
+int sppm_destroy(sppm_ctx* ctx);
+
+/* ---- setup (Scene::CreateScene, camera ctor, constants) ------------------ */
+int sppm_scene_upload(sppm_ctx* ctx, const sppm_scene_desc* scene, int bvh_width);
+int sppm_camera_set(sppm_ctx* ctx, const sppm_camera_desc* cam);
+int sppm_params_set(sppm_ctx* ctx, const sppm_params* params);
+/* initializeSharedPoints + photon slot allocation; resets the pass counter. */
+int sppm_begin(sppm_ctx* ctx);
+
+/* ---- per pass ------------------------------------------------------------- */
+/* One full pass (SPPM.cpp:770-843): resets, photon pass (step = pass+1),
+ * gather-structure build, eye pass (step = pass+1), gather + update. */
+int sppm_pass(sppm_ctx* ctx, sppm_pass_stats* stats);
+/* Split steps of the same pass, for tests.  Order: photons, build, eye, gather. */
+int sppm_pass_photons(sppm_ctx* ctx, int step, int32_t* emitted);
+int sppm_pass_build(sppm_ctx* ctx, int32_t* stored);
+int sppm_pass_eye(sppm_ctx* ctx, int step);
+int sppm_pass_gather(sppm_ctx* ctx);
+/* Synchronise the context's stream. */
+int sppm_sync(sppm_ctx* ctx);
+
+/* reconstructImageFromSharedPoints: rgba = W*H*4 floats. total_photons and
+ * passes as the reference passes TotalPhoton and numstep. */
+int sppm_reconstruct(sppm_ctx* ctx, int32_t total_photons, int32_t passes, float* rgba);
+
+/* ---- debug readback (parity tests) ---------------------------------------- */
+/* Flattened photon list of the last build, in reference order:
+ * pos/flux/indir 3 floats each, index (IOR) and pdf 1 float each. */
+int sppm_read_photons(sppm_ctx* ctx, int32_t cap, float* pos, float* flux, float* indir,
+                      float* index, float* pdf, int32_t* count);
+/* Hit points of the last eye pass (index == pixel): type (0 NONE, 1 HIT,
+ * 2 ONLY_DIRECTLIGHT), facet id (-1 if none), position, weight, out dir,
+ * normal (face-forwarded), 3 floats each. */
+int sppm_read_hitpoints(sppm_ctx* ctx, int32_t* type, int32_t* facet, float* pos,
+                        float* weight, float* outdir, float* normal);
+/* Shared hit points: radiance 3f, radius, num_photon (int N), num_gather. */
+int sppm_read_shared(sppm_ctx* ctx, float* radiance, float* radius, int32_t* num_photon,
+                     int32_t* num_gather);
+/* Per-hit-point in-radius photon count M of the last gather. */
+int sppm_read_counts(sppm_ctx* ctx, int32_t* m);
+/* Neighbour lists of the last gather (requires params.record_neighbours):
+ * CSR offsets (W*H+1) and flattened photon indices, each list sorted ascending. */
+int sppm_read_neighbours(sppm_ctx* ctx, int64_t cap, int64_t* offsets, int32_t* idx,
+                         int64_t* total);
+
+/* ---- closest hit (N_BVH::ClosestHit) on a batch of rays ------------------- */
+/* org/dir: n*3 floats (dir used as given), tmin/tmax per batch.  Outputs
+ * facet id (-1 miss), t, u, v.  Ties in t go to the lowest facet id. */
+int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* dir,
+                       float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
+
+/* ---- timing helpers for the benchmark -------------------------------------- */
+/* Average kernel duration (ms) of the named kernel over the last `sppm_pass`
+ * calls, measured with hipEvents on the context stream.  name: "photons",
+ * "build", "eye", "gather". */
+int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPPM_AMD_H */

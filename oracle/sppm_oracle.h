@@ -1,0 +1,130 @@
+/*
+ * sppm_oracle.h -- CPU restatement of the FriendsRender R SPPM pass.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product path: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Every function restates the reference algorithm (file:line under
+ * SPPM/SPPM/ of tabochans/FriendsRender-R-public) with the same float/double
+ * operation order, so that integer outputs (closest-hit facet ids, photon
+ * lists, neighbour sets, counts) are bit-exact with a strict-FP build of the
+ * reference (no FMA contraction, IEEE division/sqrt).
+ *
+ * Pinning (what checks each part, see tests/test_oracle_*.py):
+ *   RNG (mt19937 + generate_canonical) .. C++ KAT + reference MT.h built in oracle/_ref
+ *   camera sampler stream order ......... reference CameraSampler.h built in oracle/_ref
+ *   alias table (TableSampler) .......... reference TableSampler.h built in oracle/_ref
+ *   Hammersley / Halton ................. reference QRandSequence.h built in oracle/_ref
+ *   bilinear texture fetch .............. reference TextureSampler.h built in oracle/_ref
+ *   Matrix3 inverse (bump TBN) .......... reference Matrix.h/.cpp built in oracle/_ref
+ *   closest hit ......................... brute force == BVH, analytic hits
+ *   BSDFs ............................... closed forms, MC pdf normalisation,
+ *                                         reflection/refraction laws
+ *   camera .............................. closed form (central ray, pinhole geometry)
+ *   neighbour sets / update / reconstruct  brute force + hand-built recurrences
+ *   full-path draw order ................ parity unpinned against a reference run
+ *                                         (the reference does not compile as shipped)
+ */
+#ifndef SPPM_ORACLE_H
+#define SPPM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/sppm_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG: RandomMT (MT.h:7-44) ---------------------------------------- */
+typedef struct orc_mt { uint32_t mt[624]; int idx; } orc_mt;
+void orc_mt_seed(orc_mt* m, uint32_t seed);
+uint32_t orc_mt_next(orc_mt* m);
+double orc_mt_real1(orc_mt* m);
+/* n successive genrand64_real1() of RandomMT(seed) */
+void orc_mt_real1_seq(uint32_t seed, int n, double* out);
+/* n successive raw 32-bit outputs of std::mt19937(seed) */
+void orc_mt_raw_seq(uint32_t seed, int n, uint32_t* out);
+/* UniformSample(ID)::Sample() x n calls (CameraSampler.h:25-27), 3 floats per call */
+void orc_uniform_sampler_seq(int id, int ncalls, float* out);
+/* QRandSequence::HaltonCore<B>::Sample(n) and Hammersley(N).Sample(i) */
+float orc_halton(int base, int n);
+void orc_hammersley(int nseq, int i, float* out3);
+
+/* ---- alias table: TableSampler<T>::Init (TableSampler.h:44-120) ---------- */
+/* values f(t_i) for n elements; outputs prob[n], left[n], right[n], *avg. */
+void orc_table_build(const float* values, int n, float* prob, int32_t* left, int32_t* right,
+                     float* avg);
+/* TableSampler::Sample(RandomMT&) for `ndraw` samples from RandomMT(seed). */
+void orc_table_sample_seq(const float* prob, const int32_t* left, const int32_t* right, int n,
+                          uint32_t seed, int ndraw, int32_t* out);
+
+/* ---- textures (TextureSampler.h:60-79, TextureUtility.cpp:161-164) ------- */
+void orc_bilinear(const sppm_texture* tex, float u, float v, float* out4);
+/* Matrix3 inverse through cofactors (Matrix.h:180-211); m, out row-major 9 */
+void orc_matrix3_inv(const float* m, float* out);
+
+/* ---- BSDFs (material.h, Sampling.cpp) ------------------------------------ */
+typedef struct orc_fparam {
+    float in[3], out[3], n[3];
+    float from_index, to_index, roughness, absorption, reflectance;
+} orc_fparam;
+float orc_bsdf_brdf(int kind, const orc_fparam* p);
+float orc_bsdf_pdf(int kind, const orc_fparam* p);
+/* Draws from RandomMT(seed) ; returns direction and *pdf; *ndraw = doubles used */
+void orc_bsdf_sample(int kind, const orc_fparam* p, uint32_t seed, float* dir, float* pdf,
+                     int* ndraw);
+
+/* ---- scene ------------------------------------------------------------- */
+typedef struct orc_scene orc_scene;
+orc_scene* orc_scene_create(const sppm_scene_desc* d);
+void orc_scene_destroy(orc_scene* s);
+int orc_scene_num_lights(const orc_scene* s);
+/* light pick table and per-light data, for tests */
+void orc_scene_light_info(const orc_scene* s, int32_t* facet, float* flux3, int32_t* textured,
+                          float* prob, int32_t* left, int32_t* right, float* avg);
+/* closest hit, brute force over all facets (mode 0) or BVH (mode 1).
+ * Returns facet id or -1; min t then min facet id. */
+int orc_closest_hit(const orc_scene* s, int mode, const float* org, const float* dir, float tmin,
+                    float tmax, float* t, float* u, float* v);
+void orc_closest_hit_batch(const orc_scene* s, int mode, int n, const float* org, const float* dir,
+                           float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
+
+/* ---- camera ------------------------------------------------------------ */
+/* One SampleCameraRay with a given sampler value (3 floats) -> org, dir, pdf */
+void orc_camera_ray(const sppm_camera_desc* c, float img_u, float img_v, const float* sample3,
+                    float* org, float* dir, float* pdf);
+
+/* ---- the SPPM pass ------------------------------------------------------ */
+typedef struct orc_render orc_render;
+orc_render* orc_render_create(const orc_scene* s, const sppm_camera_desc* cam,
+                              const sppm_params* p, int nthreads);
+void orc_render_destroy(orc_render* r);
+/* createPhotonMap(step): returns processed photons */
+int orc_photon_pass(orc_render* r, int step);
+/* flatten slots (SPPM.cpp:812-822); returns stored count */
+int orc_flatten(orc_render* r);
+void orc_read_photons(const orc_render* r, float* pos, float* flux, float* indir, float* index,
+                      float* pdf);
+/* createHitpoints(step) */
+void orc_eye_pass(orc_render* r, int step);
+void orc_read_hitpoints(const orc_render* r, int32_t* type, int32_t* facet, float* pos,
+                        float* weight, float* outdir, float* normal);
+/* applyHitPointInfo; record=1 keeps neighbour lists */
+void orc_gather(orc_render* r, int record);
+void orc_read_counts(const orc_render* r, int32_t* m);
+int64_t orc_neighbour_total(const orc_render* r);
+void orc_read_neighbours(const orc_render* r, int64_t* offsets, int32_t* idx);
+void orc_read_shared(const orc_render* r, float* radiance, float* radius, int32_t* num_photon,
+                     int32_t* num_gather);
+/* set shared state (for hand-built update tests) */
+void orc_write_shared(orc_render* r, const float* radiance, const float* radius,
+                      const int32_t* num_photon, const int32_t* num_gather);
+void orc_reconstruct(const orc_render* r, int32_t total_photons, int32_t passes, float* rgba);
+/* one full pass in reference order (SPPM.cpp:771-841); pass = 0-based index */
+int orc_full_pass(orc_render* r, int pass, int record);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
