@@ -1,0 +1,157 @@
+"""Benchmark: plain SPPM photons/s on one MI355X.
+
+Workload (full-material SPPM): Cornell box + 200,000-triangle displaced
+geodesic sphere, GGX conductor and GGX dielectric spheres, 256x256 RGB8
+checker-textured area light, 512x512 albedo/roughness/normal textures,
+1024x1024 thin-lens camera (aperture 0.02, focus 2.0), 1,000,000 photons per
+pass, alpha 0.7, r0 = 0.02 x scene diagonal, BVH8.  Synthetic seeded scene.
+
+A step is one SPPM pass (photon pass, gather-structure build, eye pass,
+gather + progressive update) over the whole image; the timed region is K
+passes bracketed by device synchronisation.  value = photons emitted / s.
+
+Prints one JSON line.  Extra fields: per-phase kernel times, the roofline of
+the dominant phase (algorithmic bytes per launch / measured launch time,
+hipEvents on the launch stream), and a CPU baseline (the oracle restatement,
+OpenMP, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "friendsrender-r-public_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (lower bounds), per unit of work
+STORE_BYTES_PER_PHOTON = 44           # pos 12 + flux 12 + in-dir 12 + pdf 4 + IOR 4
+BUILD_BYTES_PER_PHOTON = 2 * 44 + 8   # one read, one reordered write, key/index
+GATHER_BYTES_PER_HIT = 68 + 2 * 24    # hit point record + shared state read/write
+GATHER_BYTES_PER_NEIGHBOUR = 44       # each in-radius photon record
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["full", "box"], default="full")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def workload(name):
+    import scenes
+    if name == "full":
+        scene = scenes.full_scene(freq=100, tex_size=512)
+        cam = scenes.thinlens_camera(1024, 1024, aperture=0.02, focus=2.0)
+        return scene, cam, dict(name="full-material SPPM", photons=1_000_000, width=1024, height=1024, bvh=8,
+                                alpha=0.7, r0=0.02 * scene.diagonal(), triangles=scene.nfacets, passes=256)
+    scene = scenes.box_scene()
+    cam = scenes.pinhole_camera(256, 256)
+    return scene, cam, dict(name="box SPPM", photons=100_000, width=256, height=256, bvh=4, alpha=0.7,
+                            r0=0.02 * scene.diagonal(), triangles=scene.nfacets, passes=16)
+
+
+def cpu_baseline(scene, cam, cfg, seconds):
+    """Oracle restatement (OpenMP) on a bounded sample of the workload: the
+    same scene, 1/8 of the pixels and 1/8 of the photons per pass, one pass."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = min(16, os.cpu_count() or 1)
+    frac = 8
+    c2 = dict(cam)
+    c2["width"], c2["height"] = cam["width"] // 4, cam["height"] // 2
+    c2["aspect"] = cam["aspect"]
+    n = cfg["photons"] // frac
+    os_ = pyoracle.OracleScene(scene)
+    r = pyoracle.OracleRender(os_, c2, cfg["alpha"], cfg["r0"], n, nthreads=cores)
+    t0 = time.perf_counter()
+    passes = 0
+    while True:
+        r.full_pass()
+        passes += 1
+        el = time.perf_counter() - t0
+        if el > seconds or passes >= 3:
+            break
+    return dict(value=round(n * passes / el, 1), unit="photons/s", cores=cores, kind="port",
+                sample=f"{passes} pass(es) of the same scene at {c2['width']}x{c2['height']} px with {n} photons/pass "
+                       f"(1/8 of pixels and photons), oracle/sppm_oracle.c OpenMP, {el:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np  # noqa: F401
+    import sppm_amd
+
+    scene, cam, cfg = workload(args.workload)
+    r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"])
+    for _ in range(args.warmup):
+        r.pass_(stats=False)
+    r.sync()
+    c0 = r.counters()
+    r.kernel_ms("reset")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.pass_(stats=False)
+    r.sync()
+    el = time.perf_counter() - t0
+    c1 = r.counters()
+    ms = {k: r.kernel_ms(k) for k in ("photons", "build", "eye", "gather")}
+    K = args.steps
+    hits = (c1["hit_points"] - c0["hit_points"]) / K
+    nbs = (c1["neighbours"] - c0["neighbours"]) / K
+    stored = (c1["stored"] - c0["stored"]) / K
+    algo = {
+        "photons": stored * STORE_BYTES_PER_PHOTON,
+        "build": stored * BUILD_BYTES_PER_PHOTON,
+        "gather": hits * GATHER_BYTES_PER_HIT + nbs * GATHER_BYTES_PER_NEIGHBOUR,
+    }
+    dom = max(("photons", "build", "eye", "gather"), key=lambda k: ms[k])
+    rk = dom if dom in algo else "gather"
+    achieved = algo[rk] / (ms[rk] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    value = cfg["photons"] * K / el
+    out = {
+        "metric": "photons/s (plain SPPM pass, full-material workload)",
+        "value": round(value, 1),
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / K * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": cfg["name"], "resolution": f"{cfg['width']}x{cfg['height']}",
+                   "photons_per_pass": cfg["photons"], "triangles": cfg["triangles"], "bvh_width": cfg["bvh"],
+                   "alpha": cfg["alpha"], "r0": round(cfg["r0"], 6), "camera": cam["type"] and "thin-lens" or "pinhole",
+                   "parallelism": "single GPU"},
+        "roofline": {"kernel": rk, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": int(algo[rk])},
+        "phases_ms": {k: round(v, 3) for k, v in ms.items()},
+        "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
+                     "hit_point_updates_per_s": round(hits / (ms["gather"] * 1e-3), 1)},
+        "cpu_baseline": None,
+    }
+    r.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -358,6 +358,49 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
     }
     out->depth = depth;
     HIPCHK(hipStreamSynchronize(st));
+    // Host validation of the tree before any kernel traverses it: every
+    // internal reference in range, every leaf range inside the triangle
+    // array, every triangle in exactly one leaf, depth as measured.
+    {
+        std::vector<float> hn((size_t)NF * out->nnodes);
+        HIPCHK(hipMemcpy(hn.data(), out->nodes, sizeof(float) * hn.size(), hipMemcpyDeviceToHost));
+        std::vector<unsigned char> seen(n, 0);
+        std::vector<int> stackv{0};
+        std::vector<int> dep{1};
+        std::vector<unsigned char> visited(out->nnodes, 0);
+        int maxd = 0;
+        bool ok = true;
+        while (!stackv.empty() && ok) {
+            int nd = stackv.back(), d = dep.back();
+            stackv.pop_back();
+            dep.pop_back();
+            if (nd < 0 || nd >= out->nnodes || visited[nd]) { ok = false; break; }
+            visited[nd] = 1;
+            maxd = d > maxd ? d : maxd;
+            const uint32_t* cref = reinterpret_cast<const uint32_t*>(hn.data() + (size_t)nd * NF + 6 * width);
+            for (int k = 0; k < width; ++k) {
+                uint32_t r = cref[k];
+                if (r == EMPTY_CHILD) continue;
+                if (r & LEAF_BIT) {
+                    int start = (int)((r & ~LEAF_BIT) >> 3), cnt = (int)(r & 7u) + 1;
+                    if (start < 0 || start + cnt > n) { ok = false; break; }
+                    for (int t = start; t < start + cnt; ++t) {
+                        if (seen[t]) ok = false;
+                        seen[t] = 1;
+                    }
+                } else {
+                    stackv.push_back((int)r);
+                    dep.push_back(d + 1);
+                }
+            }
+        }
+        for (int t = 0; t < n && ok; ++t) ok = seen[t] != 0;
+        if (!ok) {
+            snprintf(err, errlen, "BVH validation failed (bad reference or triangle coverage)");
+            goto fail;
+        }
+        out->depth = maxd;
+    }
     hfree(tbmin); hfree(tbmax); hfree(cb); hfree(keys); hfree(vals); hfree(tk); hfree(tv);
     hfree(hist); hfree(child); hfree(range); hfree(parent); hfree(flag); hfree(nbmin);
     hfree(nbmax); hfree(qa); hfree(qb); hfree(counters);

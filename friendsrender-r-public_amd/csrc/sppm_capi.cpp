@@ -104,6 +104,7 @@ struct RenderBuffers {
     SharedDev shared{};
     int* counts = nullptr;
     int* list_off = nullptr;
+    int* list_scan_tmp = nullptr;
     int* lists = nullptr;
     long long lists_cap = 0;
     uint32_t* words = nullptr;
@@ -111,6 +112,7 @@ struct RenderBuffers {
     int* cam_pos = nullptr;
     uint32_t* rmax_bits = nullptr;
     uint32_t* flags = nullptr;
+    Counters* ctr = nullptr;
     float4* image = nullptr;
     bool have_lists = false;
     void release() {
@@ -118,16 +120,61 @@ struct RenderBuffers {
         dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_count); dfree(cell_start); dfree(cell_scan_tmp);
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
-        dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rmax_bits); dfree(flags); dfree(image);
+        dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rmax_bits); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
 
+// Per-kernel timing: an event pair per launch from a pool; elapsed times are
+// resolved lazily (no synchronisation inside a pass).
 struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
+    std::pair<hipEvent_t, hipEvent_t> cur{nullptr, nullptr};
     double total_ms = 0.0;
     long long n = 0;
+    void start(hipStream_t st) {
+        if (pool.empty()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+            pool.push_back({a, b});
+        }
+        cur = pool.back();
+        pool.pop_back();
+        (void)hipEventRecord(cur.first, st);
+    }
+    void stop(hipStream_t st) {
+        if (!cur.first) return;
+        (void)hipEventRecord(cur.second, st);
+        pending.push_back(cur);
+        cur = {nullptr, nullptr};
+    }
+    void resolve() {
+        for (auto& e : pending) {
+            float ms = 0.0f;
+            if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+                total_ms += ms;
+                n += 1;
+            }
+            pool.push_back(e);
+        }
+        pending.clear();
+    }
+    void reset() {
+        resolve();
+        total_ms = 0.0;
+        n = 0;
+    }
+    void destroy() {
+        resolve();
+        for (auto& e : pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        pool.clear();
+    }
+    double avg() const { return n ? total_ms / n : 0.0; }
 };
 
 struct sppm_ctx {
@@ -172,10 +219,6 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
     HIPOK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     HIPOK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) {
-        HIPOK(hipEventCreate(&t->a));
-        HIPOK(hipEventCreate(&t->b));
-    }
     c->params.alpha = 0.9f;
     c->params.first_radius = 0.015f;
     c->params.photons_per_pass = 0;
@@ -191,10 +234,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipStreamSynchronize(c->stream2);
     c->rb.release();
     c->scene.release();
-    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) {
-        if (t->a) (void)hipEventDestroy(t->a);
-        if (t->b) (void)hipEventDestroy(t->b);
-    }
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->destroy();
     (void)hipEventDestroy(c->ev_fork);
     (void)hipEventDestroy(c->ev_join);
     (void)hipStreamDestroy(c->stream2);
@@ -499,13 +539,16 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
         (rc = dalloc(&R.hits.nrm, np)) || (rc = dalloc(&R.hits.out, np)) || (rc = dalloc(&R.hits.weight, np)) ||
         (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
-        (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np)) || (rc = dalloc(&R.list_off, np + 1)) ||
+        (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
+        (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
         (rc = dalloc(&R.words, 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
-        (rc = dalloc(&R.rmax_bits, 1)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.image, np))) {
+        (rc = dalloc(&R.rmax_bits, 1)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
+        (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
     }
     R.hits.np = np;
+    HIPOK(hipMemsetAsync(R.counts, 0, sizeof(int) * (size_t)(np + 1), c->stream));
     launch_init_slots(R.slots, c->stream);
     launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
     // UniformSample(ID): mt19937(ID) state, position 624 (first draw twists)
@@ -522,10 +565,14 @@ int sppm_begin(sppm_ctx* c) {
     memcpy(&rb, &r0, 4);
     launch_set_u32(R.rmax_bits, rb, c->stream);
     launch_set_u32(R.flags, 0u, c->stream);
+    {
+        Counters z{0, 0, 0};
+        HIPOK(hipMemcpyAsync(R.ctr, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
+    }
     HIPOK(hipGetLastError());
     HIPOK(hipStreamSynchronize(c->stream));
     c->pass = 0;
-    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) { t->total_ms = 0; t->n = 0; }
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->reset();
     R.ready = true;
     R.have_lists = false;
     return SPPM_OK;
@@ -566,7 +613,7 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
 static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
-    launch_build_grid(R.slots, R.off, R.nstored, R.gp, R.rmax_bits, c->scene.bmin, c->scene.ext, R.cell_cap,
+    launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rmax_bits, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_count, R.cell_start, R.cell_scan_tmp, R.keyrank, R.gpos, R.gflux, R.gdir, st);
 }
 static void do_camera_words(sppm_ctx* c, hipStream_t st) {
@@ -582,9 +629,10 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
     GridDev G = grid_dev(c);
     const float alpha = c->params.alpha;
     if (c->params.record_neighbours) {
-        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rmax_bits, R.flags, st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rmax_bits, R.ctr, R.flags, st);
         int total = 0;
-        exclusive_scan(R.counts, R.list_off, R.np + 1, nullptr, R.scan_tmp, st);
+        // counts[np] is kept at zero, so list_off[np] is the total
+        exclusive_scan(R.counts, R.list_off, R.np + 1, nullptr, R.list_scan_tmp, st);
         HIPOK(hipMemcpyAsync(&total, R.list_off + R.np, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPOK(hipStreamSynchronize(st));
         if ((long long)total > R.lists_cap) {
@@ -595,12 +643,12 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
         }
         launch_set_u32(R.rmax_bits, 0u, st);
         launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 2, R.counts, R.list_off, R.lists, R.rmax_bits,
-                      R.flags, st);
+                      R.ctr, R.flags, st);
         R.have_lists = true;
     } else {
         launch_set_u32(R.rmax_bits, 0u, st);
-        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 0, R.counts, nullptr, nullptr, R.rmax_bits, R.flags,
-                      st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 0, R.counts, nullptr, nullptr, R.rmax_bits, R.ctr,
+                      R.flags, st);
     }
     return SPPM_OK;
 }
@@ -641,39 +689,31 @@ int sppm_pass_gather(sppm_ctx* c) {
     return SPPM_OK;
 }
 
-static void tstart(Timer& t, hipStream_t st) { (void)hipEventRecord(t.a, st); }
-static void tstop(Timer& t, hipStream_t st) { (void)hipEventRecord(t.b, st); }
-static void tacc(Timer& t) {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
-        t.total_ms += ms;
-        t.n += 1;
-    }
-}
 
 int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     int rc = check_ready(c);
     if (rc) return rc;
     const int step = c->pass + 1;
     hipStream_t st = c->stream;
-    const bool timing = stats != nullptr;
     // camera stream words on the side stream, overlapping the photon pass
     HIPOK(hipEventRecord(c->ev_fork, st));
     HIPOK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     do_camera_words(c, c->stream2);
     HIPOK(hipEventRecord(c->ev_join, c->stream2));
-    if (timing) tstart(c->t_photons, st);
+    c->t_photons.start(st);
     do_photons(c, step, st);
-    if (timing) { tstop(c->t_photons, st); tstart(c->t_build, st); }
+    c->t_photons.stop(st);
+    c->t_build.start(st);
     do_build(c, st);
-    if (timing) tstop(c->t_build, st);
+    c->t_build.stop(st);
     HIPOK(hipStreamWaitEvent(st, c->ev_join, 0));
-    if (timing) tstart(c->t_eye, st);
+    c->t_eye.start(st);
     do_eye(c, step, st);
-    if (timing) { tstop(c->t_eye, st); tstart(c->t_gather, st); }
+    c->t_eye.stop(st);
+    c->t_gather.start(st);
     rc = do_gather(c, st);
     if (rc) return rc;
-    if (timing) tstop(c->t_gather, st);
+    c->t_gather.stop(st);
     HIPOK(hipGetLastError());
     c->pass += 1;
     if (stats) {
@@ -684,20 +724,33 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         HIPOK(hipMemcpyAsync(&flags, c->rb.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPOK(hipMemcpyAsync(&rmax, c->rb.rmax_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPOK(hipStreamSynchronize(st));
-        tacc(c->t_photons); tacc(c->t_build); tacc(c->t_eye); tacc(c->t_gather);
-        float fl = 0.0f, r;
+        for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->resolve();
+        float r;
         memcpy(&r, &rmax, 4);
-        (void)fl;
         stats->photons_emitted = c->rb.N;
         stats->photons_stored = stored;
         stats->max_radius = r;
         stats->flags = (int)flags;
-        stats->ms_photons = (float)(c->t_photons.n ? c->t_photons.total_ms / c->t_photons.n : 0);
-        stats->ms_build = (float)(c->t_build.n ? c->t_build.total_ms / c->t_build.n : 0);
-        stats->ms_eye = (float)(c->t_eye.n ? c->t_eye.total_ms / c->t_eye.n : 0);
-        stats->ms_gather = (float)(c->t_gather.n ? c->t_gather.total_ms / c->t_gather.n : 0);
+        stats->ms_photons = (float)c->t_photons.avg();
+        stats->ms_build = (float)c->t_build.avg();
+        stats->ms_eye = (float)c->t_eye.avg();
+        stats->ms_gather = (float)c->t_gather.avg();
         if (c->params.check_numeric && (flags & 1u)) return fail(SPPM_E_NUMERIC, "NaN/Inf radiance after pass %d", c->pass);
     }
+    return SPPM_OK;
+}
+
+int sppm_counters(sppm_ctx* c, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
+                  int64_t* passes) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    Counters h{};
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if (hit_points) *hit_points = (int64_t)h.hit_points;
+    if (neighbours) *neighbours = (int64_t)h.neighbours;
+    if (photons_stored) *photons_stored = (int64_t)h.stored;
+    if (passes) *passes = c->pass;
     return SPPM_OK;
 }
 
@@ -711,12 +764,18 @@ int sppm_sync(sppm_ctx* c) {
 int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
     if (!c || !name || !ms) return fail(SPPM_E_ARG, "NULL argument");
     Timer* t = nullptr;
+    if (!strcmp(name, "reset")) {
+        for (Timer* x : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) x->reset();
+        *ms = 0.0f;
+        return SPPM_OK;
+    }
     if (!strcmp(name, "photons")) t = &c->t_photons;
     else if (!strcmp(name, "build")) t = &c->t_build;
     else if (!strcmp(name, "eye")) t = &c->t_eye;
     else if (!strcmp(name, "gather")) t = &c->t_gather;
     if (!t) return fail(SPPM_E_ARG, "unknown kernel name '%s'", name);
-    *ms = (float)(t->n ? t->total_ms / t->n : 0.0);
+    t->resolve();
+    *ms = (float)t->avg();
     return SPPM_OK;
 }
 

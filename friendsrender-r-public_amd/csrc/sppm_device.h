@@ -485,6 +485,8 @@ __device__ inline float4 tex_bilinear(const float4* __restrict__ data, int W, in
     int iy = (int)((float)H * v);
     int x1 = abs(ix) % W;
     int y1 = abs(iy) % H;
+    if (x1 < 0) x1 = 0;  // only for |ix| == INT_MIN (u outside the int range)
+    if (y1 < 0) y1 = 0;
     int x2 = min(W - 1, x1 + 1);
     int y2 = min(H - 1, y1 + 1);
     float4 t00 = data[x1 + y1 * W];

@@ -7,7 +7,7 @@
 
 namespace sppm {
 
-constexpr int PATH_BLOCK = 128;    // lanes per block of the path kernels (2 waves)
+constexpr int PATH_BLOCK = 64;     // lanes per block of the path kernels (1 wave)
 constexpr int GATHER_BLOCK = 256;  // lanes per block of the gather (4 8x8 tiles)
 
 // Grid parameters, computed on the device each pass from the largest radius.
@@ -18,6 +18,13 @@ struct GridParams {
     int nphotons;
 };
 
+// Cumulative device counters (since sppm_begin).
+struct Counters {
+    unsigned long long hit_points;
+    unsigned long long neighbours;
+    unsigned long long stored;
+};
+
 void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st);
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, int cap, uint32_t* flags,
                    hipStream_t st);
@@ -25,7 +32,7 @@ void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, i
                 uint32_t* flags, hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, int cap, int* facet,
                   float* tuv, uint32_t* flags, hipStream_t st);
-void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, GridParams* gp,
+void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const uint32_t* rmax_bits, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
                        hipStream_t st);
@@ -33,7 +40,7 @@ void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, flo
                         hipStream_t st);
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
                    float alpha, int mode, int* counts, const int* list_off, int* lists, uint32_t* rmax_bits,
-                   uint32_t* flags, hipStream_t st);
+                   Counters* ctr, uint32_t* flags, hipStream_t st);
 void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st);
 void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st);
 void launch_init_slots(const SlotsDev& s, hipStream_t st);

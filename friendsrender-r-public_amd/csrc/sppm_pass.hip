@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
 // ---------------------------------------------------------------------------
 __global__ void k_grid_params(GridParams* gp, const uint32_t* __restrict__ rmax_bits, float bmin_x, float bmin_y,
                               float bmin_z, float ext_x, float ext_y, float ext_z, int cell_cap,
-                              const int* __restrict__ nstored) {
+                              const int* __restrict__ nstored, Counters* __restrict__ ctr) {
     float rmax = __uint_as_float(*rmax_bits);
     float cell = rmax > 0.0f ? rmax : 1e-3f;
     float ext[3] = {ext_x, ext_y, ext_z};
@@ -249,6 +249,7 @@ __global__ void k_grid_params(GridParams* gp, const uint32_t* __restrict__ rmax_
     gp->inv_cell = 1.0f / cell;
     gp->bmin[0] = bmin_x; gp->bmin[1] = bmin_y; gp->bmin[2] = bmin_z;
     gp->nphotons = *nstored;
+    ctr->stored += (unsigned long long)*nstored;
 }
 
 __device__ __forceinline__ int cell_coord(float p, float lo, float inv, int n) {
@@ -485,10 +486,11 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_gather(HitDev H, SharedDev SH,
                                                          int mode, int* __restrict__ counts,
                                                          const int* __restrict__ list_off,
                                                          int* __restrict__ lists, uint32_t* __restrict__ rmax_bits,
-                                                         uint32_t* __restrict__ flags) {
+                                                         Counters* __restrict__ ctr, uint32_t* __restrict__ flags) {
     const int lin = blockIdx.x * blockDim.x + threadIdx.x;
     const int tiles = ((W + 7) >> 3) * ((Hh + 7) >> 3) * 64;
     float rnew = 0.0f;
+    int is_hit = 0, nb = 0;
     if (lin < tiles) {
         const int p = tiled_pixel(lin, W, Hh);
         if (p >= 0) {
@@ -561,18 +563,36 @@ __global__ void __launch_bounds__(GATHER_BLOCK) k_gather(HitDev H, SharedDev SH,
                 }
             }
             if (counts) counts[p] = M;
+            is_hit = (type == 1);
+            nb = M;
         }
     }
     // block max of the (new) radius -> next pass's cell size
     __shared__ float red[GATHER_BLOCK / 64];
+    __shared__ int redh[GATHER_BLOCK / 64], redn[GATHER_BLOCK / 64];
     float v = rnew;
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, off, 64));
+        is_hit += __shfl_xor(is_hit, off, 64);
+        nb += __shfl_xor(nb, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = v;
+        redh[threadIdx.x >> 6] = is_hit;
+        redn[threadIdx.x >> 6] = nb;
+    }
     __syncthreads();
     if (threadIdx.x == 0 && mode != 1) {
         float mx = red[0];
-        for (int k = 1; k < GATHER_BLOCK / 64; ++k) mx = fmaxf(mx, red[k]);
+        unsigned long long th = 0, tn = 0;
+        for (int k = 0; k < GATHER_BLOCK / 64; ++k) {
+            mx = fmaxf(mx, red[k]);
+            th += (unsigned long long)redh[k];
+            tn += (unsigned long long)redn[k];
+        }
         atomicMax(rmax_bits, __float_as_uint(mx));  // radii are positive: bit order == value order
+        atomicAdd(&ctr->hit_points, th);
+        atomicAdd(&ctr->neighbours, tn);
     }
 }
 
@@ -658,12 +678,12 @@ void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, 
     else hipLaunchKernelGGL(k_trace<8>, g, b, stack_bytes(cap), st, S, n, org, dir, tmin, cap, facet, tuv, flags);
 }
 
-void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, GridParams* gp,
+void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const uint32_t* rmax_bits, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
                        hipStream_t st) {
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rmax_bits, bmin[0], bmin[1], bmin[2], ext[0],
-                       ext[1], ext[2], cell_cap, nstored);
+                       ext[1], ext[2], cell_cap, nstored, ctr);
     hipLaunchKernelGGL(k_zero_i32, dim3((cell_cap + 1 + 255) / 256), dim3(256), 0, st, cell_count, cell_cap + 1);
     int gP = (slots.npaths + 255) / 256;
     hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, cell_count, keyrank);
@@ -681,10 +701,10 @@ void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, flo
 
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
                    float alpha, int mode, int* counts, const int* list_off, int* lists, uint32_t* rmax_bits,
-                   uint32_t* flags, hipStream_t st) {
+                   Counters* ctr, uint32_t* flags, hipStream_t st) {
     int tiles = ((W + 7) >> 3) * ((Hh + 7) >> 3) * 64;
     hipLaunchKernelGGL(k_gather, dim3((tiles + GATHER_BLOCK - 1) / GATHER_BLOCK), dim3(GATHER_BLOCK), 0, st, H, SH, G,
-                       gp, W, Hh, alpha, mode, counts, list_off, lists, rmax_bits, flags);
+                       gp, W, Hh, alpha, mode, counts, list_off, lists, rmax_bits, ctr, flags);
 }
 
 void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st) {

@@ -25,7 +25,7 @@ EXPORTED = [
     "sppm_camera_set", "sppm_params_set", "sppm_begin", "sppm_pass", "sppm_pass_photons",
     "sppm_pass_build", "sppm_pass_eye", "sppm_pass_gather", "sppm_sync", "sppm_reconstruct",
     "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
-    "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms",
+    "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms", "sppm_counters",
 ]
 
 
@@ -103,6 +103,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
+    lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
     return lib
 
 
@@ -238,6 +239,11 @@ class Renderer:
         v = C.c_float()
         _check(self.lib.sppm_kernel_ms(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    def counters(self):
+        v = [C.c_int64() for _ in range(4)]
+        _check(self.lib.sppm_counters(self.h, *[C.byref(x) for x in v]))
+        return dict(hit_points=v[0].value, neighbours=v[1].value, stored=v[2].value, passes=v[3].value)
 
     def read_photons(self):
         n = C.c_int32()

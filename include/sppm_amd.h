@@ -195,10 +195,17 @@ int sppm_read_neighbours(sppm_ctx* ctx, int64_t cap, int64_t* offsets, int32_t* 
 int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* dir,
                        float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
 
+/* Cumulative counters since sppm_begin: valid hit points gathered (type HIT),
+ * in-radius photons summed over hit points (sum of M), photons stored
+ * (flattened list lengths), passes. */
+int sppm_counters(sppm_ctx* ctx, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
+                  int64_t* passes);
+
 /* ---- timing helpers for the benchmark -------------------------------------- */
-/* Average kernel duration (ms) of the named kernel over the last `sppm_pass`
- * calls, measured with hipEvents on the context stream.  name: "photons",
- * "build", "eye", "gather". */
+/* Average duration (ms) of the named pass phase over the `sppm_pass` calls
+ * since sppm_begin or the last reset, measured with hipEvents recorded on the
+ * context stream around the phase's launches.  name: "photons", "build",
+ * "eye", "gather"; "reset" clears the averages. */
 int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
 
 #ifdef __cplusplus

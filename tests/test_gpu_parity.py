@@ -1,0 +1,167 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar (north star): integer outputs bit-exact -- closest-hit facet ids (ties to
+the lowest id), flattened photon lists (count and order), hit point types
+and facets, per-hit-point neighbour index sets, M / N / NumGather; flux and
+image within 1e-4 relative RMSE (FP32).  Geometry (photon positions,
+directions, hit positions) is compared bit-exact as well, since both sides
+execute the same float operations without contraction.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+import scenes
+import sppm_amd
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+
+
+def rel_rmse(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.sqrt(np.mean(b * b))
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(den, 1e-30))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def random_rays(scene, n, seed):
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(0.02, 0.98, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # a share of rays aimed at triangle vertices / edges (tie cases)
+    k = n // 8
+    tri = scene.pos[rng.integers(0, scene.nfacets, size=k)].reshape(k, 3, 3)
+    w = rng.integers(0, 3, size=k)
+    tgt = tri[np.arange(k), w]
+    d[:k] = tgt - org[:k]
+    d[:k] /= np.linalg.norm(d[:k], axis=1, keepdims=True)
+    return org, d.astype(np.float32)
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+@pytest.mark.parametrize("which", ["box", "full"])
+def test_closest_hit_bit_exact(width, which, box_scene, full_scene_small):
+    scene = box_scene if which == "box" else full_scene_small
+    say(f"closest hit: scene={which} width={width}")
+    org, d = random_rays(scene, 20000, seed=width * 7 + len(which))
+    os_ = pyoracle.OracleScene(scene)
+    fo, to, uo, vo = os_.closest_hit(org, d, mode=0)  # brute force
+    r = sppm_amd.Renderer(scene, scenes.pinhole_camera(8, 8), 0.9, 0.015, 64, bvh_width=width)
+    fg, tg, ug, vg = r.trace(org, d)
+    r.close()
+    np.testing.assert_array_equal(fg, fo)
+    hit = fo >= 0
+    np.testing.assert_array_equal(tg[hit], to[hit])
+    np.testing.assert_array_equal(ug[hit], uo[hit])
+    np.testing.assert_array_equal(vg[hit], vo[hit])
+
+
+def compare_pass(gpu, ref, stored_ref, label):
+    pg = gpu.read_photons()
+    pr = ref.read_photons(stored_ref)
+    assert pg["pos"].shape[0] == stored_ref, f"{label}: stored photons {pg['pos'].shape[0]} != {stored_ref}"
+    np.testing.assert_array_equal(pg["pos"], pr["pos"])
+    np.testing.assert_array_equal(pg["indir"], pr["indir"])
+    np.testing.assert_array_equal(pg["index"], pr["index"])
+    assert rel_rmse(pg["flux"], pr["flux"]) <= REL_TOL
+    assert rel_rmse(pg["pdf"], pr["pdf"]) <= REL_TOL
+    hg, hr = gpu.read_hitpoints(), ref.read_hitpoints()
+    np.testing.assert_array_equal(hg["type"], hr["type"])
+    np.testing.assert_array_equal(hg["facet"], hr["facet"])
+    hit = hr["type"] == 1
+    np.testing.assert_array_equal(hg["pos"][hit], hr["pos"][hit])
+    np.testing.assert_array_equal(hg["normal"][hit], hr["normal"][hit])
+    np.testing.assert_array_equal(hg["outdir"][hit], hr["outdir"][hit])
+    assert rel_rmse(hg["weight"][hit], hr["weight"][hit]) <= REL_TOL
+    np.testing.assert_array_equal(gpu.read_counts(), ref.read_counts())
+    og, ig = gpu.read_neighbours()
+    orf, irf = ref.read_neighbours()
+    np.testing.assert_array_equal(og, orf)
+    np.testing.assert_array_equal(ig, irf)
+    sg, sr = gpu.read_shared(), ref.read_shared()
+    np.testing.assert_array_equal(sg["num_photon"], sr["num_photon"])
+    np.testing.assert_array_equal(sg["num_gather"], sr["num_gather"])
+    assert rel_rmse(sg["radius"], sr["radius"]) <= 1e-6
+    e = rel_rmse(sg["radiance"], sr["radiance"])
+    assert e <= REL_TOL, f"{label}: radiance rel RMSE {e}"
+    return e
+
+
+def run_parity(scene, cam, alpha, r0, N, passes, width, label):
+    gpu = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=width, record=True)
+    ref = pyoracle.OracleRender(pyoracle.OracleScene(scene), cam, alpha, r0, N)
+    for p in range(passes):
+        say(f"{label}: pass {p}")
+        st = gpu.pass_()
+        assert st.flags == 0, f"device flags {st.flags:#x}"
+        ref.full_pass(record=True)
+        stored_ref = pyoracle.orc().orc_flatten(ref.h)
+        assert st.photons_stored == stored_ref
+        compare_pass(gpu, ref, stored_ref, f"{label} pass {p}")
+    img_g, img_r = gpu.reconstruct(), ref.reconstruct()
+    assert np.isfinite(img_g).all()
+    e = rel_rmse(img_g[..., :3], img_r[..., :3])
+    gpu.close()
+    assert e <= REL_TOL, f"{label}: image rel RMSE {e}"
+
+
+def test_pass_parity_box_reference_constants(box_scene):
+    # reference constants: alpha 0.9, FirstRadius 0.015 (SPPM.cpp:20-21)
+    run_parity(box_scene, scenes.pinhole_camera(64, 64), 0.9, 0.015, 20000, 3, 4, "box a=0.9")
+
+
+def test_pass_parity_box_config_constants(box_scene):
+    # configured constants: alpha 0.7, r0 = 0.02 x scene diagonal
+    r0 = 0.02 * box_scene.diagonal()
+    run_parity(box_scene, scenes.pinhole_camera(64, 48), 0.7, r0, 15000, 3, 8, "box a=0.7")
+
+
+def test_pass_parity_full_materials_thinlens(full_scene_small):
+    r0 = 0.02 * full_scene_small.diagonal()
+    run_parity(full_scene_small, scenes.thinlens_camera(48, 48), 0.7, r0, 15000, 2, 8, "full thinlens")
+
+
+def test_halton_and_hammersley_samplers(box_scene):
+    for smp, nsample in ((sppm_amd.SAMPLER_HALTON, 1), (sppm_amd.SAMPLER_HAMMERSLEY, 16)):
+        cam = scenes.pinhole_camera(32, 32, sampler=smp)
+        cam["sampler_nsample"] = nsample
+        run_parity(box_scene, cam, 0.9, 0.015, 5000, 2, 2, f"sampler {smp}")
+
+
+def test_full_size_box_properties(box_scene):
+    """Box workload at full size (256^2, 100k photons, 16 passes, BVH4):
+    properties that need no oracle run -- radius never grows, counters add
+    up, the run is deterministic in its integer outputs, image finite."""
+    cam = scenes.pinhole_camera(256, 256)
+    r0 = 0.02 * box_scene.diagonal()
+    runs = []
+    for rep in range(2):
+        say(f"full-size box run {rep}")
+        g = sppm_amd.Renderer(box_scene, cam, 0.7, r0, 100000, bvh_width=4)
+        prev_r = None
+        msum = 0
+        for p in range(16):
+            st = g.pass_()
+            assert st.flags == 0
+            assert st.photons_emitted == 100000
+            sh = g.read_shared()
+            if prev_r is not None:
+                assert (sh["radius"] <= prev_r).all()
+            prev_r = sh["radius"]
+            msum += int(g.read_counts().sum())
+        c = g.counters()
+        assert c["neighbours"] == msum and c["passes"] == 16
+        img = g.reconstruct()
+        assert np.isfinite(img).all()
+        runs.append((sh["num_photon"].copy(), sh["num_gather"].copy(), c))
+        g.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
