@@ -110,7 +110,14 @@ struct RenderBuffers {
     uint32_t* words = nullptr;
     uint32_t* cam_mt = nullptr;
     int* cam_pos = nullptr;
-    uint32_t* rmax_bits = nullptr;
+    unsigned long long* rstat = nullptr;  // (radius sum in 2^-32 units, hit points) of the last gather
+    TileRange* tiles = nullptr;
+    int ntiles = 0;
+    int* chunkp = nullptr;
+    int* nch = nullptr;
+    int* chunk_off = nullptr;
+    int* tile_scan_tmp = nullptr;
+    float4* partial = nullptr;
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
     float4* image = nullptr;
@@ -121,7 +128,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rmax_bits); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -542,7 +549,12 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
         (rc = dalloc(&R.words, 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
-        (rc = dalloc(&R.rmax_bits, 1)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
+        (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
+        (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)gather_tile_count(W, Hh))) ||
+        (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, gather_tile_count(W, Hh) + 1)) ||
+        (rc = dalloc(&R.chunk_off, gather_tile_count(W, Hh) + 1)) ||
+        (rc = dalloc(&R.tile_scan_tmp, scan_tmp_entries(gather_tile_count(W, Hh) + 1))) ||
+        (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
@@ -560,10 +572,8 @@ int sppm_begin(sppm_ctx* c) {
         HIPOK(hipMemcpyAsync(R.cam_mt, mt, sizeof mt, hipMemcpyHostToDevice, c->stream));
         HIPOK(hipMemcpyAsync(R.cam_pos, &pos, sizeof pos, hipMemcpyHostToDevice, c->stream));
     }
-    float r0 = c->params.first_radius;
-    uint32_t rb;
-    memcpy(&rb, &r0, 4);
-    launch_set_u32(R.rmax_bits, rb, c->stream);
+    R.ntiles = gather_tile_count(W, Hh);
+    launch_set_u64x2(R.rstat, (unsigned long long)((double)c->params.first_radius * 4294967296.0), 1ull, c->stream);
     launch_set_u32(R.flags, 0u, c->stream);
     {
         Counters z{0, 0, 0};
@@ -613,7 +623,7 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
 static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
-    launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rmax_bits, c->scene.bmin, c->scene.ext, R.cell_cap,
+    launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_count, R.cell_start, R.cell_scan_tmp, R.keyrank, R.gpos, R.gflux, R.gdir, st);
 }
 static void do_camera_words(sppm_ctx* c, hipStream_t st) {
@@ -629,7 +639,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
     GridDev G = grid_dev(c);
     const float alpha = c->params.alpha;
     if (c->params.record_neighbours) {
-        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rmax_bits, R.ctr, R.flags, st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rstat, R.ctr, R.flags, st);
         int total = 0;
         // counts[np] is kept at zero, so list_off[np] is the total
         exclusive_scan(R.counts, R.list_off, R.np + 1, nullptr, R.list_scan_tmp, st);
@@ -641,14 +651,14 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
             if (rc) return rc;
             R.lists_cap = total;
         }
-        launch_set_u32(R.rmax_bits, 0u, st);
-        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 2, R.counts, R.list_off, R.lists, R.rmax_bits,
+        launch_set_u64x2(R.rstat, 0ull, 0ull, st);
+        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 2, R.counts, R.list_off, R.lists, R.rstat,
                       R.ctr, R.flags, st);
         R.have_lists = true;
     } else {
-        launch_set_u32(R.rmax_bits, 0u, st);
-        launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 0, R.counts, nullptr, nullptr, R.rmax_bits, R.ctr,
-                      R.flags, st);
+        launch_set_u64x2(R.rstat, 0ull, 0ull, st);
+        launch_gather_balanced(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, R.tiles, GATHER_CHUNK_CAP, R.chunkp, R.nch,
+                               R.chunk_off, R.tile_scan_tmp, R.partial, R.counts, R.rstat, R.ctr, R.flags, st);
     }
     return SPPM_OK;
 }
@@ -719,17 +729,18 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     if (stats) {
         memset(stats, 0, sizeof *stats);
         int stored = 0;
-        uint32_t flags = 0, rmax = 0;
+        uint32_t flags = 0;
+        unsigned long long rs[2] = {0, 0};
         HIPOK(hipMemcpyAsync(&stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPOK(hipMemcpyAsync(&flags, c->rb.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPOK(hipMemcpyAsync(&rmax, c->rb.rmax_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPOK(hipMemcpyAsync(rs, c->rb.rstat, sizeof(rs), hipMemcpyDeviceToHost, st));
         HIPOK(hipStreamSynchronize(st));
         for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->resolve();
-        float r;
-        memcpy(&r, &rmax, 4);
+        float r = rs[1] ? (float)(((double)rs[0] / 4294967296.0) / (double)rs[1]) : 0.0f;
         stats->photons_emitted = c->rb.N;
         stats->photons_stored = stored;
-        stats->max_radius = r;
+        stats->mean_radius = r;
+        stats->hit_points = (int32_t)rs[1];
         stats->flags = (int)flags;
         stats->ms_photons = (float)c->t_photons.avg();
         stats->ms_build = (float)c->t_build.avg();

@@ -8,7 +8,8 @@
 namespace sppm {
 
 constexpr int PATH_BLOCK = 64;     // lanes per block of the path kernels (1 wave)
-constexpr int GATHER_BLOCK = 256;  // lanes per block of the gather (4 8x8 tiles)
+constexpr int GATHER_WAVES = 4096; // persistent waves of the chunked gather
+constexpr int GATHER_CHUNK_CAP = 65536;  // chunk budget per pass (partial sums buffer)
 
 // Grid parameters, computed on the device each pass from the largest radius.
 struct GridParams {
@@ -33,14 +34,22 @@ void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, i
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, int cap, int* facet,
                   float* tuv, uint32_t* flags, hipStream_t st);
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
-                       const uint32_t* rmax_bits, const float* bmin, const float* ext, int cell_cap, int* cell_count,
+                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
                        hipStream_t st);
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,
                         hipStream_t st);
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
-                   float alpha, int mode, int* counts, const int* list_off, int* lists, uint32_t* rmax_bits,
+                   float alpha, int mode, int* counts, const int* list_off, int* lists, unsigned long long* rstat,
                    Counters* ctr, uint32_t* flags, hipStream_t st);
+struct TileRange;
+void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W,
+                            int Hh, float alpha, TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off,
+                            int* scan_tmp, float4* partial, int* counts, unsigned long long* rstat, Counters* ctr,
+                            uint32_t* flags, hipStream_t st);
+int gather_tile_count(int W, int Hh);
+size_t gather_tile_bytes();
+void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st);
 void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st);
 void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st);
 void launch_init_slots(const SlotsDev& s, hipStream_t st);

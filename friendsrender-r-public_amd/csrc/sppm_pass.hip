@@ -36,40 +36,54 @@ __device__ __forceinline__ uint32_t temper(uint32_t z) {
     return z;
 }
 
-__global__ void __launch_bounds__(64) k_camera_words(uint32_t* __restrict__ st_mt, int* __restrict__ st_pos,
-                                                     uint32_t* __restrict__ words, long long need) {
+// 256 lanes twist the state in its three dependency phases: new x[k] for
+// k < 227 uses old words only; k in [227, 454) uses new x[k-227] from phase
+// 1; k in [454, 624) uses new x[k-227] from phase 2 (and k = 623 the new
+// x[0]).  Inside a phase every lane reads before any lane writes.
+constexpr int CAMW_THREADS = 256;
+__device__ __forceinline__ uint32_t mt_mix(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return c ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+__global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restrict__ st_mt, int* __restrict__ st_pos,
+                                                               uint32_t* __restrict__ words, long long need) {
     __shared__ uint32_t x[624];
     const int t = threadIdx.x;
-    for (int k = t; k < 624; k += 64) x[k] = st_mt[k];
+    for (int k = t; k < 624; k += CAMW_THREADS) x[k] = st_mt[k];
     int pos = *st_pos;
     __syncthreads();
     long long out = 0;
     while (out < need) {
         if (pos >= 624) {
-            for (int base = 0; base < 624; base += 64) {
-                const int k = base + t;
-                uint32_t nv = 0;
-                if (k < 624) {
-                    uint32_t a = x[k];
-                    uint32_t b = (k + 1 < 624) ? x[k + 1] : x[0];
-                    uint32_t c = (k < 227) ? x[k + 397] : x[k - 227];
-                    uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
-                    nv = c ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-                }
-                __syncthreads();
-                if (k < 624) x[k] = nv;
-                __syncthreads();
+            uint32_t nv = 0;
+            // phase 1: k in [0, 227)
+            if (t < 227) nv = mt_mix(x[t], x[t + 1], x[t + 397]);
+            __syncthreads();
+            if (t < 227) x[t] = nv;
+            __syncthreads();
+            // phase 2: k in [227, 454)
+            if (t < 227) nv = mt_mix(x[227 + t], x[228 + t], x[t]);
+            __syncthreads();
+            if (t < 227) x[227 + t] = nv;
+            __syncthreads();
+            // phase 3: k in [454, 624)
+            if (t < 170) {
+                const int k = 454 + t;
+                nv = mt_mix(x[k], (k + 1 < 624) ? x[k + 1] : x[0], x[k - 227]);
             }
+            __syncthreads();
+            if (t < 170) x[454 + t] = nv;
+            __syncthreads();
             pos = 0;
         }
         long long rem = need - out;
         int n = (624 - pos) < rem ? (624 - pos) : (int)rem;
-        for (int j = t; j < n; j += 64) words[out + j] = temper(x[pos + j]);
+        for (int j = t; j < n; j += CAMW_THREADS) words[out + j] = temper(x[pos + j]);
         out += n;
         pos += n;
     }
     __syncthreads();
-    for (int k = t; k < 624; k += 64) st_mt[k] = x[k];
+    for (int k = t; k < 624; k += CAMW_THREADS) st_mt[k] = x[k];
     if (t == 0) *st_pos = pos;
 }
 
@@ -226,11 +240,14 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
 // are clamped to the scene box; the same monotone map is used for photons
 // and query boxes, so no neighbour is ever missed.
 // ---------------------------------------------------------------------------
-__global__ void k_grid_params(GridParams* gp, const uint32_t* __restrict__ rmax_bits, float bmin_x, float bmin_y,
+__global__ void k_grid_params(GridParams* gp, const unsigned long long* __restrict__ rstat, float bmin_x, float bmin_y,
                               float bmin_z, float ext_x, float ext_y, float ext_z, int cell_cap,
                               const int* __restrict__ nstored, Counters* __restrict__ ctr) {
-    float rmax = __uint_as_float(*rmax_bits);
-    float cell = rmax > 0.0f ? rmax : 1e-3f;
+    // cell size = mean radius of the hit points after the previous gather;
+    // any cell size gives the same neighbour sets (queries walk every cell
+    // their box overlaps), this one keeps ~3 cells per query axis.
+    const double mean = rstat[1] ? ((double)rstat[0] / 4294967296.0) / (double)rstat[1] : 1e-3;
+    float cell = (float)mean > 1e-5f ? (float)mean : 1e-5f;
     float ext[3] = {ext_x, ext_y, ext_z};
     int n[3];
     for (;;) {
@@ -481,117 +498,382 @@ __device__ __forceinline__ int tiled_pixel(int lin, int W, int H) {
     return y * W + x;
 }
 
-__global__ void __launch_bounds__(GATHER_BLOCK) k_gather(HitDev H, SharedDev SH, GridDev G,
-                                                         const GridParams* __restrict__ gp, int W, int Hh, float alpha,
-                                                         int mode, int* __restrict__ counts,
-                                                         const int* __restrict__ list_off,
-                                                         int* __restrict__ lists, uint32_t* __restrict__ rmax_bits,
-                                                         Counters* __restrict__ ctr, uint32_t* __restrict__ flags) {
-    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tiles = ((W + 7) >> 3) * ((Hh + 7) >> 3) * 64;
-    float rnew = 0.0f;
-    int is_hit = 0, nb = 0;
-    if (lin < tiles) {
-        const int p = tiled_pixel(lin, W, Hh);
-        if (p >= 0) {
-            float4 hp = H.pos[p];
-            float4 sr = SH.rad[p];
-            rnew = sr.w;
-            int type = (int)hp.w;
-            int M = 0;
-            if (type == 1) {
-                const GridParams g = *gp;
-                const f3 c = mk(hp.x, hp.y, hp.z);
-                const float R = sr.w;
-                const f3 bbmin = c - mk(1.0f, 1.0f, 1.0f) * R;
-                const f3 bbmax = c + mk(1.0f, 1.0f, 1.0f) * R;
-                const float R2 = R * R;
-                const int x0 = cell_coord(bbmin.x, g.bmin[0], g.inv_cell, g.nx);
-                const int x1 = cell_coord(bbmax.x, g.bmin[0], g.inv_cell, g.nx);
-                const int y0 = cell_coord(bbmin.y, g.bmin[1], g.inv_cell, g.ny);
-                const int y1 = cell_coord(bbmax.y, g.bmin[1], g.inv_cell, g.ny);
-                const int z0 = cell_coord(bbmin.z, g.bmin[2], g.inv_cell, g.nz);
-                const int z1 = cell_coord(bbmax.z, g.bmin[2], g.inv_cell, g.nz);
-                float4 hn = H.nrm[p], ho = H.out[p], hw = H.weight[p];
-                const int kind = H.mat[p].x;
-                const float absorb = H.absorb[p];
-                const f3 N = mk(hn.x, hn.y, hn.z);
-                const f3 outd = mk(ho.x, ho.y, ho.z);
-                const f3 wgt = mk(hw.x, hw.y, hw.z);
-                f3 add = mk(0.0f, 0.0f, 0.0f);
-                int lo = (mode == 2) ? list_off[p] : 0;
-                for (int z = z0; z <= z1; ++z)
-                    for (int y = y0; y <= y1; ++y) {
-                        const int row = (z * g.ny + y) * g.nx;
-                        const int qa = G.cell_start[row + x0];
-                        const int qb = G.cell_start[row + x1 + 1];
-                        for (int q = qa; q < qb; ++q) {
-                            const float4 pp = G.pos[q];
-                            // isIncludePoint (kdtree.h:78-88) then the strict radius test (SPPM.cpp:432)
-                            if (bbmin.x > pp.x || bbmax.x < pp.x) continue;
-                            if (bbmin.y > pp.y || bbmax.y < pp.y) continue;
-                            if (bbmin.z > pp.z || bbmax.z < pp.z) continue;
-                            const f3 d = mk(pp.x, pp.y, pp.z) - c;
-                            if (!(norm2(d) < R2)) continue;
-                            ++M;
-                            if (mode == 1) continue;
-                            const float4 pf = G.flux[q];
-                            const float4 pd = G.dir[q];
-                            const f3 indir = mk(pd.x, pd.y, pd.z);
-                            const float Ii = pp.w;
-                            const float Io = near1(pp.w) ? hn.w : 1.0f;
-                            const f3 nrm = dot(indir, N) < 0 ? N : -N;
-                            FParam fp{indir, outd, nrm, Ii, Io, ho.w, absorb, hw.w};
-                            const float b = surf_brdf(kind, fp);
-                            add = add + ((wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
-                            if (mode == 2) lists[lo++] = __float_as_int(pd.w);
+// One neighbour candidate: the reference's two tests, then BRDF accumulation.
+struct GatherLane {
+    f3 c, bbmin, bbmax, N, outd, wgt, add;
+    float R2, ior, rough, refl, absorb;
+    int kind, M, lo;
+};
+__device__ __forceinline__ void gather_candidate(GatherLane& L, const float4 pp, const float4 pf, const float4 pd,
+                                                 int mode, int* __restrict__ lists) {
+    // isIncludePoint (kdtree.h:78-88) then the strict radius test (SPPM.cpp:432)
+    if (L.bbmin.x > pp.x || L.bbmax.x < pp.x) return;
+    if (L.bbmin.y > pp.y || L.bbmax.y < pp.y) return;
+    if (L.bbmin.z > pp.z || L.bbmax.z < pp.z) return;
+    const f3 d = mk(pp.x, pp.y, pp.z) - L.c;
+    if (!(norm2(d) < L.R2)) return;
+    ++L.M;
+    if (mode == 1) return;
+    const f3 indir = mk(pd.x, pd.y, pd.z);
+    const float Io = near1(pp.w) ? L.ior : 1.0f;
+    const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
+    FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
+    const float b = surf_brdf(L.kind, fp);
+    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
+    if (mode == 2) lists[L.lo++] = __float_as_int(pd.w);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One wave per 8x8 pixel tile.  The tile's hit points usually lie on one
+// small surface patch, so their query boxes overlap almost entirely: the
+// wave walks the cells of the union box once, stages 64 photons at a time in
+// LDS with coalesced loads, and every lane tests each staged photon against
+// its own hit point (LDS broadcast reads).  Tiles whose union box is much
+// larger than the lanes' own boxes (silhouettes, depth edges) walk their own
+// cells per lane instead.  Either way each lane applies exactly the
+// reference's box + radius tests, so the neighbour set is unchanged.
+__global__ void __launch_bounds__(64) k_gather(HitDev H, SharedDev SH, GridDev G, const GridParams* __restrict__ gp,
+                                               int W, int Hh, float alpha, int mode, int* __restrict__ counts,
+                                               const int* __restrict__ list_off, int* __restrict__ lists,
+                                               unsigned long long* __restrict__ rstat, Counters* __restrict__ ctr,
+                                               uint32_t* __restrict__ flags) {
+    __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
+    const int lane = threadIdx.x;
+    const int p = tiled_pixel(blockIdx.x * 64 + lane, W, Hh);
+    float4 hp = make_float4(0, 0, 0, 0), sr = make_float4(0, 0, 0, 0);
+    if (p >= 0) {
+        hp = H.pos[p];
+        sr = SH.rad[p];
+    }
+    const bool active = p >= 0 && (int)hp.w == 1;
+    GatherLane L;
+    L.M = 0;
+    L.lo = 0;
+    L.add = mk(0.0f, 0.0f, 0.0f);
+    const float R = sr.w;
+    L.c = mk(hp.x, hp.y, hp.z);
+    L.bbmin = L.c - mk(1.0f, 1.0f, 1.0f) * R;
+    L.bbmax = L.c + mk(1.0f, 1.0f, 1.0f) * R;
+    L.R2 = R * R;
+    const GridParams g = *gp;
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1, z0 = 0, z1 = -1;
+    if (active) {
+        x0 = cell_coord(L.bbmin.x, g.bmin[0], g.inv_cell, g.nx); x1 = cell_coord(L.bbmax.x, g.bmin[0], g.inv_cell, g.nx);
+        y0 = cell_coord(L.bbmin.y, g.bmin[1], g.inv_cell, g.ny); y1 = cell_coord(L.bbmax.y, g.bmin[1], g.inv_cell, g.ny);
+        z0 = cell_coord(L.bbmin.z, g.bmin[2], g.inv_cell, g.nz); z1 = cell_coord(L.bbmax.z, g.bmin[2], g.inv_cell, g.nz);
+        float4 hn = H.nrm[p], ho = H.out[p], hw = H.weight[p];
+        L.N = mk(hn.x, hn.y, hn.z);
+        L.ior = hn.w;
+        L.outd = mk(ho.x, ho.y, ho.z);
+        L.rough = ho.w;
+        L.wgt = mk(hw.x, hw.y, hw.z);
+        L.refl = hw.w;
+        L.kind = H.mat[p].x;
+        L.absorb = H.absorb[p];
+        if (mode == 2) L.lo = list_off[p];
+    }
+    const uint64_t act = __ballot(active);
+    if (act) {
+        // union of the active lanes' cell ranges
+        const int X0 = -wave_max_i(active ? -x0 : -0x7fffffff), X1 = wave_max_i(active ? x1 : -1);
+        const int Y0 = -wave_max_i(active ? -y0 : -0x7fffffff), Y1 = wave_max_i(active ? y1 : -1);
+        const int Z0 = -wave_max_i(active ? -z0 : -0x7fffffff), Z1 = wave_max_i(active ? z1 : -1);
+        const int own = active ? (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) : 0;
+        const int own_max = wave_max_i(own);
+        const long long ucells = (long long)(X1 - X0 + 1) * (Y1 - Y0 + 1) * (Z1 - Z0 + 1);
+        if (ucells <= 8LL * own_max) {
+            for (int z = Z0; z <= Z1; ++z)
+                for (int y = Y0; y <= Y1; ++y) {
+                    const int row = (z * g.ny + y) * g.nx;
+                    const int qa = G.cell_start[row + X0];
+                    const int qb = G.cell_start[row + X1 + 1];
+                    const bool mine = active && z >= z0 && z <= z1 && y >= y0 && y <= y1;
+                    for (int base = qa; base < qb; base += 64) {
+                        const int q = base + lane;
+                        if (q < qb) {
+                            s_pos[lane] = G.pos[q];
+                            s_flux[lane] = G.flux[q];
+                            s_dir[lane] = G.dir[q];
                         }
+                        __syncthreads();
+                        const int n = min(64, qb - base);
+                        if (mine)
+                            for (int j = 0; j < n; ++j) gather_candidate(L, s_pos[j], s_flux[j], s_dir[j], mode, lists);
+                        __syncthreads();
                     }
-                if (mode != 1 && M > 0) {
-                    int2 cn = SH.cnt[p];
-                    const float num = (float)cn.x + alpha * (float)M;
-                    const float den = (float)(cn.x + M);
-                    f3 r = mk(sr.x, sr.y, sr.z);
-                    r = ((r + add) * num) / den;
-                    const float R1 = sr.w * sqrtf_(num / den);
-                    SH.rad[p] = make_float4(r.x, r.y, r.z, R1);
-                    SH.cnt[p] = make_int2((int)((float)cn.x + (float)M * alpha), cn.y + 1);
-                    rnew = R1;
-                    if (!(r.x == r.x) || !(r.y == r.y) || !(r.z == r.z) || __builtin_isinf(r.x) ||
-                        __builtin_isinf(r.y) || __builtin_isinf(r.z))
-                        atomicOr(flags, 1u);
                 }
-            }
-            if (counts) counts[p] = M;
-            is_hit = (type == 1);
-            nb = M;
+        } else if (active) {
+            for (int z = z0; z <= z1; ++z)
+                for (int y = y0; y <= y1; ++y) {
+                    const int row = (z * g.ny + y) * g.nx;
+                    const int qa = G.cell_start[row + x0];
+                    const int qb = G.cell_start[row + x1 + 1];
+                    for (int q = qa; q < qb; ++q) gather_candidate(L, G.pos[q], G.flux[q], G.dir[q], mode, lists);
+                }
         }
     }
-    // block max of the (new) radius -> next pass's cell size
-    __shared__ float red[GATHER_BLOCK / 64];
-    __shared__ int redh[GATHER_BLOCK / 64], redn[GATHER_BLOCK / 64];
-    float v = rnew;
-    for (int off = 32; off > 0; off >>= 1) {
-        v = fmaxf(v, __shfl_xor(v, off, 64));
-        is_hit += __shfl_xor(is_hit, off, 64);
-        nb += __shfl_xor(nb, off, 64);
+    float rnew = p >= 0 ? sr.w : 0.0f;
+    if (active && mode != 1 && L.M > 0) {
+        // progressive update (SPPM.cpp:444-449); N is an int: N += alpha*M truncates
+        int2 cn = SH.cnt[p];
+        const float num = (float)cn.x + alpha * (float)L.M;
+        const float den = (float)(cn.x + L.M);
+        f3 r = mk(sr.x, sr.y, sr.z);
+        r = ((r + L.add) * num) / den;
+        const float R1 = sr.w * sqrtf_(num / den);
+        SH.rad[p] = make_float4(r.x, r.y, r.z, R1);
+        SH.cnt[p] = make_int2((int)((float)cn.x + (float)L.M * alpha), cn.y + 1);
+        rnew = R1;
+        if (!(r.x == r.x) || !(r.y == r.y) || !(r.z == r.z) || __builtin_isinf(r.x) || __builtin_isinf(r.y) ||
+            __builtin_isinf(r.z))
+            atomicOr(flags, 1u);
     }
-    if ((threadIdx.x & 63) == 0) {
-        red[threadIdx.x >> 6] = v;
-        redh[threadIdx.x >> 6] = is_hit;
-        redn[threadIdx.x >> 6] = nb;
+    if (p >= 0 && counts) counts[p] = active ? L.M : 0;
+    // wave totals: radius sum (next pass's cell size), hit points, neighbours
+    unsigned long long rs = active ? (unsigned long long)((double)rnew * 4294967296.0) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off, 64);
+    const int th = wave_sum_i(active ? 1 : 0);
+    const int tn = wave_sum_i(L.M);
+    if (lane == 0 && mode != 1) {
+        atomicAdd(&rstat[0], rs);
+        atomicAdd(&rstat[1], (unsigned long long)th);
+        atomicAdd(&ctr->hit_points, (unsigned long long)th);
+        atomicAdd(&ctr->neighbours, (unsigned long long)tn);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Load-balanced gather for the production pass (no neighbour lists).
+// Candidate counts per 8x8 tile vary by orders of magnitude (caustics), so:
+//   k_gather_plan    per tile: union cell box of its hit points and its
+//                    candidate count
+//   k_chunk_size     candidates per chunk so that all chunks fit the buffer
+//   k_tile_chunks    chunks per tile
+//   scan             chunk offsets per tile
+//   k_gather_chunks  persistent waves take chunks of one tile's candidates,
+//                    stage them in LDS and write per-lane partial sums
+//   k_gather_finish  per tile: partial sums in chunk order + the update
+// ---------------------------------------------------------------------------
+struct TileRange {
+    int x0, x1, y0, y1, z0, z1, rowlen_pad, cand;
+};
+
+__device__ __forceinline__ void load_gather_lane(const HitDev& H, const SharedDev& SH, const GridParams& g, int p,
+                                                 GatherLane& L, bool& active, float4& sr, int* rng) {
+    float4 hp = make_float4(0, 0, 0, 0);
+    sr = make_float4(0, 0, 0, 0);
+    if (p >= 0) {
+        hp = H.pos[p];
+        sr = SH.rad[p];
+    }
+    active = p >= 0 && (int)hp.w == 1;
+    L.M = 0;
+    L.lo = 0;
+    L.add = mk(0.0f, 0.0f, 0.0f);
+    const float R = sr.w;
+    L.c = mk(hp.x, hp.y, hp.z);
+    L.bbmin = L.c - mk(1.0f, 1.0f, 1.0f) * R;
+    L.bbmax = L.c + mk(1.0f, 1.0f, 1.0f) * R;
+    L.R2 = R * R;
+    rng[0] = 0; rng[1] = -1; rng[2] = 0; rng[3] = -1; rng[4] = 0; rng[5] = -1;
+    if (active) {
+        rng[0] = cell_coord(L.bbmin.x, g.bmin[0], g.inv_cell, g.nx); rng[1] = cell_coord(L.bbmax.x, g.bmin[0], g.inv_cell, g.nx);
+        rng[2] = cell_coord(L.bbmin.y, g.bmin[1], g.inv_cell, g.ny); rng[3] = cell_coord(L.bbmax.y, g.bmin[1], g.inv_cell, g.ny);
+        rng[4] = cell_coord(L.bbmin.z, g.bmin[2], g.inv_cell, g.nz); rng[5] = cell_coord(L.bbmax.z, g.bmin[2], g.inv_cell, g.nz);
+        float4 hn = H.nrm[p], ho = H.out[p], hw = H.weight[p];
+        L.N = mk(hn.x, hn.y, hn.z);
+        L.ior = hn.w;
+        L.outd = mk(ho.x, ho.y, ho.z);
+        L.rough = ho.w;
+        L.wgt = mk(hw.x, hw.y, hw.z);
+        L.refl = hw.w;
+        L.kind = H.mat[p].x;
+        L.absorb = H.absorb[p];
+    }
+}
+
+__global__ void __launch_bounds__(64) k_gather_plan(HitDev H, SharedDev SH, GridDev G, const GridParams* __restrict__ gp,
+                                                    int W, int Hh, TileRange* __restrict__ tiles) {
+    const int lane = threadIdx.x;
+    const int p = tiled_pixel(blockIdx.x * 64 + lane, W, Hh);
+    const GridParams g = *gp;
+    GatherLane L;
+    bool active;
+    float4 sr;
+    int r[6];
+    load_gather_lane(H, SH, g, p, L, active, sr, r);
+    TileRange T;
+    T.x0 = -wave_max_i(active ? -r[0] : -0x7fffffff); T.x1 = wave_max_i(active ? r[1] : -1);
+    T.y0 = -wave_max_i(active ? -r[2] : -0x7fffffff); T.y1 = wave_max_i(active ? r[3] : -1);
+    T.z0 = -wave_max_i(active ? -r[4] : -0x7fffffff); T.z1 = wave_max_i(active ? r[5] : -1);
+    int cand = 0;
+    if (__ballot(active)) {
+        const int ny = T.y1 - T.y0 + 1, nrows = ny * (T.z1 - T.z0 + 1);
+        for (int k = lane; k < nrows; k += 64) {
+            const int y = T.y0 + k % ny, z = T.z0 + k / ny;
+            const int row = (z * g.ny + y) * g.nx;
+            cand += G.cell_start[row + T.x1 + 1] - G.cell_start[row + T.x0];
+        }
+    } else {
+        T.x0 = 0; T.x1 = -1; T.y0 = 0; T.y1 = -1; T.z0 = 0; T.z1 = -1;
+    }
+    cand = wave_sum_i(cand);
+    T.rowlen_pad = 0;
+    T.cand = cand;
+    if (lane == 0) tiles[blockIdx.x] = T;
+}
+
+// chunk size = max(1024, ceil(total / cap)) rounded to a multiple of 64
+__global__ void __launch_bounds__(256) k_chunk_size(const TileRange* __restrict__ tiles, int ntiles, int cap,
+                                                    int* __restrict__ chunk) {
+    __shared__ unsigned long long part[4];
+    unsigned long long s = 0;
+    for (int t = threadIdx.x; t < ntiles; t += 256) s += (unsigned long long)tiles[t].cand;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0 && mode != 1) {
-        float mx = red[0];
-        unsigned long long th = 0, tn = 0;
-        for (int k = 0; k < GATHER_BLOCK / 64; ++k) {
-            mx = fmaxf(mx, red[k]);
-            th += (unsigned long long)redh[k];
-            tn += (unsigned long long)redn[k];
+    if (threadIdx.x == 0) {
+        unsigned long long tot = part[0] + part[1] + part[2] + part[3];
+        unsigned long long c = (tot + (unsigned long long)cap - 1) / (unsigned long long)cap;
+        if (c < 1024) c = 1024;
+        c = (c + 63) & ~63ull;
+        *chunk = (int)c;
+    }
+}
+
+__global__ void k_tile_chunks(const TileRange* __restrict__ tiles, int ntiles, const int* __restrict__ chunk,
+                              int* __restrict__ nch) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) { nch[t] = 0; return; }
+    const int c = *chunk;
+    nch[t] = (tiles[t].cand + c - 1) / c;
+}
+
+__global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, GridDev G,
+                                                      const GridParams* __restrict__ gp, int W, int Hh,
+                                                      const TileRange* __restrict__ tiles, int ntiles,
+                                                      const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
+                                                      float4* __restrict__ partial) {
+    __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
+    const int lane = threadIdx.x;
+    const int total = chunk_off[ntiles];
+    const int CH = *chunkp;
+    const GridParams g = *gp;
+    for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
+        // owning tile: last t with chunk_off[t] <= ch
+        int lo = 0, hi = ntiles;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_off[mid] <= ch) lo = mid; else hi = mid;
         }
-        atomicMax(rmax_bits, __float_as_uint(mx));  // radii are positive: bit order == value order
-        atomicAdd(&ctr->hit_points, th);
+        const int t = lo;
+        const TileRange T = tiles[t];
+        const int p = tiled_pixel(t * 64 + lane, W, Hh);
+        GatherLane L;
+        bool active;
+        float4 sr;
+        int r[6];
+        load_gather_lane(H, SH, g, p, L, active, sr, r);
+        const long long first = (long long)(ch - chunk_off[t]) * CH;
+        const long long last = first + CH < T.cand ? first + CH : T.cand;
+        const int ny = T.y1 - T.y0 + 1, nrows = ny * (T.z1 - T.z0 + 1);
+        long long acc = 0;  // candidates before the current row
+        for (int k = 0; k < nrows && acc < last; ++k) {
+            const int y = T.y0 + k % ny, z = T.z0 + k / ny;
+            const int row = (z * g.ny + y) * g.nx;
+            const int qa = G.cell_start[row + T.x0];
+            const int qb = G.cell_start[row + T.x1 + 1];
+            const long long len = qb - qa;
+            const long long s0 = first > acc ? first - acc : 0;
+            const long long s1 = last - acc < len ? last - acc : len;
+            acc += len;
+            if (s0 >= s1) continue;
+            const bool mine = active && z >= r[4] && z <= r[5] && y >= r[2] && y <= r[3];
+            for (long long base = qa + s0; base < qa + s1; base += 64) {
+                const long long q = base + lane;
+                if (q < qa + s1) {
+                    s_pos[lane] = G.pos[q];
+                    s_flux[lane] = G.flux[q];
+                    s_dir[lane] = G.dir[q];
+                }
+                __syncthreads();
+                const int n = (int)((qa + s1 - base) < 64 ? (qa + s1 - base) : 64);
+                if (mine)
+                    for (int j = 0; j < n; ++j) gather_candidate(L, s_pos[j], s_flux[j], s_dir[j], 0, nullptr);
+                __syncthreads();
+            }
+        }
+        partial[(size_t)ch * 64 + lane] = make_float4(L.add.x, L.add.y, L.add.z, __int_as_float(L.M));
+    }
+}
+
+__global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, int W, int Hh, float alpha,
+                                                      const int* __restrict__ chunk_off,
+                                                      const float4* __restrict__ partial, int* __restrict__ counts,
+                                                      unsigned long long* __restrict__ rstat, Counters* __restrict__ ctr,
+                                                      uint32_t* __restrict__ flags) {
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x;
+    const int p = tiled_pixel(t * 64 + lane, W, Hh);
+    float4 hp = make_float4(0, 0, 0, 0), sr = make_float4(0, 0, 0, 0);
+    if (p >= 0) {
+        hp = H.pos[p];
+        sr = SH.rad[p];
+    }
+    const bool active = p >= 0 && (int)hp.w == 1;
+    f3 add = mk(0.0f, 0.0f, 0.0f);
+    int M = 0;
+    for (int c = chunk_off[t]; c < chunk_off[t + 1]; ++c) {
+        float4 v = partial[(size_t)c * 64 + lane];
+        add = add + mk(v.x, v.y, v.z);
+        M += __float_as_int(v.w);
+    }
+    float rnew = sr.w;
+    if (active && M > 0) {
+        int2 cn = SH.cnt[p];
+        const float num = (float)cn.x + alpha * (float)M;
+        const float den = (float)(cn.x + M);
+        f3 r = mk(sr.x, sr.y, sr.z);
+        r = ((r + add) * num) / den;
+        const float R1 = sr.w * sqrtf_(num / den);
+        SH.rad[p] = make_float4(r.x, r.y, r.z, R1);
+        SH.cnt[p] = make_int2((int)((float)cn.x + (float)M * alpha), cn.y + 1);
+        rnew = R1;
+        if (!(r.x == r.x) || !(r.y == r.y) || !(r.z == r.z) || __builtin_isinf(r.x) || __builtin_isinf(r.y) ||
+            __builtin_isinf(r.z))
+            atomicOr(flags, 1u);
+    }
+    if (p >= 0 && counts) counts[p] = active ? M : 0;
+    // mean radius of the hit points (fixed point, 2^-32 units) -> next cell size
+    unsigned long long rs = active ? (unsigned long long)((double)rnew * 4294967296.0) : 0ull;
+    unsigned long long rc = active ? 1ull : 0ull;
+    unsigned long long tn = (unsigned long long)M;
+    for (int off = 32; off > 0; off >>= 1) {
+        rs += __shfl_xor(rs, off, 64);
+        rc += __shfl_xor(rc, off, 64);
+        tn += __shfl_xor(tn, off, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&rstat[0], rs);
+        atomicAdd(&rstat[1], rc);
+        atomicAdd(&ctr->hit_points, rc);
         atomicAdd(&ctr->neighbours, tn);
     }
 }
@@ -650,7 +932,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const f
 static size_t stack_bytes(int cap) { return (size_t)cap * PATH_BLOCK * sizeof(uint32_t); }
 
 void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st) {
-    hipLaunchKernelGGL(k_camera_words, dim3(1), dim3(64), 0, st, st_mt, st_pos, words, need);
+    hipLaunchKernelGGL(k_camera_words, dim3(1), dim3(CAMW_THREADS), 0, st, st_mt, st_pos, words, need);
 }
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, int cap, uint32_t* flags,
@@ -679,10 +961,10 @@ void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, 
 }
 
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
-                       const uint32_t* rmax_bits, const float* bmin, const float* ext, int cell_cap, int* cell_count,
+                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
                        hipStream_t st) {
-    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rmax_bits, bmin[0], bmin[1], bmin[2], ext[0],
+    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rstat, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], cell_cap, nstored, ctr);
     hipLaunchKernelGGL(k_zero_i32, dim3((cell_cap + 1 + 255) / 256), dim3(256), 0, st, cell_count, cell_cap + 1);
     int gP = (slots.npaths + 255) / 256;
@@ -700,12 +982,29 @@ void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, flo
 }
 
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
-                   float alpha, int mode, int* counts, const int* list_off, int* lists, uint32_t* rmax_bits,
+                   float alpha, int mode, int* counts, const int* list_off, int* lists, unsigned long long* rstat,
                    Counters* ctr, uint32_t* flags, hipStream_t st) {
-    int tiles = ((W + 7) >> 3) * ((Hh + 7) >> 3) * 64;
-    hipLaunchKernelGGL(k_gather, dim3((tiles + GATHER_BLOCK - 1) / GATHER_BLOCK), dim3(GATHER_BLOCK), 0, st, H, SH, G,
-                       gp, W, Hh, alpha, mode, counts, list_off, lists, rmax_bits, ctr, flags);
+    int tiles = ((W + 7) >> 3) * ((Hh + 7) >> 3);
+    hipLaunchKernelGGL(k_gather, dim3(tiles), dim3(64), 0, st, H, SH, G,
+                       gp, W, Hh, alpha, mode, counts, list_off, lists, rstat, ctr, flags);
 }
+
+void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W,
+                            int Hh, float alpha, TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off,
+                            int* scan_tmp, float4* partial, int* counts, unsigned long long* rstat, Counters* ctr,
+                            uint32_t* flags, hipStream_t st) {
+    const int ntiles = ((W + 7) >> 3) * ((Hh + 7) >> 3);
+    hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, W, Hh, tiles);
+    hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp);
+    hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
+    exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
+    hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, W, Hh, tiles, ntiles,
+                       chunk_off, chunkp, partial);
+    hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, W, Hh, alpha, chunk_off, partial, counts,
+                       rstat, ctr, flags);
+}
+int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); }
+size_t gather_tile_bytes() { return sizeof(TileRange); }
 
 void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st) {
     hipLaunchKernelGGL(k_reconstruct, dim3((np + 255) / 256), dim3(256), 0, st, SH, np, total, passes, out);
@@ -716,6 +1015,13 @@ void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st) {
 void launch_init_slots(const SlotsDev& s, hipStream_t st) {
     size_t n = (size_t)NSLOT * s.npaths;
     hipLaunchKernelGGL(k_init_slots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s);
+}
+__global__ void k_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b) {
+    p[0] = a;
+    p[1] = b;
+}
+void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_u64x2, dim3(1), dim3(1), 0, st, p, a, b);
 }
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p, v);

@@ -62,7 +62,7 @@ class Params(C.Structure):
 
 class PassStats(C.Structure):
     _fields_ = [("photons_emitted", C.c_int32), ("photons_stored", C.c_int32), ("hit_points", C.c_int32),
-                ("neighbours", C.c_int64), ("max_radius", C.c_float), ("flags", C.c_int32),
+                ("neighbours", C.c_int64), ("mean_radius", C.c_float), ("flags", C.c_int32),
                 ("ms_photons", C.c_float), ("ms_build", C.c_float), ("ms_eye", C.c_float),
                 ("ms_gather", C.c_float)]
 
@@ -170,7 +170,7 @@ def camera_desc(cam):
     return c
 
 
-def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=True):
+def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False):
     p = Params()
     p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
     p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)

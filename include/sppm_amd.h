@@ -132,7 +132,7 @@ typedef struct sppm_pass_stats {
     int32_t photons_stored;    /* flattened photon count (SPPM.cpp:812-822) */
     int32_t hit_points;        /* hit points of type HIT */
     int64_t neighbours;        /* sum over hit points of in-radius photons (M) */
-    float max_radius;          /* max shared radius after the update */
+    float mean_radius;         /* mean shared radius of the hit points after the update */
     int32_t flags;             /* bit0: NaN/Inf seen; bit1: rng cursor overflow */
     float ms_photons, ms_build, ms_eye, ms_gather;  /* phase times (hipEvent) */
 } sppm_pass_stats;
