@@ -879,6 +879,15 @@ int sppm_read_shared(sppm_ctx* c, float* radiance, float* radius, int32_t* num_p
     return SPPM_OK;
 }
 
+int sppm_read_path_counts(sppm_ctx* c, int32_t* counts) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!counts) return fail(SPPM_E_ARG, "NULL output");
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(counts, c->rb.slots.count, sizeof(int) * c->rb.N, hipMemcpyDeviceToHost));
+    return SPPM_OK;
+}
+
 int sppm_read_counts(sppm_ctx* c, int32_t* m) {
     int rc = check_ready(c);
     if (rc) return rc;

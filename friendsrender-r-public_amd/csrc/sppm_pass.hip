@@ -186,6 +186,9 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
         if (g.is_sppm) {
             if (pidx >= NSLOT - 2) { fl |= 8u; break; }
             f3 ph_flux = (rad * T) * g.albedo;
+            if (!(PDF > 0.0f) || !isfinite(ph_flux.x) || !isfinite(ph_flux.y) || !isfinite(ph_flux.z) ||
+                !isfinite(PDF))
+                fl |= 16u;  // stored photon with pdf <= 0 or non-finite flux (propagated as the reference does)
             const size_t o = (size_t)pidx * NP + i;
             slots.pos[o] = make_float4(g.pos.x, g.pos.y, g.pos.z, cur_index);
             slots.flux[o] = make_float4(ph_flux.x, ph_flux.y, ph_flux.z, PDF);
@@ -474,6 +477,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev
         ++depth;
     }
     hw = hw / campdf;  // createHitpoints: PixelWeight /= pdf (SPPM.cpp:346)
+    if (type == 1 && (!isfinite(hw.x) || !isfinite(hw.y) || !isfinite(hw.z))) fl |= 32u;
     H.pos[i] = make_float4(hpos.x, hpos.y, hpos.z, (float)type);
     H.nrm[i] = make_float4(hn.x, hn.y, hn.z, hidx);
     H.out[i] = make_float4(hout.x, hout.y, hout.z, hrough);

@@ -26,6 +26,7 @@ EXPORTED = [
     "sppm_pass_build", "sppm_pass_eye", "sppm_pass_gather", "sppm_sync", "sppm_reconstruct",
     "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
     "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms", "sppm_counters",
+    "sppm_read_path_counts",
 ]
 
 
@@ -104,6 +105,7 @@ def load_library(path=LIB_PATH):
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
     lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
+    lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     return lib
 
 
@@ -268,6 +270,11 @@ class Renderer:
         n, g = np.zeros(np_, np.int32), np.zeros(np_, np.int32)
         _check(self.lib.sppm_read_shared(self.h, _fp(rad), _fp(r), _ip(n), _ip(g)))
         return dict(radiance=rad, radius=r, num_photon=n, num_gather=g)
+
+    def read_path_counts(self):
+        c = np.zeros(self.N, np.int32)
+        _check(self.lib.sppm_read_path_counts(self.h, _ip(c)))
+        return c
 
     def read_counts(self):
         m = np.zeros(self.W * self.H, np.int32)

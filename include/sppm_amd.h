@@ -133,7 +133,10 @@ typedef struct sppm_pass_stats {
     int32_t hit_points;        /* hit points of type HIT */
     int64_t neighbours;        /* sum over hit points of in-radius photons (M) */
     float mean_radius;         /* mean shared radius of the hit points after the update */
-    int32_t flags;             /* bit0: NaN/Inf seen; bit1: rng cursor overflow */
+    int32_t flags;             /* bit0 NaN/Inf radiance; bit1 rng cursor overflow; bit2 traversal
+                                  stack overflow; bit3 photon slot overflow; bit4 a stored photon
+                                  had pdf <= 0 or non-finite flux; bit5 a hit point weight was
+                                  non-finite (bits 4-5 follow the reference's arithmetic) */
     float ms_photons, ms_build, ms_eye, ms_gather;  /* phase times (hipEvent) */
 } sppm_pass_stats;
 
@@ -182,6 +185,10 @@ int sppm_read_hitpoints(sppm_ctx* ctx, int32_t* type, int32_t* facet, float* pos
 /* Shared hit points: radiance 3f, radius, num_photon (int N), num_gather. */
 int sppm_read_shared(sppm_ctx* ctx, float* radiance, float* radius, int32_t* num_photon,
                      int32_t* num_gather);
+/* Flattened photon count of each path of the last photon pass (N entries):
+ * surface photons stored + the stale slot the reference keeps, or 1 for the
+ * emission photon (SPPM.cpp:530-540, 672, 812-822). */
+int sppm_read_path_counts(sppm_ctx* ctx, int32_t* counts);
 /* Per-hit-point in-radius photon count M of the last gather. */
 int sppm_read_counts(sppm_ctx* ctx, int32_t* m);
 /* Neighbour lists of the last gather (requires params.record_neighbours):

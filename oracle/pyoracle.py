@@ -81,6 +81,9 @@ def _load_oracle():
     L.orc_write_shared.argtypes = [C.c_void_p, F, F, I, I]
     L.orc_reconstruct.argtypes = [C.c_void_p, C.c_int32, C.c_int32, F]
     L.orc_full_pass.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_uniform_sampler_at.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int64), F]
+    L.orc_eye_pixel.argtypes = [C.c_void_p, C.c_int, C.c_int, F, I, I, F, F, F, F]
+    L.orc_photon_path.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, F, F, F]
     return L
 
 
@@ -244,6 +247,39 @@ class OracleRender:
         img = np.zeros((self.H, self.W, 4), np.float32)
         orc().orc_reconstruct(self.h, total, p, fp(img))
         return img
+
+    def eye_pixels(self, step, pixels, samples):
+        """createHitpoints for selected pixels of pass `step` (samples: n x 3)."""
+        n = len(pixels)
+        out = dict(type=np.zeros(n, np.int32), facet=np.zeros(n, np.int32),
+                   pos=np.zeros((n, 3), np.float32), weight=np.zeros((n, 3), np.float32),
+                   normal=np.zeros((n, 3), np.float32), outdir=np.zeros((n, 3), np.float32))
+        smp = np.ascontiguousarray(samples, np.float32)
+        t, f = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        for j, pix in enumerate(pixels):
+            orc().orc_eye_pixel(self.h, step, int(pix), fp(smp[j]), ip(t), ip(f), fp(out["pos"][j]),
+                                fp(out["weight"][j]), fp(out["normal"][j]), fp(out["outdir"][j]))
+            out["type"][j], out["facet"][j] = t[0], f[0]
+        return out
+
+    def photon_path(self, step, path):
+        """createPhotonMap for one path: its new photons (emission photon if none stored)."""
+        pos, flux, indir = (np.zeros((32, 3), np.float32) for _ in range(3))
+        index, pdf = np.zeros(32, np.float32), np.zeros(32, np.float32)
+        n = orc().orc_photon_path(self.h, step, int(path), fp(pos), fp(flux), fp(indir), fp(index), fp(pdf))
+        return dict(pos=pos[:n], flux=flux[:n], indir=indir[:n], index=index[:n], pdf=pdf[:n])
+
+
+def uniform_samples_at(sampler_id, calls):
+    """UniformSample(sampler_id) draws number calls[j] (any order)."""
+    calls = np.asarray(calls, np.int64)
+    order = np.argsort(calls, kind="stable")
+    sc = np.ascontiguousarray(calls[order])
+    out = np.zeros((len(sc), 3), np.float32)
+    orc().orc_uniform_sampler_at(sampler_id, len(sc), sc.ctypes.data_as(C.POINTER(C.c_int64)), fp(out))
+    res = np.zeros_like(out)
+    res[order] = out
+    return res
 
 
 def camera_ray(cam, u, v, sample3):

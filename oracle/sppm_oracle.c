@@ -1117,13 +1117,8 @@ orc_render* orc_render_create(const orc_scene* s, const sppm_camera_desc* cam, c
     r->nthreads = nthreads > 0 ? nthreads : 1;
     r->W = cam->width;
     r->H = cam->height;
-    int N = p->photons_per_pass;
-    r->nslots_paths = N;
-    r->slots = (ophoton*)calloc((size_t)N * NUM_SLOTS, sizeof(ophoton));
-    for (size_t i = 0; i < (size_t)N * NUM_SLOTS; ++i) {  /* Photon() : PDF 1, Index 1 */
-        r->slots[i].pdf = 1.0f;
-        r->slots[i].index = 1.0f;
-    }
+    r->nslots_paths = p->photons_per_pass;
+    r->slots = NULL;  /* allocated by the first photon pass */
     size_t np = (size_t)r->W * (size_t)r->H;
     r->hits = (ohit*)calloc(np, sizeof(ohit));
     r->shared = (oshared*)calloc(np, sizeof(oshared));
@@ -1206,10 +1201,9 @@ static void calc_geometry(const orc_scene* s, int f, float u, float v, geom* g) 
 static inline int near1(float s_) { return (double)fabsf(s_ - 1.0f) < 0.00001; }  /* SPPM.h:27-29 */
 
 /* createPhotonMap, one path (SPPM.cpp:518-672) */
-static void photon_path(orc_render* r, int i, int step) {
+static void photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     const orc_scene* s = r->s;
     const int N = r->p.photons_per_pass;
-    ophoton* slot = r->slots + (size_t)i * NUM_SLOTS;
     orc_mt m;
     orc_mt_seed(&m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
     /* generateLightRay (SPPM.cpp:453-471) */
@@ -1331,16 +1325,28 @@ static void photon_path(orc_render* r, int i, int step) {
     slot[++pidx] = term;
 }
 
+static void ensure_slots(orc_render* r) {
+    if (r->slots) return;
+    size_t n = (size_t)r->nslots_paths * NUM_SLOTS;
+    r->slots = (ophoton*)calloc(n, sizeof(ophoton));
+    for (size_t i = 0; i < n; ++i) {  /* Photon() : PDF 1, Index 1 */
+        r->slots[i].pdf = 1.0f;
+        r->slots[i].index = 1.0f;
+    }
+}
+
 int orc_photon_pass(orc_render* r, int step) {
     int N = r->p.photons_per_pass;
     if (r->s->nlights == 0) return 0;
+    ensure_slots(r);
 #pragma omp parallel for schedule(dynamic, 256) num_threads(r->nthreads)
-    for (int i = 0; i < N; ++i) photon_path(r, i, step);
+    for (int i = 0; i < N; ++i) photon_path(r, r->slots + (size_t)i * NUM_SLOTS, i, step);
     return N;
 }
 
 int orc_flatten(orc_render* r) {
     int N = r->p.photons_per_pass;
+    ensure_slots(r);
     size_t cnt = 0;
     for (int i = 0; i < N; ++i) {
         const ophoton* sl = r->slots + (size_t)i * NUM_SLOTS;
@@ -1697,6 +1703,7 @@ void orc_reconstruct(const orc_render* r, int32_t total_photons, int32_t passes,
  * numstep, i.e. the same value. */
 int orc_full_pass(orc_render* r, int pass, int record) {
     int N = r->p.photons_per_pass;
+    ensure_slots(r);
     for (int i = 0; i < N; ++i) {  /* photons[i][0] = Photon() */
         ophoton* s0 = r->slots + (size_t)i * NUM_SLOTS;
         memset(s0, 0, sizeof *s0);
@@ -1707,4 +1714,62 @@ int orc_full_pass(orc_render* r, int pass, int record) {
     orc_eye_pass(r, pass + 1);
     orc_gather(r, record);
     return emitted;
+}
+
+
+/* ======================================================================
+ * Spot checks
+ * ====================================================================== */
+void orc_uniform_sampler_at(int id, int n, const int64_t* calls, float* out) {
+    orc_mt m;
+    orc_mt_seed(&m, (uint32_t)id);
+    int64_t pos = 0; /* index of the next call */
+    for (int j = 0; j < n; ++j) {
+        int64_t skip_words = (calls[j] - pos) * 6;
+        /* discard whole twist blocks without tempering */
+        while (skip_words > 0) {
+            if (m.idx >= 624) mt_twist(&m);
+            int64_t avail = 624 - m.idx;
+            if (skip_words >= avail) {
+                skip_words -= avail;
+                m.idx = 624;
+            } else {
+                m.idx += (int)skip_words;
+                skip_words = 0;
+            }
+        }
+        vstore(out + 3 * j, uniform_sampler_draw(&m));
+        pos = calls[j] + 1;
+    }
+}
+
+void orc_eye_pixel(const orc_render* r, int step, int pixel, const float* sample3, int32_t* type,
+                   int32_t* facet, float* pos, float* weight, float* normal, float* outdir) {
+    ohit h;
+    eye_path((orc_render*)r, pixel, step, vload(sample3), &h);
+    *type = h.type;
+    *facet = h.facet;
+    vstore(pos, h.pos);
+    vstore(weight, h.weight);
+    vstore(normal, h.normal);
+    vstore(outdir, h.outdir);
+}
+
+int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* flux, float* indir,
+                    float* index, float* pdf) {
+    /* run the path on a private slot row that starts invalid */
+    ophoton* row = (ophoton*)calloc(NUM_SLOTS, sizeof(ophoton));
+    photon_path(r, row, path, step);
+    const ophoton* sl = row;
+    int n = 0;
+    while (n < NUM_SLOTS && sl[n].valid) ++n;
+    for (int k = 0; k < n; ++k) {
+        vstore(pos + 3 * k, sl[k].pos);
+        vstore(flux + 3 * k, sl[k].flux);
+        vstore(indir + 3 * k, sl[k].indir);
+        index[k] = sl[k].index;
+        pdf[k] = sl[k].pdf;
+    }
+    free(row);
+    return n;
 }

@@ -124,6 +124,19 @@ void orc_reconstruct(const orc_render* r, int32_t total_photons, int32_t passes,
 /* one full pass in reference order (SPPM.cpp:771-841); pass = 0-based index */
 int orc_full_pass(orc_render* r, int pass, int record);
 
+/* ---- spot checks at full size (no full-pass run needed) ------------------- */
+/* UniformSample(id) draws number calls[j] (0-based, ascending) -> out 3 floats each */
+void orc_uniform_sampler_at(int id, int n, const int64_t* calls, float* out);
+/* createHitpoints for one pixel of pass `step` with the given camera sample:
+ * type, facet, pos[3], weight[3], normal[3], outdir[3] */
+void orc_eye_pixel(const orc_render* r, int step, int pixel, const float* sample3, int32_t* type,
+                   int32_t* facet, float* pos, float* weight, float* normal, float* outdir);
+/* createPhotonMap for one path of pass `step`, on a private slot array that
+ * starts invalid: returns the number of surface photons stored (k) and writes
+ * the path's new photons (emission photon when k == 0) */
+int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* flux, float* indir,
+                    float* index, float* pdf);
+
 #ifdef __cplusplus
 }
 #endif
