@@ -83,7 +83,8 @@ def _load_oracle():
     L.orc_full_pass.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.orc_uniform_sampler_at.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int64), F]
     L.orc_eye_pixel.argtypes = [C.c_void_p, C.c_int, C.c_int, F, I, I, F, F, F, F]
-    L.orc_photon_path.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, F, F, F]
+    L.orc_photon_path.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, F, F, F, I]
+    L.orc_read_path_counts.argtypes = [C.c_void_p, I]
     return L
 
 
@@ -266,8 +267,15 @@ class OracleRender:
         """createPhotonMap for one path: its new photons (emission photon if none stored)."""
         pos, flux, indir = (np.zeros((32, 3), np.float32) for _ in range(3))
         index, pdf = np.zeros(32, np.float32), np.zeros(32, np.float32)
-        n = orc().orc_photon_path(self.h, step, int(path), fp(pos), fp(flux), fp(indir), fp(index), fp(pdf))
-        return dict(pos=pos[:n], flux=flux[:n], indir=indir[:n], index=index[:n], pdf=pdf[:n])
+        nl = np.zeros(1, np.int32)
+        k = orc().orc_photon_path(self.h, step, int(path), fp(pos), fp(flux), fp(indir), fp(index), fp(pdf), ip(nl))
+        n = int(nl[0])
+        return dict(k=k, pos=pos[:n], flux=flux[:n], indir=indir[:n], index=index[:n], pdf=pdf[:n])
+
+    def read_path_counts(self):
+        c = np.zeros(self.N, np.int32)
+        orc().orc_read_path_counts(self.h, ip(c))
+        return c
 
 
 def uniform_samples_at(sampler_id, calls):

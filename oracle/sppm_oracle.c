@@ -1201,7 +1201,7 @@ static void calc_geometry(const orc_scene* s, int f, float u, float v, geom* g) 
 static inline int near1(float s_) { return (double)fabsf(s_ - 1.0f) < 0.00001; }  /* SPPM.h:27-29 */
 
 /* createPhotonMap, one path (SPPM.cpp:518-672) */
-static void photon_path(const orc_render* r, ophoton* slot, int i, int step) {
+static int photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     const orc_scene* s = r->s;
     const int N = r->p.photons_per_pass;
     orc_mt m;
@@ -1322,7 +1322,9 @@ static void photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     ophoton term;
     memset(&term, 0, sizeof term);
     term.valid = 0; term.pdf = 1.0f; term.index = 1.0f;
+    const int k = pidx;
     slot[++pidx] = term;
+    return k;
 }
 
 static void ensure_slots(orc_render* r) {
@@ -1756,13 +1758,14 @@ void orc_eye_pixel(const orc_render* r, int step, int pixel, const float* sample
 }
 
 int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* flux, float* indir,
-                    float* index, float* pdf) {
+                    float* index, float* pdf, int32_t* nlist) {
     /* run the path on a private slot row that starts invalid */
     ophoton* row = (ophoton*)calloc(NUM_SLOTS, sizeof(ophoton));
-    photon_path(r, row, path, step);
+    int k = photon_path(r, row, path, step);
     const ophoton* sl = row;
     int n = 0;
     while (n < NUM_SLOTS && sl[n].valid) ++n;
+    if (nlist) *nlist = n;
     for (int k = 0; k < n; ++k) {
         vstore(pos + 3 * k, sl[k].pos);
         vstore(flux + 3 * k, sl[k].flux);
@@ -1771,5 +1774,15 @@ int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* 
         pdf[k] = sl[k].pdf;
     }
     free(row);
-    return n;
+    return k;
+}
+
+void orc_read_path_counts(const orc_render* r, int32_t* counts) {
+    int N = r->p.photons_per_pass;
+    for (int i = 0; i < N; ++i) {
+        const ophoton* sl = r->slots + (size_t)i * NUM_SLOTS;
+        int k = 0;
+        while (k < NUM_SLOTS && sl[k].valid) ++k;
+        counts[i] = k;
+    }
 }

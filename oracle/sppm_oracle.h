@@ -135,7 +135,9 @@ void orc_eye_pixel(const orc_render* r, int step, int pixel, const float* sample
  * starts invalid: returns the number of surface photons stored (k) and writes
  * the path's new photons (emission photon when k == 0) */
 int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* flux, float* indir,
-                    float* index, float* pdf);
+                    float* index, float* pdf, int32_t* nlist);
+/* flattened photon count of each path of the last photon pass (N entries) */
+void orc_read_path_counts(const orc_render* r, int32_t* counts);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,140 @@
+"""Pins the oracle's closest hit, photon slot semantics, neighbour sets,
+progressive update and reconstruction with structure-independent checks:
+brute force vs the oracle's own BVH, analytic hits and ties, an independent
+brute-force neighbour search, closed-form update cases, and the reference's
+stale-slot rule (SPPM.cpp:530-540, 606, 672, 812-822)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import scenes
+
+
+def rays(n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.02, 0.98, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o, d.astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["box", "full"])
+def test_bvh_equals_brute_force(which, box_scene, full_scene_small):
+    sc = box_scene if which == "box" else full_scene_small
+    os_ = po.OracleScene(sc)
+    o, d = rays(20000, 11)
+    a = os_.closest_hit(o, d, mode=0)
+    b = os_.closest_hit(o, d, mode=1)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert (a[0] >= 0).mean() > 0.5
+
+
+def test_analytic_hit_and_tie_goes_to_lowest_id(box_scene):
+    os_ = po.OracleScene(box_scene)
+    # straight down onto the floor (facets 0, 1 span y = 0): t = height
+    o = np.array([[0.7, 0.5, 0.2], [0.2, 0.25, 0.8]], np.float32)
+    d = np.array([[0, -1, 0], [0, -1, 0]], np.float32)
+    f, t, u, v = os_.closest_hit(o, d, mode=0)
+    assert set(f.tolist()) <= {0, 1}
+    np.testing.assert_allclose(t, [0.5, 0.25], rtol=1e-6)
+    # the floor's diagonal x == z is shared by facets 0 and 1 at equal t
+    o = np.array([[0.15, 0.5, 0.15], [0.9, 0.9, 0.9], [0.5, 0.95, 0.5]], np.float32)
+    f, t, _, _ = os_.closest_hit(o, np.array([[0, -1, 0]] * 3, np.float32), mode=0)
+    np.testing.assert_array_equal(f, [0, 0, 0])
+    # MinT is strict: an origin on the surface does not hit it at t = 0
+    f, _, _, _ = os_.closest_hit(np.array([[0.5, 0.0, 0.5]], np.float32), np.array([[0, 1, 0]], np.float32))
+    assert f[0] != 0 and f[0] != 1
+
+
+@pytest.fixture(scope="module")
+def small_run(box_scene):
+    cam = scenes.pinhole_camera(48, 40)
+    r = po.OracleRender(po.OracleScene(box_scene), cam, 0.9, 0.015, 12000)
+    return r, cam
+
+
+def brute_neighbours(pos, c, R):
+    """Independent restatement of isIncludePoint + strict radius in float32."""
+    c = c.astype(np.float32)
+    R = np.float32(R)
+    lo, hi = c - R, c + R
+    inbox = ((lo <= pos) & (pos <= hi)).all(axis=1)
+    dd = (pos - c).astype(np.float32)
+    d2 = (dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2]
+    return np.where(inbox & (d2 < R * R))[0]
+
+
+def test_neighbour_sets_and_update_closed_form(small_run):
+    r, cam = small_run
+    r.full_pass(record=True)
+    n = po.orc().orc_flatten(r.h)
+    ph = r.read_photons(n)
+    hp = r.read_hitpoints()
+    off, idx = r.read_neighbours()
+    m = r.read_counts()
+    sh = r.read_shared()
+    hits = np.where(hp["type"] == 1)[0]
+    rng = np.random.default_rng(1)
+    for v in rng.choice(hits, min(400, len(hits)), replace=False):
+        want = brute_neighbours(ph["pos"], hp["pos"][v], np.float32(0.015))
+        np.testing.assert_array_equal(idx[off[v]:off[v + 1]], want)
+        assert m[v] == len(want)
+    # first pass from N = 0: N = int(0.9 M) (int truncation: M = 1 keeps N = 0),
+    # NumGather = 1, R = r0 * sqrt(0.9 M / M)
+    g = m > 0
+    np.testing.assert_array_equal(sh["num_photon"][g], (np.float32(0.9) * m[g].astype(np.float32)).astype(np.int32))
+    assert (sh["num_photon"][m == 1] == 0).all()
+    np.testing.assert_array_equal(sh["num_gather"][g], 1)
+    np.testing.assert_array_equal(sh["num_gather"][~g], 0)
+    np.testing.assert_allclose(sh["radius"][g], 0.015 * np.sqrt(0.9), rtol=1e-6)
+    np.testing.assert_array_equal(sh["radius"][~g], np.float32(0.015))
+
+
+def test_stale_slot_rule(box_scene):
+    """Pass p >= 2: a path that stores k >= 1 photons keeps slot k from an
+    earlier pass; it is flattened iff it holds a photon, i.e. iff the path
+    stored more than k photons when it last wrote that slot.  With fresh
+    slots (pass 1) no stale photon exists; k = 0 yields the emission photon."""
+    cam = scenes.pinhole_camera(16, 16)
+    N = 4000
+    r = po.OracleRender(po.OracleScene(box_scene), cam, 0.9, 0.015, N)
+    k_hist = []
+    for step in (1, 2, 3):
+        po.orc().orc_photon_pass(r.h, step)
+        counts = r.read_path_counts()
+        ks = np.array([r.photon_path(step, i)["k"] for i in range(N)])
+        # slot k's validity before this pass: valid iff some earlier pass wrote a
+        # photon there last, i.e. the latest earlier pass touching slot k had k' > k
+        stale = np.zeros(N, bool)
+        for i in range(N):
+            k = ks[i]
+            if k == 0:
+                continue
+            valid = False
+            for kp in k_hist:  # oldest to newest
+                if kp[i] > k:
+                    valid = True
+                elif kp[i] + 1 == k or (kp[i] == 0 and k == 1):
+                    valid = False  # terminator written at slot kp+1 (slot 1 for an emission-only path)
+            stale[i] = valid
+        want = np.where(ks == 0, 1, ks + stale.astype(np.int32))
+        np.testing.assert_array_equal(counts, want)
+        if step == 1:
+            assert not stale.any()
+        else:
+            assert stale.any()
+        k_hist.append(ks)
+
+
+def test_reconstruct_closed_form(small_run):
+    r, cam = small_run
+    sh = r.read_shared()
+    total, passes = 12000 * 3, 3
+    img = r.reconstruct(total, passes)
+    f = np.float32
+    den = (f(total) * (sh["num_gather"].astype(f) + f(1e-4)) / (f(1e-4) + f(passes))) * f(3.14159) * sh["radius"] * sh["radius"]
+    L = (f(0.0) / f(1e-4)) + sh["radiance"] / den[:, None]
+    want = np.power(L.astype(np.float64), 1 / 1.2)
+    np.testing.assert_allclose(img.reshape(-1, 4)[:, :3], want, rtol=2e-6, atol=1e-30)
+    assert (img.reshape(-1, 4)[:, 3] == 1.0).all()
