@@ -43,7 +43,25 @@ def parse():
     ap.add_argument("--workload", choices=["full", "box"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the workload, run a device elementwise copy of MIB MiB (float32, torch) three "
+                         "times: a kernel of known traffic for scaling the HBM byte counters of a profiler run")
     return ap.parse_args()
+
+
+def read_traffic(path):
+    """HBM bytes per launch of a phase from its committed counter summary
+    (line 'bytes_per_launch: N'), or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def workload(name):
@@ -83,6 +101,19 @@ def cpu_baseline(scene, cam, cfg, seconds):
                        f"{cfg['photons']} photons/pass), oracle/sppm_oracle.c with OpenMP; {el:.1f} s timed")
 
 
+def copy_calibration(mib):
+    """Reads and writes exactly mib MiB per launch: out = in * 1 (elementwise)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(f"copy calibration: 3 launches of {mib} MiB read + {mib} MiB write", file=sys.stderr, flush=True)
+    del a, b
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,11 +123,15 @@ def main():
     import numpy as np  # noqa: F401
     import sppm_amd
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     scene, cam, cfg = workload(args.workload)
+    log(f"scene ready: {cfg['name']}, {cfg['triangles']} triangles")
     r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"])
     for _ in range(args.warmup):
         r.pass_(stats=False)
     r.sync()
+    log(f"warm-up done ({args.warmup} passes); timing {args.steps} passes")
     c0 = r.counters()
     r.kernel_ms("reset")
     t0 = time.perf_counter()
@@ -104,12 +139,14 @@ def main():
         r.pass_(stats=False)
     r.sync()
     el = time.perf_counter() - t0
+    log(f"timed passes done: {el:.3f} s")
     c1 = r.counters()
     ms = {k: r.kernel_ms(k) for k in ("photons", "build", "eye", "gather")}
     K = args.steps
     hits = (c1["hit_points"] - c0["hit_points"]) / K
     nbs = (c1["neighbours"] - c0["neighbours"]) / K
     stored = (c1["stored"] - c0["stored"]) / K
+    cands = (c1["candidates"] - c0["candidates"]) / K
     algo = {
         "photons": stored * STORE_BYTES_PER_PHOTON,
         "build": stored * BUILD_BYTES_PER_PHOTON,
@@ -118,10 +155,7 @@ def main():
     dom = max(("photons", "build", "eye", "gather"), key=lambda k: ms[k])
     rk = dom if dom in algo else "gather"
     achieved = algo[rk] / (ms[rk] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.md"))
     value = cfg["photons"] * K / el
     out = {
         "metric": "photons/s (plain SPPM pass, full-material workload)",
@@ -145,6 +179,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(algo[rk])},
         "phases_ms": {k: round(v, 3) for k, v in ms.items()},
         "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
+                 "gather_candidates": int(cands),
                      "hit_point_updates_per_s": round(hits / (ms["gather"] * 1e-3), 1)},
         "cpu_baseline": None,
     }

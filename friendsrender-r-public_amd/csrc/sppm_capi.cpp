@@ -118,6 +118,7 @@ struct RenderBuffers {
     int* chunk_off = nullptr;
     int* tile_scan_tmp = nullptr;
     float4* partial = nullptr;
+    GroupBuffers groups{};
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
     float4* image = nullptr;
@@ -128,7 +129,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -555,6 +556,9 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.chunk_off, gather_tile_count(W, Hh) + 1)) ||
         (rc = dalloc(&R.tile_scan_tmp, scan_tmp_entries(gather_tile_count(W, Hh) + 1))) ||
         (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
+        (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
+        (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
+        (rc = dalloc(&R.groups.nvalid, 1)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
@@ -576,7 +580,7 @@ int sppm_begin(sppm_ctx* c) {
     launch_set_u64x2(R.rstat, (unsigned long long)((double)c->params.first_radius * 4294967296.0), 1ull, c->stream);
     launch_set_u32(R.flags, 0u, c->stream);
     {
-        Counters z{0, 0, 0};
+        Counters z{0, 0, 0, 0};
         HIPOK(hipMemcpyAsync(R.ctr, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
     }
     HIPOK(hipGetLastError());
@@ -657,8 +661,9 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
         R.have_lists = true;
     } else {
         launch_set_u64x2(R.rstat, 0ull, 0ull, st);
-        launch_gather_balanced(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, R.tiles, GATHER_CHUNK_CAP, R.chunkp, R.nch,
-                               R.chunk_off, R.tile_scan_tmp, R.partial, R.counts, R.rstat, R.ctr, R.flags, st);
+        launch_gather_balanced(R.hits, R.shared, G, R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups, alpha, R.tiles,
+                               GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
+                               R.rstat, R.ctr, R.flags, st);
     }
     return SPPM_OK;
 }
@@ -752,7 +757,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
 }
 
 int sppm_counters(sppm_ctx* c, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
-                  int64_t* passes) {
+                  int64_t* candidates, int64_t* passes) {
     int rc = check_ready(c);
     if (rc) return rc;
     Counters h{};
@@ -761,6 +766,7 @@ int sppm_counters(sppm_ctx* c, int64_t* hit_points, int64_t* neighbours, int64_t
     if (hit_points) *hit_points = (int64_t)h.hit_points;
     if (neighbours) *neighbours = (int64_t)h.neighbours;
     if (photons_stored) *photons_stored = (int64_t)h.stored;
+    if (candidates) *candidates = (int64_t)h.candidates;
     if (passes) *passes = c->pass;
     return SPPM_OK;
 }

@@ -24,6 +24,7 @@ struct Counters {
     unsigned long long hit_points;
     unsigned long long neighbours;
     unsigned long long stored;
+    unsigned long long candidates;  // photons the balanced gather tested (union boxes of its tiles)
 };
 
 void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st);
@@ -43,10 +44,16 @@ void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const
                    float alpha, int mode, int* counts, const int* list_off, int* lists, unsigned long long* rstat,
                    Counters* ctr, uint32_t* flags, hipStream_t st);
 struct TileRange;
-void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W,
-                            int Hh, float alpha, TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off,
-                            int* scan_tmp, float4* partial, int* counts, unsigned long long* rstat, Counters* ctr,
-                            uint32_t* flags, hipStream_t st);
+// Sort buffers of the hit-point groups (np entries each; hist as radix_hist_entries(np)).
+struct GroupBuffers {
+    uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
+    int* nvalid;
+};
+void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
+                            const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
+                            int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st);
+int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();
 void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st);

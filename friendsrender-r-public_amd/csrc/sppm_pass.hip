@@ -678,6 +678,42 @@ struct TileRange {
     int x0, x1, y0, y1, z0, z1, rowlen_pad, cand;
 };
 
+// Hit-point groups of the balanced gather: the valid hit points sorted by the
+// Morton code of their position (1024 steps per scene-box axis), 64 per
+// group.  World-space groups keep the union of their query boxes close to one
+// query box; screen tiles do not (a thin lens spreads a tile's hit points
+// over the circle of confusion).
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ int quant10(float p, float lo, float ext) {
+    float f = (p - lo) / ext * 1024.0f;
+    int q = (int)f;
+    return q < 0 ? 0 : (q > 1023 ? 1023 : q);
+}
+__global__ void k_hit_keys(HitDev H, int np, float bx, float by, float bz, float ex, float ey, float ez,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* __restrict__ nvalid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float4 hp = H.pos[i];
+    uint32_t key = 1u << 30;  // invalid hit points sort last
+    if ((int)hp.w == 1) {
+        key = spread10((uint32_t)quant10(hp.x, bx, ex)) | (spread10((uint32_t)quant10(hp.y, by, ey)) << 1) |
+              (spread10((uint32_t)quant10(hp.z, bz, ez)) << 2);
+        atomicAdd(nvalid, 1);
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+__device__ __forceinline__ int group_pixel(const uint32_t* __restrict__ order, int nvalid, int lin) {
+    return lin < nvalid ? (int)order[lin] : -1;
+}
+
 __device__ __forceinline__ void load_gather_lane(const HitDev& H, const SharedDev& SH, const GridParams& g, int p,
                                                  GatherLane& L, bool& active, float4& sr, int* rng) {
     float4 hp = make_float4(0, 0, 0, 0);
@@ -713,9 +749,10 @@ __device__ __forceinline__ void load_gather_lane(const HitDev& H, const SharedDe
 }
 
 __global__ void __launch_bounds__(64) k_gather_plan(HitDev H, SharedDev SH, GridDev G, const GridParams* __restrict__ gp,
-                                                    int W, int Hh, TileRange* __restrict__ tiles) {
+                                                    const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
+                                                    TileRange* __restrict__ tiles) {
     const int lane = threadIdx.x;
-    const int p = tiled_pixel(blockIdx.x * 64 + lane, W, Hh);
+    const int p = group_pixel(order, *nvalidp, blockIdx.x * 64 + lane);
     const GridParams g = *gp;
     GatherLane L;
     bool active;
@@ -745,7 +782,7 @@ __global__ void __launch_bounds__(64) k_gather_plan(HitDev H, SharedDev SH, Grid
 
 // chunk size = max(1024, ceil(total / cap)) rounded to a multiple of 64
 __global__ void __launch_bounds__(256) k_chunk_size(const TileRange* __restrict__ tiles, int ntiles, int cap,
-                                                    int* __restrict__ chunk) {
+                                                    int* __restrict__ chunk, Counters* __restrict__ ctr) {
     __shared__ unsigned long long part[4];
     unsigned long long s = 0;
     for (int t = threadIdx.x; t < ntiles; t += 256) s += (unsigned long long)tiles[t].cand;
@@ -758,6 +795,7 @@ __global__ void __launch_bounds__(256) k_chunk_size(const TileRange* __restrict_
         if (c < 1024) c = 1024;
         c = (c + 63) & ~63ull;
         *chunk = (int)c;
+        ctr->candidates += tot;
     }
 }
 
@@ -770,15 +808,36 @@ __global__ void k_tile_chunks(const TileRange* __restrict__ tiles, int ntiles, c
     nch[t] = (tiles[t].cand + c - 1) / c;
 }
 
+// The reference's two neighbour tests for one candidate: closed box
+// (kdtree.h:78-88) then the strict radius (SPPM.cpp:432).
+__device__ __forceinline__ bool in_query(const GatherLane& L, const float4 pp) {
+    if (L.bbmin.x > pp.x || L.bbmax.x < pp.x) return false;
+    if (L.bbmin.y > pp.y || L.bbmax.y < pp.y) return false;
+    if (L.bbmin.z > pp.z || L.bbmax.z < pp.z) return false;
+    const f3 d = mk(pp.x, pp.y, pp.z) - L.c;
+    return norm2(d) < L.R2;
+}
+
+// Persistent waves over the chunk list.  Per staged batch of 64 photons every
+// lane runs the two tests against its own hit point and lists the staged
+// photons that pass (LDS, one byte per entry, [entry][lane]); then every
+// lane walks its own list and accumulates W * BRDF * Flux / PDF
+// (SPPM.cpp:434-440).  Splitting the cheap tests from the BRDF evaluation
+// keeps the lanes of a wave busy in the second loop instead of evaluating
+// one lane's BRDF per candidate; each lane still adds its neighbours in
+// ascending candidate order.
 __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, GridDev G,
-                                                      const GridParams* __restrict__ gp, int W, int Hh,
+                                                      const GridParams* __restrict__ gp,
+                                                      const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
                                                       const TileRange* __restrict__ tiles, int ntiles,
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
                                                       float4* __restrict__ partial) {
     __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
+    __shared__ unsigned char s_hit[64 * 64];
     const int lane = threadIdx.x;
     const int total = chunk_off[ntiles];
     const int CH = *chunkp;
+    const int nvalid = *nvalidp;
     const GridParams g = *gp;
     for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
         // owning tile: last t with chunk_off[t] <= ch
@@ -789,7 +848,7 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
         }
         const int t = lo;
         const TileRange T = tiles[t];
-        const int p = tiled_pixel(t * 64 + lane, W, Hh);
+        const int p = group_pixel(order, nvalid, t * 64 + lane);
         GatherLane L;
         bool active;
         float4 sr;
@@ -819,8 +878,26 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                 }
                 __syncthreads();
                 const int n = (int)((qa + s1 - base) < 64 ? (qa + s1 - base) : 64);
-                if (mine)
-                    for (int j = 0; j < n; ++j) gather_candidate(L, s_pos[j], s_flux[j], s_dir[j], 0, nullptr);
+                int nh = 0;
+                if (mine) {
+                    for (int j = 0; j < n; ++j) {
+                        if (in_query(L, s_pos[j])) {
+                            s_hit[nh * 64 + lane] = (unsigned char)j;
+                            ++nh;
+                        }
+                    }
+                }
+                L.M += nh;
+                for (int e = 0; e < nh; ++e) {
+                    const int j = s_hit[e * 64 + lane];
+                    const float4 pp = s_pos[j], pf = s_flux[j], pd = s_dir[j];
+                    const f3 indir = mk(pd.x, pd.y, pd.z);
+                    const float Io = near1(pp.w) ? L.ior : 1.0f;
+                    const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
+                    FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
+                    const float b = surf_brdf(L.kind, fp);
+                    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
+                }
                 __syncthreads();
             }
         }
@@ -828,14 +905,15 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
     }
 }
 
-__global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, int W, int Hh, float alpha,
+__global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, const uint32_t* __restrict__ order,
+                                                      const int* __restrict__ nvalidp, float alpha,
                                                       const int* __restrict__ chunk_off,
                                                       const float4* __restrict__ partial, int* __restrict__ counts,
                                                       unsigned long long* __restrict__ rstat, Counters* __restrict__ ctr,
                                                       uint32_t* __restrict__ flags) {
     const int lane = threadIdx.x;
     const int t = blockIdx.x;
-    const int p = tiled_pixel(t * 64 + lane, W, Hh);
+    const int p = group_pixel(order, *nvalidp, t * 64 + lane);
     float4 hp = make_float4(0, 0, 0, 0), sr = make_float4(0, 0, 0, 0);
     if (p >= 0) {
         hp = H.pos[p];
@@ -993,21 +1071,29 @@ void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const
                        gp, W, Hh, alpha, mode, counts, list_off, lists, rstat, ctr, flags);
 }
 
-void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W,
-                            int Hh, float alpha, TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off,
-                            int* scan_tmp, float4* partial, int* counts, unsigned long long* rstat, Counters* ctr,
-                            uint32_t* flags, hipStream_t st) {
-    const int ntiles = ((W + 7) >> 3) * ((Hh + 7) >> 3);
-    hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, W, Hh, tiles);
-    hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp);
+void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
+                            const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
+                            int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st) {
+    const int ntiles = gather_group_count(np);
+    // hit-point groups: valid hit points in Morton order, 64 per group
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 1);
+    if (counts) hipLaunchKernelGGL(k_zero_i32, dim3((np + 255) / 256), dim3(256), 0, st, counts, np);
+    hipLaunchKernelGGL(k_hit_keys, dim3((np + 255) / 256), dim3(256), 0, st, H, np, bmin[0], bmin[1], bmin[2], ext[0],
+                       ext[1], ext[2], gb.keys, gb.vals, gb.nvalid);
+    radix_sort_pairs(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, gb.hist, np, 31, st);
+    const uint32_t* order = gb.vals;
+    hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles);
+    hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp, ctr);
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
-    hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, W, Hh, tiles, ntiles,
-                       chunk_off, chunkp, partial);
-    hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, W, Hh, alpha, chunk_off, partial, counts,
-                       rstat, ctr, flags);
+    hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
+                       ntiles, chunk_off, chunkp, partial);
+    hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
+                       partial, counts, rstat, ctr, flags);
 }
 int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); }
+int gather_group_count(int np) { return (np + 63) / 64; }
 size_t gather_tile_bytes() { return sizeof(TileRange); }
 
 void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st) {

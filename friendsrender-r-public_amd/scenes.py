@@ -112,13 +112,13 @@ def light_quad(b, a, size=0.25, y=0.999, cx=0.5, cz=0.5):
 def box_scene(with_spheres=True):
     """Plain SPPM box workload (diffuse + specular spheres)."""
     attrs = [
-        dict(kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),                     # 0 white
-        dict(kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),                     # 1 red
-        dict(kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),                     # 2 green
-        dict(kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),       # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),         # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),       # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
              if_opaque=1),                                                                            # 3 light
-        dict(kind=MAT_DIFFUSE, albedo=(0.75, 0.75, 0.75), is_sppm=1, if_opaque=1),                     # 4 sphere
-        dict(kind=MAT_METAL, albedo=(0.95, 0.95, 0.95), if_delta=1, is_sppm=0, if_opaque=1),           # 5 mirror
+        dict(name="diffuse_sphere", kind=MAT_DIFFUSE, albedo=(0.75, 0.75, 0.75), is_sppm=1, if_opaque=1),  # 4
+        dict(name="mirror_sphere", kind=MAT_METAL, albedo=(0.95, 0.95, 0.95), if_delta=1, is_sppm=0, if_opaque=1),  # 5
     ]
     b = _Builder()
     box_walls(b, 0, 1, 2)
@@ -272,18 +272,19 @@ def full_scene(freq=100, seed=1234, tex_size=512):
     checker = checker_texture()
     textures = [albedo, rough, normal, checker]
     attrs = [
-        dict(kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),                            # 0
-        dict(kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),                            # 1
-        dict(kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),                            # 2
-        dict(kind=MAT_DIFFUSE, albedo=(1.0, 1.0, 1.0), radiance=(255.0, 255.0, 255.0), is_sppm=1,
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),              # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),                # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),              # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(1.0, 1.0, 1.0), radiance=(255.0, 255.0, 255.0), is_sppm=1,
              if_opaque=1, tex_albedo=3),                                                                      # 3 light
         # GGX conductor: albedo = F0, scalar complex IOR (eta=1, k=20) so F(0) ~ 0.99
-        dict(kind=MAT_ROUGH_CONDUCTOR, albedo=(0.95, 0.64, 0.54), roughness=0.3, index=1.0, extinction=20.0,
+        dict(name="conductor_sphere", kind=MAT_ROUGH_CONDUCTOR, albedo=(0.95, 0.64, 0.54), roughness=0.3, index=1.0, extinction=20.0,
              is_sppm=1, if_opaque=1),                                                                         # 4
-        dict(kind=MAT_GGX_GLASS, albedo=(0.98, 0.98, 0.98), roughness=0.1, index=1.5, is_sppm=0,
+        dict(name="glass_sphere", kind=MAT_GGX_GLASS, albedo=(0.98, 0.98, 0.98), roughness=0.1, index=1.5, is_sppm=0,
              if_opaque=0),                                                                                    # 5
-        dict(kind=MAT_DIFFUSE, albedo=(0.8, 0.8, 0.8), is_sppm=1, if_opaque=1, tex_albedo=0, tex_bump=2),     # 6 floor
-        dict(kind=MAT_ROUGH_CONDUCTOR, albedo=(0.9, 0.9, 0.9), roughness=0.4, index=1.0, extinction=20.0,
+        dict(name="floor", kind=MAT_DIFFUSE, albedo=(0.8, 0.8, 0.8), is_sppm=1, if_opaque=1, tex_albedo=0,
+             tex_bump=2),                                                                             # 6
+        dict(name="displaced_mesh", kind=MAT_ROUGH_CONDUCTOR, albedo=(0.9, 0.9, 0.9), roughness=0.4, index=1.0, extinction=20.0,
              is_sppm=1, if_opaque=1, tex_roughness=1),                                                        # 7 mesh
     ]
     b = _Builder()
@@ -301,3 +302,56 @@ def full_scene(freq=100, seed=1234, tex_size=512):
     b.uv.extend(T.reshape(-1, 9))
     b.attr.extend([7] * len(V))
     return b.scene(attrs, textures)
+
+
+# ---- OBJ / MTL export ----------------------------------------------------------
+def _f(x):
+    return "%.9g" % float(np.float32(x))
+
+
+def write_obj(scene, directory, stem="scene"):
+    """Writes the scene as OBJ + MTL + binary PPM textures (the format the
+    C++ facade's Scene::CreateScene reads).  Every facet is written with its
+    own three v / vt / vn records, in facet order, so facet i of the file is
+    facet i of the scene.  Materials are written in attribute order with
+    Kd = albedo, Ke = radiance and map_Kd / map_d / map_bump; the surface
+    kind and the remaining parameters come from the material name (the
+    facade's example material manager maps the names used here).  Textures
+    must be 8-bit (stored as byte / 255, alpha 1).  Returns the OBJ path."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    tex_names = []
+    for i, t in enumerate(scene.textures):
+        b = np.rint(t[..., :3].astype(np.float64) * 255.0)
+        if not (np.all(t[..., 3] == 1.0) and np.array_equal((b.astype(np.float32) / np.float32(255.0)), t[..., :3])):
+            raise ValueError(f"texture {i} is not an 8-bit RGB image")
+        name = f"{stem}_tex{i}.ppm"
+        h, w = t.shape[:2]
+        with open(os.path.join(directory, name), "wb") as f:
+            f.write(b"P6\n%d %d\n255\n" % (w, h))
+            f.write(b.astype(np.uint8).tobytes())
+        tex_names.append(name)
+    with open(os.path.join(directory, stem + ".mtl"), "w") as f:
+        for i, a in enumerate(scene.attrs):
+            f.write(f"newmtl {a.get('name', f'material{i}')}\n")
+            f.write("Kd %s\n" % " ".join(_f(v) for v in a.get("albedo", (0.0, 0.0, 0.0))))
+            f.write("Ke %s\n" % " ".join(_f(v) for v in a.get("radiance", (0.0, 0.0, 0.0))))
+            for key, role in (("tex_albedo", "map_Kd"), ("tex_roughness", "map_d"), ("tex_bump", "map_bump")):
+                if a.get(key, -1) >= 0:
+                    f.write(f"{role} {tex_names[a[key]]}\n")
+    P, N, T = scene.pos.reshape(-1, 3), scene.nrm.reshape(-1, 3), scene.uv.reshape(-1, 3)
+    lines = [f"mtllib {stem}.mtl\n"]
+    lines += ["v %s %s %s\n" % (_f(p[0]), _f(p[1]), _f(p[2])) for p in P]
+    lines += ["vt %s %s\n" % (_f(t[0]), _f(t[1])) for t in T]
+    lines += ["vn %s %s %s\n" % (_f(n[0]), _f(n[1]), _f(n[2])) for n in N]
+    cur = None
+    for i, a in enumerate(scene.attr):
+        if a != cur:
+            lines.append(f"usemtl {scene.attrs[a].get('name', f'material{a}')}\n")
+            cur = a
+        k = 3 * i + 1
+        lines.append(f"f {k}/{k}/{k} {k + 1}/{k + 1}/{k + 1} {k + 2}/{k + 2}/{k + 2}\n")
+    path = os.path.join(directory, stem + ".obj")
+    with open(path, "w") as f:
+        f.writelines(lines)
+    return path

@@ -104,7 +104,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
-    lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
+    lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
     lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     return lib
 
@@ -128,10 +128,11 @@ def _check(rc):
         raise SppmError(f"sppm error {rc}: {lib().sppm_last_error().decode()}")
 
 
-def scene_desc(scene):
-    """Builds a SceneDesc from a scenes.Scene; returns (desc, keepalive)."""
-    attrs = (Attr * len(scene.attrs))()
-    for i, a in enumerate(scene.attrs):
+def attr_records(attr_dicts):
+    """Attribute dicts (SURFACE_PARAM fields; defaults as material.h:77-89
+    except is_sppm, which a scene states) -> Attr array."""
+    attrs = (Attr * len(attr_dicts))()
+    for i, a in enumerate(attr_dicts):
         attrs[i].kind = a["kind"]
         attrs[i].index = a.get("index", 1.0)
         attrs[i].if_delta = int(a.get("if_delta", 0))
@@ -145,6 +146,12 @@ def scene_desc(scene):
         attrs[i].tex_albedo = a.get("tex_albedo", -1)
         attrs[i].tex_roughness = a.get("tex_roughness", -1)
         attrs[i].tex_bump = a.get("tex_bump", -1)
+    return attrs
+
+
+def scene_desc(scene):
+    """Builds a SceneDesc from a scenes.Scene; returns (desc, keepalive)."""
+    attrs = attr_records(scene.attrs)
     texs = (Texture * max(1, len(scene.textures)))()
     for i, t in enumerate(scene.textures):
         texs[i].height, texs[i].width = t.shape[0], t.shape[1]
@@ -243,9 +250,10 @@ class Renderer:
         return v.value
 
     def counters(self):
-        v = [C.c_int64() for _ in range(4)]
+        v = [C.c_int64() for _ in range(5)]
         _check(self.lib.sppm_counters(self.h, *[C.byref(x) for x in v]))
-        return dict(hit_points=v[0].value, neighbours=v[1].value, stored=v[2].value, passes=v[3].value)
+        return dict(hit_points=v[0].value, neighbours=v[1].value, stored=v[2].value, candidates=v[3].value,
+                    passes=v[4].value)
 
     def read_photons(self):
         n = C.c_int32()

@@ -204,9 +204,11 @@ int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* 
 
 /* Cumulative counters since sppm_begin: valid hit points gathered (type HIT),
  * in-radius photons summed over hit points (sum of M), photons stored
- * (flattened list lengths), passes. */
+ * (flattened list lengths), candidate photons the gather tested (the
+ * photons of the grid cells its query boxes cover; 0 in record mode),
+ * passes.  Any pointer may be NULL. */
 int sppm_counters(sppm_ctx* ctx, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
-                  int64_t* passes);
+                  int64_t* candidates, int64_t* passes);
 
 /* ---- timing helpers for the benchmark -------------------------------------- */
 /* Average duration (ms) of the named pass phase over the `sppm_pass` calls
