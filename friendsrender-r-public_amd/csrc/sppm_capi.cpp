@@ -76,8 +76,13 @@ struct SceneBuffers {
     SceneDev dev{};
     float bmin[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
     int stack_cap = 0;
+    uint32_t* spill = nullptr;  // traversal stack entries beyond TRACE_LDS
+    int spill_blocks = 0;
+    int* next_path = nullptr;   // path counter of the persistent photon pass
+    TraceSpill trace_spill() const { return TraceSpill{spill, spill_blocks, stack_cap}; }
     void release() {
         free_bvh(&bvh);
+        dfree(spill); dfree(next_path);
         dfree(facet_nrm); dfree(facet_uv); dfree(facet_attr); dfree(attrs); dfree(tex); dfree(texels);
         dfree(lpick_prob); dfree(lpick_left); dfree(lpick_right); dfree(lights);
         dfree(ltab_uvw); dfree(ltab_L); dfree(ltab_prob); dfree(ltab_left); dfree(ltab_right);
@@ -463,6 +468,19 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width) {
     D.ltab_right = S.ltab_right;
     // traversal stack bound: each wide level pushes at most W entries
     S.stack_cap = bvh_width * (S.bvh.depth + 1) + 4;
+    S.spill_blocks = trace_grid_blocks(bvh_width);
+    if (S.spill_blocks <= 0) {
+        S.release();
+        return fail(SPPM_E_HIP, "cannot query the device for the path kernel grid");
+    }
+    {
+        const size_t per_lane = S.stack_cap > TRACE_LDS ? (size_t)(S.stack_cap - TRACE_LDS) : 1;
+        int rc;
+        if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 1))) {
+            S.release();
+            return rc;
+        }
+    }
     S.ready = true;
     return SPPM_OK;
 }
@@ -622,7 +640,8 @@ static GridDev grid_dev(sppm_ctx* c) {
 
 // ---- split steps --------------------------------------------------------
 static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
-    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.stack_cap, c->rb.flags, st);
+    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.trace_spill(), c->scene.next_path, c->rb.flags,
+                  st);
 }
 static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
@@ -636,7 +655,7 @@ static void do_camera_words(sppm_ctx* c, hipStream_t st) {
 }
 static void do_eye(sppm_ctx* c, int step, hipStream_t st) {
     fill_fixed_sample(c, step);
-    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.stack_cap, c->rb.words, c->rb.flags, st);
+    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.trace_spill(), c->rb.words, c->rb.flags, st);
 }
 static int do_gather(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
@@ -924,6 +943,20 @@ int sppm_read_neighbours(sppm_ctx* c, int64_t cap, int64_t* offsets, int32_t* id
     return SPPM_OK;
 }
 
+int sppm_scene_stats(sppm_ctx* c, sppm_scene_info* info) {
+    if (!c || !info) return fail(SPPM_E_ARG, "NULL argument");
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    const SceneBuffers& S = c->scene;
+    info->nfacets = S.dev.ntri;
+    info->nlights = S.dev.nlights;
+    info->bvh_width = S.bvh.width;
+    info->bvh_nodes = S.bvh.nnodes;
+    info->bvh_depth = S.bvh.depth;
+    info->stack_lds = S.stack_cap < TRACE_LDS ? S.stack_cap : TRACE_LDS;
+    info->stack_global = S.stack_cap > TRACE_LDS ? S.stack_cap - TRACE_LDS : 0;
+    return SPPM_OK;
+}
+
 int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, float tmax,
                        int32_t* facet, float* t, float* u, float* v) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
@@ -944,7 +977,7 @@ int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* di
     HIPOK(hipMemcpyAsync(dorg, org, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(ddir, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
     launch_set_u32(dfl, 0u, st);
-    launch_trace(c->scene.dev, n, dorg, ddir, tmin, c->scene.stack_cap, dfac, dtuv, dfl, st);
+    launch_trace(c->scene.dev, n, dorg, ddir, tmin, c->scene.trace_spill(), dfac, dtuv, dfl, st);
     HIPOK(hipGetLastError());
     std::vector<float> tuv(3 * (size_t)n);
     uint32_t fl = 0;

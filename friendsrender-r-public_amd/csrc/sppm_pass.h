@@ -28,12 +28,20 @@ struct Counters {
 };
 
 void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st);
-void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, int cap, uint32_t* flags,
-                   hipStream_t st);
-void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int cap, const uint32_t* words,
-                uint32_t* flags, hipStream_t st);
-void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, int cap, int* facet,
-                  float* tuv, uint32_t* flags, hipStream_t st);
+// Traversal stack spill area shared by the path kernels (they run one after
+// another on a stream): blocks * 64 lanes, cap - TRACE_LDS entries each.
+struct TraceSpill {
+    uint32_t* buf;
+    int blocks;
+    int cap;
+};
+int trace_grid_blocks(int width);
+void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
+                   uint32_t* flags, hipStream_t st);
+void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
+                const uint32_t* words, uint32_t* flags, hipStream_t st);
+void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
+                  int* facet, float* tuv, uint32_t* flags, hipStream_t st);
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,

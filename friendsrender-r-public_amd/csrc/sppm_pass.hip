@@ -105,26 +105,29 @@ __device__ __forceinline__ bool near1(float s) { return (double)fabs_(s - 1.0f) 
 // ---------------------------------------------------------------------------
 // createPhotonMap (SPPM.cpp:518-672), one lane per photon path
 // ---------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step, int stack_cap,
-                                                       uint32_t* __restrict__ flags) {
-    extern __shared__ uint32_t lds[];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    uint32_t* stk = lds + threadIdx.x;
-    uint32_t ovf = 0, fl = 0;
+// One photon path in flight (createPhotonMap loop state, SPPM.cpp:495-680).
+struct PhotonPath {
     MTCursor m;
-    mt_seed(m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
+    f3 rad, T, cur_org, cur_dir, next_dir;
+    float cur_index, next_index, PDF;
+    int i, pidx, loopcount, depth;
+};
 
-    // generateLightRay (SPPM.cpp:453-471)
-    const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, m, &ovf);
+// generateLightRay (SPPM.cpp:453-471) + the emission photon in slot 0
+// (SPPM.cpp:530-540); the first stored surface photon overwrites that slot
+// (SPPM.cpp:606), so writing it up front leaves the same slot contents.
+__device__ __forceinline__ void photon_start(const SceneDev& S, const SlotsDev& slots, int N, int step, int i,
+                                             PhotonPath& P, uint32_t* ovf) {
+    P.i = i;
+    mt_seed(P.m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
+    const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
     const DevLight& L = S.lights[li];
     f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
     f3 lpos, rad;
     float pdf_area;
     if (L.textured) {  // LightSampler.cpp:70-76
         const int off = L.tab_off;
-        int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, m, &ovf);
+        int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, P.m, ovf);
         float4 w = S.ltab_uvw[off + k];
         float4 Lk = S.ltab_L[off + k];
         lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
@@ -132,8 +135,8 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
         float gp = w.w / L.tab_avg / 1000.0f;
         pdf_area = (gp * 1000.0f) / L.area;
     } else {  // PointOnTriangle (Sampling.cpp:339-349)
-        float u = (float)real1(m, &ovf);
-        float v = (float)real1(m, &ovf);
+        float u = (float)real1(P.m, ovf);
+        float v = (float)real1(P.m, ovf);
         if (v > u) { u *= 0.5f; v -= u; }
         else { v *= 0.5f; u -= v; }
         pdf_area = 2.0f / fabs_(norm(cross(p1 - p0, p2 - p0)));
@@ -143,96 +146,144 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
     pdf_area *= L.flux_norm / S.lpick_avg / (float)S.nlights;
     const f3 lnorm = mk(L.normal[0], L.normal[1], L.normal[2]);
     float pdf_dir = 1.0f;
-    f3 outdir = cos_sampling(m, &ovf, lnorm, &pdf_dir);
+    f3 outdir = cos_sampling(P.m, ovf, lnorm, &pdf_dir);
     rad = rad * smin(1.0f, smax(0.0f, dot(lnorm, outdir)));
-    f3 cur_org = lpos + lnorm * SPPM_RAY_OFFSET;
-    f3 cur_dir = normalized(outdir);  // Ray ctor normalises
-    float cur_index = 1.0f;
-    f3 next_org = cur_org, next_dir = cur_dir;
-    float next_index = 1.0f;
-    float PDF = pdf_area * pdf_dir;
-    f3 T = mk(1.0f, 1.0f, 1.0f);
-    // emission photon for slot 0 (kept only if no surface photon overwrites it)
-    const f3 e_flux = (rad * T) * SPPM_PI;
-    const float e_pdf = PDF;
-    const f3 e_dir = cur_dir, e_pos = cur_org;
+    P.rad = rad;
+    P.cur_org = lpos + lnorm * SPPM_RAY_OFFSET;
+    P.cur_dir = normalized(outdir);  // Ray ctor normalises
+    P.cur_index = 1.0f;
+    P.next_dir = P.cur_dir;
+    P.next_index = 1.0f;
+    P.PDF = pdf_area * pdf_dir;
+    P.T = mk(1.0f, 1.0f, 1.0f);
+    P.pidx = 0;
+    P.loopcount = 0;
+    P.depth = 0;
+    const f3 e_flux = (rad * P.T) * SPPM_PI;
+    slots.pos[i] = make_float4(P.cur_org.x, P.cur_org.y, P.cur_org.z, 1.0f);
+    slots.flux[i] = make_float4(e_flux.x, e_flux.y, e_flux.z, P.PDF);
+    slots.dir[i] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
+    slots.valid[i] = 1;
+}
 
+// One iteration of the createPhotonMap loop body; false when the path ends.
+template <int W>
+__device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
+                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl) {
+    if (P.loopcount++ > 32 || P.pidx >= 31) return false;
+    if ((double)P.T.x < 1e-8 || (double)P.T.y < 1e-8 || (double)P.T.z < 1e-8) return false;
+    HitRec h;
+    if (!closest_hit<W>(S, P.cur_org, P.cur_dir, SPPM_RAY_OFFSET, ts, h, fl)) return false;
+    Geom g;
+    calc_geometry(S, h, g);
+    const f3 Nf = dot(g.N, P.cur_dir) > 0 ? -g.N : g.N;
+    f3 next_org = g.pos;
+    if ((double)norm2(g.radiance) > 0.00001) return false;
+    if (g.transl > 0.5f) {
+        P.T = P.T * g.transl;
+        P.cur_org = g.pos + P.cur_dir * SPPM_RAY_OFFSET;
+        return true;
+    }
+    if (P.depth > 16) return false;  // russianRulette_PhotonMapping (SPPM.h:33-52)
+    float a0 = g.albedo.x, a1 = g.albedo.y, a2 = g.albedo.z;
+    float pdf_rr = 0.98f * (a0 > a1 ? (a0 > a2 ? a0 : a2) : (a1 > a2 ? a1 : a2));
+    if (real1(P.m, ovf) > (double)pdf_rr) return false;
+    P.PDF *= pdf_rr;
+    float Ii, Io = 1.0f;
+    if (near1(P.cur_index)) { Ii = 1.0f; Io = g.index; }
+    else { Ii = P.cur_index; Io = 1.0f; }
+    FParam fp{P.cur_dir, P.next_dir, Nf, Ii, Io, g.rough, g.absorb, g.refl};
+    if (g.is_sppm) {
+        if (P.pidx >= NSLOT - 2) { *fl |= 8u; return false; }
+        f3 ph_flux = (P.rad * P.T) * g.albedo;
+        if (!(P.PDF > 0.0f) || !isfinite(ph_flux.x) || !isfinite(ph_flux.y) || !isfinite(ph_flux.z) ||
+            !isfinite(P.PDF))
+            *fl |= 16u;  // stored photon with pdf <= 0 or non-finite flux (propagated as the reference does)
+        const size_t o = (size_t)P.pidx * (size_t)slots.npaths + P.i;
+        slots.pos[o] = make_float4(g.pos.x, g.pos.y, g.pos.z, P.cur_index);
+        slots.flux[o] = make_float4(ph_flux.x, ph_flux.y, ph_flux.z, P.PDF);
+        slots.dir[o] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
+        slots.valid[o] = 1;
+        ++P.pidx;
+    }
+    float pdf_s = 1.0f;
+    P.next_dir = surf_sample(g.kind, fp, P.m, ovf, &pdf_s);
+    if (!rutil_valid_dir(P.next_dir)) return false;
+    if (g.opaque) {
+        if ((double)dot(P.next_dir, Nf) < 1e-6) return false;
+    } else {
+        P.next_index = rutil_is_reflect(P.cur_dir, P.next_dir, Nf) ? P.cur_index : Io;
+    }
+    fp.out = P.next_dir;
+    pdf_s = surf_pdf(g.kind, fp);
+    f3 outN = dot(Nf, P.next_dir) > 0 ? Nf : -Nf;
+    next_org = next_org + outN * SPPM_RAY_OFFSET;
+    float brdf = surf_brdf(g.kind, fp);
+    float cs = smax(0.0f, smin(1.0f, dot(P.next_dir, outN)));
+    if ((double)brdf < 1e-8) return false;
+    if (pdf_s < 0) return false;
+    P.T = P.T * ((g.albedo * cs) * brdf);
+    P.PDF *= pdf_s;
+    P.cur_org = next_org;
+    P.cur_dir = P.next_dir;
+    P.cur_index = P.next_index;
+    ++P.depth;
+    return true;
+}
+
+// End of a path: terminator at slot pidx + 1 (SPPM.cpp:672); slot pidx keeps
+// its content from an earlier pass and is flattened if valid (SPPM.cpp:812-822).
+__device__ __forceinline__ void photon_finish(const SlotsDev& slots, const PhotonPath& P) {
     const size_t NP = (size_t)slots.npaths;
-    int pidx = 0, loopcount = 0, depth = 0;
+    const int cnt = P.pidx == 0 ? 1 : P.pidx + (slots.valid[(size_t)P.pidx * NP + P.i] ? 1 : 0);
+    slots.valid[(size_t)(P.pidx + 1) * NP + P.i] = 0;
+    slots.count[P.i] = cnt;
+}
+
+// Persistent photon pass: each lane traces one path at a time; when at least
+// PHOTON_REGEN lanes of a wave have finished their paths, the wave takes the
+// next path indices from a global counter (one atomic per wave), so lanes do
+// not idle behind the longest path of their wave.  A path's result depends
+// only on its index, not on the lane or order that traces it.  Every wave
+// exits once the counter has passed N and its lanes are done.
+constexpr int PHOTON_REGEN = 16;
+template <int W>
+__global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step, int cap,
+                                                       uint32_t* __restrict__ spill, int* __restrict__ next_path,
+                                                       uint32_t* __restrict__ flags) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t ovf = 0, fl = 0;
+    PhotonPath P;
+    bool alive = false, exhausted = false;
     for (;;) {
-        if (loopcount++ > 32 || pidx >= 31) break;
-        if ((double)T.x < 1e-8 || (double)T.y < 1e-8 || (double)T.z < 1e-8) break;
-        HitRec h;
-        if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, stk, PATH_BLOCK, stack_cap, h, &fl)) break;
-        Geom g;
-        calc_geometry(S, h, g);
-        f3 Nf = dot(g.N, cur_dir) > 0 ? -g.N : g.N;
-        next_org = g.pos;
-        if ((double)norm2(g.radiance) > 0.00001) break;
-        if (g.transl > 0.5f) {
-            T = T * g.transl;
-            cur_org = g.pos + cur_dir * SPPM_RAY_OFFSET;
+        const unsigned long long idle = __ballot(!alive);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= PHOTON_REGEN) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(next_path, nidle);
+            base = __shfl(base, leader, 64);
+            if (!alive) {
+                const int i = base + __popcll(idle & below);
+                if (i < N) {
+                    photon_start(S, slots, N, step, i, P, &ovf);
+                    alive = true;
+                }
+            }
+            if (base + nidle >= N) exhausted = true;
+        }
+        if (__ballot(alive) == 0ull) {
+            if (exhausted) break;
             continue;
         }
-        if (depth > 16) break;  // russianRulette_PhotonMapping (SPPM.h:33-52)
-        float a0 = g.albedo.x, a1 = g.albedo.y, a2 = g.albedo.z;
-        float pdf_rr = 0.98f * (a0 > a1 ? (a0 > a2 ? a0 : a2) : (a1 > a2 ? a1 : a2));
-        if (real1(m, &ovf) > (double)pdf_rr) break;
-        PDF *= pdf_rr;
-        float Ii, Io = 1.0f;
-        if (near1(cur_index)) { Ii = 1.0f; Io = g.index; }
-        else { Ii = cur_index; Io = 1.0f; }
-        FParam fp{cur_dir, next_dir, Nf, Ii, Io, g.rough, g.absorb, g.refl};
-        if (g.is_sppm) {
-            if (pidx >= NSLOT - 2) { fl |= 8u; break; }
-            f3 ph_flux = (rad * T) * g.albedo;
-            if (!(PDF > 0.0f) || !isfinite(ph_flux.x) || !isfinite(ph_flux.y) || !isfinite(ph_flux.z) ||
-                !isfinite(PDF))
-                fl |= 16u;  // stored photon with pdf <= 0 or non-finite flux (propagated as the reference does)
-            const size_t o = (size_t)pidx * NP + i;
-            slots.pos[o] = make_float4(g.pos.x, g.pos.y, g.pos.z, cur_index);
-            slots.flux[o] = make_float4(ph_flux.x, ph_flux.y, ph_flux.z, PDF);
-            slots.dir[o] = make_float4(cur_dir.x, cur_dir.y, cur_dir.z, 0.0f);
-            slots.valid[o] = 1;
-            ++pidx;
+        if (alive && !photon_bounce<W>(S, slots, P, ts, &ovf, &fl)) {
+            photon_finish(slots, P);
+            alive = false;
         }
-        float pdf_s = 1.0f;
-        next_dir = surf_sample(g.kind, fp, m, &ovf, &pdf_s);
-        if (!rutil_valid_dir(next_dir)) break;
-        if (g.opaque) {
-            if ((double)dot(next_dir, Nf) < 1e-6) break;
-        } else {
-            next_index = rutil_is_reflect(cur_dir, next_dir, Nf) ? cur_index : Io;
-        }
-        fp.out = next_dir;
-        pdf_s = surf_pdf(g.kind, fp);
-        f3 outN = dot(Nf, next_dir) > 0 ? Nf : -Nf;
-        next_org = next_org + outN * SPPM_RAY_OFFSET;
-        float brdf = surf_brdf(g.kind, fp);
-        float cs = smax(0.0f, smin(1.0f, dot(next_dir, outN)));
-        if ((double)brdf < 1e-8) break;
-        if (pdf_s < 0) break;
-        T = T * ((g.albedo * cs) * brdf);
-        PDF *= pdf_s;
-        cur_org = next_org;
-        cur_dir = next_dir;
-        cur_index = next_index;
-        ++depth;
     }
-    int cnt;
-    if (pidx == 0) {
-        const size_t o = (size_t)i;
-        slots.pos[o] = make_float4(e_pos.x, e_pos.y, e_pos.z, 1.0f);
-        slots.flux[o] = make_float4(e_flux.x, e_flux.y, e_flux.z, e_pdf);
-        slots.dir[o] = make_float4(e_dir.x, e_dir.y, e_dir.z, 0.0f);
-        slots.valid[o] = 1;
-        cnt = 1;
-    } else {
-        // slot pidx keeps its content from an earlier pass (SPPM.cpp:672)
-        cnt = pidx + (slots.valid[(size_t)pidx * NP + i] ? 1 : 0);
-    }
-    slots.valid[(size_t)(pidx + 1) * NP + i] = 0;
-    slots.count[i] = cnt;
     if (ovf) fl |= 2u;
     if (fl) atomicOr(flags, fl);
 }
@@ -379,15 +430,11 @@ __device__ inline void camera_ray(const CamDev& c, float u, float v, f3 smp, f3&
 // No environment light: after the hit point is made the rest of the eye path
 // could only add IBL radiance (zero), so the path stops there.
 // ---------------------------------------------------------------------------
+// createHitpoints body for pixel i (SPPM.cpp:330-349 with getHitpointInfo).
 template <int W>
-__global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev H, int step, int stack_cap,
-                                                    const uint32_t* __restrict__ words, uint32_t* __restrict__ flags) {
-    extern __shared__ uint32_t lds[];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int NPIX = C.W * C.H;
-    if (i >= NPIX) return;
-    uint32_t* stk = lds + threadIdx.x;
-    uint32_t ovf = 0, fl = 0;
+__device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int i,
+                                         const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl) {
+    uint32_t ovf = 0;
     MTCursor m;
     mt_seed(m, (uint32_t)i + (uint32_t)C.W * (uint32_t)C.H * (uint32_t)step);
     const int w = i % C.W, hh = i / C.W;
@@ -422,7 +469,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev
             PDF *= rus;
         }
         HitRec h;
-        if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, stk, PATH_BLOCK, stack_cap, h, &fl)) {
+        if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, ts, h, &fl)) {
             if (type == 0) type = 2;
             break;
         }
@@ -485,6 +532,20 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev
     H.mat[i] = make_int2(kind, facet);
     H.absorb[i] = habs;
     if (ovf) fl |= 2u;
+}
+
+// Eye pass: a fixed grid of waves strides over the pixels.
+template <int W>
+__global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev H, int step, int cap,
+                                                    uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
+                                                    uint32_t* __restrict__ flags) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    const int NPIX = C.W * C.H;
+    uint32_t fl = 0;
+    for (int i = blockIdx.x * PATH_BLOCK + lane; i < NPIX; i += gridDim.x * PATH_BLOCK)
+        eye_path<W>(S, C, H, step, i, words, ts, fl);
     if (fl) atomicOr(flags, fl);
 }
 
@@ -990,56 +1051,70 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 // Batch closest hit (N_BVH::ClosestHit) for tests and tools.
 template <int W>
 __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const float* __restrict__ org,
-                                                      const float* __restrict__ dir, float tmin, int stack_cap,
-                                                      int* __restrict__ facet, float* __restrict__ tuv,
-                                                      uint32_t* __restrict__ flags) {
-    extern __shared__ uint32_t lds[];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                      const float* __restrict__ dir, float tmin, int cap,
+                                                      uint32_t* __restrict__ spill, int* __restrict__ facet,
+                                                      float* __restrict__ tuv, uint32_t* __restrict__ flags) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t fl = 0;
-    HitRec h;
-    f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
-    f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-    bool hit = closest_hit<W>(S, o, d, tmin, lds + threadIdx.x, PATH_BLOCK, stack_cap, h, &fl);
-    facet[i] = hit ? h.facet : -1;
-    tuv[3 * i] = hit ? h.t : 3.40282347e38f;
-    tuv[3 * i + 1] = hit ? h.u : 0.0f;
-    tuv[3 * i + 2] = hit ? h.v : 0.0f;
+    for (int i = blockIdx.x * PATH_BLOCK + lane; i < n; i += gridDim.x * PATH_BLOCK) {
+        HitRec h;
+        f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+        f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        bool hit = closest_hit<W>(S, o, d, tmin, ts, h, &fl);
+        facet[i] = hit ? h.facet : -1;
+        tuv[3 * i] = hit ? h.t : 3.40282347e38f;
+        tuv[3 * i + 1] = hit ? h.u : 0.0f;
+        tuv[3 * i + 2] = hit ? h.v : 0.0f;
+    }
     if (fl) atomicOr(flags, fl);
 }
 
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
-static size_t stack_bytes(int cap) { return (size_t)cap * PATH_BLOCK * sizeof(uint32_t); }
+// Blocks of the path kernels: enough resident waves to fill the device (the
+// traversal spill area is sized for this many lanes).
+int trace_grid_blocks(int width) {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e;
+    if (width == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<2>, PATH_BLOCK, 0);
+    else if (width == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<4>, PATH_BLOCK, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<8>, PATH_BLOCK, 0);
+    if (e != hipSuccess || nb < 1) nb = 4;
+    return cus * nb;
+}
 
 void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st) {
     hipLaunchKernelGGL(k_camera_words, dim3(1), dim3(CAMW_THREADS), 0, st, st_mt, st_pos, words, need);
 }
 
-void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, int cap, uint32_t* flags,
-                   hipStream_t st) {
-    dim3 g((N + PATH_BLOCK - 1) / PATH_BLOCK), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_photon<2>, g, b, stack_bytes(cap), st, S, slots, N, step, cap, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_photon<4>, g, b, stack_bytes(cap), st, S, slots, N, step, cap, flags);
-    else hipLaunchKernelGGL(k_photon<8>, g, b, stack_bytes(cap), st, S, slots, N, step, cap, flags);
+void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
+                   uint32_t* flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, next_path, 1);
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+    if (S.width == 2) hipLaunchKernelGGL(k_photon<2>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
+    else if (S.width == 4) hipLaunchKernelGGL(k_photon<4>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
+    else hipLaunchKernelGGL(k_photon<8>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
 }
 
-void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int cap, const uint32_t* words,
-                uint32_t* flags, hipStream_t st) {
-    int np = C.W * C.H;
-    dim3 g((np + PATH_BLOCK - 1) / PATH_BLOCK), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_eye<2>, g, b, stack_bytes(cap), st, S, C, H, step, cap, words, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_eye<4>, g, b, stack_bytes(cap), st, S, C, H, step, cap, words, flags);
-    else hipLaunchKernelGGL(k_eye<8>, g, b, stack_bytes(cap), st, S, C, H, step, cap, words, flags);
+void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
+                const uint32_t* words, uint32_t* flags, hipStream_t st) {
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+    if (S.width == 2) hipLaunchKernelGGL(k_eye<2>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
+    else if (S.width == 4) hipLaunchKernelGGL(k_eye<4>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
+    else hipLaunchKernelGGL(k_eye<8>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
 }
 
-void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, int cap, int* facet,
-                  float* tuv, uint32_t* flags, hipStream_t st) {
-    dim3 g((n + PATH_BLOCK - 1) / PATH_BLOCK), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_trace<2>, g, b, stack_bytes(cap), st, S, n, org, dir, tmin, cap, facet, tuv, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_trace<4>, g, b, stack_bytes(cap), st, S, n, org, dir, tmin, cap, facet, tuv, flags);
-    else hipLaunchKernelGGL(k_trace<8>, g, b, stack_bytes(cap), st, S, n, org, dir, tmin, cap, facet, tuv, flags);
+void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
+                  int* facet, float* tuv, uint32_t* flags, hipStream_t st) {
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+    if (S.width == 2) hipLaunchKernelGGL(k_trace<2>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
+    else if (S.width == 4) hipLaunchKernelGGL(k_trace<4>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
+    else hipLaunchKernelGGL(k_trace<8>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
 }
 
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,

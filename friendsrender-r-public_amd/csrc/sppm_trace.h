@@ -9,12 +9,23 @@
 // and the slab test is inclusive, so no box containing an accepted hit is
 // ever culled.
 //
-// Traversal keeps its stack in LDS: entry s of thread t lives at
-// stk[s * stride + t] (stride = blockDim.x), bank-conflict free.
+// Traversal descends into the nearest intersected child and pushes the
+// others far-to-near.  The first TRACE_LDS stack entries of a lane live in
+// LDS (entry s of lane t at lds[s * 64 + t], bank-conflict free); deeper
+// entries spill to a global area (entry s at glob[(s - TRACE_LDS) * gstride]),
+// so the LDS footprint, and with it the occupancy, does not depend on the
+// tree depth.  The host sizes the total capacity from the measured depth.
 #pragma once
 #include "sppm_device.h"
 
 namespace sppm {
+
+struct TraceStack {
+    uint32_t* lds;   // LDS base + lane; stride 64 (PATH_BLOCK)
+    uint32_t* glob;  // spill area + the lane's slot; stride gstride
+    int gstride;
+    int cap;         // total entries
+};
 
 struct HitRec {
     float t, u, v;
@@ -81,8 +92,8 @@ __device__ __forceinline__ void sort_children(float* d, uint32_t* r) {
 // stack capacity was exceeded (the host sizes it from the tree depth, so
 // this never happens for a well-formed build).
 template <int W>
-__device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, uint32_t* __restrict__ stk,
-                            int stride, int cap, HitRec& h, uint32_t* flags) {
+__device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const TraceStack& ts, HitRec& h,
+                            uint32_t* flags) {
     const float BIG = 1e16f;
     f3 inv;
     inv.x = fabs_(dir.x) < 1e-16f ? (dir.x > 0.0f ? BIG : -BIG) : 1.0f / dir.x;
@@ -127,20 +138,25 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, uint3
                 r[c] = hit ? cr : EMPTY_CHILD;
             }
             sort_children<W>(d, r);
-            // push far-to-near so the nearest child is popped first
+            if (r[0] != EMPTY_CHILD) {
+                // push the farther children far-to-near, descend into the nearest
 #pragma unroll
-            for (int c = W - 1; c >= 0; --c) {
-                if (r[c] != EMPTY_CHILD) {
-                    if (sp < cap) stk[sp * stride] = r[c];
-                    else *flags |= 4u;
-                    ++sp;
+                for (int c = W - 1; c >= 1; --c) {
+                    if (r[c] != EMPTY_CHILD) {
+                        if (sp < TRACE_LDS) ts.lds[sp * 64] = r[c];
+                        else if (sp < ts.cap) ts.glob[(size_t)(sp - TRACE_LDS) * ts.gstride] = r[c];
+                        else *flags |= 4u;
+                        ++sp;
+                    }
                 }
+                ref = r[0];
+                continue;
             }
         }
         if (sp == 0) break;
         --sp;
-        if (sp >= cap) break;  // overflowed stack: stop (flagged)
-        ref = stk[sp * stride];
+        if (sp >= ts.cap) break;  // overflowed stack: stop (flagged)
+        ref = sp < TRACE_LDS ? ts.lds[sp * 64] : ts.glob[(size_t)(sp - TRACE_LDS) * ts.gstride];
     }
     if (bslot < 0 || !(best < 3.40282347e38f)) {
         h.facet = -1;

@@ -26,7 +26,7 @@ EXPORTED = [
     "sppm_pass_build", "sppm_pass_eye", "sppm_pass_gather", "sppm_sync", "sppm_reconstruct",
     "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
     "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms", "sppm_counters",
-    "sppm_read_path_counts",
+    "sppm_read_path_counts", "sppm_scene_stats",
 ]
 
 
@@ -36,6 +36,11 @@ class Attr(C.Structure):
                 ("radiance", C.c_float * 3), ("reflectance", C.c_float), ("extinction", C.c_float),
                 ("is_sppm", C.c_int32), ("tex_albedo", C.c_int32), ("tex_roughness", C.c_int32),
                 ("tex_bump", C.c_int32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nfacets", "nlights", "bvh_width", "bvh_nodes", "bvh_depth", "stack_lds",
+                                         "stack_global")]
 
 
 class Texture(C.Structure):
@@ -104,6 +109,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
+    lib.sppm_scene_stats.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]
     lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
     lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     return lib
@@ -248,6 +254,11 @@ class Renderer:
         v = C.c_float()
         _check(self.lib.sppm_kernel_ms(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    def scene_stats(self):
+        info = SceneInfo()
+        _check(self.lib.sppm_scene_stats(self.h, C.byref(info)))
+        return {n: getattr(info, n) for n, _ in SceneInfo._fields_}
 
     def counters(self):
         v = [C.c_int64() for _ in range(5)]

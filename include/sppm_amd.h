@@ -210,6 +210,14 @@ int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* 
 int sppm_counters(sppm_ctx* ctx, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
                   int64_t* candidates, int64_t* passes);
 
+/* Closest-hit structure of the uploaded scene: facets, area lights, BVH
+ * width, wide nodes, depth in wide levels, traversal stack entries per lane
+ * (LDS part) and the global-memory stack entries per lane behind it. */
+typedef struct sppm_scene_info {
+    int32_t nfacets, nlights, bvh_width, bvh_nodes, bvh_depth, stack_lds, stack_global;
+} sppm_scene_info;
+int sppm_scene_stats(sppm_ctx* ctx, sppm_scene_info* info);
+
 /* ---- timing helpers for the benchmark -------------------------------------- */
 /* Average duration (ms) of the named pass phase over the `sppm_pass` calls
  * since sppm_begin or the last reset, measured with hipEvents recorded on the
