@@ -195,7 +195,12 @@ struct sppm_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;
     bool own_stream = false;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Camera sample words (UniformSample stream) are generated on stream2 one
+    // pass ahead into two buffers: block k (the words of the k-th eye pass)
+    // lands in buffer k % 2 and is overwritten only after eye pass k has run.
+    hipEvent_t ev_words[2] = {nullptr, nullptr}, ev_eye[2] = {nullptr, nullptr};
+    bool eye_recorded[2] = {false, false};
+    long long words_gen = 0, words_used = 0;
     SceneBuffers scene;
     RenderBuffers rb;
     sppm_camera_desc cam{};
@@ -230,8 +235,10 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
         c->own_stream = true;
     }
     HIPOK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-    HIPOK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    HIPOK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (int b = 0; b < 2; ++b) {
+        HIPOK(hipEventCreateWithFlags(&c->ev_words[b], hipEventDisableTiming));
+        HIPOK(hipEventCreateWithFlags(&c->ev_eye[b], hipEventDisableTiming));
+    }
     c->params.alpha = 0.9f;
     c->params.first_radius = 0.015f;
     c->params.photons_per_pass = 0;
@@ -248,8 +255,10 @@ int sppm_destroy(sppm_ctx* c) {
     c->rb.release();
     c->scene.release();
     for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->destroy();
-    (void)hipEventDestroy(c->ev_fork);
-    (void)hipEventDestroy(c->ev_join);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipEventDestroy(c->ev_words[b]);
+        (void)hipEventDestroy(c->ev_eye[b]);
+    }
     (void)hipStreamDestroy(c->stream2);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -546,6 +555,9 @@ int sppm_begin(sppm_ctx* c) {
     if (!c->have_params) return fail(SPPM_E_STATE, "no params set");
     HIPOK(hipSetDevice(c->device));
     HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamSynchronize(c->stream2));
+    c->words_gen = c->words_used = 0;
+    c->eye_recorded[0] = c->eye_recorded[1] = false;
     RenderBuffers& R = c->rb;
     R.release();
     const int N = c->params.photons_per_pass;
@@ -567,7 +579,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
         (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
-        (rc = dalloc(&R.words, 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
+        (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
         (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)gather_tile_count(W, Hh))) ||
         (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, gather_tile_count(W, Hh) + 1)) ||
@@ -649,13 +661,41 @@ static void do_build(sppm_ctx* c, hipStream_t st) {
     launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_count, R.cell_start, R.cell_scan_tmp, R.keyrank, R.gpos, R.gflux, R.gdir, st);
 }
-static void do_camera_words(sppm_ctx* c, hipStream_t st) {
-    if (c->cam.sampler == SPPM_SAMPLER_UNIFORM)
-        launch_camera_words(c->rb.cam_mt, c->rb.cam_pos, c->rb.words, 6LL * c->rb.np, st);
+// Generates camera word blocks up to and including block k on stream2, in
+// stream order (each block continues the shared mt19937 stream).
+static int words_generate_upto(sppm_ctx* c, long long k) {
+    if (c->cam.sampler != SPPM_SAMPLER_UNIFORM) return SPPM_OK;
+    const size_t blk = 6 * (size_t)c->rb.np;
+    while (c->words_gen <= k) {
+        const int b = (int)(c->words_gen & 1);
+        if (c->eye_recorded[b]) HIPOK(hipStreamWaitEvent(c->stream2, c->ev_eye[b], 0));
+        launch_camera_words(c->rb.cam_mt, c->rb.cam_pos, c->rb.words + b * blk, (long long)blk, c->stream2);
+        HIPOK(hipEventRecord(c->ev_words[b], c->stream2));
+        c->words_gen += 1;
+    }
+    return SPPM_OK;
 }
-static void do_eye(sppm_ctx* c, int step, hipStream_t st) {
+// Eye pass on stream st with the next camera word block.
+static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
     fill_fixed_sample(c, step);
-    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.trace_spill(), c->rb.words, c->rb.flags, st);
+    const bool uni = c->cam.sampler == SPPM_SAMPLER_UNIFORM;
+    const long long u = c->words_used;
+    const int b = (int)(u & 1);
+    if (uni) {
+        int rc = words_generate_upto(c, u);
+        if (rc) return rc;
+        HIPOK(hipStreamWaitEvent(st, c->ev_words[b], 0));
+    }
+    if (t) t->start(st);
+    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.trace_spill(),
+               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, st);
+    if (t) t->stop(st);
+    if (uni) {
+        HIPOK(hipEventRecord(c->ev_eye[b], st));
+        c->eye_recorded[b] = true;
+        c->words_used = u + 1;
+    }
+    return SPPM_OK;
 }
 static int do_gather(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
@@ -709,8 +749,8 @@ int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
 int sppm_pass_eye(sppm_ctx* c, int step) {
     int rc = check_ready(c);
     if (rc) return rc;
-    do_camera_words(c, c->stream);
-    do_eye(c, step, c->stream);
+    rc = do_eye(c, step, c->stream, nullptr);
+    if (rc) return rc;
     HIPOK(hipGetLastError());
     return SPPM_OK;
 }
@@ -729,21 +769,18 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     if (rc) return rc;
     const int step = c->pass + 1;
     hipStream_t st = c->stream;
-    // camera stream words on the side stream, overlapping the photon pass
-    HIPOK(hipEventRecord(c->ev_fork, st));
-    HIPOK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    do_camera_words(c, c->stream2);
-    HIPOK(hipEventRecord(c->ev_join, c->stream2));
+    // camera words of this pass (first pass only) and of the next one, on the
+    // side stream: they overlap this pass's kernels
+    rc = words_generate_upto(c, c->words_used + 1);
+    if (rc) return rc;
     c->t_photons.start(st);
     do_photons(c, step, st);
     c->t_photons.stop(st);
     c->t_build.start(st);
     do_build(c, st);
     c->t_build.stop(st);
-    HIPOK(hipStreamWaitEvent(st, c->ev_join, 0));
-    c->t_eye.start(st);
-    do_eye(c, step, st);
-    c->t_eye.stop(st);
+    rc = do_eye(c, step, st, &c->t_eye);
+    if (rc) return rc;
     c->t_gather.start(st);
     rc = do_gather(c, st);
     if (rc) return rc;
