@@ -152,10 +152,14 @@ def main():
         "build": stored * BUILD_BYTES_PER_PHOTON,
         "gather": hits * GATHER_BYTES_PER_HIT + nbs * GATHER_BYTES_PER_NEIGHBOUR,
     }
+    # the roofline is given for the gather, the HBM-bound phase with the
+    # largest time; the photon and eye phases are ray traversal (VALU / latency
+    # bound, no HBM roofline) and are reported in phases_ms
     dom = max(("photons", "build", "eye", "gather"), key=lambda k: ms[k])
-    rk = dom if dom in algo else "gather"
+    rk = "gather"
     achieved = algo[rk] / (ms[rk] * 1e-3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.md"))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.md")) \
+        if args.workload == "full" else None
     value = cfg["photons"] * K / el
     out = {
         "metric": "photons/s (plain SPPM pass, full-material workload)",
@@ -176,7 +180,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"kernel": rk, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": int(algo[rk])},
+                     "algorithmic_bytes_per_launch": int(algo[rk]), "dominant_phase": dom,
+                     "traffic_source": "profiles/traffic_gather_full.md (passes 2-4)" if traffic else None},
         "phases_ms": {k: round(v, 3) for k, v in ms.items()},
         "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
                  "gather_candidates": int(cands),
