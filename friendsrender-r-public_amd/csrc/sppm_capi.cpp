@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/sppm_amd.h"
+#include "sppm_guide.h"
 #include "sppm_host_math.h"
 #include "sppm_kernels.h"
 #include "sppm_pass.h"
@@ -79,6 +80,7 @@ struct SceneBuffers {
     uint32_t* spill = nullptr;  // traversal stack entries beyond TRACE_LDS
     int spill_blocks = 0;
     int* next_path = nullptr;   // path counter of the persistent photon pass
+    std::vector<DevLight> hlights;  // host copy of the area lights
     TraceSpill trace_spill() const { return TraceSpill{spill, spill_blocks, stack_cap}; }
     void release() {
         free_bvh(&bvh);
@@ -124,6 +126,8 @@ struct RenderBuffers {
     int* tile_scan_tmp = nullptr;
     float4* partial = nullptr;
     GroupBuffers groups{};
+    int* path_cell = nullptr;  // guided emission: emission cell per path
+    float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
     float4* image = nullptr;
@@ -134,7 +138,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -209,7 +213,12 @@ struct sppm_ctx {
     sppm_params params{};
     bool have_params = false;
     int pass = 0;
-    Timer t_photons, t_build, t_eye, t_gather;
+    Timer t_photons, t_build, t_eye, t_gather, t_guide;
+    // learned emission proposal
+    GuideBuffers guide;
+    bool guided = false;
+    sppm_guide_params gparams{};
+    GuideEmit emit{};
 };
 
 extern "C" {
@@ -254,7 +263,8 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipStreamSynchronize(c->stream2);
     c->rb.release();
     c->scene.release();
-    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->destroy();
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather, &c->t_guide}) t->destroy();
+    guide_free(c->guide);
     for (int b = 0; b < 2; ++b) {
         (void)hipEventDestroy(c->ev_words[b]);
         (void)hipEventDestroy(c->ev_eye[b]);
@@ -435,6 +445,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width) {
     HIPOK(hipMemcpyAsync(S.lpick_left, lleft.data(), sizeof(int) * lleft.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lpick_right, lright.data(), sizeof(int) * lright.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
+    S.hlights = lights;
     if (!tuvw.empty()) {
         HIPOK(hipMemcpyAsync(S.ltab_uvw, tuvw.data(), sizeof(float4) * tuvw.size(), hipMemcpyHostToDevice, st));
         HIPOK(hipMemcpyAsync(S.ltab_L, tL.data(), sizeof(float4) * tL.size(), hipMemcpyHostToDevice, st));
@@ -594,6 +605,13 @@ int sppm_begin(sppm_ctx* c) {
         return rc;
     }
     R.hits.np = np;
+    if (c->guided) {
+        if ((rc = dalloc(&R.path_cell, N)) || (rc = dalloc(&R.pdep, R.photon_cap))) {
+            R.release();
+            return rc;
+        }
+        c->emit.path_cell = R.path_cell;
+    }
     HIPOK(hipMemsetAsync(R.counts, 0, sizeof(int) * (size_t)(np + 1), c->stream));
     launch_init_slots(R.slots, c->stream);
     launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
@@ -652,8 +670,21 @@ static GridDev grid_dev(sppm_ctx* c) {
 
 // ---- split steps --------------------------------------------------------
 static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
-    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.trace_spill(), c->scene.next_path, c->rb.flags,
-                  st);
+    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.trace_spill(), c->scene.next_path,
+                  c->guided ? &c->emit : nullptr, c->rb.flags, st);
+}
+// Training records of the pass (deposit histogram), Adam steps on it, and
+// the proposal for the next pass.
+static void do_guide(sppm_ctx* c, hipStream_t st) {
+    RenderBuffers& R = c->rb;
+    guide_clear_hist(c->guide, st);
+    launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hist, st);
+    for (int k = 0; k < c->gparams.train_steps; ++k) {
+        guide_infer(c->guide, st);
+        guide_gradient(c->guide, st);
+        guide_adam(c->guide, c->gparams.learning_rate, st);
+    }
+    guide_infer(c->guide, st);
 }
 static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
@@ -720,9 +751,10 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
         R.have_lists = true;
     } else {
         launch_set_u64x2(R.rstat, 0ull, 0ull, st);
+        if (c->guided) launch_deposit_clear(R.pdep, R.nstored, st);
         launch_gather_balanced(R.hits, R.shared, G, R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups, alpha, R.tiles,
                                GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
-                               R.rstat, R.ctr, R.flags, st);
+                               R.rstat, R.ctr, R.flags, c->guided ? R.pdep : nullptr, st);
     }
     return SPPM_OK;
 }
@@ -759,6 +791,7 @@ int sppm_pass_gather(sppm_ctx* c) {
     if (rc) return rc;
     rc = do_gather(c, c->stream);
     if (rc) return rc;
+    if (c->guided) do_guide(c, c->stream);
     HIPOK(hipGetLastError());
     return SPPM_OK;
 }
@@ -785,6 +818,11 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     rc = do_gather(c, st);
     if (rc) return rc;
     c->t_gather.stop(st);
+    if (c->guided) {
+        c->t_guide.start(st);
+        do_guide(c, st);
+        c->t_guide.stop(st);
+    }
     HIPOK(hipGetLastError());
     c->pass += 1;
     if (stats) {
@@ -838,7 +876,7 @@ int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
     if (!c || !name || !ms) return fail(SPPM_E_ARG, "NULL argument");
     Timer* t = nullptr;
     if (!strcmp(name, "reset")) {
-        for (Timer* x : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) x->reset();
+        for (Timer* x : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather, &c->t_guide}) x->reset();
         *ms = 0.0f;
         return SPPM_OK;
     }
@@ -846,6 +884,7 @@ int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
     else if (!strcmp(name, "build")) t = &c->t_build;
     else if (!strcmp(name, "eye")) t = &c->t_eye;
     else if (!strcmp(name, "gather")) t = &c->t_gather;
+    else if (!strcmp(name, "guide")) t = &c->t_guide;
     if (!t) return fail(SPPM_E_ARG, "unknown kernel name '%s'", name);
     t->resolve();
     *ms = (float)t->avg();
@@ -991,6 +1030,166 @@ int sppm_scene_stats(sppm_ctx* c, sppm_scene_info* info) {
     info->bvh_depth = S.bvh.depth;
     info->stack_lds = S.stack_cap < TRACE_LDS ? S.stack_cap : TRACE_LDS;
     info->stack_global = S.stack_cap > TRACE_LDS ? S.stack_cap - TRACE_LDS : 0;
+    return SPPM_OK;
+}
+
+// ---- learned emission proposal ------------------------------------------------
+// He-uniform hidden layers from a splitmix64 stream, zero output layer: the
+// first proposal is uniform over the cells.
+static void guide_init_params(uint32_t seed, std::vector<float>& th) {
+    th.assign(GUIDE_NPARAMS, 0.0f);
+    uint64_t x = 0x9E3779B97F4A7C15ull * ((uint64_t)seed + 1u);
+    auto next = [&]() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const double lim = sqrt(6.0 / GUIDE_HID);
+    for (int l = 1; l <= 4; ++l)
+        for (int k = 0; k < GUIDE_WSZ; ++k) th[guide_w_off(l) + k] = (float)((2.0 * next() - 1.0) * lim);
+}
+
+// The proposal's light frame: exactly one untextured rectangle (two facets,
+// equal radiance), so the plain emission is uniform on it.
+static int guide_light_frame(sppm_ctx* c, GuideEmit& E) {
+    const auto& Ls = c->scene.hlights;
+    if (Ls.size() != 2 || Ls[0].textured || Ls[1].textured)
+        return fail(SPPM_E_SCENE, "guided emission needs exactly one untextured rectangular light (2 facets); scene has %d",
+                    (int)Ls.size());
+    std::vector<H::V3> v;
+    for (const DevLight& L : Ls)
+        for (int k = 0; k < 3; ++k) {
+            H::V3 p = H::ld(L.p + 3 * k);
+            bool dup = false;
+            for (const H::V3& q : v) dup = dup || H::norm2(p - q) < 1e-12f;
+            if (!dup) v.push_back(p);
+        }
+    if (v.size() != 4) return fail(SPPM_E_SCENE, "guided emission: the light's two facets do not form a quad");
+    int a = 0, b = 1;
+    float best = -1.0f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j)
+            if (H::norm2(v[i] - v[j]) > best) { best = H::norm2(v[i] - v[j]); a = i; b = j; }
+    int o[2], k = 0;
+    for (int i = 0; i < 4; ++i)
+        if (i != a && i != b) o[k++] = i;
+    const H::V3 P0 = v[a];
+    H::V3 e1 = v[o[0]] - P0, e2 = v[o[1]] - P0;
+    const H::V3 ln = H::ld(Ls[0].normal);
+    if (H::dot(H::cross(e1, e2), ln) < 0.0f) std::swap(e1, e2);
+    const float l1 = sqrtf(H::norm2(e1)), l2 = sqrtf(H::norm2(e2));
+    if (fabsf(H::dot(e1, e2)) > 1e-4f * l1 * l2) return fail(SPPM_E_SCENE, "guided emission: the light is not a rectangle");
+    const H::V3 n = H::normalized(H::cross(e1, e2));
+    if (H::dot(n, ln) < 0.999f) return fail(SPPM_E_SCENE, "guided emission: light normal differs from the facet normals");
+    const float r0 = Ls[0].flux[0] / Ls[0].area, r1 = Ls[1].flux[0] / Ls[1].area;
+    if (fabsf(r0 - r1) > 1e-5f * fabsf(r0)) return fail(SPPM_E_SCENE, "guided emission: the light facets differ in radiance");
+    const H::V3 t1 = e1 / l1, t2 = H::cross(n, t1);
+    for (int q = 0; q < 3; ++q) {
+        E.P0[q] = (&P0.x)[q]; E.e1[q] = (&e1.x)[q]; E.e2[q] = (&e2.x)[q];
+        E.n[q] = (&n.x)[q]; E.t1[q] = (&t1.x)[q]; E.t2[q] = (&t2.x)[q];
+        E.rad[q] = Ls[0].flux[q] / (2.0f * H::PI * Ls[0].area);  // as the plain emission recovers it
+    }
+    E.inv_e1 = 1.0f / H::norm2(e1);
+    E.inv_e2 = 1.0f / H::norm2(e2);
+    E.inv_area = 1.0f / (l1 * l2);
+    return SPPM_OK;
+}
+
+int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
+    if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    HIPOK(hipSetDevice(c->device));
+    if (!p->enable) {
+        c->guided = false;
+        return SPPM_OK;
+    }
+    if (!(p->mix_plain >= 0.25f && p->mix_plain <= 1.0f)) return fail(SPPM_E_ARG, "mix_plain must be in [0.25, 1]");
+    if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
+    if (c->params.record_neighbours) return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0)");
+    GuideEmit E{};
+    int rc = guide_light_frame(c, E);
+    if (rc) return rc;
+    if (!c->guide.theta && guide_alloc(c->guide)) return fail(SPPM_E_NOMEM, "guide buffers");
+    std::vector<float> th;
+    guide_init_params(p->seed, th);
+    if (guide_upload_params(c->guide, th.data(), c->stream)) return fail(SPPM_E_HIP, "guide upload");
+    guide_clear_hist(c->guide, c->stream);
+    guide_infer(c->guide, c->stream);
+    HIPOK(hipGetLastError());
+    E.w = p->mix_plain;
+    E.prob = c->guide.prob;
+    E.cdf = c->guide.cdf;
+    E.path_cell = c->rb.path_cell;
+    c->emit = E;
+    c->gparams = *p;
+    c->guided = true;
+    if (c->rb.ready && !c->rb.path_cell) {
+        RenderBuffers& R = c->rb;
+        if ((rc = dalloc(&R.path_cell, R.N)) || (rc = dalloc(&R.pdep, R.photon_cap))) return rc;
+        c->emit.path_cell = R.path_cell;
+    }
+    HIPOK(hipStreamSynchronize(c->stream));
+    return SPPM_OK;
+}
+
+static int guide_ready(sppm_ctx* c) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (!c->guide.theta) return fail(SPPM_E_STATE, "guided emission not set up (sppm_guide_set)");
+    HIPOK(hipSetDevice(c->device));
+    return SPPM_OK;
+}
+
+int sppm_guide_weights(sppm_ctx* c, float* out, const float* in) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    if (in) {
+        if (guide_upload_params(c->guide, in, c->stream)) return fail(SPPM_E_HIP, "guide upload");
+        guide_infer(c->guide, c->stream);
+    }
+    if (out) HIPOK(hipMemcpyAsync(out, c->guide.theta, sizeof(float) * GUIDE_NPARAMS, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return SPPM_OK;
+}
+
+int sppm_guide_gradient(sppm_ctx* c, const float* hist, float* loss, float* grad) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
+    guide_infer(c->guide, c->stream);
+    guide_gradient(c->guide, c->stream);
+    float sc[3];
+    HIPOK(hipMemcpyAsync(sc, c->guide.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPOK(hipMemcpyAsync(grad, c->guide.grad, sizeof(float) * GUIDE_NPARAMS, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    if (loss) *loss = sc[2];
+    return SPPM_OK;
+}
+
+int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    if (steps < 0) return fail(SPPM_E_ARG, "steps < 0");
+    if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
+    const float lr = c->gparams.learning_rate > 0.0f ? c->gparams.learning_rate : 1e-3f;
+    for (int k = 0; k < steps; ++k) {
+        guide_infer(c->guide, c->stream);
+        guide_gradient(c->guide, c->stream);
+        guide_adam(c->guide, lr, c->stream);
+    }
+    guide_infer(c->guide, c->stream);
+    HIPOK(hipGetLastError());
+    HIPOK(hipStreamSynchronize(c->stream));
+    return SPPM_OK;
+}
+
+int sppm_guide_read(sppm_ctx* c, float* prob, float* hist) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(c->stream));
+    if (prob) HIPOK(hipMemcpy(prob, c->guide.prob, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    if (hist) HIPOK(hipMemcpy(hist, c->guide.hist, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
     return SPPM_OK;
 }
 

@@ -36,8 +36,12 @@ struct TraceSpill {
     int cap;
 };
 int trace_grid_blocks(int width);
+struct GuideEmit;
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
-                   uint32_t* flags, hipStream_t st);
+                   const GuideEmit* guide, uint32_t* flags, hipStream_t st);
+void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
+void launch_guide_deposit(const int* count, const int* off, const float* pdep, const int* path_cell, int N, float* hist,
+                          hipStream_t st);
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
                 const uint32_t* words, uint32_t* flags, hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
@@ -60,7 +64,8 @@ struct GroupBuffers {
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
-                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st);
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
+                            hipStream_t st);
 int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sppm_guide.h"
 #include "sppm_kernels.h"
 #include "sppm_pass.h"
 #include "sppm_trace.h"
@@ -166,6 +167,89 @@ __device__ __forceinline__ void photon_start(const SceneDev& S, const SlotsDev& 
     slots.valid[i] = 1;
 }
 
+// Guided emission: draw from the mixture, evaluate the mixture density at the
+// drawn (position, direction) and use it as the photon's PDF; the photon
+// flux is L_e cos, as for plain emission, so flux / PDF stays an unbiased
+// weight whichever component drew the sample.
+__device__ __forceinline__ void photon_start_guided(const SceneDev& S, const SlotsDev& slots, const GuideEmit& G,
+                                                    int N, int step, int i, PhotonPath& P, uint32_t* ovf) {
+    P.i = i;
+    mt_seed(P.m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
+    const f3 P0 = mk(G.P0[0], G.P0[1], G.P0[2]), E1 = mk(G.e1[0], G.e1[1], G.e1[2]), E2 = mk(G.e2[0], G.e2[1], G.e2[2]);
+    const f3 Nn = mk(G.n[0], G.n[1], G.n[2]), T1 = mk(G.t1[0], G.t1[1], G.t1[2]), T2 = mk(G.t2[0], G.t2[1], G.t2[2]);
+    const float total = G.cdf[GUIDE_CELLS - 1];
+    f3 lpos, outdir;
+    if (real1(P.m, ovf) < (double)G.w) {
+        // plain: light pick + PointOnTriangle + cosine lobe (SPPM.cpp:453-468)
+        const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
+        const DevLight& L = S.lights[li];
+        f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
+        float u = (float)real1(P.m, ovf);
+        float v = (float)real1(P.m, ovf);
+        if (v > u) { u *= 0.5f; v -= u; }
+        else { v *= 0.5f; u -= v; }
+        lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
+        float pdf_dir = 1.0f;
+        outdir = normalized(cos_sampling(P.m, ovf, Nn, &pdf_dir));
+    } else {
+        // proposal: a cell by its CDF, then uniform inside the cell
+        const float x = (float)real1(P.m, ovf) * total;
+        int lo = 0, hi = GUIDE_CELLS - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (G.cdf[mid] > x) hi = mid; else lo = mid + 1;
+        }
+        const int c = lo;
+        const int it = c % GUIDE_GT, is = (c / GUIDE_GT) % GUIDE_GS, iv = (c / (GUIDE_GS * GUIDE_GT)) % GUIDE_GV,
+                  iu = c / (GUIDE_GV * GUIDE_GS * GUIDE_GT);
+        const float u = ((float)iu + (float)real1(P.m, ovf)) / (float)GUIDE_GU;
+        const float v = ((float)iv + (float)real1(P.m, ovf)) / (float)GUIDE_GV;
+        const float s = -1.0f + ((float)is + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GS);
+        const float t = -1.0f + ((float)it + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GT);
+        lpos = (P0 + E1 * u) + E2 * v;
+        const float px = 0.5f * (s + t), py = 0.5f * (s - t);
+        const float pz = fmaxf(0.0f, 1.0f - fabs_(px) - fabs_(py));
+        outdir = normalized((T1 * px + T2 * py) + Nn * pz);
+    }
+    // mixture density at (lpos, outdir)
+    const f3 rel = lpos - P0;
+    const float u = fminf(fmaxf(dot(rel, E1) * G.inv_e1, 0.0f), 0.99999994f);
+    const float v = fminf(fmaxf(dot(rel, E2) * G.inv_e2, 0.0f), 0.99999994f);
+    const float lx = dot(outdir, T1), ly = dot(outdir, T2), lz = dot(outdir, Nn);
+    const float cs = fmaxf(lz, 0.0f);
+    const float l1 = fabs_(lx) + fabs_(ly) + cs;
+    const float ox = lx / l1, oy = ly / l1, oz = cs / l1;  // octahedron point (|ox|+|oy|+oz = 1)
+    const float s = ox + oy, t = ox - oy;
+    const int iu = min((int)(u * (float)GUIDE_GU), GUIDE_GU - 1), iv = min((int)(v * (float)GUIDE_GV), GUIDE_GV - 1);
+    const int is = min(max((int)((s + 1.0f) * (0.5f * (float)GUIDE_GS)), 0), GUIDE_GS - 1);
+    const int it = min(max((int)((t + 1.0f) * (0.5f * (float)GUIDE_GT)), 0), GUIDE_GT - 1);
+    const int cell = ((iu * GUIDE_GV + iv) * GUIDE_GS + is) * GUIDE_GT + it;
+    const float plen = sqrtf(ox * ox + oy * oy + oz * oz);
+    // cell density in (u, v, s, t) -> area x solid angle: / A, * 2 |P|^3 (d omega = ds dt / (2 |P|^3))
+    const float p_mlp = G.prob[cell] / total * (float)(GUIDE_GU * GUIDE_GV) * (float)(GUIDE_GS * GUIDE_GT / 4) *
+                        G.inv_area * (2.0f * plen * plen * plen);
+    const float p_plain = G.inv_area * cs / SPPM_PI;
+    const float q = G.w * p_plain + (1.0f - G.w) * p_mlp;
+    G.path_cell[i] = cell;
+    const f3 rad = mk(G.rad[0], G.rad[1], G.rad[2]) * smin(1.0f, cs);
+    P.rad = rad;
+    P.cur_org = lpos + Nn * SPPM_RAY_OFFSET;
+    P.cur_dir = outdir;
+    P.cur_index = 1.0f;
+    P.next_dir = P.cur_dir;
+    P.next_index = 1.0f;
+    P.PDF = q;
+    P.T = mk(1.0f, 1.0f, 1.0f);
+    P.pidx = 0;
+    P.loopcount = 0;
+    P.depth = 0;
+    const f3 e_flux = (rad * P.T) * SPPM_PI;
+    slots.pos[i] = make_float4(P.cur_org.x, P.cur_org.y, P.cur_org.z, 1.0f);
+    slots.flux[i] = make_float4(e_flux.x, e_flux.y, e_flux.z, P.PDF);
+    slots.dir[i] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
+    slots.valid[i] = 1;
+}
+
 // One iteration of the createPhotonMap loop body; false when the path ends.
 template <int W>
 __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
@@ -247,10 +331,10 @@ __device__ __forceinline__ void photon_finish(const SlotsDev& slots, const Photo
 // only on its index, not on the lane or order that traces it.  Every wave
 // exits once the counter has passed N and its lanes are done.
 constexpr int PHOTON_REGEN = 16;
-template <int W>
+template <int W, bool GUIDED>
 __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step, int cap,
                                                        uint32_t* __restrict__ spill, int* __restrict__ next_path,
-                                                       uint32_t* __restrict__ flags) {
+                                                       GuideEmit G, uint32_t* __restrict__ flags) {
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
@@ -269,7 +353,8 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slot
             if (!alive) {
                 const int i = base + __popcll(idle & below);
                 if (i < N) {
-                    photon_start(S, slots, N, step, i, P, &ovf);
+                    if constexpr (GUIDED) photon_start_guided(S, slots, G, N, step, i, P, &ovf);
+                    else photon_start(S, slots, N, step, i, P, &ovf);
                     alive = true;
                 }
             }
@@ -892,9 +977,10 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
                                                       const TileRange* __restrict__ tiles, int ntiles,
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
-                                                      float4* __restrict__ partial) {
+                                                      float4* __restrict__ partial, float* __restrict__ pdep) {
     __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
     __shared__ unsigned char s_hit[64 * 64];
+    __shared__ float s_dep[64];  // luminance each staged photon deposited on this group (guided emission)
     const int lane = threadIdx.x;
     const int total = chunk_off[ntiles];
     const int CH = *chunkp;
@@ -949,6 +1035,10 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     }
                 }
                 L.M += nh;
+                if (pdep) {
+                    s_dep[lane] = 0.0f;
+                    __syncthreads();
+                }
                 for (int e = 0; e < nh; ++e) {
                     const int j = s_hit[e * 64 + lane];
                     const float4 pp = s_pos[j], pf = s_flux[j], pd = s_dir[j];
@@ -957,7 +1047,16 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
                     FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
                     const float b = surf_brdf(L.kind, fp);
-                    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
+                    const f3 c = ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
+                    L.add = L.add + c;
+                    if (pdep) {
+                        const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+                        if (lum > 0.0f && lum < 3.0e38f) atomicAdd(&s_dep[j], lum);
+                    }
+                }
+                if (pdep) {
+                    __syncthreads();
+                    if (lane < n && s_dep[lane] > 0.0f) atomicAdd(&pdep[__float_as_int(s_dir[lane].w)], s_dep[lane]);
                 }
                 __syncthreads();
             }
@@ -1081,9 +1180,9 @@ int trace_grid_blocks(int width) {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     hipError_t e;
-    if (width == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<2>, PATH_BLOCK, 0);
-    else if (width == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<4>, PATH_BLOCK, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<8>, PATH_BLOCK, 0);
+    if (width == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<2, false>, PATH_BLOCK, 0);
+    else if (width == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<4, false>, PATH_BLOCK, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photon<8, false>, PATH_BLOCK, 0);
     if (e != hipSuccess || nb < 1) nb = 4;
     return cus * nb;
 }
@@ -1093,12 +1192,22 @@ void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long lon
 }
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
-                   uint32_t* flags, hipStream_t st) {
+                   const GuideEmit* guide, uint32_t* flags, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, next_path, 1);
     dim3 g(sp.blocks), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_photon<2>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_photon<4>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
-    else hipLaunchKernelGGL(k_photon<8>, g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, flags);
+    const GuideEmit G = guide ? *guide : GuideEmit{};
+#define SPPM_PHOTON(WW, GG) \
+    hipLaunchKernelGGL((k_photon<WW, GG>), g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, G, flags)
+    if (guide) {
+        if (S.width == 2) SPPM_PHOTON(2, true);
+        else if (S.width == 4) SPPM_PHOTON(4, true);
+        else SPPM_PHOTON(8, true);
+    } else {
+        if (S.width == 2) SPPM_PHOTON(2, false);
+        else if (S.width == 4) SPPM_PHOTON(4, false);
+        else SPPM_PHOTON(8, false);
+    }
+#undef SPPM_PHOTON
 }
 
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
@@ -1149,7 +1258,8 @@ void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
-                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st) {
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
+                            hipStream_t st) {
     const int ntiles = gather_group_count(np);
     // hit-point groups: valid hit points in Morton order, 64 per group
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 1);
@@ -1163,7 +1273,7 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
-                       ntiles, chunk_off, chunkp, partial);
+                       ntiles, chunk_off, chunkp, partial, pdep);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
                        partial, counts, rstat, ctr, flags);
 }
@@ -1192,4 +1302,32 @@ void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p, v);
 }
 
+}  // namespace sppm
+
+namespace sppm {
+// ---- guided emission: training records ----------------------------------
+// Deposited luminance per flattened photon -> per path -> the histogram of
+// the paths' emission cells (float atomics: the summation order, and with it
+// the last bits of the histogram, vary from run to run).
+__global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n) {
+    const int cnt = *n;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) p[k] = 0.0f;
+}
+__global__ void k_guide_deposit(const int* __restrict__ count, const int* __restrict__ off,
+                                const float* __restrict__ pdep, const int* __restrict__ path_cell, int N,
+                                float* __restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int c = count[i], o = off[i];
+    float d = 0.0f;
+    for (int k = 0; k < c; ++k) d += pdep[o + k];
+    if (d > 0.0f) atomicAdd(&hist[path_cell[i]], d);
+}
+void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_f32_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
+}
+void launch_guide_deposit(const int* count, const int* off, const float* pdep, const int* path_cell, int N, float* hist,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, count, off, pdep, path_cell, N, hist);
+}
 }  // namespace sppm

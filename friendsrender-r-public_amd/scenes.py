@@ -355,3 +355,51 @@ def write_obj(scene, directory, stem="scene"):
     with open(path, "w") as f:
         f.writelines(lines)
     return path
+
+
+def occluded_scene(clutter_freq=61, seed=99):
+    """Occluded-light workload: two 1x1x1 rooms (x in [0,1] and [1,2]) joined
+    by a 0.1 x 0.6 door gap in the shared wall x = 1 (z in [0.45, 0.55],
+    y in [0, 0.6]); a 0.3 x 0.3 ceiling light with emission 20 in room B
+    (x in [1,2]); the camera (occluded_camera) looks from room A toward the
+    door, so light reaches the image only through the gap.  Clutter: four
+    geodesic spheres displaced by 3-octave value noise, 20 * freq^2 triangles
+    each (~300k total at freq 61), two per room, diffuse."""
+    attrs = [
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),        # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),          # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),        # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(20.0, 20.0, 20.0),
+             is_sppm=1, if_opaque=1),                                                                 # 3
+        dict(name="clutter", kind=MAT_DIFFUSE, albedo=(0.6, 0.55, 0.5), is_sppm=1, if_opaque=1),        # 4
+    ]
+    b = _Builder()
+    up, down = (0, 1, 0), (0, -1, 0)
+    b.quad((0, 0, 0), (2, 0, 0), (2, 0, 1), (0, 0, 1), up, 0)          # floor y = 0
+    b.quad((0, 1, 0), (0, 1, 1), (2, 1, 1), (2, 1, 0), down, 0)        # ceiling y = 1
+    b.quad((0, 0, 0), (0, 1, 0), (2, 1, 0), (2, 0, 0), (0, 0, 1), 0)   # z = 0
+    b.quad((0, 0, 1), (2, 0, 1), (2, 1, 1), (0, 1, 1), (0, 0, -1), 0)  # z = 1
+    b.quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), 1)   # x = 0 (room A end wall)
+    b.quad((2, 0, 0), (2, 1, 0), (2, 1, 1), (2, 0, 1), (-1, 0, 0), 2)  # x = 2 (room B end wall)
+    # shared wall x = 1 around the door gap z in [0.45, 0.55], y in [0, 0.6]
+    n = (-1, 0, 0)
+    b.quad((1, 0, 0), (1, 1, 0), (1, 1, 0.45), (1, 0, 0.45), n, 0)
+    b.quad((1, 0, 0.55), (1, 1, 0.55), (1, 1, 1), (1, 0, 1), n, 0)
+    b.quad((1, 0.6, 0.45), (1, 1, 0.45), (1, 1, 0.55), (1, 0.6, 0.55), n, 0)
+    light_quad(b, 3, size=0.3, y=0.999, cx=1.5, cz=0.5)
+    centers = [((0.3, 0.16, 0.3), 0.15), ((0.62, 0.13, 0.75), 0.12), ((1.4, 0.15, 0.3), 0.14),
+               ((1.7, 0.12, 0.7), 0.11)]
+    for k, (c, r) in enumerate(centers):
+        V, N, T = geodesic_sphere(c, r, clutter_freq, seed + k, amp=0.03)
+        b.pos.extend(V.reshape(-1, 9))
+        b.nrm.extend(N.reshape(-1, 9))
+        b.uv.extend(T.reshape(-1, 9))
+        b.attr.extend([4] * len(V))
+    return b.scene(attrs)
+
+
+def occluded_camera(width, height, sampler_id=0):
+    """Pinhole camera in room A looking at the shared wall and its door."""
+    return dict(type=CAM_PINHOLE, width=width, height=height, pos=(0.08, 0.5, 0.5), target=(1.0, 0.35, 0.5),
+                fov_deg=60.0, F=1.0, dist_to_lens=1.0, aspect=width / height, sampler=SAMPLER_UNIFORM,
+                sampler_id=sampler_id, sampler_nsample=1)

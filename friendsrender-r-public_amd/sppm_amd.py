@@ -26,8 +26,16 @@ EXPORTED = [
     "sppm_pass_build", "sppm_pass_eye", "sppm_pass_gather", "sppm_sync", "sppm_reconstruct",
     "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
     "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms", "sppm_counters",
-    "sppm_read_path_counts", "sppm_scene_stats",
+    "sppm_read_path_counts", "sppm_scene_stats", "sppm_guide_set", "sppm_guide_weights", "sppm_guide_gradient",
+    "sppm_guide_train", "sppm_guide_read",
 ]
+GUIDE_CELLS = 262144
+GUIDE_NPARAMS = 16705
+
+
+class GuideParams(C.Structure):
+    _fields_ = [("enable", C.c_int32), ("mix_plain", C.c_float), ("learning_rate", C.c_float),
+                ("train_steps", C.c_int32), ("seed", C.c_uint32)]
 
 
 class Attr(C.Structure):
@@ -110,6 +118,11 @@ def load_library(path=LIB_PATH):
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
     lib.sppm_scene_stats.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]
+    lib.sppm_guide_set.argtypes = [C.c_void_p, C.POINTER(GuideParams)]
+    lib.sppm_guide_weights.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.sppm_guide_gradient.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.sppm_guide_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
+    lib.sppm_guide_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
     lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     return lib
@@ -254,6 +267,44 @@ class Renderer:
         v = C.c_float()
         _check(self.lib.sppm_kernel_ms(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    # ---- learned emission proposal --------------------------------------
+    def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1):
+        g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed)
+        _check(self.lib.sppm_guide_set(self.h, C.byref(g)))
+
+    def guide_weights(self, new=None):
+        out = np.zeros(GUIDE_NPARAMS, np.float32)
+        src = None
+        if new is not None:
+            new = np.ascontiguousarray(new, np.float32)
+            assert new.shape == (GUIDE_NPARAMS,)
+            src = _fp(new)
+        _check(self.lib.sppm_guide_weights(self.h, _fp(out), src))
+        return out
+
+    def guide_gradient(self, hist=None):
+        grad = np.zeros(GUIDE_NPARAMS, np.float32)
+        loss = C.c_float()
+        hp = None
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, np.float32)
+            hp = _fp(hist)
+        _check(self.lib.sppm_guide_gradient(self.h, hp, C.byref(loss), _fp(grad)))
+        return loss.value, grad
+
+    def guide_train(self, hist=None, steps=1):
+        hp = None
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, np.float32)
+            hp = _fp(hist)
+        _check(self.lib.sppm_guide_train(self.h, hp, steps))
+
+    def guide_read(self):
+        prob = np.zeros(GUIDE_CELLS, np.float32)
+        hist = np.zeros(GUIDE_CELLS, np.float32)
+        _check(self.lib.sppm_guide_read(self.h, _fp(prob), _fp(hist)))
+        return prob, hist
 
     def scene_stats(self):
         info = SceneInfo()

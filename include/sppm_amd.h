@@ -218,11 +218,44 @@ typedef struct sppm_scene_info {
 } sppm_scene_info;
 int sppm_scene_stats(sppm_ctx* ctx, sppm_scene_info* info);
 
+/* ---- learned photon-emission proposal --------------------------------------
+ * No counterpart in the reference: it replaces the position x direction
+ * sampling of generateLightRay (SPPM.cpp:453-471) and the PDF the photon
+ * carries (SPPM.cpp:527).  Photons are drawn from the mixture
+ * q = w * plain + (1 - w) * proposal and carry PDF = q, so the estimate stays
+ * consistent.  The proposal is a softmax over 32x32 light (u, v) cells x 16x16
+ * hemi-octahedral direction cells whose logits come from a 64->64x4->1 MLP of
+ * Fourier features of the cell centre, trained on the GPU (bf16 MFMA, Adam)
+ * from the luminance the photons of each cell deposit on visible hit points.
+ * Scenes: exactly one untextured rectangular light made of two facets. */
+#define SPPM_GUIDE_CELLS 262144
+#define SPPM_GUIDE_NPARAMS 16705
+typedef struct sppm_guide_params {
+    int32_t enable;        /* 0: plain emission */
+    float mix_plain;       /* w, probability of the plain emission, in [0.25, 1] */
+    float learning_rate;   /* Adam step size */
+    int32_t train_steps;   /* Adam steps after each pass's gather */
+    uint32_t seed;         /* weight initialisation */
+} sppm_guide_params;
+/* Enable (after sppm_scene_upload) or disable guided emission; re-initialises the weights. */
+int sppm_guide_set(sppm_ctx* ctx, const sppm_guide_params* params);
+/* Read (out) and/or replace (in, resets Adam) the float32 weights
+ * (W1[in][out], b1, W2, b2, W3, b3, W4, b4, w5, b5). */
+int sppm_guide_weights(sppm_ctx* ctx, float* out, const float* in);
+/* Loss and gradient of the current weights for a deposit histogram
+ * (SPPM_GUIDE_CELLS floats; NULL: the last pass's histogram). */
+int sppm_guide_gradient(sppm_ctx* ctx, const float* hist, float* loss, float* grad);
+/* `steps` Adam steps on a histogram (NULL: the last pass's), then the proposal is rebuilt. */
+int sppm_guide_train(sppm_ctx* ctx, const float* hist, int steps);
+/* Current proposal probabilities per cell and the last deposit histogram (either may be NULL). */
+int sppm_guide_read(sppm_ctx* ctx, float* prob, float* hist);
+
 /* ---- timing helpers for the benchmark -------------------------------------- */
 /* Average duration (ms) of the named pass phase over the `sppm_pass` calls
  * since sppm_begin or the last reset, measured with hipEvents recorded on the
  * context stream around the phase's launches.  name: "photons", "build",
- * "eye", "gather"; "reset" clears the averages. */
+ * "eye", "gather", "guide" (training, guided emission only); "reset" clears
+ * the averages. */
 int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
 
 #ifdef __cplusplus

@@ -1,0 +1,131 @@
+"""GPU tests of the learned emission proposal, through the C ABI:
+- the MFMA forward/backward: loss, gradient and softmax against the CPU
+  oracle's bf16 evaluation on all 262,144 cells;
+- Adam's first step in closed form;
+- guided emission is unbiased: with a fixed radius (alpha = 1) guided and
+  plain SPPM estimate the same smoothed radiance;
+- what the network gains: at equal pass counts on the occluded-light scene the
+  guided render is closer to a long plain reference than the plain render."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import scenes
+import sppm_amd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(box_scene):
+    r = sppm_amd.Renderer(box_scene, scenes.pinhole_camera(32, 32), 0.9, 0.015, 2000, bvh_width=4)
+    r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=0, seed=7)
+    yield r
+    r.close()
+
+
+def test_initial_proposal_is_uniform(ctx):
+    prob, _ = ctx.guide_read()
+    np.testing.assert_allclose(prob, 1.0 / mo.NCELLS, rtol=1e-5)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    rng = np.random.default_rng(11)
+    theta = mo.init_params(5)
+    p = mo.unpack(theta.astype(np.float64))
+    p["w5"][:] = rng.normal(0, 0.5, mo.HID)
+    p["b5"][:] = 0.3
+    for l in range(1, 5):
+        p[f"b{l}"][:] = rng.normal(0, 0.1, mo.HID)
+    theta = mo.pack(p).astype(np.float32)
+    hist = (rng.gamma(0.3, 1.0, mo.NCELLS) * (rng.random(mo.NCELLS) < 0.05)).astype(np.float32)
+    X = mo.features(mo.cell_coords())
+    return theta, hist, X
+
+
+def test_gradient_matches_oracle(ctx, problem):
+    theta, hist, X = problem
+    ctx.guide_weights(theta)
+    loss, g = ctx.guide_gradient(hist)
+    want_loss, want = mo.loss_and_grad(theta, X, hist, "bf16")
+    print("loss gpu %.6f oracle %.6f" % (loss, want_loss), flush=True)
+    assert abs(loss - want_loss) <= 1e-3 * abs(want_loss)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in mo.SHAPES])
+    for k, (name, _) in enumerate(mo.SHAPES):
+        a, b = g[offs[k]:offs[k + 1]].astype(np.float64), want[offs[k]:offs[k + 1]]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"  {name}: rel L2 error {err:.2e}", flush=True)
+        assert err < 2e-2, name
+    cos = g.astype(np.float64) @ want / np.linalg.norm(g) / np.linalg.norm(want)
+    assert cos > 0.9995
+
+
+def test_softmax_matches_oracle(ctx, problem):
+    theta, hist, X = problem
+    ctx.guide_weights(theta)
+    prob, _ = ctx.guide_read()
+    logit, _ = mo.forward(theta, X, "bf16")
+    want, _ = mo.softmax(logit)
+    assert abs(prob.sum() - 1.0) < 1e-4
+    err = np.abs(prob - want) / want
+    assert np.quantile(err, 0.999) < 2e-2 and err.max() < 0.1
+
+
+def test_first_adam_step_is_sign_of_gradient(ctx, problem):
+    theta, hist, X = problem
+    ctx.guide_weights(theta)
+    _, g = ctx.guide_gradient(hist)
+    ctx.guide_train(hist, 1)
+    t1 = ctx.guide_weights()
+    big = np.abs(g) > 1e-5
+    # first bias-corrected step: -lr * g / (|g| + eps)
+    np.testing.assert_allclose((t1 - theta)[big], -1e-3 * np.sign(g[big]), rtol=2e-3, atol=1e-7)
+    assert np.all(t1[~big] - theta[~big] <= 1.1e-3)
+
+
+def occluded_small():
+    return scenes.occluded_scene(clutter_freq=6), scenes.occluded_camera(48, 48)
+
+
+def render(scene, cam, r0, photons, passes, guided, steps=4, seed=3):
+    """alpha = 1: the radius stays r0, so the accumulated flux divided by the
+    emitted photons estimates the same r0-smoothed radiance for any emission
+    density (the reference's reconstruct also rescales by passes / NumGather,
+    which depends on how often a pixel saw photons; it is left out here)."""
+    r = sppm_amd.Renderer(scene, cam, 1.0, r0, photons, bvh_width=4)
+    if guided:
+        r.guide_set(True, mix_plain=0.5, learning_rate=5e-3, train_steps=steps, seed=seed)
+    for _ in range(passes):
+        r.pass_(stats=False)
+    sh = r.read_shared()
+    r.close()
+    assert np.all(sh["radius"] == np.float32(r0))
+    est = sh["radiance"].astype(np.float64) / (float(photons) * passes * 3.14159 * r0 * r0)
+    return est.reshape(cam["height"], cam["width"], 3)
+
+
+def test_guided_is_unbiased_at_fixed_radius():
+    scene, cam = occluded_small()
+    r0 = 0.05
+    plain = render(scene, cam, r0, 100000, 60, False)
+    guided = render(scene, cam, r0, 100000, 60, True)
+    mp, mg = plain.mean(), guided.mean()
+    print("mean L plain %.5f guided %.5f" % (mp, mg), flush=True)
+    assert abs(mg - mp) < 0.05 * mp
+    # coarse blocks agree within noise
+    bp = plain.reshape(6, 8, 6, 8, 3).mean((1, 3))
+    bg = guided.reshape(6, 8, 6, 8, 3).mean((1, 3))
+    assert np.median(np.abs(bg - bp) / np.maximum(bp, 1e-6)) < 0.1
+
+
+def test_guided_beats_plain_at_equal_passes():
+    scene, cam = occluded_small()
+    r0 = 0.05
+    ref = render(scene, cam, r0, 100000, 600, False)
+    plain = render(scene, cam, r0, 100000, 30, False)
+    guided = render(scene, cam, r0, 100000, 30, True)
+    ep = np.sqrt(np.mean((plain - ref) ** 2))
+    eg = np.sqrt(np.mean((guided - ref) ** 2))
+    print("RMSE vs 600-pass plain: plain %.5f guided %.5f (ratio %.2f)" % (ep, eg, ep / eg), flush=True)
+    assert eg < 0.8 * ep
