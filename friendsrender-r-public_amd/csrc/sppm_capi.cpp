@@ -677,8 +677,8 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
 // the proposal for the next pass.
 static void do_guide(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
-    guide_clear_hist(c->guide, st);
-    launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hist, st);
+    launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hpass, st);
+    guide_fold_hist(c->guide, c->gparams.history_decay, st);
     for (int k = 0; k < c->gparams.train_steps; ++k) {
         guide_infer(c->guide, st);
         guide_gradient(c->guide, st);
@@ -1107,6 +1107,7 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     }
     if (!(p->mix_plain >= 0.25f && p->mix_plain <= 1.0f)) return fail(SPPM_E_ARG, "mix_plain must be in [0.25, 1]");
     if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
+    if (!(p->history_decay >= 0.0f && p->history_decay < 1.0f)) return fail(SPPM_E_ARG, "history_decay must be in [0, 1)");
     if (c->params.record_neighbours) return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0)");
     GuideEmit E{};
     int rc = guide_light_frame(c, E);

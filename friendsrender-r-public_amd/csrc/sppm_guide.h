@@ -52,7 +52,8 @@ struct GuideBuffers {
     float* logits = nullptr;    // per cell
     float* prob = nullptr;      // softmax
     float* cdf = nullptr;       // inclusive prefix sum of prob
-    float* hist = nullptr;      // deposit histogram
+    float* hist = nullptr;      // training histogram (decayed sum of the passes' deposits)
+    float* hpass = nullptr;     // this pass's deposit histogram
     float* part = nullptr;      // per-block partials: GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)
     float* part2 = nullptr;     // per-block (max, sumexp) and scan block sums
     float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss
@@ -71,5 +72,7 @@ void guide_gradient(GuideBuffers& g, hipStream_t st);
 // Adam step with g.grad (skipped on the device when the histogram is empty).
 void guide_adam(GuideBuffers& g, float lr, hipStream_t st);
 void guide_clear_hist(GuideBuffers& g, hipStream_t st);
+// hist = decay * hist + hpass; hpass = 0
+void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st);
 
 }  // namespace sppm

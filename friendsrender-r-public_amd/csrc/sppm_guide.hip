@@ -450,6 +450,13 @@ __global__ void k_guide_adam(float* __restrict__ theta, float* __restrict__ m, f
     theta[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + 1e-8f);
 }
 
+__global__ void k_guide_fold(float* __restrict__ hist, float* __restrict__ hpass, float decay) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= GUIDE_CELLS) return;
+    hist[k] = decay * hist[k] + hpass[k];
+    hpass[k] = 0.0f;
+}
+
 __global__ void k_guide_zero(float* __restrict__ p, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0.0f;
@@ -470,7 +477,7 @@ int guide_alloc(GuideBuffers& g) {
     if (galloc(&g.theta, GUIDE_NPARAMS) || galloc(&g.m, GUIDE_NPARAMS) || galloc(&g.v, GUIDE_NPARAMS) ||
         galloc(&g.grad, GUIDE_NPARAMS) || galloc(&g.wfwd, 4 * GUIDE_WSZ) || galloc(&g.wbwd, 4 * GUIDE_WSZ) ||
         galloc(&g.logits, GUIDE_CELLS) || galloc(&g.prob, GUIDE_CELLS) || galloc(&g.cdf, GUIDE_CELLS) ||
-        galloc(&g.hist, GUIDE_CELLS) || galloc(&g.part, (size_t)GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)) ||
+        galloc(&g.hist, GUIDE_CELLS) || galloc(&g.hpass, GUIDE_CELLS) || galloc(&g.part, (size_t)GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)) ||
         galloc(&g.part2, 2 * 1024) || galloc(&g.scal, 8)) {
         guide_free(g);
         return -4;
@@ -497,7 +504,8 @@ int guide_alloc(GuideBuffers& g) {
 
 void guide_free(GuideBuffers& g) {
     for (void* p : {(void*)g.theta, (void*)g.m, (void*)g.v, (void*)g.grad, (void*)g.wfwd, (void*)g.wbwd,
-                    (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.hist, (void*)g.part, (void*)g.part2,
+                    (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.hist, (void*)g.hpass, (void*)g.part,
+                    (void*)g.part2,
                     (void*)g.scal})
         if (p) (void)hipFree(p);
     g = GuideBuffers();
@@ -540,6 +548,11 @@ void guide_adam(GuideBuffers& g, float lr, hipStream_t st) {
 
 void guide_clear_hist(GuideBuffers& g, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, GUIDE_CELLS);
+    hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hpass, GUIDE_CELLS);
+}
+
+void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_fold, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, g.hpass, decay);
 }
 
 }  // namespace sppm

@@ -399,7 +399,10 @@ def occluded_scene(clutter_freq=61, seed=99):
 
 
 def occluded_camera(width, height, sampler_id=0):
-    """Pinhole camera in room A looking at the shared wall and its door."""
-    return dict(type=CAM_PINHOLE, width=width, height=height, pos=(0.08, 0.5, 0.5), target=(1.0, 0.35, 0.5),
-                fov_deg=60.0, F=1.0, dist_to_lens=1.0, aspect=width / height, sampler=SAMPLER_UNIFORM,
+    """Pinhole camera in room A, next to the shared wall, looking back into the
+    room: every visible surface is lit only through the door, which is out of
+    view.  Rays start on the image plane DistanceToLens behind the camera
+    position (Camera.h:73-82), so it is kept small to stay inside the room."""
+    return dict(type=CAM_PINHOLE, width=width, height=height, pos=(0.95, 0.55, 0.1), target=(0.0, 0.3, 0.8),
+                fov_deg=60.0, F=1.0, dist_to_lens=0.01, aspect=width / height, sampler=SAMPLER_UNIFORM,
                 sampler_id=sampler_id, sampler_nsample=1)

@@ -35,7 +35,7 @@ GUIDE_NPARAMS = 16705
 
 class GuideParams(C.Structure):
     _fields_ = [("enable", C.c_int32), ("mix_plain", C.c_float), ("learning_rate", C.c_float),
-                ("train_steps", C.c_int32), ("seed", C.c_uint32)]
+                ("train_steps", C.c_int32), ("seed", C.c_uint32), ("history_decay", C.c_float)]
 
 
 class Attr(C.Structure):
@@ -269,8 +269,8 @@ class Renderer:
         return v.value
 
     # ---- learned emission proposal --------------------------------------
-    def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1):
-        g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed)
+    def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9):
+        g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay)
         _check(self.lib.sppm_guide_set(self.h, C.byref(g)))
 
     def guide_weights(self, new=None):

@@ -236,6 +236,7 @@ typedef struct sppm_guide_params {
     float learning_rate;   /* Adam step size */
     int32_t train_steps;   /* Adam steps after each pass's gather */
     uint32_t seed;         /* weight initialisation */
+    float history_decay;   /* training histogram = decay * previous + this pass's deposits, in [0, 1) */
 } sppm_guide_params;
 /* Enable (after sppm_scene_upload) or disable guided emission; re-initialises the weights. */
 int sppm_guide_set(sppm_ctx* ctx, const sppm_guide_params* params);
@@ -247,7 +248,7 @@ int sppm_guide_weights(sppm_ctx* ctx, float* out, const float* in);
 int sppm_guide_gradient(sppm_ctx* ctx, const float* hist, float* loss, float* grad);
 /* `steps` Adam steps on a histogram (NULL: the last pass's), then the proposal is rebuilt. */
 int sppm_guide_train(sppm_ctx* ctx, const float* hist, int steps);
-/* Current proposal probabilities per cell and the last deposit histogram (either may be NULL). */
+/* Current proposal probabilities per cell and the training histogram (either may be NULL). */
 int sppm_guide_read(sppm_ctx* ctx, float* prob, float* hist);
 
 /* ---- timing helpers for the benchmark -------------------------------------- */
