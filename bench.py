@@ -43,6 +43,18 @@ def parse():
     ap.add_argument("--workload", choices=["full", "box"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--rmse", action="store_true",
+                    help="time-to-RMSE mode on the occluded-light workload: plain vs guided emission against a "
+                         "long plain reference render")
+    ap.add_argument("--ref-passes", type=int, default=10000)
+    ap.add_argument("--max-passes", type=int, default=2000)
+    ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--target-rmse", type=float, default=0.01)
+    ap.add_argument("--resolution", type=int, default=1024)
+    ap.add_argument("--alpha", type=float, default=1.0,
+                    help="time-to-RMSE mode: alpha = 1 keeps the radius at r0, so guided and plain estimate the same "
+                         "r0-smoothed image and the RMSE measures variance; with alpha < 1 the radius schedule "
+                         "depends on the photons gathered and a finite plain reference is not bias-converged")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the workload, run a device elementwise copy of MIB MiB (float32, torch) three "
                          "times: a kernel of known traffic for scaling the HBM byte counters of a profiler run")
@@ -114,12 +126,96 @@ def copy_calibration(mib):
     del a, b
 
 
+def rmse_mode(args):
+    """Seconds of rendering until the linear image (reconstruct output ^ 1.2,
+    before the reference's 1/1.2 gamma), normalised to the reference's mean,
+    is within RMSE target of a long plain render of the same scene with
+    independent random streams, for plain and for guided emission.  The timed
+    region is the passes only (synchronised chunks of check_every passes,
+    guided training included); the RMSE checks between chunks are not timed."""
+    import numpy as np
+    import scenes
+    import sppm_amd
+    scene = scenes.occluded_scene()
+    res = args.resolution
+    cam = scenes.occluded_camera(res, res)
+    N, alpha, r0 = 1_000_000, args.alpha, 0.02 * scene.diagonal()
+    log(f"occluded-light scene: {scene.nfacets} triangles, {res}x{res}, {N} photons/pass")
+
+    def linear(r, passes):
+        img = r.reconstruct(N * passes, passes).reshape(res, res, 4)[..., :3].astype(np.float64)
+        return np.power(np.maximum(img, 0.0), 1.2)
+
+    # reference: independent random streams (pass steps shifted by 10^6,
+    # another camera sample stream) so the timed renders do not replay it
+    t0 = time.perf_counter()
+    ref_cam = dict(cam, sampler_id=7919)
+    ref_r = sppm_amd.Renderer(scene, ref_cam, alpha, r0, N, bvh_width=8)
+    for p in range(args.ref_passes):
+        step = 1_000_001 + p
+        ref_r.photons(step)
+        ref_r.build()
+        ref_r.eye(step)
+        ref_r.gather()
+        if (p + 1) % 500 == 0:
+            ref_r.sync()
+            log(f"reference: {p + 1}/{args.ref_passes} passes, {time.perf_counter() - t0:.1f} s")
+    ref_r.sync()
+    ref = linear(ref_r, args.ref_passes)
+    ref_r.close()
+    scale = float(ref.mean())  # RMSE is measured on the image normalised to the reference's mean
+    log(f"reference mean linear radiance {scale:.5g}")
+    out = {}
+    for mode in ("plain", "guided"):
+        r = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=8)
+        if mode == "guided":
+            r.guide_set(True, mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=1, history_decay=0.95)
+        el, passes, e = 0.0, 0, float("inf")
+        curve = []
+        while passes < args.max_passes:
+            ts = time.perf_counter()
+            for _ in range(args.check_every):
+                r.pass_(stats=False)
+            r.sync()
+            el += time.perf_counter() - ts
+            passes += args.check_every
+            e = float(np.sqrt(np.mean((linear(r, passes) - ref) ** 2))) / scale
+            curve.append((passes, round(el, 4), e))
+            if passes % (8 * args.check_every) == 0:
+                log(f"{mode}: {passes} passes, {el:.2f} s, RMSE {e:.4g}")
+            if e <= args.target_rmse:
+                break
+        out[mode] = dict(seconds=round(el, 3), passes=passes, rmse=e, reached=e <= args.target_rmse,
+                         ms_per_pass=round(el / passes * 1e3, 3),
+                         guide_ms=round(r.kernel_ms("guide"), 3) if mode == "guided" else 0.0,
+                         curve=curve[:: max(1, len(curve) // 20)])
+        r.close()
+        log(f"{mode}: {out[mode]['seconds']} s, {passes} passes, RMSE {e:.4g}")
+    both = out["plain"]["reached"] and out["guided"]["reached"]
+    line = {
+        "metric": f"seconds to RMSE<={args.target_rmse} (image normalised to the reference's mean) vs "
+                  f"{args.ref_passes}-pass plain render, guided vs plain",
+        "value": out["guided"]["seconds"], "unit": "s", "higher_is_better": False,
+        "speedup_guided_vs_plain": round(out["plain"]["seconds"] / out["guided"]["seconds"], 3) if both else None,
+        "n_gpus": 1, "dtype": "f32 (MLP bf16 MFMA, f32 accumulation)", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "occluded light", "resolution": f"{res}x{res}", "photons_per_pass": N,
+                   "triangles": scene.nfacets, "alpha": alpha, "r0": round(r0, 6), "bvh_width": 8,
+                   "reference_passes": args.ref_passes, "reference_mean_linear_radiance": scale, "mix_plain": 0.5,
+                   "train_steps_per_pass": 1, "learning_rate": 1e-2, "history_decay": 0.95},
+        "plain": out["plain"], "guided": out["guided"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.rmse:
+        rmse_mode(args)
+        return
     import numpy as np  # noqa: F401
     import sppm_amd
 

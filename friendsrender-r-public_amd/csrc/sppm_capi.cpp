@@ -679,8 +679,10 @@ static void do_guide(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hpass, st);
     guide_fold_hist(c->guide, c->gparams.history_decay, st);
+    guide_hist_sum(c->guide, st);
+    // the log-sum-exp of the current weights is the one of the last inference
     for (int k = 0; k < c->gparams.train_steps; ++k) {
-        guide_infer(c->guide, st);
+        if (k > 0) guide_infer(c->guide, st);
         guide_gradient(c->guide, st);
         guide_adam(c->guide, c->gparams.learning_rate, st);
     }
@@ -891,11 +893,11 @@ int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
     return SPPM_OK;
 }
 
-int sppm_reconstruct(sppm_ctx* c, int32_t total_photons, int32_t passes, float* rgba) {
+int sppm_reconstruct(sppm_ctx* c, int64_t total_photons, int32_t passes, float* rgba) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!rgba) return fail(SPPM_E_ARG, "rgba is NULL");
-    launch_reconstruct(c->rb.shared, c->rb.np, total_photons, passes, c->rb.image, c->stream);
+    launch_reconstruct(c->rb.shared, c->rb.np, (long long)total_photons, passes, c->rb.image, c->stream);
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(rgba, c->rb.image, sizeof(float4) * c->rb.np, hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
@@ -1159,6 +1161,7 @@ int sppm_guide_gradient(sppm_ctx* c, const float* hist, float* loss, float* grad
     if (rc) return rc;
     if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
     guide_infer(c->guide, c->stream);
+    guide_hist_sum(c->guide, c->stream);
     guide_gradient(c->guide, c->stream);
     float sc[3];
     HIPOK(hipMemcpyAsync(sc, c->guide.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -1174,6 +1177,7 @@ int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
     if (steps < 0) return fail(SPPM_E_ARG, "steps < 0");
     if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
     const float lr = c->gparams.learning_rate > 0.0f ? c->gparams.learning_rate : 1e-3f;
+    guide_hist_sum(c->guide, c->stream);
     for (int k = 0; k < steps; ++k) {
         guide_infer(c->guide, c->stream);
         guide_gradient(c->guide, c->stream);

@@ -5,9 +5,9 @@
 // A 64 -> 64x4 (ReLU) -> 1 MLP maps the Fourier features of a cell centre of
 // the 32x32 (light u, v) x 16x16 (hemi-octahedral direction s, t) grid to a
 // logit; the proposal is the softmax over the 262,144 cells, uniform inside a
-// cell.  Training minimises -sum_c W_c log P_c over the histogram W of the
-// luminance that the photons emitted from each cell deposited on visible hit
-// points.  Forward and backward run on bf16 MFMA (16x16x32) with float32
+// cell.  Training minimises -sum_c W_c log P_c over the histogram W of what
+// the photons emitted from each cell deposited on visible hit points (hit
+// points gathering a photon x its luminance / PDF), decayed over passes.  Forward and backward run on bf16 MFMA (16x16x32) with float32
 // accumulation; Adam keeps float32 master weights.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -55,7 +55,7 @@ struct GuideBuffers {
     float* hist = nullptr;      // training histogram (decayed sum of the passes' deposits)
     float* hpass = nullptr;     // this pass's deposit histogram
     float* part = nullptr;      // per-block partials: GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)
-    float* part2 = nullptr;     // per-block (max, sumexp) and scan block sums
+    float* part2 = nullptr;     // per-block (max, sumexp), scan block sums, histogram partials
     float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss
     int steps = 0;              // Adam steps taken
 };
@@ -66,8 +66,11 @@ void guide_free(GuideBuffers& g);
 int guide_upload_params(GuideBuffers& g, const float* theta, hipStream_t st);
 // Forward over all cells: logits, log-sum-exp, probabilities and their CDF.
 void guide_infer(GuideBuffers& g, hipStream_t st);
-// Gradient of the loss for the histogram in g.hist (needs guide_infer of the
-// same weights first for the log-sum-exp); writes g.grad and the loss.
+// Sum of the training histogram (scal[1]).
+void guide_hist_sum(GuideBuffers& g, hipStream_t st);
+// Gradient of the loss for the histogram in g.hist (needs the log-sum-exp of
+// the same weights from guide_infer and the histogram sum); writes g.grad and
+// the loss.
 void guide_gradient(GuideBuffers& g, hipStream_t st);
 // Adam step with g.grad (skipped on the device when the histogram is empty).
 void guide_adam(GuideBuffers& g, float lr, hipStream_t st);

@@ -198,9 +198,33 @@ __global__ void __launch_bounds__(64) k_guide_forward(const uint16_t* __restrict
     }
 }
 
-// log-sum-exp over the block partials; sum of the histogram
-__global__ void k_guide_lse(const float* __restrict__ part2, int nparts, const float* __restrict__ hist,
-                            float* __restrict__ scal) {
+// sum of the histogram: per-block partials (256 blocks x 1024 cells) ...
+__global__ void __launch_bounds__(256) k_guide_hsum(const float* __restrict__ hist, double* __restrict__ hpart) {
+    __shared__ double red[256];
+    const int t = threadIdx.x, base = blockIdx.x * 1024 + 4 * t;
+    red[t] = (double)hist[base] + (double)hist[base + 1] + (double)hist[base + 2] + (double)hist[base + 3];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) hpart[blockIdx.x] = red[0];
+}
+// ... and their total (fixed order) into scal[1]
+__global__ void __launch_bounds__(256) k_guide_hsum_total(const double* __restrict__ hpart, float* __restrict__ scal) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    red[t] = hpart[t];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) scal[1] = (float)red[0];
+}
+
+// log-sum-exp over the block partials of k_guide_forward
+__global__ void k_guide_lse(const float* __restrict__ part2, int nparts, float* __restrict__ scal) {
     __shared__ double red[256];
     __shared__ float mx[256];
     const int t = threadIdx.x;
@@ -222,16 +246,6 @@ __global__ void k_guide_lse(const float* __restrict__ part2, int nparts, const f
         __syncthreads();
     }
     if (t == 0) scal[0] = (float)((double)M + log(red[0]));
-    __syncthreads();
-    double h = 0.0;
-    for (int c = t; c < GUIDE_CELLS; c += 256) h += (double)hist[c];
-    red[t] = h;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) red[t] += red[t + o];
-        __syncthreads();
-    }
-    if (t == 0) scal[1] = (float)red[0];
 }
 
 // probabilities and their inclusive prefix sum (per block of 1024, then offsets)
@@ -478,7 +492,7 @@ int guide_alloc(GuideBuffers& g) {
         galloc(&g.grad, GUIDE_NPARAMS) || galloc(&g.wfwd, 4 * GUIDE_WSZ) || galloc(&g.wbwd, 4 * GUIDE_WSZ) ||
         galloc(&g.logits, GUIDE_CELLS) || galloc(&g.prob, GUIDE_CELLS) || galloc(&g.cdf, GUIDE_CELLS) ||
         galloc(&g.hist, GUIDE_CELLS) || galloc(&g.hpass, GUIDE_CELLS) || galloc(&g.part, (size_t)GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)) ||
-        galloc(&g.part2, 2 * 1024) || galloc(&g.scal, 8)) {
+        galloc(&g.part2, 4 * 1024) || galloc(&g.scal, 8)) {
         guide_free(g);
         return -4;
     }
@@ -524,10 +538,16 @@ int guide_upload_params(GuideBuffers& g, const float* theta, hipStream_t st) {
 void guide_infer(GuideBuffers& g, hipStream_t st) {
     const int nb = 1024;
     hipLaunchKernelGGL(k_guide_forward, dim3(nb), dim3(64), 0, st, g_features, g.wfwd, g.theta, g.logits, g.part2);
-    hipLaunchKernelGGL(k_guide_lse, dim3(1), dim3(256), 0, st, g.part2, nb, g.hist, g.scal);
+    hipLaunchKernelGGL(k_guide_lse, dim3(1), dim3(256), 0, st, g.part2, nb, g.scal);
     hipLaunchKernelGGL(k_guide_prob, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.logits, g.scal, g.prob, g.cdf,
                        g.part2);
     hipLaunchKernelGGL(k_guide_cdf_offsets, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.cdf, g.part2);
+}
+
+void guide_hist_sum(GuideBuffers& g, hipStream_t st) {
+    double* hpart = reinterpret_cast<double*>(g.part2 + 2048);  // after the forward partials (2 * 1024 floats)
+    hipLaunchKernelGGL(k_guide_hsum, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.hist, hpart);
+    hipLaunchKernelGGL(k_guide_hsum_total, dim3(1), dim3(256), 0, st, hpart, g.scal);
 }
 
 void guide_gradient(GuideBuffers& g, hipStream_t st) {

@@ -70,7 +70,7 @@ int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();
 void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st);
-void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st);
+void launch_reconstruct(const SharedDev& SH, int np, long long total, int passes, float4* out, hipStream_t st);
 void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st);
 void launch_init_slots(const SlotsDev& s, hipStream_t st);
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st);

@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       float4* __restrict__ partial, float* __restrict__ pdep) {
     __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
     __shared__ unsigned char s_hit[64 * 64];
-    __shared__ float s_dep[64];  // luminance each staged photon deposited on this group (guided emission)
+    __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
     const int lane = threadIdx.x;
     const int total = chunk_off[ntiles];
     const int CH = *chunkp;
@@ -1026,7 +1026,18 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                 __syncthreads();
                 const int n = (int)((qa + s1 - base) < 64 ? (qa + s1 - base) : 64);
                 int nh = 0;
-                if (mine) {
+                if (pdep) {
+                    // also count, per staged photon, the hit points that gather it
+                    for (int j = 0; j < n; ++j) {
+                        const bool hit = mine && in_query(L, s_pos[j]);
+                        if (hit) {
+                            s_hit[nh * 64 + lane] = (unsigned char)j;
+                            ++nh;
+                        }
+                        const unsigned long long m = __ballot(hit);
+                        if (lane == 0) s_cnt[j] = __popcll(m);
+                    }
+                } else if (mine) {
                     for (int j = 0; j < n; ++j) {
                         if (in_query(L, s_pos[j])) {
                             s_hit[nh * 64 + lane] = (unsigned char)j;
@@ -1035,10 +1046,6 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     }
                 }
                 L.M += nh;
-                if (pdep) {
-                    s_dep[lane] = 0.0f;
-                    __syncthreads();
-                }
                 for (int e = 0; e < nh; ++e) {
                     const int j = s_hit[e * 64 + lane];
                     const float4 pp = s_pos[j], pf = s_flux[j], pd = s_dir[j];
@@ -1047,16 +1054,16 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
                     FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
                     const float b = surf_brdf(L.kind, fp);
-                    const f3 c = ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
-                    L.add = L.add + c;
-                    if (pdep) {
-                        const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
-                        if (lum > 0.0f && lum < 3.0e38f) atomicAdd(&s_dep[j], lum);
-                    }
+                    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
                 }
                 if (pdep) {
+                    // training signal of the emission proposal: hit points reached x photon power
                     __syncthreads();
-                    if (lane < n && s_dep[lane] > 0.0f) atomicAdd(&pdep[__float_as_int(s_dir[lane].w)], s_dep[lane]);
+                    if (lane < n && s_cnt[lane] > 0) {
+                        const float4 pf = s_flux[lane];
+                        const float d = (float)s_cnt[lane] * (0.2126f * pf.x + 0.7152f * pf.y + 0.0722f * pf.z) / pf.w;
+                        if (d > 0.0f && d < 3.0e38f) atomicAdd(&pdep[__float_as_int(s_dir[lane].w)], d);
+                    }
                 }
                 __syncthreads();
             }
@@ -1121,7 +1128,7 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
 }
 
 // reconstructImageFromSharedPoints (SPPM.cpp:473-493)
-__global__ void k_reconstruct(SharedDev SH, int np, int total_photons, int passes, float4* __restrict__ out) {
+__global__ void k_reconstruct(SharedDev SH, int np, long long total_photons, int passes, float4* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     float4 sr = SH.rad[i];
@@ -1281,7 +1288,7 @@ int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); 
 int gather_group_count(int np) { return (np + 63) / 64; }
 size_t gather_tile_bytes() { return sizeof(TileRange); }
 
-void launch_reconstruct(const SharedDev& SH, int np, int total, int passes, float4* out, hipStream_t st) {
+void launch_reconstruct(const SharedDev& SH, int np, long long total, int passes, float4* out, hipStream_t st) {
     hipLaunchKernelGGL(k_reconstruct, dim3((np + 255) / 256), dim3(256), 0, st, SH, np, total, passes, out);
 }
 void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st) {
@@ -1306,7 +1313,7 @@ void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st) {
 
 namespace sppm {
 // ---- guided emission: training records ----------------------------------
-// Deposited luminance per flattened photon -> per path -> the histogram of
+// Deposit per flattened photon -> per path -> the histogram of
 // the paths' emission cells (float atomics: the summation order, and with it
 // the last bits of the histogram, vary from run to run).
 __global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n) {

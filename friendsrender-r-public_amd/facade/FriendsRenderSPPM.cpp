@@ -527,13 +527,12 @@ void StochasticProgressivePhotonMapping::prepare(const Scene& scene, const iCame
 
 void StochasticProgressivePhotonMapping::finish(Texture2D& renderTarget, int64_t totalPhotons, int passes) {
     Impl& I = *m_Impl;
-    if (totalPhotons > INT32_MAX) throw std::overflow_error("total photon count exceeds the reference's int");
     int64_t hp, nb, st, ps;
     check(sppm_counters(I.ctx, &hp, &nb, &st, nullptr, &ps), "sppm_counters");
     m_Stats.hit_points = hp;
     m_Stats.neighbours = nb;
     m_Stats.photons_stored = st;
-    check(sppm_reconstruct(I.ctx, (int32_t)totalPhotons, passes, renderTarget.Data()), "sppm_reconstruct");
+    check(sppm_reconstruct(I.ctx, totalPhotons, passes, renderTarget.Data()), "sppm_reconstruct");
 }
 
 static void check_target(const Texture2D& rt, const iCamera& cam) {

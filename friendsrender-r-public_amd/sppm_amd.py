@@ -106,7 +106,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_pass_eye.argtypes = [C.c_void_p, C.c_int]
     lib.sppm_pass_gather.argtypes = [C.c_void_p]
     lib.sppm_sync.argtypes = [C.c_void_p]
-    lib.sppm_reconstruct.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+    lib.sppm_reconstruct.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
     lib.sppm_read_photons.argtypes = [C.c_void_p, C.c_int32] + [C.POINTER(C.c_float)] * 5 + [C.POINTER(C.c_int32)]
     lib.sppm_read_hitpoints.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 4
     lib.sppm_read_shared.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int32),

@@ -169,8 +169,10 @@ int sppm_pass_gather(sppm_ctx* ctx);
 int sppm_sync(sppm_ctx* ctx);
 
 /* reconstructImageFromSharedPoints: rgba = W*H*4 floats. total_photons and
- * passes as the reference passes TotalPhoton and numstep. */
-int sppm_reconstruct(sppm_ctx* ctx, int32_t total_photons, int32_t passes, float* rgba);
+ * passes as the reference passes TotalPhoton and numstep; total_photons is
+ * 64-bit here (the reference's int overflows beyond 2^31 emitted photons)
+ * and is converted to float as the reference converts its int. */
+int sppm_reconstruct(sppm_ctx* ctx, int64_t total_photons, int32_t passes, float* rgba);
 
 /* ---- debug readback (parity tests) ---------------------------------------- */
 /* Flattened photon list of the last build, in reference order:
@@ -226,7 +228,9 @@ int sppm_scene_stats(sppm_ctx* ctx, sppm_scene_info* info);
  * consistent.  The proposal is a softmax over 32x32 light (u, v) cells x 16x16
  * hemi-octahedral direction cells whose logits come from a 64->64x4->1 MLP of
  * Fourier features of the cell centre, trained on the GPU (bf16 MFMA, Adam)
- * from the luminance the photons of each cell deposit on visible hit points.
+ * from the photons' deposits on visible hit points: per cell, the sum over the
+ * photons emitted from it of (hit points that gather the photon) x (photon
+ * luminance / PDF).
  * Scenes: exactly one untextured rectangular light made of two facets. */
 #define SPPM_GUIDE_CELLS 262144
 #define SPPM_GUIDE_NPARAMS 16705
