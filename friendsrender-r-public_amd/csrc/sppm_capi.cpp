@@ -126,6 +126,7 @@ struct RenderBuffers {
     int* tile_scan_tmp = nullptr;
     float4* partial = nullptr;
     GroupBuffers groups{};
+    int* pass_dev = nullptr;   // passes done (device copy of the pass counter, read by graph replays)
     int* path_cell = nullptr;  // guided emission: emission cell per path
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
     uint32_t* flags = nullptr;
@@ -138,7 +139,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -203,6 +204,10 @@ struct sppm_ctx {
     // pass ahead into two buffers: block k (the words of the k-th eye pass)
     // lands in buffer k % 2 and is overwritten only after eye pass k has run.
     hipEvent_t ev_words[2] = {nullptr, nullptr}, ev_eye[2] = {nullptr, nullptr};
+    hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;  // camera-word branch inside a captured pass
+    hipGraph_t graph = nullptr;                          // captured pass (sppm_graph_capture)
+    hipGraphExec_t graph_exec = nullptr;
+    int tuned_blocks = 0;                                // path-kernel grid chosen by the autotuner (0: default)
     bool eye_recorded[2] = {false, false};
     long long words_gen = 0, words_used = 0;
     SceneBuffers scene;
@@ -248,6 +253,8 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
         HIPOK(hipEventCreateWithFlags(&c->ev_words[b], hipEventDisableTiming));
         HIPOK(hipEventCreateWithFlags(&c->ev_eye[b], hipEventDisableTiming));
     }
+    HIPOK(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming));
+    HIPOK(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming));
     c->params.alpha = 0.9f;
     c->params.first_radius = 0.015f;
     c->params.photons_per_pass = 0;
@@ -269,6 +276,10 @@ int sppm_destroy(sppm_ctx* c) {
         (void)hipEventDestroy(c->ev_words[b]);
         (void)hipEventDestroy(c->ev_eye[b]);
     }
+    (void)hipEventDestroy(c->ev_gfork);
+    (void)hipEventDestroy(c->ev_gjoin);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipStreamDestroy(c->stream2);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -569,6 +580,10 @@ int sppm_begin(sppm_ctx* c) {
     HIPOK(hipStreamSynchronize(c->stream2));
     c->words_gen = c->words_used = 0;
     c->eye_recorded[0] = c->eye_recorded[1] = false;
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
     RenderBuffers& R = c->rb;
     R.release();
     const int N = c->params.photons_per_pass;
@@ -599,7 +614,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
-        (rc = dalloc(&R.groups.nvalid, 1)) ||
+        (rc = dalloc(&R.groups.nvalid, 1)) || (rc = dalloc(&R.pass_dev, 1)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
@@ -614,6 +629,7 @@ int sppm_begin(sppm_ctx* c) {
     }
     HIPOK(hipMemsetAsync(R.counts, 0, sizeof(int) * (size_t)(np + 1), c->stream));
     launch_init_slots(R.slots, c->stream);
+    launch_zero_i32(R.pass_dev, 1, c->stream);
     launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
     // UniformSample(ID): mt19937(ID) state, position 624 (first draw twists)
     if (c->cam.sampler == SPPM_SAMPLER_UNIFORM) {
@@ -669,8 +685,14 @@ static GridDev grid_dev(sppm_ctx* c) {
 }
 
 // ---- split steps --------------------------------------------------------
+// traversal spill area and the path-kernel grid (the autotuner may halve it)
+static TraceSpill trace_spill(const sppm_ctx* c) {
+    TraceSpill sp = c->scene.trace_spill();
+    if (c->tuned_blocks > 0 && c->tuned_blocks < sp.blocks) sp.blocks = c->tuned_blocks;
+    return sp;
+}
 static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
-    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, c->scene.trace_spill(), c->scene.next_path,
+    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, trace_spill(c), c->scene.next_path,
                   c->guided ? &c->emit : nullptr, c->rb.flags, st);
 }
 // Training records of the pass (deposit histogram), Adam steps on it, and
@@ -720,7 +742,7 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
         HIPOK(hipStreamWaitEvent(st, c->ev_words[b], 0));
     }
     if (t) t->start(st);
-    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, c->scene.trace_spill(),
+    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill(c),
                c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, st);
     if (t) t->stop(st);
     if (uni) {
@@ -764,6 +786,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
 int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     do_photons(c, step, c->stream);
     HIPOK(hipGetLastError());
     if (emitted) *emitted = c->rb.N;
@@ -772,6 +795,7 @@ int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
 int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     do_build(c, c->stream);
     HIPOK(hipGetLastError());
     if (stored) {
@@ -783,6 +807,7 @@ int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
 int sppm_pass_eye(sppm_ctx* c, int step) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     rc = do_eye(c, step, c->stream, nullptr);
     if (rc) return rc;
     HIPOK(hipGetLastError());
@@ -791,6 +816,7 @@ int sppm_pass_eye(sppm_ctx* c, int step) {
 int sppm_pass_gather(sppm_ctx* c) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     rc = do_gather(c, c->stream);
     if (rc) return rc;
     if (c->guided) do_guide(c, c->stream);
@@ -799,11 +825,42 @@ int sppm_pass_gather(sppm_ctx* c) {
 }
 
 
+// Per-pass statistics (synchronises the stream).
+static int read_pass_stats(sppm_ctx* c, sppm_pass_stats* stats) {
+    hipStream_t st = c->stream;
+    memset(stats, 0, sizeof *stats);
+    int stored = 0;
+    uint32_t flags = 0;
+    unsigned long long rs[2] = {0, 0};
+    HIPOK(hipMemcpyAsync(&stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(&flags, c->rb.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(rs, c->rb.rstat, sizeof(rs), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->resolve();
+    float r = rs[1] ? (float)(((double)rs[0] / 4294967296.0) / (double)rs[1]) : 0.0f;
+    stats->photons_emitted = c->rb.N;
+    stats->photons_stored = stored;
+    stats->mean_radius = r;
+    stats->hit_points = (int32_t)rs[1];
+    stats->flags = (int)flags;
+    stats->ms_photons = (float)c->t_photons.avg();
+    stats->ms_build = (float)c->t_build.avg();
+    stats->ms_eye = (float)c->t_eye.avg();
+    stats->ms_gather = (float)c->t_gather.avg();
+    if (c->params.check_numeric && (flags & 1u)) return fail(SPPM_E_NUMERIC, "NaN/Inf radiance after pass %d", c->pass);
+    return SPPM_OK;
+}
+
 int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     int rc = check_ready(c);
     if (rc) return rc;
     const int step = c->pass + 1;
     hipStream_t st = c->stream;
+    if (c->graph_exec) {
+        HIPOK(hipGraphLaunch(c->graph_exec, st));
+        c->pass += 1;
+        return stats ? read_pass_stats(c, stats) : SPPM_OK;
+    }
     // camera words of this pass (first pass only) and of the next one, on the
     // side stream: they overlap this pass's kernels
     rc = words_generate_upto(c, c->words_used + 1);
@@ -825,30 +882,234 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         do_guide(c, st);
         c->t_guide.stop(st);
     }
+    launch_add_i32(c->rb.pass_dev, 1, st);
     HIPOK(hipGetLastError());
     c->pass += 1;
-    if (stats) {
-        memset(stats, 0, sizeof *stats);
-        int stored = 0;
-        uint32_t flags = 0;
-        unsigned long long rs[2] = {0, 0};
-        HIPOK(hipMemcpyAsync(&stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPOK(hipMemcpyAsync(&flags, c->rb.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPOK(hipMemcpyAsync(rs, c->rb.rstat, sizeof(rs), hipMemcpyDeviceToHost, st));
-        HIPOK(hipStreamSynchronize(st));
-        for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather}) t->resolve();
-        float r = rs[1] ? (float)(((double)rs[0] / 4294967296.0) / (double)rs[1]) : 0.0f;
-        stats->photons_emitted = c->rb.N;
-        stats->photons_stored = stored;
-        stats->mean_radius = r;
-        stats->hit_points = (int32_t)rs[1];
-        stats->flags = (int)flags;
-        stats->ms_photons = (float)c->t_photons.avg();
-        stats->ms_build = (float)c->t_build.avg();
-        stats->ms_eye = (float)c->t_eye.avg();
-        stats->ms_gather = (float)c->t_gather.avg();
-        if (c->params.check_numeric && (flags & 1u)) return fail(SPPM_E_NUMERIC, "NaN/Inf radiance after pass %d", c->pass);
+    return stats ? read_pass_stats(c, stats) : SPPM_OK;
+}
+
+// ---- graph-captured passes -------------------------------------------------
+// One pass as a capturable sequence: the pass's camera words on st2 (forked
+// from st) while the photon pass and the build run on st, then eye, gather,
+// guided training and the device pass counter.  Every size and step is read
+// on the device, so a replay continues where the previous one ended; memory
+// is cleared by kernels, not by memset nodes.
+static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
+    RenderBuffers& R = c->rb;
+    HIPOK(hipEventRecord(c->ev_gfork, st));
+    HIPOK(hipStreamWaitEvent(st2, c->ev_gfork, 0));
+    launch_camera_words(R.cam_mt, R.cam_pos, R.words, 6LL * R.np, st2);
+    HIPOK(hipEventRecord(c->ev_gjoin, st2));
+    launch_photon(c->scene.dev, R.slots, R.N, 0, R.pass_dev, trace_spill(c), c->scene.next_path,
+                  c->guided ? &c->emit : nullptr, R.flags, st);
+    do_build(c, st);
+    HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
+    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, st);
+    int rc = do_gather(c, st);
+    if (rc) return rc;
+    if (c->guided) do_guide(c, st);
+    launch_add_i32(R.pass_dev, 1, st);
+    return SPPM_OK;
+}
+
+static int graph_ok(sppm_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->pass != 0 || c->words_gen != 0)
+        return fail(SPPM_E_STATE, "capture a pass right after sppm_begin (pass %d done)", c->pass);
+    if (c->cam.sampler != SPPM_SAMPLER_UNIFORM)
+        return fail(SPPM_E_STATE, "graph-captured passes need the UniformSample camera stream");
+    if (c->params.record_neighbours) return fail(SPPM_E_STATE, "graph-captured passes need record_neighbours 0");
+    return SPPM_OK;
+}
+
+int sppm_graph_capture(sppm_ctx* c) {
+    int rc = graph_ok(c);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_pass_graph(c, c->stream, c->stream2);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
     }
+    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
+    }
+    c->graph = g;
+    c->graph_exec = x;
+    return SPPM_OK;
+}
+
+}  // extern "C"
+
+// Several contexts' passes in one graph: one branch per context (two streams
+// each), forked from and joined into the batch's origin stream.
+struct sppm_batch {
+    int device = 0;
+    std::vector<sppm_ctx*> ctx;
+    std::vector<hipStream_t> s1, s2;
+    std::vector<hipEvent_t> join;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+// ---- autotuner ------------------------------------------------------------------
+// Times `passes` passes (after one warm-up pass) for each BVH width {2, 4, 8}
+// x path-kernel grid {device-filling, half}, keeps the fastest, and leaves the
+// context re-begun with it.  The closest hit is width-independent, so the
+// choice changes time only.
+int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tune_result* out) {
+    if (!c || !scene || !out) return fail(SPPM_E_ARG, "NULL argument");
+    if (passes < 1) return fail(SPPM_E_ARG, "passes < 1");
+    if (!c->have_cam || !c->have_params) return fail(SPPM_E_STATE, "camera and params must be set before tuning");
+    memset(out, 0, sizeof *out);
+    const int widths[3] = {2, 4, 8};
+    float best = 3.4e38f;
+    int bw = 8, bb = 0, k = 0, rc;
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    for (int w : widths) {
+        if ((rc = sppm_scene_upload(c, scene, w))) break;
+        const int full = c->scene.spill_blocks;
+        for (int half = 0; half < 2; ++half, ++k) {
+            c->tuned_blocks = half ? full / 2 : 0;
+            if ((rc = sppm_begin(c)) || (rc = sppm_pass(c, nullptr))) break;
+            HIPOK(hipEventRecord(e0, c->stream));
+            for (int p = 0; p < passes && !rc; ++p) rc = sppm_pass(c, nullptr);
+            if (rc) break;
+            HIPOK(hipEventRecord(e1, c->stream));
+            HIPOK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPOK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= (float)passes;
+            out->trial_ms[k] = ms;
+            if (ms < best) {
+                best = ms;
+                bw = w;
+                bb = c->tuned_blocks;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = sppm_scene_upload(c, scene, bw))) return rc;
+    c->tuned_blocks = bb;
+    c->params.bvh_width = bw;
+    if ((rc = sppm_begin(c))) return rc;
+    out->bvh_width = bw;
+    out->grid_blocks = bb ? bb : c->scene.spill_blocks;
+    out->ms_per_pass = best;
+    return SPPM_OK;
+}
+
+int sppm_batch_create(sppm_batch** out, int device) {
+    if (!out) return fail(SPPM_E_ARG, "out is NULL");
+    HIPOK(hipSetDevice(device));
+    sppm_batch* b = new sppm_batch();
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(SPPM_E_HIP, "batch stream / event creation failed");
+    }
+    *out = b;
+    return SPPM_OK;
+}
+
+int sppm_batch_add(sppm_batch* b, sppm_ctx* c) {
+    if (!b || !c) return fail(SPPM_E_ARG, "NULL argument");
+    if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
+    if (c->device != b->device) return fail(SPPM_E_ARG, "context is on device %d, batch on %d", c->device, b->device);
+    int rc = graph_ok(c);
+    if (rc) return rc;
+    hipStream_t x = nullptr, y = nullptr;
+    hipEvent_t j = nullptr;
+    HIPOK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    HIPOK(hipStreamCreateWithFlags(&y, hipStreamNonBlocking));
+    HIPOK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+    b->ctx.push_back(c);
+    b->s1.push_back(x);
+    b->s2.push_back(y);
+    b->join.push_back(j);
+    return SPPM_OK;
+}
+
+int sppm_batch_capture(sppm_batch* b) {
+    if (!b || b->ctx.empty()) return fail(SPPM_E_ARG, "empty batch");
+    if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
+    HIPOK(hipSetDevice(b->device));
+    for (sppm_ctx* c : b->ctx) {
+        int rc = graph_ok(c);
+        if (rc) return rc;
+        HIPOK(hipStreamSynchronize(c->stream));
+    }
+    HIPOK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+    int rc = SPPM_OK;
+    if (hipEventRecord(b->fork, b->origin) != hipSuccess) rc = fail(SPPM_E_HIP, "fork record");
+    for (size_t k = 0; k < b->ctx.size() && !rc; ++k) {
+        if (hipStreamWaitEvent(b->s1[k], b->fork, 0) != hipSuccess) { rc = fail(SPPM_E_HIP, "fork wait"); break; }
+        rc = enqueue_pass_graph(b->ctx[k], b->s1[k], b->s2[k]);
+        if (rc) break;
+        if (hipEventRecord(b->join[k], b->s1[k]) != hipSuccess ||
+            hipStreamWaitEvent(b->origin, b->join[k], 0) != hipSuccess)
+            rc = fail(SPPM_E_HIP, "join");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->origin, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
+    }
+    b->graph = g;
+    return SPPM_OK;
+}
+
+int sppm_batch_pass(sppm_batch* b) {
+    if (!b) return fail(SPPM_E_ARG, "NULL batch");
+    if (!b->exec) return fail(SPPM_E_STATE, "batch not captured");
+    HIPOK(hipGraphLaunch(b->exec, b->origin));
+    for (sppm_ctx* c : b->ctx) c->pass += 1;
+    return SPPM_OK;
+}
+
+int sppm_batch_sync(sppm_batch* b) {
+    if (!b) return fail(SPPM_E_ARG, "NULL batch");
+    HIPOK(hipStreamSynchronize(b->origin));
+    return SPPM_OK;
+}
+
+int sppm_batch_destroy(sppm_batch* b) {
+    if (!b) return SPPM_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->origin);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        (void)hipStreamDestroy(b->s1[k]);
+        (void)hipStreamDestroy(b->s2[k]);
+        (void)hipEventDestroy(b->join[k]);
+    }
+    (void)hipEventDestroy(b->fork);
+    (void)hipStreamDestroy(b->origin);
+    delete b;
     return SPPM_OK;
 }
 

@@ -56,8 +56,7 @@ struct GuideBuffers {
     float* hpass = nullptr;     // this pass's deposit histogram
     float* part = nullptr;      // per-block partials: GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)
     float* part2 = nullptr;     // per-block (max, sumexp), scan block sums, histogram partials
-    float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss
-    int steps = 0;              // Adam steps taken
+    float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss, [3] Adam steps taken
 };
 
 int guide_alloc(GuideBuffers& g);

@@ -451,11 +451,13 @@ __global__ void k_guide_bf16(const float* __restrict__ theta, uint16_t* __restri
 }
 
 // Adam (float32 master weights); skipped when the histogram is empty.
+// Adam step count t + 1 = scal[3] + 1 (kept on the device so graph replays count).
 __global__ void k_guide_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
-                             const float* __restrict__ grad, const float* __restrict__ scal, float lr, float c1,
-                             float c2) {
+                             const float* __restrict__ grad, const float* __restrict__ scal, float lr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= GUIDE_NPARAMS || !(scal[1] > 0.0f)) return;
+    const float t = scal[3] + 1.0f;
+    const float c1 = 1.0f / (1.0f - powf(0.9f, t)), c2 = 1.0f / (1.0f - powf(0.999f, t));
     const float g = grad[k];
     const float mk = 0.9f * m[k] + 0.1f * g;
     const float vk = 0.999f * v[k] + 0.001f * g * g;
@@ -469,6 +471,10 @@ __global__ void k_guide_fold(float* __restrict__ hist, float* __restrict__ hpass
     if (k >= GUIDE_CELLS) return;
     hist[k] = decay * hist[k] + hpass[k];
     hpass[k] = 0.0f;
+}
+
+__global__ void k_guide_tick(float* __restrict__ scal) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && scal[1] > 0.0f) scal[3] += 1.0f;
 }
 
 __global__ void k_guide_zero(float* __restrict__ p, int n) {
@@ -531,7 +537,7 @@ int guide_upload_params(GuideBuffers& g, const float* theta, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_zero, dim3((GUIDE_NPARAMS + 255) / 256), dim3(256), 0, st, g.m, GUIDE_NPARAMS);
     hipLaunchKernelGGL(k_guide_zero, dim3((GUIDE_NPARAMS + 255) / 256), dim3(256), 0, st, g.v, GUIDE_NPARAMS);
     hipLaunchKernelGGL(k_guide_bf16, dim3(4 * GUIDE_WSZ / 256), dim3(256), 0, st, g.theta, g.wfwd, g.wbwd);
-    g.steps = 0;
+    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g.scal, 8);  // lse, sums, loss, Adam step count
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -558,11 +564,9 @@ void guide_gradient(GuideBuffers& g, hipStream_t st) {
 }
 
 void guide_adam(GuideBuffers& g, float lr, hipStream_t st) {
-    g.steps += 1;
-    const float c1 = (float)(1.0 / (1.0 - pow(0.9, g.steps)));
-    const float c2 = (float)(1.0 / (1.0 - pow(0.999, g.steps)));
     hipLaunchKernelGGL(k_guide_adam, dim3((GUIDE_NPARAMS + 255) / 256), dim3(256), 0, st, g.theta, g.m, g.v, g.grad,
-                       g.scal, lr, c1, c2);
+                       g.scal, lr);
+    hipLaunchKernelGGL(k_guide_tick, dim3(1), dim3(64), 0, st, g.scal);
     hipLaunchKernelGGL(k_guide_bf16, dim3(4 * GUIDE_WSZ / 256), dim3(256), 0, st, g.theta, g.wfwd, g.wbwd);
 }
 

@@ -37,13 +37,16 @@ struct TraceSpill {
 };
 int trace_grid_blocks(int width);
 struct GuideEmit;
-void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
-                   const GuideEmit* guide, uint32_t* flags, hipStream_t st);
+// step: the pass step (SPPM.cpp:798), or, when step_dev is given, *step_dev + 1
+void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
+                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags, hipStream_t st);
+void launch_add_i32(int* p, int v, hipStream_t st);
+void launch_zero_i32(int* p, int n, hipStream_t st);
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
 void launch_guide_deposit(const int* count, const int* off, const float* pdep, const int* path_cell, int N, float* hist,
                           hipStream_t st);
-void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
-                const uint32_t* words, uint32_t* flags, hipStream_t st);
+void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
                   int* facet, float* tuv, uint32_t* flags, hipStream_t st);
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,

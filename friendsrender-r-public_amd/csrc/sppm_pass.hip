@@ -332,9 +332,12 @@ __device__ __forceinline__ void photon_finish(const SlotsDev& slots, const Photo
 // exits once the counter has passed N and its lanes are done.
 constexpr int PHOTON_REGEN = 16;
 template <int W, bool GUIDED>
-__global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step, int cap,
+__global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
+                                                       const int* __restrict__ step_dev, int cap,
                                                        uint32_t* __restrict__ spill, int* __restrict__ next_path,
                                                        GuideEmit G, uint32_t* __restrict__ flags) {
+    // the pass step comes from the device pass counter when the pass is replayed from a graph
+    const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
@@ -621,9 +624,11 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
 
 // Eye pass: a fixed grid of waves strides over the pixels.
 template <int W>
-__global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev H, int step, int cap,
+__global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
+                                                    const int* __restrict__ step_dev, int cap,
                                                     uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
                                                     uint32_t* __restrict__ flags) {
+    const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
@@ -1198,13 +1203,14 @@ void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long lon
     hipLaunchKernelGGL(k_camera_words, dim3(1), dim3(CAMW_THREADS), 0, st, st_mt, st_pos, words, need);
 }
 
-void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const TraceSpill& sp, int* next_path,
-                   const GuideEmit* guide, uint32_t* flags, hipStream_t st) {
+void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
+                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, next_path, 1);
     dim3 g(sp.blocks), b(PATH_BLOCK);
     const GuideEmit G = guide ? *guide : GuideEmit{};
 #define SPPM_PHOTON(WW, GG) \
-    hipLaunchKernelGGL((k_photon<WW, GG>), g, b, 0, st, S, slots, N, step, sp.cap, sp.buf, next_path, G, flags)
+    hipLaunchKernelGGL((k_photon<WW, GG>), g, b, 0, st, S, slots, N, step, step_dev, sp.cap, sp.buf, next_path, G, \
+                       flags)
     if (guide) {
         if (S.width == 2) SPPM_PHOTON(2, true);
         else if (S.width == 4) SPPM_PHOTON(4, true);
@@ -1217,12 +1223,20 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
 #undef SPPM_PHOTON
 }
 
-void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const TraceSpill& sp,
-                const uint32_t* words, uint32_t* flags, hipStream_t st) {
+void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, hipStream_t st) {
     dim3 g(sp.blocks), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_eye<2>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_eye<4>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
-    else hipLaunchKernelGGL(k_eye<8>, g, b, 0, st, S, C, H, step, sp.cap, sp.buf, words, flags);
+    if (S.width == 2) hipLaunchKernelGGL(k_eye<2>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
+    else if (S.width == 4) hipLaunchKernelGGL(k_eye<4>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
+    else hipLaunchKernelGGL(k_eye<8>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
+}
+
+__global__ void k_add_i32(int* __restrict__ p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+void launch_add_i32(int* p, int v, hipStream_t st) { hipLaunchKernelGGL(k_add_i32, dim3(1), dim3(64), 0, st, p, v); }
+void launch_zero_i32(int* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,

@@ -27,8 +27,14 @@ EXPORTED = [
     "sppm_read_photons", "sppm_read_hitpoints", "sppm_read_shared", "sppm_read_counts",
     "sppm_read_neighbours", "sppm_trace_closest", "sppm_kernel_ms", "sppm_counters",
     "sppm_read_path_counts", "sppm_scene_stats", "sppm_guide_set", "sppm_guide_weights", "sppm_guide_gradient",
-    "sppm_guide_train", "sppm_guide_read",
+    "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
+    "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
 ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("bvh_width", C.c_int32), ("grid_blocks", C.c_int32), ("ms_per_pass", C.c_float),
+                ("trial_ms", C.c_float * 6)]
 GUIDE_CELLS = 262144
 GUIDE_NPARAMS = 16705
 
@@ -119,6 +125,12 @@ def load_library(path=LIB_PATH):
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
     lib.sppm_scene_stats.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]
     lib.sppm_guide_set.argtypes = [C.c_void_p, C.POINTER(GuideParams)]
+    lib.sppm_graph_capture.argtypes = [C.c_void_p]
+    lib.sppm_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    lib.sppm_batch_add.argtypes = [C.c_void_p, C.c_void_p]
+    for n in ("sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy"):
+        getattr(lib, n).argtypes = [C.c_void_p]
+    lib.sppm_autotune.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(TuneResult)]
     lib.sppm_guide_weights.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_gradient.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
@@ -268,6 +280,20 @@ class Renderer:
         _check(self.lib.sppm_kernel_ms(self.h, name.encode(), C.byref(v)))
         return v.value
 
+    # ---- graph replay, autotuner --------------------------------------------
+    def graph_capture(self):
+        _check(self.lib.sppm_graph_capture(self.h))
+
+    def autotune(self, scene, passes=2):
+        d, keep = scene_desc(scene)
+        res = TuneResult()
+        _check(self.lib.sppm_autotune(self.h, C.byref(d), passes, C.byref(res)))
+        self._keep = keep
+        self.passes = 0
+        self.total_photons = 0
+        return dict(bvh_width=res.bvh_width, grid_blocks=res.grid_blocks, ms_per_pass=res.ms_per_pass,
+                    trial_ms=list(res.trial_ms))
+
     # ---- learned emission proposal --------------------------------------
     def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9):
         g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay)
@@ -375,3 +401,38 @@ class Renderer:
         t, u, v = (np.zeros(n, np.float32) for _ in range(3))
         _check(self.lib.sppm_trace_closest(self.h, n, _fp(org), _fp(dirs), tmin, tmax, _ip(f), _fp(t), _fp(u), _fp(v)))
         return f, t, u, v
+
+
+class Batch:
+    """One graph launch per pass over several Renderers (each begun, not yet
+    stepped); their passes run as concurrent branches of the graph."""
+
+    def __init__(self, renderers, device=0):
+        self.lib = lib()
+        h = C.c_void_p()
+        _check(self.lib.sppm_batch_create(C.byref(h), device))
+        self.h = h
+        self.renderers = list(renderers)
+        for r in self.renderers:
+            _check(self.lib.sppm_batch_add(self.h, r.h))
+        _check(self.lib.sppm_batch_capture(self.h))
+
+    def pass_(self):
+        _check(self.lib.sppm_batch_pass(self.h))
+        for r in self.renderers:
+            r.passes += 1
+            r.total_photons += r.N
+
+    def sync(self):
+        _check(self.lib.sppm_batch_sync(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.sppm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

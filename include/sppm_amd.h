@@ -255,6 +255,38 @@ int sppm_guide_train(sppm_ctx* ctx, const float* hist, int steps);
 /* Current proposal probabilities per cell and the training histogram (either may be NULL). */
 int sppm_guide_read(sppm_ctx* ctx, float* prob, float* hist);
 
+/* ---- graph-captured passes, batches of scenes, autotuner --------------------
+ * No counterpart in the reference (it renders one scene per process,
+ * main.cpp:117-167).  A captured pass replays every kernel of sppm_pass from
+ * one hipGraph launch; a batch captures one pass of several contexts (each
+ * set up with its own scene, camera and params and begun) into one graph whose
+ * branches run concurrently.  Capture right after sppm_begin; needs the
+ * UniformSample camera stream and record_neighbours 0.  After capture,
+ * sppm_pass replays the graph and the split pass steps are refused;
+ * sppm_begin drops the graph. */
+int sppm_graph_capture(sppm_ctx* ctx);
+typedef struct sppm_batch sppm_batch;
+int sppm_batch_create(sppm_batch** out, int device);
+int sppm_batch_add(sppm_batch* batch, sppm_ctx* ctx);
+int sppm_batch_capture(sppm_batch* batch);
+/* one pass of every context in the batch (one graph launch; asynchronous) */
+int sppm_batch_pass(sppm_batch* batch);
+int sppm_batch_sync(sppm_batch* batch);
+/* destroys the batch (not its contexts) */
+int sppm_batch_destroy(sppm_batch* batch);
+
+/* Autotuner: times `passes` passes of each BVH width {2, 4, 8} x path-kernel
+ * grid {device-filling, half}, keeps the fastest (the context is left begun
+ * with it).  Call after sppm_camera_set and sppm_params_set; `scene` is the
+ * scene to (re)upload. */
+typedef struct sppm_tune_result {
+    int32_t bvh_width;     /* chosen BVH width */
+    int32_t grid_blocks;   /* chosen path-kernel grid (blocks of 64 lanes) */
+    float ms_per_pass;     /* chosen configuration */
+    float trial_ms[6];     /* width 2, 4, 8 x grid full, half */
+} sppm_tune_result;
+int sppm_autotune(sppm_ctx* ctx, const sppm_scene_desc* scene, int passes, sppm_tune_result* out);
+
 /* ---- timing helpers for the benchmark -------------------------------------- */
 /* Average duration (ms) of the named pass phase over the `sppm_pass` calls
  * since sppm_begin or the last reset, measured with hipEvents recorded on the
