@@ -55,6 +55,10 @@ def parse():
                     help="time-to-RMSE mode: alpha = 1 keeps the radius at r0, so guided and plain estimate the same "
                          "r0-smoothed image and the RMSE measures variance; with alpha < 1 the radius schedule "
                          "depends on the photons gathered and a finite plain reference is not bias-converged")
+    ap.add_argument("--batch", action="store_true",
+                    help="batched mode: many uneven seeded scenes, one graph launch per pass over all of them")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the workload, run a device elementwise copy of MIB MiB (float32, torch) three "
                          "times: a kernel of known traffic for scaling the HBM byte counters of a profiler run")
@@ -207,6 +211,72 @@ def rmse_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def batch_mode(args):
+    """Aggregate photons/s of K passes over all scenes of the batched workload,
+    each pass one graph launch whose branches are the scenes' passes; beside
+    it the same passes run scene after scene without graphs."""
+    import scenes
+    import sppm_amd
+    t0 = time.perf_counter()
+    work = scenes.batch_workload(n=args.scenes, progress=lambda s: s % 8 == 0 and log(f"generating scene {s}"))
+    tris = [w["scene"].nfacets for w in work]
+    log(f"{len(work)} scenes generated in {time.perf_counter() - t0:.1f} s: {sum(tris)} triangles, "
+        f"{sum(w['photons'] for w in work)} photons per pass")
+    rs, tuned = [], []
+    for k, w in enumerate(work):
+        r = sppm_amd.Renderer(w["scene"], w["cam"], w["alpha"], w["r0"], w["photons"], bvh_width=8)
+        if not args.no_autotune:
+            res = r.autotune(w["scene"], passes=2)
+            tuned.append((res["bvh_width"], res["grid_blocks"]))
+        rs.append(r)
+        if (k + 1) % 8 == 0:
+            log(f"set up {k + 1}/{len(work)} scenes")
+    K, W = args.steps, args.warmup
+    # sequential eager passes (reference point): scene after scene
+    ts = time.perf_counter()
+    for r in rs:
+        for _ in range(W + K):
+            r.pass_(stats=False)
+    for r in rs:
+        r.sync()
+    seq = time.perf_counter() - ts
+    log(f"sequential eager passes: {seq:.3f} s")
+    # re-begin every scene (autotuned configuration kept) and capture the batch graph
+    for r in rs:
+        sppm_amd._check(r.lib.sppm_begin(r.h))
+        r.passes = r.total_photons = 0
+    batch = sppm_amd.Batch(rs)
+    for _ in range(W):
+        batch.pass_()
+    batch.sync()
+    tb = time.perf_counter()
+    for _ in range(K):
+        batch.pass_()
+    batch.sync()
+    el = time.perf_counter() - tb
+    photons = sum(w["photons"] for w in work) * K
+    widths = {}
+    for bw, _ in tuned:
+        widths[bw] = widths.get(bw, 0) + 1
+    line = {
+        "metric": "photons/s (batched plain SPPM, aggregate over scenes, one graph launch per pass)",
+        "value": round(photons / el, 1), "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(el / K * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": f"batched SPPM, {len(work)} scenes", "triangles_total": sum(tris),
+                   "triangles_min": min(tris), "triangles_max": max(tris),
+                   "resolutions": sorted({w["cam"]["width"] for w in work}),
+                   "photons_per_pass_total": sum(w["photons"] for w in work), "alpha": 0.7,
+                   "autotune": not args.no_autotune, "tuned_bvh_widths": widths},
+        "sequential_eager_photons_per_s": round(photons * (W + K) / K / seq, 1),
+        "batch_speedup_vs_sequential": round(seq * K / (W + K) / el, 3),
+    }
+    batch.close()
+    for r in rs:
+        r.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -215,6 +285,9 @@ def main():
         sys.exit(2)
     if args.rmse:
         rmse_mode(args)
+        return
+    if args.batch:
+        batch_mode(args)
         return
     import numpy as np  # noqa: F401
     import sppm_amd

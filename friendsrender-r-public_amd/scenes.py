@@ -406,3 +406,138 @@ def occluded_camera(width, height, sampler_id=0):
     return dict(type=CAM_PINHOLE, width=width, height=height, pos=(0.95, 0.55, 0.1), target=(0.0, 0.3, 0.8),
                 fov_deg=60.0, F=1.0, dist_to_lens=0.01, aspect=width / height, sampler=SAMPLER_UNIFORM,
                 sampler_id=sampler_id, sampler_nsample=1)
+
+
+# ---- batched workload: many uneven scenes ----------------------------------------
+_ICO = None
+
+
+def _icosahedron():
+    global _ICO
+    if _ICO is None:
+        t = (1 + 5 ** 0.5) / 2
+        v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                      (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], np.float64)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                      (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                      (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)])
+        _ICO = (v, f)
+    return _ICO
+
+
+def displaced_mesh(center, r, freq, seed, amp=0.05):
+    """Vectorised geodesic sphere: 20 * freq^2 triangles, radially displaced by
+    3-octave value noise on a 16^3 lattice (a function of the direction, so
+    shared vertices stay shared).  Per-facet vertex arrays (V, N, UV), face
+    normals as vertex normals."""
+    v, f = _icosahedron()
+    i, j = np.meshgrid(np.arange(freq), np.arange(freq), indexing="ij")
+    up = j < freq - i
+    dn = j < freq - i - 1
+    iu, ju, idn, jdn = i[up], j[up], i[dn], j[dn]
+    # barycentric (i/f, j/f) corners of the up and down triangles
+    cu = np.stack([np.stack([iu, ju], -1), np.stack([iu + 1, ju], -1), np.stack([iu, ju + 1], -1)], 1)
+    cd = np.stack([np.stack([idn + 1, jdn], -1), np.stack([idn + 1, jdn + 1], -1), np.stack([idn, jdn + 1], -1)], 1)
+    cor = np.concatenate([cu, cd], 0).astype(np.float64) / freq  # (T, 3, 2)
+    A, B, Cc = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]  # (20, 3)
+    P = (A[:, None, None, :] + (B - A)[:, None, None, :] * cor[None, ..., 0:1] +
+         (Cc - A)[:, None, None, :] * cor[None, ..., 1:2]).reshape(-1, 3, 3)
+    P /= np.linalg.norm(P, axis=-1, keepdims=True)
+    rng = np.random.default_rng(seed)
+    offs = [rng.random(3) * 10 for _ in range(3)]
+    lat = rng.random((3, 16, 16, 16))
+    flat = P.reshape(-1, 3)
+    val = np.zeros(len(flat))
+    amp_, tot = 1.0, 0.0
+    for o in range(3):
+        q = (flat * (2.0 ** o) * 2.0 + offs[o]) % 16
+        i0 = np.floor(q).astype(np.int64)
+        fr = q - i0
+        fr = fr * fr * (3 - 2 * fr)
+        i1 = (i0 + 1) % 16
+        acc = np.zeros(len(flat))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    ix = i1[:, 0] if dx else i0[:, 0]
+                    iy = i1[:, 1] if dy else i0[:, 1]
+                    iz = i1[:, 2] if dz else i0[:, 2]
+                    w = (fr[:, 0] if dx else 1 - fr[:, 0]) * (fr[:, 1] if dy else 1 - fr[:, 1]) * \
+                        (fr[:, 2] if dz else 1 - fr[:, 2])
+                    acc += w * lat[o][ix, iy, iz]
+        val += amp_ * acc
+        tot += amp_
+        amp_ *= 0.5
+    disp = 1.0 + amp / r * (val / tot - 0.5) * 2.0
+    V = (flat * (r * disp)[:, None] + np.asarray(center)).reshape(-1, 3, 3)
+    e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+    fn = np.cross(e1, e2)
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True) + 1e-30
+    N = np.repeat(fn[:, None, :], 3, axis=1)
+    uv = np.zeros_like(V)
+    uv[..., 0] = 0.5 + np.arctan2(P[..., 2], P[..., 0]) / (2 * np.pi)
+    uv[..., 1] = 0.5 - np.arcsin(np.clip(P[..., 1], -1, 1)) / np.pi
+    return V.astype(np.float32), N.astype(np.float32), uv.astype(np.float32)
+
+
+def random_material(rng, name):
+    k = rng.integers(4)
+    alb = tuple(float(x) for x in rng.uniform(0.3, 0.9, 3))
+    if k == 0:
+        return dict(name=name, kind=MAT_DIFFUSE, albedo=alb, is_sppm=1, if_opaque=1)
+    if k == 1:
+        return dict(name=name, kind=MAT_METAL, albedo=alb, if_delta=1, is_sppm=0, if_opaque=1)
+    if k == 2:
+        return dict(name=name, kind=MAT_ROUGH_CONDUCTOR, albedo=alb, roughness=float(rng.uniform(0.1, 0.5)),
+                    index=1.0, extinction=20.0, is_sppm=1, if_opaque=1)
+    return dict(name=name, kind=MAT_GGX_GLASS, albedo=(0.98, 0.98, 0.98), roughness=float(rng.uniform(0.05, 0.3)),
+                index=1.5, is_sppm=0, if_opaque=0)
+
+
+def batch_scene(seed, tri_range=(10_000, 500_000), max_meshes=8):
+    """One scene of the batched workload: the box (walls + 0.25^2 light with
+    emission 15) plus 1-8 displaced meshes whose triangle counts are drawn
+    log-uniform in tri_range, each with a random material over diffuse /
+    specular / GGX conductor / GGX dielectric."""
+    rng = np.random.default_rng(seed)
+    attrs = [
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+             if_opaque=1),
+    ]
+    b = _Builder()
+    box_walls(b, 0, 1, 2)
+    light_quad(b, 3)
+    nmesh = int(rng.integers(1, max_meshes + 1))
+    for k in range(nmesh):
+        ntri = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        freq = max(2, int(round(np.sqrt(ntri / 20.0))))
+        r = float(rng.uniform(0.06, 0.15))
+        c = (float(rng.uniform(0.2, 0.8)), float(rng.uniform(r, 0.6)), float(rng.uniform(0.2, 0.8)))
+        attrs.append(random_material(rng, f"mesh{k}"))
+        V, N, T = displaced_mesh(c, r, freq, seed * 31 + k, amp=0.2 * r)
+        b.pos.extend(V.reshape(-1, 9))
+        b.nrm.extend(N.reshape(-1, 9))
+        b.uv.extend(T.reshape(-1, 9))
+        b.attr.extend([len(attrs) - 1] * len(V))
+    return b.scene(attrs)
+
+
+def batch_workload(n=64, seed=2024, tri_range=(10_000, 500_000), max_meshes=8, resolutions=(256, 512, 1024),
+                   progress=None):
+    """The batched workload: n scenes of uneven size, square resolutions drawn
+    from `resolutions`, photons per pass = U(0.5, 2) x pixels."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n):
+        if progress:
+            progress(s)
+        res = int(rng.choice(resolutions))
+        ppp = float(rng.uniform(0.5, 2.0))
+        sc = batch_scene(seed * 1000 + s, tri_range, max_meshes)
+        out.append(dict(scene=sc, cam=pinhole_camera(res, res), photons=int(ppp * res * res), alpha=0.7,
+                        r0=0.02 * sc.diagonal()))
+    return out
