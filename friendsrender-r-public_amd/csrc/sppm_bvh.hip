@@ -120,10 +120,12 @@ __global__ void k_karras(const uint32_t* __restrict__ keys, int n, int2* __restr
 }
 
 // Leaf k boxes copied to node n-1+k, then walk up; the second thread to reach
-// a node computes its box (union of its two children).
+// a node computes its box (union of its two children); the walk ends at the
+// root (node 0 of the radix tree, the root's split position for SAH).
 __global__ void k_refit(const float4* __restrict__ tbmin, const float4* __restrict__ tbmax,
                         const uint32_t* __restrict__ order, int n, const int2* __restrict__ child,
-                        const int* __restrict__ parent, float4* nbmin, float4* nbmax, int* __restrict__ flag) {
+                        const int* __restrict__ parent, float4* nbmin, float4* nbmax, int* __restrict__ flag,
+                        int root) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int node = n - 1 + k;
@@ -132,8 +134,10 @@ __global__ void k_refit(const float4* __restrict__ tbmin, const float4* __restri
     nbmax[node] = tbmax[tri];
     if (n == 1) return;
     for (;;) {
+        if (node == root) return;
         __threadfence();
         int p = parent[node];
+        if (p < 0 || p >= n - 1) return;  // not reached for a well-formed tree (host validation follows)
         if (atomicAdd(&flag[p], 1) == 0) return;
         __threadfence();
         int2 c = child[p];
@@ -144,7 +148,6 @@ __global__ void k_refit(const float4* __restrict__ tbmin, const float4* __restri
         nbmin[p] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
         nbmax[p] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
         node = p;
-        if (p == 0) return;
     }
 }
 
@@ -233,6 +236,190 @@ __global__ void k_tri_reorder(const float* __restrict__ pos, const uint32_t* __r
     tri[3 * k + 2] = make_float4(p[6] - v0x, p[7] - v0y, p[8] - v0z, 0.0f);
 }
 
+// ---- binned-SAH top-down builder ---------------------------------------------
+// One workgroup per open segment [lo, hi] of the primitive order: centroid
+// bounds, the axis of largest extent, 32 bins (counts and boxes), the split
+// with the lowest A_L N_L + A_R N_R, then a stable in-place partition.  The
+// segment's internal node id is its split position s (left = [lo, s]); every
+// position boundary is split exactly once, so ids are unique and the result
+// has the radix-tree format the refit and the wide collapse read: child
+// (left, right) with leaves n-1+k, range (lo, hi), parent.
+constexpr int SAH_BINS = 32;
+constexpr int SAH_TB = 256;
+struct SahSeg {
+    int lo, hi, parent, side;  // side 0: left child of parent, 1: right
+};
+__device__ __forceinline__ int fkey(float f) {  // order-preserving float -> int
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float fkey_inv(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void __launch_bounds__(SAH_TB) k_sah_split(const SahSeg* __restrict__ qin, int nin, SahSeg* __restrict__ qout,
+                                                      int* __restrict__ qcount, uint32_t* __restrict__ ord,
+                                                      uint32_t* __restrict__ tmp, const float4* __restrict__ tbmin,
+                                                      const float4* __restrict__ tbmax, int n, int2* __restrict__ child,
+                                                      int2* __restrict__ range, int* __restrict__ parent,
+                                                      int* __restrict__ root) {
+    __shared__ int cmin[3], cmax[3];
+    __shared__ int bcnt[SAH_BINS];
+    __shared__ int bmn[SAH_BINS][3], bmx[SAH_BINS][3];
+    __shared__ int s_split, s_axis;
+    __shared__ float s_lo, s_scale;
+    __shared__ int wsum[SAH_TB / 64];
+    const int t = threadIdx.x;
+    const SahSeg sg = qin[blockIdx.x];
+    const int lo = sg.lo, hi = sg.hi, cnt = hi - lo + 1;
+    if (t < 3) { cmin[t] = 0x7fffffff; cmax[t] = (int)0x80000000; }
+    if (t < SAH_BINS) {
+        bcnt[t] = 0;
+        for (int a = 0; a < 3; ++a) { bmn[t][a] = 0x7fffffff; bmx[t][a] = (int)0x80000000; }
+    }
+    __syncthreads();
+    // centroid bounds
+    for (int p = lo + t; p <= hi; p += SAH_TB) {
+        const uint32_t f = ord[p];
+        const float4 a = tbmin[f], b = tbmax[f];
+        const float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&cmin[k], fkey(c[k]));
+            atomicMax(&cmax[k], fkey(c[k]));
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        int ax = 0;
+        float ext = -1.0f;
+        for (int k = 0; k < 3; ++k) {
+            const float e = fkey_inv(cmax[k]) - fkey_inv(cmin[k]);
+            if (e > ext) { ext = e; ax = k; }
+        }
+        s_axis = ax;
+        s_lo = fkey_inv(cmin[ax]);
+        s_scale = ext > 0.0f ? (float)SAH_BINS * (1.0f - 1e-6f) / ext : 0.0f;
+    }
+    __syncthreads();
+    const int ax = s_axis;
+    const float blo = s_lo, bs = s_scale;
+    auto bin_of = [&](uint32_t f) {
+        const float4 a = tbmin[f], b = tbmax[f];
+        const float c = ax == 0 ? 0.5f * (a.x + b.x) : (ax == 1 ? 0.5f * (a.y + b.y) : 0.5f * (a.z + b.z));
+        int k = (int)((c - blo) * bs);
+        return k < 0 ? 0 : (k >= SAH_BINS ? SAH_BINS - 1 : k);
+    };
+    if (bs > 0.0f) {
+        for (int p = lo + t; p <= hi; p += SAH_TB) {
+            const uint32_t f = ord[p];
+            const int k = bin_of(f);
+            const float4 a = tbmin[f], b = tbmax[f];
+            atomicAdd(&bcnt[k], 1);
+            atomicMin(&bmn[k][0], fkey(a.x)); atomicMin(&bmn[k][1], fkey(a.y)); atomicMin(&bmn[k][2], fkey(a.z));
+            atomicMax(&bmx[k][0], fkey(b.x)); atomicMax(&bmx[k][1], fkey(b.y)); atomicMax(&bmx[k][2], fkey(b.z));
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        int best = -1;
+        if (bs > 0.0f) {
+            // suffix areas / counts, then the sweep
+            float ra[SAH_BINS];
+            int rc[SAH_BINS];
+            float mn[3] = {3e38f, 3e38f, 3e38f}, mx[3] = {-3e38f, -3e38f, -3e38f};
+            int c = 0;
+            for (int k = SAH_BINS - 1; k >= 0; --k) {
+                if (bcnt[k]) {
+                    for (int a = 0; a < 3; ++a) {
+                        mn[a] = fminf(mn[a], fkey_inv(bmn[k][a]));
+                        mx[a] = fmaxf(mx[a], fkey_inv(bmx[k][a]));
+                    }
+                }
+                c += bcnt[k];
+                const float x = mx[0] - mn[0], y = mx[1] - mn[1], z = mx[2] - mn[2];
+                ra[k] = c ? x * y + y * z + z * x : 0.0f;
+                rc[k] = c;
+            }
+            float lmn[3] = {3e38f, 3e38f, 3e38f}, lmx[3] = {-3e38f, -3e38f, -3e38f};
+            int lc = 0;
+            float bestc = 3.4e38f;
+            for (int k = 0; k < SAH_BINS - 1; ++k) {
+                if (bcnt[k]) {
+                    for (int a = 0; a < 3; ++a) {
+                        lmn[a] = fminf(lmn[a], fkey_inv(bmn[k][a]));
+                        lmx[a] = fmaxf(lmx[a], fkey_inv(bmx[k][a]));
+                    }
+                }
+                lc += bcnt[k];
+                if (lc == 0 || rc[k + 1] == 0) continue;
+                const float x = lmx[0] - lmn[0], y = lmx[1] - lmn[1], z = lmx[2] - lmn[2];
+                const float cost = (x * y + y * z + z * x) * (float)lc + ra[k + 1] * (float)rc[k + 1];
+                if (cost < bestc) { bestc = cost; best = k; }
+            }
+        }
+        s_split = best;  // -1: split at the middle of the range
+    }
+    __syncthreads();
+    const int split_bin = s_split;
+    int nleft;
+    if (split_bin < 0) {
+        nleft = cnt / 2;
+    } else {
+        // stable partition through tmp: left keeps order, then right keeps order
+        const int lane = t & 63, w = t >> 6;
+        int base_l = 0, base_r = 0;
+        int total_left = 0;
+        for (int k = 0; k <= split_bin; ++k) total_left += bcnt[k];
+        for (int c0 = lo; c0 <= hi; c0 += SAH_TB) {
+            const int p = c0 + t;
+            const bool valid = p <= hi;
+            const uint32_t f = valid ? ord[p] : 0u;
+            const bool left = valid && bin_of(f) <= split_bin;
+            const unsigned long long bl = __ballot(left);
+            const int rank_w = __popcll(bl & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[w] = __popcll(bl);
+            __syncthreads();
+            int before = 0, tot = 0;
+            for (int k = 0; k < SAH_TB / 64; ++k) {
+                if (k < w) before += wsum[k];
+                tot += wsum[k];
+            }
+            const int nchunk = min(SAH_TB, hi - c0 + 1);
+            if (valid) {
+                if (left) tmp[lo + base_l + before + rank_w] = f;
+                else tmp[lo + total_left + base_r + (t - (before + rank_w))] = f;
+            }
+            base_l += tot;
+            base_r += nchunk - tot;
+            __syncthreads();
+        }
+        for (int p = lo + t; p <= hi; p += SAH_TB) ord[p] = tmp[p];
+        nleft = total_left;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int s = lo + nleft - 1;  // this node's id
+        const int lid = nleft == 1 ? (n - 1 + lo) : -1;
+        const int rid = cnt - nleft == 1 ? (n - 1 + hi) : -1;
+        child[s] = make_int2(lid, rid);
+        range[s] = make_int2(lo, hi);
+        if (sg.parent < 0) {
+            *root = s;
+        } else {
+            if (sg.side == 0) child[sg.parent].x = s;
+            else child[sg.parent].y = s;
+            parent[s] = sg.parent;
+        }
+        if (lid >= 0) parent[lid] = s;
+        if (rid >= 0) parent[rid] = s;
+        if (nleft > 1) qout[atomicAdd(qcount, 1)] = SahSeg{lo, s, s, 0};
+        if (cnt - nleft > 1) qout[atomicAdd(qcount, 1)] = SahSeg{s + 1, hi, s, 1};
+    }
+}
+
+__global__ void k_iota_u32(uint32_t* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
 __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -259,7 +446,8 @@ void free_bvh(BvhResult* r) {
     r->nnodes = 0;
 }
 
-int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out, char* err, size_t errlen) {
+int build_bvh(const float* pos, int n, int width, bool sah, hipStream_t st, BvhResult* out, char* err,
+              size_t errlen) {
     const int TB = 256;
     const int G = (n + TB - 1) / TB;
     const int NF = node_floats(width);
@@ -267,6 +455,8 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
     uint32_t *cb = nullptr, *keys = nullptr, *vals = nullptr, *tk = nullptr, *tv = nullptr, *hist = nullptr;
     int2 *child = nullptr, *range = nullptr, *qa = nullptr, *qb = nullptr;
     int *parent = nullptr, *flag = nullptr, *counters = nullptr;
+    SahSeg *sa = nullptr, *sb = nullptr;
+    int root_id = 0;
     int nint = n > 1 ? n - 1 : 1;
     int maxw = n > 1 ? n - 1 : 1;
     int hcnt[2];
@@ -294,6 +484,10 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
     HIPCHK(hipMalloc(&qa, sizeof(int2) * maxw));
     HIPCHK(hipMalloc(&qb, sizeof(int2) * maxw));
     HIPCHK(hipMalloc(&counters, sizeof(int) * 2));
+    if (sah) {
+        HIPCHK(hipMalloc(&sa, sizeof(SahSeg) * (size_t)n));
+        HIPCHK(hipMalloc(&sb, sizeof(SahSeg) * (size_t)n));
+    }
     HIPCHK(hipMalloc(&out->nodes, sizeof(float) * NF * (size_t)maxw));
     HIPCHK(hipMalloc(&out->tri, sizeof(float4) * 3 * (size_t)n));
     {
@@ -301,14 +495,66 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
         HIPCHK(hipMemcpyAsync(cb, init, sizeof(init), hipMemcpyHostToDevice, st));
     }
     hipLaunchKernelGGL(k_tri_prep, dim3(G), dim3(TB), 0, st, pos, n, tbmin, tbmax, cb);
-    hipLaunchKernelGGL(k_morton, dim3(G), dim3(TB), 0, st, pos, n, cb, keys, vals);
-    radix_sort_pairs(keys, vals, tk, tv, hist, n, 30, st);
-    HIPCHK(hipGetLastError());
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3((n - 1 + TB - 1) / TB), dim3(TB), 0, st, keys, n, child, range, parent);
+    if (sah && n > 1) {
+        // binned-SAH top-down, one launch per level (one workgroup per open segment)
+        hipLaunchKernelGGL(k_iota_u32, dim3(G), dim3(TB), 0, st, vals, n);
+        hipLaunchKernelGGL(k_fill_u32, dim3((2 * n + TB - 1) / TB), dim3(TB), 0, st, (uint32_t*)parent, 2 * n,
+                           0xffffffffu);
+        SahSeg r0{0, n - 1, -1, 0};
+        HIPCHK(hipMemcpyAsync(sa, &r0, sizeof r0, hipMemcpyHostToDevice, st));
+        int* rootp = counters;  // counters[0] = root id, counters[1] = next queue size
+        int nseg = 1;
+        while (nseg > 0) {
+            int zero = 0;
+            HIPCHK(hipMemcpyAsync(counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_sah_split, dim3(nseg), dim3(SAH_TB), 0, st, sa, nseg, sb, counters + 1, vals, tv,
+                               tbmin, tbmax, n, child, range, parent, rootp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hcnt, counters, sizeof(hcnt), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (hcnt[1] < 0 || hcnt[1] > n) {
+                snprintf(err, errlen, "SAH build: bad segment count %d", hcnt[1]);
+                goto fail;
+            }
+            nseg = hcnt[1];
+            SahSeg* t = sa; sa = sb; sb = t;
+        }
+        root_id = hcnt[0];
+        if (root_id < 0 || root_id >= n - 1) {
+            snprintf(err, errlen, "SAH build: bad root %d", root_id);
+            goto fail;
+        }
+        {
+            // host check of the binary tree before the refit and the collapse
+            // read it: every child reference in range, every node but the root
+            // referenced exactly once
+            std::vector<int2> hc((size_t)n - 1);
+            HIPCHK(hipMemcpy(hc.data(), child, sizeof(int2) * hc.size(), hipMemcpyDeviceToHost));
+            std::vector<unsigned char> refd((size_t)2 * n - 1, 0);
+            bool ok = true;
+            for (int i = 0; i < n - 1 && ok; ++i)
+                for (int r : {hc[i].x, hc[i].y}) {
+                    if (r < 0 || r > 2 * n - 2 || refd[r]) { ok = false; break; }
+                    refd[r] = 1;
+                }
+            for (int i = 0; i < 2 * n - 1 && ok; ++i) ok = (i == root_id) ? !refd[i] : refd[i] != 0;
+            if (!ok) {
+                snprintf(err, errlen, "SAH build: binary tree validation failed");
+                goto fail;
+            }
+        }
         hipLaunchKernelGGL(k_fill_u32, dim3((nint + TB - 1) / TB), dim3(TB), 0, st, (uint32_t*)flag, nint, 0u);
+    } else {
+        hipLaunchKernelGGL(k_morton, dim3(G), dim3(TB), 0, st, pos, n, cb, keys, vals);
+        radix_sort_pairs(keys, vals, tk, tv, hist, n, 30, st);
+        HIPCHK(hipGetLastError());
+        if (n > 1) {
+            hipLaunchKernelGGL(k_karras, dim3((n - 1 + TB - 1) / TB), dim3(TB), 0, st, keys, n, child, range, parent);
+            hipLaunchKernelGGL(k_fill_u32, dim3((nint + TB - 1) / TB), dim3(TB), 0, st, (uint32_t*)flag, nint, 0u);
+        }
     }
-    hipLaunchKernelGGL(k_refit, dim3(G), dim3(TB), 0, st, tbmin, tbmax, vals, n, child, parent, nbmin, nbmax, flag);
+    hipLaunchKernelGGL(k_refit, dim3(G), dim3(TB), 0, st, tbmin, tbmax, vals, n, child, parent, nbmin, nbmax, flag,
+                       root_id);
     hipLaunchKernelGGL(k_tri_reorder, dim3(G), dim3(TB), 0, st, pos, vals, n, out->tri);
     HIPCHK(hipGetLastError());
     if (n == 1) {
@@ -330,7 +576,7 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
         out->nnodes = 1;
         depth = 1;
     } else {
-        int2 root = make_int2(0, 0);
+        int2 root = make_int2(root_id, 0);  // radix tree: node 0; SAH: the root's split position
         int init[2] = {1, 0};
         HIPCHK(hipMemcpyAsync(qa, &root, sizeof(int2), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
@@ -403,14 +649,14 @@ int build_bvh(const float* pos, int n, int width, hipStream_t st, BvhResult* out
     }
     hfree(tbmin); hfree(tbmax); hfree(cb); hfree(keys); hfree(vals); hfree(tk); hfree(tv);
     hfree(hist); hfree(child); hfree(range); hfree(parent); hfree(flag); hfree(nbmin);
-    hfree(nbmax); hfree(qa); hfree(qb); hfree(counters);
+    hfree(nbmax); hfree(qa); hfree(qb); hfree(counters); hfree(sa); hfree(sb);
     return 0;
 fail:
     hfree(tbmin); hfree(tbmax); hfree(cb); hfree(keys);
     hfree(vals); hfree(tk); hfree(tv); hfree(hist);
     hfree(child); hfree(range); hfree(parent); hfree(flag);
     hfree(nbmin); hfree(nbmax); hfree(qa); hfree(qb);
-    hfree(counters);
+    hfree(counters); hfree(sa); hfree(sb);
     free_bvh(out);
     return -1;
 }

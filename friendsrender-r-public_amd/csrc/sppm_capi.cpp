@@ -289,8 +289,10 @@ int sppm_destroy(sppm_ctx* c) {
 // ---------------------------------------------------------------------------
 // Scene upload
 // ---------------------------------------------------------------------------
-int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width) {
+int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags) {
     if (!c || !d) return fail(SPPM_E_ARG, "NULL argument");
+    const bool lbvh = (bvh_width_flags & SPPM_BVH_LBVH) != 0;
+    const int bvh_width = bvh_width_flags & ~SPPM_BVH_LBVH;
     if (bvh_width != 2 && bvh_width != 4 && bvh_width != 8) return fail(SPPM_E_ARG, "bvh_width must be 2, 4 or 8");
     const int nf = d->nfacets;
     if (nf <= 0 || !d->pos || !d->nrm || !d->uv || !d->attr) return fail(SPPM_E_SCENE, "empty or incomplete facet arrays");
@@ -465,7 +467,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width) {
         HIPOK(hipMemcpyAsync(S.ltab_right, tright.data(), sizeof(int) * tright.size(), hipMemcpyHostToDevice, st));
     }
     char err[256] = {0};
-    if (build_bvh(dpos, nf, bvh_width, st, &S.bvh, err, sizeof err) != 0) {
+    if (build_bvh(dpos, nf, bvh_width, !lbvh, st, &S.bvh, err, sizeof err) != 0) {
         dfree(dpos);
         S.release();
         return fail(SPPM_E_HIP, "BVH build failed: %s", err);

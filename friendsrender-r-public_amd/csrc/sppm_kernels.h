@@ -25,7 +25,9 @@ struct BvhResult {
     int width = 0;
 };
 // pos: device array of 9 floats per triangle. Returns 0 on success.
-int build_bvh(const float* pos, int ntri, int width, hipStream_t st, BvhResult* out, char* err, size_t errlen);
+// sah: binned-SAH top-down build; otherwise a radix tree over Morton codes.
+int build_bvh(const float* pos, int ntri, int width, bool sah, hipStream_t st, BvhResult* out, char* err,
+              size_t errlen);
 void free_bvh(BvhResult* r);
 
 // ---- pass kernels (sppm_pass.hip) -----------------------------------------

@@ -36,6 +36,7 @@ class TuneResult(C.Structure):
     _fields_ = [("bvh_width", C.c_int32), ("grid_blocks", C.c_int32), ("ms_per_pass", C.c_float),
                 ("trial_ms", C.c_float * 6)]
 GUIDE_CELLS = 262144
+BVH_LBVH = 0x100  # bvh_width flag: radix tree over Morton codes instead of binned SAH
 GUIDE_NPARAMS = 16705
 
 

@@ -150,6 +150,11 @@ int sppm_create(sppm_ctx** out, int device, void* stream);
 int sppm_destroy(sppm_ctx* ctx);
 
 /* ---- setup (Scene::CreateScene, camera ctor, constants) ------------------ */
+/* bvh_width: 2, 4 or 8, optionally | SPPM_BVH_LBVH.  The closest-hit
+ * structure is built on the GPU: binned-SAH top-down by default (replaces
+ * N_BVH::Build3, bvh8.h:135-189), or with SPPM_BVH_LBVH a radix tree over
+ * Morton codes; either is collapsed to `bvh_width`-wide nodes. */
+#define SPPM_BVH_LBVH 0x100
 int sppm_scene_upload(sppm_ctx* ctx, const sppm_scene_desc* scene, int bvh_width);
 int sppm_camera_set(sppm_ctx* ctx, const sppm_camera_desc* cam);
 int sppm_params_set(sppm_ctx* ctx, const sppm_params* params);

@@ -45,15 +45,17 @@ def random_rays(scene, n, seed):
     return org, d.astype(np.float32)
 
 
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
 @pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("which", ["box", "full"])
-def test_closest_hit_bit_exact(width, which, box_scene, full_scene_small):
+def test_closest_hit_bit_exact(width, which, builder, box_scene, full_scene_small):
     scene = box_scene if which == "box" else full_scene_small
-    say(f"closest hit: scene={which} width={width}")
+    say(f"closest hit: scene={which} width={width} builder={builder}")
     org, d = random_rays(scene, 20000, seed=width * 7 + len(which))
     os_ = pyoracle.OracleScene(scene)
     fo, to, uo, vo = os_.closest_hit(org, d, mode=0)  # brute force
-    r = sppm_amd.Renderer(scene, scenes.pinhole_camera(8, 8), 0.9, 0.015, 64, bvh_width=width)
+    flags = width | (sppm_amd.BVH_LBVH if builder == "lbvh" else 0)
+    r = sppm_amd.Renderer(scene, scenes.pinhole_camera(8, 8), 0.9, 0.015, 64, bvh_width=flags)
     fg, tg, ug, vg = r.trace(org, d)
     r.close()
     np.testing.assert_array_equal(fg, fo)
