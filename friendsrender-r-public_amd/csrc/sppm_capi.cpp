@@ -616,7 +616,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
-        (rc = dalloc(&R.groups.nvalid, 1)) || (rc = dalloc(&R.pass_dev, 1)) ||
+        (rc = dalloc(&R.groups.nvalid, 2)) || (rc = dalloc(&R.pass_dev, 1)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;

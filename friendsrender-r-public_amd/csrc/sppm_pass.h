@@ -62,7 +62,7 @@ struct TileRange;
 // Sort buffers of the hit-point groups (np entries each; hist as radix_hist_entries(np)).
 struct GroupBuffers {
     uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
-    int* nvalid;
+    int* nvalid;  // [0] valid hit points, [1] chunk queue of the gather
 };
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,

@@ -959,6 +959,14 @@ __global__ void k_tile_chunks(const TileRange* __restrict__ tiles, int ntiles, c
     nch[t] = (tiles[t].cand + c - 1) / c;
 }
 
+// Branch-free form of in_query (same two tests, evaluated without early exit).
+__device__ __forceinline__ bool in_query_nb(const GatherLane& L, const float4 pp) {
+    const bool box = (L.bbmin.x <= pp.x) & (pp.x <= L.bbmax.x) & (L.bbmin.y <= pp.y) & (pp.y <= L.bbmax.y) &
+                     (L.bbmin.z <= pp.z) & (pp.z <= L.bbmax.z);
+    const f3 d = mk(pp.x, pp.y, pp.z) - L.c;
+    return box & (norm2(d) < L.R2);
+}
+
 // The reference's two neighbour tests for one candidate: closed box
 // (kdtree.h:78-88) then the strict radius (SPPM.cpp:432).
 __device__ __forceinline__ bool in_query(const GatherLane& L, const float4 pp) {
@@ -982,7 +990,8 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
                                                       const TileRange* __restrict__ tiles, int ntiles,
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
-                                                      float4* __restrict__ partial, float* __restrict__ pdep) {
+                                                      int* __restrict__ queue, float4* __restrict__ partial,
+                                                      float* __restrict__ pdep) {
     __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
     __shared__ unsigned char s_hit[64 * 64];
     __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
@@ -991,7 +1000,15 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
     const int CH = *chunkp;
     const int nvalid = *nvalidp;
     const GridParams g = *gp;
-    for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
+    // chunks are claimed from a counter (zeroed before the launch); the
+    // counter only grows, so every wave ends once it passes the total.  The
+    // partial sums go to the chunk's own slot, so the result does not depend
+    // on which wave took which chunk.
+    for (;;) {
+        int ch = 0;
+        if (lane == 0) ch = atomicAdd(queue, 1);
+        ch = __shfl(ch, 0, 64);
+        if (ch >= total) break;
         // owning tile: last t with chunk_off[t] <= ch
         int lo = 0, hi = ntiles;
         while (hi - lo > 1) {
@@ -1043,11 +1060,26 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                         if (lane == 0) s_cnt[j] = __popcll(m);
                     }
                 } else if (mine) {
-                    for (int j = 0; j < n; ++j) {
-                        if (in_query(L, s_pos[j])) {
-                            s_hit[nh * 64 + lane] = (unsigned char)j;
-                            ++nh;
-                        }
+                    // four staged photons per step, no branches: the list slot nh is
+                    // written every time and kept only when the photon passes (nh <= j,
+                    // so the slot stays below 64)
+                    int j = 0;
+                    for (; j + 4 <= n; j += 4) {
+                        const float4 p0 = s_pos[j], p1 = s_pos[j + 1], p2 = s_pos[j + 2], p3 = s_pos[j + 3];
+                        const bool h0 = in_query_nb(L, p0), h1 = in_query_nb(L, p1);
+                        const bool h2 = in_query_nb(L, p2), h3 = in_query_nb(L, p3);
+                        s_hit[nh * 64 + lane] = (unsigned char)j;
+                        nh += h0;
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + 1);
+                        nh += h1;
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + 2);
+                        nh += h2;
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + 3);
+                        nh += h3;
+                    }
+                    for (; j < n; ++j) {
+                        s_hit[nh * 64 + lane] = (unsigned char)j;
+                        nh += in_query_nb(L, s_pos[j]);
                     }
                 }
                 L.M += nh;
@@ -1283,7 +1315,7 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
                             hipStream_t st) {
     const int ntiles = gather_group_count(np);
     // hit-point groups: valid hit points in Morton order, 64 per group
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 1);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 2);  // valid count, chunk queue
     if (counts) hipLaunchKernelGGL(k_zero_i32, dim3((np + 255) / 256), dim3(256), 0, st, counts, np);
     hipLaunchKernelGGL(k_hit_keys, dim3((np + 255) / 256), dim3(256), 0, st, H, np, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], gb.keys, gb.vals, gb.nvalid);
@@ -1294,7 +1326,7 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
-                       ntiles, chunk_off, chunkp, partial, pdep);
+                       ntiles, chunk_off, chunkp, gb.nvalid + 1, partial, pdep);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
                        partial, counts, rstat, ctr, flags);
 }
