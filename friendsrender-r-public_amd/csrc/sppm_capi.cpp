@@ -139,7 +139,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -616,7 +616,8 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
-        (rc = dalloc(&R.groups.nvalid, 2)) || (rc = dalloc(&R.pass_dev, 1)) ||
+        (rc = dalloc(&R.groups.nvalid, 2)) || (rc = dalloc(&R.groups.stat, 3 * (size_t)gather_group_count(np))) ||
+        (rc = dalloc(&R.pass_dev, 1)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;

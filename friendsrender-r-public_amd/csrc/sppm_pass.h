@@ -63,6 +63,7 @@ struct TileRange;
 struct GroupBuffers {
     uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
     int* nvalid;  // [0] valid hit points, [1] chunk queue of the gather
+    unsigned long long* stat;  // 3 per group: radius sum, active hit points, neighbours
 };
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,

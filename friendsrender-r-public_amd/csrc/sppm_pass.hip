@@ -847,19 +847,43 @@ __device__ __forceinline__ int quant10(float p, float lo, float ext) {
     int q = (int)f;
     return q < 0 ? 0 : (q > 1023 ? 1023 : q);
 }
+constexpr uint32_t HIT_KEY_INVALID = 1u << 30;  // above every 30-bit Morton code: invalid hit points sort last
 __global__ void k_hit_keys(HitDev H, int np, float bx, float by, float bz, float ex, float ey, float ez,
-                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* __restrict__ nvalid) {
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const float4 hp = H.pos[i];
-    uint32_t key = 1u << 30;  // invalid hit points sort last
-    if ((int)hp.w == 1) {
+    uint32_t key = HIT_KEY_INVALID;
+    if ((int)hp.w == 1)
         key = spread10((uint32_t)quant10(hp.x, bx, ex)) | (spread10((uint32_t)quant10(hp.y, by, ey)) << 1) |
               (spread10((uint32_t)quant10(hp.z, bz, ez)) << 2);
-        atomicAdd(nvalid, 1);
-    }
     keys[i] = key;
     vals[i] = (uint32_t)i;
+}
+// Number of valid hit points = first position of HIT_KEY_INVALID in the
+// sorted keys.  One wave probes 64 evenly spaced positions per round, so a
+// million keys take about four rounds of dependent loads (one global counter
+// bumped by every wave would serialise thousands of atomics on one address).
+__global__ void __launch_bounds__(64) k_count_valid(const uint32_t* __restrict__ keys, int n, int* __restrict__ nvalid) {
+    const int lane = threadIdx.x;
+    int lo = 0, hi = n;  // keys[< lo] valid, keys[>= hi] invalid
+    while (lo < hi) {
+        const int step = (hi - lo + 63) / 64;
+        const int q = lo + lane * step;
+        const bool inr = q < hi;
+        const bool ge = inr && keys[q] >= HIT_KEY_INVALID;
+        const unsigned long long m = __ballot(ge || !inr);
+        if (m == 0ull) {  // all 64 probes valid
+            lo = lo + 63 * step + 1;
+            continue;
+        }
+        const int k = __builtin_ctzll(m);
+        const int qk = lo + k * step;
+        const int nlo = k > 0 ? lo + (k - 1) * step + 1 : lo;
+        hi = qk < hi ? qk : hi;
+        lo = nlo;
+    }
+    if (lane == 0) *nvalid = lo;
 }
 __device__ __forceinline__ int group_pixel(const uint32_t* __restrict__ order, int nvalid, int lin) {
     return lin < nvalid ? (int)order[lin] : -1;
@@ -1113,7 +1137,7 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
                                                       const int* __restrict__ nvalidp, float alpha,
                                                       const int* __restrict__ chunk_off,
                                                       const float4* __restrict__ partial, int* __restrict__ counts,
-                                                      unsigned long long* __restrict__ rstat, Counters* __restrict__ ctr,
+                                                      unsigned long long* __restrict__ gstat,
                                                       uint32_t* __restrict__ flags) {
     const int lane = threadIdx.x;
     const int t = blockIdx.x;
@@ -1126,6 +1150,7 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
     const bool active = p >= 0 && (int)hp.w == 1;
     f3 add = mk(0.0f, 0.0f, 0.0f);
     int M = 0;
+#pragma unroll 4
     for (int c = chunk_off[t]; c < chunk_off[t + 1]; ++c) {
         float4 v = partial[(size_t)c * 64 + lane];
         add = add + mk(v.x, v.y, v.z);
@@ -1156,11 +1181,51 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
         rc += __shfl_xor(rc, off, 64);
         tn += __shfl_xor(tn, off, 64);
     }
+    // per-group sums, added up by k_group_stats (not one atomic per group on
+    // the same four addresses)
     if (lane == 0) {
-        atomicAdd(&rstat[0], rs);
-        atomicAdd(&rstat[1], rc);
-        atomicAdd(&ctr->hit_points, rc);
-        atomicAdd(&ctr->neighbours, tn);
+        gstat[3 * t] = rs;
+        gstat[3 * t + 1] = rc;
+        gstat[3 * t + 2] = tn;
+    }
+}
+
+// Sums the per-group statistics of k_gather_finish: radius sum, active hit
+// points (the next pass's cell size) and the counters.  One workgroup; the
+// only writer of rstat and of these counters in the pass.
+__global__ void __launch_bounds__(1024) k_group_stats(const unsigned long long* __restrict__ gstat, int ngroups,
+                                                      unsigned long long* __restrict__ rstat,
+                                                      Counters* __restrict__ ctr) {
+    __shared__ unsigned long long part[3][16];
+    unsigned long long rs = 0, rc = 0, tn = 0;
+    for (int g = threadIdx.x; g < ngroups; g += 1024) {
+        rs += gstat[3 * g];
+        rc += gstat[3 * g + 1];
+        tn += gstat[3 * g + 2];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rs += __shfl_xor(rs, off, 64);
+        rc += __shfl_xor(rc, off, 64);
+        tn += __shfl_xor(tn, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = rs;
+        part[1][w] = rc;
+        part[2][w] = tn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, c = 0;
+        for (int k = 0; k < 16; ++k) {
+            a += part[0][k];
+            b += part[1][k];
+            c += part[2][k];
+        }
+        rstat[0] += a;
+        rstat[1] += b;
+        ctr->hit_points += b;
+        ctr->neighbours += c;
     }
 }
 
@@ -1318,8 +1383,9 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 2);  // valid count, chunk queue
     if (counts) hipLaunchKernelGGL(k_zero_i32, dim3((np + 255) / 256), dim3(256), 0, st, counts, np);
     hipLaunchKernelGGL(k_hit_keys, dim3((np + 255) / 256), dim3(256), 0, st, H, np, bmin[0], bmin[1], bmin[2], ext[0],
-                       ext[1], ext[2], gb.keys, gb.vals, gb.nvalid);
+                       ext[1], ext[2], gb.keys, gb.vals);
     radix_sort_pairs(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, gb.hist, np, 31, st);
+    hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st, gb.keys, np, gb.nvalid);
     const uint32_t* order = gb.vals;
     hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles);
     hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp, ctr);
@@ -1328,7 +1394,8 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
                        ntiles, chunk_off, chunkp, gb.nvalid + 1, partial, pdep);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
-                       partial, counts, rstat, ctr, flags);
+                       partial, counts, gb.stat, flags);
+    hipLaunchKernelGGL(k_group_stats, dim3(1), dim3(1024), 0, st, gb.stat, ntiles, rstat, ctr);
 }
 int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); }
 int gather_group_count(int np) { return (np + 63) / 64; }
