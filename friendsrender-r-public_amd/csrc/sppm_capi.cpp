@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,7 @@
 #include "sppm_guide.h"
 #include "sppm_host_math.h"
 #include "sppm_kernels.h"
+#include "sppm_mtjump.h"
 #include "sppm_pass.h"
 
 using namespace sppm;
@@ -115,8 +117,10 @@ struct RenderBuffers {
     int* lists = nullptr;
     long long lists_cap = 0;
     uint32_t* words = nullptr;
-    uint32_t* cam_mt = nullptr;
-    int* cam_pos = nullptr;
+    uint32_t* cam_states = nullptr;  // cam_K windows of 624 raw words (camera stream segments)
+    uint32_t* cam_phi = nullptr;     // jump polynomial over the other cam_K - 1 segments
+    int cam_K = 0;
+    long long cam_L = 0;
     unsigned long long* rstat = nullptr;  // (radius sum in 2^-32 units, hit points) of the last gather
     TileRange* tiles = nullptr;
     int ntiles = 0;
@@ -139,7 +143,7 @@ struct RenderBuffers {
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
         dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_mt); dfree(cam_pos); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
 };
@@ -572,6 +576,42 @@ int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
 }
 
 // initializeSharedPoints (SPPM.cpp:356-373) + buffers of the pass loop
+// Camera stream segments: the block of a pass (6 words per pixel) is cut
+// into K segments of L words, K the largest divisor of the block up to 64.
+// Segment k starts k * L words into the stream; after a pass every segment
+// jumps over the K - 1 others.
+static int init_camera_stream(sppm_ctx* c) {
+    RenderBuffers& R = c->rb;
+    const long long block = 6LL * R.np;
+    int K = 1;
+    for (int k = 64; k >= 1; --k)
+        if (block % k == 0) {
+            K = k;
+            break;
+        }
+    const long long L = block / K;
+    int rc = dalloc(&R.cam_states, (size_t)K * MT_WORDS);
+    if (rc) return rc;
+    std::vector<uint32_t> win((size_t)K * MT_WORDS);
+    mt_window_seed((uint32_t)c->cam.sampler_id, win.data());
+    for (int k = 1; k < K; ++k) {
+        std::copy(win.begin() + (size_t)(k - 1) * MT_WORDS, win.begin() + (size_t)k * MT_WORDS,
+                  win.begin() + (size_t)k * MT_WORDS);
+        mt_window_advance(win.data() + (size_t)k * MT_WORDS, (uint64_t)L);
+    }
+    uint32_t phi[MT_WORDS];
+    try {
+        mt_jump_poly((uint64_t)(K - 1) * (uint64_t)L, phi);
+    } catch (const std::exception& e) {
+        return fail(SPPM_E_STATE, "camera stream jump polynomial: %s", e.what());
+    }
+    HIPOK(hipMemcpy(R.cam_states, win.data(), win.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.cam_phi, phi, sizeof phi, hipMemcpyHostToDevice));
+    R.cam_K = K;
+    R.cam_L = L;
+    return SPPM_OK;
+}
+
 int sppm_begin(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
@@ -607,7 +647,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
         (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
-        (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_mt, 624)) || (rc = dalloc(&R.cam_pos, 1)) ||
+        (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_phi, MT_WORDS)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
         (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)gather_tile_count(W, Hh))) ||
         (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, gather_tile_count(W, Hh) + 1)) ||
@@ -634,14 +674,13 @@ int sppm_begin(sppm_ctx* c) {
     launch_init_slots(R.slots, c->stream);
     launch_zero_i32(R.pass_dev, 1, c->stream);
     launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
-    // UniformSample(ID): mt19937(ID) state, position 624 (first draw twists)
+    // UniformSample(ID): the mt19937(ID) stream in K segments per pass
     if (c->cam.sampler == SPPM_SAMPLER_UNIFORM) {
-        uint32_t mt[624];
-        mt[0] = (uint32_t)c->cam.sampler_id;
-        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
-        int pos = 624;
-        HIPOK(hipMemcpyAsync(R.cam_mt, mt, sizeof mt, hipMemcpyHostToDevice, c->stream));
-        HIPOK(hipMemcpyAsync(R.cam_pos, &pos, sizeof pos, hipMemcpyHostToDevice, c->stream));
+        rc = init_camera_stream(c);
+        if (rc) {
+            R.release();
+            return rc;
+        }
     }
     R.ntiles = gather_tile_count(W, Hh);
     launch_set_u64x2(R.rstat, (unsigned long long)((double)c->params.first_radius * 4294967296.0), 1ull, c->stream);
@@ -727,7 +766,7 @@ static int words_generate_upto(sppm_ctx* c, long long k) {
     while (c->words_gen <= k) {
         const int b = (int)(c->words_gen & 1);
         if (c->eye_recorded[b]) HIPOK(hipStreamWaitEvent(c->stream2, c->ev_eye[b], 0));
-        launch_camera_words(c->rb.cam_mt, c->rb.cam_pos, c->rb.words + b * blk, (long long)blk, c->stream2);
+        launch_camera_words(c->rb.cam_states, c->rb.cam_phi, c->rb.cam_K, c->rb.cam_L, c->rb.words + b * blk, c->stream2);
         HIPOK(hipEventRecord(c->ev_words[b], c->stream2));
         c->words_gen += 1;
     }
@@ -901,7 +940,7 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     RenderBuffers& R = c->rb;
     HIPOK(hipEventRecord(c->ev_gfork, st));
     HIPOK(hipStreamWaitEvent(st2, c->ev_gfork, 0));
-    launch_camera_words(R.cam_mt, R.cam_pos, R.words, 6LL * R.np, st2);
+    launch_camera_words(R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words, st2);
     HIPOK(hipEventRecord(c->ev_gjoin, st2));
     launch_photon(c->scene.dev, R.slots, R.N, 0, R.pass_dev, trace_spill(c), c->scene.next_path,
                   c->guided ? &c->emit : nullptr, R.flags, st);

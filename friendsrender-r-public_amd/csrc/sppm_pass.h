@@ -27,7 +27,9 @@ struct Counters {
     unsigned long long candidates;  // photons the balanced gather tested (union boxes of its tiles)
 };
 
-void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st);
+// Camera words of one pass: K segments of L words (K * L = 6 * pixels);
+// states holds K windows of 624 raw words, phi the jump polynomial of (K-1)L.
+void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long L, uint32_t* words, hipStream_t st);
 // Traversal stack spill area shared by the path kernels (they run one after
 // another on a stream): blocks * 64 lanes, cap - TRACE_LDS entries each.
 struct TraceSpill {

@@ -25,9 +25,12 @@ namespace sppm {
 // ---------------------------------------------------------------------------
 // Shared camera sampler stream: UniformSample(ID) owns one mt19937 for the
 // whole render (CameraSampler.h:19-31); pixel i of a pass draws 3 doubles in
-// pixel order.  One wave keeps the 624-word state in LDS and twists it in
-// 64-word chunks in index order (new x[k] needs new x[k-227], which an earlier
-// chunk produced), emitting the tempered words for the pass.
+// pixel order, 6 words per pixel.  The block of a pass (6 * pixels words) is
+// cut into K segments of L words.  Workgroup k holds the state window where
+// its segment starts, emits the segment's tempered words, and then jumps its
+// window over the other K-1 segments (t^((K-1)L) mod P applied as a sum of
+// shifted windows), so that it holds the start of its segment of the next
+// pass.  The jump polynomial and the starting windows come from the host.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t temper(uint32_t z) {
     z ^= (z >> 11);
@@ -36,56 +39,64 @@ __device__ __forceinline__ uint32_t temper(uint32_t z) {
     z ^= (z >> 18);
     return z;
 }
-
-// 256 lanes twist the state in its three dependency phases: new x[k] for
-// k < 227 uses old words only; k in [227, 454) uses new x[k-227] from phase
-// 1; k in [454, 624) uses new x[k-227] from phase 2 (and k = 623 the new
-// x[0]).  Inside a phase every lane reads before any lane writes.
-constexpr int CAMW_THREADS = 256;
 __device__ __forceinline__ uint32_t mt_mix(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
     return c ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
 }
-__global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restrict__ st_mt, int* __restrict__ st_pos,
-                                                               uint32_t* __restrict__ words, long long need) {
-    __shared__ uint32_t x[624];
+constexpr int CAMW_THREADS = 256;
+constexpr int CAMW_RING = 1024;                    // ring of raw words while emitting (>= 624 + 227)
+constexpr int CAMW_SEQ = 624 + 19937 - 1;          // raw words a jump reads
+constexpr int CAMW_DEG = 19937;
+// x[m] for m >= 624 depends on x[m-624], x[m-623], x[m-227]: 227 new words
+// at a time depend only on earlier ones.
+__global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restrict__ states,
+                                                               const uint32_t* __restrict__ phi, long long L,
+                                                               uint32_t* __restrict__ words) {
+    __shared__ uint32_t ring[CAMW_RING];
+    __shared__ uint32_t xs[CAMW_SEQ];
     const int t = threadIdx.x;
-    for (int k = t; k < 624; k += CAMW_THREADS) x[k] = st_mt[k];
-    int pos = *st_pos;
+    uint32_t* st = states + (size_t)blockIdx.x * 624;
+    for (int k = t; k < 624; k += CAMW_THREADS) ring[k] = st[k];
     __syncthreads();
-    long long out = 0;
-    while (out < need) {
-        if (pos >= 624) {
-            uint32_t nv = 0;
-            // phase 1: k in [0, 227)
-            if (t < 227) nv = mt_mix(x[t], x[t + 1], x[t + 397]);
-            __syncthreads();
-            if (t < 227) x[t] = nv;
-            __syncthreads();
-            // phase 2: k in [227, 454)
-            if (t < 227) nv = mt_mix(x[227 + t], x[228 + t], x[t]);
-            __syncthreads();
-            if (t < 227) x[227 + t] = nv;
-            __syncthreads();
-            // phase 3: k in [454, 624)
-            if (t < 170) {
-                const int k = 454 + t;
-                nv = mt_mix(x[k], (k + 1 < 624) ? x[k + 1] : x[0], x[k - 227]);
-            }
-            __syncthreads();
-            if (t < 170) x[454 + t] = nv;
-            __syncthreads();
-            pos = 0;
+    uint32_t* out = words + (size_t)blockIdx.x * (size_t)L;
+    // segment words: x[624 + q] -> out[q]; each step writes ring slots
+    // [m0, m0 + 227) mod 1024, older than any slot read in the step
+    for (long long m0 = 624; m0 < 624 + L; m0 += 227) {
+        const long long m = m0 + t;
+        if (t < 227 && m < 624 + L) {
+            const uint32_t x = mt_mix(ring[(m - 624) & (CAMW_RING - 1)], ring[(m - 623) & (CAMW_RING - 1)],
+                                      ring[(m - 227) & (CAMW_RING - 1)]);
+            ring[m & (CAMW_RING - 1)] = x;
+            out[m - 624] = temper(x);
         }
-        long long rem = need - out;
-        int n = (624 - pos) < rem ? (624 - pos) : (int)rem;
-        for (int j = t; j < n; j += CAMW_THREADS) words[out + j] = temper(x[pos + j]);
-        out += n;
-        pos += n;
+        __syncthreads();
     }
+    // window after the segment: x[L .. L+623]; continue it linearly
+    for (int j = t; j < 624; j += CAMW_THREADS) xs[j] = ring[(L + j) & (CAMW_RING - 1)];
     __syncthreads();
-    for (int k = t; k < 624; k += CAMW_THREADS) st_mt[k] = x[k];
-    if (t == 0) *st_pos = pos;
+    for (int m0 = 624; m0 < CAMW_SEQ; m0 += 227) {
+        const int m = m0 + t;
+        if (t < 227 && m < CAMW_SEQ) xs[m] = mt_mix(xs[m - 624], xs[m - 623], xs[m - 227]);
+        __syncthreads();
+    }
+    // jumped window: word j = XOR of xs[i + j] over the coefficients phi_i = 1
+    const int j0 = t, j1 = t + CAMW_THREADS, j2 = t + 2 * CAMW_THREADS;
+    const bool has2 = j2 < 624;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    for (int c = 0; c < 624; ++c) {
+        uint32_t bits = phi[c];
+        if (c == 623) bits &= (1u << (CAMW_DEG - 623 * 32)) - 1u;  // degree < 19937
+        while (bits) {
+            const int i = c * 32 + __builtin_ctz(bits);
+            bits &= bits - 1u;
+            a0 ^= xs[i + j0];
+            a1 ^= xs[i + j1];
+            if (has2) a2 ^= xs[i + j2];
+        }
+    }
+    st[j0] = a0;
+    st[j1] = a1;
+    if (has2) st[j2] = a2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1296,8 +1307,8 @@ int trace_grid_blocks(int width) {
     return cus * nb;
 }
 
-void launch_camera_words(uint32_t* st_mt, int* st_pos, uint32_t* words, long long need, hipStream_t st) {
-    hipLaunchKernelGGL(k_camera_words, dim3(1), dim3(CAMW_THREADS), 0, st, st_mt, st_pos, words, need);
+void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long L, uint32_t* words, hipStream_t st) {
+    hipLaunchKernelGGL(k_camera_words, dim3(K), dim3(CAMW_THREADS), 0, st, states, phi, L, words);
 }
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,

@@ -29,7 +29,19 @@ EXPORTED = [
     "sppm_read_path_counts", "sppm_scene_stats", "sppm_guide_set", "sppm_guide_weights", "sppm_guide_gradient",
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
+    "sppm_mt19937_jump",
 ]
+
+
+def mt19937_jump(state, steps, lib=None):
+    """Window of 624 raw mt19937 words `steps` words further on (host only)."""
+    lib = lib or load_library()
+    st = np.ascontiguousarray(state, dtype=np.uint32)
+    assert st.shape == (624,)
+    out = np.zeros(624, np.uint32)
+    _check(lib.sppm_mt19937_jump(st.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint64(steps),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
 
 
 class TuneResult(C.Structure):
@@ -132,6 +144,7 @@ def load_library(path=LIB_PATH):
     for n in ("sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy"):
         getattr(lib, n).argtypes = [C.c_void_p]
     lib.sppm_autotune.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(TuneResult)]
+    lib.sppm_mt19937_jump.argtypes = [C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32)]
     lib.sppm_guide_weights.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_gradient.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]

@@ -300,6 +300,17 @@ int sppm_autotune(sppm_ctx* ctx, const sppm_scene_desc* scene, int passes, sppm_
  * the averages. */
 int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
 
+/* ---- host utility ----------------------------------------------------------- */
+/* mt19937 jump-ahead on the host (no counterpart in the reference, whose
+ * UniformSample stream, CameraSampler.h:19-27, is consumed sequentially; the
+ * context splits that stream the same way).  `state` is the window of the
+ * 624 most recent raw (untempered) words, oldest first -- for a freshly
+ * seeded std::mt19937 the 624 seed words; the next output is the tempered
+ * x[624] = x[397] ^ mix(x[0], x[1]).  `out` receives the window `steps` words
+ * further on (the low 31 bits of out[0] may differ from sequential
+ * generation; no later output reads them).  No GPU involved. */
+int sppm_mt19937_jump(const uint32_t* state, uint64_t steps, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif
