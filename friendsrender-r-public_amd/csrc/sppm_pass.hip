@@ -1,16 +1,20 @@
 // sppm_pass.hip -- the per-pass kernels of the SPPM core on gfx950.
 //
 // One pass (SPPM.cpp:770-843):
-//   k_camera_words   shared UniformSample stream words for this pass (1 wave)
-//   k_photon<W>      createPhotonMap: one lane per photon path, writes slots
+//   k_camera_words   shared UniformSample stream words for this pass (K
+//                    segments, each one workgroup, jump-ahead between passes)
+//   k_photon<W>      createPhotonMap: persistent, one lane per photon path at
+//                    a time, writes slots
 //   scan(count)      flatten offsets (SPPM.cpp:812-822 order)
-//   k_grid_params    cell size from the largest radius (device-side)
+//   k_grid_params    cell size from the mean radius (device-side)
 //   k_photon_keys    per flattened photon: cell key + rank (atomic count)
 //   scan(cells)      cell starts
 //   k_photon_scatter cell-sorted photon records (kdtree::CreateTree replacement)
 //   k_eye<W>         createHitpoints: one lane per pixel
-//   k_gather         applyHitPointInfo + progressive update, one lane per
-//                    hit point in 8x8 pixel tiles
+//   k_hit_keys ... k_gather_finish, k_group_stats
+//                    applyHitPointInfo + progressive update over Morton
+//                    groups of 64 hit points, chunked candidate lists
+//   k_gather         the same with neighbour-list recording (tests)
 //   k_reconstruct    reconstructImageFromSharedPoints
 #include <hip/hip_runtime.h>
 #include <stdint.h>
