@@ -8,6 +8,10 @@
 namespace sppm {
 
 constexpr int PATH_BLOCK = 64;     // lanes per block of the path kernels (1 wave)
+// Waves per SIMD the path kernels are compiled for: caps them at 128 VGPRs
+// (they would take 134-175 and run 2-3 waves per SIMD); traversal is latency
+// bound, and the fourth wave is worth more than the spared registers.
+constexpr int PATH_WAVES_PER_SIMD = 4;
 constexpr int GATHER_WAVES = 4096; // persistent waves of the chunked gather
 constexpr int GATHER_CHUNK_CAP = 65536;  // chunk budget per pass (partial sums buffer)
 

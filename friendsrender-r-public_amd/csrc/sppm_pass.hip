@@ -347,7 +347,7 @@ __device__ __forceinline__ void photon_finish(const SlotsDev& slots, const Photo
 // exits once the counter has passed N and its lanes are done.
 constexpr int PHOTON_REGEN = 16;
 template <int W, bool GUIDED>
-__global__ void __launch_bounds__(PATH_BLOCK) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
+__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
                                                        const int* __restrict__ step_dev, int cap,
                                                        uint32_t* __restrict__ spill, int* __restrict__ next_path,
                                                        GuideEmit G, uint32_t* __restrict__ flags) {
@@ -639,7 +639,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
 
 // Eye pass: a fixed grid of waves strides over the pixels.
 template <int W>
-__global__ void __launch_bounds__(PATH_BLOCK) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
+__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
                                                     const int* __restrict__ step_dev, int cap,
                                                     uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
                                                     uint32_t* __restrict__ flags) {
