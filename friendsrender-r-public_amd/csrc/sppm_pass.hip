@@ -47,7 +47,7 @@ __device__ __forceinline__ uint32_t mt_mix(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
     return c ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
 }
-constexpr int CAMW_THREADS = 256;
+constexpr int CAMW_THREADS = 1024;  // 16 waves: the jump's LDS reads are latency bound
 constexpr int CAMW_RING = 1024;                    // ring of raw words while emitting (>= 624 + 227)
 constexpr int CAMW_SEQ = 624 + 19937 - 1;          // raw words a jump reads
 constexpr int CAMW_DEG = 19937;
@@ -83,24 +83,38 @@ __global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restr
         if (t < 227 && m < CAMW_SEQ) xs[m] = mt_mix(xs[m - 624], xs[m - 623], xs[m - 227]);
         __syncthreads();
     }
-    // jumped window: word j = XOR of xs[i + j] over the coefficients phi_i = 1
-    const int j0 = t, j1 = t + CAMW_THREADS, j2 = t + 2 * CAMW_THREADS;
-    const bool has2 = j2 < 624;
-    uint32_t a0 = 0, a1 = 0, a2 = 0;
-    for (int c = 0; c < 624; ++c) {
-        uint32_t bits = phi[c];
-        if (c == 623) bits &= (1u << (CAMW_DEG - 623 * 32)) - 1u;  // degree < 19937
-        while (bits) {
-            const int i = c * 32 + __builtin_ctz(bits);
-            bits &= bits - 1u;
-            a0 ^= xs[i + j0];
-            a1 ^= xs[i + j1];
-            if (has2) a2 ^= xs[i + j2];
+    // jumped window: word j = XOR of xs[i + j] over the coefficients phi_i = 1;
+    // up to four coefficients per step keep four LDS reads in flight
+    if (t < 624) {
+        uint32_t acc = 0;
+        for (int c = 0; c < 624; ++c) {
+            uint32_t bits = phi[c];
+            if (c == 623) bits &= (1u << (CAMW_DEG - 623 * 32)) - 1u;  // degree < 19937
+            const uint32_t* row = xs + c * 32 + t;
+            while (bits) {
+                const int b0 = __builtin_ctz(bits);
+                bits &= bits - 1u;
+                uint32_t v = row[b0];
+                if (bits) {
+                    const int b1 = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    v ^= row[b1];
+                }
+                if (bits) {
+                    const int b2 = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    v ^= row[b2];
+                }
+                if (bits) {
+                    const int b3 = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    v ^= row[b3];
+                }
+                acc ^= v;
+            }
         }
+        st[t] = acc;
     }
-    st[j0] = a0;
-    st[j1] = a1;
-    if (has2) st[j2] = a2;
 }
 
 // ---------------------------------------------------------------------------
