@@ -744,9 +744,7 @@ static void do_guide(sppm_ctx* c, hipStream_t st) {
     launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hpass, st);
     guide_fold_hist(c->guide, c->gparams.history_decay, st);
     guide_hist_sum(c->guide, st);
-    // the log-sum-exp of the current weights is the one of the last inference
     for (int k = 0; k < c->gparams.train_steps; ++k) {
-        if (k > 0) guide_infer(c->guide, st);
         guide_gradient(c->guide, st);
         guide_adam(c->guide, c->gparams.learning_rate, st);
     }
@@ -1341,8 +1339,8 @@ int sppm_scene_stats(sppm_ctx* c, sppm_scene_info* info) {
 // ---- learned emission proposal ------------------------------------------------
 // He-uniform hidden layers from a splitmix64 stream, zero output layer: the
 // first proposal is uniform over the cells.
-static void guide_init_params(uint32_t seed, std::vector<float>& th) {
-    th.assign(GUIDE_NPARAMS, 0.0f);
+static void guide_init_params(const GuideNet& net, uint32_t seed, std::vector<float>& th) {
+    th.assign(guide_nparams(net), 0.0f);
     uint64_t x = 0x9E3779B97F4A7C15ull * ((uint64_t)seed + 1u);
     auto next = [&]() {
         uint64_t z = (x += 0x9E3779B97F4A7C15ull);
@@ -1351,9 +1349,11 @@ static void guide_init_params(uint32_t seed, std::vector<float>& th) {
         z ^= z >> 31;
         return (double)(z >> 11) * (1.0 / 9007199254740992.0);
     };
-    const double lim = sqrt(6.0 / GUIDE_HID);
-    for (int l = 1; l <= 4; ++l)
-        for (int k = 0; k < GUIDE_WSZ; ++k) th[guide_w_off(l) + k] = (float)((2.0 * next() - 1.0) * lim);
+    for (int l = 1; l <= net.layers; ++l) {
+        const double lim = sqrt(6.0 / guide_in(net, l));  // fan-in of the layer
+        for (int k = 0; k < guide_in(net, l) * net.hidden; ++k)
+            th[guide_w_off(net, l) + k] = (float)((2.0 * next() - 1.0) * lim);
+    }
 }
 
 // The proposal's light frame: exactly one untextured rectangle (two facets,
@@ -1414,12 +1414,21 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
     if (!(p->history_decay >= 0.0f && p->history_decay < 1.0f)) return fail(SPPM_E_ARG, "history_decay must be in [0, 1)");
     if (c->params.record_neighbours) return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0)");
+    GuideNet net;
+    net.hidden = p->hidden ? p->hidden : 64;
+    net.layers = p->layers ? p->layers : 4;
+    net.fp16 = p->fp16 ? 1 : 0;
+    if (!guide_net_ok(net)) return fail(SPPM_E_ARG, "network: hidden must be 64 or 128, layers 4 or 6");
     GuideEmit E{};
     int rc = guide_light_frame(c, E);
     if (rc) return rc;
+    if (c->guide.theta && (c->guide.net.hidden != net.hidden || c->guide.net.layers != net.layers ||
+                           c->guide.net.fp16 != net.fp16))
+        guide_free(c->guide);
+    c->guide.net = net;
     if (!c->guide.theta && guide_alloc(c->guide)) return fail(SPPM_E_NOMEM, "guide buffers");
     std::vector<float> th;
-    guide_init_params(p->seed, th);
+    guide_init_params(net, p->seed, th);
     if (guide_upload_params(c->guide, th.data(), c->stream)) return fail(SPPM_E_HIP, "guide upload");
     guide_clear_hist(c->guide, c->stream);
     guide_infer(c->guide, c->stream);
@@ -1454,7 +1463,7 @@ int sppm_guide_weights(sppm_ctx* c, float* out, const float* in) {
         if (guide_upload_params(c->guide, in, c->stream)) return fail(SPPM_E_HIP, "guide upload");
         guide_infer(c->guide, c->stream);
     }
-    if (out) HIPOK(hipMemcpyAsync(out, c->guide.theta, sizeof(float) * GUIDE_NPARAMS, hipMemcpyDeviceToHost, c->stream));
+    if (out) HIPOK(hipMemcpyAsync(out, c->guide.theta, sizeof(float) * c->guide.nparams, hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
     return SPPM_OK;
 }
@@ -1463,12 +1472,11 @@ int sppm_guide_gradient(sppm_ctx* c, const float* hist, float* loss, float* grad
     int rc = guide_ready(c);
     if (rc) return rc;
     if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
-    guide_infer(c->guide, c->stream);
     guide_hist_sum(c->guide, c->stream);
     guide_gradient(c->guide, c->stream);
     float sc[3];
     HIPOK(hipMemcpyAsync(sc, c->guide.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
-    if (grad) HIPOK(hipMemcpyAsync(grad, c->guide.grad, sizeof(float) * GUIDE_NPARAMS, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPOK(hipMemcpyAsync(grad, c->guide.grad, sizeof(float) * c->guide.nparams, hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
     if (loss) *loss = sc[2];
     return SPPM_OK;
@@ -1482,7 +1490,6 @@ int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
     const float lr = c->gparams.learning_rate > 0.0f ? c->gparams.learning_rate : 1e-3f;
     guide_hist_sum(c->guide, c->stream);
     for (int k = 0; k < steps; ++k) {
-        guide_infer(c->guide, c->stream);
         guide_gradient(c->guide, c->stream);
         guide_adam(c->guide, lr, c->stream);
     }

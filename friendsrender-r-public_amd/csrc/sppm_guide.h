@@ -2,13 +2,15 @@
 // reference; it replaces the position x direction sampling of
 // generateLightRay, SPPM/SPPM/SPPM.cpp:453-471, when guiding is on).
 //
-// A 64 -> 64x4 (ReLU) -> 1 MLP maps the Fourier features of a cell centre of
-// the 32x32 (light u, v) x 16x16 (hemi-octahedral direction s, t) grid to a
-// logit; the proposal is the softmax over the 262,144 cells, uniform inside a
-// cell.  Training minimises -sum_c W_c log P_c over the histogram W of what
-// the photons emitted from each cell deposited on visible hit points (hit
-// points gathering a photon x its luminance / PDF), decayed over passes.  Forward and backward run on bf16 MFMA (16x16x32) with float32
-// accumulation; Adam keeps float32 master weights.
+// An MLP (64 Fourier features -> `layers` hidden ReLU layers of `hidden`
+// units -> 1) maps the cell centre of the 32x32 (light u, v) x 16x16
+// (hemi-octahedral direction s, t) grid to a logit; the proposal is the
+// softmax over the 262,144 cells, uniform inside a cell.  Training minimises
+// -sum_c W_c log P_c over the histogram W of what the photons emitted from
+// each cell deposited on visible hit points (hit points gathering a photon x
+// its luminance / PDF), decayed over passes.  Forward and backward run on
+// MFMA 16x16x32 in bf16 or fp16 with float32 accumulation; Adam keeps float32
+// master weights.  Networks: hidden 64 or 128, layers 4 or 6.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,15 +19,31 @@ namespace sppm {
 
 constexpr int GUIDE_GU = 32, GUIDE_GV = 32, GUIDE_GS = 16, GUIDE_GT = 16;
 constexpr int GUIDE_CELLS = GUIDE_GU * GUIDE_GV * GUIDE_GS * GUIDE_GT;
-constexpr int GUIDE_FEAT = 64, GUIDE_HID = 64, GUIDE_LAYERS = 4;
-// parameter layout (float32): W1 [in][out], b1, W2, b2, W3, b3, W4, b4, w5, b5
-constexpr int GUIDE_WSZ = GUIDE_HID * GUIDE_HID;
-constexpr int guide_w_off(int l) { return (l - 1) * (GUIDE_WSZ + GUIDE_HID); }   // l = 1..4
-constexpr int guide_b_off(int l) { return guide_w_off(l) + GUIDE_WSZ; }
-constexpr int GUIDE_W5 = 4 * (GUIDE_WSZ + GUIDE_HID);
-constexpr int GUIDE_B5 = GUIDE_W5 + GUIDE_HID;
-constexpr int GUIDE_NPARAMS = GUIDE_B5 + 1;  // 16,705
-constexpr int GUIDE_BLOCKS = 256;            // training / inference grid (one wave per block)
+constexpr int GUIDE_FEAT = 64;  // 4 coordinates x 8 frequencies x (sin, cos)
+
+// Network shape.  Parameters (float32, in this order): for l = 1..layers
+// W_l [in_l][hidden] (in_1 = GUIDE_FEAT, else hidden) and b_l [hidden]; then
+// the output weights w_out [hidden] and bias b_out.
+struct GuideNet {
+    int hidden = 64, layers = 4, fp16 = 0;
+};
+constexpr int guide_in(const GuideNet& n, int l) { return l == 1 ? GUIDE_FEAT : n.hidden; }
+constexpr int guide_w_off(const GuideNet& n, int l) {
+    return l == 1 ? 0 : GUIDE_FEAT * n.hidden + n.hidden + (l - 2) * (n.hidden * n.hidden + n.hidden);
+}
+constexpr int guide_b_off(const GuideNet& n, int l) { return guide_w_off(n, l) + guide_in(n, l) * n.hidden; }
+constexpr int guide_wout_off(const GuideNet& n) { return guide_b_off(n, n.layers) + n.hidden; }
+constexpr int guide_nparams(const GuideNet& n) { return guide_wout_off(n) + n.hidden + 1; }
+// low-precision weight copies: layer l starts at sum over l' < l of in_l' * hidden
+constexpr int guide_lw_off(const GuideNet& n, int l) { return l == 1 ? 0 : GUIDE_FEAT * n.hidden + (l - 2) * n.hidden * n.hidden; }
+constexpr int guide_lw_size(const GuideNet& n) { return guide_lw_off(n, n.layers + 1); }
+constexpr bool guide_net_ok(const GuideNet& n) {
+    return (n.hidden == 64 || n.hidden == 128) && (n.layers == 4 || n.layers == 6) && (n.fp16 == 0 || n.fp16 == 1);
+}
+// gradients of the hidden layers are carried scaled by this power of two (no
+// fp16 subnormals for the ~1e-6 per-cell gradients; exact for bf16)
+constexpr float GUIDE_GRAD_SCALE = 1024.0f;
+constexpr int GUIDE_BLOCKS = 256;            // workgroups of the per-layer gradient kernels
 
 // Emission from the mixture of the plain emission (probability w) and the
 // proposal, for a scene whose only light is one untextured rectangle split
@@ -43,25 +61,30 @@ struct GuideEmit {
 };
 
 struct GuideBuffers {
+    GuideNet net;
+    int nparams = 0;
     float* theta = nullptr;     // master weights
     float* m = nullptr;         // Adam moments
     float* v = nullptr;
-    float* grad = nullptr;      // last gradient (+ loss at index NPARAMS)
-    uint16_t* wfwd = nullptr;   // bf16 W_l in [out][in] layout, 4 layers
-    uint16_t* wbwd = nullptr;   // bf16 W_l in [in][out] layout, 4 layers
+    float* grad = nullptr;      // last gradient
+    uint16_t* wfwd = nullptr;   // low-precision W_l in [hidden][in_l] layout
+    uint16_t* wbwd = nullptr;   // low-precision W_l in [in_l][hidden] layout
+    uint16_t* acts = nullptr;   // training forward: layers x cells x hidden activations
+    uint16_t* dh = nullptr;     // backward: 2 x cells x hidden (scaled) activation gradients
     float* logits = nullptr;    // per cell
     float* prob = nullptr;      // softmax
     float* cdf = nullptr;       // inclusive prefix sum of prob
     float* hist = nullptr;      // training histogram (decayed sum of the passes' deposits)
     float* hpass = nullptr;     // this pass's deposit histogram
-    float* part = nullptr;      // per-block partials: GUIDE_BLOCKS * (GUIDE_NPARAMS + 1)
+    float* part = nullptr;      // per-block gradient partials: GUIDE_BLOCKS * (nparams + 1)
     float* part2 = nullptr;     // per-block (max, sumexp), scan block sums, histogram partials
     float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss, [3] Adam steps taken
 };
 
+// Allocates for the network g.net (set it first).
 int guide_alloc(GuideBuffers& g);
 void guide_free(GuideBuffers& g);
-// theta (host, GUIDE_NPARAMS floats) -> device, Adam state reset, bf16 copies
+// theta (host, g.nparams floats) -> device, Adam state reset, low-precision copies
 int guide_upload_params(GuideBuffers& g, const float* theta, hipStream_t st);
 // Forward over all cells: logits, log-sum-exp, probabilities and their CDF.
 void guide_infer(GuideBuffers& g, hipStream_t st);

@@ -54,7 +54,13 @@ GUIDE_NPARAMS = 16705
 
 class GuideParams(C.Structure):
     _fields_ = [("enable", C.c_int32), ("mix_plain", C.c_float), ("learning_rate", C.c_float),
-                ("train_steps", C.c_int32), ("seed", C.c_uint32), ("history_decay", C.c_float)]
+                ("train_steps", C.c_int32), ("seed", C.c_uint32), ("history_decay", C.c_float),
+                ("hidden", C.c_int32), ("layers", C.c_int32), ("fp16", C.c_int32)]
+
+
+def guide_nparams(hidden=64, layers=4):
+    """Parameter count of the proposal network (64 features -> hidden x layers -> 1)."""
+    return 64 * hidden + hidden + (layers - 1) * (hidden * hidden + hidden) + hidden + 1
 
 
 class Attr(C.Structure):
@@ -309,22 +315,25 @@ class Renderer:
                     trial_ms=list(res.trial_ms))
 
     # ---- learned emission proposal --------------------------------------
-    def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9):
-        g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay)
+    def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9,
+                  hidden=64, layers=4, fp16=False):
+        g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay, hidden, layers,
+                        int(fp16))
         _check(self.lib.sppm_guide_set(self.h, C.byref(g)))
+        self.guide_np = guide_nparams(hidden, layers)
 
     def guide_weights(self, new=None):
-        out = np.zeros(GUIDE_NPARAMS, np.float32)
+        out = np.zeros(self.guide_np, np.float32)
         src = None
         if new is not None:
             new = np.ascontiguousarray(new, np.float32)
-            assert new.shape == (GUIDE_NPARAMS,)
+            assert new.shape == (self.guide_np,)
             src = _fp(new)
         _check(self.lib.sppm_guide_weights(self.h, _fp(out), src))
         return out
 
     def guide_gradient(self, hist=None):
-        grad = np.zeros(GUIDE_NPARAMS, np.float32)
+        grad = np.zeros(self.guide_np, np.float32)
         loss = C.c_float()
         hp = None
         if hist is not None:

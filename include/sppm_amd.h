@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPPM_ABI_VERSION 1
+#define SPPM_ABI_VERSION 2  /* 2: sppm_guide_params gained hidden, layers, fp16 */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -231,14 +231,15 @@ int sppm_scene_stats(sppm_ctx* ctx, sppm_scene_info* info);
  * carries (SPPM.cpp:527).  Photons are drawn from the mixture
  * q = w * plain + (1 - w) * proposal and carry PDF = q, so the estimate stays
  * consistent.  The proposal is a softmax over 32x32 light (u, v) cells x 16x16
- * hemi-octahedral direction cells whose logits come from a 64->64x4->1 MLP of
- * Fourier features of the cell centre, trained on the GPU (bf16 MFMA, Adam)
+ * hemi-octahedral direction cells whose logits come from an MLP of Fourier
+ * features of the cell centre (64 -> hidden x layers -> 1, ReLU), trained on
+ * the GPU (MFMA in bf16 or fp16 with float32 accumulation, Adam)
  * from the photons' deposits on visible hit points: per cell, the sum over the
  * photons emitted from it of (hit points that gather the photon) x (photon
  * luminance / PDF).
  * Scenes: exactly one untextured rectangular light made of two facets. */
 #define SPPM_GUIDE_CELLS 262144
-#define SPPM_GUIDE_NPARAMS 16705
+#define SPPM_GUIDE_NPARAMS 16705 /* parameters of the default network (hidden 64, 4 layers) */
 typedef struct sppm_guide_params {
     int32_t enable;        /* 0: plain emission */
     float mix_plain;       /* w, probability of the plain emission, in [0.25, 1] */
@@ -246,11 +247,15 @@ typedef struct sppm_guide_params {
     int32_t train_steps;   /* Adam steps after each pass's gather */
     uint32_t seed;         /* weight initialisation */
     float history_decay;   /* training histogram = decay * previous + this pass's deposits, in [0, 1) */
+    int32_t hidden;        /* units per hidden layer: 64 or 128 (0: 64) */
+    int32_t layers;        /* hidden layers: 4 or 6 (0: 4) */
+    int32_t fp16;          /* 1: fp16 MFMA, 0: bf16 */
 } sppm_guide_params;
+/* Parameter count: 64*hidden + hidden + (layers-1)*(hidden*hidden + hidden) + hidden + 1. */
 /* Enable (after sppm_scene_upload) or disable guided emission; re-initialises the weights. */
 int sppm_guide_set(sppm_ctx* ctx, const sppm_guide_params* params);
 /* Read (out) and/or replace (in, resets Adam) the float32 weights
- * (W1[in][out], b1, W2, b2, W3, b3, W4, b4, w5, b5). */
+ * (W1[in][out], b1, ..., W_layers, b_layers, w_out, b_out). */
 int sppm_guide_weights(sppm_ctx* ctx, float* out, const float* in);
 /* Loss and gradient of the current weights for a deposit histogram
  * (SPPM_GUIDE_CELLS floats; NULL: the last pass's histogram). */

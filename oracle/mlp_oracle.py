@@ -10,42 +10,66 @@ Cells: 32 x 32 light-surface (u, v) positions x 16 x 16 hemi-octahedral
 directions (s, t), 262,144 cells, index ((iu * 32 + iv) * 16 + is) * 16 + it.
 Input of a cell: its centre (2u-1, 2v-1, s, t), each coordinate encoded as
 sin / cos at 8 frequencies pi * 2^(k/2), k = 0..7 (64 features).
-Network: 64 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 1 logit; parameters in the
-order W1 [in][out], b1, W2, b2, W3, b3, W4, b4, w5, b5 (16,705 floats).
+Network: 64 features -> `layers` hidden ReLU layers of `hidden` units -> 1
+logit; parameters in the order W1 [in][out], b1, ..., W_L, b_L, w_out, b_out
+(the default, hidden 64 x 4 layers, has 16,705).  Networks are `Net`
+tuples; the module-level SHAPES / NPARAMS / HID / LAYERS describe the
+default.
 Proposal P(cell) = softmax over all cells of the logits.
 Loss over a deposit histogram W (luminance deposited by the photons emitted
 in each cell): L = -sum_c (W_c / sum W) log P_c, so dL/dlogit_c =
 P_c - W_c / sum W.
 
 Two evaluations: `exact` (float64, no rounding; pinned by finite
-differences in the tests) and `bf16` (what the MFMA kernels do: layer inputs
-and weights rounded to bfloat16, products accumulated in float32, the output
-layer in float32 on the last hidden layer before rounding).
+differences in the tests) and low precision, `bf16` or `fp16` (what the MFMA
+kernels do: layer inputs and weights rounded, products accumulated in
+float32, the output layer in float32 on the last hidden layer before
+rounding; the backward carries dL/dlogit scaled by GRAD_SCALE = 1024 through
+the rounded layers and divides the sums by it).
 """
+from collections import namedtuple
+
 import numpy as np
 
 GU, GV, GS, GT = 32, 32, 16, 16
 NCELLS = GU * GV * GS * GT
 NFREQ = 8
 FEAT = 4 * 2 * NFREQ
-HID = 64
-LAYERS = 4
-SHAPES = [("W1", (FEAT, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)), ("W3", (HID, HID)),
-          ("b3", (HID,)), ("W4", (HID, HID)), ("b4", (HID,)), ("w5", (HID,)), ("b5", (1,))]
-NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
+GRAD_SCALE = 1024.0
+Net = namedtuple("Net", "hidden layers dtype")
+DEFAULT = Net(64, 4, "bf16")
+WIDE = Net(128, 6, "fp16")
 
 
-def unpack(theta):
+def shapes(net=DEFAULT):
+    """Parameter names and shapes; the output layer is named w{L+1}, b{L+1}."""
+    out = []
+    for l in range(1, net.layers + 1):
+        out += [(f"W{l}", (FEAT if l == 1 else net.hidden, net.hidden)), (f"b{l}", (net.hidden,))]
+    return out + [(f"w{net.layers + 1}", (net.hidden,)), (f"b{net.layers + 1}", (1,))]
+
+
+def nparams(net=DEFAULT):
+    return sum(int(np.prod(s)) for _, s in shapes(net))
+
+
+HID = DEFAULT.hidden
+LAYERS = DEFAULT.layers
+SHAPES = shapes(DEFAULT)
+NPARAMS = nparams(DEFAULT)
+
+
+def unpack(theta, net=DEFAULT):
     out, o = {}, 0
-    for name, shp in SHAPES:
+    for name, shp in shapes(net):
         n = int(np.prod(shp))
         out[name] = theta[o:o + n].reshape(shp)
         o += n
     return out
 
 
-def pack(d):
-    return np.concatenate([np.asarray(d[name], np.float64).reshape(-1) for name, _ in SHAPES])
+def pack(d, net=DEFAULT):
+    return np.concatenate([np.asarray(d[name], np.float64).reshape(-1) for name, _ in shapes(net)])
 
 
 def freqs():
@@ -85,38 +109,53 @@ def to_bf16(x):
     return r.astype(np.uint32).view(np.float32)
 
 
-def init_params(seed):
+def to_fp16(x):
+    """Round float32 values to IEEE half (nearest even), returned as float32."""
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
+
+
+def rounder(mode, net):
+    if mode == "bf16" or (mode == "lowp" and net.dtype == "bf16"):
+        return to_bf16
+    if mode == "fp16" or (mode == "lowp" and net.dtype == "fp16"):
+        return to_fp16
+    raise ValueError(mode)
+
+
+def init_params(seed, net=DEFAULT):
     """He-uniform hidden layers, zero output layer (uniform proposal at start)."""
     rng = np.random.default_rng(seed)
     d = {}
-    for name, shp in SHAPES:
+    for name, shp in shapes(net):
         if name.startswith("W"):
             lim = np.sqrt(6.0 / shp[0])
             d[name] = rng.uniform(-lim, lim, shp)
         else:
             d[name] = np.zeros(shp)
-    return pack(d).astype(np.float32)
+    return pack(d, net).astype(np.float32)
 
 
-def forward(theta, X, mode="exact"):
+def forward(theta, X, mode="exact", net=DEFAULT):
     """Returns logits [n] and the list of post-ReLU hidden activations."""
-    p = unpack(np.asarray(theta, np.float64))
+    p = unpack(np.asarray(theta, np.float64), net)
+    L = net.layers
     if mode == "exact":
         h = np.asarray(X, np.float64)
         hs = [h]
-        for l in range(1, LAYERS + 1):
+        for l in range(1, L + 1):
             h = np.maximum(h @ p[f"W{l}"] + p[f"b{l}"], 0.0)
             hs.append(h)
-        return h @ p["w5"] + p["b5"][0], hs
-    h = to_bf16(X)
+        return h @ p[f"w{L + 1}"] + p[f"b{L + 1}"][0], hs
+    rnd = rounder(mode, net)
+    h = rnd(X)
     hs = [h]
-    for l in range(1, LAYERS + 1):
-        z = (h.astype(np.float32) @ to_bf16(p[f"W{l}"].astype(np.float32))).astype(np.float32) + \
+    for l in range(1, L + 1):
+        z = (h.astype(np.float32) @ rnd(p[f"W{l}"].astype(np.float32))).astype(np.float32) + \
             p[f"b{l}"].astype(np.float32)
         hf = np.maximum(z, np.float32(0.0))
         hs.append(hf)  # float32 (the output layer uses it unrounded)
-        h = to_bf16(hf)
-    logit = (hs[-1] @ p["w5"].astype(np.float32)).astype(np.float32) + np.float32(p["b5"][0])
+        h = rnd(hf)
+    logit = (hs[-1] @ p[f"w{L + 1}"].astype(np.float32)).astype(np.float32) + np.float32(p[f"b{L + 1}"][0])
     return logit, hs
 
 
@@ -126,37 +165,42 @@ def softmax(logits):
     return e / e.sum(), m + np.log(e.sum())
 
 
-def loss_and_grad(theta, X, W, mode="exact"):
+def loss_and_grad(theta, X, W, mode="exact", net=DEFAULT):
     """Loss and gradient (flat, like theta) for deposit histogram W."""
     W = np.asarray(W, np.float64)
     tot = W.sum()
-    logit, hs = forward(theta, X, mode)
+    L = net.layers
+    logit, hs = forward(theta, X, mode, net)
     P, lse = softmax(logit)
     loss = -np.sum(W * (np.asarray(logit, np.float64) - lse)) / tot
     g = P - W / tot  # dL/dlogit
-    p = unpack(np.asarray(theta, np.float64))
+    p = unpack(np.asarray(theta, np.float64), net)
     grads = {}
+    wo, bo = f"w{L + 1}", f"b{L + 1}"
     if mode == "exact":
-        grads["w5"] = hs[-1].T @ g
-        grads["b5"] = np.array([g.sum()])
-        dh = np.outer(g, p["w5"]) * (hs[-1] > 0)
-        for l in range(LAYERS, 0, -1):
+        grads[wo] = hs[-1].T @ g
+        grads[bo] = np.array([g.sum()])
+        dh = np.outer(g, p[wo]) * (hs[-1] > 0)
+        for l in range(L, 0, -1):
             grads[f"W{l}"] = hs[l - 1].T @ dh
             grads[f"b{l}"] = dh.sum(0)
             if l > 1:
                 dh = (dh @ p[f"W{l}"].T) * (hs[l - 1] > 0)
-        return loss, pack(grads)
-    g32 = g.astype(np.float32)
-    grads["w5"] = hs[-1].astype(np.float64).T @ g32.astype(np.float64)
-    grads["b5"] = np.array([g32.astype(np.float64).sum()])
-    dh = np.outer(g32, p["w5"].astype(np.float32)).astype(np.float32) * (hs[-1] > 0)
-    for l in range(LAYERS, 0, -1):
-        hin = hs[l - 1] if l == 1 else to_bf16(hs[l - 1])  # hs[0] is already rounded
-        grads[f"W{l}"] = to_bf16(hin).astype(np.float64).T @ to_bf16(dh).astype(np.float64)
-        grads[f"b{l}"] = dh.astype(np.float64).sum(0)
+        return loss, pack(grads, net)
+    rnd = rounder(mode, net)
+    S = GRAD_SCALE
+    g32 = (g * S).astype(np.float32)  # scaled dL/dlogit
+    hL = rnd(hs[-1])
+    grads[wo] = hL.astype(np.float64).T @ g32.astype(np.float64) / S
+    grads[bo] = np.array([g32.astype(np.float64).sum() / S])
+    dh = np.outer(g32, p[wo].astype(np.float32)).astype(np.float32) * (hL > 0)
+    for l in range(L, 0, -1):
+        hin = hs[l - 1] if l == 1 else rnd(hs[l - 1])  # hs[0] is already rounded
+        grads[f"W{l}"] = rnd(hin).astype(np.float64).T @ rnd(dh).astype(np.float64) / S
+        grads[f"b{l}"] = dh.astype(np.float64).sum(0) / S
         if l > 1:
-            dh = (to_bf16(dh) @ to_bf16(p[f"W{l}"].astype(np.float32)).T).astype(np.float32) * (hs[l - 1] > 0)
-    return loss, pack(grads)
+            dh = (rnd(dh) @ rnd(p[f"W{l}"].astype(np.float32)).T).astype(np.float32) * (rnd(hs[l - 1]) > 0)
+    return loss, pack(grads, net)
 
 
 def adam(theta, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

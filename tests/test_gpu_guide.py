@@ -1,6 +1,7 @@
 """GPU tests of the learned emission proposal, through the C ABI:
 - the MFMA forward/backward: loss, gradient and softmax against the CPU
-  oracle's bf16 evaluation on all 262,144 cells;
+  oracle's low-precision evaluation on all 262,144 cells, for the default
+  network (64 x 4, bf16) and the wide one (128 x 6, fp16);
 - Adam's first step in closed form;
 - guided emission is unbiased: with a fixed radius (alpha = 1) guided and
   plain SPPM estimate the same smoothed radiance;
@@ -85,6 +86,49 @@ def test_first_adam_step_is_sign_of_gradient(ctx, problem):
     # first bias-corrected step: -lr * g / (|g| + eps)
     np.testing.assert_allclose((t1 - theta)[big], -1e-3 * np.sign(g[big]), rtol=2e-3, atol=1e-7)
     assert np.all(t1[~big] - theta[~big] <= 1.1e-3)
+
+
+@pytest.fixture(scope="module")
+def wide_ctx(box_scene):
+    r = sppm_amd.Renderer(box_scene, scenes.pinhole_camera(32, 32), 0.9, 0.015, 2000, bvh_width=4)
+    r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=0, seed=7, hidden=128, layers=6, fp16=True)
+    yield r
+    r.close()
+
+
+def test_wide_fp16_gradient_and_softmax_match_oracle(wide_ctx):
+    net = mo.WIDE
+    rng = np.random.default_rng(12)
+    p = mo.unpack(mo.init_params(5, net).astype(np.float64), net)
+    p["w7"][:] = rng.normal(0, 0.5, net.hidden)
+    p["b7"][:] = 0.3
+    for l in range(1, net.layers + 1):
+        p[f"b{l}"][:] = rng.normal(0, 0.1, net.hidden)
+    theta = mo.pack(p, net).astype(np.float32)
+    hist = (rng.gamma(0.3, 1.0, mo.NCELLS) * (rng.random(mo.NCELLS) < 0.05)).astype(np.float32)
+    X = mo.features(mo.cell_coords())
+    wide_ctx.guide_weights(theta)
+    prob, _ = wide_ctx.guide_read()
+    loss, g = wide_ctx.guide_gradient(hist)
+    want_loss, want = mo.loss_and_grad(theta, X, hist, "fp16", net)
+    print("wide loss gpu %.6f oracle %.6f" % (loss, want_loss), flush=True)
+    assert abs(loss - want_loss) <= 1e-3 * abs(want_loss)
+    sh = mo.shapes(net)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in sh])
+    assert g.shape == (offs[-1],)
+    assert abs(g[-1]) < 1e-3
+    for k, (name, _) in enumerate(sh[:-1]):
+        a, b = g[offs[k]:offs[k + 1]].astype(np.float64), want[offs[k]:offs[k + 1]]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"  {name}: rel L2 error {err:.2e}", flush=True)
+        assert err < 2e-2, name
+    cos = g[:-1].astype(np.float64) @ want[:-1] / np.linalg.norm(g[:-1]) / np.linalg.norm(want[:-1])
+    assert cos > 0.9995
+    logit, _ = mo.forward(theta, X, "fp16", net)
+    wp, _ = mo.softmax(logit)
+    assert abs(prob.sum() - 1.0) < 1e-4
+    err = np.abs(prob - wp) / wp
+    assert np.quantile(err, 0.999) < 2e-2 and err.max() < 0.1
 
 
 def occluded_small():

@@ -1,6 +1,7 @@
 """Pins the CPU oracle of the learned emission proposal: the analytic
 gradient against central finite differences (float64), the bf16 evaluation
-against the exact one, Adam's first step in closed form, and the
+against the exact one (and the same for the wide fp16 network, 128 x 6),
+Adam's first step in closed form, and the
 hemi-octahedral direction map (round trip, Jacobian integrates to the
 hemisphere's 2*pi, Jacobian against a finite-difference area ratio)."""
 import numpy as np
@@ -60,6 +61,63 @@ def test_bf16_evaluation_tracks_exact(small_problem):
     assert abs(l0 - l1) < 2e-2 * abs(l0)
     cos = g0 @ g1 / np.linalg.norm(g0) / np.linalg.norm(g1)
     assert cos > 0.98  # bf16 rounding of 4 layers; a wrong term drops this far lower
+
+
+@pytest.fixture(scope="module")
+def wide_problem():
+    net = mo.WIDE
+    rng = np.random.default_rng(6)
+    cells = rng.choice(mo.NCELLS, 200, replace=False)
+    X = mo.features(mo.cell_coords(cells))
+    theta = mo.init_params(4, net).astype(np.float64)
+    p = mo.unpack(theta, net)
+    p["w7"][:] = rng.normal(0, 0.3, net.hidden)
+    p["b7"][:] = 0.1
+    theta = mo.pack(p, net)
+    W = rng.gamma(0.5, 1.0, len(cells)) * (rng.random(len(cells)) < 0.4)
+    return X, theta, W
+
+
+def test_wide_network_shapes():
+    net = mo.WIDE
+    assert [n for n, _ in mo.shapes(net)][-2:] == ["w7", "b7"]
+    assert mo.nparams(net) == 64 * 128 + 128 + 5 * (128 * 128 + 128) + 128 + 1
+    assert mo.nparams(mo.DEFAULT) == 16705 == mo.NPARAMS
+
+
+def test_wide_gradient_matches_finite_differences(wide_problem):
+    net = mo.WIDE
+    X, theta, W = wide_problem
+    loss, g = mo.loss_and_grad(theta, X, W, "exact", net)
+    rng = np.random.default_rng(10)
+    sh = mo.shapes(net)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in sh])
+    idx = np.concatenate([rng.choice(np.arange(offs[k], offs[k + 1]), min(6, offs[k + 1] - offs[k]), replace=False)
+                          for k in range(len(sh))])
+    eps = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        num = (mo.loss_and_grad(tp, X, W, "exact", net)[0] - mo.loss_and_grad(tm, X, W, "exact", net)[0]) / (2 * eps)
+        assert abs(num - g[i]) <= 1e-6 + 1e-5 * abs(num), (i, num, g[i])
+
+
+def test_fp16_evaluation_tracks_exact(wide_problem):
+    net = mo.WIDE
+    X, theta, W = wide_problem
+    l0, g0 = mo.loss_and_grad(theta, X, W, "exact", net)
+    l1, g1 = mo.loss_and_grad(theta.astype(np.float32), X, W, "fp16", net)
+    assert abs(l0 - l1) < 1e-2 * abs(l0)
+    cos = g0 @ g1 / np.linalg.norm(g0) / np.linalg.norm(g1)
+    assert cos > 0.99  # fp16 keeps 11 bits; the scaled gradients stay out of its subnormal range
+
+
+def test_fp16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, 65504.0, 1e-8], np.float32)
+    r = mo.to_fp16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2.0 ** -9)
+    assert r[3] == 65504.0 and r[4] == 0.0
 
 
 def test_bf16_rounding_is_nearest_even():
