@@ -59,6 +59,9 @@ def parse():
                     help="batched mode: many uneven seeded scenes, one graph launch per pass over all of them")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--guide-train", action="store_true",
+                    help="training rate of the learned proposal's networks (64x4 bf16, 128x6 fp16): Adam steps "
+                         "over all 262,144 cells on a fixed histogram")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the workload, run a device elementwise copy of MIB MiB (float32, torch) three "
                          "times: a kernel of known traffic for scaling the HBM byte counters of a profiler run")
@@ -277,12 +280,48 @@ def batch_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def guide_train_mode(args):
+    """Wall time per Adam step of the proposal network (training forward with
+    the activations kept, per-layer backward, gradient reduction, Adam) for
+    the default and the wide network; FLOPs counted as 6 x parameters x cells."""
+    import numpy as np
+    import scenes
+    import sppm_amd
+    box = scenes.box_scene()
+    rng = np.random.default_rng(1)
+    hist = (rng.gamma(0.3, 1.0, sppm_amd.GUIDE_CELLS) * (rng.random(sppm_amd.GUIDE_CELLS) < 0.05)).astype(np.float32)
+    steps = max(1, args.steps if args.steps != 255 else 20)
+    nets = {}
+    for hid, lay, f16 in ((64, 4, False), (128, 6, True)):
+        r = sppm_amd.Renderer(box, scenes.pinhole_camera(32, 32), 0.9, 0.015, 2000, bvh_width=4)
+        r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=0, seed=7, hidden=hid, layers=lay, fp16=f16)
+        r.guide_train(hist, 2)
+        r.sync()
+        t0 = time.perf_counter()
+        r.guide_train(hist, steps)
+        r.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        n = sppm_amd.guide_nparams(hid, lay)
+        tf = 6.0 * n * sppm_amd.GUIDE_CELLS / (ms * 1e-3) / 1e12
+        nets[f"{hid}x{lay}_{'fp16' if f16 else 'bf16'}"] = dict(params=n, ms_per_step=round(ms, 4),
+                                                                tflops=round(tf, 2), frac_of_2500=round(tf / 2500, 4))
+        log(f"network {hid}x{lay} {'fp16' if f16 else 'bf16'}: {ms:.3f} ms per Adam step, {tf:.1f} TFLOP/s")
+        r.close()
+    print(json.dumps({"metric": "proposal training: ms per Adam step over 262,144 cells", "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "steps": steps, "data": "synthetic histogram",
+                      "networks": nets}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+              f"(no multi-GPU path is built: photon and eye paths would split by index, the stored photons "
+              f"would be all-gathered)", file=sys.stderr)
         sys.exit(2)
+    if args.guide_train:
+        guide_train_mode(args)
+        return
     if args.rmse:
         rmse_mode(args)
         return

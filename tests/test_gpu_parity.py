@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
-Bar (north star): integer outputs bit-exact -- closest-hit facet ids (ties to
+Bar: integer outputs bit-exact -- closest-hit facet ids (ties to
 the lowest id), flattened photon lists (count and order), hit point types
 and facets, per-hit-point neighbour index sets, M / N / NumGather; flux and
 image within 1e-4 relative RMSE (FP32).  Geometry (photon positions,
