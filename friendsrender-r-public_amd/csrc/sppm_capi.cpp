@@ -83,6 +83,8 @@ struct SceneBuffers {
     int spill_blocks = 0;
     int* next_path = nullptr;   // path counter of the persistent photon pass
     std::vector<DevLight> hlights;  // host copy of the area lights
+    std::vector<float4> h_tuvw, h_tL;  // host copy of the textured lights' 1000-point tables
+    float h_lpick_avg = 1.0f;           // TableSampler average of the light pick table
     TraceSpill trace_spill() const { return TraceSpill{spill, spill_blocks, stack_cap}; }
     void release() {
         free_bvh(&bvh);
@@ -463,6 +465,9 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     HIPOK(hipMemcpyAsync(S.lpick_right, lright.data(), sizeof(int) * lright.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
     S.hlights = lights;
+    S.h_tuvw = tuvw;
+    S.h_tL = tL;
+    S.h_lpick_avg = lavg;
     if (!tuvw.empty()) {
         HIPOK(hipMemcpyAsync(S.ltab_uvw, tuvw.data(), sizeof(float4) * tuvw.size(), hipMemcpyHostToDevice, st));
         HIPOK(hipMemcpyAsync(S.ltab_L, tL.data(), sizeof(float4) * tL.size(), hipMemcpyHostToDevice, st));
@@ -1356,13 +1361,14 @@ static void guide_init_params(const GuideNet& net, uint32_t seed, std::vector<fl
     }
 }
 
-// The proposal's light frame: exactly one untextured rectangle (two facets,
-// equal radiance), so the plain emission is uniform on it.
+// The proposal's light frame: exactly one rectangle (two facets).  Untextured:
+// equal radiance, so the plain emission is uniform on it.  Textured: both
+// facets emit from their 1000-point tables.
 static int guide_light_frame(sppm_ctx* c, GuideEmit& E) {
     const auto& Ls = c->scene.hlights;
-    if (Ls.size() != 2 || Ls[0].textured || Ls[1].textured)
-        return fail(SPPM_E_SCENE, "guided emission needs exactly one untextured rectangular light (2 facets); scene has %d",
-                    (int)Ls.size());
+    if (Ls.size() != 2 || Ls[0].textured != Ls[1].textured)
+        return fail(SPPM_E_SCENE, "guided emission needs exactly one rectangular light (2 facets, both textured or "
+                    "both not); scene has %d emitting facets", (int)Ls.size());
     std::vector<H::V3> v;
     for (const DevLight& L : Ls)
         for (int k = 0; k < 3; ++k) {
@@ -1389,7 +1395,9 @@ static int guide_light_frame(sppm_ctx* c, GuideEmit& E) {
     const H::V3 n = H::normalized(H::cross(e1, e2));
     if (H::dot(n, ln) < 0.999f) return fail(SPPM_E_SCENE, "guided emission: light normal differs from the facet normals");
     const float r0 = Ls[0].flux[0] / Ls[0].area, r1 = Ls[1].flux[0] / Ls[1].area;
-    if (fabsf(r0 - r1) > 1e-5f * fabsf(r0)) return fail(SPPM_E_SCENE, "guided emission: the light facets differ in radiance");
+    if (!Ls[0].textured && fabsf(r0 - r1) > 1e-5f * fabsf(r0))
+        return fail(SPPM_E_SCENE, "guided emission: the light facets differ in radiance");
+    E.textured = Ls[0].textured;
     const H::V3 t1 = e1 / l1, t2 = H::cross(n, t1);
     for (int q = 0; q < 3; ++q) {
         E.P0[q] = (&P0.x)[q]; E.e1[q] = (&e1.x)[q]; E.e2[q] = (&e2.x)[q];
@@ -1399,6 +1407,73 @@ static int guide_light_frame(sppm_ctx* c, GuideEmit& E) {
     E.inv_e1 = 1.0f / H::norm2(e1);
     E.inv_e2 = 1.0f / H::norm2(e2);
     E.inv_area = 1.0f / (l1 * l2);
+    return SPPM_OK;
+}
+
+// Textured light: the table points grouped by position cell.  Point k of
+// facet f has plain probability pi = P(pick f) * value_k / (1000 * average)
+// (the pick and the point tables, LightSampler.cpp:68-76); its position cell
+// is fixed here once, so both mixture components use the same cell.
+static int guide_light_points(sppm_ctx* c, const GuideEmit& E) {
+    const auto& Ls = c->scene.hlights;
+    const auto& S = c->scene;
+    const int ncell = GUIDE_GU * GUIDE_GV;
+    std::vector<int> pt_cell(S.h_tuvw.size(), -1);
+    std::vector<std::vector<std::pair<int, int>>> cells(ncell);
+    std::vector<std::vector<double>> probs(ncell);
+    const H::V3 P0 = H::v3(E.P0[0], E.P0[1], E.P0[2]), e1 = H::v3(E.e1[0], E.e1[1], E.e1[2]),
+                e2 = H::v3(E.e2[0], E.e2[1], E.e2[2]);
+    for (int li = 0; li < 2; ++li) {
+        const DevLight& L = Ls[li];
+        const double pick = (double)L.flux_norm / ((double)S.h_lpick_avg * (double)Ls.size());
+        const H::V3 p0 = H::ld(L.p), p1 = H::ld(L.p + 3), p2 = H::ld(L.p + 6);
+        for (int k = 0; k < 1000; ++k) {
+            const int j = L.tab_off + k;
+            const float4 w = S.h_tuvw[j];
+            const double pi = pick * (double)w.w / ((double)L.tab_avg * 1000.0);
+            const H::V3 pos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
+            const H::V3 rel = pos - P0;
+            const float u = std::min(std::max(H::dot(rel, e1) * E.inv_e1, 0.0f), 0.99999994f);
+            const float v = std::min(std::max(H::dot(rel, e2) * E.inv_e2, 0.0f), 0.99999994f);
+            const int iu = std::min((int)(u * (float)GUIDE_GU), GUIDE_GU - 1);
+            const int iv = std::min((int)(v * (float)GUIDE_GV), GUIDE_GV - 1);
+            const int cell = iu * GUIDE_GV + iv;
+            pt_cell[j] = cell;
+            if (pi > 0.0) {
+                cells[cell].push_back({li, j});
+                probs[cell].push_back(pi);
+            }
+        }
+    }
+    std::vector<int> start(ncell + 1, 0);
+    std::vector<int2> pts;
+    std::vector<float> cum, mass(ncell, 0.0f);
+    for (int cc = 0; cc < ncell; ++cc) {
+        start[cc] = (int)pts.size();
+        double run = 0.0;
+        for (size_t q = 0; q < cells[cc].size(); ++q) {
+            run += probs[cc][q];
+            pts.push_back(make_int2(cells[cc][q].first, cells[cc][q].second));
+            cum.push_back((float)run);
+        }
+        mass[cc] = cum.empty() || cells[cc].empty() ? 0.0f : cum.back();
+    }
+    start[ncell] = (int)pts.size();
+    GuideBuffers& g = c->guide;
+    dfree(g.pt_cell); dfree(g.pcell_mass); dfree(g.pc_start); dfree(g.pc_pt); dfree(g.pc_cum);
+    g.pt_cell = nullptr; g.pcell_mass = nullptr; g.pc_start = nullptr; g.pc_pt = nullptr; g.pc_cum = nullptr;
+    int rc = 0;
+    if ((rc = dalloc(&g.pt_cell, pt_cell.size())) || (rc = dalloc(&g.pcell_mass, ncell)) ||
+        (rc = dalloc(&g.pc_start, ncell + 1)) || (rc = dalloc(&g.pc_pt, std::max<size_t>(1, pts.size()))) ||
+        (rc = dalloc(&g.pc_cum, std::max<size_t>(1, cum.size()))))
+        return rc;
+    HIPOK(hipMemcpy(g.pt_cell, pt_cell.data(), sizeof(int) * pt_cell.size(), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(g.pcell_mass, mass.data(), sizeof(float) * ncell, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(g.pc_start, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice));
+    if (!pts.empty()) {
+        HIPOK(hipMemcpy(g.pc_pt, pts.data(), sizeof(int2) * pts.size(), hipMemcpyHostToDevice));
+        HIPOK(hipMemcpy(g.pc_cum, cum.data(), sizeof(float) * cum.size(), hipMemcpyHostToDevice));
+    }
     return SPPM_OK;
 }
 
@@ -1427,6 +1502,18 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
         guide_free(c->guide);
     c->guide.net = net;
     if (!c->guide.theta && guide_alloc(c->guide)) return fail(SPPM_E_NOMEM, "guide buffers");
+    if (E.textured) {
+        if ((rc = guide_light_points(c, E))) return rc;
+        E.pt_cell = c->guide.pt_cell;
+        E.pcell_mass = c->guide.pcell_mass;
+        E.pc_start = c->guide.pc_start;
+        E.pc_pt = c->guide.pc_pt;
+        E.pc_cum = c->guide.pc_cum;
+    } else {
+        GuideBuffers& g = c->guide;
+        dfree(g.pt_cell); dfree(g.pcell_mass); dfree(g.pc_start); dfree(g.pc_pt); dfree(g.pc_cum);
+        g.pt_cell = nullptr; g.pcell_mass = nullptr; g.pc_start = nullptr; g.pc_pt = nullptr; g.pc_cum = nullptr;
+    }
     std::vector<float> th;
     guide_init_params(net, p->seed, th);
     if (guide_upload_params(c->guide, th.data(), c->stream)) return fail(SPPM_E_HIP, "guide upload");

@@ -46,9 +46,13 @@ constexpr float GUIDE_GRAD_SCALE = 1024.0f;
 constexpr int GUIDE_BLOCKS = 256;            // workgroups of the per-layer gradient kernels
 
 // Emission from the mixture of the plain emission (probability w) and the
-// proposal, for a scene whose only light is one untextured rectangle split
-// into two triangles: position P0 + u e1 + v e2, direction in the frame
-// (t1, t2, n) through the hemi-octahedral map.
+// proposal, for a scene whose only light is one rectangle split into two
+// triangles: position P0 + u e1 + v e2, direction in the frame (t1, t2, n)
+// through the hemi-octahedral map.  An untextured light emits from any point
+// (the proposal is uniform inside a cell); a textured one emits from its
+// 1000-point tables (LightSampler.cpp:30-66), so the proposal picks a table
+// point inside the chosen position cell with the plain emission's own
+// probabilities, and position cells without points get no proposal mass.
 struct GuideEmit {
     float P0[3], e1[3], e2[3], n[3], t1[3], t2[3];
     float inv_e1, inv_e2;  // 1 / |e1|^2, 1 / |e2|^2
@@ -58,6 +62,13 @@ struct GuideEmit {
     const float* prob;     // proposal over cells (unnormalised: total = cdf[last])
     const float* cdf;
     int* path_cell;        // emission cell of each photon path (training record)
+    // textured light only
+    int textured;
+    const int* pt_cell;        // position cell of each table point (indexed like the light tables)
+    const float* pcell_mass;   // plain probability of the table points of each position cell
+    const int* pc_start;       // points of position cell c: entries [pc_start[c], pc_start[c + 1])
+    const int2* pc_pt;         // (light, table entry) of each point, in cell order
+    const float* pc_cum;       // inclusive cumulative plain probability within the cell
 };
 
 struct GuideBuffers {
@@ -79,6 +90,12 @@ struct GuideBuffers {
     float* part = nullptr;      // per-block gradient partials: GUIDE_BLOCKS * (nparams + 1)
     float* part2 = nullptr;     // per-block (max, sumexp), scan block sums, histogram partials
     float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss, [3] Adam steps taken
+    // textured light: table points by position cell (see GuideEmit)
+    int* pt_cell = nullptr;
+    float* pcell_mass = nullptr;  // also masks the proposal: cells of zero mass get probability 0
+    int* pc_start = nullptr;
+    int2* pc_pt = nullptr;
+    float* pc_cum = nullptr;
 };
 
 // Allocates for the network g.net (set it first).

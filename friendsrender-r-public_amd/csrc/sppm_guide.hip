@@ -263,9 +263,10 @@ __global__ void k_guide_lse(const float* __restrict__ part2, int nparts, float* 
 }
 
 // probabilities and their inclusive prefix sum (per block of 1024, then offsets)
+// mask (textured light): cells whose position cell holds no table point get 0
 __global__ void __launch_bounds__(256) k_guide_prob(const float* __restrict__ logits, const float* __restrict__ scal,
                                                     float* __restrict__ prob, float* __restrict__ cdf,
-                                                    float* __restrict__ bsum) {
+                                                    float* __restrict__ bsum, const float* __restrict__ mask) {
     __shared__ float ws[4];
     const int t = threadIdx.x, base = blockIdx.x * 1024 + 4 * t;
     const float lse = scal[0];
@@ -273,6 +274,7 @@ __global__ void __launch_bounds__(256) k_guide_prob(const float* __restrict__ lo
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         v[k] = expf(logits[base + k] - lse);
+        if (mask && !(mask[(base + k) / (GUIDE_GS * GUIDE_GT)] > 0.0f)) v[k] = 0.0f;
         prob[base + k] = v[k];
         run += v[k];
         v[k] = run;  // thread-local inclusive
@@ -600,7 +602,8 @@ int guide_alloc(GuideBuffers& g) {
 void guide_free(GuideBuffers& g) {
     for (void* p : {(void*)g.theta, (void*)g.m, (void*)g.v, (void*)g.grad, (void*)g.wfwd, (void*)g.wbwd,
                     (void*)g.acts, (void*)g.dh, (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.hist,
-                    (void*)g.hpass, (void*)g.part, (void*)g.part2, (void*)g.scal})
+                    (void*)g.hpass, (void*)g.part, (void*)g.part2, (void*)g.scal, (void*)g.pt_cell,
+                    (void*)g.pcell_mass, (void*)g.pc_start, (void*)g.pc_pt, (void*)g.pc_cum})
         if (p) (void)hipFree(p);
     const GuideNet n = g.net;
     g = GuideBuffers();
@@ -677,7 +680,7 @@ void guide_infer(GuideBuffers& g, hipStream_t st) {
     with_net(g.net, [&](auto op) { op.forward(g, false, st); });
     hipLaunchKernelGGL(k_guide_lse, dim3(1), dim3(256), 0, st, g.part2, FWD_BLOCKS, g.scal);
     hipLaunchKernelGGL(k_guide_prob, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.logits, g.scal, g.prob, g.cdf,
-                       g.part2);
+                       g.part2, (const float*)g.pcell_mass);
     hipLaunchKernelGGL(k_guide_cdf_offsets, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.cdf, g.part2);
 }
 

@@ -200,6 +200,21 @@ __device__ __forceinline__ void photon_start(const SceneDev& S, const SlotsDev& 
 // drawn (position, direction) and use it as the photon's PDF; the photon
 // flux is L_e cos, as for plain emission, so flux / PDF stays an unbiased
 // weight whichever component drew the sample.
+//
+// Textured light: positions are the table points.  Point k has plain
+// probability pi_k and the plain PDF D_k * cos / pi, where D_k = pi_k * 1000 /
+// area is the reference's area density of the point (LightSampler.cpp:70-76);
+// the proposal picks a cell c, then a point of c's position cell with
+// probability pi_k / mass(c).  The PDF is D_k * (w cos / pi + (1 - w) P(c)
+// d(omega | c) / mass(c)), the mixture's probability of (k, omega) over pi_k
+// times D_k, which keeps the plain emission's expectation.
+__device__ __forceinline__ int cdf_search(const float* __restrict__ cdf, int lo, int hi, float x) {
+    while (lo < hi) {  // first entry > x in [lo, hi]
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] > x) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
 __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const SlotsDev& slots, const GuideEmit& G,
                                                     int N, int step, int i, PhotonPath& P, uint32_t* ovf) {
     P.i = i;
@@ -207,60 +222,104 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
     const f3 P0 = mk(G.P0[0], G.P0[1], G.P0[2]), E1 = mk(G.e1[0], G.e1[1], G.e1[2]), E2 = mk(G.e2[0], G.e2[1], G.e2[2]);
     const f3 Nn = mk(G.n[0], G.n[1], G.n[2]), T1 = mk(G.t1[0], G.t1[1], G.t1[2]), T2 = mk(G.t2[0], G.t2[1], G.t2[2]);
     const float total = G.cdf[GUIDE_CELLS - 1];
-    f3 lpos, outdir;
+    f3 lpos, outdir, rad0;
+    int tj = -1, tl = 0;  // textured: table entry and light of the point
     if (real1(P.m, ovf) < (double)G.w) {
-        // plain: light pick + PointOnTriangle + cosine lobe (SPPM.cpp:453-468)
+        // plain: light pick, then PointOnTriangle or a table point, cosine lobe (SPPM.cpp:453-468)
         const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
         const DevLight& L = S.lights[li];
         f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
-        float u = (float)real1(P.m, ovf);
-        float v = (float)real1(P.m, ovf);
-        if (v > u) { u *= 0.5f; v -= u; }
-        else { v *= 0.5f; u -= v; }
-        lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
+        if (G.textured) {
+            const int off = L.tab_off;
+            const int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, P.m, ovf);
+            const float4 w = S.ltab_uvw[off + k];
+            lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
+            tj = off + k;
+            tl = li;
+        } else {
+            float u = (float)real1(P.m, ovf);
+            float v = (float)real1(P.m, ovf);
+            if (v > u) { u *= 0.5f; v -= u; }
+            else { v *= 0.5f; u -= v; }
+            lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
+        }
         float pdf_dir = 1.0f;
         outdir = normalized(cos_sampling(P.m, ovf, Nn, &pdf_dir));
     } else {
-        // proposal: a cell by its CDF, then uniform inside the cell
-        const float x = (float)real1(P.m, ovf) * total;
-        int lo = 0, hi = GUIDE_CELLS - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (G.cdf[mid] > x) hi = mid; else lo = mid + 1;
-        }
-        const int c = lo;
+        // proposal: a cell by its CDF, then a position and a direction inside it
+        // x < total strictly, so the cell drawn has probability > 0 (masked cells have none)
+        const float x = fminf((float)real1(P.m, ovf) * total, __uint_as_float(__float_as_uint(total) - 1u));
+        const int c = cdf_search(G.cdf, 0, GUIDE_CELLS - 1, x);
         const int it = c % GUIDE_GT, is = (c / GUIDE_GT) % GUIDE_GS, iv = (c / (GUIDE_GS * GUIDE_GT)) % GUIDE_GV,
                   iu = c / (GUIDE_GV * GUIDE_GS * GUIDE_GT);
-        const float u = ((float)iu + (float)real1(P.m, ovf)) / (float)GUIDE_GU;
-        const float v = ((float)iv + (float)real1(P.m, ovf)) / (float)GUIDE_GV;
+        if (G.textured) {
+            // a table point of the position cell, by the plain probabilities
+            const int pc = iu * GUIDE_GV + iv;
+            const int a = G.pc_start[pc], e = G.pc_start[pc + 1];
+            const float x = (float)real1(P.m, ovf) * G.pcell_mass[pc];
+            const int q = e > a ? cdf_search(G.pc_cum, a, e - 1, x) : a;
+            const int2 pt = G.pc_pt[q];
+            const DevLight& L = S.lights[pt.x];
+            f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
+            const float4 w = S.ltab_uvw[pt.y];
+            lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
+            tj = pt.y;
+            tl = pt.x;
+        } else {
+            const float u = ((float)iu + (float)real1(P.m, ovf)) / (float)GUIDE_GU;
+            const float v = ((float)iv + (float)real1(P.m, ovf)) / (float)GUIDE_GV;
+            lpos = (P0 + E1 * u) + E2 * v;
+        }
         const float s = -1.0f + ((float)is + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GS);
         const float t = -1.0f + ((float)it + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GT);
-        lpos = (P0 + E1 * u) + E2 * v;
         const float px = 0.5f * (s + t), py = 0.5f * (s - t);
         const float pz = fmaxf(0.0f, 1.0f - fabs_(px) - fabs_(py));
         outdir = normalized((T1 * px + T2 * py) + Nn * pz);
     }
     // mixture density at (lpos, outdir)
-    const f3 rel = lpos - P0;
-    const float u = fminf(fmaxf(dot(rel, E1) * G.inv_e1, 0.0f), 0.99999994f);
-    const float v = fminf(fmaxf(dot(rel, E2) * G.inv_e2, 0.0f), 0.99999994f);
     const float lx = dot(outdir, T1), ly = dot(outdir, T2), lz = dot(outdir, Nn);
     const float cs = fmaxf(lz, 0.0f);
     const float l1 = fabs_(lx) + fabs_(ly) + cs;
     const float ox = lx / l1, oy = ly / l1, oz = cs / l1;  // octahedron point (|ox|+|oy|+oz = 1)
     const float s = ox + oy, t = ox - oy;
-    const int iu = min((int)(u * (float)GUIDE_GU), GUIDE_GU - 1), iv = min((int)(v * (float)GUIDE_GV), GUIDE_GV - 1);
     const int is = min(max((int)((s + 1.0f) * (0.5f * (float)GUIDE_GS)), 0), GUIDE_GS - 1);
     const int it = min(max((int)((t + 1.0f) * (0.5f * (float)GUIDE_GT)), 0), GUIDE_GT - 1);
-    const int cell = ((iu * GUIDE_GV + iv) * GUIDE_GS + is) * GUIDE_GT + it;
+    int pcell;
+    if (G.textured) {
+        pcell = G.pt_cell[tj];
+    } else {
+        const f3 rel = lpos - P0;
+        const float u = fminf(fmaxf(dot(rel, E1) * G.inv_e1, 0.0f), 0.99999994f);
+        const float v = fminf(fmaxf(dot(rel, E2) * G.inv_e2, 0.0f), 0.99999994f);
+        const int iu = min((int)(u * (float)GUIDE_GU), GUIDE_GU - 1), iv = min((int)(v * (float)GUIDE_GV), GUIDE_GV - 1);
+        pcell = iu * GUIDE_GV + iv;
+    }
+    const int cell = (pcell * GUIDE_GS + is) * GUIDE_GT + it;
     const float plen = sqrtf(ox * ox + oy * oy + oz * oz);
-    // cell density in (u, v, s, t) -> area x solid angle: / A, * 2 |P|^3 (d omega = ds dt / (2 |P|^3))
-    const float p_mlp = G.prob[cell] / total * (float)(GUIDE_GU * GUIDE_GV) * (float)(GUIDE_GS * GUIDE_GT / 4) *
-                        G.inv_area * (2.0f * plen * plen * plen);
-    const float p_plain = G.inv_area * cs / SPPM_PI;
-    const float q = G.w * p_plain + (1.0f - G.w) * p_mlp;
+    // direction density inside a cell: 1 / (cell area in (s, t)) * 2 |P|^3 (d omega = ds dt / (2 |P|^3))
+    const float dd = (float)(GUIDE_GS * GUIDE_GT / 4) * (2.0f * plen * plen * plen);
+    const float pc_prob = G.prob[cell] / total;
+    float q;
+    if (G.textured) {
+        const DevLight& L = S.lights[tl];
+        const float4 w = S.ltab_uvw[tj];
+        const float4 Lk = S.ltab_L[tj];
+        // D_k as the plain emission computes it (photon_start)
+        const float gp = w.w / L.tab_avg / 1000.0f;
+        const float Dk = ((gp * 1000.0f) / L.area) * (L.flux_norm / S.lpick_avg / (float)S.nlights);
+        const float mass = G.pcell_mass[pcell];
+        const float prop = mass > 0.0f ? pc_prob * dd / mass : 0.0f;
+        q = Dk * (G.w * cs / SPPM_PI + (1.0f - G.w) * prop);
+        rad0 = mk(Lk.x, Lk.y, Lk.z);
+    } else {
+        // cell density in (u, v, s, t) -> area x solid angle: / A
+        const float p_mlp = pc_prob * (float)(GUIDE_GU * GUIDE_GV) * G.inv_area * dd;
+        const float p_plain = G.inv_area * cs / SPPM_PI;
+        q = G.w * p_plain + (1.0f - G.w) * p_mlp;
+        rad0 = mk(G.rad[0], G.rad[1], G.rad[2]);
+    }
     G.path_cell[i] = cell;
-    const f3 rad = mk(G.rad[0], G.rad[1], G.rad[2]) * smin(1.0f, cs);
+    const f3 rad = rad0 * smin(1.0f, cs);
     P.rad = rad;
     P.cur_org = lpos + Nn * SPPM_RAY_OFFSET;
     P.cur_dir = outdir;

@@ -4,7 +4,8 @@
   network (64 x 4, bf16) and the wide one (128 x 6, fp16);
 - Adam's first step in closed form;
 - guided emission is unbiased: with a fixed radius (alpha = 1) guided and
-  plain SPPM estimate the same smoothed radiance;
+  plain SPPM estimate the same smoothed radiance, for an untextured light
+  and for a textured one (emission from its 1000-point tables);
 - what the network gains: on the occluded-light scene the per-pass variance
   of the estimate is lower with the trained proposal than with plain
   emission."""
@@ -161,6 +162,23 @@ def test_guided_is_unbiased_at_fixed_radius():
     print("mean L plain %.5f guided %.5f" % (mp, mg), flush=True)
     assert abs(mg - mp) < 0.05 * mp
     # coarse blocks agree within noise
+    bp = plain.reshape(6, 8, 6, 8, 3).mean((1, 3))
+    bg = guided.reshape(6, 8, 6, 8, 3).mean((1, 3))
+    assert np.median(np.abs(bg - bp) / np.maximum(bp, 1e-6)) < 0.1
+
+
+def test_guided_textured_light_is_unbiased(full_scene_small):
+    """Full-material scene, whose only light is a textured quad: the proposal
+    picks table points inside its position cells.  Fixed radius, mean and
+    coarse blocks of the smoothed radiance against plain emission."""
+    cam = scenes.pinhole_camera(48, 48)
+    r0 = 0.05
+    plain = render(full_scene_small, cam, r0, 60000, 50, False)
+    guided = render(full_scene_small, cam, r0, 60000, 50, True)
+    mp, mg = plain.mean(), guided.mean()
+    print("textured light: mean L plain %.5f guided %.5f" % (mp, mg), flush=True)
+    assert np.isfinite(guided).all()
+    assert abs(mg - mp) < 0.05 * mp
     bp = plain.reshape(6, 8, 6, 8, 3).mean((1, 3))
     bg = guided.reshape(6, 8, 6, 8, 3).mean((1, 3))
     assert np.median(np.abs(bg - bp) / np.maximum(bp, 1e-6)) < 0.1
