@@ -59,6 +59,13 @@ def parse():
                     help="batched mode: many uneven seeded scenes, one graph launch per pass over all of them")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--caustic", action="store_true",
+                    help="caustic workload: ~2M triangles of GGX glass on a 2048^2-textured floor, small textured "
+                         "light, CAMERAS thin-lens cameras at RES^2 batched per graph launch; plain and guided "
+                         "(128 x 6 fp16 network) photons/s")
+    ap.add_argument("--cameras", type=int, default=8)
+    ap.add_argument("--caustic-res", type=int, default=2048)
+    ap.add_argument("--caustic-photons", type=int, default=4_000_000)
     ap.add_argument("--guide-train", action="store_true",
                     help="training rate of the learned proposal's networks (64x4 bf16, 128x6 fp16): Adam steps "
                          "over all 262,144 cells on a fixed histogram")
@@ -280,6 +287,67 @@ def batch_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def caustic_mode(args):
+    """Aggregate photons/s of K passes over the caustic workload's camera
+    variants, one graph launch per pass (each camera a context and a graph
+    branch), with plain emission and with the guided proposal (128 x 6 fp16
+    network, mixture 0.5, one Adam step per pass)."""
+    import numpy as np
+    import scenes
+    import sppm_amd
+    t0 = time.perf_counter()
+    scene = scenes.caustic_scene()
+    cams = scenes.caustic_cameras(args.cameras, args.caustic_res, args.caustic_res)
+    N = args.caustic_photons
+    alpha, r0 = 0.7, 0.01 * scene.diagonal()
+    log(f"caustic scene: {scene.nfacets} triangles, generated in {time.perf_counter() - t0:.1f} s")
+    K, W = args.steps, args.warmup
+    out = {}
+    for mode in ("plain", "guided"):
+        rs = []
+        for k, cam in enumerate(cams):
+            r = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=8)
+            if mode == "guided":
+                r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=1, seed=11 + k, history_decay=0.95,
+                            hidden=128, layers=6, fp16=True)
+            rs.append(r)
+        log(f"{mode}: {len(rs)} contexts set up")
+        batch = sppm_amd.Batch(rs)
+        for _ in range(W):
+            batch.pass_()
+        batch.sync()
+        tb = time.perf_counter()
+        for k in range(K):
+            batch.pass_()
+            if (k + 1) % 5 == 0:
+                batch.sync()
+                log(f"{mode}: {k + 1}/{K} passes")
+        batch.sync()
+        el = time.perf_counter() - tb
+        c = rs[0].counters()
+        img = rs[0].reconstruct()
+        out[mode] = dict(photons_per_s=round(len(rs) * N * K / el, 1), ms_per_pass_all_cameras=round(el / K * 1e3, 3),
+                         camera0_hit_points_per_pass=c["hit_points"] // max(1, c["passes"]),
+                         camera0_neighbours_per_pass=c["neighbours"] // max(1, c["passes"]),
+                         camera0_image_finite=bool(np.isfinite(img).all()))
+        log(f"{mode}: {out[mode]['photons_per_s']:.4g} photons/s, {out[mode]['ms_per_pass_all_cameras']} ms per pass")
+        batch.close()
+        for r in rs:
+            r.close()
+    line = {
+        "metric": "photons/s (caustic workload, aggregate over cameras, one graph launch per pass)",
+        "value": out["plain"]["photons_per_s"], "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (MLP fp16 MFMA)",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "caustic", "triangles": scene.nfacets, "cameras": len(cams),
+                   "resolution": f"{args.caustic_res}x{args.caustic_res}", "photons_per_pass_per_camera": N,
+                   "alpha": alpha, "r0": round(r0, 6), "bvh_width": 8, "network": "128x6 fp16",
+                   "mix_plain": 0.5},
+        "plain": out["plain"], "guided": out["guided"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def guide_train_mode(args):
     """Wall time per Adam step of the proposal network (training forward with
     the activations kept, per-layer backward, gradient reduction, Adam) for
@@ -321,6 +389,9 @@ def main():
         sys.exit(2)
     if args.guide_train:
         guide_train_mode(args)
+        return
+    if args.caustic:
+        caustic_mode(args)
         return
     if args.rmse:
         rmse_mode(args)

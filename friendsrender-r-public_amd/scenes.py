@@ -541,3 +541,56 @@ def batch_workload(n=64, seed=2024, tri_range=(10_000, 500_000), max_meshes=8, r
         out.append(dict(scene=sc, cam=pinhole_camera(res, res), photons=int(ppp * res * res), alpha=0.7,
                         r0=0.02 * sc.diagonal()))
     return out
+
+
+# ---- caustic workload: glass objects on a textured floor, small textured light --
+def caustic_scene(seed=5, tris=2_000_000, tex_size=2048, nobj=4):
+    """Caustic-heavy workload: `nobj` displaced geodesic meshes of GGX glass
+    (IOR 1.5, roughness 0.05), ~tris triangles in total, on a diffuse floor
+    with tex_size^2 procedural albedo / roughness / normal textures, lit by a
+    0.05 x 0.05 textured light (emission up to 200) 1.0 above the floor."""
+    rng = np.random.default_rng(seed)
+    albedo, rough, normal = noise_textures(tex_size, seed)
+    light_tex = checker_texture(n=256, cells=8, lo=128, hi=255)
+    attrs = [
+        dict(name="floor", kind=MAT_DIFFUSE, albedo=(0.8, 0.8, 0.8), is_sppm=1, if_opaque=1, tex_albedo=0,
+             tex_roughness=1, tex_bump=2),                                                                   # 0
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(1.0, 1.0, 1.0), radiance=(200.0, 200.0, 200.0), is_sppm=1,
+             if_opaque=1, tex_albedo=3),                                                                     # 1
+        dict(name="glass", kind=MAT_GGX_GLASS, albedo=(0.98, 0.98, 0.98), roughness=0.05, index=1.5, is_sppm=0,
+             if_opaque=0),                                                                                   # 2
+    ]
+    b = _Builder()
+    b.quad((-1.0, 0.0, -1.0), (-1.0, 0.0, 2.0), (2.0, 0.0, 2.0), (2.0, 0.0, -1.0), (0, 1, 0), 0)
+    h = 0.025
+    b.quad((0.5 - h, 1.0, 0.5 - h), (0.5 - h, 1.0, 0.5 + h), (0.5 + h, 1.0, 0.5 + h), (0.5 + h, 1.0, 0.5 - h),
+           (0, -1, 0), 1)
+    base = b.scene(attrs)
+    freq = max(2, int(round(np.sqrt(tris / nobj / 20.0))))
+    parts = [base]
+    for k in range(nobj):
+        ang = 2.0 * np.pi * (k + rng.uniform(-0.15, 0.15)) / nobj
+        r = float(rng.uniform(0.14, 0.2))
+        c = (0.5 + 0.32 * np.cos(ang), r * 1.05, 0.5 + 0.32 * np.sin(ang))
+        V, N, T = displaced_mesh(c, r, freq, seed * 101 + k, amp=0.15 * r)
+        parts.append(Scene(V.reshape(-1, 9), N.reshape(-1, 9), T.reshape(-1, 9), np.full(len(V), 2, np.int32),
+                           attrs, []))
+    return Scene(np.concatenate([p.pos for p in parts]), np.concatenate([p.nrm for p in parts]),
+                 np.concatenate([p.uv for p in parts]), np.concatenate([p.attr for p in parts]), attrs,
+                 [albedo, rough, normal, light_tex])
+
+
+def caustic_cameras(n=8, width=2048, height=2048, aperture=0.02):
+    """`n` thin-lens variants on a circle around the objects, each focused on
+    the scene centre (0.5, 0.15, 0.5)."""
+    cams = []
+    tgt = np.array([0.5, 0.15, 0.5])
+    for k in range(n):
+        ang = 2.0 * np.pi * (k + 0.5) / n
+        pos = tgt + np.array([1.6 * np.cos(ang), 0.75, 1.6 * np.sin(ang)])
+        focus = float(np.linalg.norm(pos - tgt))
+        c = thinlens_camera(width, height, aperture=aperture, focus=focus, fov_deg=40.0, sampler_id=k)
+        c["pos"] = tuple(float(x) for x in pos)
+        c["target"] = tuple(float(x) for x in tgt)
+        cams.append(c)
+    return cams
