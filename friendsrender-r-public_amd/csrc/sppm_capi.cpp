@@ -1180,6 +1180,122 @@ int sppm_sync(sppm_ctx* c) {
     return SPPM_OK;
 }
 
+// ---- checkpoint / resume --------------------------------------------------------
+// Everything a pass carries to the next: the photon slots (a path's stale slot
+// is flattened again, SPPM.cpp:812-822), the shared hit points, the grid's
+// radius statistic, the counters and flags, the camera stream (segment
+// windows, pre-generated word blocks and their counters), the pass counter,
+// and the learned proposal's weights, Adam state and histogram.  Layout: a
+// header, then the arrays in the order of ckpt_sections.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version;
+    int32_t N, W, H, np, nslot, cam_K;
+    int64_t cam_L;
+    int32_t guided, nparams, pass;
+    int64_t words_gen, words_used;
+};
+struct CkptSection {
+    void* dev;
+    size_t bytes;
+};
+}  // namespace
+
+static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
+    RenderBuffers& R = c->rb;
+    const size_t NS = (size_t)NSLOT * R.N;
+    std::vector<CkptSection> v = {
+        {R.slots.pos, sizeof(float4) * NS}, {R.slots.flux, sizeof(float4) * NS}, {R.slots.dir, sizeof(float4) * NS},
+        {R.slots.valid, NS}, {R.shared.rad, sizeof(float4) * R.np}, {R.shared.cnt, sizeof(int2) * R.np},
+        {R.rstat, 2 * sizeof(unsigned long long)}, {R.ctr, sizeof(Counters)}, {R.flags, sizeof(uint32_t)},
+        {R.pass_dev, sizeof(int)}, {R.words, sizeof(uint32_t) * 2 * 6 * (size_t)R.np}};
+    if (R.cam_states) v.push_back({R.cam_states, sizeof(uint32_t) * (size_t)R.cam_K * MT_WORDS});
+    if (c->guided) {
+        GuideBuffers& g = c->guide;
+        v.push_back({g.theta, sizeof(float) * g.nparams});
+        v.push_back({g.m, sizeof(float) * g.nparams});
+        v.push_back({g.v, sizeof(float) * g.nparams});
+        v.push_back({g.hist, sizeof(float) * GUIDE_CELLS});
+        v.push_back({g.scal, sizeof(float) * 8});
+    }
+    return v;
+}
+
+static CkptHeader ckpt_header(sppm_ctx* c) {
+    CkptHeader h{};
+    memcpy(h.magic, "SPPMCKPT", 8);
+    h.version = 1;
+    h.N = c->rb.N; h.W = c->rb.W; h.H = c->rb.H; h.np = c->rb.np; h.nslot = NSLOT;
+    h.cam_K = c->rb.cam_K; h.cam_L = c->rb.cam_L;
+    h.guided = c->guided ? 1 : 0;
+    h.nparams = c->guided ? c->guide.nparams : 0;
+    h.pass = c->pass;
+    h.words_gen = c->words_gen; h.words_used = c->words_used;
+    return h;
+}
+
+int sppm_checkpoint_size(sppm_ctx* c, int64_t* bytes) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!bytes) return fail(SPPM_E_ARG, "bytes is NULL");
+    size_t n = sizeof(CkptHeader);
+    for (const CkptSection& s : ckpt_sections(c)) n += s.bytes;
+    *bytes = (int64_t)n;
+    return SPPM_OK;
+}
+
+int sppm_checkpoint_save(sppm_ctx* c, void* buf, int64_t bytes) {
+    int64_t need = 0;
+    int rc = sppm_checkpoint_size(c, &need);
+    if (rc) return rc;
+    if (!buf || bytes < need) return fail(SPPM_E_ARG, "checkpoint buffer: %lld bytes needed", (long long)need);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamSynchronize(c->stream2));
+    const CkptHeader h = ckpt_header(c);
+    char* p = static_cast<char*>(buf);
+    memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    for (const CkptSection& s : ckpt_sections(c)) {
+        HIPOK(hipMemcpy(p, s.dev, s.bytes, hipMemcpyDeviceToHost));
+        p += s.bytes;
+    }
+    return SPPM_OK;
+}
+
+int sppm_checkpoint_load(sppm_ctx* c, const void* buf, int64_t bytes) {
+    int64_t need = 0;
+    int rc = sppm_checkpoint_size(c, &need);
+    if (rc) return rc;
+    if (!buf || bytes < (int64_t)sizeof(CkptHeader)) return fail(SPPM_E_ARG, "checkpoint buffer too small");
+    CkptHeader h;
+    memcpy(&h, buf, sizeof h);
+    const CkptHeader mine = ckpt_header(c);
+    if (memcmp(h.magic, "SPPMCKPT", 8) || h.version != 1) return fail(SPPM_E_ARG, "not a checkpoint of this format");
+    if (h.N != mine.N || h.W != mine.W || h.H != mine.H || h.np != mine.np || h.nslot != mine.nslot ||
+        h.cam_K != mine.cam_K || h.cam_L != mine.cam_L || h.guided != mine.guided || h.nparams != mine.nparams)
+        return fail(SPPM_E_STATE, "checkpoint was taken with another resolution, photon count, camera sampler or "
+                    "emission setup");
+    if (bytes < need) return fail(SPPM_E_ARG, "checkpoint buffer: %lld bytes needed", (long long)need);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamSynchronize(c->stream2));
+    const char* p = static_cast<const char*>(buf) + sizeof h;
+    for (const CkptSection& s : ckpt_sections(c)) {
+        HIPOK(hipMemcpy(s.dev, p, s.bytes, hipMemcpyHostToDevice));
+        p += s.bytes;
+    }
+    c->pass = h.pass;
+    c->words_gen = h.words_gen;
+    c->words_used = h.words_used;
+    c->eye_recorded[0] = c->eye_recorded[1] = false;  // both streams are idle
+    c->rb.have_lists = false;
+    if (c->guided) {
+        guide_restore(c->guide, c->stream);
+        HIPOK(hipStreamSynchronize(c->stream));
+    }
+    return SPPM_OK;
+}
+
 int sppm_kernel_ms(sppm_ctx* c, const char* name, float* ms) {
     if (!c || !name || !ms) return fail(SPPM_E_ARG, "NULL argument");
     Timer* t = nullptr;

@@ -114,6 +114,9 @@ void guide_gradient(GuideBuffers& g, hipStream_t st);
 // Adam step with g.grad (skipped on the device when the histogram is empty).
 void guide_adam(GuideBuffers& g, float lr, hipStream_t st);
 void guide_clear_hist(GuideBuffers& g, hipStream_t st);
+// After theta, Adam state, histogram and scalars were copied in: the
+// low-precision weight copies and the proposal.
+void guide_restore(GuideBuffers& g, hipStream_t st);
 // hist = decay * hist + hpass; hpass = 0
 void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st);
 

@@ -706,6 +706,11 @@ void guide_adam(GuideBuffers& g, float lr, hipStream_t st) {
     launch_lowp(g, st);
 }
 
+void guide_restore(GuideBuffers& g, hipStream_t st) {
+    launch_lowp(g, st);
+    guide_infer(g, st);
+}
+
 void guide_clear_hist(GuideBuffers& g, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, GUIDE_CELLS);
     hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hpass, GUIDE_CELLS);

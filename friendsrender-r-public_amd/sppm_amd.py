@@ -29,7 +29,7 @@ EXPORTED = [
     "sppm_read_path_counts", "sppm_scene_stats", "sppm_guide_set", "sppm_guide_weights", "sppm_guide_gradient",
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
-    "sppm_mt19937_jump",
+    "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
 ]
 
 
@@ -151,6 +151,9 @@ def load_library(path=LIB_PATH):
         getattr(lib, n).argtypes = [C.c_void_p]
     lib.sppm_autotune.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(TuneResult)]
     lib.sppm_mt19937_jump.argtypes = [C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32)]
+    lib.sppm_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.sppm_checkpoint_save.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    lib.sppm_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     lib.sppm_guide_weights.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_gradient.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_guide_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
@@ -354,6 +357,20 @@ class Renderer:
         hist = np.zeros(GUIDE_CELLS, np.float32)
         _check(self.lib.sppm_guide_read(self.h, _fp(prob), _fp(hist)))
         return prob, hist
+
+    def checkpoint(self):
+        """Bytes of a checkpoint of the render state (see sppm_checkpoint_save)."""
+        n = C.c_int64()
+        _check(self.lib.sppm_checkpoint_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _check(self.lib.sppm_checkpoint_save(self.h, buf.ctypes.data_as(C.c_void_p), n))
+        return buf, self.passes, self.total_photons
+
+    def restore(self, ckpt):
+        buf, passes, total = ckpt
+        buf = np.ascontiguousarray(buf, np.uint8)
+        _check(self.lib.sppm_checkpoint_load(self.h, buf.ctypes.data_as(C.c_void_p), C.c_int64(buf.size)))
+        self.passes, self.total_photons = passes, total
 
     def scene_stats(self):
         info = SceneInfo()

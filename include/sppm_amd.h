@@ -305,6 +305,20 @@ int sppm_autotune(sppm_ctx* ctx, const sppm_scene_desc* scene, int passes, sppm_
  * the averages. */
 int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
 
+/* ---- checkpoint / resume ------------------------------------------------------
+ * No counterpart in the reference, whose only persistent state between passes
+ * is SharedHitPoint (SPPM.h:112-126) and the photon slot array
+ * (SPPM.cpp:753), never saved.  A checkpoint holds what a pass carries to the
+ * next: photon slots, shared hit points, radius statistic, counters, camera
+ * stream state, pass counter and, when guided, the proposal's weights, Adam
+ * state and histogram.  Load it into a context set up the same way (scene,
+ * camera, parameters, guided emission) and begun; the render continues as if
+ * never interrupted.  The caller keeps the emitted-photon total for
+ * sppm_reconstruct (passes x photons per pass). */
+int sppm_checkpoint_size(sppm_ctx* ctx, int64_t* bytes);
+int sppm_checkpoint_save(sppm_ctx* ctx, void* buf, int64_t bytes);
+int sppm_checkpoint_load(sppm_ctx* ctx, const void* buf, int64_t bytes);
+
 /* ---- host utility ----------------------------------------------------------- */
 /* mt19937 jump-ahead on the host (no counterpart in the reference, whose
  * UniformSample stream, CameraSampler.h:19-27, is consumed sequentially; the
