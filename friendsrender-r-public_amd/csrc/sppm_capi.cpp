@@ -143,8 +143,8 @@ struct RenderBuffers {
         dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
         dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_count); dfree(cell_start); dfree(cell_scan_tmp);
         dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
-        dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb);
-        dfree(shared.rad); dfree(shared.cnt); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
+        dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
+        dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
         dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
         ready = false;
     }
@@ -313,6 +313,8 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         for (int t : {A.tex_albedo, A.tex_roughness, A.tex_bump})
             if (t < -1 || t >= d->ntextures) return fail(SPPM_E_SCENE, "attribute %d: texture id %d out of range", a, t);
     }
+    if (d->has_ibl && (d->ibl_texture < 0 || d->ibl_texture >= d->ntextures))
+        return fail(SPPM_E_SCENE, "environment map texture id %d out of range", d->ibl_texture);
     for (int t = 0; t < d->ntextures; ++t)
         if (d->textures[t].width <= 0 || d->textures[t].height <= 0 || !d->textures[t].rgba)
             return fail(SPPM_E_SCENE, "texture %d is empty", t);
@@ -411,14 +413,26 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         D.tex_albedo = A.tex_albedo; D.tex_rough = A.tex_roughness; D.tex_bump = A.tex_bump;
         D.radiance_norm2 = H::norm2(H::ld(A.radiance));
     }
-    // ---- textures ----
-    std::vector<TexDesc> texd(std::max(1, d->ntextures));
+    // ---- textures (+ the environment map after IBL::Init's conversion
+    // 0.4 * c^2, LightSampler.cpp:22-26, appended as one more texture) ----
+    const int ntex_all = d->ntextures + (d->has_ibl ? 1 : 0);
+    std::vector<TexDesc> texd(std::max(1, ntex_all));
     long long ntexel = 0;
-    for (int t = 0; t < d->ntextures; ++t) {
-        texd[t].width = d->textures[t].width;
-        texd[t].height = d->textures[t].height;
+    for (int t = 0; t < ntex_all; ++t) {
+        const sppm_texture& T = d->textures[t < d->ntextures ? t : d->ibl_texture];
+        texd[t].width = T.width;
+        texd[t].height = T.height;
         texd[t].offset = ntexel;
-        ntexel += (long long)d->textures[t].width * d->textures[t].height;
+        ntexel += (long long)T.width * T.height;
+    }
+    std::vector<float4> ibl_conv;
+    if (d->has_ibl) {
+        const sppm_texture& T = d->textures[d->ibl_texture];
+        ibl_conv.resize((size_t)T.width * T.height);
+        for (size_t k = 0; k < ibl_conv.size(); ++k) {
+            const float* c = T.rgba + 4 * k;
+            ibl_conv[k] = make_float4(0.4f * (c[0] * c[0]), 0.4f * (c[1] * c[1]), 0.4f * (c[2] * c[2]), 0.4f * 0.0f);
+        }
     }
     // ---- facets ----
     std::vector<float2> uv2(3 * (size_t)nf);
@@ -460,6 +474,11 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     for (int t = 0; t < d->ntextures; ++t)
         HIPOK(hipMemcpyAsync(S.texels + texd[t].offset, d->textures[t].rgba,
                              sizeof(float4) * (size_t)texd[t].width * texd[t].height, hipMemcpyHostToDevice, st));
+    if (d->has_ibl) {
+        HIPOK(hipMemcpyAsync(S.texels + texd[d->ntextures].offset, ibl_conv.data(), sizeof(float4) * ibl_conv.size(),
+                             hipMemcpyHostToDevice, st));
+        HIPOK(hipStreamSynchronize(st));  // ibl_conv is a local
+    }
     HIPOK(hipMemcpyAsync(S.lpick_prob, lprob.data(), sizeof(float) * lprob.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lpick_left, lleft.data(), sizeof(int) * lleft.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lpick_right, lright.data(), sizeof(int) * lright.size(), hipMemcpyHostToDevice, st));
@@ -497,6 +516,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     D.attrs = S.attrs;
     D.tex = S.tex;
     D.texels = S.texels;
+    D.ibl_tex = d->has_ibl ? d->ntextures : -1;
     D.nlights = (int)lights.size();
     D.lpick_avg = lavg;
     D.lpick_prob = S.lpick_prob;
@@ -650,7 +670,8 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
         (rc = dalloc(&R.hits.nrm, np)) || (rc = dalloc(&R.hits.out, np)) || (rc = dalloc(&R.hits.weight, np)) ||
         (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
-        (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
+        (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.shared.direct, np)) ||
+        (rc = dalloc(&R.hits.direct, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
         (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_phi, MT_WORDS)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
@@ -825,6 +846,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
                                GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
                                R.rstat, R.ctr, R.flags, c->guided ? R.pdep : nullptr, st);
     }
+    if (c->scene.dev.ibl_tex >= 0) launch_direct_light(R.hits, R.shared, R.np, st);
     return SPPM_OK;
 }
 
@@ -1208,6 +1230,7 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
     std::vector<CkptSection> v = {
         {R.slots.pos, sizeof(float4) * NS}, {R.slots.flux, sizeof(float4) * NS}, {R.slots.dir, sizeof(float4) * NS},
         {R.slots.valid, NS}, {R.shared.rad, sizeof(float4) * R.np}, {R.shared.cnt, sizeof(int2) * R.np},
+        {R.shared.direct, sizeof(float4) * R.np},
         {R.rstat, 2 * sizeof(unsigned long long)}, {R.ctr, sizeof(Counters)}, {R.flags, sizeof(uint32_t)},
         {R.pass_dev, sizeof(int)}, {R.words, sizeof(uint32_t) * 2 * 6 * (size_t)R.np}};
     if (R.cam_states) v.push_back({R.cam_states, sizeof(uint32_t) * (size_t)R.cam_K * MT_WORDS});
@@ -1400,6 +1423,31 @@ int sppm_read_shared(sppm_ctx* c, float* radiance, float* radius, int32_t* num_p
         if (radius) radius[i] = r[i].w;
         if (num_photon) num_photon[i] = k[i].x;
         if (num_gather) num_gather[i] = k[i].y;
+    }
+    return SPPM_OK;
+}
+
+int sppm_read_direct(sppm_ctx* c, float* hit_direct, float* shared_direct, int32_t* num_direct) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(c->stream));
+    const size_t np = (size_t)c->rb.np;
+    std::vector<float4> h(np), sh(np);
+    HIPOK(hipMemcpy(h.data(), c->rb.hits.direct, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(sh.data(), c->rb.shared.direct, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    const bool ibl = c->scene.dev.ibl_tex >= 0;
+    for (size_t i = 0; i < np; ++i) {
+        if (hit_direct) {  // written by the eye pass only with an environment light
+            hit_direct[3 * i] = ibl ? h[i].x : 0.0f;
+            hit_direct[3 * i + 1] = ibl ? h[i].y : 0.0f;
+            hit_direct[3 * i + 2] = ibl ? h[i].z : 0.0f;
+        }
+        if (shared_direct) {
+            shared_direct[3 * i] = sh[i].x;
+            shared_direct[3 * i + 1] = sh[i].y;
+            shared_direct[3 * i + 2] = sh[i].z;
+        }
+        if (num_direct) memcpy(&num_direct[i], &sh[i].w, sizeof(int32_t));  // NumDirectLight, int bits
     }
     return SPPM_OK;
 }

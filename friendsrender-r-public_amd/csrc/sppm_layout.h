@@ -82,6 +82,8 @@ struct SceneDev {
     const float* __restrict__ ltab_prob;
     const int* __restrict__ ltab_left;
     const int* __restrict__ ltab_right;
+    // environment light: texture id of the converted map (0.4 c^2), -1 none
+    int ibl_tex;
 };
 
 // ---- camera (iCamera, Camera.h:11-53), precomputed on the host ------------
@@ -129,12 +131,14 @@ struct HitDev {
     float4* weight;  // (PixelWeight.xyz, reflectance)
     int2* mat;       // (kind, facet id)
     float* absorb;   // extinction
+    float4* direct;  // (DirectLight.xyz, 0): the environment seen by escaping eye paths
 };
 
 // ---- SharedHitPoint (SPPM.h:112-126) --------------------------------------
 struct SharedDev {
     float4* rad;   // (Radiance.xyz, Radius)
     int2* cnt;     // (NumPhoton, NumGather)
+    float4* direct;  // (DirectLight.xyz, NumDirectLight as int bits)
 };
 
 }  // namespace sppm

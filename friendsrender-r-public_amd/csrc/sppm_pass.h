@@ -80,6 +80,8 @@ int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();
 void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st);
+// Adds the hit points' DirectLight into the shared hit points (environment light only).
+void launch_direct_light(const HitDev& H, const SharedDev& SH, int np, hipStream_t st);
 void launch_reconstruct(const SharedDev& SH, int np, long long total, int passes, float4* out, hipStream_t st);
 void launch_init_shared(const SharedDev& SH, int np, float r0, hipStream_t st);
 void launch_init_slots(const SlotsDev& s, hipStream_t st);

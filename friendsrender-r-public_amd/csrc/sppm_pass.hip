@@ -607,6 +607,22 @@ __device__ inline void camera_ray(const CamDev& c, float u, float v, f3 smp, f3&
 // could only add IBL radiance (zero), so the path stops there.
 // ---------------------------------------------------------------------------
 // createHitpoints body for pixel i (SPPM.cpp:330-349 with getHitpointInfo).
+// IBL::GetUVFromDirection, SPHERE mapping (IBL.h:90-103; unqualified acos on a
+// float is the double function, u and v narrowed to float), then the
+// bilinear fetch of GetIBLColorFromDirection (IBL.h:122-129).
+__device__ __forceinline__ f3 ibl_color(const SceneDev& S, f3 dir) {
+    float u = 0.0f, v = 0.0f;
+    if (!((double)fabs_(dir.x) < 1e-6 && (double)fabs_(dir.z) < 1e-6)) {
+        const f3 xz = normalized(mk(dir.x, 0.0f, dir.z));
+        const double a = acos((double)xz.x);
+        u = (float)((xz.z >= 0.0f ? a : (double)(2.0f * SPPM_PI) - a) / (double)(2.0f * SPPM_PI));
+        v = (float)(acos((double)dir.y) / (double)SPPM_PI);
+    }
+    const TexDesc td = S.tex[S.ibl_tex];
+    const float4 c = tex_bilinear(S.texels + td.offset, td.width, td.height, u, v);
+    return mk(c.x, c.y, c.z);
+}
+
 template <int W>
 __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int i,
                                          const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl) {
@@ -637,6 +653,8 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
     int type = 0, facet = -1, kind = 0;
     f3 hpos = mk(0, 0, 0), hout = mk(0, 1, 0), hw = mk(1, 1, 1), hn = mk(0, 0, 0);
     float hidx = 1.0f, hrough = 1.0f, habs = 1.0f, hrefl = 1.0f;
+    f3 direct = mk(0.0f, 0.0f, 0.0f);
+    const bool ibl = S.ibl_tex >= 0;
     for (;;) {
         if (depth > 6) break;
         else if (depth > 4) {
@@ -646,6 +664,8 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         }
         HitRec h;
         if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, ts, h, &fl)) {
+            // escape: the environment's colour (SPPM.cpp:290-300)
+            if (ibl) direct = direct + (ibl_color(S, cur_dir) * T) / PDF;
             if (type == 0) type = 2;
             break;
         }
@@ -658,7 +678,8 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
             cur_org = g.pos + cur_dir * SPPM_RAY_OFFSET;
             continue;
         }
-        if (g.is_sppm) {  // makeHitPoint (SPPM.cpp:79-96)
+        if (g.is_sppm && type != 1) {  // makeHitPoint (SPPM.cpp:79-96, 171-174)
+            direct = mk(0.0f, 0.0f, 0.0f);  // the new HitPoint's DirectLight
             type = 1;
             facet = h.facet;
             hpos = g.pos;
@@ -670,7 +691,9 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
             hrough = g.rough;
             habs = g.absorb;
             hrefl = g.refl;
-            break;
+            // the path goes on after its hit point only to reach the
+            // environment; without one nothing later changes the hit point
+            if (!ibl) break;
         }
         float Ii, Io = 1.0f;
         if (near1(cur_index)) { Ii = 1.0f; Io = g.index; }
@@ -699,7 +722,11 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         cur_index = next_index;
         ++depth;
     }
-    hw = hw / campdf;  // createHitpoints: PixelWeight /= pdf (SPPM.cpp:346)
+    hw = hw / campdf;  // createHitpoints: PixelWeight /= pdf, DirectLight /= pdf (SPPM.cpp:346-347)
+    if (ibl) {
+        direct = direct / campdf;
+        H.direct[i] = make_float4(direct.x, direct.y, direct.z, 0.0f);
+    }
     if (type == 1 && (!isfinite(hw.x) || !isfinite(hw.y) || !isfinite(hw.z))) fl |= 32u;
     H.pos[i] = make_float4(hpos.x, hpos.y, hpos.z, (float)type);
     H.nrm[i] = make_float4(hn.x, hn.y, hn.z, hidx);
@@ -1317,16 +1344,33 @@ __global__ void __launch_bounds__(1024) k_group_stats(const unsigned long long* 
     }
 }
 
+// DirectLight of every hit point that is not NONE (SPPM.cpp:404-418); only
+// an environment light makes it nonzero.  norm2() > 1e-8 compares the float
+// norm against a double.
+__global__ void k_direct_light(HitDev H, SharedDev SH, int np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if ((int)H.pos[i].w == 0) return;
+    const float4 d = H.direct[i];
+    const f3 dv = mk(d.x, d.y, d.z);
+    if (!((double)norm2(dv) > 1e-8)) return;
+    float4 s = SH.direct[i];
+    SH.direct[i] = make_float4(s.x + dv.x, s.y + dv.y, s.z + dv.z, __int_as_float(__float_as_int(s.w) + 1));
+}
+void launch_direct_light(const HitDev& H, const SharedDev& SH, int np, hipStream_t st) {
+    hipLaunchKernelGGL(k_direct_light, dim3((np + 255) / 256), dim3(256), 0, st, H, SH, np);
+}
+
 // reconstructImageFromSharedPoints (SPPM.cpp:473-493)
 __global__ void k_reconstruct(SharedDev SH, int np, long long total_photons, int passes, float4* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     float4 sr = SH.rad[i];
     int2 cn = SH.cnt[i];
-    // DirectLight is zero without an environment light: 0 / (0 + 1e-4) = 0
-    const float dl = 0.0f / (0.0f + 0.0001f);
+    const float4 d4 = SH.direct[i];
+    const f3 dl = mk(d4.x, d4.y, d4.z) / (float)((float)__float_as_int(d4.w) + 0.0001f);
     float den = ((float)total_photons * ((float)cn.y + 0.0001f) / (0.0001f + (float)passes)) * SPPM_PI * sr.w * sr.w;
-    f3 L = mk(dl, dl, dl) + mk(sr.x, sr.y, sr.z) / den;
+    f3 L = dl + mk(sr.x, sr.y, sr.z) / den;
     const float b = 1.2f;
     out[i] = make_float4(powf(L.x, 1.0f / b), powf(L.y, 1.0f / b), powf(L.z, 1.0f / b), 1.0f);
 }
@@ -1336,6 +1380,7 @@ __global__ void k_init_shared(SharedDev SH, int np, float r0) {
     if (i >= np) return;
     SH.rad[i] = make_float4(0.0f, 0.0f, 0.0f, r0);
     SH.cnt[i] = make_int2(0, 0);
+    SH.direct[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
 }
 __global__ void k_init_slots(SlotsDev s) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

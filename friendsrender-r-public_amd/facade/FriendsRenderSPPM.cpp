@@ -256,16 +256,13 @@ void Scene::ClearScene() {
     m_Textures.clear();
     m_TextureNames.clear();
     m_Error.clear();
+    m_IBL = -1;
     ++m_Version;
 }
 
 bool Scene::CreateScene(const char* OBJ_FileName, const PTUtility::SceneMaterialManager* sceneMaterial,
                         Texture2D* IBLTexture) {
     ClearScene();
-    if (IBLTexture != nullptr) {
-        m_Error = "an environment light (IBL texture) is not supported by the SPPM pass core";
-        return false;
-    }
     PTUtility::SceneMaterialManager defmat;
     const PTUtility::SceneMaterialManager* smat = sceneMaterial ? sceneMaterial : &defmat;
     if (!OBJ_FileName) {
@@ -406,6 +403,21 @@ bool Scene::CreateScene(const char* OBJ_FileName, const PTUtility::SceneMaterial
         m_TexIds.push_back(ids);
         m_Attributes.push_back(std::move(at));
     }
+    // environment light (Scene.cpp:214 hands it to the light sampler): a copy
+    // of the map; the core converts it as IBL::Init does
+    if (IBLTexture != nullptr) {
+        if (IBLTexture->GetWidth() <= 0 || IBLTexture->GetHeight() <= 0) {
+            m_Error = "empty IBL texture";
+            return false;
+        }
+        std::unique_ptr<Texture2D> t(new Texture2D);
+        t->Init(IBLTexture->GetWidth(), IBLTexture->GetHeight());
+        memcpy(t->Data(), IBLTexture->Data(),
+               sizeof(float) * 4 * (size_t)IBLTexture->GetWidth() * (size_t)IBLTexture->GetHeight());
+        m_Textures.push_back(std::move(t));
+        m_TextureNames.push_back("(environment map)");
+        m_IBL = (int)m_Textures.size() - 1;
+    }
     ++m_Version;
     return true;
 }
@@ -491,6 +503,8 @@ void StochasticProgressivePhotonMapping::prepare(const Scene& scene, const iCame
         d.attrs = attrs.data();
         d.ntextures = (int32_t)tex.size();
         d.textures = tex.data();
+        d.has_ibl = scene.IBLTextureId() >= 0 ? 1 : 0;
+        d.ibl_texture = scene.IBLTextureId();
         check(sppm_scene_upload(I.ctx, &d, m_BVHWidth), "sppm_scene_upload");
         I.scene = &scene;
         I.version = scene.Version();

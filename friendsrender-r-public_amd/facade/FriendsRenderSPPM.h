@@ -16,8 +16,9 @@
 //   Texture2D                                 Texture.h:10-42
 //
 // Differences, each an error rather than a silent change:
-//   * the environment light (IBL texture argument) is not supported: a
-//     non-null IBL makes CreateScene return false;
+//   * the environment light (IBL texture argument) lights escaping eye
+//     paths as the reference's does (SPHERE mapping, colour 0.4 c^2); like
+//     the reference, it emits no photons;
 //   * ThinLensCamera with a diaphragm image is not supported (throws);
 //   * textures referenced by an MTL file are read from binary PPM (P6, 8-bit,
 //     stored as byte/255 like the reference's loader) or PFM (float) files;
@@ -206,6 +207,8 @@ public:
     // Attribute i as the C ABI record (kind, parameters, texture ids).
     void AttributeRecord(int i, void* sppm_attr_out) const;
     int NumTextures() const { return (int)m_Textures.size(); }
+    // texture id of the environment map copied from CreateScene's IBL argument, or -1
+    int IBLTextureId() const { return m_IBL; }
     const Texture2D& GetTexture(int i) const { return *m_Textures[i]; }
     float Diagonal() const;
     uint64_t Version() const { return m_Version; }
@@ -219,6 +222,7 @@ private:
     std::vector<std::string> m_TextureNames;
     std::string m_Error;
     uint64_t m_Version = 0;
+    int m_IBL = -1;
     int textureId(const std::string& path);
 };
 

@@ -15,6 +15,7 @@
 //   --alpha a --radius r  SPPM constants (default 0.9 / 0.015, SPPM.cpp:20-21)
 //   --bvh W               BVH width 2, 4 or 8
 //   --sampler-id ID       UniformSample seed ID
+//   --ibl FILE            environment map (binary PPM or PFM), Scene::CreateScene's IBL argument
 //   --dump-scene FILE     write the parsed facet records and attribute table
 //                         (no rendering, no GPU)
 #include <cstdio>
@@ -98,6 +99,7 @@ int main(int argc, char** argv) {
     float A = 1.0f, F = 1.0f, fov = 45.0f, alpha = 0.9f, radius = 0.015f;
     float pos[3] = {0.5f, 0.5f, 2.2f}, tgt[3] = {0.5f, 0.5f, 0.5f};
     const char* dump = nullptr;
+    const char* ibl = nullptr;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
         auto need = [&](int n) {
@@ -119,6 +121,7 @@ int main(int argc, char** argv) {
         else if (a == "--alpha") { need(1); alpha = (float)atof(argv[++i]); }
         else if (a == "--radius") { need(1); radius = (float)atof(argv[++i]); }
         else if (a == "--bvh") { need(1); bvh = atoi(argv[++i]); }
+        else if (a == "--ibl") { need(1); ibl = argv[++i]; }
         else if (a == "--sampler-id") { need(1); sid = atoi(argv[++i]); }
         else if (a == "--dump-scene") { need(1); dump = argv[++i]; }
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
@@ -126,7 +129,12 @@ int main(int argc, char** argv) {
 
     TestSceneMaterial material;
     Scene scene;
-    if (!scene.CreateScene(obj, &material, nullptr)) {
+    Texture2D iblTex;
+    if (ibl && !TextureUtility::LoadTextureFromFile(iblTex, ibl)) {
+        fprintf(stderr, "cannot read the environment map %s\n", ibl);
+        return 1;
+    }
+    if (!scene.CreateScene(obj, &material, ibl ? &iblTex : nullptr)) {
         fprintf(stderr, "CreateScene failed: %s\n", scene.LastError().c_str());
         return 1;
     }

@@ -24,6 +24,8 @@ from sppm_amd import (CAM_PINHOLE, CAM_THINLENS, MAT_DIFFUSE, MAT_GGX_GLASS, MAT
 
 
 class Scene:
+    ibl = None  # texture id of an environment map (SPHERE mapping), or None
+
     def __init__(self, pos, nrm, uv, attr, attrs, textures):
         self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
         self.nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 9)
@@ -594,3 +596,25 @@ def caustic_cameras(n=8, width=2048, height=2048, aperture=0.02):
         c["target"] = tuple(float(x) for x in tgt)
         cams.append(c)
     return cams
+
+
+# ---- environment light --------------------------------------------------------
+def sky_texture(w=256, h=128):
+    """Equirectangular sky for the SPHERE mapping (u = azimuth, v = polar
+    angle from +y): a vertical gradient with a warm sun patch, RGB8."""
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    base = np.stack([0.35 + 0.5 * (1 - v), 0.5 + 0.4 * (1 - v), 0.9 - 0.3 * v], -1)[:, None, :] * np.ones((1, w, 1))
+    sun = np.exp(-(((u[None, :] - 0.3) / 0.04) ** 2 + ((v[:, None] - 0.25) / 0.06) ** 2))
+    img = np.clip(base + sun[..., None] * np.array([1.0, 0.8, 0.5]), 0, 1)
+    return rgb8_to_rgba32f((img * 255).astype(np.uint8))
+
+
+def box_scene_ibl():
+    """The box scene lit also by an environment map through its open side
+    (z = 1): camera rays that leave the box and eye paths that bounce out of it
+    see the sky."""
+    sc = box_scene()
+    out = Scene(sc.pos, sc.nrm, sc.uv, sc.attr, sc.attrs, list(sc.textures) + [sky_texture()])
+    out.ibl = len(out.textures) - 1
+    return out

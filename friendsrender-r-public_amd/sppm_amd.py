@@ -30,6 +30,7 @@ EXPORTED = [
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
+    "sppm_read_direct",
 ]
 
 
@@ -83,7 +84,8 @@ class Texture(C.Structure):
 class SceneDesc(C.Structure):
     _fields_ = [("nfacets", C.c_int32), ("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)),
                 ("uv", C.POINTER(C.c_float)), ("attr", C.POINTER(C.c_int32)), ("nattrs", C.c_int32),
-                ("attrs", C.POINTER(Attr)), ("ntextures", C.c_int32), ("textures", C.POINTER(Texture))]
+                ("attrs", C.POINTER(Attr)), ("ntextures", C.c_int32), ("textures", C.POINTER(Texture)),
+                ("has_ibl", C.c_int32), ("ibl_texture", C.c_int32)]
 
 
 class CameraDesc(C.Structure):
@@ -217,6 +219,9 @@ def scene_desc(scene):
     d.attrs = C.cast(attrs, C.POINTER(Attr))
     d.ntextures = len(scene.textures)
     d.textures = C.cast(texs, C.POINTER(Texture))
+    ibl = getattr(scene, "ibl", None)
+    d.has_ibl = 0 if ibl is None else 1
+    d.ibl_texture = -1 if ibl is None else int(ibl)
     return d, (attrs, texs, scene)
 
 
@@ -406,6 +411,13 @@ class Renderer:
         n, g = np.zeros(np_, np.int32), np.zeros(np_, np.int32)
         _check(self.lib.sppm_read_shared(self.h, _fp(rad), _fp(r), _ip(n), _ip(g)))
         return dict(radiance=rad, radius=r, num_photon=n, num_gather=g)
+
+    def read_direct(self):
+        np_ = self.W * self.H
+        hd, sd = np.zeros((np_, 3), np.float32), np.zeros((np_, 3), np.float32)
+        n = np.zeros(np_, np.int32)
+        _check(self.lib.sppm_read_direct(self.h, _fp(hd), _fp(sd), _ip(n)))
+        return dict(hit=hd, shared=sd, num=n)
 
     def read_path_counts(self):
         c = np.zeros(self.N, np.int32)

@@ -90,6 +90,13 @@ typedef struct sppm_scene_desc {
     const sppm_attr* attrs;
     int32_t ntextures;
     const sppm_texture* textures;
+    /* Environment light (IBL.h, LightSampler.cpp:22-26; Scene::CreateScene's
+     * IBLTexture): texture id of the map, used with the SPHERE mapping and the
+     * colour conversion 0.4 * c^2 the reference applies; eye paths that escape
+     * add it to the hit point's DirectLight (SPPM.cpp:290-300).  Photons are
+     * not emitted from it, as in the reference. */
+    int32_t has_ibl;        /* 0: no environment light */
+    int32_t ibl_texture;    /* texture id when has_ibl */
 } sppm_scene_desc;
 
 /* ---- camera ------------------------------------------------------------ */
@@ -192,6 +199,10 @@ int sppm_read_hitpoints(sppm_ctx* ctx, int32_t* type, int32_t* facet, float* pos
 /* Shared hit points: radiance 3f, radius, num_photon (int N), num_gather. */
 int sppm_read_shared(sppm_ctx* ctx, float* radiance, float* radius, int32_t* num_photon,
                      int32_t* num_gather);
+/* DirectLight of the last eye pass's hit points (3 floats each; 0 without an
+ * environment light), and SharedHitPoint::DirectLight / NumDirectLight
+ * (SPPM.h:112-126, accumulated as SPPM.cpp:415-418).  Any may be NULL. */
+int sppm_read_direct(sppm_ctx* ctx, float* hit_direct, float* shared_direct, int32_t* num_direct);
 /* Flattened photon count of each path of the last photon pass (N entries):
  * surface photons stored + the stale slot the reference keeps, or 1 for the
  * emission photon (SPPM.cpp:530-540, 672, 812-822). */

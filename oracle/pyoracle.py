@@ -78,6 +78,9 @@ def _load_oracle():
     L.orc_neighbour_total.restype = C.c_int64
     L.orc_read_neighbours.argtypes = [C.c_void_p, C.POINTER(C.c_int64), I]
     L.orc_read_shared.argtypes = [C.c_void_p, F, F, I, I]
+    L.orc_read_direct.argtypes = [C.c_void_p, F, F, I]
+    L.orc_ibl_color.argtypes = [C.c_void_p, F, F]
+    L.orc_ibl_uv.argtypes = [F, F]
     L.orc_write_shared.argtypes = [C.c_void_p, F, F, I, I]
     L.orc_reconstruct.argtypes = [C.c_void_p, C.c_int32, C.c_int32, F]
     L.orc_full_pass.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -146,6 +149,19 @@ class OracleScene:
             orc().orc_scene_destroy(self.h)
         except Exception:
             pass
+
+    @staticmethod
+    def ibl_uv(d):
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(2, np.float32)
+        orc().orc_ibl_uv(fp(d), fp(out))
+        return out
+
+    def ibl_color(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(3, np.float32)
+        orc().orc_ibl_color(self.h, fp(d), fp(out))
+        return out
 
     def closest_hit(self, org, dirs, mode=1, tmin=1e-4, tmax=3.4028234663852886e38):
         org = np.ascontiguousarray(org, np.float32)
@@ -234,6 +250,12 @@ class OracleRender:
         k, g = np.zeros(n, np.int32), np.zeros(n, np.int32)
         orc().orc_read_shared(self.h, fp(rad), fp(r), ip(k), ip(g))
         return dict(radiance=rad, radius=r, num_photon=k, num_gather=g)
+
+    def read_direct(self):
+        n = self.W * self.H
+        hd, sd, k = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.int32)
+        orc().orc_read_direct(self.h, fp(hd), fp(sd), ip(k))
+        return dict(hit=hd, shared=sd, num=k)
 
     def write_shared(self, radiance, radius, num_photon, num_gather):
         radiance = np.ascontiguousarray(radiance, np.float32)

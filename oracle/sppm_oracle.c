@@ -705,6 +705,10 @@ struct orc_scene {
     int32_t* lleft;
     int32_t* lright;
     float lavg;
+    /* environment light: the converted map (LightSampler.cpp:22-26) */
+    int has_ibl;
+    sppm_texture ibl;
+    float* ibl_data;
     /* acceleration for the oracle's own speed (results equal brute force) */
     obvh_node* nodes;
     int nnodes;
@@ -845,6 +849,21 @@ orc_scene* orc_scene_create(const sppm_scene_desc* d) {
         s->tex[i].height = d->textures[i].height;
         s->tex[i].rgba = s->texdata[i];
     }
+    /* environment light: IBL::Init copies the map through 0.4 * c^2
+     * (LightSampler.cpp:22-26, IBL.h:163-175); operator*(float, Vec4) */
+    if (d->has_ibl && d->ibl_texture >= 0 && d->ibl_texture < d->ntextures) {
+        const sppm_texture* t = &d->textures[d->ibl_texture];
+        size_t n = 4 * (size_t)t->width * (size_t)t->height;
+        s->ibl_data = (float*)malloc(sizeof(float) * n);
+        for (size_t k = 0; k < n; k += 4) {
+            for (int c = 0; c < 3; ++c) s->ibl_data[k + c] = 0.4f * (t->rgba[k + c] * t->rgba[k + c]);
+            s->ibl_data[k + 3] = 0.4f * 0.0f;
+        }
+        s->ibl.width = t->width;
+        s->ibl.height = t->height;
+        s->ibl.rgba = s->ibl_data;
+        s->has_ibl = 1;
+    }
     /* Area lights, registered in facet order (Scene.cpp:145-154) */
     s->lights = (olight*)calloc((size_t)(nf > 0 ? nf : 1), sizeof(olight));
     s->nlights = 0;
@@ -894,6 +913,7 @@ void orc_scene_destroy(orc_scene* s) {
     free(s->texdata); free(s->tex); free(s->lights);
     free(s->lprob); free(s->lleft); free(s->lright);
     free(s->pos); free(s->nrm); free(s->uv); free(s->attr); free(s->attrs); free(s->nodes);
+    free(s->ibl_data);
     free(s);
 }
 
@@ -1377,6 +1397,34 @@ void orc_read_photons(const orc_render* r, float* pos, float* flux, float* indir
 
 /* getHitpointInfo (SPPM.cpp:98-306) with no IBL: after the hit point is made
  * the rest of the path can only add IBL light (zero), so it stops there. */
+/* IBL::GetUVFromDirection, SPHERE mapping (IBL.h:90-103): unqualified acos
+ * on a float binds to the double function; u, v are narrowed to float.
+ * IBL::GetIBLColorFromDirection (IBL.h:122-129) fetches bilinearly. */
+static void ibl_uv(v3 dir, float* u_, float* v_) {
+    float u = 0.0f, v = 0.0f;
+    if (!((double)fabsf(dir.x) < 1e-6 && (double)fabsf(dir.z) < 1e-6)) {
+        const v3 xz = vnormalized(V3(dir.x, 0.0f, dir.z));
+        const double a = acos((double)xz.x);
+        u = (float)((xz.z >= 0 ? a : (double)(2.0f * PIf) - a) / (double)(2.0f * PIf));
+        v = (float)(acos((double)dir.y) / (double)PIf);
+    }
+    *u_ = u;
+    *v_ = v;
+}
+static v3 ibl_color(const orc_scene* s, v3 dir) {
+    float u, v;
+    ibl_uv(dir, &u, &v);
+    float c[4];
+    orc_bilinear(&s->ibl, u, v, c);
+    return V3(c[0], c[1], c[2]);
+}
+
+void orc_ibl_uv(const float* dir, float* uv) { ibl_uv(vload(dir), uv, uv + 1); }
+void orc_ibl_color(const orc_scene* s, const float* dir, float* out3) {
+    const v3 c = s->has_ibl ? ibl_color(s, vload(dir)) : V3(0, 0, 0);
+    vstore(out3, c);
+}
+
 static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
     const orc_scene* s = r->s;
     const int W = r->W, H = r->H;
@@ -1398,6 +1446,7 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
     hp->facet = -1;
     hp->weight = V3(1, 1, 1);
     hp->outdir = V3(0, 1, 0);
+    v3 direct = V3(0, 0, 0);
     for (;;) {
         if (depth > 6) break;
         else if (depth > 4) {
@@ -1409,6 +1458,8 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
         int f = orc_closest_hit(s, 1, (const float*)&cur_org, (const float*)&cur_dir, RAY_OFFSET,
                                 FLT_MAX, &t, &hu, &hv);
         if (f < 0) {
+            /* escape: the environment's colour (SPPM.cpp:290-300) */
+            if (s->has_ibl) direct = vadd(direct, vdivs(vmul(ibl_color(s, cur_dir), T), PDF));
             if (hp->type == 0) hp->type = 2;
             break;
         }
@@ -1421,7 +1472,8 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
             cur_org = vadd(g.pos, vscale(cur_dir, RAY_OFFSET));
             continue;
         }
-        if (g.is_sppm) {  /* makeHitPoint (SPPM.cpp:79-96) */
+        if (g.is_sppm && hp->type != 1) {  /* makeHitPoint (SPPM.cpp:79-96, 171-174) */
+            direct = V3(0, 0, 0);  /* the new HitPoint's DirectLight */
             hp->type = 1;
             hp->facet = f;
             hp->pos = g.pos;
@@ -1433,7 +1485,9 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
             hp->rough = g.rough;
             hp->absorb = g.absorb;
             hp->refl = g.refl;
-            break;
+            /* the path continues after its hit point only to reach the
+             * environment; without one nothing later changes the hit point */
+            if (!s->has_ibl) break;
         }
         float Ii, Io = 1.0f;
         if (near1(cur_index)) { Ii = 1.0f; Io = g.index; }
@@ -1463,7 +1517,7 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
     }
     /* createHitpoints: PixelWeight /= pdf, DirectLight /= pdf (SPPM.cpp:346-347) */
     hp->weight = vdivs(hp->weight, campdf);
-    hp->direct = V3(0, 0, 0);
+    hp->direct = vdivs(direct, campdf);
 }
 
 static v3 camera_sample(orc_render* r, int step) {
@@ -1652,6 +1706,25 @@ void orc_gather(orc_render* r, int record) {
     free(lists);
     free(g.cell_start);
     free(g.order);
+    /* DirectLight of every hit point that is not NONE (SPPM.cpp:404-418):
+     * norm2() > 1e-8 compares the float norm against a double */
+    for (int v = 0; v < np; ++v) {
+        const ohit* hp = &r->hits[v];
+        if (hp->type == 0) continue;
+        if ((double)vnorm2(hp->direct) > 1e-8) {
+            r->shared[v].direct = vadd(r->shared[v].direct, hp->direct);
+            r->shared[v].num_direct++;
+        }
+    }
+}
+
+void orc_read_direct(const orc_render* r, float* hit_direct, float* shared_direct, int32_t* num_direct) {
+    size_t np = (size_t)r->W * (size_t)r->H;
+    for (size_t i = 0; i < np; ++i) {
+        if (hit_direct) vstore(hit_direct + 3 * i, r->hits[i].direct);
+        if (shared_direct) vstore(shared_direct + 3 * i, r->shared[i].direct);
+        if (num_direct) num_direct[i] = r->shared[i].num_direct;
+    }
 }
 void orc_read_counts(const orc_render* r, int32_t* m) {
     memcpy(m, r->counts, sizeof(int32_t) * (size_t)r->W * (size_t)r->H);

@@ -115,6 +115,12 @@ void orc_gather(orc_render* r, int record);
 void orc_read_counts(const orc_render* r, int32_t* m);
 int64_t orc_neighbour_total(const orc_render* r);
 void orc_read_neighbours(const orc_render* r, int64_t* offsets, int32_t* idx);
+/* IBL::GetIBLColorFromDirection of the scene's converted environment map (0 without one). */
+void orc_ibl_color(const orc_scene* s, const float* dir, float* out3);
+/* IBL::GetUVFromDirection, SPHERE mapping: (u, v) of a direction. */
+void orc_ibl_uv(const float* dir, float* uv);
+/* DirectLight of the hit points and the shared hit points (IBL escapes). */
+void orc_read_direct(const orc_render* r, float* hit_direct, float* shared_direct, int32_t* num_direct);
 void orc_read_shared(const orc_render* r, float* radiance, float* radius, int32_t* num_photon,
                      int32_t* num_gather);
 /* set shared state (for hand-built update tests) */

@@ -95,15 +95,20 @@ def rel_rmse(a, b):
 
 
 @pytest.mark.gpu
-def test_facade_render_matches_capi_and_oracle(tmp_path, box_scene):
+@pytest.mark.parametrize("env", [False, True], ids=["box", "box+environment"])
+def test_facade_render_matches_capi_and_oracle(tmp_path, box_scene, env):
     import pyoracle
     W, H, N, P = 64, 48, 20000, 3
+    if env:
+        box_scene = scenes.box_scene_ibl()
     obj = scenes.write_obj(box_scene, str(tmp_path), "box")
     out = str(tmp_path / "img.pfm")
     cam = scenes.pinhole_camera(W, H)
     args = [RENDER_OBJ, obj, out, "--size", str(W), str(H), "--photons", str(N), "--passes", str(P), "--pinhole",
             "--pos", *map(str, cam["pos"]), "--target", *map(str, cam["target"]), "--fov", str(cam["fov_deg"]),
             "--alpha", "0.9", "--radius", "0.015", "--bvh", "4"]
+    if env:  # write_obj wrote the environment map as box_tex<id>.ppm
+        args += ["--ibl", str(tmp_path / f"box_tex{box_scene.ibl}.ppm")]
     print("facade render:", " ".join(args[2:]), flush=True)
     p = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr

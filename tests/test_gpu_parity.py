@@ -88,6 +88,11 @@ def compare_pass(gpu, ref, stored_ref, label, record=True):
         orf, irf = ref.read_neighbours()
         np.testing.assert_array_equal(og, orf)
         np.testing.assert_array_equal(ig, irf)
+    # DirectLight (environment light): hit points and shared sums, counts exact
+    dg, dr = gpu.read_direct(), ref.read_direct()
+    np.testing.assert_array_equal(dg["num"], dr["num"])
+    assert rel_rmse(dg["hit"], dr["hit"]) <= REL_TOL
+    assert rel_rmse(dg["shared"], dr["shared"]) <= REL_TOL
     sg, sr = gpu.read_shared(), ref.read_shared()
     np.testing.assert_array_equal(sg["num_photon"], sr["num_photon"])
     np.testing.assert_array_equal(sg["num_gather"], sr["num_gather"])
@@ -141,6 +146,17 @@ def test_pass_parity_balanced_gather_full(full_scene_small):
     r0 = 0.02 * full_scene_small.diagonal()
     run_parity(full_scene_small, scenes.thinlens_camera(64, 64), 0.7, r0, 20000, 3, 8, "full balanced",
                record=False)
+
+
+def test_pass_parity_environment_light():
+    """Box lit also by an environment map through its open side: eye paths
+    continue past their hit points and add the sky when they leave
+    (SPPM.cpp:290-300); DirectLight, its counts and the image against the
+    oracle."""
+    sc = scenes.box_scene_ibl()
+    cam = scenes.pinhole_camera(48, 40)
+    cam["fov_deg"] = 70.0
+    run_parity(sc, cam, 0.9, 0.015, 20000, 3, 4, "box + environment")
 
 
 def test_halton_and_hammersley_samplers(box_scene):

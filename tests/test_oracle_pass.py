@@ -138,3 +138,65 @@ def test_reconstruct_closed_form(small_run):
     want = np.power(L.astype(np.float64), 1 / 1.2)
     np.testing.assert_allclose(img.reshape(-1, 4)[:, :3], want, rtol=2e-6, atol=1e-30)
     assert (img.reshape(-1, 4)[:, 3] == 1.0).all()
+
+
+# ---- environment light (IBL.h, SPPM.cpp:290-300, 404-418, 485) -----------------
+def test_ibl_sphere_mapping_and_conversion():
+    """GetUVFromDirection (SPHERE): u = azimuth from +x towards +z over 2 pi,
+    v = polar angle from +y over pi (PI = 3.14159f, so values sit within
+    1e-6 of the exact ones); straight up maps to (0, 0).  The colour is the
+    bilinear fetch at (u, v) of the map stored as 0.4 * c^2."""
+    for d, u, v in (((1, 0, 0), 0.0, 0.5), ((0, 0, 1), 0.25, 0.5), ((-1, 0, 0), 0.5, 0.5), ((0, 0, -1), 0.75, 0.5),
+                    ((0.6, 0.8, 0), 0.0, np.arccos(0.8) / np.pi), ((0.0, -0.6, -0.8), 0.75, np.arccos(-0.6) / np.pi),
+                    ((0, 1, 0), 0.0, 0.0)):
+        uv = po.OracleScene.ibl_uv(np.array(d, np.float32))
+        assert abs(uv[0] - u) < 2e-6 and abs(uv[1] - v) < 2e-6, (d, uv)
+    sc = scenes.box_scene_ibl()
+    o = po.OracleScene(sc)
+    conv = (np.float32(0.4) * (sc.textures[sc.ibl] * sc.textures[sc.ibl])).astype(np.float32)
+    conv[..., 3] = 0.0
+    import sppm_amd
+    td = sppm_amd.Texture()
+    td.height, td.width = conv.shape[0], conv.shape[1]
+    td.rgba = po.fp(conv)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        d = rng.normal(size=3).astype(np.float32)
+        d /= np.linalg.norm(d)
+        uv = po.OracleScene.ibl_uv(d)
+        want = np.zeros(4, np.float32)
+        po.orc().orc_bilinear(po.C.byref(td), float(uv[0]), float(uv[1]), po.fp(want))
+        np.testing.assert_array_equal(o.ibl_color(d), want[:3])
+    o2 = po.OracleScene(scenes.box_scene())
+    assert (o2.ibl_color(np.array([1, 0, 0], np.float32)) == 0).all()
+
+
+def test_ibl_direct_light_accumulates_and_reconstructs():
+    """Pixels whose camera ray leaves the box see the sky: type
+    ONLY_DIRECTLIGHT with DirectLight = sky colour / camera pdf (pinhole: 1,
+    SPPM.cpp:346-347); the shared sum and count follow SPPM.cpp:415-418 and
+    reconstruct adds DirectLight / (NumDirectLight + 1e-4) (SPPM.cpp:485)."""
+    sc = scenes.box_scene_ibl()
+    osc = po.OracleScene(sc)
+    cam = scenes.pinhole_camera(24, 24, sampler=2)  # HammersleySample
+    cam["sampler_nsample"] = 1
+    cam["pos"] = (0.5, 0.5, 3.0)
+    cam["fov_deg"] = 80.0
+    o = po.OracleRender(osc, cam, 0.9, 0.015, 2000)
+    o.full_pass(record=False)
+    hp = o.read_hitpoints()
+    dl = o.read_direct()
+    only = hp["type"] == 2
+    assert only.any() and (hp["type"] == 1).any()
+    # an escaping camera ray: DirectLight is the sky in its direction (throughput 1, pdf 1)
+    smp = np.zeros(3, np.float32)
+    po.orc().orc_hammersley(1, 1, po.fp(smp))  # HammersleySample(NumSample 1) at step 1
+    for p in np.nonzero(only)[0][:5]:
+        _, d, pdf = po.camera_ray(cam, np.float32(p % 24) / np.float32(24), np.float32(p // 24) / np.float32(24), smp)
+        np.testing.assert_array_equal(dl["hit"][p], (osc.ibl_color(d) / np.float32(pdf)).astype(np.float32))
+    nz = np.sum(dl["hit"].astype(np.float64) ** 2, 1) > 1e-8
+    np.testing.assert_array_equal(dl["num"], (nz & (hp["type"] != 0)).astype(np.int32))
+    np.testing.assert_array_equal(dl["shared"], np.where((nz & (hp["type"] != 0))[:, None], dl["hit"], 0))
+    img = o.reconstruct(2000, 1).reshape(-1, 4)
+    want = np.power(dl["shared"][only] / np.float32(1.0001), np.float32(1 / 1.2))
+    np.testing.assert_allclose(img[only][:, :3], want, rtol=1e-5)
