@@ -1790,12 +1790,44 @@ int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* di
     dfree(dorg); dfree(ddir); dfree(dtuv); dfree(dfac); dfree(dfl);
     for (int i = 0; i < n; ++i) {
         // hits are accepted only below tmax (MaxT); FLT_MAX means "no limit"
-        if (facet[i] >= 0 && !(tuv[3 * i] < tmax)) facet[i] = -1;
+        if (facet[i] >= 0 && !((double)tuv[3 * i] < (double)tmax - 1e-10)) facet[i] = -1;  // bvh8.h:640
         if (t) t[i] = facet[i] >= 0 ? tuv[3 * i] : 3.40282347e38f;
         if (u) u[i] = facet[i] >= 0 ? tuv[3 * i + 1] : 0.0f;
         if (v) v[i] = facet[i] >= 0 ? tuv[3 * i + 2] : 0.0f;
     }
     if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow");
+    return SPPM_OK;
+}
+
+int sppm_trace_occluded(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, const float* tmax,
+                        float translucency, int32_t* occluded) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (n <= 0) return SPPM_OK;
+    if (!org || !dir || !tmax || !occluded) return fail(SPPM_E_ARG, "NULL argument");
+    HIPOK(hipSetDevice(c->device));
+    float *dorg = nullptr, *ddir = nullptr, *dmax = nullptr;
+    int* docc = nullptr;
+    uint32_t* dfl = nullptr;
+    int rc = 0;
+    if ((rc = dalloc(&dorg, 3 * (size_t)n)) || (rc = dalloc(&ddir, 3 * (size_t)n)) || (rc = dalloc(&dmax, n)) ||
+        (rc = dalloc(&docc, n)) || (rc = dalloc(&dfl, 1))) {
+        dfree(dorg); dfree(ddir); dfree(dmax); dfree(docc); dfree(dfl);
+        return rc;
+    }
+    hipStream_t st = c->stream;
+    HIPOK(hipMemcpyAsync(dorg, org, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(ddir, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(dmax, tmax, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    launch_set_u32(dfl, 0u, st);
+    launch_occluded(c->scene.dev, n, dorg, ddir, tmin, dmax, translucency, c->scene.trace_spill(), docc, dfl, st);
+    HIPOK(hipGetLastError());
+    uint32_t fl = 0;
+    HIPOK(hipMemcpyAsync(occluded, docc, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(&fl, dfl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    dfree(dorg); dfree(ddir); dfree(dmax); dfree(docc); dfree(dfl);
+    if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow or more than 256 translucent layers");
     return SPPM_OK;
 }
 

@@ -55,6 +55,10 @@ void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, c
                 const TraceSpill& sp, const uint32_t* words, uint32_t* flags, hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
                   int* facet, float* tuv, uint32_t* flags, hipStream_t st);
+// Scene::IsOccluded for a batch of rays: occluded[i] = 0 / 1.
+void launch_occluded(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const float* tmax,
+                     float transl,
+                     const TraceSpill& sp, int* occluded, uint32_t* flags, hipStream_t st);
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
                        int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,

@@ -1412,6 +1412,63 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const f
     if (fl) atomicOr(flags, fl);
 }
 
+// Scene::IsOccluded (Scene.cpp:246-281): the closest hit in [MinDist, MaxDist)
+// (MaxDist per ray, as at its call site SPPM.cpp:205)
+// occludes unless its surface's albedo texture makes it more translucent
+// than the threshold (1 - alpha > translucency); then the ray goes on from
+// just behind it with the range shortened, renormalised as Ray's constructor
+// does.  At most OCC_LAYERS surfaces are passed (flag bit 2 beyond).
+constexpr int OCC_LAYERS = 256;
+template <int W>
+__global__ void __launch_bounds__(PATH_BLOCK) k_occluded(SceneDev S, int n, const float* __restrict__ org,
+                                                         const float* __restrict__ dir, float tmin,
+                                                         const float* __restrict__ tmax, float transl, int cap,
+                                                         uint32_t* __restrict__ spill,
+                                                         int* __restrict__ occluded, uint32_t* __restrict__ flags) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    uint32_t fl = 0;
+    for (int i = blockIdx.x * PATH_BLOCK + lane; i < n; i += gridDim.x * PATH_BLOCK) {
+        f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+        f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        float mn = tmin, mx = tmax[i];
+        int occ = 0;
+        for (int layer = 0;; ++layer) {
+            if (layer == OCC_LAYERS) {
+                occ = 1;
+                fl |= 4u;
+                break;
+            }
+            HitRec h;
+            if (!closest_hit<W>(S, o, d, mn, ts, h, &fl) || !((double)h.t < (double)mx - 1e-10)) break;
+            const DevAttr& a = S.attrs[S.facet_attr[h.facet]];
+            if (a.tex_albedo < 0) {
+                occ = 1;
+                break;
+            }
+            const float2* tv = S.facet_uv + 3 * (size_t)h.facet;
+            const float2 t0 = tv[0], t1 = tv[1], t2 = tv[2];
+            const float uvx = ((t1.x - t0.x) * h.u + (t2.x - t0.x) * h.v) + t0.x;
+            const float uvy = ((t1.y - t0.y) * h.u + (t2.y - t0.y) * h.v) + t0.y;
+            const TexDesc td = S.tex[a.tex_albedo];
+            const float4 c = tex_bilinear(S.texels + td.offset, td.width, td.height, uvx, uvy);
+            const float tr = (float)(1.0 - (double)c.w);
+            if (!(tr > transl)) {
+                occ = 1;
+                break;
+            }
+            const float dist = h.t + SPPM_RAY_OFFSET;
+            o = o + d * dist;
+            d = normalized(d);
+            mn = fmaxf(mn - dist, SPPM_RAY_OFFSET);
+            mx = mx - dist;
+        }
+        occluded[i] = occ;
+    }
+    if (fl) atomicOr(flags, fl);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
@@ -1475,6 +1532,18 @@ void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, 
     if (S.width == 2) hipLaunchKernelGGL(k_trace<2>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
     else if (S.width == 4) hipLaunchKernelGGL(k_trace<4>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
     else hipLaunchKernelGGL(k_trace<8>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
+}
+
+void launch_occluded(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const float* tmax,
+                     float transl,
+                     const TraceSpill& sp, int* occluded, uint32_t* flags, hipStream_t st) {
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+    if (S.width == 2)
+        hipLaunchKernelGGL(k_occluded<2>, g, b, 0, st, S, n, org, dir, tmin, tmax, transl, sp.cap, sp.buf, occluded, flags);
+    else if (S.width == 4)
+        hipLaunchKernelGGL(k_occluded<4>, g, b, 0, st, S, n, org, dir, tmin, tmax, transl, sp.cap, sp.buf, occluded, flags);
+    else
+        hipLaunchKernelGGL(k_occluded<8>, g, b, 0, st, S, n, org, dir, tmin, tmax, transl, sp.cap, sp.buf, occluded, flags);
 }
 
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,

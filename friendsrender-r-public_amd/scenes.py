@@ -618,3 +618,47 @@ def box_scene_ibl():
     out = Scene(sc.pos, sc.nrm, sc.uv, sc.attr, sc.attrs, list(sc.textures) + [sky_texture()])
     out.ibl = len(out.textures) - 1
     return out
+
+
+def alpha_texture(alpha):
+    """RGBA32F texture, grey, with the given alpha plane (H, W)."""
+    a = np.asarray(alpha, np.float32)
+    out = np.full(a.shape + (4,), 0.7, np.float32)
+    out[..., 3] = a
+    return out
+
+
+def screens_scene(seed=11, stack=8):
+    """The box with screens facing the open side (z = 1), for shadow queries
+    through alpha-textured surfaces:
+      A  z = 0.5,  x, y in [0.1, 0.9]: alpha 0.2 for u < 0.5 (1 - alpha = 0.8),
+         0.9 for u >= 0.5 (1 - alpha = 0.1), 64x64 texels;
+      B  z = 0.3,  x in [0.1, 0.3]: untextured (always occludes);
+      C  z = 0.15, x in [0.5, 0.9]: value-noise alpha in [0, 1], 128x128;
+      D  `stack` fully clear layers (alpha 0) at z = 0.9, 0.85, ... over
+         x in [0.91, 0.99]."""
+    half = np.where(np.arange(64)[None, :] < 32, 0.2, 0.9) * np.ones((64, 1))
+    nz = _value_noise(128, seed)
+    nz = (nz - nz.min()) / (nz.max() - nz.min())
+    textures = [alpha_texture(half), alpha_texture(nz), alpha_texture(np.zeros((4, 4)))]
+    attrs = [
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),   # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),     # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),   # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+             if_opaque=1),                                                                        # 3
+        dict(name="screen_halves", kind=MAT_DIFFUSE, albedo=(1, 1, 1), is_sppm=1, if_opaque=1, tex_albedo=0),  # 4
+        dict(name="screen_noise", kind=MAT_DIFFUSE, albedo=(1, 1, 1), is_sppm=1, if_opaque=1, tex_albedo=1),   # 5
+        dict(name="screen_clear", kind=MAT_DIFFUSE, albedo=(1, 1, 1), is_sppm=1, if_opaque=1, tex_albedo=2),   # 6
+    ]
+    b = _Builder()
+    box_walls(b, 0, 1, 2)
+    light_quad(b, 3)
+    n = (0, 0, 1)
+    b.quad((0.1, 0.1, 0.5), (0.9, 0.1, 0.5), (0.9, 0.9, 0.5), (0.1, 0.9, 0.5), n, 4)
+    b.quad((0.1, 0.1, 0.3), (0.3, 0.1, 0.3), (0.3, 0.9, 0.3), (0.1, 0.9, 0.3), n, 0)
+    b.quad((0.5, 0.1, 0.15), (0.9, 0.1, 0.15), (0.9, 0.9, 0.15), (0.5, 0.9, 0.15), n, 5)
+    for k in range(stack):
+        z = 0.9 - 0.05 * k
+        b.quad((0.91, 0.1, z), (0.99, 0.1, z), (0.99, 0.9, z), (0.91, 0.9, z), n, 6)
+    return b.scene(attrs, textures)

@@ -30,7 +30,7 @@ EXPORTED = [
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
-    "sppm_read_direct",
+    "sppm_read_direct", "sppm_trace_occluded",
 ]
 
 
@@ -143,6 +143,8 @@ def load_library(path=LIB_PATH):
                                          C.POINTER(C.c_int64)]
     lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
+    lib.sppm_trace_occluded.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.c_float, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int32)]
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
     lib.sppm_scene_stats.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]
     lib.sppm_guide_set.argtypes = [C.c_void_p, C.POINTER(GuideParams)]
@@ -453,6 +455,18 @@ class Renderer:
         t, u, v = (np.zeros(n, np.float32) for _ in range(3))
         _check(self.lib.sppm_trace_closest(self.h, n, _fp(org), _fp(dirs), tmin, tmax, _ip(f), _fp(t), _fp(u), _fp(v)))
         return f, t, u, v
+
+    def occluded(self, org, dirs, tmin, tmax, translucency=0.5):
+        """Scene::IsOccluded per ray (Scene.cpp:246): 1 where a surface in
+        [tmin, tmax) blocks it (tmax scalar or per ray), surfaces with
+        1 - alpha > translucency passed."""
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        tmax = np.ascontiguousarray(np.broadcast_to(np.float32(tmax) if np.isscalar(tmax) else tmax, (n,)), np.float32)
+        out = np.zeros(n, np.int32)
+        _check(self.lib.sppm_trace_occluded(self.h, n, _fp(org), _fp(dirs), tmin, _fp(tmax), translucency, _ip(out)))
+        return out
 
 
 class Batch:

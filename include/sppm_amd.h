@@ -219,6 +219,13 @@ int sppm_read_neighbours(sppm_ctx* ctx, int64_t cap, int64_t* offsets, int32_t* 
  * facet id (-1 miss), t, u, v.  Ties in t go to the lowest facet id. */
 int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* dir,
                        float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
+/* Scene::IsOccluded (Scene.cpp:246-281) for a batch of rays: whether a
+ * surface lies in [tmin, tmax[i]) along ray i, passing through surfaces whose
+ * albedo texture alpha gives 1 - alpha > translucency (the call site
+ * SPPM.cpp:205 passes 0.5 and a per-ray light distance).  occluded[i] = 0 / 1;
+ * SPPM_E_OVERFLOW past 256 translucent layers on one ray. */
+int sppm_trace_occluded(sppm_ctx* ctx, int32_t n, const float* org, const float* dir, float tmin,
+                        const float* tmax, float translucency, int32_t* occluded);
 
 /* Cumulative counters since sppm_begin: valid hit points gathered (type HIT),
  * in-radius photons summed over hit points (sum of M), photons stored

@@ -63,6 +63,7 @@ def _load_oracle():
     L.orc_scene_num_lights.argtypes = [C.c_void_p]
     L.orc_scene_light_info.argtypes = [C.c_void_p, I, F, I, F, I, I, F]
     L.orc_closest_hit_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, C.c_float, C.c_float, I, F, F, F]
+    L.orc_occluded_batch.argtypes = [C.c_void_p, C.c_int, F, F, C.c_float, F, C.c_float, I]
     L.orc_camera_ray.argtypes = [C.POINTER(CameraDesc), C.c_float, C.c_float, F, F, F, F]
     L.orc_render_create.argtypes = [C.c_void_p, C.POINTER(CameraDesc), C.POINTER(Params), C.c_int]
     L.orc_render_create.restype = C.c_void_p
@@ -171,6 +172,15 @@ class OracleScene:
         t, u, v = (np.zeros(n, np.float32) for _ in range(3))
         orc().orc_closest_hit_batch(self.h, mode, n, fp(org), fp(dirs), tmin, tmax, ip(f), fp(t), fp(u), fp(v))
         return f, t, u, v
+
+    def occluded(self, org, dirs, tmin, tmax, translucency=0.5):
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        tmax = np.ascontiguousarray(np.broadcast_to(np.float32(tmax) if np.isscalar(tmax) else tmax, (n,)), np.float32)
+        out = np.zeros(n, np.int32)
+        orc().orc_occluded_batch(self.h, n, fp(org), fp(dirs), tmin, fp(tmax), translucency, ip(out))
+        return out
 
     def lights(self):
         n = orc().orc_scene_num_lights(self.h)

@@ -1018,6 +1018,38 @@ void orc_closest_hit_batch(const orc_scene* s, int mode, int n, const float* org
         facet[i] = orc_closest_hit(s, mode, org + 3 * i, dir + 3 * i, tmin, tmax, t + i, u + i, v + i);
 }
 
+/* Scene::IsOccluded (Scene.cpp:246-281), recursive as written there: the
+ * closest hit in [MinDist, MaxDist) occludes unless its albedo texture gives
+ * 1 - alpha > translucency; then a new Ray (ctor normalises) starts
+ * RAY_OFFSET behind the hit with MinT = max(MinDist - dist, RAY_OFFSET) and
+ * MaxT = MaxDist - dist.  depth guards the recursion (returns 2 past it). */
+static int occluded_rec(const orc_scene* s, v3 org, v3 dir, float mind, float maxd, float transl, int depth) {
+    if (depth > 4096) return 2;
+    float t, u, v;
+    float o3[3] = {org.x, org.y, org.z}, d3[3] = {dir.x, dir.y, dir.z};
+    int f = orc_closest_hit(s, 1, o3, d3, mind, maxd, &t, &u, &v);
+    if (f < 0) return 0;
+    const sppm_attr* a = &s->attrs[s->attr[f]];
+    if (a->tex_albedo < 0) return 1;
+    const float* UV = s->uv + 9 * f;
+    v3 t0 = vload(UV), t1 = vload(UV + 3), t2 = vload(UV + 6);
+    v3 uv = vadd(vadd(vscale(vsub(t1, t0), u), vscale(vsub(t2, t0), v)), t0);
+    float c[4];
+    orc_bilinear(&s->tex[a->tex_albedo], uv.x, uv.y, c);
+    float tr = (float)(1.0 - (double)c[3]);
+    if (!(tr > transl)) return 1;
+    float dist = t + RAY_OFFSET;
+    v3 pos = vadd(org, vscale(dir, dist));
+    float mn = mind - dist;
+    return occluded_rec(s, pos, vnormalized(dir), mn > RAY_OFFSET ? mn : RAY_OFFSET, maxd - dist, transl, depth + 1);
+}
+void orc_occluded_batch(const orc_scene* s, int n, const float* org, const float* dir, float tmin,
+                        const float* tmax, float transl, int32_t* occluded) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i)
+        occluded[i] = occluded_rec(s, vload(org + 3 * i), vload(dir + 3 * i), tmin, tmax[i], transl, 0);
+}
+
 /* ======================================================================
  * Camera (Camera.h:34-129)
  * ====================================================================== */

@@ -89,6 +89,9 @@ int orc_closest_hit(const orc_scene* s, int mode, const float* org, const float*
                     float tmax, float* t, float* u, float* v);
 void orc_closest_hit_batch(const orc_scene* s, int mode, int n, const float* org, const float* dir,
                            float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
+/* Scene::IsOccluded per ray: 0 / 1 (2 if more than 4096 translucent layers) */
+void orc_occluded_batch(const orc_scene* s, int n, const float* org, const float* dir, float tmin,
+                        const float* tmax, float transl, int32_t* occluded);
 
 /* ---- camera ------------------------------------------------------------ */
 /* One SampleCameraRay with a given sampler value (3 floats) -> org, dir, pdf */

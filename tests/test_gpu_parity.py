@@ -196,3 +196,32 @@ def test_full_size_box_properties(box_scene):
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
     assert runs[0][2] == runs[1][2]
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_occlusion_bit_exact(width):
+    """Scene::IsOccluded through alpha-textured screens: GPU and oracle agree
+    on every ray (random origins, directions, per-ray MaxDist, two
+    thresholds)."""
+    scene = scenes.screens_scene()
+    say(f"occlusion: width={width}")
+    org, d = random_rays(scene, 20000, seed=40 + width)
+    rng = np.random.default_rng(width)
+    tmax = rng.uniform(0.02, 1.8, size=len(org)).astype(np.float32)
+    os_ = pyoracle.OracleScene(scene)
+    r = sppm_amd.Renderer(scene, scenes.pinhole_camera(8, 8), 0.9, 0.015, 64, bvh_width=width)
+    for tr in (0.5, 0.3):
+        og = r.occluded(org, d, 1e-4, tmax, tr)
+        oo = os_.occluded(org, d, 1e-4, tmax, tr)
+        np.testing.assert_array_equal(og, oo)
+        say(f"  threshold {tr}: {og.mean():.3f} occluded")
+    # shadow rays as SPPM.cpp:205 casts them: to points on the light, MaxDist = distance - 2 RAY_OFFSET
+    lp = np.stack([rng.uniform(0.375, 0.625, 4000), np.full(4000, 0.999), rng.uniform(0.375, 0.625, 4000)], 1)
+    p = rng.uniform(0.02, 0.98, size=(4000, 3))
+    v = lp - p
+    dist = np.linalg.norm(v, axis=1)
+    dirs = (v / dist[:, None]).astype(np.float32)
+    mx = (dist.astype(np.float32) - np.float32(2e-4)).astype(np.float32)
+    np.testing.assert_array_equal(r.occluded(p.astype(np.float32), dirs, 1e-4, mx, 0.5),
+                                  os_.occluded(p.astype(np.float32), dirs, 1e-4, mx, 0.5))
+    r.close()

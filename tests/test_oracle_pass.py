@@ -200,3 +200,26 @@ def test_ibl_direct_light_accumulates_and_reconstructs():
     img = o.reconstruct(2000, 1).reshape(-1, 4)
     want = np.power(dl["shared"][only] / np.float32(1.0001), np.float32(1 / 1.2))
     np.testing.assert_allclose(img[only][:, :3], want, rtol=1e-5)
+
+
+def test_occlusion_through_alpha_screens():
+    """Scene::IsOccluded (Scene.cpp:246-281) on rays along -z from z = 0.95
+    whose answers follow from the screens' layout: an opaque half, a
+    translucent half in front of an opaque screen or of nothing, a stack of
+    clear layers, MinDist past a screen, and the translucency threshold."""
+    os_ = po.OracleScene(scenes.screens_scene())
+    d = np.array([[0, 0, -1]], np.float32)
+
+    def occ(x, tmax, tmin=1e-4, tr=0.5):
+        return int(os_.occluded(np.array([[x, 0.5, 0.95]], np.float32), d, tmin, tmax, tr)[0])
+
+    # back wall at t = 0.95, A at 0.45, B at 0.65, C at 0.8, stack at 0.05 .. 0.4
+    assert occ(0.7, 0.4) == 0 and occ(0.7, 0.5) == 1      # A's opaque half
+    assert occ(0.4, 0.9) == 0 and occ(0.4, 1.0) == 1      # A's clear half, then the back wall
+    assert occ(0.2, 0.6) == 0 and occ(0.2, 0.7) == 1      # A's clear half, then B
+    assert occ(0.7, 0.6, tmin=0.5) == 0                    # MinDist beyond A
+    assert occ(0.2, 0.7, tmin=0.3) == 1                    # past A, MinDist restarts at RAY_OFFSET: B
+    assert occ(0.4, 0.9, tr=0.85) == 1                     # 1 - 0.2 = 0.8 no longer passes
+    assert occ(0.95, 0.9) == 0 and occ(0.95, 1.0) == 1    # 8 clear layers, then the back wall
+    assert occ(0.95, 0.9, tr=1.0) == 1                     # 1 - 0 = 1 is not > 1
+    assert occ(0.05, 0.9) == 0                             # nothing in front of the back wall
