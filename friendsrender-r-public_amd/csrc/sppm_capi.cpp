@@ -309,7 +309,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
             return fail(SPPM_E_SCENE, "facet %d has attribute %d outside [0,%d)", f, d->attr[f], d->nattrs);
     for (int a = 0; a < d->nattrs; ++a) {
         const sppm_attr& A = d->attrs[a];
-        if (A.kind < SPPM_MAT_DIFFUSE || A.kind > SPPM_MAT_GLASS) return fail(SPPM_E_SCENE, "attribute %d: unknown kind %d", a, A.kind);
+        if (A.kind < SPPM_MAT_DIFFUSE || A.kind > SPPM_MAT_GLOSS_TRANSMIT) return fail(SPPM_E_SCENE, "attribute %d: unknown kind %d", a, A.kind);
         for (int t : {A.tex_albedo, A.tex_roughness, A.tex_bump})
             if (t < -1 || t >= d->ntextures) return fail(SPPM_E_SCENE, "attribute %d: texture id %d out of range", a, t);
     }

@@ -387,6 +387,35 @@ __device__ inline float brdf_ggx_glass(const FParam& p) {  // material.h:314-390
     return (wi_wm * wo_wm) / (wi_n * wo_n) * (Io * Io) * (1.0f - F) * G * D * powf(Ii * wo_wm + Io * wi_wm, -2.0f);
 }
 
+// GlossTransmitSurface::ReflectByN (material.h:692-695): mirror r about the
+// plane of n (its unused normalised tangent is not formed).
+__device__ __forceinline__ f3 reflect_by_n(f3 r, f3 n) { return r - n * (2.0f * dot(n, r)); }
+
+__device__ inline float brdf_gloss_transmit(const FParam& p) {  // material.h:719-756
+    const float power = dot(p.n, p.in) * dot(p.n, p.out) < 0.0f ? p.refl : 1.0f - p.refl;
+    const f3 N = dot(p.n, p.in) < 0 ? p.n : -p.n;
+    f3 out = p.out;
+    if (dot(out, N) < 0) out = reflect_by_n(out, N);
+    const float sv = 1e-10f;
+    if (dot(p.in, N) > -sv || dot(out, N) < sv) return 0.0f;
+    const float alpha = p.rough;
+    const f3 h = normalized((-p.in) + out);
+    const float ccs = dot(h, N);
+    if (ccs < sv || dot(h, p.in) > -sv || dot(h, out) < sv) return 0.0f;
+    const float q = (1.0001f - (1.0f - alpha * alpha) * ccs * ccs);
+    const float D = smax(SPPM_SMIN, (alpha * alpha) / (SPPM_PI * (q * q)));
+    const float lc = smax(0.0f, smin(0.9999f, dot(p.in, -N)));
+    const float lv = smax(0.0f, smin(0.9999f, dot(out, N)));
+    if (lc < SPPM_SMIN || lv < SPPM_SMIN) return 0.0f;
+    const float L_l =
+        (float)((-1.0 + __builtin_sqrt(1.0 + (double)(alpha * alpha) * ((double)(1.0f / (lc * lc)) - 1.0))) * 0.5);
+    const float L_v =
+        (float)((-1.0 + __builtin_sqrt(1.0 + (double)(alpha * alpha) * ((double)(1.0f / (lv * lv)) - 1.0))) * 0.5);
+    const float G = 1.0f / ((1.0f + L_l) * (1.0f + L_v));
+    const float fx = (4.0f * dot(p.in, -N) * dot(out, N));
+    return power * (G * D) / fx;
+}
+
 __device__ inline float surf_brdf(int kind, const FParam& p) {
     switch (kind) {
     case SPPM_MAT_DIFFUSE:  // material.h:168-173
@@ -409,6 +438,18 @@ __device__ inline float surf_brdf(int kind, const FParam& p) {
         if (dot(p.n, p.in) * dot(p.n, p.out) < 0.0f) return (float)((double)(SPPM_DELTA_PDF * refl) / denom);
         return (float)((double)(SPPM_DELTA_PDF * (1.0f - refl)) / denom);
     }
+    case SPPM_MAT_BIMETAL: {  // material.h:437-449
+        const f3 N = dot(p.n, p.in) < 0 ? p.n : -p.n;
+        if (!opaque_reflect(p)) return 0.0f;
+        f3 ref = reflect_sampling(N, p.in);
+        if (dot(ref, p.out) > 1.0f - 1.0f / SPPM_DELTA_PDF)
+            return (float)((double)SPPM_DELTA_PDF / ((double)dot(p.out, N) + 0.00001));
+        return 0.0f;
+    }
+    case SPPM_MAT_DIFFUSE_TRANSMIT:  // material.h:666-674
+        return dot(p.n, p.in) * dot(p.n, p.out) < 0.0f ? p.refl / SPPM_PI : (1.0f - p.refl) / SPPM_PI;
+    case SPPM_MAT_GLOSS_TRANSMIT:
+        return brdf_gloss_transmit(p);
     }
     return 0.0f;
 }
@@ -435,6 +476,18 @@ __device__ inline float surf_pdf(int kind, const FParam& p) {
         f3 rac = refract_sampling(NN, p.in, p.from, p.to);
         return ((double)fabs_(dot(rac, p.out)) > 0.99) ? SPPM_DELTA_PDF * (1.0f - refl) : SPPM_SMIN;
     }
+    case SPPM_MAT_BIMETAL: {  // material.h:450-461
+        const f3 N = dot(p.n, p.in) < 0 ? p.n : -p.n;
+        f3 ref = reflect_sampling(N, p.in);
+        return (dot(ref, p.out) > 1.0f - 1.0f / SPPM_DELTA_PDF) ? SPPM_DELTA_PDF : SPPM_SMIN;
+    }
+    case SPPM_MAT_DIFFUSE_TRANSMIT: {  // material.h:675-683
+        const float c = (float)__builtin_fabs((double)dot(p.out, p.n) + 0.00001);
+        return dot(p.n, p.in) * dot(p.n, p.out) < 0.0f ? p.refl * c / SPPM_PI : (1.0f - p.refl) * c / SPPM_PI;
+    }
+    case SPPM_MAT_GLOSS_TRANSMIT:  // material.h:757-765
+        if (dot(p.n, p.in) * dot(p.n, p.out) < 0.0f) return p.refl * ggx_sampling_pdf(p.out, p.n, p.in, p.rough);
+        return (1.0f - p.refl) * ggx_sampling_pdf(reflect_by_n(p.out, p.n), p.n, p.in, p.rough);
     }
     return SPPM_SMIN;
 }
@@ -471,6 +524,31 @@ __device__ inline f3 surf_sample(int kind, const FParam& p, MTCursor& m, uint32_
         float sq = (float)(1.0 - (double)(eta * eta) * (1.0 - (double)(LN * LN)));
         if ((double)sq < 0.000001) return normalized(p.in - p.n * (2.0f * dot(p.n, p.in)));
         return glass_refract(p.from, p.to, p.in, p.n);
+    }
+    case SPPM_MAT_BIMETAL:  // material.h:428-436
+        *pdf = SPPM_DELTA_PDF;
+        return reflect_sampling(dot(p.n, p.in) < 0 ? p.n : -p.n, p.in);
+    case SPPM_MAT_DIFFUSE_TRANSMIT: {  // material.h:651-665: reflect about -n, transmit about n
+        const f3 n = dot(p.n, p.in) > 0 ? p.n : -p.n;
+        if ((double)p.refl > real1(m, ovf)) {
+            f3 r = cos_sampling(m, ovf, -n, pdf);
+            *pdf *= p.refl;
+            return r;
+        }
+        f3 r = cos_sampling(m, ovf, n, pdf);
+        *pdf *= (1.0f - p.refl);
+        return r;
+    }
+    case SPPM_MAT_GLOSS_TRANSMIT: {  // material.h:698-717
+        const f3 n = dot(p.n, p.in) < 0 ? p.n : -p.n;
+        if ((double)p.refl > real1(m, ovf)) {
+            f3 r = ggx_sampling(m, ovf, n, p.in, p.rough, pdf);
+            *pdf *= p.refl;
+            return r;
+        }
+        f3 r = reflect_by_n(ggx_sampling(m, ovf, n, p.in, p.rough, pdf), n);
+        *pdf *= (1.0f - p.refl);
+        return r;
     }
     }
     *pdf = 1.0f;

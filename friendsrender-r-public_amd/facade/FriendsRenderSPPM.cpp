@@ -23,6 +23,9 @@ int MetalSurface::Kind() const { return SPPM_MAT_METAL; }
 int Roughconductor::Kind() const { return SPPM_MAT_ROUGH_CONDUCTOR; }
 int GGXGlass::Kind() const { return SPPM_MAT_GGX_GLASS; }
 int GlassSurface::Kind() const { return SPPM_MAT_GLASS; }
+int BiMetalSurface::Kind() const { return SPPM_MAT_BIMETAL; }
+int DiffuseTransmitSurface::Kind() const { return SPPM_MAT_DIFFUSE_TRANSMIT; }
+int GlossTransmitSurface::Kind() const { return SPPM_MAT_GLOSS_TRANSMIT; }
 }  // namespace Material
 
 // Default plug-in (SceneMaterial.h:13-16): a diffuse surface, other

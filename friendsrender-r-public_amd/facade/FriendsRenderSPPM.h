@@ -9,7 +9,9 @@
 //   PTUtility::SceneMaterialManager::SetParamtoAttribute_p  SceneMaterial.h:9-33
 //   PTUtility::Attribute / SURFACE_PARAM      Normal_Facet.h:86-105, material.h:35-90
 //   PTUtility::Material::{DiffuseSurface, MetalSurface, Roughconductor,
-//                         GGXGlass, GlassSurface}            material.h
+//                         GGXGlass, GlassSurface, BiMetalSurface,
+//                         DiffuseTransmitSurface, GlossTransmitSurface}
+//                                             material.h
 //   iCamera / PinholeCamera / ThinLensCamera  Camera.h:11-160
 //   iCameraSsmpler / UniformSample / HaltonSample / HammersleySample
 //                                             CameraSampler.h:7-55
@@ -61,6 +63,9 @@ struct MetalSurface : Surface { int Kind() const override; };
 struct Roughconductor : Surface { int Kind() const override; };
 struct GGXGlass : Surface { int Kind() const override; };
 struct GlassSurface : Surface { int Kind() const override; };
+struct BiMetalSurface : Surface { int Kind() const override; };
+struct DiffuseTransmitSurface : Surface { int Kind() const override; };
+struct GlossTransmitSurface : Surface { int Kind() const override; };
 
 // SURFACE_PARAM with the reference's defaults (material.h:77-89).
 struct SURFACE_PARAM {

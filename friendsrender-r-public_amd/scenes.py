@@ -19,8 +19,8 @@ Workloads:
 """
 import numpy as np
 
-from sppm_amd import (CAM_PINHOLE, CAM_THINLENS, MAT_DIFFUSE, MAT_GGX_GLASS, MAT_METAL, MAT_ROUGH_CONDUCTOR,
-                      SAMPLER_UNIFORM)
+from sppm_amd import (CAM_PINHOLE, CAM_THINLENS, MAT_BIMETAL, MAT_DIFFUSE, MAT_DIFFUSE_TRANSMIT, MAT_GGX_GLASS,
+                      MAT_GLOSS_TRANSMIT, MAT_METAL, MAT_ROUGH_CONDUCTOR, SAMPLER_UNIFORM)
 
 
 class Scene:
@@ -662,3 +662,39 @@ def screens_scene(seed=11, stack=8):
         z = 0.9 - 0.05 * k
         b.quad((0.91, 0.1, z), (0.99, 0.1, z), (0.99, 0.9, z), (0.91, 0.9, z), n, 6)
     return b.scene(attrs, textures)
+
+
+def transmit_scene():
+    """The box with the reference main scene's extra surfaces
+    (main.cpp:14-114 sets them up the same way):
+      leaf    DiffuseTransmit quad (Reflectance 0.88, not opaque, hit points),
+      bell    DiffuseTransmit sphere (Reflectance 0.7, not opaque, hit points),
+      star    GlossTransmit sphere (roughness 0.25, Reflectance 0.1, not
+              opaque, hit points),
+      panel   BiMetal quad (delta, opaque, no hit points) leaning on the
+              right wall, facing away from the light on one side."""
+    attrs = [
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),     # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),       # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),     # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+             if_opaque=1),                                                                          # 3
+        dict(name="tree_leaf", kind=MAT_DIFFUSE_TRANSMIT, albedo=(0.3, 0.7, 0.2), reflectance=0.88, is_sppm=1,
+             if_opaque=0),                                                                          # 4
+        dict(name="ornament_bell_top", kind=MAT_DIFFUSE_TRANSMIT, albedo=(0.9, 0.8, 0.3), reflectance=0.7,
+             is_sppm=1, if_opaque=0),                                                               # 5
+        dict(name="star_outside", kind=MAT_GLOSS_TRANSMIT, albedo=(0.97, 0.98, 0.98), roughness=0.25,
+             reflectance=0.1, is_sppm=1, if_opaque=0),                                              # 6
+        dict(name="spotlight_cone", kind=MAT_BIMETAL, albedo=(0.9, 0.9, 0.9), if_delta=1, is_sppm=0,
+             if_opaque=1),                                                                          # 7
+    ]
+    b = _Builder()
+    box_walls(b, 0, 1, 2)
+    light_quad(b, 3)
+    b.quad((0.12, 0.35, 0.6), (0.45, 0.35, 0.45), (0.45, 0.7, 0.45), (0.12, 0.7, 0.6), tuple(
+        np.cross(np.subtract((0.45, 0.35, 0.45), (0.12, 0.35, 0.6)), (0, 1, 0)) /
+        np.linalg.norm(np.cross(np.subtract((0.45, 0.35, 0.45), (0.12, 0.35, 0.6)), (0, 1, 0)))), 4)
+    b.mesh(*uv_sphere((0.3, 0.15, 0.3), 0.15, nseg=24, nring=12), a=5)
+    b.mesh(*uv_sphere((0.7, 0.22, 0.55), 0.2, nseg=32, nring=16), a=6)
+    b.quad((0.97, 0.05, 0.1), (0.75, 0.05, 0.1), (0.75, 0.6, 0.35), (0.97, 0.6, 0.35), (0, -0.41, 0.91), 7)
+    return b.scene(attrs)

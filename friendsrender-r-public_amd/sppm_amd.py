@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsppm_amd.so")
 
 MAT_DIFFUSE, MAT_METAL, MAT_ROUGH_CONDUCTOR, MAT_GGX_GLASS, MAT_GLASS = range(5)
+MAT_BIMETAL, MAT_DIFFUSE_TRANSMIT, MAT_GLOSS_TRANSMIT = 5, 6, 7
 CAM_PINHOLE, CAM_THINLENS = 0, 1
 SAMPLER_UNIFORM, SAMPLER_HALTON, SAMPLER_HAMMERSLEY = 0, 1, 2
 

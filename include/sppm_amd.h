@@ -51,6 +51,9 @@ extern "C" {
 #define SPPM_MAT_ROUGH_CONDUCTOR 2  /* Roughconductor   material.h:222-278 */
 #define SPPM_MAT_GGX_GLASS 3        /* GGXGlass         material.h:280-394 */
 #define SPPM_MAT_GLASS 4            /* GlassSurface     material.h:465-532 */
+#define SPPM_MAT_BIMETAL 5          /* BiMetalSurface   material.h:427-463 */
+#define SPPM_MAT_DIFFUSE_TRANSMIT 6 /* DiffuseTransmitSurface material.h:646-683 (Reflectance = reflected share) */
+#define SPPM_MAT_GLOSS_TRANSMIT 7   /* GlossTransmitSurface   material.h:685-766 (Reflectance, Roughness) */
 
 /* Attribute = SURFACE_PARAM (material.h:35-90) + surface kind + texture
  * roles map_Kd / map_d / map_bump (Scene.cpp:72-75). */

@@ -559,6 +559,35 @@ static float brdf_ggx_glass(const fparam* p) {  /* material.h:314-390 */
     }
 }
 
+/* GlossTransmitSurface::ReflectByN (material.h:692-695): r - 2 (n.r) n */
+static v3 reflect_by_n(v3 r, v3 n) { return vsub(r, vscale(n, 2.0f * vdot(n, r))); }
+
+/* GlossTransmitSurface::BRDF (material.h:719-756) */
+static float brdf_gloss_transmit(const fparam* p) {
+    float power = vdot(p->n, p->in) * vdot(p->n, p->out) < 0.0f ? p->refl : 1.0f - p->refl;
+    v3 InDir = p->in;
+    v3 Normal = vdot(p->n, InDir) < 0 ? p->n : vneg(p->n);
+    v3 OutDir = p->out;
+    if (vdot(OutDir, Normal) < 0) OutDir = reflect_by_n(OutDir, Normal);
+    const float smallValue = 1e-10f;
+    if (vdot(InDir, Normal) > -smallValue || vdot(OutDir, Normal) < smallValue) return 0.0f;
+    float alpha = p->rough;
+    v3 h = vnormalized(vadd(vneg(InDir), OutDir));
+    float ccs = vdot(h, Normal);
+    if (ccs < smallValue || vdot(h, InDir) > -smallValue || vdot(h, OutDir) < smallValue) return 0.0f;
+    /* std::pow(float, 2.0f) is the float overload: the rounded square */
+    float q = (1.0001f - (1.0f - alpha * alpha) * ccs * ccs);
+    float D = fmaxf_(SMIN, (alpha * alpha) / (PIf * (q * q)));
+    float lc = fmaxf_(0.0f, fminf_(0.9999f, vdot(InDir, vneg(Normal))));
+    float lv = fmaxf_(0.0f, fminf_(0.9999f, vdot(OutDir, Normal)));
+    if (lc < SMIN || lv < SMIN) return 0.0f;
+    float L_l = (float)((-1.0 + sqrt(1.0 + (double)(alpha * alpha) * ((double)(1.0f / (lc * lc)) - 1.0))) * 0.5);
+    float L_v = (float)((-1.0 + sqrt(1.0 + (double)(alpha * alpha) * ((double)(1.0f / (lv * lv)) - 1.0))) * 0.5);
+    float G = 1.0f / ((1.0f + L_l) * (1.0f + L_v));
+    float fx = (4.0f * vdot(InDir, vneg(Normal)) * vdot(OutDir, Normal));
+    return power * (G * D) / fx;
+}
+
 static float surf_brdf(int kind, const fparam* p) {
     switch (kind) {
     case SPPM_MAT_DIFFUSE:  /* material.h:168-173 */
@@ -581,6 +610,19 @@ static float surf_brdf(int kind, const fparam* p) {
         if (vdot(p->n, p->in) * vdot(p->n, p->out) < 0.0f) return (float)((double)(DELTA_PDF * refl) / denom);
         return (float)((double)(DELTA_PDF * (1.0f - refl)) / denom);
     }
+    case SPPM_MAT_BIMETAL: {  /* material.h:437-449: MetalSurface about the flipped normal */
+        v3 N = vdot(p->n, p->in) < 0 ? p->n : vneg(p->n);
+        if (!opaque_reflect(p)) return 0.0f;
+        v3 ref = reflect_sampling(N, p->in);
+        if (vdot(ref, p->out) > 1.0f - 1.0f / DELTA_PDF)
+            return (float)((double)DELTA_PDF / ((double)vdot(p->out, N) + 0.00001));
+        return 0.0f;
+    }
+    case SPPM_MAT_DIFFUSE_TRANSMIT:  /* material.h:666-674 */
+        if (vdot(p->n, p->in) * vdot(p->n, p->out) < 0.0f) return p->refl / PIf;
+        return (1.0f - p->refl) / PIf;
+    case SPPM_MAT_GLOSS_TRANSMIT:
+        return brdf_gloss_transmit(p);
     }
     return 0.0f;
 }
@@ -610,6 +652,19 @@ static float surf_pdf(int kind, const fparam* p) {
             return SMIN;
         }
     }
+    case SPPM_MAT_BIMETAL: {  /* material.h:450-461 */
+        v3 N = vdot(p->n, p->in) < 0 ? p->n : vneg(p->n);
+        v3 ref = reflect_sampling(N, p->in);
+        return (vdot(ref, p->out) > 1.0f - 1.0f / DELTA_PDF) ? DELTA_PDF : SMIN;
+    }
+    case SPPM_MAT_DIFFUSE_TRANSMIT: {  /* material.h:675-683: std::abs of a double */
+        float c = (float)fabs((double)vdot(p->out, p->n) + 0.00001);
+        if (vdot(p->n, p->in) * vdot(p->n, p->out) < 0.0f) return p->refl * c / PIf;
+        return (1.0f - p->refl) * c / PIf;
+    }
+    case SPPM_MAT_GLOSS_TRANSMIT:  /* material.h:757-765 */
+        if (vdot(p->n, p->in) * vdot(p->n, p->out) < 0.0f) return p->refl * ggx_sampling_pdf(p->out, p->n, p->in, p->rough);
+        return (1.0f - p->refl) * ggx_sampling_pdf(reflect_by_n(p->out, p->n), p->n, p->in, p->rough);
     }
     return SMIN;
 }
@@ -647,6 +702,35 @@ static v3 surf_sample(int kind, const fparam* p, orc_mt* m, float* pdf) {
         float sq = (float)(1.0 - (double)(eta * eta) * (1.0 - (double)(LN * LN)));
         if ((double)sq < 0.000001) return glass_reflect(p->in, p->n);
         return glass_refract(p->from, p->to, p->in, p->n);
+    }
+    case SPPM_MAT_BIMETAL: {  /* material.h:428-436 */
+        v3 N = vdot(p->n, p->in) < 0 ? p->n : vneg(p->n);
+        *pdf = DELTA_PDF;
+        return reflect_sampling(N, p->in);
+    }
+    case SPPM_MAT_DIFFUSE_TRANSMIT: {  /* material.h:651-665 */
+        v3 n = vdot(p->n, p->in) > 0 ? p->n : vneg(p->n);
+        v3 rs;
+        if ((double)p->refl > orc_mt_real1(m)) {
+            rs = cos_sampling(m, vneg(n), pdf);
+            *pdf *= p->refl;
+        } else {
+            rs = cos_sampling(m, n, pdf);
+            *pdf *= (1.0f - p->refl);
+        }
+        return rs;
+    }
+    case SPPM_MAT_GLOSS_TRANSMIT: {  /* material.h:698-717 */
+        v3 n = vdot(p->n, p->in) < 0 ? p->n : vneg(p->n);
+        v3 rs;
+        if ((double)p->refl > orc_mt_real1(m)) {
+            rs = ggx_sampling(m, n, p->in, p->rough, pdf);
+            *pdf *= p->refl;
+        } else {
+            rs = reflect_by_n(ggx_sampling(m, n, p->in, p->rough, pdf), n);
+            *pdf *= (1.0f - p->refl);
+        }
+        return rs;
     }
     }
     *pdf = 1.0f;

@@ -225,3 +225,12 @@ def test_occlusion_bit_exact(width):
     np.testing.assert_array_equal(r.occluded(p.astype(np.float32), dirs, 1e-4, mx, 0.5),
                                   os_.occluded(p.astype(np.float32), dirs, 1e-4, mx, 0.5))
     r.close()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_pass_parity_transmit_surfaces(width):
+    """DiffuseTransmit, GlossTransmit and BiMetal surfaces (material.h:427-463,
+    646-766): photons through and off them, hit points on them, against the
+    oracle."""
+    sc = scenes.transmit_scene()
+    run_parity(sc, scenes.pinhole_camera(48, 48), 0.9, 0.015, 20000, 3, width, f"transmit w{width}")
