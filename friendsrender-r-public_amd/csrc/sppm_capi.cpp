@@ -76,6 +76,7 @@ struct SceneBuffers {
     float* ltab_prob = nullptr;
     int* ltab_left = nullptr;
     int* ltab_right = nullptr;
+    int* facet_light = nullptr;
     SceneDev dev{};
     float bmin[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
     int stack_cap = 0;
@@ -91,7 +92,7 @@ struct SceneBuffers {
         dfree(spill); dfree(next_path);
         dfree(facet_nrm); dfree(facet_uv); dfree(facet_attr); dfree(attrs); dfree(tex); dfree(texels);
         dfree(lpick_prob); dfree(lpick_left); dfree(lpick_right); dfree(lights);
-        dfree(ltab_uvw); dfree(ltab_L); dfree(ltab_prob); dfree(ltab_left); dfree(ltab_right);
+        dfree(ltab_uvw); dfree(ltab_L); dfree(ltab_prob); dfree(ltab_left); dfree(ltab_right); dfree(facet_light);
         ready = false;
     }
 };
@@ -396,6 +397,9 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         lights.push_back(L);
     }
     if (lights.empty()) return fail(SPPM_E_SCENE, "scene has no emitting facet (Radiance sum > 1e-5)");
+    // LightSampler::GetAreaLightFromFacetID (LightSampler.cpp:98-106) as a table
+    std::vector<int> facet_light(nf, -1);
+    for (size_t i = 0; i < lights.size(); ++i) facet_light[lights[i].facet] = (int)i;
     std::vector<float> lvals(lights.size()), lprob;
     std::vector<int> lleft, lright;
     for (size_t i = 0; i < lights.size(); ++i) lvals[i] = lights[i].flux_norm;
@@ -412,6 +416,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         D.rough = A.roughness; D.refl = A.reflectance; D.absorb = A.extinction;
         D.tex_albedo = A.tex_albedo; D.tex_rough = A.tex_roughness; D.tex_bump = A.tex_bump;
         D.radiance_norm2 = H::norm2(H::ld(A.radiance));
+        D.delta = A.if_delta ? 1 : 0;
     }
     // ---- textures (+ the environment map after IBL::Init's conversion
     // 0.4 * c^2, LightSampler.cpp:22-26, appended as one more texture) ----
@@ -460,7 +465,8 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         (rc = dalloc(&S.lpick_left, lleft.size())) || (rc = dalloc(&S.lpick_right, lright.size())) ||
         (rc = dalloc(&S.lights, lights.size())) || (rc = dalloc(&S.ltab_uvw, std::max<size_t>(1, tuvw.size()))) ||
         (rc = dalloc(&S.ltab_L, std::max<size_t>(1, tL.size()))) || (rc = dalloc(&S.ltab_prob, std::max<size_t>(1, tprob.size()))) ||
-        (rc = dalloc(&S.ltab_left, std::max<size_t>(1, tleft.size()))) || (rc = dalloc(&S.ltab_right, std::max<size_t>(1, tright.size())))) {
+        (rc = dalloc(&S.ltab_left, std::max<size_t>(1, tleft.size()))) || (rc = dalloc(&S.ltab_right, std::max<size_t>(1, tright.size()))) ||
+        (rc = dalloc(&S.facet_light, nf))) {
         dfree(dpos);
         S.release();
         return rc;
@@ -483,6 +489,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     HIPOK(hipMemcpyAsync(S.lpick_left, lleft.data(), sizeof(int) * lleft.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lpick_right, lright.data(), sizeof(int) * lright.size(), hipMemcpyHostToDevice, st));
     HIPOK(hipMemcpyAsync(S.lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(S.facet_light, facet_light.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
     S.hlights = lights;
     S.h_tuvw = tuvw;
     S.h_tL = tL;
@@ -528,6 +535,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     D.ltab_prob = S.ltab_prob;
     D.ltab_left = S.ltab_left;
     D.ltab_right = S.ltab_right;
+    D.facet_light = S.facet_light;
     // traversal stack bound: each wide level pushes at most W entries
     S.stack_cap = bvh_width * (S.bvh.depth + 1) + 4;
     S.spill_blocks = trace_grid_blocks(bvh_width);
@@ -594,6 +602,7 @@ int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
     if (p->photons_per_pass <= 0) return fail(SPPM_E_ARG, "photons_per_pass must be > 0");
     if (!(p->alpha > 0.0f && p->alpha <= 1.0f)) return fail(SPPM_E_ARG, "alpha must be in (0, 1]");
     if (!(p->first_radius > 0.0f)) return fail(SPPM_E_ARG, "first_radius must be > 0");
+    if (p->eye_nee < SPPM_NEE_OFF || p->eye_nee > SPPM_NEE_KEEP) return fail(SPPM_E_ARG, "eye_nee must be 0, 1 or 2");
     c->params = *p;
     c->have_params = true;
     c->rb.ready = false;
@@ -809,7 +818,7 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
     }
     if (t) t->start(st);
     launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill(c),
-               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, st);
+               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, c->params.eye_nee, st);
     if (t) t->stop(st);
     if (uni) {
         HIPOK(hipEventRecord(c->ev_eye[b], st));
@@ -846,7 +855,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
                                GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
                                R.rstat, R.ctr, R.flags, c->guided ? R.pdep : nullptr, st);
     }
-    if (c->scene.dev.ibl_tex >= 0) launch_direct_light(R.hits, R.shared, R.np, st);
+    if (c->scene.dev.ibl_tex >= 0 || c->params.eye_nee != SPPM_NEE_OFF) launch_direct_light(R.hits, R.shared, R.np, st);
     return SPPM_OK;
 }
 
@@ -971,7 +980,7 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
                   c->guided ? &c->emit : nullptr, R.flags, st);
     do_build(c, st);
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
-    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, st);
+    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, c->params.eye_nee, st);
     int rc = do_gather(c, st);
     if (rc) return rc;
     if (c->guided) do_guide(c, st);
@@ -1435,7 +1444,7 @@ int sppm_read_direct(sppm_ctx* c, float* hit_direct, float* shared_direct, int32
     std::vector<float4> h(np), sh(np);
     HIPOK(hipMemcpy(h.data(), c->rb.hits.direct, sizeof(float4) * np, hipMemcpyDeviceToHost));
     HIPOK(hipMemcpy(sh.data(), c->rb.shared.direct, sizeof(float4) * np, hipMemcpyDeviceToHost));
-    const bool ibl = c->scene.dev.ibl_tex >= 0;
+    const bool ibl = c->scene.dev.ibl_tex >= 0 || c->params.eye_nee != SPPM_NEE_OFF;
     for (size_t i = 0; i < np; ++i) {
         if (hit_direct) {  // written by the eye pass only with an environment light
             hit_direct[3 * i] = ibl ? h[i].x : 0.0f;

@@ -38,6 +38,7 @@ struct DevAttr {       // SURFACE_PARAM subset + surface kind + texture roles
     float absorb;
     int tex_albedo, tex_rough, tex_bump;
     float radiance_norm2;
+    int delta;  // IfDeltaSurface
 };
 
 struct TexDesc {
@@ -82,6 +83,7 @@ struct SceneDev {
     const float* __restrict__ ltab_prob;
     const int* __restrict__ ltab_left;
     const int* __restrict__ ltab_right;
+    const int* __restrict__ facet_light;  // area light of each facet, -1 none
     // environment light: texture id of the converted map (0.4 c^2), -1 none
     int ibl_tex;
 };

@@ -143,6 +143,37 @@ struct PhotonPath {
     int i, pidx, loopcount, depth;
 };
 
+// LightSampler::SamplePointOnAreaLight (LightSampler.cpp:68-82): a point of
+// light L, its radiance and its area density.
+__device__ __forceinline__ void light_point(const SceneDev& S, const DevLight& L, MTCursor& m, uint32_t* ovf,
+                                            f3& lpos, f3& rad, float& pdf_area) {
+    f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
+    if (L.textured) {  // LightSampler.cpp:70-76
+        const int off = L.tab_off;
+        int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, m, ovf);
+        float4 w = S.ltab_uvw[off + k];
+        float4 Lk = S.ltab_L[off + k];
+        lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
+        rad = mk(Lk.x, Lk.y, Lk.z);
+        float gp = w.w / L.tab_avg / 1000.0f;
+        pdf_area = (gp * 1000.0f) / L.area;
+    } else {  // PointOnTriangle (Sampling.cpp:339-349)
+        float u = (float)real1(m, ovf);
+        float v = (float)real1(m, ovf);
+        if (v > u) { u *= 0.5f; v -= u; }
+        else { v *= 0.5f; u -= v; }
+        pdf_area = 2.0f / fabs_(norm(cross(p1 - p0, p2 - p0)));
+        lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
+        rad = mk(L.flux[0], L.flux[1], L.flux[2]) / (2.0f * SPPM_PI * L.area);
+    }
+}
+
+// LightSampler::GetAreaLightPDF (LightSampler.h:87): TableSampler::GetPDF of
+// the light pick table, f / average / N.
+__device__ __forceinline__ float light_pick_pdf(const SceneDev& S, const DevLight& L) {
+    return L.flux_norm / S.lpick_avg / (float)S.nlights;
+}
+
 // generateLightRay (SPPM.cpp:453-471) + the emission photon in slot 0
 // (SPPM.cpp:530-540); the first stored surface photon overwrites that slot
 // (SPPM.cpp:606), so writing it up front leaves the same slot contents.
@@ -152,28 +183,10 @@ __device__ __forceinline__ void photon_start(const SceneDev& S, const SlotsDev& 
     mt_seed(P.m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
     const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
     const DevLight& L = S.lights[li];
-    f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
     f3 lpos, rad;
     float pdf_area;
-    if (L.textured) {  // LightSampler.cpp:70-76
-        const int off = L.tab_off;
-        int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, P.m, ovf);
-        float4 w = S.ltab_uvw[off + k];
-        float4 Lk = S.ltab_L[off + k];
-        lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
-        rad = mk(Lk.x, Lk.y, Lk.z);
-        float gp = w.w / L.tab_avg / 1000.0f;
-        pdf_area = (gp * 1000.0f) / L.area;
-    } else {  // PointOnTriangle (Sampling.cpp:339-349)
-        float u = (float)real1(P.m, ovf);
-        float v = (float)real1(P.m, ovf);
-        if (v > u) { u *= 0.5f; v -= u; }
-        else { v *= 0.5f; u -= v; }
-        pdf_area = 2.0f / fabs_(norm(cross(p1 - p0, p2 - p0)));
-        lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
-        rad = mk(L.flux[0], L.flux[1], L.flux[2]) / (2.0f * SPPM_PI * L.area);
-    }
-    pdf_area *= L.flux_norm / S.lpick_avg / (float)S.nlights;
+    light_point(S, L, P.m, ovf, lpos, rad, pdf_area);
+    pdf_area *= light_pick_pdf(S, L);
     const f3 lnorm = mk(L.normal[0], L.normal[1], L.normal[2]);
     float pdf_dir = 1.0f;
     f3 outdir = cos_sampling(P.m, ovf, lnorm, &pdf_dir);
@@ -623,9 +636,47 @@ __device__ __forceinline__ f3 ibl_color(const SceneDev& S, f3 dir) {
     return mk(c.x, c.y, c.z);
 }
 
+// Scene::IsOccluded (Scene.cpp:246-281): the closest hit in [MinDist, MaxDist)
+// occludes unless its surface's albedo texture makes it more translucent
+// than the threshold (1 - alpha > translucency); then the ray goes on from
+// just behind it with the range shortened, renormalised as Ray's constructor
+// does.  The reference recurses; this loops, passing at most OCC_LAYERS
+// surfaces (flag bit 2 beyond, counted as occluded).
+constexpr int OCC_LAYERS = 256;
 template <int W>
+__device__ __forceinline__ bool occluded_ray(const SceneDev& S, f3 o, f3 d, float mn, float mx, float transl,
+                                             const TraceStack& ts, uint32_t& fl) {
+    for (int layer = 0;; ++layer) {
+        if (layer == OCC_LAYERS) {
+            fl |= 4u;
+            return true;
+        }
+        HitRec h;
+        if (!closest_hit<W>(S, o, d, mn, ts, h, &fl) || !((double)h.t < (double)mx - 1e-10)) return false;
+        const DevAttr& a = S.attrs[S.facet_attr[h.facet]];
+        if (a.tex_albedo < 0) return true;
+        const float2* tv = S.facet_uv + 3 * (size_t)h.facet;
+        const float2 t0 = tv[0], t1 = tv[1], t2 = tv[2];
+        const float uvx = ((t1.x - t0.x) * h.u + (t2.x - t0.x) * h.v) + t0.x;
+        const float uvy = ((t1.y - t0.y) * h.u + (t2.y - t0.y) * h.v) + t0.y;
+        const TexDesc td = S.tex[a.tex_albedo];
+        const float4 c = tex_bilinear(S.texels + td.offset, td.width, td.height, uvx, uvy);
+        const float tr = (float)(1.0 - (double)c.w);
+        if (!(tr > transl)) return true;
+        const float dist = h.t + SPPM_RAY_OFFSET;
+        o = o + d * dist;
+        d = normalized(d);
+        mn = fmaxf(mn - dist, SPPM_RAY_OFFSET);
+        mx = mx - dist;
+    }
+}
+
+// NEE: the compiled-out next-event estimation of getHitpointInfo
+// (SPPM.cpp:144-168, 187-229); keep: keep what it estimates (SPPM_NEE_KEEP).
+template <int W, bool NEE>
 __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int i,
-                                         const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl) {
+                                         const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl,
+                                         bool keep) {
     uint32_t ovf = 0;
     MTCursor m;
     mt_seed(m, (uint32_t)i + (uint32_t)C.W * (uint32_t)C.H * (uint32_t)step);
@@ -655,6 +706,10 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
     float hidx = 1.0f, hrough = 1.0f, habs = 1.0f, hrefl = 1.0f;
     f3 direct = mk(0.0f, 0.0f, 0.0f);
     const bool ibl = S.ibl_tex >= 0;
+    // TraceParam::PrevGmt (position, IfDeltaSurface) and pdf_prevDir (SPPM.h:78-80)
+    f3 prev_pos = mk(0.0f, 0.0f, 0.0f);
+    float pdf_prev = 0.0f;
+    bool prev_delta = false;
     for (;;) {
         if (depth > 6) break;
         else if (depth > 4) {
@@ -678,8 +733,32 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
             cur_org = g.pos + cur_dir * SPPM_RAY_OFFSET;
             continue;
         }
+        if (NEE && (double)norm2(g.radiance) > 1e-6) {
+            // an emitter ends the path; before the hit point its radiance is
+            // DirectLight, MIS-weighted against the light sample of the
+            // previous vertex unless that was a delta surface (SPPM.cpp:145-167)
+            if (type != 1) {
+                float mis = 1.0f;
+                if (depth > 0 && !prev_delta) {
+                    const int li = S.facet_light[h.facet];
+                    float pdf_nee = 0.0f;  // GetAreaLightPDF of a facet that is no light: 0
+                    if (li >= 0) {
+                        const DevLight& L = S.lights[li];
+                        // SampleAreaLight_pdf (LightSampler.cpp:84-96)
+                        const float ratio = L.textured ? (norm(g.radiance) / L.tab_avg / 1000.0f) * 1000.0f : 1.0f;
+                        const float tri = ratio / L.area;
+                        const float cosL = smax(0.0001f, smin(1.0f, dot(Nf, -cur_dir)));
+                        pdf_nee = (light_pick_pdf(S, L) * tri) * norm2(prev_pos - g.pos) / cosL;
+                    }
+                    mis = (pdf_prev * pdf_prev) / ((pdf_nee * pdf_nee) + (pdf_prev * pdf_prev));
+                }
+                direct = direct + (((g.albedo * mis) * g.radiance) * T) / PDF;
+                if (keep && type == 0) type = 2;
+            }
+            break;
+        }
         if (g.is_sppm && type != 1) {  // makeHitPoint (SPPM.cpp:79-96, 171-174)
-            direct = mk(0.0f, 0.0f, 0.0f);  // the new HitPoint's DirectLight
+            if (!(NEE && keep)) direct = mk(0.0f, 0.0f, 0.0f);  // the new HitPoint's DirectLight
             type = 1;
             facet = h.facet;
             hpos = g.pos;
@@ -699,6 +778,38 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         if (near1(cur_index)) { Ii = 1.0f; Io = g.index; }
         else { Ii = cur_index; Io = 1.0f; }
         FParam fp{cur_dir, next_dir, Nf, Ii, Io, g.rough, g.absorb, g.refl};
+        bool delta = false;
+        if (NEE) {
+            delta = S.attrs[S.facet_attr[h.facet]].delta != 0;
+            if (!delta && type != 1) {
+                // a light sample from this vertex (SPPM.cpp:190-226): pick a
+                // light, a point on it, and a shadow ray from the hit position
+                const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, m, &ovf);
+                const DevLight& L = S.lights[li];
+                const float psel = light_pick_pdf(S, L);
+                f3 lpos, rad;
+                float parea;
+                light_point(S, L, m, &ovf, lpos, rad, parea);
+                const f3 ln = mk(L.normal[0], L.normal[1], L.normal[2]);
+                const f3 dv = lpos - g.pos;
+                const f3 ldir = normalized(normalized(dv));  // normalised, then again by Ray's ctor
+                const float dist = norm(dv);
+                const float maxd = (float)((double)dist - (double)SPPM_RAY_OFFSET * 2.0);
+                if (!occluded_ray<W>(S, g.pos, ldir, SPPM_RAY_OFFSET, maxd, 0.5f, ts, fl)) {
+                    const float cosL = smin(1.0f, smax(0.00001f, dot(ln, -ldir)));
+                    const float cosE = fabs_(dot(Nf, ldir));
+                    FParam fl2 = fp;
+                    fl2.out = ldir;
+                    const float b = smax(0.0f, surf_brdf(g.kind, fl2));
+                    const float pp = parea * psel;
+                    const f3 dlr = (((((rad * cosL) * powf(dist, -2.0f)) / pp) * cosE) * g.albedo) * b;
+                    const float pn = pp * (dist * dist) / cosL;
+                    const float pt = surf_pdf(g.kind, fl2);
+                    const float w = (pn * pn) / ((pn * pn) + (pt * pt));
+                    direct = direct + ((dlr * w) * T) / PDF;
+                }
+            }
+        }
         float pdf_s = 1.0f;
         next_dir = surf_sample(g.kind, fp, m, &ovf, &pdf_s);
         if (!rutil_valid_dir(next_dir)) break;
@@ -717,13 +828,18 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         if (pdf_s < 0) break;
         T = T * ((g.albedo * cs) * brdf);
         PDF *= pdf_s;
+        if (NEE) {
+            pdf_prev = pdf_s;
+            prev_pos = g.pos;
+            prev_delta = delta;
+        }
         cur_org = next_org;
         cur_dir = next_dir;
         cur_index = next_index;
         ++depth;
     }
     hw = hw / campdf;  // createHitpoints: PixelWeight /= pdf, DirectLight /= pdf (SPPM.cpp:346-347)
-    if (ibl) {
+    if (ibl || NEE) {
         direct = direct / campdf;
         H.direct[i] = make_float4(direct.x, direct.y, direct.z, 0.0f);
     }
@@ -738,11 +854,11 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
 }
 
 // Eye pass: a fixed grid of waves strides over the pixels.
-template <int W>
+template <int W, bool NEE>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
                                                     const int* __restrict__ step_dev, int cap,
                                                     uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
-                                                    uint32_t* __restrict__ flags) {
+                                                    uint32_t* __restrict__ flags, int keep) {
     const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
@@ -750,7 +866,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDe
     const int NPIX = C.W * C.H;
     uint32_t fl = 0;
     for (int i = blockIdx.x * PATH_BLOCK + lane; i < NPIX; i += gridDim.x * PATH_BLOCK)
-        eye_path<W>(S, C, H, step, i, words, ts, fl);
+        eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0);
     if (fl) atomicOr(flags, fl);
 }
 
@@ -1412,13 +1528,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const f
     if (fl) atomicOr(flags, fl);
 }
 
-// Scene::IsOccluded (Scene.cpp:246-281): the closest hit in [MinDist, MaxDist)
-// (MaxDist per ray, as at its call site SPPM.cpp:205)
-// occludes unless its surface's albedo texture makes it more translucent
-// than the threshold (1 - alpha > translucency); then the ray goes on from
-// just behind it with the range shortened, renormalised as Ray's constructor
-// does.  At most OCC_LAYERS surfaces are passed (flag bit 2 beyond).
-constexpr int OCC_LAYERS = 256;
+// Scene::IsOccluded for a batch of rays (MaxDist per ray).
 template <int W>
 __global__ void __launch_bounds__(PATH_BLOCK) k_occluded(SceneDev S, int n, const float* __restrict__ org,
                                                          const float* __restrict__ dir, float tmin,
@@ -1430,41 +1540,9 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_occluded(SceneDev S, int n, cons
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t fl = 0;
     for (int i = blockIdx.x * PATH_BLOCK + lane; i < n; i += gridDim.x * PATH_BLOCK) {
-        f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
-        f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-        float mn = tmin, mx = tmax[i];
-        int occ = 0;
-        for (int layer = 0;; ++layer) {
-            if (layer == OCC_LAYERS) {
-                occ = 1;
-                fl |= 4u;
-                break;
-            }
-            HitRec h;
-            if (!closest_hit<W>(S, o, d, mn, ts, h, &fl) || !((double)h.t < (double)mx - 1e-10)) break;
-            const DevAttr& a = S.attrs[S.facet_attr[h.facet]];
-            if (a.tex_albedo < 0) {
-                occ = 1;
-                break;
-            }
-            const float2* tv = S.facet_uv + 3 * (size_t)h.facet;
-            const float2 t0 = tv[0], t1 = tv[1], t2 = tv[2];
-            const float uvx = ((t1.x - t0.x) * h.u + (t2.x - t0.x) * h.v) + t0.x;
-            const float uvy = ((t1.y - t0.y) * h.u + (t2.y - t0.y) * h.v) + t0.y;
-            const TexDesc td = S.tex[a.tex_albedo];
-            const float4 c = tex_bilinear(S.texels + td.offset, td.width, td.height, uvx, uvy);
-            const float tr = (float)(1.0 - (double)c.w);
-            if (!(tr > transl)) {
-                occ = 1;
-                break;
-            }
-            const float dist = h.t + SPPM_RAY_OFFSET;
-            o = o + d * dist;
-            d = normalized(d);
-            mn = fmaxf(mn - dist, SPPM_RAY_OFFSET);
-            mx = mx - dist;
-        }
-        occluded[i] = occ;
+        const f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+        const f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        occluded[i] = occluded_ray<W>(S, o, d, tmin, tmax[i], transl, ts, fl) ? 1 : 0;
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -1510,12 +1588,21 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
 #undef SPPM_PHOTON
 }
 
-void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, hipStream_t st) {
+template <bool NEE>
+static void launch_eye_t(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
+                         const TraceSpill& sp, const uint32_t* words, uint32_t* flags, int keep, hipStream_t st) {
     dim3 g(sp.blocks), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_eye<2>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_eye<4>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
-    else hipLaunchKernelGGL(k_eye<8>, g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags);
+    if (S.width == 2)
+        hipLaunchKernelGGL((k_eye<2, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+    else if (S.width == 4)
+        hipLaunchKernelGGL((k_eye<4, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+    else
+        hipLaunchKernelGGL((k_eye<8, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+}
+void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, int nee, hipStream_t st) {
+    if (nee == SPPM_NEE_OFF) launch_eye_t<false>(S, C, H, step, step_dev, sp, words, flags, 0, st);
+    else launch_eye_t<true>(S, C, H, step, step_dev, sp, words, flags, nee == SPPM_NEE_KEEP ? 1 : 0, st);
 }
 
 __global__ void k_add_i32(int* __restrict__ p, int v) {

@@ -537,6 +537,7 @@ void StochasticProgressivePhotonMapping::prepare(const Scene& scene, const iCame
     p.first_radius = m_FirstRadius;
     p.photons_per_pass = photonsPerPass;
     p.bvh_width = m_BVHWidth;
+    p.eye_nee = m_EyeNEE;
     check(sppm_params_set(I.ctx, &p), "sppm_params_set");
     check(sppm_begin(I.ctx), "sppm_begin");
     m_Stats = RenderStatistics();

@@ -274,6 +274,10 @@ public:
     void SetFirstRadius(float r) { m_FirstRadius = r; }
     void SetBVHWidth(int w) { m_BVHWidth = w; }
     void SetDevice(int device) { m_Device = device; }
+    // Next-event estimation on eye paths, the reference's compiled-out
+    // blocks of getHitpointInfo (SPPM.cpp:144-168, 187-229): SPPM_NEE_OFF
+    // (default, as the reference is built), SPPM_NEE_AS_WRITTEN, SPPM_NEE_KEEP.
+    void SetEyeNEE(int mode) { m_EyeNEE = mode; }
     const RenderStatistics& GetStatistics() const { return m_Stats; }
 
 private:
@@ -283,6 +287,7 @@ private:
     float m_FirstRadius = 0.015f;
     int m_BVHWidth = 8;
     int m_Device = 0;
+    int m_EyeNEE = 0;
     RenderStatistics m_Stats;
     void prepare(const Scene& scene, const iCamera& camera, int photonsPerPass);
     void finish(Texture2D& renderTarget, int64_t totalPhotons, int passes);

@@ -16,6 +16,7 @@
 //   --bvh W               BVH width 2, 4 or 8
 //   --sampler-id ID       UniformSample seed ID
 //   --ibl FILE            environment map (binary PPM or PFM), Scene::CreateScene's IBL argument
+//   --nee M               eye-path next-event estimation: 0 off, 1 as written, 2 kept
 //   --dump-scene FILE     write the parsed facet records and attribute table
 //                         (no rendering, no GPU)
 #include <cstdio>
@@ -94,7 +95,7 @@ int main(int argc, char** argv) {
     }
     const char* obj = argv[1];
     const char* out = argv[2];
-    int W = 256, H = 256, photons = 100000, passes = 16, seconds = -1, bvh = 8, sid = 0;
+    int W = 256, H = 256, photons = 100000, passes = 16, seconds = -1, bvh = 8, sid = 0, nee = 0;
     bool thin = false, fixstep = false;
     float A = 1.0f, F = 1.0f, fov = 45.0f, alpha = 0.9f, radius = 0.015f;
     float pos[3] = {0.5f, 0.5f, 2.2f}, tgt[3] = {0.5f, 0.5f, 0.5f};
@@ -122,6 +123,7 @@ int main(int argc, char** argv) {
         else if (a == "--radius") { need(1); radius = (float)atof(argv[++i]); }
         else if (a == "--bvh") { need(1); bvh = atoi(argv[++i]); }
         else if (a == "--ibl") { need(1); ibl = argv[++i]; }
+        else if (a == "--nee") { need(1); nee = atoi(argv[++i]); }
         else if (a == "--sampler-id") { need(1); sid = atoi(argv[++i]); }
         else if (a == "--dump-scene") { need(1); dump = argv[++i]; }
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
@@ -150,6 +152,7 @@ int main(int argc, char** argv) {
         render.SetAlpha(alpha);
         render.SetFirstRadius(radius);
         render.SetBVHWidth(bvh);
+        render.SetEyeNEE(nee);
         PinholeCamera pin(W, H, P, T, fov, 1.0f, 1.0f, aspect, &sampler);
         ThinLensCamera lens(W, H, P, T, fov, F, A, aspect, &sampler, nullptr);
         const iCamera& cam = thin ? (const iCamera&)lens : (const iCamera&)pin;

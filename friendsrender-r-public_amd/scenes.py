@@ -698,3 +698,29 @@ def transmit_scene():
     b.mesh(*uv_sphere((0.7, 0.22, 0.55), 0.2, nseg=32, nring=16), a=6)
     b.quad((0.97, 0.05, 0.1), (0.75, 0.05, 0.1), (0.75, 0.6, 0.35), (0.97, 0.6, 0.35), (0, -0.41, 0.91), 7)
     return b.scene(attrs)
+
+
+def nee_scene():
+    """The box with surfaces that carry no hit points but are not delta, where
+    eye-path next-event estimation applies: a GGX conductor sphere (roughness
+    0.3) and a GGX glass sphere, beside a mirror sphere (delta).  The light is
+    seen directly and in the spheres."""
+    attrs = [
+        dict(name="white", kind=MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73), is_sppm=1, if_opaque=1),       # 0
+        dict(name="red", kind=MAT_DIFFUSE, albedo=(0.63, 0.06, 0.05), is_sppm=1, if_opaque=1),         # 1
+        dict(name="green", kind=MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09), is_sppm=1, if_opaque=1),       # 2
+        dict(name="light", kind=MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0), is_sppm=1,
+             if_opaque=1),                                                                            # 3
+        dict(name="lamp_body", kind=MAT_ROUGH_CONDUCTOR, albedo=(0.95, 0.64, 0.54), roughness=0.3, index=1.1978,
+             extinction=7.0488, is_sppm=0, if_opaque=1),                                               # 4
+        dict(name="lamp_glass", kind=MAT_GGX_GLASS, albedo=(0.97, 0.98, 0.98), roughness=0.1, index=1.5,
+             is_sppm=0, if_opaque=0),                                                                 # 5
+        dict(name="mirror", kind=MAT_METAL, albedo=(0.95, 0.95, 0.95), if_delta=1, is_sppm=0, if_opaque=1),  # 6
+    ]
+    b = _Builder()
+    box_walls(b, 0, 1, 2)
+    light_quad(b, 3)
+    b.mesh(*uv_sphere((0.3, 0.25, 0.45), 0.22, nseg=32, nring=16), a=4)
+    b.mesh(*uv_sphere((0.72, 0.2, 0.6), 0.18, nseg=32, nring=16), a=5)
+    b.mesh(*uv_sphere((0.75, 0.62, 0.3), 0.12, nseg=24, nring=12), a=6)
+    return b.scene(attrs)

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libsppm_amd.so")
 MAT_DIFFUSE, MAT_METAL, MAT_ROUGH_CONDUCTOR, MAT_GGX_GLASS, MAT_GLASS = range(5)
 MAT_BIMETAL, MAT_DIFFUSE_TRANSMIT, MAT_GLOSS_TRANSMIT = 5, 6, 7
 CAM_PINHOLE, CAM_THINLENS = 0, 1
+NEE_OFF, NEE_AS_WRITTEN, NEE_KEEP = 0, 1, 2  # sppm_params.eye_nee
 SAMPLER_UNIFORM, SAMPLER_HALTON, SAMPLER_HAMMERSLEY = 0, 1, 2
 
 EXPORTED = [
@@ -99,7 +100,7 @@ class CameraDesc(C.Structure):
 class Params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("first_radius", C.c_float), ("photons_per_pass", C.c_int32),
                 ("bvh_width", C.c_int32), ("block_size", C.c_int32), ("check_numeric", C.c_int32),
-                ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32)]
+                ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32), ("eye_nee", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -241,18 +242,19 @@ def camera_desc(cam):
     return c
 
 
-def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False):
+def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False, nee=0):
     p = Params()
     p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
     p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)
     p.record_neighbours, p.max_neighbours = int(record), 0
+    p.eye_nee = nee
     return p
 
 
 class Renderer:
     """One SPPM render on one GPU: scene + camera + params, then passes."""
 
-    def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0):
+    def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0, nee=NEE_OFF):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_create(C.byref(h), device, None))
@@ -263,7 +265,7 @@ class Renderer:
         _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
         self._cam = camera_desc(cam)
         _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
-        self._params = params_struct(alpha, first_radius, photons, bvh_width, record)
+        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee)
         _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
         _check(self.lib.sppm_begin(self.h))
         self.passes = 0

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPPM_ABI_VERSION 2  /* 2: sppm_guide_params gained hidden, layers, fp16 */
+#define SPPM_ABI_VERSION 3  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -134,7 +134,30 @@ typedef struct sppm_params {
     int32_t check_numeric;     /* 1: fail a pass with SPPM_E_NUMERIC on NaN/Inf flux */
     int32_t record_neighbours; /* 1: keep per-hit-point neighbour lists (debug/parity) */
     int32_t max_neighbours;    /* capacity of the neighbour list buffer (entries) */
+    int32_t eye_nee;           /* SPPM_NEE_*: next-event estimation on eye paths */
 } sppm_params;
+
+/* Next-event estimation with MIS on eye paths: the two blocks of
+ * getHitpointInfo that the reference compiles out with #if 0
+ * (SPPM.cpp:144-168 at emitter hits, SPPM.cpp:187-229 at surfaces before the
+ * hit point; shadow rays through Scene::IsOccluded with translucency 0.5).
+ *   SPPM_NEE_OFF         the reference as compiled (default);
+ *   SPPM_NEE_AS_WRITTEN  both blocks enabled exactly as written: eye paths end
+ *                        at emitters, and the direct light they estimate is
+ *                        kept only by paths that leave the scene before making
+ *                        a hit point (makeHitPoint resets DirectLight, and a
+ *                        path ending on an emitter keeps type NONE);
+ *   SPPM_NEE_KEEP        the same estimates, kept: a hit point starts from
+ *                        the DirectLight gathered before it, and a path ending
+ *                        on an emitter with no hit point counts as
+ *                        ONLY_DIRECTLIGHT.  This adds the light that reaches
+ *                        the eye through surfaces without hit points
+ *                        (emitters seen directly or through glossy and
+ *                        specular non-SPPM surfaces), which photons never
+ *                        carry. */
+#define SPPM_NEE_OFF 0
+#define SPPM_NEE_AS_WRITTEN 1
+#define SPPM_NEE_KEEP 2
 
 /* Per-pass statistics. */
 typedef struct sppm_pass_stats {

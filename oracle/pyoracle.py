@@ -195,10 +195,10 @@ class OracleScene:
 class OracleRender:
     """CPU restatement of one SPPM render (same API shape as sppm_amd.Renderer)."""
 
-    def __init__(self, oscene, cam, alpha, first_radius, photons, nthreads=None):
+    def __init__(self, oscene, cam, alpha, first_radius, photons, nthreads=None, nee=0):
         self.oscene = oscene
         self.cam = camera_desc(cam)
-        self.p = params_struct(alpha, first_radius, photons)
+        self.p = params_struct(alpha, first_radius, photons, nee=nee)
         nt = nthreads or min(16, os.cpu_count() or 1)
         self.h = orc().orc_render_create(oscene.h, C.byref(self.cam), C.byref(self.p), nt)
         self.W, self.H, self.N = cam["width"], cam["height"], photons

@@ -789,6 +789,7 @@ struct orc_scene {
     int32_t* lleft;
     int32_t* lright;
     float lavg;
+    int32_t* facet_light;  /* GetAreaLightFromFacetID (LightSampler.cpp:98-106): light of a facet, -1 none */
     /* environment light: the converted map (LightSampler.cpp:22-26) */
     int has_ibl;
     sppm_texture ibl;
@@ -959,6 +960,9 @@ orc_scene* orc_scene_create(const sppm_scene_desc* d) {
             create_area_light(&s->lights[s->nlights++], s, f, rad, t);
         }
     }
+    s->facet_light = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nf > 0 ? nf : 1));
+    for (int f = 0; f < nf; ++f) s->facet_light[f] = -1;
+    for (int i = 0; i < s->nlights; ++i) s->facet_light[s->lights[i].facet] = i;
     /* light pick table over |Flux| (LightSampler.cpp:6-28) */
     if (s->nlights > 0) {
         float* vals = (float*)malloc(sizeof(float) * s->nlights);
@@ -995,7 +999,7 @@ void orc_scene_destroy(orc_scene* s) {
     }
     for (int i = 0; i < s->ntex; ++i) free(s->texdata[i]);
     free(s->texdata); free(s->tex); free(s->lights);
-    free(s->lprob); free(s->lleft); free(s->lright);
+    free(s->lprob); free(s->lleft); free(s->lright); free(s->facet_light);
     free(s->pos); free(s->nrm); free(s->uv); free(s->attr); free(s->attrs); free(s->nodes);
     free(s->ibl_data);
     free(s);
@@ -1336,6 +1340,38 @@ static void calc_geometry(const orc_scene* s, int f, float u, float v, geom* g) 
 
 static inline int near1(float s_) { return (double)fabsf(s_ - 1.0f) < 0.00001; }  /* SPPM.h:27-29 */
 
+/* LightSampler::SamplePointOnAreaLight (LightSampler.cpp:68-82) */
+static void light_point(const orc_scene* s, const olight* L, orc_mt* m, v3* lpos, v3* rad, float* pdf_area) {
+    const float* P = s->pos + 9 * L->facet;
+    v3 p0 = vload(P), p1 = vload(P + 3), p2 = vload(P + 6);
+    float area = facet_area(P);
+    if (L->textured) {  /* LightSampler.cpp:70-76 */
+        int k = table_sample(L->prob, L->left, L->right, LIGHT_TABLE_SAMPLES, m);
+        v3 w = L->uvw[k];
+        *lpos = vadd(vadd(vscale(p0, w.x), vscale(p1, w.y)), vscale(p2, 1.0f - w.x - w.y));
+        *rad = L->L[k];
+        float gp = vnorm(L->L[k]) / L->avg / (float)LIGHT_TABLE_SAMPLES;
+        *pdf_area = (gp * (float)LIGHT_TABLE_SAMPLES) / area;
+    } else {  /* PointOnTriangle (Sampling.cpp:339-349) */
+        float u = (float)orc_mt_real1(m);
+        float v = (float)orc_mt_real1(m);
+        if (v > u) { u *= 0.5f; v -= u; }
+        else { v *= 0.5f; u -= v; }
+        *pdf_area = 2.0f / fabsf(vnorm(vcross(vsub(p1, p0), vsub(p2, p0))));
+        *lpos = vadd(vadd(vscale(p0, u), vscale(p1, v)), vscale(p2, 1.0f - u - v));
+        *rad = vdivs(L->flux, 2.0f * PIf * area);
+    }
+}
+/* LightSampler::GetAreaLightPDF (LightSampler.h:87) = TableSampler::GetPDF */
+static float light_pick_pdf(const orc_scene* s, const olight* L) {
+    return vnorm(L->flux) / s->lavg / (float)s->nlights;
+}
+/* (n0 + n1 + n2).normalized() (SPPM.cpp:465-467, 201) */
+static v3 light_normal(const orc_scene* s, const olight* L) {
+    const float* Ln = s->nrm + 9 * L->facet;
+    return vnormalized(vadd(vadd(vload(Ln), vload(Ln + 3)), vload(Ln + 6)));
+}
+
 /* createPhotonMap, one path (SPPM.cpp:518-672) */
 static int photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     const orc_scene* s = r->s;
@@ -1345,30 +1381,11 @@ static int photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     /* generateLightRay (SPPM.cpp:453-471) */
     int li = table_sample(s->lprob, s->lleft, s->lright, s->nlights, &m);
     const olight* L = &s->lights[li];
-    const float* P = s->pos + 9 * L->facet;
-    v3 p0 = vload(P), p1 = vload(P + 3), p2 = vload(P + 6);
     v3 lpos, rad;
     float pdf_area;
-    float area = facet_area(P);
-    if (L->textured) {  /* LightSampler.cpp:70-76 */
-        int k = table_sample(L->prob, L->left, L->right, LIGHT_TABLE_SAMPLES, &m);
-        v3 w = L->uvw[k];
-        lpos = vadd(vadd(vscale(p0, w.x), vscale(p1, w.y)), vscale(p2, 1.0f - w.x - w.y));
-        rad = L->L[k];
-        float gp = vnorm(L->L[k]) / L->avg / (float)LIGHT_TABLE_SAMPLES;
-        pdf_area = (gp * (float)LIGHT_TABLE_SAMPLES) / area;
-    } else {  /* PointOnTriangle (Sampling.cpp:339-349) */
-        float u = (float)orc_mt_real1(&m);
-        float v = (float)orc_mt_real1(&m);
-        if (v > u) { u *= 0.5f; v -= u; }
-        else { v *= 0.5f; u -= v; }
-        pdf_area = 2.0f / fabsf(vnorm(vcross(vsub(p1, p0), vsub(p2, p0))));
-        lpos = vadd(vadd(vscale(p0, u), vscale(p1, v)), vscale(p2, 1.0f - u - v));
-        rad = vdivs(L->flux, 2.0f * PIf * area);
-    }
-    pdf_area *= vnorm(L->flux) / s->lavg / (float)s->nlights;
-    const float* Ln = s->nrm + 9 * L->facet;
-    v3 lnorm = vnormalized(vadd(vadd(vload(Ln), vload(Ln + 3)), vload(Ln + 6)));
+    light_point(s, L, &m, &lpos, &rad, &pdf_area);
+    pdf_area *= light_pick_pdf(s, L);
+    v3 lnorm = light_normal(s, L);
     float pdf_dir = 1.0f;
     v3 outdir = cos_sampling(&m, lnorm, &pdf_dir);
     rad = vscale(rad, fminf_(1.0f, fmaxf_(0.0f, vdot(lnorm, outdir))));
@@ -1563,6 +1580,16 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
     hp->weight = V3(1, 1, 1);
     hp->outdir = V3(0, 1, 0);
     v3 direct = V3(0, 0, 0);
+    /* getHitpointInfo's compiled-out next-event estimation (SPPM.cpp:144-168,
+     * 187-229), enabled by eye_nee; keep = SPPM_NEE_KEEP.  eye_nee = 3 exists
+     * only here, as the checker of the MIS estimator: SPPM_NEE_KEEP without
+     * the light samples, emitter hits at full weight (plain BSDF sampling). */
+    const int nee = r->p.eye_nee != SPPM_NEE_OFF;
+    const int keep = r->p.eye_nee >= SPPM_NEE_KEEP;
+    const int light_samples = r->p.eye_nee != 3;
+    v3 prev_pos = V3(0, 0, 0);  /* TraceParam::PrevGmt.Position, .param.IfDeltaSurface, pdf_prevDir */
+    int prev_delta = 0;
+    float pdf_prev = 0.0f;
     for (;;) {
         if (depth > 6) break;
         else if (depth > 4) {
@@ -1588,8 +1615,32 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
             cur_org = vadd(g.pos, vscale(cur_dir, RAY_OFFSET));
             continue;
         }
+        if (nee && (double)vnorm2(g.radiance) > 1e-6) {  /* SPPM.cpp:145-167 */
+            if (hp->type != 1) {
+                float mis = 1.0f;
+                if (depth > 0 && !prev_delta && light_samples) {
+                    int li = s->facet_light[f];
+                    float pdf_nee = 0.0f;  /* a default AreaLight has |Flux| = 0 */
+                    if (li >= 0) {
+                        const olight* L = &s->lights[li];
+                        /* SampleAreaLight_pdf (LightSampler.cpp:84-96) */
+                        float ratio = 1.0f;
+                        if (L->textured)
+                            ratio = (vnorm(g.radiance) / L->avg / (float)LIGHT_TABLE_SAMPLES) * (float)LIGHT_TABLE_SAMPLES;
+                        float tri = ratio / facet_area(s->pos + 9 * f);
+                        float cosL = fmaxf_(0.0001f, fminf_(1.0f, vdot(Nf, vneg(cur_dir))));
+                        pdf_nee = (light_pick_pdf(s, L) * tri) * vnorm2(vsub(prev_pos, g.pos)) / cosL;
+                    }
+                    float pt = pdf_prev;
+                    mis = (pt * pt) / ((pdf_nee * pdf_nee) + (pt * pt));
+                }
+                direct = vadd(direct, vdivs(vmul(vmul(vscale(g.albedo, mis), g.radiance), T), PDF));
+                if (keep && hp->type == 0) hp->type = 2;
+            }
+            break;
+        }
         if (g.is_sppm && hp->type != 1) {  /* makeHitPoint (SPPM.cpp:79-96, 171-174) */
-            direct = V3(0, 0, 0);  /* the new HitPoint's DirectLight */
+            if (!(nee && keep)) direct = V3(0, 0, 0);  /* the new HitPoint's DirectLight */
             hp->type = 1;
             hp->facet = f;
             hp->pos = g.pos;
@@ -1609,6 +1660,33 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
         if (near1(cur_index)) { Ii = 1.0f; Io = g.index; }
         else { Ii = cur_index; Io = 1.0f; }
         fparam fp = {cur_dir, next_dir, Nf, Ii, Io, g.rough, g.absorb, g.refl};
+        const int delta = s->attrs[s->attr[f]].if_delta != 0;
+        if (nee && light_samples && !delta && hp->type != 1) {  /* SPPM.cpp:190-226 */
+            int li = table_sample(s->lprob, s->lleft, s->lright, s->nlights, &m);
+            const olight* L = &s->lights[li];
+            float pdf_sel = light_pick_pdf(s, L);
+            v3 lpos, rad;
+            float pdf_area;
+            light_point(s, L, &m, &lpos, &rad, &pdf_area);
+            v3 gpos = g.pos;
+            v3 lnorm = light_normal(s, L);
+            v3 ldir = vnormalized(vnormalized(vsub(lpos, gpos)));  /* Ray's ctor normalises again */
+            float dist = vnorm(vsub(lpos, gpos));
+            float maxd = (float)((double)dist - (double)RAY_OFFSET * 2.0);
+            if (occluded_rec(s, gpos, ldir, RAY_OFFSET, maxd, 0.5f, 0) == 0) {
+                float cosL = fminf_(1.0f, fmaxf_(0.00001f, vdot(lnorm, vneg(ldir))));
+                float cosE = fabsf(vdot(Nf, ldir));
+                fp.out = ldir;
+                float brdf = fmaxf_(0.0f, surf_brdf(g.kind, &fp));
+                /* Radiance * cosL * pow(d, -2) / (pdf_area * pdf_sel) * cosE * Albedo * BRDF */
+                v3 dlr = vscale(vmul(vscale(vdivs(vscale(vscale(rad, cosL), powf(dist, -2.0f)), pdf_area * pdf_sel), cosE),
+                                     g.albedo), brdf);
+                float pn = (pdf_area * pdf_sel) * (dist * dist) / cosL;
+                float pt = surf_pdf(g.kind, &fp);
+                float w = (pn * pn) / ((pn * pn) + (pt * pt));
+                direct = vadd(direct, vdivs(vmul(vscale(dlr, w), T), PDF));
+            }
+        }
         float pdf_s = 1.0f;
         next_dir = surf_sample(g.kind, &fp, &m, &pdf_s);
         if (!rutil_valid_dir(next_dir)) break;
@@ -1628,6 +1706,9 @@ static void eye_path(orc_render* r, int pix, int step, v3 smp, ohit* hp) {
         if (pdf_s < 0) break;
         T = vmul(T, vscale(vscale(g.albedo, cs), brdf));
         PDF *= pdf_s;
+        pdf_prev = pdf_s;
+        prev_pos = g.pos;
+        prev_delta = delta;
         cur_org = next_org; cur_dir = next_dir; cur_index = next_index;
         depth++;
     }

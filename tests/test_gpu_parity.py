@@ -102,9 +102,9 @@ def compare_pass(gpu, ref, stored_ref, label, record=True):
     return e
 
 
-def run_parity(scene, cam, alpha, r0, N, passes, width, label, record=True):
-    gpu = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=width, record=record)
-    ref = pyoracle.OracleRender(pyoracle.OracleScene(scene), cam, alpha, r0, N)
+def run_parity(scene, cam, alpha, r0, N, passes, width, label, record=True, nee=sppm_amd.NEE_OFF):
+    gpu = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=width, record=record, nee=nee)
+    ref = pyoracle.OracleRender(pyoracle.OracleScene(scene), cam, alpha, r0, N, nee=nee)
     for p in range(passes):
         say(f"{label}: pass {p}")
         st = gpu.pass_()
@@ -234,3 +234,19 @@ def test_pass_parity_transmit_surfaces(width):
     oracle."""
     sc = scenes.transmit_scene()
     run_parity(sc, scenes.pinhole_camera(48, 48), 0.9, 0.015, 20000, 3, width, f"transmit w{width}")
+
+
+@pytest.mark.parametrize("nee", [sppm_amd.NEE_AS_WRITTEN, sppm_amd.NEE_KEEP])
+def test_pass_parity_eye_nee(nee):
+    """Next-event estimation on eye paths (SPPM.cpp:144-168, 187-229): light
+    samples with shadow rays at the conductor and glass spheres, MIS-weighted
+    emitter hits, paths ending at the light."""
+    run_parity(scenes.nee_scene(), scenes.pinhole_camera(48, 48), 0.9, 0.015, 20000, 3, 8, f"nee={nee}", nee=nee)
+
+
+def test_pass_parity_eye_nee_textured_light(full_scene_small):
+    """The same on the full-material scene: light samples from the textured
+    light's 1000-point table, emitter hits priced by its table density."""
+    r0 = 0.02 * full_scene_small.diagonal()
+    run_parity(full_scene_small, scenes.thinlens_camera(48, 48), 0.7, r0, 15000, 2, 8, "full nee",
+               nee=sppm_amd.NEE_KEEP)

@@ -223,3 +223,58 @@ def test_occlusion_through_alpha_screens():
     assert occ(0.95, 0.9) == 0 and occ(0.95, 1.0) == 1    # 8 clear layers, then the back wall
     assert occ(0.95, 0.9, tr=1.0) == 1                     # 1 - 0 = 1 is not > 1
     assert occ(0.05, 0.9) == 0                             # nothing in front of the back wall
+
+
+# ---- eye-path next-event estimation (SPPM.cpp:144-168, 187-229, compiled out there) ----
+def light_filled_camera(n=16):
+    """Pinhole under the box light looking up at it with a narrow field:
+    every camera ray hits the light first."""
+    cam = scenes.pinhole_camera(n, n)
+    cam.update(pos=(0.5, 0.6, 0.5), target=(0.5, 1.0, 0.51), fov_deg=6.0, dist_to_lens=0.05)
+    return cam
+
+
+def test_nee_modes_on_directly_seen_light():
+    """Off: the light is a hit-point surface like any other.  Otherwise the
+    path ends at the emitter with DirectLight = emitter albedo (1) x radiance
+    x T / PDF / camera pdf = 15 exactly (pinhole pdf 1, depth 0, MIS weight
+    1).  As written the pixel keeps type NONE, so applyHitPointInfo drops it;
+    with KEEP it is ONLY_DIRECTLIGHT and reaches the shared sums."""
+    sc = scenes.nee_scene()
+    os_ = po.OracleScene(sc)
+    cam = light_filled_camera()
+    for nee, want_type in ((0, 1), (1, 0), (2, 2)):
+        r = po.OracleRender(os_, cam, 0.9, 0.015, 1000, nee=nee)
+        r.full_pass()
+        hp, d = r.read_hitpoints(), r.read_direct()
+        assert (hp["type"] == want_type).all(), (nee, np.bincount(hp["type"]))
+        if nee == 0:
+            assert (sc.attr[hp["facet"]] == 3).all()
+        assert (d["hit"] == np.float32(0.0 if nee == 0 else 15.0)).all()
+        assert (d["shared"] == np.float32(15.0 if nee == 2 else 0.0)).all()
+        assert (d["num"] == (1 if nee == 2 else 0)).all()
+
+
+def test_nee_mis_matches_plain_bsdf_sampling():
+    """The MIS combination of light samples and emitter hits (KEEP) has the
+    expectation of plain BSDF sampling with emitter hits at full weight (the
+    oracle-only eye_nee = 3), and a lower variance.  Both renders share every
+    camera ray, so the per-pass difference of their DirectLight totals
+    isolates the light reaching the eye through the conductor and glass
+    spheres (surfaces without hit points).  Dropping cosL from the light
+    sample moves the mean by ~4 standard errors here."""
+    sc = scenes.nee_scene()
+    os_ = po.OracleScene(sc)
+    cam = scenes.pinhole_camera(64, 64)
+    tot = {}
+    for nee in (2, 3):
+        r = po.OracleRender(os_, cam, 0.9, 0.015, 500, nee=nee)
+        t = []
+        for _ in range(300):
+            r.full_pass()
+            t.append(r.read_direct()["hit"].astype(np.float64).sum(0))
+        tot[nee] = np.array(t)
+    diff = tot[2] - tot[3]
+    se = diff.std(0) / np.sqrt(len(diff))
+    assert (np.abs(diff.mean(0)) < 3.0 * se).all(), (diff.mean(0), se)
+    assert (tot[2].var(0) < tot[3].var(0)).all()
