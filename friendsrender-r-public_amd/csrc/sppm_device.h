@@ -2,7 +2,8 @@
 //
 // Float semantics follow the reference exactly (FriendsRender R, SPPM/SPPM/):
 //  * PTUtility::Vec3 op order (Vec.h:179-243): dot = (x*x' + y*y') + z*z',
-//    normalized() divides by (float)sqrt((double)norm2).
+//    normalized() divides by (float)sqrt((double)norm2), which is the
+//    correctly rounded float sqrt (sqrt_narrowed below).
 //  * Unqualified sqrt/cos/sin/exp of a float in the reference bind to the
 //    double functions; they are evaluated in FP64 here and narrowed.
 //  * std::pow(float, 2.0f) is x*x (g++ folds it), std::pow(float, int) is
@@ -41,7 +42,14 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ float norm2(f3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
-__device__ __forceinline__ float norm(f3 a) { return (float)__builtin_sqrt((double)norm2(a)); }
+// The reference's (float)sqrt((double)x) of a float x equals the correctly
+// rounded float sqrt of x: rounding the exact root to double (53 bits) and
+// then to float (24) never differs from rounding it to float once, since
+// 53 >= 2*24 + 2 (Figueroa, "When is double rounding innocuous?", 1995).
+// gfx950's v_sqrt_f32 is refined to the correctly rounded result by the
+// compiler, so the float form is bit-identical and avoids an FP64 sequence.
+__device__ __forceinline__ float sqrt_narrowed(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float norm(f3 a) { return sqrt_narrowed(norm2(a)); }
 __device__ __forceinline__ f3 normalized(f3 a) {
     float d = norm(a);
     return mk(a.x / d, a.y / d, a.z / d);
@@ -163,7 +171,7 @@ __device__ inline f3 cos_sampling(MTCursor& m, uint32_t* ovf, f3 n, float* pdf) 
     frame(n, e2, e3);
     float ph = (float)(real1(m, ovf) * 2.0 * (double)SPPM_PI);
     float th = (float)real1(m, ovf);
-    float th2 = (float)__builtin_sqrt((double)th);
+    float th2 = sqrt_narrowed(th);
     f3 a = n * (float)__builtin_sqrt(1.000000001 - (double)th);
     f3 b = (e2 * (float)cos((double)ph)) * th2;
     f3 c = (e3 * (float)sin((double)ph)) * th2;
