@@ -462,6 +462,53 @@ __device__ inline float surf_brdf(int kind, const FParam& p) {
     return 0.0f;
 }
 
+// Roughconductor::BRDF (material.h:227-274) for the gather, where the value
+// only scales a hit point's flux (SPPM.cpp:440) and never decides a path:
+// the same terms in FP32 with hardware reciprocals, square roots and
+// inverse square roots (about 1 ulp each) instead of the reference's FP64
+// pieces and correctly rounded divisions.  Relative error ~1e-6, far inside
+// the flux tolerance; photon and eye paths keep brdf_rough_conductor.
+__device__ inline float brdf_rough_conductor_flux(const FParam& p) {
+    const float sv = 1e-10f;
+    const float cin = dot(p.in, p.n), cout = dot(p.out, p.n);
+    if (cin > -sv || cout < sv) return 0.0f;
+    const float eta = p.to * __builtin_amdgcn_rcpf(p.from);
+    const float k = p.absorb;
+    const float alpha = p.rough, a2 = alpha * alpha;
+    const f3 hv = (-p.in) + p.out;
+    const f3 h = hv * __builtin_amdgcn_rsqf(norm2(hv));
+    const float ccs = dot(h, p.n);
+    if (ccs < sv || dot(h, p.in) > -sv || dot(h, p.out) < sv) return 0.0f;
+    const float q = 1.0001f - (1.0f - a2) * ccs * ccs;
+    const float D = smax(SPPM_SMIN, a2 * __builtin_amdgcn_rcpf(SPPM_PI * (q * q)));
+    const float lc = smax(0.0f, smin(0.9999f, -cin));
+    const float lv = smax(0.0f, smin(0.9999f, cout));
+    if (lc < SPPM_SMIN || lv < SPPM_SMIN) return 0.0f;
+    const float L_l = 0.5f * (__builtin_amdgcn_sqrtf(1.0f + a2 * (__builtin_amdgcn_rcpf(lc * lc) - 1.0f)) - 1.0f);
+    const float L_v = 0.5f * (__builtin_amdgcn_sqrtf(1.0f + a2 * (__builtin_amdgcn_rcpf(lv * lv) - 1.0f)) - 1.0f);
+    const float G = __builtin_amdgcn_rcpf((1.0f + L_l) * (1.0f + L_v));
+    const float fcos = -cin;
+    const float s2 = 1.0f - fcos * fcos;
+    const float temp = eta * eta - k * k - s2;
+    const float apb = __builtin_amdgcn_sqrtf(temp * temp + k * k * eta * eta);
+    const float a = __builtin_amdgcn_sqrtf((apb + temp) * 0.5f);
+    const float tm1 = apb + fcos * fcos;
+    const float tm2 = a * 2.0f * fcos;
+    const float rs2 = (tm1 - tm2) * __builtin_amdgcn_rcpf(tm1 + tm2);
+    const float tm3 = apb * fcos * fcos + s2 * s2;
+    const float tm4 = tm2 * s2;
+    const float rp2 = rs2 * (tm3 - tm4) * __builtin_amdgcn_rcpf(tm3 + tm4);
+    const float F = 0.5f * (rp2 + rs2);
+    return (F * G * D) * __builtin_amdgcn_rcpf(4.0f * fcos * cout);
+}
+
+// BRDF of a gather term (flux only): the rough conductor in fast FP32, the
+// other surfaces as on the paths.
+__device__ __forceinline__ float gather_brdf(int kind, const FParam& p) {
+    if (kind == SPPM_MAT_ROUGH_CONDUCTOR) return brdf_rough_conductor_flux(p);
+    return surf_brdf(kind, p);
+}
+
 __device__ inline float surf_pdf(int kind, const FParam& p) {
     switch (kind) {
     case SPPM_MAT_DIFFUSE:  // material.h:174-176

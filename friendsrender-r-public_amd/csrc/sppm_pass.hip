@@ -1345,8 +1345,8 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     const float Io = near1(pp.w) ? L.ior : 1.0f;
                     const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
                     FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
-                    const float b = surf_brdf(L.kind, fp);
-                    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) / pf.w;
+                    const float b = gather_brdf(L.kind, fp);
+                    L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) * __builtin_amdgcn_rcpf(pf.w);
                 }
                 if (pdep) {
                     // training signal of the emission proposal: hit points reached x photon power

@@ -98,6 +98,7 @@ def compare_pass(gpu, ref, stored_ref, label, record=True):
     np.testing.assert_array_equal(sg["num_gather"], sr["num_gather"])
     assert rel_rmse(sg["radius"], sr["radius"]) <= 1e-6
     e = rel_rmse(sg["radiance"], sr["radiance"])
+    say(f"  {label}: radiance rel RMSE {e:.2e}")
     assert e <= REL_TOL, f"{label}: radiance rel RMSE {e}"
     return e
 
