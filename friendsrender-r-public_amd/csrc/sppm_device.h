@@ -54,6 +54,22 @@ __device__ __forceinline__ f3 normalized(f3 a) {
     float d = norm(a);
     return mk(a.x / d, a.y / d, a.z / d);
 }
+// The reference compares floats with double constants ((double)x > c).  For
+// a float x that equals one float comparison with c's float neighbour on the
+// right side: x > c  <=>  x > lo(c), the largest float <= c, and x < c  <=>
+// x < hi(c), the smallest float >= c (NaN is false both ways).  No FP64
+// conversion or comparison is then issued.
+constexpr float LO_0P999999 = 0x1.ffffdep-1f;  // lo(0.999999)
+constexpr float LO_0P99 = 0x1.fae146p-1f;      // lo(0.99)
+constexpr float LO_0P9 = 0x1.ccccccp-1f;       // lo(0.9)
+constexpr float LO_1EM4 = 0x1.a36e2ep-14f;     // lo(1e-4)
+constexpr float LO_1EM5 = 0x1.4f8b58p-17f;     // lo(1e-5)
+constexpr float HI_1EM5 = 0x1.4f8b5ap-17f;     // hi(1e-5)
+constexpr float LO_1EM6 = 0x1.0c6f7ap-20f;     // lo(1e-6)
+constexpr float HI_1EM6 = 0x1.0c6f7cp-20f;     // hi(1e-6)
+constexpr float LO_1EM8 = 0x1.5798eep-27f;     // lo(1e-8)
+constexpr float HI_1EM8 = 0x1.5798f0p-27f;     // hi(1e-8)
+
 __device__ __forceinline__ float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 __device__ __forceinline__ f3 ld3(const float4 v) { return mk(v.x, v.y, v.z); }
 // std::max(a,b) = (a<b)?b:a, std::min(a,b) = (b<a)?b:a
@@ -134,7 +150,7 @@ __device__ inline float fresnel_reflectance(f3 n, f3 in, float ii, float io) {  
     float cosA = fabs_(dot(n, in));
     float sinA = sqrtf_(1.0000001f - cosA * cosA);
     float sinB = n1n2 * sinA;
-    if ((double)sinB > 0.999999) return 1.0f;
+    if (sinB > LO_0P999999) return 1.0f;  // (double)sinB > 0.999999
     float cosB = sqrtf_(1.0000001f - sinB * sinB);
     float rp = (io * cosA - ii * cosB) / (io * cosA + ii * cosB);
     float rs = (ii * cosA - io * cosB) / (ii * cosA + io * cosB);
@@ -160,7 +176,7 @@ __device__ __forceinline__ bool rutil_is_reflect(f3 in, f3 out, f3 n) {  // RayU
 __device__ __forceinline__ bool rutil_valid_dir(f3 d) { return norm2(d) > 0.9999f; }
 
 __device__ __forceinline__ void frame(f3 e1, f3& e2, f3& e3) {  // Sampling.cpp:12-18
-    if ((double)fabs_(e1.x) > 0.000001) e2 = normalized(cross(mk(0, 1, 0), e1));
+    if (fabs_(e1.x) > LO_1EM6) e2 = normalized(cross(mk(0, 1, 0), e1));  // (double)|e1.x| > 0.000001
     else e2 = normalized(cross(mk(1, 0, 0), e1));
     e3 = normalized(cross(e1, e2));
 }
@@ -282,7 +298,7 @@ __device__ inline float ggxglass_sampling_pdf(f3 out, f3 n, f3 in, float a, floa
         float c = fabs_(dot(in, out));
         float eta = io / ii;
         float isq = eta * eta - 2.0f * eta * c + 1.0f;
-        if ((double)isq < 1e-8) h = mk(0, 0, 0);
+        if (isq < HI_1EM8) h = mk(0, 0, 0);  // (double)isq < 1e-8
         else {
             float ss = 1.0f / sqrtf_(isq);
             h = normalized(out * (-eta * ss) + in * ss);
@@ -309,7 +325,7 @@ struct FParam {
 };
 
 __device__ __forceinline__ bool opaque_reflect(const FParam& p) {  // material.h:139-141
-    return ((double)dot(p.in, -p.n) > 1e-4 && (double)dot(p.out, p.n) > 1e-4);
+    return (dot(p.in, -p.n) > LO_1EM4 && dot(p.out, p.n) > LO_1EM4);  // (double)... > 1e-4
 }
 
 __device__ inline float brdf_rough_conductor(const FParam& p) {  // material.h:227-274
@@ -526,10 +542,10 @@ __device__ inline float surf_pdf(int kind, const FParam& p) {
         float refl = fresnel_reflectance(NN, p.in, p.from, p.to);
         if (dot(p.n, p.in) * dot(p.n, p.out) < 0.0f) {
             f3 ref = reflect_sampling(NN, p.in);
-            return ((double)fabs_(dot(ref, p.out)) > 0.99) ? SPPM_DELTA_PDF * refl : SPPM_SMIN;
+            return (fabs_(dot(ref, p.out)) > LO_0P99) ? SPPM_DELTA_PDF * refl : SPPM_SMIN;  // (double)|.| > 0.99
         }
         f3 rac = refract_sampling(NN, p.in, p.from, p.to);
-        return ((double)fabs_(dot(rac, p.out)) > 0.99) ? SPPM_DELTA_PDF * (1.0f - refl) : SPPM_SMIN;
+        return (fabs_(dot(rac, p.out)) > LO_0P99) ? SPPM_DELTA_PDF * (1.0f - refl) : SPPM_SMIN;
     }
     case SPPM_MAT_BIMETAL: {  // material.h:450-461
         const f3 N = dot(p.n, p.in) < 0 ? p.n : -p.n;
@@ -577,7 +593,7 @@ __device__ inline f3 surf_sample(int kind, const FParam& p, MTCursor& m, uint32_
         float eta = p.from / p.to;
         float LN = -dot(p.in, NN);
         float sq = (float)(1.0 - (double)(eta * eta) * (1.0 - (double)(LN * LN)));
-        if ((double)sq < 0.000001) return normalized(p.in - p.n * (2.0f * dot(p.n, p.in)));
+        if (sq < HI_1EM6) return normalized(p.in - p.n * (2.0f * dot(p.n, p.in)));  // (double)sq < 0.000001
         return glass_refract(p.from, p.to, p.in, p.n);
     }
     case SPPM_MAT_BIMETAL:  // material.h:428-436

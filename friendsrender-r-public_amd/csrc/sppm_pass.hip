@@ -130,7 +130,7 @@ __device__ __forceinline__ int table_sample(const float* __restrict__ prob, cons
     return id < 0 ? left[bin] : id;
 }
 
-__device__ __forceinline__ bool near1(float s) { return (double)fabs_(s - 1.0f) < 0.00001; }  // SPPM.h:27-29
+__device__ __forceinline__ bool near1(float s) { return fabs_(s - 1.0f) < HI_1EM5; }  // SPPM.h:27-29, (double)|s-1| < 0.00001
 
 // ---------------------------------------------------------------------------
 // createPhotonMap (SPPM.cpp:518-672), one lane per photon path
@@ -356,14 +356,14 @@ template <int W>
 __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
                                               const TraceStack& ts, uint32_t* ovf, uint32_t* fl) {
     if (P.loopcount++ > 32 || P.pidx >= 31) return false;
-    if ((double)P.T.x < 1e-8 || (double)P.T.y < 1e-8 || (double)P.T.z < 1e-8) return false;
+    if (P.T.x < HI_1EM8 || P.T.y < HI_1EM8 || P.T.z < HI_1EM8) return false;  // (double)T < 1e-8
     HitRec h;
     if (!closest_hit<W>(S, P.cur_org, P.cur_dir, SPPM_RAY_OFFSET, ts, h, fl)) return false;
     Geom g;
     calc_geometry(S, h, g);
     const f3 Nf = dot(g.N, P.cur_dir) > 0 ? -g.N : g.N;
     f3 next_org = g.pos;
-    if ((double)norm2(g.radiance) > 0.00001) return false;
+    if (norm2(g.radiance) > LO_1EM5) return false;  // (double)norm2 > 0.00001
     if (g.transl > 0.5f) {
         P.T = P.T * g.transl;
         P.cur_org = g.pos + P.cur_dir * SPPM_RAY_OFFSET;
@@ -395,7 +395,7 @@ __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev&
     P.next_dir = surf_sample(g.kind, fp, P.m, ovf, &pdf_s);
     if (!rutil_valid_dir(P.next_dir)) return false;
     if (g.opaque) {
-        if ((double)dot(P.next_dir, Nf) < 1e-6) return false;
+        if (dot(P.next_dir, Nf) < HI_1EM6) return false;  // (double)dot < 1e-6
     } else {
         P.next_index = rutil_is_reflect(P.cur_dir, P.next_dir, Nf) ? P.cur_index : Io;
     }
@@ -405,7 +405,7 @@ __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev&
     next_org = next_org + outN * SPPM_RAY_OFFSET;
     float brdf = surf_brdf(g.kind, fp);
     float cs = smax(0.0f, smin(1.0f, dot(P.next_dir, outN)));
-    if ((double)brdf < 1e-8) return false;
+    if (brdf < HI_1EM8) return false;  // (double)brdf < 1e-8
     if (pdf_s < 0) return false;
     P.T = P.T * ((g.albedo * cs) * brdf);
     P.PDF *= pdf_s;
@@ -625,7 +625,7 @@ __device__ inline void camera_ray(const CamDev& c, float u, float v, f3 smp, f3&
 // bilinear fetch of GetIBLColorFromDirection (IBL.h:122-129).
 __device__ __forceinline__ f3 ibl_color(const SceneDev& S, f3 dir) {
     float u = 0.0f, v = 0.0f;
-    if (!((double)fabs_(dir.x) < 1e-6 && (double)fabs_(dir.z) < 1e-6)) {
+    if (!(fabs_(dir.x) < HI_1EM6 && fabs_(dir.z) < HI_1EM6)) {  // (double)|.| < 1e-6
         const f3 xz = normalized(mk(dir.x, 0.0f, dir.z));
         const double a = acos((double)xz.x);
         u = (float)((xz.z >= 0.0f ? a : (double)(2.0f * SPPM_PI) - a) / (double)(2.0f * SPPM_PI));
@@ -733,7 +733,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
             cur_org = g.pos + cur_dir * SPPM_RAY_OFFSET;
             continue;
         }
-        if (NEE && (double)norm2(g.radiance) > 1e-6) {
+        if (NEE && norm2(g.radiance) > LO_1EM6) {  // (double)norm2 > 1e-6
             // an emitter ends the path; before the hit point its radiance is
             // DirectLight, MIS-weighted against the light sample of the
             // previous vertex unless that was a delta surface (SPPM.cpp:145-167)
@@ -814,7 +814,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         next_dir = surf_sample(g.kind, fp, m, &ovf, &pdf_s);
         if (!rutil_valid_dir(next_dir)) break;
         if (g.opaque) {
-            if ((double)dot(next_dir, Nf) < 1e-6) break;
+            if (dot(next_dir, Nf) < HI_1EM6) break;  // (double)dot < 1e-6
         } else {
             next_index = !(dot(next_dir, Nf) < 0.0f) ? cur_index : Io;
         }
@@ -824,7 +824,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         next_org = next_org + outN * SPPM_RAY_OFFSET;
         float brdf = surf_brdf(g.kind, fp);
         float cs = smax(0.0f, smin(1.0f, dot(next_dir, outN)));
-        if ((double)brdf < 1e-8) break;
+        if (brdf < HI_1EM8) break;  // (double)brdf < 1e-8
         if (pdf_s < 0) break;
         T = T * ((g.albedo * cs) * brdf);
         PDF *= pdf_s;
@@ -1469,7 +1469,7 @@ __global__ void k_direct_light(HitDev H, SharedDev SH, int np) {
     if ((int)H.pos[i].w == 0) return;
     const float4 d = H.direct[i];
     const f3 dv = mk(d.x, d.y, d.z);
-    if (!((double)norm2(dv) > 1e-8)) return;
+    if (!(norm2(dv) > LO_1EM8)) return;  // (double)norm2 > 1e-8
     float4 s = SH.direct[i];
     SH.direct[i] = make_float4(s.x + dv.x, s.y + dv.y, s.z + dv.z, __int_as_float(__float_as_int(s.w) + 1));
 }

@@ -220,7 +220,7 @@ __device__ inline void calc_geometry(const SceneDev& S, const HitRec& h, Geom& g
         f3 nm = normalized(mk(2.0f * c.x - 1.0f, 2.0f * c.y - 1.0f, 2.0f * c.z - 1.0f));
         nm = normalized(mk(nm.x, nm.z, nm.y));
         f3 E2 = g.N;
-        f3 E1 = (double)fabs_(E2.y) > 0.9 ? cross(E2, mk(1, 0, 0)) : cross(E2, mk(0, 1, 0));
+        f3 E1 = fabs_(E2.y) > LO_0P9 ? cross(E2, mk(1, 0, 0)) : cross(E2, mk(0, 1, 0));  // (double)|y| > 0.9
         f3 E3 = normalized(cross(E1, E2));
         // Matrix3 with columns E1,E2,E3; inverse via cofactors (Matrix.h:180-211)
         float m[9] = {E1.x, E2.x, E3.x, E1.y, E2.y, E3.y, E1.z, E2.z, E3.z};
