@@ -96,3 +96,31 @@ def test_bench_refuses_multi_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert p.returncode == 2
     assert "one GPU" in p.stderr
+
+
+def test_float_thresholds_equal_double_compares():
+    """The device replaces (double)x > c and (double)x < c by float compares
+    with lo(c) (largest float <= c) and hi(c) (smallest float >= c); each
+    constant must be exactly that neighbour of the double constant it names."""
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "friendsrender-r-public_amd", "csrc",
+                            "sppm_device.h")).read()
+    found = re.findall(r"constexpr float (\w+) = (0x[0-9a-fp.+-]+)f;\s*// (lo|hi)\(([0-9.e-]+)\)", src)
+    assert len(found) == 10
+    for name, hexv, side, c in found:
+        v = np.float32(float.fromhex(hexv))
+        assert float(v) == float.fromhex(hexv), name  # representable as written
+        c = float(c)
+        up, down = np.nextafter(v, np.float32(np.inf)), np.nextafter(v, np.float32(-np.inf))
+        if side == "lo":
+            assert float(v) <= c < float(up), name
+        else:
+            assert float(down) < c <= float(v), name
+        # and the comparisons agree on the floats around c
+        x = down
+        for _ in range(5):
+            if side == "lo":
+                assert (float(x) > c) == bool(x > v)
+            else:
+                assert (float(x) < c) == bool(x < v)
+            x = np.nextafter(x, np.float32(np.inf))
