@@ -1338,14 +1338,25 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     }
                 }
                 L.M += nh;
+                const float on = dot(L.outd, L.N);
                 for (int e = 0; e < nh; ++e) {
                     const int j = s_hit[e * 64 + lane];
                     const float4 pp = s_pos[j], pf = s_flux[j], pd = s_dir[j];
                     const f3 indir = mk(pd.x, pd.y, pd.z);
-                    const float Io = near1(pp.w) ? L.ior : 1.0f;
-                    const f3 nrm = dot(indir, L.N) < 0 ? L.N : -L.N;
-                    FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
-                    const float b = gather_brdf(L.kind, fp);
+                    const float din = dot(indir, L.N);
+                    float b;
+                    if (L.kind == SPPM_MAT_DIFFUSE) {
+                        // DiffuseSurface::BRDF (material.h:168-173) with n = din < 0 ? N : -N:
+                        // dot(in, -n) = |din| and dot(out, n) = +-dot(out, N), exactly
+                        // (negating a dot product's operand negates its rounded value)
+                        const float co = din < 0 ? on : -on;
+                        b = (fabs_(din) > LO_1EM4 && co > LO_1EM4) ? 1.0f / SPPM_PI : 0.0f;
+                    } else {
+                        const float Io = near1(pp.w) ? L.ior : 1.0f;
+                        const f3 nrm = din < 0 ? L.N : -L.N;
+                        FParam fp{indir, L.outd, nrm, pp.w, Io, L.rough, L.absorb, L.refl};
+                        b = gather_brdf(L.kind, fp);
+                    }
                     L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) * __builtin_amdgcn_rcpf(pf.w);
                 }
                 if (pdep) {
