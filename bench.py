@@ -426,6 +426,8 @@ def main():
     nbs = (c1["neighbours"] - c0["neighbours"]) / K
     stored = (c1["stored"] - c0["stored"]) / K
     cands = (c1["candidates"] - c0["candidates"]) / K
+    prays = (c1["photon_rays"] - c0["photon_rays"]) / K
+    erays = (c1["eye_rays"] - c0["eye_rays"]) / K
     algo = {
         "photons": stored * STORE_BYTES_PER_PHOTON,
         "build": stored * BUILD_BYTES_PER_PHOTON,
@@ -463,8 +465,11 @@ def main():
                      "traffic_source": "profiles/traffic_gather_full.md (passes 2-4)" if traffic else None},
         "phases_ms": {k: round(v, 3) for k, v in ms.items()},
         "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
-                 "gather_candidates": int(cands),
-                     "hit_point_updates_per_s": round(hits / (ms["gather"] * 1e-3), 1)},
+                     "gather_candidates": int(cands), "photon_rays": int(prays), "eye_rays": int(erays),
+                     "hit_point_updates_per_s": round(hits / (ms["gather"] * 1e-3), 1),
+                     "photon_stage_photons_per_s": round(cfg["photons"] / (ms["photons"] * 1e-3), 1),
+                     "photon_stage_rays_per_s": round(prays / (ms["photons"] * 1e-3), 1),
+                     "eye_stage_rays_per_s": round(erays / (ms["eye"] * 1e-3), 1)},
         "cpu_baseline": None,
     }
     r.close()

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -768,9 +769,16 @@ static TraceSpill trace_spill(const sppm_ctx* c) {
     if (c->tuned_blocks > 0 && c->tuned_blocks < sp.blocks) sp.blocks = c->tuned_blocks;
     return sp;
 }
+// device addresses of the ray counters inside the Counters record
+static unsigned long long* photon_rays(RenderBuffers& R) {
+    return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, photon_rays));
+}
+static unsigned long long* eye_rays(RenderBuffers& R) {
+    return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, eye_rays));
+}
 static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
     launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, trace_spill(c), c->scene.next_path,
-                  c->guided ? &c->emit : nullptr, c->rb.flags, st);
+                  c->guided ? &c->emit : nullptr, c->rb.flags, photon_rays(c->rb), st);
 }
 // Training records of the pass (deposit histogram), Adam steps on it, and
 // the proposal for the next pass.
@@ -818,7 +826,7 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
     }
     if (t) t->start(st);
     launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill(c),
-               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, c->params.eye_nee, st);
+               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->params.eye_nee, st);
     if (t) t->stop(st);
     if (uni) {
         HIPOK(hipEventRecord(c->ev_eye[b], st));
@@ -977,10 +985,11 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     launch_camera_words(R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words, st2);
     HIPOK(hipEventRecord(c->ev_gjoin, st2));
     launch_photon(c->scene.dev, R.slots, R.N, 0, R.pass_dev, trace_spill(c), c->scene.next_path,
-                  c->guided ? &c->emit : nullptr, R.flags, st);
+                  c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
     do_build(c, st);
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
-    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, c->params.eye_nee, st);
+    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, eye_rays(R),
+               c->params.eye_nee, st);
     int rc = do_gather(c, st);
     if (rc) return rc;
     if (c->guided) do_guide(c, st);
@@ -1201,6 +1210,17 @@ int sppm_counters(sppm_ctx* c, int64_t* hit_points, int64_t* neighbours, int64_t
     if (photons_stored) *photons_stored = (int64_t)h.stored;
     if (candidates) *candidates = (int64_t)h.candidates;
     if (passes) *passes = c->pass;
+    return SPPM_OK;
+}
+
+int sppm_ray_counts(sppm_ctx* c, int64_t* photon_rays, int64_t* eye_rays) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    Counters h{};
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if (photon_rays) *photon_rays = (int64_t)h.photon_rays;
+    if (eye_rays) *eye_rays = (int64_t)h.eye_rays;
     return SPPM_OK;
 }
 

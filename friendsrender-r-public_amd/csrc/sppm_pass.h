@@ -29,6 +29,8 @@ struct Counters {
     unsigned long long neighbours;
     unsigned long long stored;
     unsigned long long candidates;  // photons the balanced gather tested (union boxes of its tiles)
+    unsigned long long photon_rays; // closest-hit queries of the photon pass
+    unsigned long long eye_rays;    // closest-hit queries of the eye pass (shadow-ray layers included)
 };
 
 // Camera words of one pass: K segments of L words (K * L = 6 * pixels);
@@ -44,8 +46,10 @@ struct TraceSpill {
 int trace_grid_blocks(int width);
 struct GuideEmit;
 // step: the pass step (SPPM.cpp:798), or, when step_dev is given, *step_dev + 1
+// rays: a 64-bit counter the launch adds its closest-hit queries to
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
-                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags, hipStream_t st);
+                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags,
+                   unsigned long long* rays, hipStream_t st);
 void launch_add_i32(int* p, int v, hipStream_t st);
 void launch_zero_i32(int* p, int n, hipStream_t st);
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
@@ -53,7 +57,8 @@ void launch_guide_deposit(const int* count, const int* off, const float* pdep, c
                           hipStream_t st);
 // nee: SPPM_NEE_* (the compiled-out next-event estimation of getHitpointInfo)
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, int nee, hipStream_t st);
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int nee,
+                hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
                   int* facet, float* tuv, uint32_t* flags, hipStream_t st);
 // Scene::IsOccluded for a batch of rays: occluded[i] = 0 / 1.

@@ -143,6 +143,13 @@ struct PhotonPath {
     int i, pidx, loopcount, depth;
 };
 
+// Adds a per-lane count to a 64-bit device counter, one atomic per wave.
+__device__ __forceinline__ void add_wave_count(unsigned long long* __restrict__ ctr, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(ctr, x);
+}
+
 // LightSampler::SamplePointOnAreaLight (LightSampler.cpp:68-82): a point of
 // light L, its radiance and its area density.
 __device__ __forceinline__ void light_point(const SceneDev& S, const DevLight& L, MTCursor& m, uint32_t* ovf,
@@ -354,10 +361,11 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
 // One iteration of the createPhotonMap loop body; false when the path ends.
 template <int W>
 __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
-                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl) {
+                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl, uint32_t& nray) {
     if (P.loopcount++ > 32 || P.pidx >= 31) return false;
     if (P.T.x < HI_1EM8 || P.T.y < HI_1EM8 || P.T.z < HI_1EM8) return false;  // (double)T < 1e-8
     HitRec h;
+    ++nray;
     if (!closest_hit<W>(S, P.cur_org, P.cur_dir, SPPM_RAY_OFFSET, ts, h, fl)) return false;
     Geom g;
     calc_geometry(S, h, g);
@@ -436,14 +444,15 @@ template <int W, bool GUIDED>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
                                                        const int* __restrict__ step_dev, int cap,
                                                        uint32_t* __restrict__ spill, int* __restrict__ next_path,
-                                                       GuideEmit G, uint32_t* __restrict__ flags) {
+                                                       GuideEmit G, uint32_t* __restrict__ flags,
+                                                       unsigned long long* __restrict__ rays) {
     // the pass step comes from the device pass counter when the pass is replayed from a graph
     const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t ovf = 0, fl = 0;
+    uint32_t ovf = 0, fl = 0, nray = 0;
     PhotonPath P;
     bool alive = false, exhausted = false;
     for (;;) {
@@ -468,13 +477,14 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
             if (exhausted) break;
             continue;
         }
-        if (alive && !photon_bounce<W>(S, slots, P, ts, &ovf, &fl)) {
+        if (alive && !photon_bounce<W>(S, slots, P, ts, &ovf, &fl, nray)) {
             photon_finish(slots, P);
             alive = false;
         }
     }
     if (ovf) fl |= 2u;
     if (fl) atomicOr(flags, fl);
+    add_wave_count(rays, nray);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,7 +686,7 @@ __device__ __forceinline__ bool occluded_ray(const SceneDev& S, f3 o, f3 d, floa
 template <int W, bool NEE>
 __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int i,
                                          const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl,
-                                         bool keep) {
+                                         bool keep, uint32_t& nray) {
     uint32_t ovf = 0;
     MTCursor m;
     mt_seed(m, (uint32_t)i + (uint32_t)C.W * (uint32_t)C.H * (uint32_t)step);
@@ -718,6 +728,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
             PDF *= rus;
         }
         HitRec h;
+        ++nray;
         if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, ts, h, &fl)) {
             // escape: the environment's colour (SPPM.cpp:290-300)
             if (ibl) direct = direct + (ibl_color(S, cur_dir) * T) / PDF;
@@ -795,6 +806,7 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
                 const f3 ldir = normalized(normalized(dv));  // normalised, then again by Ray's ctor
                 const float dist = norm(dv);
                 const float maxd = (float)((double)dist - (double)SPPM_RAY_OFFSET * 2.0);
+                ++nray;
                 if (!occluded_ray<W>(S, g.pos, ldir, SPPM_RAY_OFFSET, maxd, 0.5f, ts, fl)) {
                     const float cosL = smin(1.0f, smax(0.00001f, dot(ln, -ldir)));
                     const float cosE = fabs_(dot(Nf, ldir));
@@ -858,16 +870,18 @@ template <int W, bool NEE>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
                                                     const int* __restrict__ step_dev, int cap,
                                                     uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
-                                                    uint32_t* __restrict__ flags, int keep) {
+                                                    uint32_t* __restrict__ flags, int keep,
+                                                    unsigned long long* __restrict__ rays) {
     const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int NPIX = C.W * C.H;
-    uint32_t fl = 0;
+    uint32_t fl = 0, nray = 0;
     for (int i = blockIdx.x * PATH_BLOCK + lane; i < NPIX; i += gridDim.x * PATH_BLOCK)
-        eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0);
+        eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray);
     if (fl) atomicOr(flags, fl);
+    add_wave_count(rays, nray);
 }
 
 // ---------------------------------------------------------------------------
@@ -1580,13 +1594,14 @@ void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long
 }
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
-                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags, hipStream_t st) {
+                   const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags,
+                   unsigned long long* rays, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, next_path, 1);
     dim3 g(sp.blocks), b(PATH_BLOCK);
     const GuideEmit G = guide ? *guide : GuideEmit{};
 #define SPPM_PHOTON(WW, GG) \
     hipLaunchKernelGGL((k_photon<WW, GG>), g, b, 0, st, S, slots, N, step, step_dev, sp.cap, sp.buf, next_path, G, \
-                       flags)
+                       flags, rays)
     if (guide) {
         if (S.width == 2) SPPM_PHOTON(2, true);
         else if (S.width == 4) SPPM_PHOTON(4, true);
@@ -1601,19 +1616,24 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
 
 template <bool NEE>
 static void launch_eye_t(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                         const TraceSpill& sp, const uint32_t* words, uint32_t* flags, int keep, hipStream_t st) {
+                         const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays,
+                         int keep, hipStream_t st) {
     dim3 g(sp.blocks), b(PATH_BLOCK);
     if (S.width == 2)
-        hipLaunchKernelGGL((k_eye<2, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+        hipLaunchKernelGGL((k_eye<2, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
+                           rays);
     else if (S.width == 4)
-        hipLaunchKernelGGL((k_eye<4, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+        hipLaunchKernelGGL((k_eye<4, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
+                           rays);
     else
-        hipLaunchKernelGGL((k_eye<8, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep);
+        hipLaunchKernelGGL((k_eye<8, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
+                           rays);
 }
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, int nee, hipStream_t st) {
-    if (nee == SPPM_NEE_OFF) launch_eye_t<false>(S, C, H, step, step_dev, sp, words, flags, 0, st);
-    else launch_eye_t<true>(S, C, H, step, step_dev, sp, words, flags, nee == SPPM_NEE_KEEP ? 1 : 0, st);
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int nee,
+                hipStream_t st) {
+    if (nee == SPPM_NEE_OFF) launch_eye_t<false>(S, C, H, step, step_dev, sp, words, flags, rays, 0, st);
+    else launch_eye_t<true>(S, C, H, step, step_dev, sp, words, flags, rays, nee == SPPM_NEE_KEEP ? 1 : 0, st);
 }
 
 __global__ void k_add_i32(int* __restrict__ p, int v) {

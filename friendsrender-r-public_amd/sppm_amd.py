@@ -32,7 +32,7 @@ EXPORTED = [
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
-    "sppm_read_direct", "sppm_trace_occluded",
+    "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts",
 ]
 
 
@@ -390,8 +390,10 @@ class Renderer:
     def counters(self):
         v = [C.c_int64() for _ in range(5)]
         _check(self.lib.sppm_counters(self.h, *[C.byref(x) for x in v]))
+        rays = [C.c_int64() for _ in range(2)]
+        _check(self.lib.sppm_ray_counts(self.h, C.byref(rays[0]), C.byref(rays[1])))
         return dict(hit_points=v[0].value, neighbours=v[1].value, stored=v[2].value, candidates=v[3].value,
-                    passes=v[4].value)
+                    passes=v[4].value, photon_rays=rays[0].value, eye_rays=rays[1].value)
 
     def read_photons(self):
         n = C.c_int32()

@@ -258,6 +258,11 @@ int sppm_trace_occluded(sppm_ctx* ctx, int32_t n, const float* org, const float*
  * (flattened list lengths), candidate photons the gather tested (the
  * photons of the grid cells its query boxes cover; 0 in record mode),
  * passes.  Any pointer may be NULL. */
+/* Closest-hit queries traced since sppm_begin: by the photon passes
+ * (Scene::GetOneFacetFromRay calls of createPhotonMap, SPPM.cpp:555) and by
+ * the eye passes (getHitpointInfo, SPPM.cpp:125, plus shadow-ray layers when
+ * eye_nee is on). */
+int sppm_ray_counts(sppm_ctx* ctx, int64_t* photon_rays, int64_t* eye_rays);
 int sppm_counters(sppm_ctx* ctx, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
                   int64_t* candidates, int64_t* passes);
 

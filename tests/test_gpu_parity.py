@@ -251,3 +251,18 @@ def test_pass_parity_eye_nee_textured_light(full_scene_small):
     r0 = 0.02 * full_scene_small.diagonal()
     run_parity(full_scene_small, scenes.thinlens_camera(48, 48), 0.7, r0, 15000, 2, 8, "full nee",
                nee=sppm_amd.NEE_KEEP)
+
+
+def test_ray_counts_are_consistent(box_scene):
+    """Every photon path traces at least its emission ray and at most the 33
+    loop iterations of createPhotonMap (SPPM.cpp:549); every pixel traces at
+    least its camera ray and at most 7 (eye depth <= 6, SPPM.cpp:99-100)."""
+    N, W, H = 20000, 32, 32
+    r = sppm_amd.Renderer(box_scene, scenes.pinhole_camera(W, H), 0.9, 0.015, N, bvh_width=4)
+    for _ in range(3):
+        r.pass_(stats=False)
+    c = r.counters()
+    r.close()
+    say(f"rays: photon {c['photon_rays']} eye {c['eye_rays']}")
+    assert 3 * N <= c["photon_rays"] <= 3 * N * 33
+    assert 3 * W * H <= c["eye_rays"] <= 3 * W * H * 7
