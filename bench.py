@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--rmse", action="store_true",
                     help="time-to-RMSE mode on the occluded-light workload: plain vs guided emission against a "
                          "long plain reference render")
+    ap.add_argument("--rmse-scene", choices=["occluded", "caustic"], default="occluded",
+                    help="time-to-RMSE workload: the occluded-light scene (1024^2, 1M photons, 64x4 bf16 network) "
+                         "or camera 0 of the caustic workload (2048^2, 4M photons, 128x6 fp16 network)")
     ap.add_argument("--ref-passes", type=int, default=10000)
     ap.add_argument("--max-passes", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=16)
@@ -150,11 +153,21 @@ def rmse_mode(args):
     import numpy as np
     import scenes
     import sppm_amd
-    scene = scenes.occluded_scene()
-    res = args.resolution
-    cam = scenes.occluded_camera(res, res)
-    N, alpha, r0 = 1_000_000, args.alpha, 0.02 * scene.diagonal()
-    log(f"occluded-light scene: {scene.nfacets} triangles, {res}x{res}, {N} photons/pass")
+    if args.rmse_scene == "caustic":
+        scene = scenes.caustic_scene()
+        res = args.caustic_res
+        cam = scenes.caustic_cameras(1, res, res)[0]
+        N, alpha, r0 = args.caustic_photons, args.alpha, 0.01 * scene.diagonal()
+        net = dict(hidden=128, layers=6, fp16=True)
+        workload_name, dtype = "caustic, camera 0", "f32 (MLP fp16 MFMA, f32 accumulation)"
+    else:
+        scene = scenes.occluded_scene()
+        res = args.resolution
+        cam = scenes.occluded_camera(res, res)
+        N, alpha, r0 = 1_000_000, args.alpha, 0.02 * scene.diagonal()
+        net = {}
+        workload_name, dtype = "occluded light", "f32 (MLP bf16 MFMA, f32 accumulation)"
+    log(f"{workload_name}: {scene.nfacets} triangles, {res}x{res}, {N} photons/pass")
 
     def linear(r, passes):
         img = r.reconstruct(N * passes, passes).reshape(res, res, 4)[..., :3].astype(np.float64)
@@ -183,7 +196,7 @@ def rmse_mode(args):
     for mode in ("plain", "guided"):
         r = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=8)
         if mode == "guided":
-            r.guide_set(True, mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=1, history_decay=0.95)
+            r.guide_set(True, mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=1, history_decay=0.95, **net)
         el, passes, e = 0.0, 0, float("inf")
         curve = []
         while passes < args.max_passes:
@@ -211,11 +224,12 @@ def rmse_mode(args):
                   f"{args.ref_passes}-pass plain render, guided vs plain",
         "value": out["guided"]["seconds"], "unit": "s", "higher_is_better": False,
         "speedup_guided_vs_plain": round(out["plain"]["seconds"] / out["guided"]["seconds"], 3) if both else None,
-        "n_gpus": 1, "dtype": "f32 (MLP bf16 MFMA, f32 accumulation)", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": "occluded light", "resolution": f"{res}x{res}", "photons_per_pass": N,
+        "n_gpus": 1, "dtype": dtype, "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": workload_name, "resolution": f"{res}x{res}", "photons_per_pass": N,
                    "triangles": scene.nfacets, "alpha": alpha, "r0": round(r0, 6), "bvh_width": 8,
                    "reference_passes": args.ref_passes, "reference_mean_linear_radiance": scale, "mix_plain": 0.5,
-                   "train_steps_per_pass": 1, "learning_rate": 1e-2, "history_decay": 0.95},
+                   "train_steps_per_pass": 1, "learning_rate": 1e-2, "history_decay": 0.95,
+                   "network": f"{net.get('hidden', 64)}x{net.get('layers', 4)} {'fp16' if net.get('fp16') else 'bf16'}"},
         "plain": out["plain"], "guided": out["guided"],
     }
     print(json.dumps(line), flush=True)
