@@ -50,6 +50,10 @@ def parse():
                     help="time-to-RMSE workload: the occluded-light scene (1024^2, 1M photons, 64x4 bf16 network) "
                          "or camera 0 of the caustic workload (2048^2, 4M photons, 128x6 fp16 network)")
     ap.add_argument("--ref-passes", type=int, default=10000)
+    ap.add_argument("--ref-save", default=None, metavar="NPY",
+                    help="time-to-RMSE mode: write the reference's linear image and stop")
+    ap.add_argument("--ref-load", default=None, metavar="NPY",
+                    help="time-to-RMSE mode: read the reference's linear image instead of rendering it")
     ap.add_argument("--max-passes", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=16)
     ap.add_argument("--target-rmse", type=float, default=0.01)
@@ -175,23 +179,35 @@ def rmse_mode(args):
 
     # reference: independent random streams (pass steps shifted by 10^6,
     # another camera sample stream) so the timed renders do not replay it
-    t0 = time.perf_counter()
-    ref_cam = dict(cam, sampler_id=7919)
-    ref_r = sppm_amd.Renderer(scene, ref_cam, alpha, r0, N, bvh_width=8)
-    for p in range(args.ref_passes):
-        step = 1_000_001 + p
-        ref_r.photons(step)
-        ref_r.build()
-        ref_r.eye(step)
-        ref_r.gather()
-        if (p + 1) % 500 == 0:
-            ref_r.sync()
-            log(f"reference: {p + 1}/{args.ref_passes} passes, {time.perf_counter() - t0:.1f} s")
-    ref_r.sync()
-    ref = linear(ref_r, args.ref_passes)
-    ref_r.close()
-    scale = float(ref.mean())  # RMSE is measured on the image normalised to the reference's mean
-    log(f"reference mean linear radiance {scale:.5g}")
+    if args.ref_load:
+        ref = np.load(args.ref_load).astype(np.float64)
+        log(f"reference read from {args.ref_load}")
+    else:
+        t0 = time.perf_counter()
+        ref_cam = dict(cam, sampler_id=7919)
+        ref_r = sppm_amd.Renderer(scene, ref_cam, alpha, r0, N, bvh_width=8)
+        for p in range(args.ref_passes):
+            step = 1_000_001 + p
+            ref_r.photons(step)
+            ref_r.build()
+            ref_r.eye(step)
+            ref_r.gather()
+            if (p + 1) % 500 == 0:
+                ref_r.sync()
+                log(f"reference: {p + 1}/{args.ref_passes} passes, {time.perf_counter() - t0:.1f} s")
+        ref_r.sync()
+        ref = linear(ref_r, args.ref_passes)
+        ref_r.close()
+        if args.ref_save:
+            np.save(args.ref_save, ref.astype(np.float32))
+            log(f"reference written to {args.ref_save}")
+            return
+    # Pixels whose value is not finite (a NaN pixel weight of the reference's
+    # own arithmetic, propagated as it does) are left out of the RMSE, in the
+    # reference and in the renders alike; their number is reported.
+    ref_ok = np.isfinite(ref).all(axis=-1)
+    scale = float(ref[ref_ok].mean())  # RMSE is measured on the image normalised to the reference's mean
+    log(f"reference mean linear radiance {scale:.5g}, {int((~ref_ok).sum())} non-finite pixels")
     out = {}
     for mode in ("plain", "guided"):
         r = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=8)
@@ -206,13 +222,16 @@ def rmse_mode(args):
             r.sync()
             el += time.perf_counter() - ts
             passes += args.check_every
-            e = float(np.sqrt(np.mean((linear(r, passes) - ref) ** 2))) / scale
+            img = linear(r, passes)
+            ok = ref_ok & np.isfinite(img).all(axis=-1)
+            e = float(np.sqrt(np.mean((img[ok] - ref[ok]) ** 2))) / scale
             curve.append((passes, round(el, 4), e))
             if passes % (8 * args.check_every) == 0:
                 log(f"{mode}: {passes} passes, {el:.2f} s, RMSE {e:.4g}")
             if e <= args.target_rmse:
                 break
         out[mode] = dict(seconds=round(el, 3), passes=passes, rmse=e, reached=e <= args.target_rmse,
+                         nonfinite_pixels=int((~ok).sum()),
                          ms_per_pass=round(el / passes * 1e3, 3),
                          guide_ms=round(r.kernel_ms("guide"), 3) if mode == "guided" else 0.0,
                          curve=curve[:: max(1, len(curve) // 20)])
