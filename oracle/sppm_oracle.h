@@ -23,6 +23,13 @@
  *                                         reflection/refraction laws
  *   camera .............................. closed form (central ray, pinhole geometry)
  *   neighbour sets / update / reconstruct  brute force + hand-built recurrences
+ *   BiMetal / DiffuseTransmit / GlossTransmit  closed forms, density normalisation,
+ *                                         mirror symmetry, agreement with Metal/Roughconductor
+ *   environment light (SPHERE map, 0.4c^2)  closed-form directions, conversion, DL sums
+ *   IsOccluded .......................... rays whose answer follows from a scene's layout
+ *   eye-path NEE / MIS .................. closed form (camera filled by the light) and
+ *                                         equality of means with plain BSDF sampling
+ *                                         (eye_nee = 3, an oracle-only checker mode)
  *   full-path draw order ................ parity unpinned against a reference run
  *                                         (the reference does not compile as shipped)
  */
