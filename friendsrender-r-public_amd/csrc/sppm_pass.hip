@@ -626,10 +626,10 @@ __device__ inline void camera_ray(const CamDev& c, float u, float v, f3 smp, f3&
 
 // ---------------------------------------------------------------------------
 // createHitpoints + getHitpointInfo (SPPM.cpp:98-353), one lane per pixel.
-// No environment light: after the hit point is made the rest of the eye path
-// could only add IBL radiance (zero), so the path stops there.
+// After the hit point is made the rest of the eye path can only add
+// environment light (SPPM.cpp:290-300), so without an environment map (and
+// without next-event estimation) the path stops there.
 // ---------------------------------------------------------------------------
-// createHitpoints body for pixel i (SPPM.cpp:330-349 with getHitpointInfo).
 // IBL::GetUVFromDirection, SPHERE mapping (IBL.h:90-103; unqualified acos on a
 // float is the double function, u and v narrowed to float), then the
 // bilinear fetch of GetIBLColorFromDirection (IBL.h:122-129).
