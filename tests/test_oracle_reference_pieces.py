@@ -175,8 +175,8 @@ def test_matrix3_inverse_matches_reference_and_inverts():
 
 @pytest.mark.skipif(not po.ref_available(), reason="reference checker not built in this tree")
 def test_live_reference_random_sequences():
-    # Wider sweep against the live reference build (also runs on the GPU box,
-    # where the prebuilt checker travels with the tree).
+    # Wider sweep against the live reference build (also runs on machines without
+    # the reference checkout, where the prebuilt checker travels with the tree).
     R = po.ref()
     rng = np.random.default_rng(99)
     for seed in rng.integers(-2**31, 2**31 - 1, size=20):
