@@ -83,7 +83,7 @@ struct SceneBuffers {
     int stack_cap = 0;
     uint32_t* spill = nullptr;  // traversal stack entries beyond TRACE_LDS
     int spill_blocks = 0;
-    int* next_path = nullptr;   // path counter of the persistent photon pass
+    int* next_path = nullptr;   // [0] path counter of the photon pass, [1] pixel counter of the eye pass
     std::vector<DevLight> hlights;  // host copy of the area lights
     std::vector<float4> h_tuvw, h_tL;  // host copy of the textured lights' 1000-point tables
     float h_lpick_avg = 1.0f;           // TableSampler average of the light pick table
@@ -547,7 +547,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     {
         const size_t per_lane = S.stack_cap > TRACE_LDS ? (size_t)(S.stack_cap - TRACE_LDS) : 1;
         int rc;
-        if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 1))) {
+        if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 2))) {
             S.release();
             return rc;
         }
@@ -826,7 +826,8 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
     }
     if (t) t->start(st);
     launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill(c),
-               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->params.eye_nee, st);
+               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->scene.next_path + 1,
+               c->params.eye_nee, st);
     if (t) t->stop(st);
     if (uni) {
         HIPOK(hipEventRecord(c->ev_eye[b], st));
@@ -989,7 +990,7 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     do_build(c, st);
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
     launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, eye_rays(R),
-               c->params.eye_nee, st);
+               c->scene.next_path + 1, c->params.eye_nee, st);
     int rc = do_gather(c, st);
     if (rc) return rc;
     if (c->guided) do_guide(c, st);

@@ -57,8 +57,8 @@ void launch_guide_deposit(const int* count, const int* off, const float* pdep, c
                           hipStream_t st);
 // nee: SPPM_NEE_* (the compiled-out next-event estimation of getHitpointInfo)
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int nee,
-                hipStream_t st);
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,
+                int nee, hipStream_t st);
 void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
                   int* facet, float* tuv, uint32_t* flags, hipStream_t st);
 // Scene::IsOccluded for a batch of rays: occluded[i] = 0 / 1.

@@ -865,21 +865,30 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
     if (ovf) fl |= 2u;
 }
 
-// Eye pass: a fixed grid of waves strides over the pixels.
+// Eye pass: persistent waves take 64 consecutive pixels at a time from a
+// counter (zeroed before the launch), so waves whose pixels have long paths
+// (glass, mirrors) do not hold back the others; a pixel's result does not
+// depend on the wave that traces it.
 template <int W, bool NEE>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
                                                     const int* __restrict__ step_dev, int cap,
                                                     uint32_t* __restrict__ spill, const uint32_t* __restrict__ words,
                                                     uint32_t* __restrict__ flags, int keep,
-                                                    unsigned long long* __restrict__ rays) {
+                                                    unsigned long long* __restrict__ rays, int* __restrict__ queue) {
     const int step = step_dev ? *step_dev + 1 : step_arg;
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int NPIX = C.W * C.H;
     uint32_t fl = 0, nray = 0;
-    for (int i = blockIdx.x * PATH_BLOCK + lane; i < NPIX; i += gridDim.x * PATH_BLOCK)
-        eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray);
+    for (;;) {
+        int base = 0;
+        if ((lane & 63) == 0) base = atomicAdd(queue, 64);
+        base = __shfl(base, 0, 64);
+        if (base >= NPIX) break;
+        const int i = base + (lane & 63);
+        if (i < NPIX) eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray);
+    }
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
 }
@@ -1617,23 +1626,25 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
 template <bool NEE>
 static void launch_eye_t(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                          const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays,
-                         int keep, hipStream_t st) {
+                         int* queue, int keep, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, queue, 1);
     dim3 g(sp.blocks), b(PATH_BLOCK);
     if (S.width == 2)
         hipLaunchKernelGGL((k_eye<2, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
-                           rays);
+                           rays, queue);
     else if (S.width == 4)
         hipLaunchKernelGGL((k_eye<4, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
-                           rays);
+                           rays, queue);
     else
         hipLaunchKernelGGL((k_eye<8, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
-                           rays);
+                           rays, queue);
 }
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
-                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int nee,
-                hipStream_t st) {
-    if (nee == SPPM_NEE_OFF) launch_eye_t<false>(S, C, H, step, step_dev, sp, words, flags, rays, 0, st);
-    else launch_eye_t<true>(S, C, H, step, step_dev, sp, words, flags, rays, nee == SPPM_NEE_KEEP ? 1 : 0, st);
+                const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,
+                int nee, hipStream_t st) {
+    if (nee == SPPM_NEE_OFF) launch_eye_t<false>(S, C, H, step, step_dev, sp, words, flags, rays, queue, 0, st);
+    else
+        launch_eye_t<true>(S, C, H, step, step_dev, sp, words, flags, rays, queue, nee == SPPM_NEE_KEEP ? 1 : 0, st);
 }
 
 __global__ void k_add_i32(int* __restrict__ p, int v) {
