@@ -82,15 +82,17 @@ struct SceneBuffers {
     float bmin[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
     int stack_cap = 0;
     uint32_t* spill = nullptr;  // traversal stack entries beyond TRACE_LDS
+    uint32_t* spill_eye = nullptr;  // the same for the eye pass, which runs beside the photon pass
     int spill_blocks = 0;
     int* next_path = nullptr;   // [0] path counter of the photon pass, [1] pixel counter of the eye pass
     std::vector<DevLight> hlights;  // host copy of the area lights
     std::vector<float4> h_tuvw, h_tL;  // host copy of the textured lights' 1000-point tables
     float h_lpick_avg = 1.0f;           // TableSampler average of the light pick table
     TraceSpill trace_spill() const { return TraceSpill{spill, spill_blocks, stack_cap}; }
+    TraceSpill trace_spill_eye() const { return TraceSpill{spill_eye, spill_blocks, stack_cap}; }
     void release() {
         free_bvh(&bvh);
-        dfree(spill); dfree(next_path);
+        dfree(spill); dfree(spill_eye); dfree(next_path);
         dfree(facet_nrm); dfree(facet_uv); dfree(facet_attr); dfree(attrs); dfree(tex); dfree(texels);
         dfree(lpick_prob); dfree(lpick_left); dfree(lpick_right); dfree(lights);
         dfree(ltab_uvw); dfree(ltab_L); dfree(ltab_prob); dfree(ltab_left); dfree(ltab_right); dfree(facet_light);
@@ -547,7 +549,8 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     {
         const size_t per_lane = S.stack_cap > TRACE_LDS ? (size_t)(S.stack_cap - TRACE_LDS) : 1;
         int rc;
-        if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 2))) {
+        if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) ||
+            (rc = dalloc(&S.spill_eye, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 2))) {
             S.release();
             return rc;
         }
@@ -769,6 +772,12 @@ static TraceSpill trace_spill(const sppm_ctx* c) {
     if (c->tuned_blocks > 0 && c->tuned_blocks < sp.blocks) sp.blocks = c->tuned_blocks;
     return sp;
 }
+// the eye pass's own spill area: in a full pass it runs beside the photon pass
+static TraceSpill trace_spill_eye(const sppm_ctx* c) {
+    TraceSpill sp = c->scene.trace_spill_eye();
+    if (c->tuned_blocks > 0 && c->tuned_blocks < sp.blocks) sp.blocks = c->tuned_blocks;
+    return sp;
+}
 // device addresses of the ray counters inside the Counters record
 static unsigned long long* photon_rays(RenderBuffers& R) {
     return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, photon_rays));
@@ -825,7 +834,7 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
         HIPOK(hipStreamWaitEvent(st, c->ev_words[b], 0));
     }
     if (t) t->start(st);
-    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill(c),
+    launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill_eye(c),
                c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->scene.next_path + 1,
                c->params.eye_nee, st);
     if (t) t->stop(st);
@@ -946,9 +955,17 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         c->pass += 1;
         return stats ? read_pass_stats(c, stats) : SPPM_OK;
     }
-    // camera words of this pass (first pass only) and of the next one, on the
-    // side stream: they overlap this pass's kernels
-    rc = words_generate_upto(c, c->words_used + 1);
+    // The eye pass runs on the side stream beside the photon pass and the
+    // grid build (createHitpoints runs beside the kd-tree build in the
+    // reference too, SPPM.cpp:809-838): it waits for what the main stream has
+    // queued so far (the previous gather reads the hit points it rewrites),
+    // then this pass's camera words; the next pass's words follow it there.
+    HIPOK(hipEventRecord(c->ev_gfork, st));
+    HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+    rc = do_eye(c, step, c->stream2, &c->t_eye);
+    if (rc) return rc;
+    HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
+    rc = words_generate_upto(c, c->words_used);
     if (rc) return rc;
     c->t_photons.start(st);
     do_photons(c, step, st);
@@ -956,8 +973,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     c->t_build.start(st);
     do_build(c, st);
     c->t_build.stop(st);
-    rc = do_eye(c, step, st, &c->t_eye);
-    if (rc) return rc;
+    HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
     c->t_gather.start(st);
     rc = do_gather(c, st);
     if (rc) return rc;
@@ -974,9 +990,9 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
 }
 
 // ---- graph-captured passes -------------------------------------------------
-// One pass as a capturable sequence: the pass's camera words on st2 (forked
-// from st) while the photon pass and the build run on st, then eye, gather,
-// guided training and the device pass counter.  Every size and step is read
+// One pass as a capturable sequence: the pass's camera words and the eye
+// pass on st2 (forked from st) while the photon pass and the build run on st,
+// then, joined, the gather, guided training and the device pass counter.  Every size and step is read
 // on the device, so a replay continues where the previous one ended; memory
 // is cleared by kernels, not by memset nodes.
 static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
@@ -984,13 +1000,13 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     HIPOK(hipEventRecord(c->ev_gfork, st));
     HIPOK(hipStreamWaitEvent(st2, c->ev_gfork, 0));
     launch_camera_words(R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words, st2);
+    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill_eye(c), R.words, R.flags, eye_rays(R),
+               c->scene.next_path + 1, c->params.eye_nee, st2);
     HIPOK(hipEventRecord(c->ev_gjoin, st2));
     launch_photon(c->scene.dev, R.slots, R.N, 0, R.pass_dev, trace_spill(c), c->scene.next_path,
                   c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
     do_build(c, st);
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
-    launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill(c), R.words, R.flags, eye_rays(R),
-               c->scene.next_path + 1, c->params.eye_nee, st);
     int rc = do_gather(c, st);
     if (rc) return rc;
     if (c->guided) do_guide(c, st);
