@@ -209,6 +209,12 @@ struct sppm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;
+    // The photon pass of a pass may start on stream3 as soon as the previous
+    // pass's build has read the photon slots (ev_built), beside that pass's
+    // gather; lead_ok says that nothing but sppm_pass has queued work since.
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_built = nullptr, ev_photons = nullptr;
+    bool lead_ok = false;
     bool own_stream = false;
     // Camera sample words (UniformSample stream) are generated on stream2 one
     // pass ahead into two buffers: block k (the words of the k-th eye pass)
@@ -259,6 +265,9 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
         c->own_stream = true;
     }
     HIPOK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPOK(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
+    HIPOK(hipEventCreateWithFlags(&c->ev_built, hipEventDisableTiming));
+    HIPOK(hipEventCreateWithFlags(&c->ev_photons, hipEventDisableTiming));
     for (int b = 0; b < 2; ++b) {
         HIPOK(hipEventCreateWithFlags(&c->ev_words[b], hipEventDisableTiming));
         HIPOK(hipEventCreateWithFlags(&c->ev_eye[b], hipEventDisableTiming));
@@ -278,6 +287,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream3);
     c->rb.release();
     c->scene.release();
     for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather, &c->t_guide}) t->destroy();
@@ -288,9 +298,12 @@ int sppm_destroy(sppm_ctx* c) {
     }
     (void)hipEventDestroy(c->ev_gfork);
     (void)hipEventDestroy(c->ev_gjoin);
+    (void)hipEventDestroy(c->ev_built);
+    (void)hipEventDestroy(c->ev_photons);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipStreamDestroy(c->stream2);
+    (void)hipStreamDestroy(c->stream3);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPPM_OK;
@@ -655,6 +668,7 @@ int sppm_begin(sppm_ctx* c) {
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
     if (!c->have_cam) return fail(SPPM_E_STATE, "no camera set");
     if (!c->have_params) return fail(SPPM_E_STATE, "no params set");
+    c->lead_ok = false;
     HIPOK(hipSetDevice(c->device));
     HIPOK(hipStreamSynchronize(c->stream));
     HIPOK(hipStreamSynchronize(c->stream2));
@@ -739,6 +753,7 @@ int sppm_begin(sppm_ctx* c) {
 
 static int check_ready(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    c->lead_ok = false;
     if (!c->rb.ready) return fail(SPPM_E_STATE, "sppm_begin has not been called since the last setup change");
     return SPPM_OK;
 }
@@ -785,8 +800,14 @@ static unsigned long long* photon_rays(RenderBuffers& R) {
 static unsigned long long* eye_rays(RenderBuffers& R) {
     return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, eye_rays));
 }
-static void do_photons(sppm_ctx* c, int step, hipStream_t st) {
-    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, trace_spill(c), c->scene.next_path,
+// beside_gather: the photon pass runs beside the previous pass's gather and
+// takes 9/16 of the path-kernel grid, so the gather's kernels find free wave
+// slots instead of waiting for the persistent photon waves to drain (full
+// grid 5.33 ms per pass, 9/16: 5.02 on the full-material workload).
+static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather = false) {
+    TraceSpill sp = trace_spill(c);
+    if (beside_gather) sp.blocks = sp.blocks * 9 / 16 > 0 ? sp.blocks * 9 / 16 : 1;
+    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, sp, c->scene.next_path,
                   c->guided ? &c->emit : nullptr, c->rb.flags, photon_rays(c->rb), st);
 }
 // Training records of the pass (deposit histogram), Adam steps on it, and
@@ -881,7 +902,9 @@ int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
+    c->t_photons.start(c->stream);
     do_photons(c, step, c->stream);
+    c->t_photons.stop(c->stream);
     HIPOK(hipGetLastError());
     if (emitted) *emitted = c->rb.N;
     return SPPM_OK;
@@ -890,7 +913,9 @@ int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
+    c->t_build.start(c->stream);
     do_build(c, c->stream);
+    c->t_build.stop(c->stream);
     HIPOK(hipGetLastError());
     if (stored) {
         HIPOK(hipMemcpyAsync(stored, c->rb.nstored, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -902,7 +927,7 @@ int sppm_pass_eye(sppm_ctx* c, int step) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
-    rc = do_eye(c, step, c->stream, nullptr);
+    rc = do_eye(c, step, c->stream, &c->t_eye);
     if (rc) return rc;
     HIPOK(hipGetLastError());
     return SPPM_OK;
@@ -911,8 +936,10 @@ int sppm_pass_gather(sppm_ctx* c) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
+    c->t_gather.start(c->stream);
     rc = do_gather(c, c->stream);
     if (rc) return rc;
+    c->t_gather.stop(c->stream);
     if (c->guided) do_guide(c, c->stream);
     HIPOK(hipGetLastError());
     return SPPM_OK;
@@ -946,6 +973,10 @@ static int read_pass_stats(sppm_ctx* c, sppm_pass_stats* stats) {
 }
 
 int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
+    // unguided passes in a row: this photon pass need not wait for the
+    // previous gather (guided emission samples the proposal that the previous
+    // pass trains after its gather, from that pass's photon slots)
+    const bool lead = c && c->lead_ok && !c->guided;
     int rc = check_ready(c);
     if (rc) return rc;
     const int step = c->pass + 1;
@@ -967,12 +998,24 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
     rc = words_generate_upto(c, c->words_used);
     if (rc) return rc;
-    c->t_photons.start(st);
-    do_photons(c, step, st);
-    c->t_photons.stop(st);
+    // The photon pass rewrites the photon slots, which the previous pass's
+    // build read; its gather reads only the grid built from them.
+    hipStream_t sp = st;
+    if (lead) {
+        sp = c->stream3;
+        HIPOK(hipStreamWaitEvent(sp, c->ev_built, 0));
+    }
+    c->t_photons.start(sp);
+    do_photons(c, step, sp, lead);
+    c->t_photons.stop(sp);
+    if (lead) {
+        HIPOK(hipEventRecord(c->ev_photons, sp));
+        HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
+    }
     c->t_build.start(st);
     do_build(c, st);
     c->t_build.stop(st);
+    HIPOK(hipEventRecord(c->ev_built, st));
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
     c->t_gather.start(st);
     rc = do_gather(c, st);
@@ -986,6 +1029,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     launch_add_i32(c->rb.pass_dev, 1, st);
     HIPOK(hipGetLastError());
     c->pass += 1;
+    c->lead_ok = true;
     return stats ? read_pass_stats(c, stats) : SPPM_OK;
 }
 
@@ -1074,6 +1118,7 @@ int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tu
     if (!c || !scene || !out) return fail(SPPM_E_ARG, "NULL argument");
     if (passes < 1) return fail(SPPM_E_ARG, "passes < 1");
     if (!c->have_cam || !c->have_params) return fail(SPPM_E_STATE, "camera and params must be set before tuning");
+    c->lead_ok = false;
     memset(out, 0, sizeof *out);
     const int widths[3] = {2, 4, 8};
     float best = 3.4e38f;
@@ -1243,6 +1288,7 @@ int sppm_ray_counts(sppm_ctx* c, int64_t* photon_rays, int64_t* eye_rays) {
 
 int sppm_sync(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    HIPOK(hipStreamSynchronize(c->stream3));
     HIPOK(hipStreamSynchronize(c->stream2));
     HIPOK(hipStreamSynchronize(c->stream));
     return SPPM_OK;
@@ -1689,6 +1735,7 @@ static int guide_light_points(sppm_ctx* c, const GuideEmit& E) {
 
 int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
+    c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
     HIPOK(hipSetDevice(c->device));
     if (!p->enable) {
@@ -1748,6 +1795,7 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
 
 static int guide_ready(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    c->lead_ok = false;
     if (!c->guide.theta) return fail(SPPM_E_STATE, "guided emission not set up (sppm_guide_set)");
     HIPOK(hipSetDevice(c->device));
     return SPPM_OK;
@@ -1808,6 +1856,7 @@ int sppm_guide_read(sppm_ctx* c, float* prob, float* hist) {
 int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, float tmax,
                        int32_t* facet, float* t, float* u, float* v) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
     if (n <= 0) return SPPM_OK;
     if (!org || !dir || !facet) return fail(SPPM_E_ARG, "NULL argument");
@@ -1848,6 +1897,7 @@ int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* di
 int sppm_trace_occluded(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, const float* tmax,
                         float translucency, int32_t* occluded) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
     if (n <= 0) return SPPM_OK;
     if (!org || !dir || !tmax || !occluded) return fail(SPPM_E_ARG, "NULL argument");
