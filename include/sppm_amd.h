@@ -6,7 +6,8 @@
  * context handle, plain pointers and sizes, int status (0 = ok, <0 = error,
  * message in sppm_last_error()).  Device memory is owned by the context;
  * host buffers are owned by the caller.  A context is bound to one HIP
- * stream and is not thread-safe; distinct contexts are independent.
+ * stream (two private side streams join it inside each pass) and is not
+ * thread-safe; distinct contexts are independent.
  *
  * Reference interfaces replaced (file:line under SPPM/SPPM/):
  *   Scene::CreateScene ............... Scene.cpp:13-218   -> sppm_scene_upload
@@ -196,14 +197,21 @@ int sppm_begin(sppm_ctx* ctx);
 
 /* ---- per pass ------------------------------------------------------------- */
 /* One full pass (SPPM.cpp:770-843): resets, photon pass (step = pass+1),
- * gather-structure build, eye pass (step = pass+1), gather + update. */
+ * gather-structure build, eye pass (step = pass+1), gather + update.  The
+ * eye pass runs on a side stream beside the photon pass and the build; when
+ * the previous call was also sppm_pass and emission is not guided, the
+ * photon pass starts on a second side stream beside the previous pass's
+ * gather.  Work queued on the context stream afterwards sees the whole
+ * pass. */
 int sppm_pass(sppm_ctx* ctx, sppm_pass_stats* stats);
-/* Split steps of the same pass, for tests.  Order: photons, build, eye, gather. */
+/* Split steps of the same pass, for tests, one after another on the context
+ * stream.  Order: photons, build, eye, gather; they do not advance the pass
+ * counter (steps are explicit, as in createPhotonMap / createHitpoints). */
 int sppm_pass_photons(sppm_ctx* ctx, int step, int32_t* emitted);
 int sppm_pass_build(sppm_ctx* ctx, int32_t* stored);
 int sppm_pass_eye(sppm_ctx* ctx, int step);
 int sppm_pass_gather(sppm_ctx* ctx);
-/* Synchronise the context's stream. */
+/* Synchronise the context's stream and its side streams. */
 int sppm_sync(sppm_ctx* ctx);
 
 /* reconstructImageFromSharedPoints: rgba = W*H*4 floats. total_photons and
@@ -347,9 +355,10 @@ typedef struct sppm_tune_result {
 int sppm_autotune(sppm_ctx* ctx, const sppm_scene_desc* scene, int passes, sppm_tune_result* out);
 
 /* ---- timing helpers for the benchmark -------------------------------------- */
-/* Average duration (ms) of the named pass phase over the `sppm_pass` calls
- * since sppm_begin or the last reset, measured with hipEvents recorded on the
- * context stream around the phase's launches.  name: "photons", "build",
+/* Average duration (ms) of the named pass phase over the `sppm_pass` and
+ * split-step calls since sppm_begin or the last reset, measured with
+ * hipEvents recorded around the phase's launches on the stream it runs on
+ * (in sppm_pass, phases on different streams overlap).  name: "photons", "build",
  * "eye", "gather", "guide" (training, guided emission only); "reset" clears
  * the averages. */
 int sppm_kernel_ms(sppm_ctx* ctx, const char* name, float* ms);
