@@ -672,6 +672,7 @@ int sppm_begin(sppm_ctx* c) {
     HIPOK(hipSetDevice(c->device));
     HIPOK(hipStreamSynchronize(c->stream));
     HIPOK(hipStreamSynchronize(c->stream2));
+    HIPOK(hipStreamSynchronize(c->stream3));
     c->words_gen = c->words_used = 0;
     c->eye_recorded[0] = c->eye_recorded[1] = false;
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -738,6 +739,7 @@ int sppm_begin(sppm_ctx* c) {
     R.ntiles = gather_tile_count(W, Hh);
     launch_set_u64x2(R.rstat, (unsigned long long)((double)c->params.first_radius * 4294967296.0), 1ull, c->stream);
     launch_set_u32(R.flags, 0u, c->stream);
+    launch_set_u32(reinterpret_cast<uint32_t*>(R.nstored), 0u, c->stream);
     {
         Counters z{0, 0, 0, 0};
         HIPOK(hipMemcpyAsync(R.ctr, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
