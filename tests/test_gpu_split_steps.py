@@ -59,3 +59,35 @@ def test_split_steps_match_whole_passes(which, box_scene, full_scene_small):
                                whole.reconstruct(total_photons=passes * n, passes=passes), rtol=1e-4, atol=1e-6)
     whole.close()
     split.close()
+
+
+def test_calls_between_passes_keep_the_render(box_scene):
+    """Whole passes with readbacks, ray queries and counter reads between
+    them (each of which makes the next pass wait for the main stream instead
+    of starting its photon pass beside the previous gather) == the same
+    passes back to back."""
+    cam = scenes.pinhole_camera(48, 40)
+    plain = sppm_amd.Renderer(box_scene, cam, 0.7, 0.03, 20000, bvh_width=4)
+    mixed = sppm_amd.Renderer(box_scene, cam, 0.7, 0.03, 20000, bvh_width=4)
+    for _ in range(6):
+        plain.pass_(stats=False)
+    rng = np.random.default_rng(3)
+    org = np.tile(np.array([[0.0, 0.5, 0.5]], np.float32), (256, 1))
+    dirs = rng.normal(size=(256, 3)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    mixed.pass_(stats=False)
+    mixed.pass_(stats=False)
+    n1 = len(mixed.read_photons()["pos"])
+    mixed.pass_(stats=False)
+    mixed.trace(org, dirs)
+    mixed.pass_(stats=True)
+    mixed.counters()
+    mixed.pass_(stats=False)
+    mixed.read_hitpoints()
+    mixed.pass_(stats=False)
+    print(f"interleaved: {n1} photons stored after pass 2", flush=True)
+    plain.sync()
+    mixed.sync()
+    same_render(plain, mixed)
+    plain.close()
+    mixed.close()
