@@ -57,6 +57,8 @@ def _load_oracle():
     L.orc_bsdf_pdf.argtypes = [C.c_int, C.POINTER(FParam)]
     L.orc_bsdf_pdf.restype = C.c_float
     L.orc_bsdf_sample.argtypes = [C.c_int, C.POINTER(FParam), C.c_uint32, F, F, C.POINTER(C.c_int)]
+    L.orc_bsdf_eval_batch.argtypes = [C.c_int, C.POINTER(FParam), C.c_int, F, F, F]
+    L.orc_bsdf_sample_batch.argtypes = [C.c_int, C.POINTER(FParam), C.c_uint32, C.c_int, F, F]
     L.orc_scene_create.argtypes = [C.POINTER(SceneDesc)]
     L.orc_scene_create.restype = C.c_void_p
     L.orc_scene_destroy.argtypes = [C.c_void_p]
@@ -89,7 +91,96 @@ def _load_oracle():
     L.orc_eye_pixel.argtypes = [C.c_void_p, C.c_int, C.c_int, F, I, I, F, F, F, F]
     L.orc_photon_path.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, F, F, F, I]
     L.orc_read_path_counts.argtypes = [C.c_void_p, I]
+    L.orc_script_set.argtypes = [D, C.c_int]
+    L.orc_script_used.restype = C.c_int
+    L.orc_guide_create.argtypes = [C.c_void_p, C.c_float]
+    L.orc_guide_create.restype = C.c_void_p
+    L.orc_guide_destroy.argtypes = [C.c_void_p]
+    L.orc_guide_set_tables.argtypes = [C.c_void_p, F, F]
+    L.orc_render_set_guide.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_guide_frame.argtypes = [C.c_void_p, F]
+    L.orc_guide_point_cells.argtypes = [C.c_void_p, F, I, I]
+    L.orc_guide_density.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, F, F, I]
+    L.orc_guide_density.restype = C.c_float
+    L.orc_guide_emit.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, F, I]
+    L.orc_guide_density_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, I, I, F, F, F]
+    L.orc_guide_emit_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, F, F, F, I]
     return L
+
+
+class script:
+    """Context manager: RandomMT draws on this thread come from `draws`
+    (0.5 past the end).  `used` afterwards = draws taken."""
+
+    def __init__(self, draws):
+        self.draws = np.ascontiguousarray(draws, np.float64)
+        self.used = None
+
+    def __enter__(self):
+        orc().orc_script_set(self.draws.ctypes.data_as(D), len(self.draws))
+        return self
+
+    def __exit__(self, *exc):
+        self.used = orc().orc_script_used()
+        orc().orc_script_set(None, -1)
+        return False
+
+
+GUIDE_CELLS = 262144
+
+
+class OracleGuide:
+    """Guided emission over an OracleScene's single rectangular light."""
+
+    def __init__(self, oscene, mix_plain):
+        self.oscene = oscene
+        self.h = orc().orc_guide_create(oscene.h, mix_plain)
+        if not self.h:
+            raise ValueError("guided emission needs exactly one rectangular light of two facets")
+        self.textured = bool(oscene.lights()["textured"][0])
+
+    def __del__(self):
+        try:
+            orc().orc_guide_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_tables(self, prob, cdf=None):
+        prob = np.ascontiguousarray(prob, np.float32)
+        if cdf is None:
+            cdf = np.cumsum(prob, dtype=np.float64).astype(np.float32)
+        cdf = np.ascontiguousarray(cdf, np.float32)
+        orc().orc_guide_set_tables(self.h, fp(prob), fp(cdf))
+
+    def frame(self):
+        out = np.zeros(18, np.float32)
+        orc().orc_guide_frame(self.h, fp(out))
+        return dict(P0=out[0:3], e1=out[3:6], e2=out[6:9], n=out[9:12], t1=out[12:15], t2=out[15:18])
+
+    def point_cells(self):
+        mass = np.zeros(1024, np.float32)
+        start = np.zeros(1025, np.int32)
+        cell = np.zeros(2000, np.int32)
+        orc().orc_guide_point_cells(self.h, fp(mass), ip(start), ip(cell))
+        return mass, start, cell
+
+    def density_batch(self, pos, d, light=None, point=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = pos.shape[0]
+        q = np.zeros(n, np.float32)
+        la = np.ascontiguousarray(light, np.int32) if light is not None else None
+        pa = np.ascontiguousarray(point, np.int32) if point is not None else None
+        orc().orc_guide_density_batch(self.oscene.h, self.h, n, ip(la) if la is not None else None,
+                                      ip(pa) if pa is not None else None, fp(pos), fp(d), fp(q))
+        return q
+
+    def density(self, pos, d, light=0, point=0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        c = np.zeros(1, np.int32)
+        q = orc().orc_guide_density(self.oscene.h, self.h, light, point, fp(pos), fp(d), ip(c))
+        return q, int(c[0])
 
 
 _ORC = None
@@ -137,6 +228,23 @@ def fparam(indir, outdir, n, fi=1.0, ti=1.0, rough=1.0, absorb=1.0, refl=1.0):
     p.n[:] = [float(x) for x in n]
     p.from_index, p.to_index, p.roughness, p.absorption, p.reflectance = fi, ti, rough, absorb, refl
     return p
+
+
+def bsdf_eval(kind, p, outdirs):
+    """PDF() and BRDF() of surface `kind` at each outgoing direction (n x 3)."""
+    o = np.ascontiguousarray(outdirs, np.float32)
+    n = o.shape[0]
+    pdf, brdf = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    orc().orc_bsdf_eval_batch(kind, C.byref(p), n, fp(o), fp(pdf), fp(brdf))
+    return pdf, brdf
+
+
+def bsdf_sample(kind, p, n, seed0=1):
+    """n sampled directions and their sampling pdfs (RandomMT(seed0 + j))."""
+    d = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    orc().orc_bsdf_sample_batch(kind, C.byref(p), seed0, n, fp(d), fp(pdf))
+    return d, pdf
 
 
 class OracleScene:
@@ -210,6 +318,23 @@ class OracleRender:
             orc().orc_render_destroy(self.h)
         except Exception:
             pass
+
+    def set_guide(self, guide):
+        """Photon passes emit from `guide` (an OracleGuide; None: plain)."""
+        self._guide = guide
+        orc().orc_render_set_guide(self.h, guide.h if guide is not None else None)
+
+    def guide_emit_batch(self, step, first, n):
+        pos, d = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        pdf, c = np.zeros(n, np.float32), np.zeros(n, np.int32)
+        orc().orc_guide_emit_batch(self.h, step, first, n, fp(pos), fp(d), fp(pdf), ip(c))
+        return pos, d, pdf, c
+
+    def guide_emit(self, step, path):
+        pos, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        pdf, c = np.zeros(1, np.float32), np.zeros(1, np.int32)
+        orc().orc_guide_emit(self.h, step, int(path), fp(pos), fp(d), fp(pdf), ip(c))
+        return pos, d, float(pdf[0]), int(c[0])
 
     def full_pass(self, record=False):
         e = orc().orc_full_pass(self.h, self.passes, int(record))

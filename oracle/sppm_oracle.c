@@ -98,7 +98,24 @@ uint32_t orc_mt_next(orc_mt* m) {
     z ^= (z >> 18);
     return z;
 }
+/* Scripted draws (pinning tests only): while a script is installed on the
+ * calling thread, every genrand64_real1 / real2 returns the next scripted
+ * value instead of the engine's output, so a test can see which draw feeds
+ * which variable.  Past the end of the script the draws return 0.5 and the
+ * overrun is counted. */
+static _Thread_local const double* g_script = NULL;
+static _Thread_local int g_script_n = 0, g_script_pos = 0;
+void orc_script_set(const double* draws, int n) {
+    g_script = n >= 0 ? draws : NULL;
+    g_script_n = n > 0 ? n : 0;
+    g_script_pos = 0;
+}
+int orc_script_used(void) { return g_script_pos; }
 double orc_mt_real1(orc_mt* m) {
+    if (g_script) {
+        const int k = g_script_pos++;
+        return k < g_script_n ? g_script[k] : 0.5;
+    }
     double sum = (double)orc_mt_next(m);
     sum += (double)orc_mt_next(m) * 4294967296.0;
     double r = sum / 18446744073709551616.0;
@@ -744,6 +761,26 @@ static void to_fparam(const orc_fparam* a, fparam* b) {
 }
 float orc_bsdf_brdf(int kind, const orc_fparam* p) { fparam q; to_fparam(p, &q); return surf_brdf(kind, &q); }
 float orc_bsdf_pdf(int kind, const orc_fparam* p) { fparam q; to_fparam(p, &q); return surf_pdf(kind, &q); }
+/* batches for the tests' Monte Carlo integrals: PDF()/BRDF() at n directions
+ * `out` (the rest of p fixed), and n samples from RandomMT(seed0 + j) */
+void orc_bsdf_eval_batch(int kind, const orc_fparam* p, int n, const float* outdirs, float* pdf, float* brdf) {
+    fparam q;
+    to_fparam(p, &q);
+    for (int j = 0; j < n; ++j) {
+        q.out = vload(outdirs + 3 * (size_t)j);
+        if (pdf) pdf[j] = surf_pdf(kind, &q);
+        if (brdf) brdf[j] = surf_brdf(kind, &q);
+    }
+}
+void orc_bsdf_sample_batch(int kind, const orc_fparam* p, uint32_t seed0, int n, float* dirs, float* pdf) {
+    fparam q;
+    to_fparam(p, &q);
+    for (int j = 0; j < n; ++j) {
+        orc_mt m;
+        orc_mt_seed(&m, seed0 + (uint32_t)j);
+        vstore(dirs + 3 * (size_t)j, surf_sample(kind, &q, &m, pdf + j));
+    }
+}
 void orc_bsdf_sample(int kind, const orc_fparam* p, uint32_t seed, float* dir, float* pdf, int* ndraw) {
     fparam q;
     to_fparam(p, &q);
@@ -1227,7 +1264,9 @@ typedef struct {  /* SharedHitPoint (SPPM.h:112-126) */
     int num_gather, num_direct;
 } oshared;
 
+struct orc_guide;
 struct orc_render {
+    const struct orc_guide* guide;  /* guided emission (NULL: the reference's plain emission) */
     const orc_scene* s;
     ocam cam;
     sppm_camera_desc camdesc;
@@ -1362,6 +1401,7 @@ static void light_point(const orc_scene* s, const olight* L, orc_mt* m, v3* lpos
         *rad = vdivs(L->flux, 2.0f * PIf * area);
     }
 }
+static v3 light_normal_of(const orc_scene* s, const olight* L);
 /* LightSampler::GetAreaLightPDF (LightSampler.h:87) = TableSampler::GetPDF */
 static float light_pick_pdf(const orc_scene* s, const olight* L) {
     return vnorm(L->flux) / s->lavg / (float)s->nlights;
@@ -1371,6 +1411,322 @@ static v3 light_normal(const orc_scene* s, const olight* L) {
     const float* Ln = s->nrm + 9 * L->facet;
     return vnormalized(vadd(vadd(vload(Ln), vload(Ln + 3)), vload(Ln + 6)));
 }
+static v3 light_normal_of(const orc_scene* s, const olight* L) { return light_normal(s, L); }
+
+/* ======================================================================
+ * Guided emission (no counterpart in the reference: it replaces the
+ * position x direction sampling of generateLightRay, SPPM.cpp:453-471, and
+ * the PDF the photon carries, SPPM.cpp:527).  Restates the product's
+ * mixture draw and density: with probability w the plain emission (light
+ * pick, point on the light or table point, cosine lobe about the light
+ * rectangle's normal), otherwise a proposal cell (32x32 light (u, v) x 16x16
+ * hemi-octahedral direction (s, t)) by CDF search and a uniform point and
+ * direction inside it (a table point of the position cell, by the plain
+ * probabilities, for a textured light).  The photon's PDF is the mixture
+ * density at the drawn (point, direction).
+ * ====================================================================== */
+#define G_GU 32
+#define G_GV 32
+#define G_GS 16
+#define G_GT 16
+#define G_CELLS (G_GU * G_GV * G_GS * G_GT)
+
+struct orc_guide {
+    v3 P0, e1, e2, n, t1, t2;
+    float inv_e1, inv_e2, inv_area;
+    v3 rad;
+    float w;
+    int textured;
+    float* prob;  /* G_CELLS, unnormalised; total = cdf[G_CELLS - 1] */
+    float* cdf;
+    /* textured: table point (light li, entry k) -> position cell; points by cell */
+    int32_t* pt_cell;  /* 2 * 1000, index li * 1000 + k */
+    float* pcell_mass; /* G_GU * G_GV */
+    int32_t* pc_start; /* G_GU * G_GV + 1 */
+    int32_t* pc_light; /* light and entry of each point, in cell order */
+    int32_t* pc_entry;
+    float* pc_cum;     /* inclusive cumulative plain probability within the cell */
+};
+
+/* The light frame: exactly one rectangle made of two facets.  Corner P0 is
+ * one end of the longest vertex-to-vertex distance (the diagonal), e1 and e2
+ * run to the two other vertices, ordered so that e1 x e2 faces like the
+ * facets' normal. */
+orc_guide* orc_guide_create(const orc_scene* s, float mix_plain) {
+    if (s->nlights != 2 || s->lights[0].textured != s->lights[1].textured) return NULL;
+    v3 v[6];
+    int nv = 0;
+    for (int li = 0; li < 2; ++li) {
+        const float* P = s->pos + 9 * s->lights[li].facet;
+        for (int k = 0; k < 3; ++k) {
+            v3 p = vload(P + 3 * k);
+            int dup = 0;
+            for (int q = 0; q < nv; ++q) dup = dup || vnorm2(vsub(p, v[q])) < 1e-12f;
+            if (!dup) v[nv++] = p;
+        }
+    }
+    if (nv != 4) return NULL;
+    int a = 0, b = 1;
+    float best = -1.0f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j)
+            if (vnorm2(vsub(v[i], v[j])) > best) { best = vnorm2(vsub(v[i], v[j])); a = i; b = j; }
+    int o[2], k = 0;
+    for (int i = 0; i < 4; ++i)
+        if (i != a && i != b) o[k++] = i;
+    orc_guide* g = (orc_guide*)calloc(1, sizeof(orc_guide));
+    g->P0 = v[a];
+    v3 e1 = vsub(v[o[0]], g->P0), e2 = vsub(v[o[1]], g->P0);
+    const v3 ln = light_normal_of(s, &s->lights[0]);
+    if (vdot(vcross(e1, e2), ln) < 0.0f) { v3 t = e1; e1 = e2; e2 = t; }
+    const float l1 = sqrtf(vnorm2(e1)), l2 = sqrtf(vnorm2(e2));
+    g->e1 = e1;
+    g->e2 = e2;
+    g->n = vnormalized(vcross(e1, e2));
+    g->t1 = vdivs(e1, l1);
+    g->t2 = vcross(g->n, g->t1);
+    g->inv_e1 = 1.0f / vnorm2(e1);
+    g->inv_e2 = 1.0f / vnorm2(e2);
+    g->inv_area = 1.0f / (l1 * l2);
+    {
+        const olight* L0 = &s->lights[0];
+        const float area = facet_area(s->pos + 9 * L0->facet);
+        g->rad = V3(L0->flux.x / (2.0f * PIf * area), L0->flux.y / (2.0f * PIf * area), L0->flux.z / (2.0f * PIf * area));
+    }
+    g->w = mix_plain;
+    g->textured = s->lights[0].textured;
+    g->prob = (float*)calloc(G_CELLS, sizeof(float));
+    g->cdf = (float*)calloc(G_CELLS, sizeof(float));
+    if (g->textured) {
+        /* plain probability of point k of light li: P(pick li) * value_k /
+         * (1000 * average) (LightSampler.cpp:68-76); its position cell */
+        const int ncell = G_GU * G_GV;
+        g->pt_cell = (int32_t*)malloc(sizeof(int32_t) * 2 * LIGHT_TABLE_SAMPLES);
+        double* pi = (double*)malloc(sizeof(double) * 2 * LIGHT_TABLE_SAMPLES);
+        for (int li = 0; li < 2; ++li) {
+            const olight* L = &s->lights[li];
+            const double pick = (double)vnorm(L->flux) / ((double)s->lavg * (double)s->nlights);
+            const float* P = s->pos + 9 * L->facet;
+            const v3 p0 = vload(P), p1 = vload(P + 3), p2 = vload(P + 6);
+            for (int q = 0; q < LIGHT_TABLE_SAMPLES; ++q) {
+                const v3 w = L->uvw[q];
+                const float val = vnorm(L->L[q]);
+                pi[li * LIGHT_TABLE_SAMPLES + q] = pick * (double)val / ((double)L->avg * 1000.0);
+                const v3 pos = vadd(vadd(vscale(p0, w.x), vscale(p1, w.y)), vscale(p2, 1.0f - w.x - w.y));
+                const v3 rel = vsub(pos, g->P0);
+                const float u = fminf_(fmaxf_(vdot(rel, e1) * g->inv_e1, 0.0f), 0.99999994f);
+                const float vv = fminf_(fmaxf_(vdot(rel, e2) * g->inv_e2, 0.0f), 0.99999994f);
+                int iu = (int)(u * (float)G_GU), iv = (int)(vv * (float)G_GV);
+                if (iu > G_GU - 1) iu = G_GU - 1;
+                if (iv > G_GV - 1) iv = G_GV - 1;
+                g->pt_cell[li * LIGHT_TABLE_SAMPLES + q] = iu * G_GV + iv;
+            }
+        }
+        g->pcell_mass = (float*)calloc(ncell, sizeof(float));
+        g->pc_start = (int32_t*)calloc(ncell + 1, sizeof(int32_t));
+        g->pc_light = (int32_t*)malloc(sizeof(int32_t) * 2 * LIGHT_TABLE_SAMPLES);
+        g->pc_entry = (int32_t*)malloc(sizeof(int32_t) * 2 * LIGHT_TABLE_SAMPLES);
+        g->pc_cum = (float*)malloc(sizeof(float) * 2 * LIGHT_TABLE_SAMPLES);
+        int npts = 0;
+        for (int c = 0; c < ncell; ++c) {  /* cells in order, points in (light, entry) order */
+            g->pc_start[c] = npts;
+            double run = 0.0;
+            for (int j = 0; j < 2 * LIGHT_TABLE_SAMPLES; ++j) {
+                if (g->pt_cell[j] != c || !(pi[j] > 0.0)) continue;
+                run += pi[j];
+                g->pc_light[npts] = j / LIGHT_TABLE_SAMPLES;
+                g->pc_entry[npts] = j % LIGHT_TABLE_SAMPLES;
+                g->pc_cum[npts] = (float)run;
+                ++npts;
+            }
+            g->pcell_mass[c] = npts > g->pc_start[c] ? g->pc_cum[npts - 1] : 0.0f;
+        }
+        g->pc_start[ncell] = npts;
+        free(pi);
+    }
+    return g;
+}
+void orc_guide_destroy(orc_guide* g) {
+    if (!g) return;
+    free(g->prob); free(g->cdf); free(g->pt_cell); free(g->pcell_mass); free(g->pc_start);
+    free(g->pc_light); free(g->pc_entry); free(g->pc_cum);
+    free(g);
+}
+void orc_guide_set_tables(orc_guide* g, const float* prob, const float* cdf) {
+    memcpy(g->prob, prob, sizeof(float) * G_CELLS);
+    memcpy(g->cdf, cdf, sizeof(float) * G_CELLS);
+}
+void orc_render_set_guide(orc_render* r, const orc_guide* g) { r->guide = g; }
+void orc_guide_frame(const orc_guide* g, float* out) {  /* P0, e1, e2, n, t1, t2 (18 floats) */
+    vstore(out, g->P0); vstore(out + 3, g->e1); vstore(out + 6, g->e2);
+    vstore(out + 9, g->n); vstore(out + 12, g->t1); vstore(out + 15, g->t2);
+}
+/* textured light: plain probability mass and point count of each position cell */
+void orc_guide_point_cells(const orc_guide* g, float* mass, int32_t* start, int32_t* pt_cell) {
+    if (!g->textured) return;
+    if (mass) memcpy(mass, g->pcell_mass, sizeof(float) * G_GU * G_GV);
+    if (start) memcpy(start, g->pc_start, sizeof(int32_t) * (G_GU * G_GV + 1));
+    if (pt_cell) memcpy(pt_cell, g->pt_cell, sizeof(int32_t) * 2 * LIGHT_TABLE_SAMPLES);
+}
+
+static int cdf_search(const float* cdf, int lo, int hi, float x) {  /* first entry > x in [lo, hi] */
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] > x) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+/* Mixture density of (table point (tl, tk) or position lpos, direction
+ * outdir); also the emission cell and the emitted radiance. */
+static float guide_density(const orc_scene* s, const orc_guide* G, int tl, int tk, v3 lpos, v3 outdir, int* cell_out,
+                           v3* rad0) {
+    const float total = G->cdf[G_CELLS - 1];
+    const float lx = vdot(outdir, G->t1), ly = vdot(outdir, G->t2), lz = vdot(outdir, G->n);
+    const float cs = lz > 0.0f ? lz : 0.0f;
+    const float l1 = fabsf(lx) + fabsf(ly) + cs;
+    const float ox = lx / l1, oy = ly / l1, oz = cs / l1;
+    const float ss = ox + oy, tt = ox - oy;
+    int is = (int)((ss + 1.0f) * (0.5f * (float)G_GS)), it = (int)((tt + 1.0f) * (0.5f * (float)G_GT));
+    is = is < 0 ? 0 : (is > G_GS - 1 ? G_GS - 1 : is);
+    it = it < 0 ? 0 : (it > G_GT - 1 ? G_GT - 1 : it);
+    int pcell;
+    if (G->textured) {
+        pcell = G->pt_cell[tl * LIGHT_TABLE_SAMPLES + tk];
+    } else {
+        const v3 rel = vsub(lpos, G->P0);
+        const float u = fminf(fmaxf(vdot(rel, G->e1) * G->inv_e1, 0.0f), 0.99999994f);
+        const float v = fminf(fmaxf(vdot(rel, G->e2) * G->inv_e2, 0.0f), 0.99999994f);
+        int iu = (int)(u * (float)G_GU), iv = (int)(v * (float)G_GV);
+        if (iu > G_GU - 1) iu = G_GU - 1;
+        if (iv > G_GV - 1) iv = G_GV - 1;
+        pcell = iu * G_GV + iv;
+    }
+    const int cell = (pcell * G_GS + is) * G_GT + it;
+    const float plen = sqrtf(ox * ox + oy * oy + oz * oz);
+    /* direction density inside a cell: 1 / (cell area in (s, t)) * 2|P|^3, since d omega = ds dt / (2|P|^3) */
+    const float dd = (float)(G_GS * G_GT / 4) * (2.0f * plen * plen * plen);
+    const float pc_prob = G->prob[cell] / total;
+    float q;
+    if (G->textured) {
+        const olight* L = &s->lights[tl];
+        const float area = facet_area(s->pos + 9 * L->facet);
+        const float gp = vnorm(L->L[tk]) / L->avg / 1000.0f;
+        const float Dk = ((gp * 1000.0f) / area) * (vnorm(L->flux) / s->lavg / (float)s->nlights);
+        const float mass = G->pcell_mass[pcell];
+        const float prop = mass > 0.0f ? pc_prob * dd / mass : 0.0f;
+        q = Dk * (G->w * cs / PIf + (1.0f - G->w) * prop);
+        if (rad0) *rad0 = L->L[tk];
+    } else {
+        const float p_mlp = pc_prob * (float)(G_GU * G_GV) * G->inv_area * dd;
+        const float p_plain = G->inv_area * cs / PIf;
+        q = G->w * p_plain + (1.0f - G->w) * p_mlp;
+        if (rad0) *rad0 = G->rad;
+    }
+    if (cell_out) *cell_out = cell;
+    return q;
+}
+float orc_guide_density(const orc_scene* s, const orc_guide* g, int light, int point, const float* pos,
+                        const float* dir, int32_t* cell) {
+    int c = 0;
+    const float q = guide_density(s, g, light, point, vload(pos), vload(dir), &c, NULL);
+    if (cell) *cell = c;
+    return q;
+}
+
+void orc_guide_density_batch(const orc_scene* s, const orc_guide* g, int n, const int32_t* light,
+                             const int32_t* point, const float* pos, const float* dir, float* q) {
+    for (int j = 0; j < n; ++j)
+        q[j] = guide_density(s, g, light ? light[j] : 0, point ? point[j] : 0, vload(pos + 3 * (size_t)j),
+                             vload(dir + 3 * (size_t)j), NULL, NULL);
+}
+
+/* The mixture draw: position, direction, PDF, radiance and cell. */
+static void guided_start(const orc_scene* s, const orc_guide* G, orc_mt* m, v3* lpos_o, v3* dir_o, float* pdf_o,
+                         v3* rad_o, int* cell_o) {
+    const float total = G->cdf[G_CELLS - 1];
+    v3 lpos, outdir;
+    int tl = 0, tk = 0;
+    if (orc_mt_real1(m) < (double)G->w) {
+        /* plain: light pick, then PointOnTriangle or a table point, cosine lobe (SPPM.cpp:453-468) */
+        const int li = table_sample(s->lprob, s->lleft, s->lright, s->nlights, m);
+        const olight* L = &s->lights[li];
+        const float* P = s->pos + 9 * L->facet;
+        const v3 p0 = vload(P), p1 = vload(P + 3), p2 = vload(P + 6);
+        if (G->textured) {
+            const int k = table_sample(L->prob, L->left, L->right, LIGHT_TABLE_SAMPLES, m);
+            const v3 w = L->uvw[k];
+            lpos = vadd(vadd(vscale(p0, w.x), vscale(p1, w.y)), vscale(p2, 1.0f - w.x - w.y));
+            tl = li;
+            tk = k;
+        } else {
+            float u = (float)orc_mt_real1(m);
+            float v = (float)orc_mt_real1(m);
+            if (v > u) { u *= 0.5f; v -= u; }
+            else { v *= 0.5f; u -= v; }
+            lpos = vadd(vadd(vscale(p0, u), vscale(p1, v)), vscale(p2, 1.0f - u - v));
+        }
+        float pdf_dir = 1.0f;
+        outdir = vnormalized(cos_sampling(m, G->n, &pdf_dir));
+    } else {
+        /* proposal: a cell by its CDF (x < total, so the cell has probability > 0) */
+        const float xr = (float)orc_mt_real1(m) * total;
+        union { float f; uint32_t u; } below;
+        below.f = total;
+        below.u -= 1u;
+        const float x = fminf(xr, below.f);
+        const int c = cdf_search(G->cdf, 0, G_CELLS - 1, x);
+        const int it = c % G_GT, is = (c / G_GT) % G_GS, iv = (c / (G_GS * G_GT)) % G_GV, iu = c / (G_GV * G_GS * G_GT);
+        if (G->textured) {
+            const int pc = iu * G_GV + iv;
+            const int a = G->pc_start[pc], e = G->pc_start[pc + 1];
+            const float x2 = (float)orc_mt_real1(m) * G->pcell_mass[pc];
+            const int q = e > a ? cdf_search(G->pc_cum, a, e - 1, x2) : a;
+            tl = G->pc_light[q];
+            tk = G->pc_entry[q];
+            const olight* L = &s->lights[tl];
+            const float* P = s->pos + 9 * L->facet;
+            const v3 p0 = vload(P), p1 = vload(P + 3), p2 = vload(P + 6);
+            const v3 w = L->uvw[tk];
+            lpos = vadd(vadd(vscale(p0, w.x), vscale(p1, w.y)), vscale(p2, 1.0f - w.x - w.y));
+        } else {
+            const float u = ((float)iu + (float)orc_mt_real1(m)) / (float)G_GU;
+            const float v = ((float)iv + (float)orc_mt_real1(m)) / (float)G_GV;
+            lpos = vadd(vadd(G->P0, vscale(G->e1, u)), vscale(G->e2, v));
+        }
+        const float ss = -1.0f + ((float)is + (float)orc_mt_real1(m)) * (2.0f / (float)G_GS);
+        const float tt = -1.0f + ((float)it + (float)orc_mt_real1(m)) * (2.0f / (float)G_GT);
+        const float px = 0.5f * (ss + tt), py = 0.5f * (ss - tt);
+        const float pz = fmaxf(0.0f, 1.0f - fabsf(px) - fabsf(py));
+        outdir = vnormalized(vadd(vadd(vscale(G->t1, px), vscale(G->t2, py)), vscale(G->n, pz)));
+    }
+    v3 rad0;
+    const float q = guide_density(s, G, tl, tk, lpos, outdir, cell_o, &rad0);
+    const float cs = fmaxf(vdot(outdir, G->n), 0.0f);
+    *rad_o = vscale(rad0, fminf_(1.0f, cs));
+    *lpos_o = lpos;
+    *dir_o = outdir;
+    *pdf_o = q;
+}
+
+/* The start of guided photon path i of pass `step`: slot-0 position, direction, PDF, cell. */
+void orc_guide_emit(const orc_render* r, int step, int path, float* pos, float* dir, float* pdf, int32_t* cell) {
+    const int N = r->p.photons_per_pass;
+    orc_mt m;
+    orc_mt_seed(&m, (uint32_t)path + 135u + (uint32_t)step * (uint32_t)N);
+    v3 lpos, d, rad;
+    int c = 0;
+    guided_start(r->s, r->guide, &m, &lpos, &d, pdf, &rad, &c);
+    vstore(pos, vadd(lpos, vscale(r->guide->n, RAY_OFFSET)));
+    vstore(dir, d);
+    if (cell) *cell = c;
+}
+
+void orc_guide_emit_batch(const orc_render* r, int step, int first, int n, float* pos, float* dir, float* pdf,
+                          int32_t* cell) {
+    for (int j = 0; j < n; ++j)
+        orc_guide_emit(r, step, first + j, pos + 3 * (size_t)j, dir + 3 * (size_t)j, pdf + j, cell ? cell + j : NULL);
+}
 
 /* createPhotonMap, one path (SPPM.cpp:518-672) */
 static int photon_path(const orc_render* r, ophoton* slot, int i, int step) {
@@ -1378,24 +1734,34 @@ static int photon_path(const orc_render* r, ophoton* slot, int i, int step) {
     const int N = r->p.photons_per_pass;
     orc_mt m;
     orc_mt_seed(&m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
-    /* generateLightRay (SPPM.cpp:453-471) */
-    int li = table_sample(s->lprob, s->lleft, s->lright, s->nlights, &m);
-    const olight* L = &s->lights[li];
-    v3 lpos, rad;
-    float pdf_area;
-    light_point(s, L, &m, &lpos, &rad, &pdf_area);
-    pdf_area *= light_pick_pdf(s, L);
-    v3 lnorm = light_normal(s, L);
-    float pdf_dir = 1.0f;
-    v3 outdir = cos_sampling(&m, lnorm, &pdf_dir);
-    rad = vscale(rad, fminf_(1.0f, fmaxf_(0.0f, vdot(lnorm, outdir))));
-    /* Ray(lightPos + n*RAY_OFFSET, outDir): ctor normalises */
-    v3 cur_org = vadd(lpos, vscale(lnorm, RAY_OFFSET));
-    v3 cur_dir = vnormalized(outdir);
+    v3 rad, cur_org, cur_dir;
+    float PDF;
+    if (r->guide) {
+        /* guided emission: the mixture draw and its density */
+        v3 lpos;
+        int cell;
+        guided_start(s, r->guide, &m, &lpos, &cur_dir, &PDF, &rad, &cell);
+        cur_org = vadd(lpos, vscale(r->guide->n, RAY_OFFSET));
+    } else {
+        /* generateLightRay (SPPM.cpp:453-471) */
+        int li = table_sample(s->lprob, s->lleft, s->lright, s->nlights, &m);
+        const olight* L = &s->lights[li];
+        v3 lpos;
+        float pdf_area;
+        light_point(s, L, &m, &lpos, &rad, &pdf_area);
+        pdf_area *= light_pick_pdf(s, L);
+        v3 lnorm = light_normal(s, L);
+        float pdf_dir = 1.0f;
+        v3 outdir = cos_sampling(&m, lnorm, &pdf_dir);
+        rad = vscale(rad, fminf_(1.0f, fmaxf_(0.0f, vdot(lnorm, outdir))));
+        /* Ray(lightPos + n*RAY_OFFSET, outDir): ctor normalises */
+        cur_org = vadd(lpos, vscale(lnorm, RAY_OFFSET));
+        cur_dir = vnormalized(outdir);
+        PDF = pdf_area * pdf_dir;
+    }
     float cur_index = 1.0f;
     v3 next_org = cur_org, next_dir = cur_dir;
     float next_index = 1.0f;
-    float PDF = pdf_area * pdf_dir;
     v3 T = V3(1, 1, 1);
     int depth = 0;
     /* emission photon (SPPM.cpp:530-540) */

@@ -30,8 +30,16 @@
  *   eye-path NEE / MIS .................. closed form (camera filled by the light) and
  *                                         equality of means with plain BSDF sampling
  *                                         (eye_nee = 3, an oracle-only checker mode)
- *   full-path draw order ................ parity unpinned against a reference run
- *                                         (the reference does not compile as shipped)
+ *   draw order (which draw feeds which variable, per routine and along the
+ *   eye and photon paths) ............... scripted draws with closed-form
+ *                                         expectations (orc_script_set)
+ *   GGX dielectric ....................... PDF mass over the sphere = the
+ *                                         microfacet mass it covers, sampled
+ *                                         density == PDF(), BRDF closed forms
+ *                                         at normal incidence
+ *   guided emission ..................... mixture density integrates to 1,
+ *                                         sample frequencies == density,
+ *                                         unbiased against plain emission
  */
 #ifndef SPPM_ORACLE_H
 #define SPPM_ORACLE_H
@@ -81,6 +89,11 @@ float orc_bsdf_pdf(int kind, const orc_fparam* p);
 /* Draws from RandomMT(seed) ; returns direction and *pdf; *ndraw = doubles used */
 void orc_bsdf_sample(int kind, const orc_fparam* p, uint32_t seed, float* dir, float* pdf,
                      int* ndraw);
+
+/* PDF() and BRDF() at n outgoing directions (either output may be NULL), and
+ * n samples drawn from RandomMT(seed0 + j), for Monte Carlo checks */
+void orc_bsdf_eval_batch(int kind, const orc_fparam* p, int n, const float* outdirs, float* pdf, float* brdf);
+void orc_bsdf_sample_batch(int kind, const orc_fparam* p, uint32_t seed0, int n, float* dirs, float* pdf);
 
 /* ---- scene ------------------------------------------------------------- */
 typedef struct orc_scene orc_scene;
@@ -154,6 +167,38 @@ int orc_photon_path(const orc_render* r, int step, int path, float* pos, float* 
                     float* index, float* pdf, int32_t* nlist);
 /* flattened photon count of each path of the last photon pass (N entries) */
 void orc_read_path_counts(const orc_render* r, int32_t* counts);
+
+/* ---- scripted draws (pinning tests) ---------------------------------------
+ * While installed on the calling thread, every RandomMT draw returns the next
+ * of `n` scripted values (0.5 past the end); n < 0 removes the script.
+ * orc_script_used() = draws taken since it was installed. */
+void orc_script_set(const double* draws, int n);
+int orc_script_used(void);
+
+/* ---- guided emission (no reference counterpart; replaces generateLightRay's
+ * sampling, SPPM.cpp:453-471, and the photon PDF, SPPM.cpp:527) ------------ */
+typedef struct orc_guide orc_guide;
+/* NULL unless the scene's only light is one rectangle of two facets */
+orc_guide* orc_guide_create(const orc_scene* s, float mix_plain);
+void orc_guide_destroy(orc_guide* g);
+/* proposal over 262,144 cells (unnormalised) and its inclusive prefix sum */
+void orc_guide_set_tables(orc_guide* g, const float* prob, const float* cdf);
+/* photon passes of r emit from the mixture (g NULL: plain emission) */
+void orc_render_set_guide(orc_render* r, const orc_guide* g);
+/* P0, e1, e2, n, t1, t2 of the light frame (18 floats) */
+void orc_guide_frame(const orc_guide* g, float* out);
+/* textured light: plain mass per position cell (1024), CSR starts (1025), cell per table point (2000) */
+void orc_guide_point_cells(const orc_guide* g, float* mass, int32_t* start, int32_t* pt_cell);
+/* mixture density at (point, dir); textured lights: (light, point) is the table point, pos ignored */
+float orc_guide_density(const orc_scene* s, const orc_guide* g, int light, int point, const float* pos,
+                        const float* dir, int32_t* cell);
+/* the start of guided path `path` of pass `step`: slot-0 position, direction, PDF, cell */
+void orc_guide_emit(const orc_render* r, int step, int path, float* pos, float* dir, float* pdf, int32_t* cell);
+/* batches of the two (light / point may be NULL for an untextured light; cell may be NULL) */
+void orc_guide_density_batch(const orc_scene* s, const orc_guide* g, int n, const int32_t* light,
+                             const int32_t* point, const float* pos, const float* dir, float* q);
+void orc_guide_emit_batch(const orc_render* r, int step, int first, int n, float* pos, float* dir, float* pdf,
+                          int32_t* cell);
 
 #ifdef __cplusplus
 }
