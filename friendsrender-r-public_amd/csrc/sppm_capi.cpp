@@ -111,7 +111,7 @@ struct RenderBuffers {
     int* cell_count = nullptr;
     int* cell_start = nullptr;
     int* cell_scan_tmp = nullptr;
-    int2* keyrank = nullptr;
+    PhotonSortBuffers psort{};  // stable sort of the flattened photons by cell
     float4 *gpos = nullptr, *gflux = nullptr, *gdir = nullptr;
     int photon_cap = 0;
     GridParams* gp = nullptr;
@@ -143,10 +143,17 @@ struct RenderBuffers {
     Counters* ctr = nullptr;
     float4* image = nullptr;
     bool have_lists = false;
+    // record_neighbours 2: (hit point, photon) pairs appended by the production gather
+    int2* pairs = nullptr;
+    unsigned long long* npairs = nullptr;
+    long long pairs_cap = 0;
     void release() {
+        dfree(pairs); dfree(npairs);
+        pairs_cap = 0;
         dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
         dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_count); dfree(cell_start); dfree(cell_scan_tmp);
-        dfree(keyrank); dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
+        dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.src); dfree(psort.hist);
+        dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
         dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
@@ -205,6 +212,23 @@ struct Timer {
     double avg() const { return n ? total_ms / n : 0.0; }
 };
 
+// Several contexts' passes in one graph: one branch per context (two streams
+// each), forked from and joined into the batch's origin stream.  `done` is
+// recorded after every launch; member contexts wait for it before reading or
+// freeing what the graph uses.
+struct sppm_batch {
+    int device = 0;
+    std::vector<sppm_ctx*> ctx;
+    std::vector<hipStream_t> s1, s2;
+    std::vector<hipEvent_t> join;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool launched = false;
+    bool broken = false;  // a member was destroyed: the graph is gone
+};
+
 struct sppm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -223,6 +247,13 @@ struct sppm_ctx {
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;  // camera-word branch inside a captured pass
     hipGraph_t graph = nullptr;                          // captured pass (sppm_graph_capture)
     hipGraphExec_t graph_exec = nullptr;
+    // The batch whose graph holds this context's buffers (sppm_batch_add), and
+    // whether it has launched since: readbacks and synchronisation wait for
+    // the batch's last launch, setup changes are refused while it holds them.
+    sppm_batch* batch = nullptr;
+    // Hash of the scene, camera, parameters and emission setup the render
+    // was begun with (written into checkpoints, checked when loading one).
+    uint64_t setup_hash = 0, scene_hash = 0;
     int tuned_blocks = 0;                                // path-kernel grid chosen by the autotuner (0: default)
     bool eye_recorded[2] = {false, false};
     long long words_gen = 0, words_used = 0;
@@ -241,6 +272,35 @@ struct sppm_ctx {
     sppm_guide_params gparams{};
     GuideEmit emit{};
 };
+
+// Waits for the last launch of the batch whose graph uses c's buffers.
+static int batch_wait(sppm_ctx* c) {
+    if (c->batch && c->batch->launched) HIPOK(hipEventSynchronize(c->batch->done));
+    return SPPM_OK;
+}
+// Setup changes and the context's own passes are refused while a batch
+// holds its buffers (the batch graph would replay them, or freed memory).
+static int not_in_batch(sppm_ctx* c, const char* what) {
+    if (c->batch) return fail(SPPM_E_STATE, "%s: the context belongs to a batch (destroy the batch first)", what);
+    return SPPM_OK;
+}
+// A captured pass holds the scene, render and proposal buffers by value.
+static void drop_graph(sppm_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+// FNV-1a over bytes (setup fingerprints of checkpoints).
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+static const uint64_t FNV_BASIS = 0xcbf29ce484222325ull;
 
 extern "C" {
 
@@ -285,6 +345,20 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
 int sppm_destroy(sppm_ctx* c) {
     if (!c) return SPPM_OK;
     (void)hipSetDevice(c->device);
+    if (c->batch) {
+        // the batch's graph uses this context's buffers: wait for it, then
+        // leave the batch, which can no longer launch
+        sppm_batch* b = c->batch;
+        (void)batch_wait(c);
+        for (size_t k = 0; k < b->ctx.size(); ++k)
+            if (b->ctx[k] == c) b->ctx[k] = nullptr;
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        b->broken = true;
+        c->batch = nullptr;
+    }
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream3);
@@ -300,8 +374,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipEventDestroy(c->ev_gjoin);
     (void)hipEventDestroy(c->ev_built);
     (void)hipEventDestroy(c->ev_photons);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    drop_graph(c);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream3);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -335,10 +408,37 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
     for (int t = 0; t < d->ntextures; ++t)
         if (d->textures[t].width <= 0 || d->textures[t].height <= 0 || !d->textures[t].rgba)
             return fail(SPPM_E_SCENE, "texture %d is empty", t);
+    {
+        int rc0 = not_in_batch(c, "sppm_scene_upload");
+        if (rc0) return rc0;
+    }
     HIPOK(hipSetDevice(c->device));
     HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamSynchronize(c->stream2));
+    HIPOK(hipStreamSynchronize(c->stream3));
+    // the render buffers (photon slots, hit points) and any captured pass
+    // belong to the old scene: the render must be begun again
+    c->lead_ok = false;
+    c->rb.ready = false;
+    drop_graph(c);
     c->scene.release();
     SceneBuffers& S = c->scene;
+    {
+        uint64_t h = FNV_BASIS;
+        h = fnv1a(h, &d->nfacets, sizeof d->nfacets);
+        h = fnv1a(h, d->pos, sizeof(float) * 9 * (size_t)nf);
+        h = fnv1a(h, d->nrm, sizeof(float) * 9 * (size_t)nf);
+        h = fnv1a(h, d->uv, sizeof(float) * 9 * (size_t)nf);
+        h = fnv1a(h, d->attr, sizeof(int32_t) * (size_t)nf);
+        h = fnv1a(h, d->attrs, sizeof(sppm_attr) * (size_t)d->nattrs);
+        for (int t = 0; t < d->ntextures; ++t) {
+            h = fnv1a(h, &d->textures[t].width, 2 * sizeof(int32_t));
+            h = fnv1a(h, d->textures[t].rgba, sizeof(float) * 4 * (size_t)d->textures[t].width * d->textures[t].height);
+        }
+        h = fnv1a(h, &d->has_ibl, sizeof d->has_ibl);
+        h = fnv1a(h, &d->ibl_texture, sizeof d->ibl_texture);
+        c->scene_hash = h;  // the BVH width is left out: closest hits do not depend on it
+    }
     const hipStream_t st = c->stream;
 
     // ---- lights, in facet order (Scene.cpp:145-154) ----
@@ -577,6 +677,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
 // ---------------------------------------------------------------------------
 int sppm_camera_set(sppm_ctx* c, const sppm_camera_desc* cd) {
     if (!c || !cd) return fail(SPPM_E_ARG, "NULL argument");
+    if (int rc = not_in_batch(c, "sppm_camera_set")) return rc;
     if (cd->type != SPPM_CAM_PINHOLE && cd->type != SPPM_CAM_THINLENS) return fail(SPPM_E_ARG, "unknown camera type");
     if (cd->width <= 0 || cd->height <= 0) return fail(SPPM_E_ARG, "bad resolution %dx%d", cd->width, cd->height);
     if ((long long)cd->width * cd->height >= (1LL << 28)) return fail(SPPM_E_ARG, "resolution too large");
@@ -616,10 +717,14 @@ int sppm_camera_set(sppm_ctx* c, const sppm_camera_desc* cd) {
 
 int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
     if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
+    if (int rc = not_in_batch(c, "sppm_params_set")) return rc;
     if (p->photons_per_pass <= 0) return fail(SPPM_E_ARG, "photons_per_pass must be > 0");
     if (!(p->alpha > 0.0f && p->alpha <= 1.0f)) return fail(SPPM_E_ARG, "alpha must be in (0, 1]");
     if (!(p->first_radius > 0.0f)) return fail(SPPM_E_ARG, "first_radius must be > 0");
     if (p->eye_nee < SPPM_NEE_OFF || p->eye_nee > SPPM_NEE_KEEP) return fail(SPPM_E_ARG, "eye_nee must be 0, 1 or 2");
+    if (p->record_neighbours < 0 || p->record_neighbours > 2) return fail(SPPM_E_ARG, "record_neighbours must be 0, 1 or 2");
+    if (p->record_neighbours == 2 && p->max_neighbours <= 0)
+        return fail(SPPM_E_ARG, "record_neighbours 2 needs max_neighbours > 0 (pairs recorded per pass)");
     c->params = *p;
     c->have_params = true;
     c->rb.ready = false;
@@ -668,6 +773,7 @@ int sppm_begin(sppm_ctx* c) {
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
     if (!c->have_cam) return fail(SPPM_E_STATE, "no camera set");
     if (!c->have_params) return fail(SPPM_E_STATE, "no params set");
+    if (int rc = not_in_batch(c, "sppm_begin")) return rc;
     c->lead_ok = false;
     HIPOK(hipSetDevice(c->device));
     HIPOK(hipStreamSynchronize(c->stream));
@@ -675,10 +781,7 @@ int sppm_begin(sppm_ctx* c) {
     HIPOK(hipStreamSynchronize(c->stream3));
     c->words_gen = c->words_used = 0;
     c->eye_recorded[0] = c->eye_recorded[1] = false;
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
+    drop_graph(c);
     RenderBuffers& R = c->rb;
     R.release();
     const int N = c->params.photons_per_pass;
@@ -693,7 +796,10 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.slots.valid, NS)) || (rc = dalloc(&R.slots.count, N)) || (rc = dalloc(&R.off, N)) ||
         (rc = dalloc(&R.nstored, 1)) || (rc = dalloc(&R.scan_tmp, scan_tmp_entries(N))) ||
         (rc = dalloc(&R.cell_count, R.cell_cap + 1)) || (rc = dalloc(&R.cell_start, R.cell_cap + 1)) ||
-        (rc = dalloc(&R.cell_scan_tmp, scan_tmp_entries(R.cell_cap + 1))) || (rc = dalloc(&R.keyrank, R.photon_cap)) ||
+        (rc = dalloc(&R.cell_scan_tmp, scan_tmp_entries(R.cell_cap + 1))) || (rc = dalloc(&R.psort.keys, R.photon_cap)) ||
+        (rc = dalloc(&R.psort.vals, R.photon_cap)) || (rc = dalloc(&R.psort.tmp_k, R.photon_cap)) ||
+        (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.psort.src, R.photon_cap)) ||
+        (rc = dalloc(&R.psort.hist, radix_hist_entries(R.photon_cap))) ||
         (rc = dalloc(&R.gpos, R.photon_cap)) || (rc = dalloc(&R.gflux, R.photon_cap)) ||
         (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
         (rc = dalloc(&R.hits.nrm, np)) || (rc = dalloc(&R.hits.out, np)) || (rc = dalloc(&R.hits.weight, np)) ||
@@ -717,6 +823,13 @@ int sppm_begin(sppm_ctx* c) {
         return rc;
     }
     R.hits.np = np;
+    if (c->params.record_neighbours == 2) {
+        R.pairs_cap = c->params.max_neighbours;
+        if ((rc = dalloc(&R.pairs, (size_t)R.pairs_cap)) || (rc = dalloc(&R.npairs, 2))) {
+            R.release();
+            return rc;
+        }
+    }
     if (c->guided) {
         if ((rc = dalloc(&R.path_cell, N)) || (rc = dalloc(&R.pdep, R.photon_cap))) {
             R.release();
@@ -757,7 +870,13 @@ static int check_ready(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
     c->lead_ok = false;
     if (!c->rb.ready) return fail(SPPM_E_STATE, "sppm_begin has not been called since the last setup change");
-    return SPPM_OK;
+    return batch_wait(c);
+}
+// check_ready for the calls that queue pass work on the context itself
+static int check_own_pass(sppm_ctx* c, const char* what) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return not_in_batch(c, what);
 }
 
 static void fill_fixed_sample(sppm_ctx* c, int step) {
@@ -829,7 +948,7 @@ static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
     launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
-                      R.cell_count, R.cell_start, R.cell_scan_tmp, R.keyrank, R.gpos, R.gflux, R.gdir, st);
+                      R.cell_count, R.cell_start, R.cell_scan_tmp, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
 }
 // Generates camera word blocks up to and including block k on stream2, in
 // stream order (each block continues the shared mt19937 stream).
@@ -872,7 +991,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     GridDev G = grid_dev(c);
     const float alpha = c->params.alpha;
-    if (c->params.record_neighbours) {
+    if (c->params.record_neighbours == 1) {
         launch_gather(R.hits, R.shared, G, R.gp, R.W, R.H, alpha, 1, R.counts, nullptr, nullptr, R.rstat, R.ctr, R.flags, st);
         int total = 0;
         // counts[np] is kept at zero, so list_off[np] is the total
@@ -892,16 +1011,18 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
     } else {
         launch_set_u64x2(R.rstat, 0ull, 0ull, st);
         if (c->guided) launch_deposit_clear(R.pdep, R.nstored, st);
+        const PairRecord rec{R.pairs, R.npairs, R.pairs_cap};
         launch_gather_balanced(R.hits, R.shared, G, R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups, alpha, R.tiles,
                                GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
-                               R.rstat, R.ctr, R.flags, c->guided ? R.pdep : nullptr, st);
+                               R.rstat, R.ctr, R.flags, c->guided ? R.pdep : nullptr, rec, st);
+        R.have_lists = R.pairs != nullptr;
     }
     if (c->scene.dev.ibl_tex >= 0 || c->params.eye_nee != SPPM_NEE_OFF) launch_direct_light(R.hits, R.shared, R.np, st);
     return SPPM_OK;
 }
 
 int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
-    int rc = check_ready(c);
+    int rc = check_own_pass(c, "sppm_pass_photons");
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     c->t_photons.start(c->stream);
@@ -912,7 +1033,7 @@ int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
     return SPPM_OK;
 }
 int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
-    int rc = check_ready(c);
+    int rc = check_own_pass(c, "sppm_pass_build");
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     c->t_build.start(c->stream);
@@ -926,7 +1047,7 @@ int sppm_pass_build(sppm_ctx* c, int32_t* stored) {
     return SPPM_OK;
 }
 int sppm_pass_eye(sppm_ctx* c, int step) {
-    int rc = check_ready(c);
+    int rc = check_own_pass(c, "sppm_pass_eye");
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     rc = do_eye(c, step, c->stream, &c->t_eye);
@@ -935,7 +1056,7 @@ int sppm_pass_eye(sppm_ctx* c, int step) {
     return SPPM_OK;
 }
 int sppm_pass_gather(sppm_ctx* c) {
-    int rc = check_ready(c);
+    int rc = check_own_pass(c, "sppm_pass_gather");
     if (rc) return rc;
     if (c->graph_exec) return fail(SPPM_E_STATE, "split pass steps are not available after sppm_graph_capture");
     c->t_gather.start(c->stream);
@@ -979,7 +1100,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     // previous gather (guided emission samples the proposal that the previous
     // pass trains after its gather, from that pass's photon slots)
     const bool lead = c && c->lead_ok && !c->guided;
-    int rc = check_ready(c);
+    int rc = check_own_pass(c, "sppm_pass");
     if (rc) return rc;
     const int step = c->pass + 1;
     hipStream_t st = c->stream;
@@ -1060,9 +1181,10 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     return SPPM_OK;
 }
 
-static int graph_ok(sppm_ctx* c) {
+static int graph_ok(sppm_ctx* c, const sppm_batch* member_of = nullptr) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->batch && c->batch != member_of) return not_in_batch(c, "graph capture");
     if (c->pass != 0 || c->words_gen != 0)
         return fail(SPPM_E_STATE, "capture a pass right after sppm_begin (pass %d done)", c->pass);
     if (c->cam.sampler != SPPM_SAMPLER_UNIFORM)
@@ -1096,18 +1218,6 @@ int sppm_graph_capture(sppm_ctx* c) {
 
 }  // extern "C"
 
-// Several contexts' passes in one graph: one branch per context (two streams
-// each), forked from and joined into the batch's origin stream.
-struct sppm_batch {
-    int device = 0;
-    std::vector<sppm_ctx*> ctx;
-    std::vector<hipStream_t> s1, s2;
-    std::vector<hipEvent_t> join;
-    hipStream_t origin = nullptr;
-    hipEvent_t fork = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
 
 extern "C" {
 
@@ -1120,6 +1230,7 @@ int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tu
     if (!c || !scene || !out) return fail(SPPM_E_ARG, "NULL argument");
     if (passes < 1) return fail(SPPM_E_ARG, "passes < 1");
     if (!c->have_cam || !c->have_params) return fail(SPPM_E_STATE, "camera and params must be set before tuning");
+    if (int rc0 = not_in_batch(c, "sppm_autotune")) return rc0;
     c->lead_ok = false;
     memset(out, 0, sizeof *out);
     const int widths[3] = {2, 4, 8};
@@ -1133,7 +1244,10 @@ int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tu
         const int full = c->scene.spill_blocks;
         for (int half = 0; half < 2; ++half, ++k) {
             c->tuned_blocks = half ? full / 2 : 0;
-            if ((rc = sppm_begin(c)) || (rc = sppm_pass(c, nullptr))) break;
+            // warm-up pass, finished on every stream before the timed passes
+            // start (the next photon pass may otherwise begin before e0)
+            if ((rc = sppm_begin(c)) || (rc = sppm_pass(c, nullptr)) || (rc = sppm_sync(c))) break;
+            c->lead_ok = false;
             HIPOK(hipEventRecord(e0, c->stream));
             for (int p = 0; p < passes && !rc; ++p) rc = sppm_pass(c, nullptr);
             if (rc) break;
@@ -1170,7 +1284,8 @@ int sppm_batch_create(sppm_batch** out, int device) {
     sppm_batch* b = new sppm_batch();
     b->device = device;
     if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
         delete b;
         return fail(SPPM_E_HIP, "batch stream / event creation failed");
     }
@@ -1182,14 +1297,17 @@ int sppm_batch_add(sppm_batch* b, sppm_ctx* c) {
     if (!b || !c) return fail(SPPM_E_ARG, "NULL argument");
     if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
     if (c->device != b->device) return fail(SPPM_E_ARG, "context is on device %d, batch on %d", c->device, b->device);
+    if (b->broken) return fail(SPPM_E_STATE, "a member of this batch was destroyed");
     int rc = graph_ok(c);
     if (rc) return rc;
+    if (c->graph_exec) return fail(SPPM_E_STATE, "the context has its own captured pass");
     hipStream_t x = nullptr, y = nullptr;
     hipEvent_t j = nullptr;
     HIPOK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
     HIPOK(hipStreamCreateWithFlags(&y, hipStreamNonBlocking));
     HIPOK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
     b->ctx.push_back(c);
+    c->batch = b;
     b->s1.push_back(x);
     b->s2.push_back(y);
     b->join.push_back(j);
@@ -1200,8 +1318,9 @@ int sppm_batch_capture(sppm_batch* b) {
     if (!b || b->ctx.empty()) return fail(SPPM_E_ARG, "empty batch");
     if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
     HIPOK(hipSetDevice(b->device));
+    if (b->broken) return fail(SPPM_E_STATE, "a member of this batch was destroyed");
     for (sppm_ctx* c : b->ctx) {
-        int rc = graph_ok(c);
+        int rc = graph_ok(c, b);
         if (rc) return rc;
         HIPOK(hipStreamSynchronize(c->stream));
     }
@@ -1233,8 +1352,11 @@ int sppm_batch_capture(sppm_batch* b) {
 
 int sppm_batch_pass(sppm_batch* b) {
     if (!b) return fail(SPPM_E_ARG, "NULL batch");
+    if (b->broken) return fail(SPPM_E_STATE, "a member of this batch was destroyed");
     if (!b->exec) return fail(SPPM_E_STATE, "batch not captured");
     HIPOK(hipGraphLaunch(b->exec, b->origin));
+    HIPOK(hipEventRecord(b->done, b->origin));
+    b->launched = true;
     for (sppm_ctx* c : b->ctx) c->pass += 1;
     return SPPM_OK;
 }
@@ -1252,11 +1374,13 @@ int sppm_batch_destroy(sppm_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (size_t k = 0; k < b->ctx.size(); ++k) {
+        if (b->ctx[k] && b->ctx[k]->batch == b) b->ctx[k]->batch = nullptr;
         (void)hipStreamDestroy(b->s1[k]);
         (void)hipStreamDestroy(b->s2[k]);
         (void)hipEventDestroy(b->join[k]);
     }
     (void)hipEventDestroy(b->fork);
+    (void)hipEventDestroy(b->done);
     (void)hipStreamDestroy(b->origin);
     delete b;
     return SPPM_OK;
@@ -1290,6 +1414,7 @@ int sppm_ray_counts(sppm_ctx* c, int64_t* photon_rays, int64_t* eye_rays) {
 
 int sppm_sync(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (int rc = batch_wait(c)) return rc;
     HIPOK(hipStreamSynchronize(c->stream3));
     HIPOK(hipStreamSynchronize(c->stream2));
     HIPOK(hipStreamSynchronize(c->stream));
@@ -1311,7 +1436,30 @@ struct CkptHeader {
     int64_t cam_L;
     int32_t guided, nparams, pass;
     int64_t words_gen, words_used;
+    uint64_t setup;  // fingerprint of scene, camera, parameters and emission setup
 };
+// What decides the render besides the saved state: the scene, the camera,
+// alpha / r0 / photons / eye NEE, and the guided-emission settings (the
+// weights themselves are saved).  BVH width and recording do not change it.
+uint64_t setup_fingerprint(const sppm_ctx* c) {
+    uint64_t h = c->scene_hash;
+    h = fnv1a(h, &c->cam, sizeof c->cam);
+    h = fnv1a(h, &c->params.alpha, sizeof(float));
+    h = fnv1a(h, &c->params.first_radius, sizeof(float));
+    h = fnv1a(h, &c->params.photons_per_pass, sizeof(int32_t));
+    h = fnv1a(h, &c->params.eye_nee, sizeof(int32_t));
+    const int32_t g = c->guided ? 1 : 0;
+    h = fnv1a(h, &g, sizeof g);
+    if (c->guided) {
+        const sppm_guide_params& q = c->gparams;
+        h = fnv1a(h, &q.mix_plain, sizeof(float));
+        h = fnv1a(h, &q.learning_rate, sizeof(float));
+        h = fnv1a(h, &q.train_steps, sizeof(int32_t));
+        h = fnv1a(h, &q.history_decay, sizeof(float));
+        h = fnv1a(h, &q.hidden, 3 * sizeof(int32_t));  // hidden, layers, fp16
+    }
+    return h;
+}
 struct CkptSection {
     void* dev;
     size_t bytes;
@@ -1342,7 +1490,8 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
 static CkptHeader ckpt_header(sppm_ctx* c) {
     CkptHeader h{};
     memcpy(h.magic, "SPPMCKPT", 8);
-    h.version = 1;
+    h.version = 2;
+    h.setup = setup_fingerprint(c);
     h.N = c->rb.N; h.W = c->rb.W; h.H = c->rb.H; h.np = c->rb.np; h.nslot = NSLOT;
     h.cam_K = c->rb.cam_K; h.cam_L = c->rb.cam_L;
     h.guided = c->guided ? 1 : 0;
@@ -1388,11 +1537,14 @@ int sppm_checkpoint_load(sppm_ctx* c, const void* buf, int64_t bytes) {
     CkptHeader h;
     memcpy(&h, buf, sizeof h);
     const CkptHeader mine = ckpt_header(c);
-    if (memcmp(h.magic, "SPPMCKPT", 8) || h.version != 1) return fail(SPPM_E_ARG, "not a checkpoint of this format");
+    if (memcmp(h.magic, "SPPMCKPT", 8) || h.version != 2) return fail(SPPM_E_ARG, "not a checkpoint of this format");
     if (h.N != mine.N || h.W != mine.W || h.H != mine.H || h.np != mine.np || h.nslot != mine.nslot ||
         h.cam_K != mine.cam_K || h.cam_L != mine.cam_L || h.guided != mine.guided || h.nparams != mine.nparams)
         return fail(SPPM_E_STATE, "checkpoint was taken with another resolution, photon count, camera sampler or "
                     "emission setup");
+    if (h.setup != mine.setup)
+        return fail(SPPM_E_STATE, "checkpoint was taken with another scene, camera, alpha / radius / NEE setting or "
+                    "guided-emission setting");
     if (bytes < need) return fail(SPPM_E_ARG, "checkpoint buffer: %lld bytes needed", (long long)need);
     HIPOK(hipStreamSynchronize(c->stream));
     HIPOK(hipStreamSynchronize(c->stream2));
@@ -1570,6 +1722,28 @@ int sppm_read_neighbours(sppm_ctx* c, int64_t cap, int64_t* offsets, int32_t* id
     RenderBuffers& R = c->rb;
     if (!R.have_lists) return fail(SPPM_E_STATE, "no neighbour lists (set params.record_neighbours and run a gather)");
     const int np = R.np;
+    if (R.pairs) {
+        // pairs of the production gather: grouped by hit point, sorted
+        unsigned long long n = 0;
+        HIPOK(hipStreamSynchronize(c->stream));
+        HIPOK(hipMemcpy(&n, R.npairs, sizeof n, hipMemcpyDeviceToHost));
+        if ((long long)n > R.pairs_cap)
+            return fail(SPPM_E_OVERFLOW, "%llu neighbour pairs, more than max_neighbours %lld", n, R.pairs_cap);
+        if (total) *total = (int64_t)n;
+        if (!offsets) return SPPM_OK;
+        if ((long long)n > cap) return fail(SPPM_E_ARG, "capacity %lld < %llu neighbours", (long long)cap, n);
+        std::vector<int2> pr(n);
+        if (n) HIPOK(hipMemcpy(pr.data(), R.pairs, sizeof(int2) * n, hipMemcpyDeviceToHost));
+        std::sort(pr.begin(), pr.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+        std::fill(offsets, offsets + np + 1, (int64_t)0);
+        for (size_t k = 0; k < n; ++k) {
+            if (pr[k].x < 0 || pr[k].x >= np) return fail(SPPM_E_STATE, "recorded pair with hit point %d", pr[k].x);
+            offsets[pr[k].x + 1] += 1;
+            idx[k] = pr[k].y;
+        }
+        for (int i = 0; i < np; ++i) offsets[i + 1] += offsets[i];
+        return SPPM_OK;
+    }
     std::vector<int> cnt(np);
     HIPOK(hipStreamSynchronize(c->stream));
     HIPOK(hipMemcpy(cnt.data(), R.counts, sizeof(int) * np, hipMemcpyDeviceToHost));
@@ -1739,7 +1913,13 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
     c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (int rc0 = not_in_batch(c, "sppm_guide_set")) return rc0;
     HIPOK(hipSetDevice(c->device));
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipStreamSynchronize(c->stream2));
+    HIPOK(hipStreamSynchronize(c->stream3));
+    // a captured pass holds the proposal's buffers and the emission mode
+    drop_graph(c);
     if (!p->enable) {
         c->guided = false;
         return SPPM_OK;
@@ -1747,7 +1927,8 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!(p->mix_plain >= 0.25f && p->mix_plain <= 1.0f)) return fail(SPPM_E_ARG, "mix_plain must be in [0.25, 1]");
     if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
     if (!(p->history_decay >= 0.0f && p->history_decay < 1.0f)) return fail(SPPM_E_ARG, "history_decay must be in [0, 1)");
-    if (c->params.record_neighbours) return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0)");
+    if (c->params.record_neighbours == 1)
+        return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0 or 2)");
     GuideNet net;
     net.hidden = p->hidden ? p->hidden : 64;
     net.layers = p->layers ? p->layers : 4;
@@ -1800,7 +1981,7 @@ static int guide_ready(sppm_ctx* c) {
     c->lead_ok = false;
     if (!c->guide.theta) return fail(SPPM_E_STATE, "guided emission not set up (sppm_guide_set)");
     HIPOK(hipSetDevice(c->device));
-    return SPPM_OK;
+    return batch_wait(c);
 }
 
 int sppm_guide_weights(sppm_ctx* c, float* out, const float* in) {
@@ -1846,6 +2027,15 @@ int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
     return SPPM_OK;
 }
 
+int sppm_guide_tables(sppm_ctx* c, float* prob, float* cdf) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    HIPOK(hipStreamSynchronize(c->stream));
+    if (prob) HIPOK(hipMemcpy(prob, c->guide.prob, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    if (cdf) HIPOK(hipMemcpy(cdf, c->guide.cdf, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    return SPPM_OK;
+}
+
 int sppm_guide_read(sppm_ctx* c, float* prob, float* hist) {
     int rc = guide_ready(c);
     if (rc) return rc;
@@ -1860,6 +2050,7 @@ int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* di
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
     c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (int rc0 = batch_wait(c)) return rc0;  // the batch's passes use the traversal spill area
     if (n <= 0) return SPPM_OK;
     if (!org || !dir || !facet) return fail(SPPM_E_ARG, "NULL argument");
     HIPOK(hipSetDevice(c->device));
@@ -1901,6 +2092,7 @@ int sppm_trace_occluded(sppm_ctx* c, int32_t n, const float* org, const float* d
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
     c->lead_ok = false;
     if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (int rc0 = batch_wait(c)) return rc0;  // the batch's passes use the traversal spill area
     if (n <= 0) return SPPM_OK;
     if (!org || !dir || !tmax || !occluded) return fail(SPPM_E_ARG, "NULL argument");
     HIPOK(hipSetDevice(c->device));

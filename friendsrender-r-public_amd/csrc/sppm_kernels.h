@@ -8,8 +8,9 @@
 namespace sppm {
 
 // ---- sort / scan (sppm_sort.hip) ------------------------------------------
+// stable; n_dev (optional): sort only the first *n_dev <= n pairs
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* hist, int n,
-                      int bits, hipStream_t st);
+                      int bits, hipStream_t st, const int* n_dev = nullptr);
 size_t radix_hist_entries(int n);
 __global__ void k_tile_total_prep(int* tmp, int nt);
 __global__ void k_scan_small(uint32_t* data, int n);

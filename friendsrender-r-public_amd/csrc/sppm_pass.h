@@ -66,10 +66,17 @@ void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, 
 void launch_occluded(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const float* tmax,
                      float transl,
                      const TraceSpill& sp, int* occluded, uint32_t* flags, hipStream_t st);
+// Sort buffers of the grid build, photon_cap entries each (hist:
+// radix_hist_entries(photon_cap)): cell key, flattened index and slot
+// address of every flattened photon.
+struct PhotonSortBuffers {
+    uint32_t *keys, *vals, *tmp_k, *tmp_v, *src, *hist;
+};
+constexpr int PHOTON_KEY_BITS = 22;  // cell keys are below the cell capacity 2^22
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
-                       int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
-                       hipStream_t st);
+                       int* cell_start, int* scan_tmp, const PhotonSortBuffers& sb, int photon_cap, float4* gpos,
+                       float4* gflux, float4* gdir, hipStream_t st);
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,
                         hipStream_t st);
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
@@ -82,11 +89,19 @@ struct GroupBuffers {
     int* nvalid;  // [0] valid hit points, [1] chunk queue of the gather
     unsigned long long* stat;  // 3 per group: radius sum, active hit points, neighbours
 };
+// Neighbour pairs (hit point, flattened photon index) that the production
+// gather appends when recording (parity tests): at most cap, *count may run
+// past cap (the overflow is reported by the reader).
+struct PairRecord {
+    int2* pairs;
+    unsigned long long* count;
+    long long cap;
+};
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
                             int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
-                            hipStream_t st);
+                            const PairRecord& rec, hipStream_t st);
 int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();

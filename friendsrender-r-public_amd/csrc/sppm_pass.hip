@@ -533,10 +533,12 @@ __global__ void k_zero_i32(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// per path: key + rank of each flattened photon
+// Per path: the cell key, flattened index and slot address of each of its
+// flattened photons, and the cell counts (atomic adds; the totals do not
+// depend on their order).
 __global__ void __launch_bounds__(256) k_photon_keys(SlotsDev slots, const int* __restrict__ off,
                                                      const GridParams* __restrict__ gp, int* __restrict__ cell_count,
-                                                     int2* __restrict__ keyrank) {
+                                                     PhotonSortBuffers sb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots.npaths) return;
     const GridParams g = *gp;
@@ -544,34 +546,35 @@ __global__ void __launch_bounds__(256) k_photon_keys(SlotsDev slots, const int* 
     const int base = off[i];
     const size_t NP = (size_t)slots.npaths;
     for (int s = 0; s < c; ++s) {
-        float4 p = slots.pos[(size_t)s * NP + i];
+        const size_t o = (size_t)s * NP + i;
+        float4 p = slots.pos[o];
         int cx = cell_coord(p.x, g.bmin[0], g.inv_cell, g.nx);
         int cy = cell_coord(p.y, g.bmin[1], g.inv_cell, g.ny);
         int cz = cell_coord(p.z, g.bmin[2], g.inv_cell, g.nz);
         int key = (cz * g.ny + cy) * g.nx + cx;
-        int rank = atomicAdd(&cell_count[key], 1);
-        keyrank[base + s] = make_int2(key, rank);
+        atomicAdd(&cell_count[key], 1);
+        sb.keys[base + s] = (uint32_t)key;
+        sb.vals[base + s] = (uint32_t)(base + s);
+        sb.src[base + s] = (uint32_t)o;
     }
 }
 
-__global__ void __launch_bounds__(256) k_photon_scatter(SlotsDev slots, const int* __restrict__ off,
-                                                        const int* __restrict__ cell_start,
-                                                        const int2* __restrict__ keyrank, float4* __restrict__ gpos,
-                                                        float4* __restrict__ gflux, float4* __restrict__ gdir) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slots.npaths) return;
-    const int c = slots.count[i];
-    const int base = off[i];
-    const size_t NP = (size_t)slots.npaths;
-    for (int s = 0; s < c; ++s) {
-        const size_t o = (size_t)s * NP + i;
-        int2 kr = keyrank[base + s];
-        int dst = cell_start[kr.x] + kr.y;
+// Cell-sorted photon records: after the stable sort by cell key, position j
+// holds flattened photon vals[j], so photons of a cell are in flattened
+// (reference) order and every run lays them out the same way.  The record
+// carries its flattened index in dir.w.
+__global__ void __launch_bounds__(256) k_photon_sorted(SlotsDev slots, const int* __restrict__ nstored,
+                                                       PhotonSortBuffers sb, float4* __restrict__ gpos,
+                                                       float4* __restrict__ gflux, float4* __restrict__ gdir) {
+    const int n = *nstored;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t flat = sb.vals[j];
+        const size_t o = sb.src[flat];
         float4 d = slots.dir[o];
-        d.w = __int_as_float(base + s);
-        gpos[dst] = slots.pos[o];
-        gflux[dst] = slots.flux[o];
-        gdir[dst] = d;
+        d.w = __int_as_float((int)flat);
+        gpos[j] = slots.pos[o];
+        gflux[j] = slots.flux[o];
+        gdir[j] = d;
     }
 }
 
@@ -1269,7 +1272,7 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       const TileRange* __restrict__ tiles, int ntiles,
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
                                                       int* __restrict__ queue, float4* __restrict__ partial,
-                                                      float* __restrict__ pdep) {
+                                                      float* __restrict__ pdep, PairRecord rec) {
     __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
     __shared__ unsigned char s_hit[64 * 64];
     __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
@@ -1361,6 +1364,22 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                     }
                 }
                 L.M += nh;
+                if (rec.pairs) {
+                    // recording (parity tests): this lane's neighbours among the staged
+                    // photons, appended at a wave-wide prefix of the lanes' counts
+                    int incl = nh;
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int v = __shfl_up(incl, off, 64);
+                        if (lane >= off) incl += v;
+                    }
+                    const int tot = __shfl(incl, 63, 64);
+                    unsigned long long base = 0;
+                    if (lane == 0 && tot > 0) base = atomicAdd(rec.count, (unsigned long long)tot);
+                    base = __shfl(base, 0, 64) + (unsigned long long)(incl - nh);
+                    for (int e = 0; e < nh; ++e)
+                        if ((long long)(base + e) < rec.cap)
+                            rec.pairs[base + e] = make_int2(p, __float_as_int(s_dir[s_hit[e * 64 + lane]].w));
+                }
                 const float on = dot(L.outd, L.N);
                 for (int e = 0; e < nh; ++e) {
                     const int j = s_hit[e * 64 + lane];
@@ -1677,17 +1696,19 @@ void launch_occluded(const SceneDev& S, int n, const float* org, const float* di
 
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
-                       int* cell_start, int* scan_tmp, int2* keyrank, float4* gpos, float4* gflux, float4* gdir,
-                       hipStream_t st) {
+                       int* cell_start, int* scan_tmp, const PhotonSortBuffers& sb, int photon_cap, float4* gpos,
+                       float4* gflux, float4* gdir, hipStream_t st) {
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rstat, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], cell_cap, nstored, ctr);
     hipLaunchKernelGGL(k_zero_i32, dim3((cell_cap + 1 + 255) / 256), dim3(256), 0, st, cell_count, cell_cap + 1);
     int gP = (slots.npaths + 255) / 256;
-    hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, cell_count, keyrank);
+    hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, cell_count, sb);
     // cell_start = exclusive scan over cell_cap+1 counts (unused cells are zero)
     exclusive_scan(cell_count, cell_start, cell_cap + 1, nullptr, scan_tmp, st);
-    hipLaunchKernelGGL(k_photon_scatter, dim3(gP), dim3(256), 0, st, slots, off, cell_start, keyrank, gpos, gflux,
-                       gdir);
+    // stable sort of the flattened photons by cell: cell keys are below
+    // cell_cap = 2^22, the flattened order is kept inside a cell
+    radix_sort_pairs(sb.keys, sb.vals, sb.tmp_k, sb.tmp_v, sb.hist, photon_cap, PHOTON_KEY_BITS, st, nstored);
+    hipLaunchKernelGGL(k_photon_sorted, dim3(2048), dim3(256), 0, st, slots, nstored, sb, gpos, gflux, gdir);
 }
 
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,
@@ -1708,7 +1729,7 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
                             int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
-                            hipStream_t st) {
+                            const PairRecord& rec, hipStream_t st) {
     const int ntiles = gather_group_count(np);
     // hit-point groups: valid hit points in Morton order, 64 per group
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 2);  // valid count, chunk queue
@@ -1722,8 +1743,9 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp, ctr);
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
+    if (rec.pairs) launch_set_u64x2(rec.count, 0ull, 0ull, st);
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
-                       ntiles, chunk_off, chunkp, gb.nvalid + 1, partial, pdep);
+                       ntiles, chunk_off, chunkp, gb.nvalid + 1, partial, pdep, rec);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
                        partial, counts, gb.stat, flags);
     hipLaunchKernelGGL(k_group_stats, dim3(1), dim3(1024), 0, st, gb.stat, ntiles, rstat, ctr);

@@ -19,8 +19,18 @@ constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS; // 4096
 constexpr int RADIX_BITS = 4;
 constexpr int RADIX = 1 << RADIX_BITS;
 
-__global__ void __launch_bounds__(SORT_THREADS) k_radix_count(const uint32_t* __restrict__ keys, int n, int shift,
+// n_dev (optional): the element count on the device, at most n (the grid is
+// sized for n; tiles past the count write empty histograms)
+__device__ __forceinline__ int sort_count(int n, const int* n_dev) {
+    if (!n_dev) return n;
+    const int m = *n_dev;
+    return m < n ? m : n;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_radix_count(const uint32_t* __restrict__ keys, int n,
+                                                              const int* __restrict__ n_dev, int shift,
                                                               int nblocks, uint32_t* __restrict__ hist) {
+    n = sort_count(n, n_dev);
     __shared__ uint32_t h[RADIX];
     if (threadIdx.x < RADIX) h[threadIdx.x] = 0;
     __syncthreads();
@@ -61,8 +71,11 @@ __global__ void __launch_bounds__(1024) k_scan_small(uint32_t* __restrict__ data
 __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
-                                                                uint32_t* __restrict__ vout, int n, int shift,
+                                                                uint32_t* __restrict__ vout, int n,
+                                                                const int* __restrict__ n_dev, int shift,
                                                                 int nblocks, const uint32_t* __restrict__ hist) {
+    n = sort_count(n, n_dev);
+    if ((int)blockIdx.x * SORT_TILE >= n) return;  // uniform: the whole tile is past the count
     __shared__ uint32_t base[RADIX];
     __shared__ uint32_t wcnt[SORT_WAVES][RADIX];
     const int lane = threadIdx.x & 63;
@@ -108,11 +121,12 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* 
     }
 }
 
-// Sorts n pairs by the low `bits` bits of the key.  Buffers tmp_k/tmp_v have
+// Sorts n pairs (or the first *n_dev of them, *n_dev <= n, when n_dev is
+// given) by the low `bits` bits of the key, stably.  Buffers tmp_k/tmp_v have
 // n entries; hist has RADIX * ceil(n / SORT_TILE) entries.  Result ends in
 // keys/vals.
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* hist, int n,
-                      int bits, hipStream_t st) {
+                      int bits, hipStream_t st, const int* n_dev) {
     if (n <= 1) return;
     int nblocks = (n + SORT_TILE - 1) / SORT_TILE;
     int passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
@@ -121,10 +135,11 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t*
     for (int p = 0; p < passes; ++p) {
         int shift = p * RADIX_BITS;
         if (shift >= 32) shift = 28;  // extra pass on a high digit keeps order (stable)
-        hipLaunchKernelGGL(k_radix_count, dim3(nblocks), dim3(SORT_THREADS), 0, st, ka, n, shift, nblocks, hist);
+        hipLaunchKernelGGL(k_radix_count, dim3(nblocks), dim3(SORT_THREADS), 0, st, ka, n, n_dev, shift, nblocks,
+                           hist);
         hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, st, hist, RADIX * nblocks);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, st, ka, va, kb, vb, n, shift,
-                           nblocks, hist);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, st, ka, va, kb, vb, n, n_dev,
+                           shift, nblocks, hist);
         uint32_t* t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }

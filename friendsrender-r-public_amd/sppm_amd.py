@@ -32,7 +32,7 @@ EXPORTED = [
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
-    "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts",
+    "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables",
 ]
 
 
@@ -242,11 +242,17 @@ def camera_desc(cam):
     return c
 
 
-def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False, nee=0):
+RECORD_OFF, RECORD_LISTS, RECORD_PAIRS = 0, 1, 2  # sppm_params.record_neighbours
+
+
+def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False, nee=0,
+                  max_pairs=0):
+    """record: False/0 none, True/1 lists from the recording gather kernel, 2
+    pairs appended by the production gather (at most max_pairs per pass)."""
     p = Params()
     p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
     p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)
-    p.record_neighbours, p.max_neighbours = int(record), 0
+    p.record_neighbours, p.max_neighbours = int(record), int(max_pairs)
     p.eye_nee = nee
     return p
 
@@ -254,7 +260,8 @@ def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check
 class Renderer:
     """One SPPM render on one GPU: scene + camera + params, then passes."""
 
-    def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0, nee=NEE_OFF):
+    def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0, nee=NEE_OFF,
+                 max_pairs=0):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_create(C.byref(h), device, None))
@@ -265,8 +272,20 @@ class Renderer:
         _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
         self._cam = camera_desc(cam)
         _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
-        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee)
+        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs)
         _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
+        _check(self.lib.sppm_begin(self.h))
+        self.passes = 0
+        self.total_photons = 0
+
+    def upload_scene(self, scene, bvh_width=8):
+        """Scene::CreateScene again: the render must be begun again (begin())."""
+        d, keep = scene_desc(scene)
+        _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
+        self._keep = keep
+
+    def begin(self):
+        """initializeSharedPoints: a new render with the current setup."""
         _check(self.lib.sppm_begin(self.h))
         self.passes = 0
         self.total_photons = 0
@@ -367,6 +386,13 @@ class Renderer:
         hist = np.zeros(GUIDE_CELLS, np.float32)
         _check(self.lib.sppm_guide_read(self.h, _fp(prob), _fp(hist)))
         return prob, hist
+
+    def guide_tables(self):
+        """(prob, cdf) the next guided photon pass samples."""
+        prob = np.zeros(GUIDE_CELLS, np.float32)
+        cdf = np.zeros(GUIDE_CELLS, np.float32)
+        _check(self.lib.sppm_guide_tables(self.h, _fp(prob), _fp(cdf)))
+        return prob, cdf
 
     def checkpoint(self):
         """Bytes of a checkpoint of the render state (see sppm_checkpoint_save)."""

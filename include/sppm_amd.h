@@ -133,8 +133,10 @@ typedef struct sppm_params {
     int32_t bvh_width;         /* 2, 4 or 8 (closest hit is width-independent) */
     int32_t block_size;        /* threads per block for path kernels (0 = default) */
     int32_t check_numeric;     /* 1: fail a pass with SPPM_E_NUMERIC on NaN/Inf flux */
-    int32_t record_neighbours; /* 1: keep per-hit-point neighbour lists (debug/parity) */
-    int32_t max_neighbours;    /* capacity of the neighbour list buffer (entries) */
+    int32_t record_neighbours; /* 1: neighbour lists from a separate recording gather kernel;
+                                  2: the production gather itself appends its (hit point,
+                                  photon) pairs, at most max_neighbours per pass (debug/parity) */
+    int32_t max_neighbours;    /* capacity of the pair buffer in mode 2 (entries) */
     int32_t eye_nee;           /* SPPM_NEE_*: next-event estimation on eye paths */
 } sppm_params;
 
@@ -321,6 +323,9 @@ int sppm_guide_gradient(sppm_ctx* ctx, const float* hist, float* loss, float* gr
 int sppm_guide_train(sppm_ctx* ctx, const float* hist, int steps);
 /* Current proposal probabilities per cell and the training histogram (either may be NULL). */
 int sppm_guide_read(sppm_ctx* ctx, float* prob, float* hist);
+/* The tables the next guided photon pass samples: the proposal per cell
+ * (unnormalised) and its inclusive prefix sum (either may be NULL). */
+int sppm_guide_tables(sppm_ctx* ctx, float* prob, float* cdf);
 
 /* ---- graph-captured passes, batches of scenes, autotuner --------------------
  * No counterpart in the reference (it renders one scene per process,
