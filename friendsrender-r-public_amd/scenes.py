@@ -432,20 +432,28 @@ def displaced_mesh(center, r, freq, seed, amp=0.05):
     """Vectorised geodesic sphere: 20 * freq^2 triangles, radially displaced by
     3-octave value noise on a 16^3 lattice (a function of the direction, so
     shared vertices stay shared).  Per-facet vertex arrays (V, N, UV), face
-    normals as vertex normals."""
+    normals as vertex normals.  The noise is evaluated once per lattice point
+    (i, j), i + j <= freq, of each icosahedron face, and the facets gather
+    their corners from those points."""
     v, f = _icosahedron()
     i, j = np.meshgrid(np.arange(freq), np.arange(freq), indexing="ij")
     up = j < freq - i
     dn = j < freq - i - 1
     iu, ju, idn, jdn = i[up], j[up], i[dn], j[dn]
-    # barycentric (i/f, j/f) corners of the up and down triangles
+    # integer corners (i, j) of the up and down triangles
     cu = np.stack([np.stack([iu, ju], -1), np.stack([iu + 1, ju], -1), np.stack([iu, ju + 1], -1)], 1)
     cd = np.stack([np.stack([idn + 1, jdn], -1), np.stack([idn + 1, jdn + 1], -1), np.stack([idn, jdn + 1], -1)], 1)
-    cor = np.concatenate([cu, cd], 0).astype(np.float64) / freq  # (T, 3, 2)
+    corners = np.concatenate([cu, cd], 0)  # (T, 3, 2)
+    gi, gj = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    keep = gi + gj <= freq
+    gi, gj = gi[keep], gj[keep]
+    lut = np.full((freq + 1, freq + 1), -1, np.int64)
+    lut[gi, gj] = np.arange(len(gi))
+    tri = lut[corners[..., 0], corners[..., 1]]  # (T, 3) lattice point of each corner
+    bary = np.stack([gi, gj], -1).astype(np.float64) / freq  # (K, 2)
     A, B, Cc = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]  # (20, 3)
-    P = (A[:, None, None, :] + (B - A)[:, None, None, :] * cor[None, ..., 0:1] +
-         (Cc - A)[:, None, None, :] * cor[None, ..., 1:2]).reshape(-1, 3, 3)
-    P /= np.linalg.norm(P, axis=-1, keepdims=True)
+    P = (A[:, None, :] + (B - A)[:, None, :] * bary[None, :, 0:1] + (Cc - A)[:, None, :] * bary[None, :, 1:2])
+    P /= np.linalg.norm(P, axis=-1, keepdims=True)  # (20, K, 3)
     rng = np.random.default_rng(seed)
     offs = [rng.random(3) * 10 for _ in range(3)]
     lat = rng.random((3, 16, 16, 16))
@@ -472,14 +480,16 @@ def displaced_mesh(center, r, freq, seed, amp=0.05):
         tot += amp_
         amp_ *= 0.5
     disp = 1.0 + amp / r * (val / tot - 0.5) * 2.0
-    V = (flat * (r * disp)[:, None] + np.asarray(center)).reshape(-1, 3, 3)
+    Vu = (flat * (r * disp)[:, None] + np.asarray(center)).reshape(20, -1, 3)
+    uvu = np.zeros_like(P)
+    uvu[..., 0] = 0.5 + np.arctan2(P[..., 2], P[..., 0]) / (2 * np.pi)
+    uvu[..., 1] = 0.5 - np.arcsin(np.clip(P[..., 1], -1, 1)) / np.pi
+    V = Vu[:, tri].reshape(-1, 3, 3)
+    uv = uvu[:, tri].reshape(-1, 3, 3)
     e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
     fn = np.cross(e1, e2)
     fn /= np.linalg.norm(fn, axis=1, keepdims=True) + 1e-30
     N = np.repeat(fn[:, None, :], 3, axis=1)
-    uv = np.zeros_like(V)
-    uv[..., 0] = 0.5 + np.arctan2(P[..., 2], P[..., 0]) / (2 * np.pi)
-    uv[..., 1] = 0.5 - np.arcsin(np.clip(P[..., 1], -1, 1)) / np.pi
     return V.astype(np.float32), N.astype(np.float32), uv.astype(np.float32)
 
 
@@ -513,6 +523,7 @@ def batch_scene(seed, tri_range=(10_000, 500_000), max_meshes=8):
     b = _Builder()
     box_walls(b, 0, 1, 2)
     light_quad(b, 3)
+    parts = [(np.array(b.pos), np.array(b.nrm), np.array(b.uv), np.array(b.attr, np.int32))]
     nmesh = int(rng.integers(1, max_meshes + 1))
     for k in range(nmesh):
         ntri = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
@@ -521,11 +532,8 @@ def batch_scene(seed, tri_range=(10_000, 500_000), max_meshes=8):
         c = (float(rng.uniform(0.2, 0.8)), float(rng.uniform(r, 0.6)), float(rng.uniform(0.2, 0.8)))
         attrs.append(random_material(rng, f"mesh{k}"))
         V, N, T = displaced_mesh(c, r, freq, seed * 31 + k, amp=0.2 * r)
-        b.pos.extend(V.reshape(-1, 9))
-        b.nrm.extend(N.reshape(-1, 9))
-        b.uv.extend(T.reshape(-1, 9))
-        b.attr.extend([len(attrs) - 1] * len(V))
-    return b.scene(attrs)
+        parts.append((V.reshape(-1, 9), N.reshape(-1, 9), T.reshape(-1, 9), np.full(len(V), len(attrs) - 1, np.int32)))
+    return Scene(*(np.concatenate([p[k] for p in parts]) for k in range(4)), attrs, [])
 
 
 def batch_workload(n=64, seed=2024, tri_range=(10_000, 500_000), max_meshes=8, resolutions=(256, 512, 1024),

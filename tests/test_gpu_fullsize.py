@@ -10,6 +10,7 @@ import pytest
 import pyoracle
 import scenes
 import sppm_amd
+from gpu_compare import spot_check
 
 pytestmark = pytest.mark.gpu
 
@@ -19,35 +20,6 @@ def workload():
     scene = scenes.full_scene(freq=100, tex_size=512)
     cam = scenes.thinlens_camera(1024, 1024)
     return scene, cam, pyoracle.OracleScene(scene)
-
-
-def spot_check(gpu, orender, step, npix, npaths, seed, extra_pixels=()):
-    rng = np.random.default_rng(seed)
-    NP = gpu.W * gpu.H
-    pix = np.concatenate([rng.choice(NP, npix, replace=False), np.asarray(extra_pixels, np.int64)]).astype(np.int64)
-    smp = pyoracle.uniform_samples_at(0, (step - 1) * NP + pix)
-    ref = orender.eye_pixels(step, pix, smp)
-    hg = gpu.read_hitpoints()
-    np.testing.assert_array_equal(hg["type"][pix], ref["type"])
-    np.testing.assert_array_equal(hg["facet"][pix], ref["facet"])
-    hit = ref["type"] == 1
-    for k in ("pos", "normal", "outdir"):
-        np.testing.assert_array_equal(hg[k][pix][hit], ref[k][hit])
-    np.testing.assert_allclose(hg["weight"][pix][hit], ref["weight"][hit], rtol=1e-5, equal_nan=True)
-    counts = gpu.read_path_counts()
-    off = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
-    ph = gpu.read_photons()
-    for p in rng.choice(gpu.N, npaths, replace=False):
-        r = orender.photon_path(step, p)
-        n = len(r["pos"])
-        assert counts[p] in (n, n + 1) if n > 1 or r["index"].size == 0 else counts[p] >= 1
-        sl = slice(off[p], off[p] + n)
-        np.testing.assert_array_equal(ph["pos"][sl], r["pos"])
-        np.testing.assert_array_equal(ph["indir"][sl], r["indir"])
-        np.testing.assert_array_equal(ph["index"][sl], r["index"])
-        np.testing.assert_allclose(ph["flux"][sl], r["flux"], rtol=1e-5)
-        np.testing.assert_allclose(ph["pdf"][sl], r["pdf"], rtol=1e-5)
-    return ref
 
 
 def test_full_workload_spot_parity(workload):

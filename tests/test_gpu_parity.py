@@ -13,21 +13,9 @@ import pytest
 import pyoracle
 import scenes
 import sppm_amd
+from gpu_compare import REL_TOL, compare_pass, rel_rmse, say
 
 pytestmark = pytest.mark.gpu
-
-REL_TOL = 1e-4
-
-
-def rel_rmse(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    den = np.sqrt(np.mean(b * b))
-    return float(np.sqrt(np.mean((a - b) ** 2)) / max(den, 1e-30))
-
-
-def say(msg):
-    print(msg, flush=True)
 
 
 def random_rays(scene, n, seed):
@@ -65,52 +53,27 @@ def test_closest_hit_bit_exact(width, which, builder, box_scene, full_scene_smal
     np.testing.assert_array_equal(vg[hit], vo[hit])
 
 
-def compare_pass(gpu, ref, stored_ref, label, record=True):
-    pg = gpu.read_photons()
-    pr = ref.read_photons(stored_ref)
-    assert pg["pos"].shape[0] == stored_ref, f"{label}: stored photons {pg['pos'].shape[0]} != {stored_ref}"
-    np.testing.assert_array_equal(pg["pos"], pr["pos"])
-    np.testing.assert_array_equal(pg["indir"], pr["indir"])
-    np.testing.assert_array_equal(pg["index"], pr["index"])
-    assert rel_rmse(pg["flux"], pr["flux"]) <= REL_TOL
-    assert rel_rmse(pg["pdf"], pr["pdf"]) <= REL_TOL
-    hg, hr = gpu.read_hitpoints(), ref.read_hitpoints()
-    np.testing.assert_array_equal(hg["type"], hr["type"])
-    np.testing.assert_array_equal(hg["facet"], hr["facet"])
-    hit = hr["type"] == 1
-    np.testing.assert_array_equal(hg["pos"][hit], hr["pos"][hit])
-    np.testing.assert_array_equal(hg["normal"][hit], hr["normal"][hit])
-    np.testing.assert_array_equal(hg["outdir"][hit], hr["outdir"][hit])
-    assert rel_rmse(hg["weight"][hit], hr["weight"][hit]) <= REL_TOL
-    np.testing.assert_array_equal(gpu.read_counts(), ref.read_counts())
-    if record:
-        og, ig = gpu.read_neighbours()
-        orf, irf = ref.read_neighbours()
-        np.testing.assert_array_equal(og, orf)
-        np.testing.assert_array_equal(ig, irf)
-    # DirectLight (environment light): hit points and shared sums, counts exact
-    dg, dr = gpu.read_direct(), ref.read_direct()
-    np.testing.assert_array_equal(dg["num"], dr["num"])
-    assert rel_rmse(dg["hit"], dr["hit"]) <= REL_TOL
-    assert rel_rmse(dg["shared"], dr["shared"]) <= REL_TOL
-    sg, sr = gpu.read_shared(), ref.read_shared()
-    np.testing.assert_array_equal(sg["num_photon"], sr["num_photon"])
-    np.testing.assert_array_equal(sg["num_gather"], sr["num_gather"])
-    assert rel_rmse(sg["radius"], sr["radius"]) <= 1e-6
-    e = rel_rmse(sg["radiance"], sr["radiance"])
-    say(f"  {label}: radiance rel RMSE {e:.2e}")
-    assert e <= REL_TOL, f"{label}: radiance rel RMSE {e}"
-    return e
-
-
-def run_parity(scene, cam, alpha, r0, N, passes, width, label, record=True, nee=sppm_amd.NEE_OFF):
-    gpu = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=width, record=record, nee=nee)
-    ref = pyoracle.OracleRender(pyoracle.OracleScene(scene), cam, alpha, r0, N, nee=nee)
+def run_parity(scene, cam, alpha, r0, N, passes, width, label, record=True, nee=sppm_amd.NEE_OFF, guide=None,
+               max_pairs=0):
+    """record: True lists from the recording gather kernel, RECORD_PAIRS the
+    production gather's own pairs, False none (counts only).  guide: kwargs of
+    Renderer.guide_set; each pass then emits from the proposal the GPU holds
+    before it, which the oracle is given too."""
+    gpu = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=width, record=record, nee=nee, max_pairs=max_pairs)
+    oscene = pyoracle.OracleScene(scene)
+    ref = pyoracle.OracleRender(oscene, cam, alpha, r0, N, nee=nee)
+    og = None
+    if guide is not None:
+        gpu.guide_set(**guide)
+        og = pyoracle.OracleGuide(oscene, guide.get("mix_plain", 0.5))
+        ref.set_guide(og)
     for p in range(passes):
         say(f"{label}: pass {p}")
+        if og is not None:
+            og.set_tables(*gpu.guide_tables())
         st = gpu.pass_()
         assert st.flags == 0, f"device flags {st.flags:#x}"
-        ref.full_pass(record=record)
+        ref.full_pass(record=bool(record))
         stored_ref = pyoracle.orc().orc_flatten(ref.h)
         assert st.photons_stored == stored_ref
         compare_pass(gpu, ref, stored_ref, f"{label} pass {p}", record=record)
@@ -266,3 +229,52 @@ def test_ray_counts_are_consistent(box_scene):
     say(f"rays: photon {c['photon_rays']} eye {c['eye_rays']}")
     assert 3 * N <= c["photon_rays"] <= 3 * N * 33
     assert 3 * W * H <= c["eye_rays"] <= 3 * W * H * 7
+
+
+@pytest.mark.parametrize("which", ["box", "full"])
+def test_production_gather_neighbour_sets(which, box_scene, full_scene_small):
+    """The neighbour index sets of the production gather (k_gather_chunks,
+    the kernel every measured pass runs) against the oracle's, bit-exact,
+    with the update that follows."""
+    if which == "box":
+        r0 = 0.02 * box_scene.diagonal()
+        run_parity(box_scene, scenes.pinhole_camera(96, 64), 0.7, r0, 30000, 3, 4, "pairs box",
+                   record=sppm_amd.RECORD_PAIRS, max_pairs=40_000_000)
+    else:
+        r0 = 0.02 * full_scene_small.diagonal()
+        run_parity(full_scene_small, scenes.thinlens_camera(64, 64), 0.7, r0, 20000, 3, 8, "pairs full",
+                   record=sppm_amd.RECORD_PAIRS, max_pairs=40_000_000)
+
+
+def test_pass_parity_guided_untextured(box_scene):
+    """Guided emission (the proposal mixture, untextured light) against the
+    oracle's restatement of it, each pass sampling the proposal trained by the
+    GPU so far: photon lists bit-exact, PDFs and flux within 1e-4, counts and
+    update exact."""
+    r0 = 0.02 * box_scene.diagonal()
+    run_parity(box_scene, scenes.pinhole_camera(64, 64), 0.7, r0, 20000, 4, 4, "guided box", record=False,
+               guide=dict(mix_plain=0.5, learning_rate=1e-2, train_steps=4))
+
+
+def test_pass_parity_guided_textured(full_scene_small):
+    """The same with the textured light's 1000-point tables (table points
+    picked inside the proposal's position cell)."""
+    r0 = 0.02 * full_scene_small.diagonal()
+    run_parity(full_scene_small, scenes.thinlens_camera(48, 48), 0.7, r0, 15000, 3, 8, "guided full",
+               record=sppm_amd.RECORD_PAIRS, max_pairs=20_000_000,
+               guide=dict(mix_plain=0.25, learning_rate=1e-2, train_steps=4))
+
+
+def test_flux_is_reproducible(full_scene_small):
+    """Photons sit in their grid cells in flattened order (a stable sort by
+    cell), so two renders of the same setup give bit-identical flux."""
+    r0 = 0.02 * full_scene_small.diagonal()
+    out = []
+    for rep in range(2):
+        g = sppm_amd.Renderer(full_scene_small, scenes.thinlens_camera(96, 96), 0.7, r0, 40000, bvh_width=8)
+        for _ in range(4):
+            g.pass_(stats=False)
+        out.append(g.read_shared())
+        g.close()
+    for k in ("radiance", "radius", "num_photon", "num_gather"):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
