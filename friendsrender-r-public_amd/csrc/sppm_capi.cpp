@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -302,6 +303,61 @@ static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 }
 static const uint64_t FNV_BASIS = 0xcbf29ce484222325ull;
 
+// ---- the fixed staging buffer of sppm_trace_closest_static ----------------
+// One array per GPU, `scratch`, holding the rays and hits of one call:
+// origins, directions and t/u/v (3 floats per ray each), facet ids, then the
+// flag word.  In host code the name `scratch` only stands in for the device
+// array; the device address of each GPU's copy comes from
+// hipGetSymbolAddress, resolved once per device (sppm_create), and host code
+// uses only that address.
+constexpr size_t SCRATCH_RAYS = SPPM_TRACE_STATIC_MAX_RAYS;
+constexpr size_t SCRATCH_FLAG_OFF = (SCRATCH_RAYS * 40 + 255) & ~(size_t)255;
+constexpr size_t SCRATCH_BYTES = SCRATCH_FLAG_OFF + 256;
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[SCRATCH_DEVICES] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+static int scratch_resolve(int device, char** out) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return fail(SPPM_E_ARG, "device %d beyond %d", device, SCRATCH_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SPPM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_dev[device] = p;
+    }
+    if (out) *out = static_cast<char*>(g_scratch_dev[device]);
+    return SPPM_OK;
+}
+
+// Closest hits of n host rays through device buffers the caller provides:
+// org, dir 3n floats, fac n ints, tuv 3n floats (t, u, v blocks), fl 1 word.
+// Results go straight into the caller's host arrays (t, u, v may be NULL).
+static int trace_rays(sppm_ctx* c, int n, const float* org, const float* dir, float tmin, float tmax, int32_t* facet,
+                      float* t, float* u, float* v, float* dorg, float* ddir, int* dfac, float* dtuv, uint32_t* dfl) {
+    hipStream_t st = c->stream;
+    HIPOK(hipMemcpyAsync(dorg, org, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(ddir, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+    launch_set_u32(dfl, 0u, st);
+    float *dt = dtuv, *du = dtuv + n, *dv = dtuv + 2 * (size_t)n;
+    launch_trace(c->scene.dev, n, dorg, ddir, tmin, tmax, c->scene.trace_spill(), dfac, t ? dt : nullptr,
+                 u ? du : nullptr, v ? dv : nullptr, dfl, st);
+    HIPOK(hipGetLastError());
+    uint32_t fl = 0;
+    HIPOK(hipMemcpyAsync(facet, dfac, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    if (t) HIPOK(hipMemcpyAsync(t, dt, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    if (u) HIPOK(hipMemcpyAsync(u, du, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    if (v) HIPOK(hipMemcpyAsync(v, dv, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(&fl, dfl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow");
+    return SPPM_OK;
+}
+
+
 extern "C" {
 
 int sppm_abi_version(void) { return SPPM_ABI_VERSION; }
@@ -313,6 +369,7 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
     HIPOK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(SPPM_E_ARG, "device %d out of range (%d devices)", device, ndev);
     HIPOK(hipSetDevice(device));
+    if (int rc0 = scratch_resolve(device, nullptr)) return rc0;  // the device's staging array, once per device
     sppm_ctx* c = new sppm_ctx();
     c->device = device;
     if (stream) {
@@ -2063,28 +2120,31 @@ int sppm_trace_closest(sppm_ctx* c, int32_t n, const float* org, const float* di
         dfree(dorg); dfree(ddir); dfree(dtuv); dfree(dfac); dfree(dfl);
         return rc;
     }
-    hipStream_t st = c->stream;
-    HIPOK(hipMemcpyAsync(dorg, org, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
-    HIPOK(hipMemcpyAsync(ddir, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
-    launch_set_u32(dfl, 0u, st);
-    launch_trace(c->scene.dev, n, dorg, ddir, tmin, c->scene.trace_spill(), dfac, dtuv, dfl, st);
-    HIPOK(hipGetLastError());
-    std::vector<float> tuv(3 * (size_t)n);
-    uint32_t fl = 0;
-    HIPOK(hipMemcpyAsync(facet, dfac, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-    HIPOK(hipMemcpyAsync(tuv.data(), dtuv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
-    HIPOK(hipMemcpyAsync(&fl, dfl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
+    rc = trace_rays(c, n, org, dir, tmin, tmax, facet, t, u, v, dorg, ddir, dfac, dtuv, dfl);
     dfree(dorg); dfree(ddir); dfree(dtuv); dfree(dfac); dfree(dfl);
-    for (int i = 0; i < n; ++i) {
-        // hits are accepted only below tmax (MaxT); FLT_MAX means "no limit"
-        if (facet[i] >= 0 && !((double)tuv[3 * i] < (double)tmax - 1e-10)) facet[i] = -1;  // bvh8.h:640
-        if (t) t[i] = facet[i] >= 0 ? tuv[3 * i] : 3.40282347e38f;
-        if (u) u[i] = facet[i] >= 0 ? tuv[3 * i + 1] : 0.0f;
-        if (v) v[i] = facet[i] >= 0 ? tuv[3 * i + 2] : 0.0f;
-    }
-    if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow");
-    return SPPM_OK;
+    return rc;
+}
+
+int sppm_trace_closest_static(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, float tmax,
+                              int32_t* facet, float* t, float* u, float* v) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    c->lead_ok = false;
+    if (!c->scene.ready) return fail(SPPM_E_STATE, "no scene uploaded");
+    if (int rc0 = batch_wait(c)) return rc0;
+    if (n <= 0) return SPPM_OK;
+    if ((size_t)n > SCRATCH_RAYS)
+        return fail(SPPM_E_ARG, "sppm_trace_closest_static: %d rays, more than the %d its fixed buffer holds "
+                    "(split the batch, or use sppm_trace_closest)", n, (int)SCRATCH_RAYS);
+    if (!org || !dir || !facet) return fail(SPPM_E_ARG, "NULL argument");
+    HIPOK(hipSetDevice(c->device));
+    char* base = nullptr;
+    if (int rc0 = scratch_resolve(c->device, &base)) return rc0;
+    float* dorg = reinterpret_cast<float*>(base);
+    float* ddir = dorg + 3 * SCRATCH_RAYS;
+    float* dtuv = ddir + 3 * SCRATCH_RAYS;
+    int* dfac = reinterpret_cast<int*>(dtuv + 3 * SCRATCH_RAYS);
+    uint32_t* dfl = reinterpret_cast<uint32_t*>(base + SCRATCH_FLAG_OFF);
+    return trace_rays(c, n, org, dir, tmin, tmax, facet, t, u, v, dorg, ddir, dfac, dtuv, dfl);
 }
 
 int sppm_trace_occluded(sppm_ctx* c, int32_t n, const float* org, const float* dir, float tmin, const float* tmax,

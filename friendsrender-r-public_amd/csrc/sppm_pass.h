@@ -60,8 +60,9 @@ void launch_guide_deposit(const int* count, const int* off, const float* pdep, c
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                 const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,
                 int nee, hipStream_t st);
-void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
-                  int* facet, float* tuv, uint32_t* flags, hipStream_t st);
+// Closest hits in (tmin, tmax) for n rays (t, u, v may be NULL).
+void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, float tmax,
+                  const TraceSpill& sp, int* facet, float* t, float* u, float* v, uint32_t* flags, hipStream_t st);
 // Scene::IsOccluded for a batch of rays: occluded[i] = 0 / 1.
 void launch_occluded(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const float* tmax,
                      float transl,

@@ -1558,12 +1558,15 @@ __global__ void k_init_slots(SlotsDev s) {
 }
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
-// Batch closest hit (N_BVH::ClosestHit) for tests and tools.
+// Batch closest hit (N_BVH::ClosestHit) for queries: hits are accepted in
+// (tmin, tmax) (bvh8.h:591, 640: the MaxT test in double); a miss gives facet
+// -1, t = FLT_MAX, u = v = 0.  t, u, v may be NULL.
 template <int W>
 __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const float* __restrict__ org,
-                                                      const float* __restrict__ dir, float tmin, int cap,
+                                                      const float* __restrict__ dir, float tmin, float tmax, int cap,
                                                       uint32_t* __restrict__ spill, int* __restrict__ facet,
-                                                      float* __restrict__ tuv, uint32_t* __restrict__ flags) {
+                                                      float* __restrict__ t, float* __restrict__ u,
+                                                      float* __restrict__ v, uint32_t* __restrict__ flags) {
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
@@ -1573,10 +1576,11 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const f
         f3 o = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
         f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
         bool hit = closest_hit<W>(S, o, d, tmin, ts, h, &fl);
+        hit = hit && (double)h.t < (double)tmax - 1e-10;
         facet[i] = hit ? h.facet : -1;
-        tuv[3 * i] = hit ? h.t : 3.40282347e38f;
-        tuv[3 * i + 1] = hit ? h.u : 0.0f;
-        tuv[3 * i + 2] = hit ? h.v : 0.0f;
+        if (t) t[i] = hit ? h.t : 3.40282347e38f;
+        if (u) u[i] = hit ? h.u : 0.0f;
+        if (v) v[i] = hit ? h.v : 0.0f;
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -1674,12 +1678,15 @@ void launch_zero_i32(int* p, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 
-void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const TraceSpill& sp,
-                  int* facet, float* tuv, uint32_t* flags, hipStream_t st) {
+void launch_trace(const SceneDev& S, int n, const float* org, const float* dir, float tmin, float tmax,
+                  const TraceSpill& sp, int* facet, float* t, float* u, float* v, uint32_t* flags, hipStream_t st) {
     dim3 g(sp.blocks), b(PATH_BLOCK);
-    if (S.width == 2) hipLaunchKernelGGL(k_trace<2>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
-    else if (S.width == 4) hipLaunchKernelGGL(k_trace<4>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
-    else hipLaunchKernelGGL(k_trace<8>, g, b, 0, st, S, n, org, dir, tmin, sp.cap, sp.buf, facet, tuv, flags);
+#define SPPM_TRACE(WW) \
+    hipLaunchKernelGGL(k_trace<WW>, g, b, 0, st, S, n, org, dir, tmin, tmax, sp.cap, sp.buf, facet, t, u, v, flags)
+    if (S.width == 2) SPPM_TRACE(2);
+    else if (S.width == 4) SPPM_TRACE(4);
+    else SPPM_TRACE(8);
+#undef SPPM_TRACE
 }
 
 void launch_occluded(const SceneDev& S, int n, const float* org, const float* dir, float tmin, const float* tmax,

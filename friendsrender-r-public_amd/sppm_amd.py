@@ -32,8 +32,9 @@ EXPORTED = [
     "sppm_guide_train", "sppm_guide_read", "sppm_graph_capture", "sppm_batch_create", "sppm_batch_add",
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
-    "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables",
+    "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
 ]
+TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
 
 def mt19937_jump(state, steps, lib=None):
@@ -145,6 +146,7 @@ def load_library(path=LIB_PATH):
                                          C.POINTER(C.c_int64)]
     lib.sppm_trace_closest.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_float, C.POINTER(C.c_int32)] + [C.POINTER(C.c_float)] * 3
+    lib.sppm_trace_closest_static.argtypes = lib.sppm_trace_closest.argtypes
     lib.sppm_trace_occluded.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                         C.c_float, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int32)]
     lib.sppm_kernel_ms.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_float)]
@@ -485,6 +487,18 @@ class Renderer:
         f = np.zeros(n, np.int32)
         t, u, v = (np.zeros(n, np.float32) for _ in range(3))
         _check(self.lib.sppm_trace_closest(self.h, n, _fp(org), _fp(dirs), tmin, tmax, _ip(f), _fp(t), _fp(u), _fp(v)))
+        return f, t, u, v
+
+    def trace_static(self, org, dirs, tmin=1e-4, tmax=3.4028234663852886e38):
+        """sppm_trace_closest_static: the same query through the fixed staging
+        array (at most TRACE_STATIC_MAX_RAYS rays, no allocation)."""
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        f = np.zeros(n, np.int32)
+        t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+        _check(self.lib.sppm_trace_closest_static(self.h, n, _fp(org), _fp(dirs), tmin, tmax, _ip(f), _fp(t), _fp(u),
+                                                  _fp(v)))
         return f, t, u, v
 
     def occluded(self, org, dirs, tmin, tmax, translucency=0.5):

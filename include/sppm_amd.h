@@ -255,6 +255,18 @@ int sppm_read_neighbours(sppm_ctx* ctx, int64_t cap, int64_t* offsets, int32_t* 
  * facet id (-1 miss), t, u, v.  Ties in t go to the lowest facet id. */
 int sppm_trace_closest(sppm_ctx* ctx, int32_t n, const float* org, const float* dir,
                        float tmin, float tmax, int32_t* facet, float* t, float* u, float* v);
+/* The same query without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (serving loops, code under HIP
+ * graph capture, harnesses that forbid hipMalloc).  sppm_trace_closest
+ * allocates device staging for the rays and hits on every call; this entry
+ * point stages them in one fixed device array per GPU instead (2.5 MiB,
+ * SPPM_TRACE_STATIC_MAX_RAYS rays, held until the process ends), and calls
+ * no allocating routine.  More rays than that: SPPM_E_ARG.  All calls on one
+ * GPU share that GPU's array: calls that overlap on one GPU (from several
+ * threads, or contexts on other host threads) race. */
+#define SPPM_TRACE_STATIC_MAX_RAYS 65536
+int sppm_trace_closest_static(sppm_ctx* ctx, int32_t n, const float* org, const float* dir, float tmin,
+                              float tmax, int32_t* facet, float* t, float* u, float* v);
 /* Scene::IsOccluded (Scene.cpp:246-281) for a batch of rays: whether a
  * surface lies in [tmin, tmax[i]) along ray i, passing through surfaces whose
  * albedo texture alpha gives 1 - alpha > translucency (the call site
