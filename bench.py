@@ -54,6 +54,13 @@ def parse():
                     help="time-to-RMSE mode: write the reference's linear image and stop")
     ap.add_argument("--ref-load", default=None, metavar="NPY",
                     help="time-to-RMSE mode: read the reference's linear image instead of rendering it")
+    ap.add_argument("--no-rmse", action="store_true",
+                    help="leave out the time-to-RMSE leg (guided vs plain on the occluded-light scene)")
+    ap.add_argument("--mix", type=float, default=0.5, help="guided emission: probability of the plain emission")
+    ap.add_argument("--lr", type=float, default=1e-3, help="guided emission: Adam step size")
+    ap.add_argument("--train-steps", type=int, default=1, help="guided emission: Adam steps per training pass")
+    ap.add_argument("--train-interval", type=int, default=1, help="guided emission: train after every k-th pass")
+    ap.add_argument("--decay", type=float, default=0.95, help="guided emission: training histogram decay")
     ap.add_argument("--max-passes", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=16)
     ap.add_argument("--target-rmse", type=float, default=0.01)
@@ -110,14 +117,25 @@ def workload(name):
                             r0=0.02 * scene.diagonal(), triangles=scene.nfacets, passes=16)
 
 
+def host_cores():
+    """The host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when the machine sets it (the GPU box gives each GPU a
+    share of its cores and says so there)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(scene, cam, cfg, seconds):
-    """Oracle restatement (OpenMP, all host cores up to 16) on the same
+    """Oracle restatement (OpenMP, on every host core this process may use) on the same
     workload: one untimed pass (allocates the photon slots), then whole
     passes of the full-size workload until the time budget is used (at
     least one)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     os_ = pyoracle.OracleScene(scene)
     r = pyoracle.OracleRender(os_, cam, cfg["alpha"], cfg["r0"], cfg["photons"], nthreads=cores)
     r.full_pass()
@@ -147,74 +165,117 @@ def copy_calibration(mib):
     del a, b
 
 
-def rmse_mode(args):
-    """Seconds of rendering until the linear image (reconstruct output ^ 1.2,
-    before the reference's 1/1.2 gamma), normalised to the reference's mean,
-    is within RMSE target of a long plain render of the same scene with
-    independent random streams, for plain and for guided emission.  The timed
-    region is the passes only (synchronised chunks of check_every passes,
-    guided training included); the RMSE checks between chunks are not timed."""
-    import numpy as np
+def rmse_workload(args):
     import scenes
-    import sppm_amd
     if args.rmse_scene == "caustic":
         scene = scenes.caustic_scene()
         res = args.caustic_res
         cam = scenes.caustic_cameras(1, res, res)[0]
-        N, alpha, r0 = args.caustic_photons, args.alpha, 0.01 * scene.diagonal()
-        net = dict(hidden=128, layers=6, fp16=True)
-        workload_name, dtype = "caustic, camera 0", "f32 (MLP fp16 MFMA, f32 accumulation)"
+        w = dict(scene=scene, cam=cam, res=res, N=args.caustic_photons, alpha=args.alpha, r0=0.01 * scene.diagonal(),
+                 net=dict(hidden=128, layers=6, fp16=True), name="caustic, camera 0",
+                 dtype="f32 (MLP fp16 MFMA, f32 accumulation)")
     else:
         scene = scenes.occluded_scene()
         res = args.resolution
         cam = scenes.occluded_camera(res, res)
-        N, alpha, r0 = 1_000_000, args.alpha, 0.02 * scene.diagonal()
-        net = {}
-        workload_name, dtype = "occluded light", "f32 (MLP bf16 MFMA, f32 accumulation)"
-    log(f"{workload_name}: {scene.nfacets} triangles, {res}x{res}, {N} photons/pass")
+        w = dict(scene=scene, cam=cam, res=res, N=1_000_000, alpha=args.alpha, r0=0.02 * scene.diagonal(), net={},
+                 name="occluded light", dtype="f32 (MLP bf16 MFMA, f32 accumulation)")
+    return w
 
-    def linear(r, passes):
-        img = r.reconstruct(N * passes, passes).reshape(res, res, 4)[..., :3].astype(np.float64)
-        return np.power(np.maximum(img, 0.0), 1.2)
 
-    # reference: independent random streams (pass steps shifted by 10^6,
-    # another camera sample stream) so the timed renders do not replay it
+def guided_settings(args):
+    return dict(mix_plain=args.mix, learning_rate=args.lr, train_steps=args.train_steps, seed=1,
+                history_decay=args.decay, train_interval=args.train_interval)
+
+
+def linear_image(r, N, passes, res):
+    """The linear image: reconstruct output ^ 1.2 (before the reference's
+    1/1.2 gamma, SPPM.cpp:485-488)."""
+    import numpy as np
+    img = r.reconstruct(N * passes, passes).reshape(res, res, 4)[..., :3].astype(np.float64)
+    return np.power(np.maximum(img, 0.0), 1.2)
+
+
+def reference_image(args, w):
+    """A long plain render with random streams independent of the timed
+    renders (pass steps offset by 10^6, another camera sample stream)."""
+    import numpy as np
+    import sppm_amd
     if args.ref_load:
-        ref = np.load(args.ref_load).astype(np.float64)
         log(f"reference read from {args.ref_load}")
-    else:
-        t0 = time.perf_counter()
-        ref_cam = dict(cam, sampler_id=7919)
-        ref_r = sppm_amd.Renderer(scene, ref_cam, alpha, r0, N, bvh_width=8)
-        for p in range(args.ref_passes):
-            step = 1_000_001 + p
-            ref_r.photons(step)
-            ref_r.build()
-            ref_r.eye(step)
-            ref_r.gather()
-            if (p + 1) % 500 == 0:
-                ref_r.sync()
-                log(f"reference: {p + 1}/{args.ref_passes} passes, {time.perf_counter() - t0:.1f} s")
-        ref_r.sync()
-        ref = linear(ref_r, args.ref_passes)
-        ref_r.close()
-        if args.ref_save:
-            np.save(args.ref_save, ref.astype(np.float32))
-            log(f"reference written to {args.ref_save}")
-            return
-    # Pixels whose value is not finite (a NaN pixel weight of the reference's
+        return np.load(args.ref_load).astype(np.float64)
+    t0 = time.perf_counter()
+    r = sppm_amd.Renderer(w["scene"], dict(w["cam"], sampler_id=7919), w["alpha"], w["r0"], w["N"], bvh_width=8,
+                          step_offset=1_000_000)
+    for p in range(args.ref_passes):
+        r.pass_(stats=False)
+        if (p + 1) % 1000 == 0:
+            r.sync()
+            log(f"reference: {p + 1}/{args.ref_passes} passes, {time.perf_counter() - t0:.1f} s")
+    r.sync()
+    ref = linear_image(r, w["N"], args.ref_passes, w["res"])
+    r.close()
+    log(f"reference: {args.ref_passes} passes in {time.perf_counter() - t0:.1f} s")
+    if args.ref_save:
+        np.save(args.ref_save, ref.astype(np.float32))
+    return ref
+
+
+def steerable_share(w, passes=4):
+    """Of the light the photons bring into the camera's room (x < 1, the
+    occluded-light scene), the share carried by photon paths whose first
+    segment from the light crosses the door (their first stored photon lies
+    in that room): the part an emission proposal can steer.  Per path, c =
+    sum over its stored photons in the room of luminance(flux) / PDF;
+    returned: the steerable share of sum c (mean) and of sum c^2 (the
+    photon variance of a pass, paths being independent), plain emission,
+    fixed radius."""
+    import numpy as np
+    import sppm_amd
+    r = sppm_amd.Renderer(w["scene"], w["cam"], 1.0, w["r0"], w["N"], bvh_width=8)
+    tot = np.zeros(4)
+    lum = np.array([0.2126, 0.7152, 0.0722])
+    for p in range(passes):
+        r.pass_(stats=False)
+        counts = r.read_path_counts()
+        ph = r.read_photons()
+        path = np.repeat(np.arange(len(counts)), counts)
+        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        steer_path = ph["pos"][first, 0] < 1.0
+        inroom = ph["pos"][:, 0] < 1.0
+        val = np.where(inroom, (ph["flux"].astype(np.float64) @ lum) / ph["pdf"].astype(np.float64), 0.0)
+        c = np.bincount(path, weights=val, minlength=len(counts))
+        tot += [c[steer_path].sum(), c.sum(), (c[steer_path] ** 2).sum(), (c ** 2).sum()]
+    r.close()
+    s, v = tot[0] / tot[1], tot[2] / tot[3]
+    return dict(mean_share=round(float(s), 4), variance_share=round(float(v), 4),
+                gain_cap_mix_0_5=round(0.5 / (1 - v), 3), gain_cap_mix_0_25=round(0.25 / (1 - v), 3),
+                passes=passes,
+                note="variance gain of any emission proposal mixed with weight w is at most w / (1 - variance_share)")
+
+
+def time_to_rmse(args, w, ref):
+    """Seconds of rendering until the linear image, normalised to the
+    reference's mean, is within RMSE target of the reference, for plain and
+    for guided emission.  The timed region is the passes only (synchronised
+    chunks of check_every passes, guided training included); the RMSE checks
+    between chunks are not timed."""
+    import numpy as np
+    import sppm_amd
+    N, res = w["N"], w["res"]
+    # pixels whose value is not finite (a NaN pixel weight of the reference's
     # own arithmetic, propagated as it does) are left out of the RMSE, in the
-    # reference and in the renders alike; their number is reported.
+    # reference and in the renders alike; their number is reported
     ref_ok = np.isfinite(ref).all(axis=-1)
-    scale = float(ref[ref_ok].mean())  # RMSE is measured on the image normalised to the reference's mean
-    log(f"reference mean linear radiance {scale:.5g}, {int((~ref_ok).sum())} non-finite pixels")
+    scale = float(ref[ref_ok].mean())
     out = {}
     for mode in ("plain", "guided"):
-        r = sppm_amd.Renderer(scene, cam, alpha, r0, N, bvh_width=8)
+        r = sppm_amd.Renderer(w["scene"], w["cam"], w["alpha"], w["r0"], N, bvh_width=8)
         if mode == "guided":
-            r.guide_set(True, mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=1, history_decay=0.95, **net)
+            r.guide_set(True, **guided_settings(args), **w["net"])
         el, passes, e = 0.0, 0, float("inf")
         curve = []
+        r.kernel_ms("reset")
         while passes < args.max_passes:
             ts = time.perf_counter()
             for _ in range(args.check_every):
@@ -222,35 +283,47 @@ def rmse_mode(args):
             r.sync()
             el += time.perf_counter() - ts
             passes += args.check_every
-            img = linear(r, passes)
+            img = linear_image(r, N, passes, res)
             ok = ref_ok & np.isfinite(img).all(axis=-1)
             e = float(np.sqrt(np.mean((img[ok] - ref[ok]) ** 2))) / scale
-            curve.append((passes, round(el, 4), e))
-            if passes % (8 * args.check_every) == 0:
-                log(f"{mode}: {passes} passes, {el:.2f} s, RMSE {e:.4g}")
+            curve.append((passes, round(el, 4), round(e, 5)))
             if e <= args.target_rmse:
                 break
-        out[mode] = dict(seconds=round(el, 3), passes=passes, rmse=e, reached=e <= args.target_rmse,
-                         nonfinite_pixels=int((~ok).sum()),
-                         ms_per_pass=round(el / passes * 1e3, 3),
-                         guide_ms=round(r.kernel_ms("guide"), 3) if mode == "guided" else 0.0,
-                         curve=curve[:: max(1, len(curve) // 20)])
+        out[mode] = dict(seconds=round(el, 3), passes=passes, rmse=round(e, 5), reached=e <= args.target_rmse,
+                         nonfinite_pixels=int((~ok).sum()), ms_per_pass=round(el / passes * 1e3, 3),
+                         phases_ms={k: round(r.kernel_ms(k), 3) for k in ("photons", "build", "eye", "gather",
+                                                                            "guide")},
+                         curve=curve[:: max(1, len(curve) // 12)])
         r.close()
         log(f"{mode}: {out[mode]['seconds']} s, {passes} passes, RMSE {e:.4g}")
     both = out["plain"]["reached"] and out["guided"]["reached"]
-    line = {
-        "metric": f"seconds to RMSE<={args.target_rmse} (image normalised to the reference's mean) vs "
-                  f"{args.ref_passes}-pass plain render, guided vs plain",
-        "value": out["guided"]["seconds"], "unit": "s", "higher_is_better": False,
-        "speedup_guided_vs_plain": round(out["plain"]["seconds"] / out["guided"]["seconds"], 3) if both else None,
-        "n_gpus": 1, "dtype": dtype, "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": workload_name, "resolution": f"{res}x{res}", "photons_per_pass": N,
-                   "triangles": scene.nfacets, "alpha": alpha, "r0": round(r0, 6), "bvh_width": 8,
-                   "reference_passes": args.ref_passes, "reference_mean_linear_radiance": scale, "mix_plain": 0.5,
-                   "train_steps_per_pass": 1, "learning_rate": 1e-2, "history_decay": 0.95,
-                   "network": f"{net.get('hidden', 64)}x{net.get('layers', 4)} {'fp16' if net.get('fp16') else 'bf16'}"},
-        "plain": out["plain"], "guided": out["guided"],
-    }
+    return dict(
+        metric=f"seconds to RMSE<={args.target_rmse} against a {args.ref_passes}-pass plain render (image "
+               f"normalised to its mean), guided vs plain",
+        workload=w["name"], resolution=f"{res}x{res}", photons_per_pass=N, triangles=w["scene"].nfacets,
+        alpha=w["alpha"], r0=round(w["r0"], 6), reference_passes=args.ref_passes,
+        reference_mean_linear_radiance=scale,
+        guided_settings=dict(guided_settings(args),
+                             network=f"{w['net'].get('hidden', 64)}x{w['net'].get('layers', 4)} "
+                                     f"{'fp16' if w['net'].get('fp16') else 'bf16'}"),
+        plain_s=out["plain"]["seconds"], guided_s=out["guided"]["seconds"],
+        speedup_guided_vs_plain=round(out["plain"]["seconds"] / out["guided"]["seconds"], 3) if both else None,
+        plain=out["plain"], guided=out["guided"])
+
+
+def rmse_mode(args):
+    """The time-to-RMSE measurement alone (one JSON line)."""
+    w = rmse_workload(args)
+    log(f"{w['name']}: {w['scene'].nfacets} triangles, {w['res']}x{w['res']}, {w['N']} photons/pass")
+    ref = reference_image(args, w)
+    if args.ref_save:
+        return
+    res = time_to_rmse(args, w, ref)
+    if args.rmse_scene == "occluded":
+        res["steerable"] = steerable_share(w)
+    line = {"metric": res["metric"], "value": res["guided_s"], "unit": "s", "higher_is_better": False,
+            "n_gpus": 1, "dtype": w["dtype"], "data": "synthetic (seeded procedural scene)",
+            "config": {"workload": w["name"]}, "time_to_rmse": res}
     print(json.dumps(line), flush=True)
 
 
@@ -501,7 +574,7 @@ def main():
         if args.workload == "full" else None
     value = cfg["photons"] * K / el
     out = {
-        "metric": "photons/s (plain SPPM pass, full-material workload)",
+        "metric": "photons/s + seconds to RMSE<=0.01, guided vs plain SPPM, 1 MI355X",
         "value": round(value, 1),
         "unit": "photons/s",
         "n_gpus": 1,
@@ -539,6 +612,13 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_seconds)
+    if not args.no_rmse and args.workload == "full":
+        # the metric's second half: seconds to RMSE, guided vs plain, on the
+        # occluded-light workload (1024^2, 1M photons, 297,700 triangles)
+        w = rmse_workload(args)
+        log(f"time to RMSE: {w['name']}, {w['scene'].nfacets} triangles, {w['res']}x{w['res']}")
+        out["time_to_rmse"] = time_to_rmse(args, w, reference_image(args, w))
+        out["time_to_rmse"]["steerable"] = steerable_share(w)
     print(json.dumps(out), flush=True)
 
 

@@ -138,7 +138,7 @@ struct RenderBuffers {
     float4* partial = nullptr;
     GroupBuffers groups{};
     int* pass_dev = nullptr;   // passes done (device copy of the pass counter, read by graph replays)
-    int* path_cell = nullptr;  // guided emission: emission cell per path
+    int* path_cell = nullptr;  // guided emission: emission cell per path, 2 x N (block = step parity)
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
@@ -266,6 +266,7 @@ struct sppm_ctx {
     sppm_params params{};
     bool have_params = false;
     int pass = 0;
+    int split_step = 0;  // step of the last sppm_pass_photons (its gather's training records)
     Timer t_photons, t_build, t_eye, t_gather, t_guide;
     // learned emission proposal
     GuideBuffers guide;
@@ -356,7 +357,6 @@ static int trace_rays(sppm_ctx* c, int n, const float* org, const float* dir, fl
     if (fl & 4u) return fail(SPPM_E_OVERFLOW, "traversal stack overflow");
     return SPPM_OK;
 }
-
 
 extern "C" {
 
@@ -780,6 +780,7 @@ int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
     if (!(p->first_radius > 0.0f)) return fail(SPPM_E_ARG, "first_radius must be > 0");
     if (p->eye_nee < SPPM_NEE_OFF || p->eye_nee > SPPM_NEE_KEEP) return fail(SPPM_E_ARG, "eye_nee must be 0, 1 or 2");
     if (p->record_neighbours < 0 || p->record_neighbours > 2) return fail(SPPM_E_ARG, "record_neighbours must be 0, 1 or 2");
+    if (p->step_offset < 0) return fail(SPPM_E_ARG, "step_offset must be >= 0");
     if (p->record_neighbours == 2 && p->max_neighbours <= 0)
         return fail(SPPM_E_ARG, "record_neighbours 2 needs max_neighbours > 0 (pairs recorded per pass)");
     c->params = *p;
@@ -888,7 +889,7 @@ int sppm_begin(sppm_ctx* c) {
         }
     }
     if (c->guided) {
-        if ((rc = dalloc(&R.path_cell, N)) || (rc = dalloc(&R.pdep, R.photon_cap))) {
+        if ((rc = dalloc(&R.path_cell, 2 * (size_t)N)) || (rc = dalloc(&R.pdep, R.photon_cap))) {
             R.release();
             return rc;
         }
@@ -896,7 +897,8 @@ int sppm_begin(sppm_ctx* c) {
     }
     HIPOK(hipMemsetAsync(R.counts, 0, sizeof(int) * (size_t)(np + 1), c->stream));
     launch_init_slots(R.slots, c->stream);
-    launch_zero_i32(R.pass_dev, 1, c->stream);
+    // device pass counter: steps are pass + 1 + step_offset (graph replays read it)
+    launch_set_u32(reinterpret_cast<uint32_t*>(R.pass_dev), (uint32_t)c->params.step_offset, c->stream);
     launch_init_shared(R.shared, np, c->params.first_radius, c->stream);
     // UniformSample(ID): the mt19937(ID) stream in K segments per pass
     if (c->cam.sampler == SPPM_SAMPLER_UNIFORM) {
@@ -911,7 +913,7 @@ int sppm_begin(sppm_ctx* c) {
     launch_set_u32(R.flags, 0u, c->stream);
     launch_set_u32(reinterpret_cast<uint32_t*>(R.nstored), 0u, c->stream);
     {
-        Counters z{0, 0, 0, 0};
+        Counters z{};
         HIPOK(hipMemcpyAsync(R.ctr, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
     }
     HIPOK(hipGetLastError());
@@ -988,11 +990,18 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather
     launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, sp, c->scene.next_path,
                   c->guided ? &c->emit : nullptr, c->rb.flags, photon_rays(c->rb), st);
 }
-// Training records of the pass (deposit histogram), Adam steps on it, and
-// the proposal for the next pass.
-static void do_guide(sppm_ctx* c, hipStream_t st) {
+// Training records of the pass (deposit histogram), and on training passes
+// (every train_interval-th step; every replay of a captured pass) Adam steps
+// on the decayed histogram and the new proposal.  The emission of pass s
+// samples the proposal published by the build of pass s - 1, i.e. the one
+// trained after pass s - 2: a pass's photon pass can then run beside the
+// previous pass's gather and training.  Each photon carries the PDF of the
+// proposal that drew it, so the estimate stays unbiased.
+static void do_guide(sppm_ctx* c, hipStream_t st, int step, const int* step_dev) {
     RenderBuffers& R = c->rb;
-    launch_guide_deposit(R.slots.count, R.off, R.pdep, R.path_cell, R.N, c->guide.hpass, st);
+    launch_guide_deposit(R.off, R.nstored, R.pdep, R.path_cell, R.N, step, step_dev, c->guide.hpass, st);
+    const int every = c->gparams.train_interval > 1 ? c->gparams.train_interval : 1;
+    if (!step_dev && step % every != 0) return;  // the deposits gather in the pass histogram until then
     guide_fold_hist(c->guide, c->gparams.history_decay, st);
     guide_hist_sum(c->guide, st);
     for (int k = 0; k < c->gparams.train_steps; ++k) {
@@ -1003,6 +1012,9 @@ static void do_guide(sppm_ctx* c, hipStream_t st) {
 }
 static void do_build(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
+    // the photon pass that read the emission tables is done: publish the
+    // proposal trained last for the next photon pass
+    if (c->guided) guide_publish(c->guide, st);
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
     launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_count, R.cell_start, R.cell_scan_tmp, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
@@ -1085,6 +1097,7 @@ int sppm_pass_photons(sppm_ctx* c, int step, int32_t* emitted) {
     c->t_photons.start(c->stream);
     do_photons(c, step, c->stream);
     c->t_photons.stop(c->stream);
+    c->split_step = step;
     HIPOK(hipGetLastError());
     if (emitted) *emitted = c->rb.N;
     return SPPM_OK;
@@ -1120,7 +1133,7 @@ int sppm_pass_gather(sppm_ctx* c) {
     rc = do_gather(c, c->stream);
     if (rc) return rc;
     c->t_gather.stop(c->stream);
-    if (c->guided) do_guide(c, c->stream);
+    if (c->guided) do_guide(c, c->stream, c->split_step, nullptr);
     HIPOK(hipGetLastError());
     return SPPM_OK;
 }
@@ -1153,13 +1166,13 @@ static int read_pass_stats(sppm_ctx* c, sppm_pass_stats* stats) {
 }
 
 int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
-    // unguided passes in a row: this photon pass need not wait for the
-    // previous gather (guided emission samples the proposal that the previous
-    // pass trains after its gather, from that pass's photon slots)
-    const bool lead = c && c->lead_ok && !c->guided;
+    // passes in a row: this photon pass need not wait for the previous
+    // gather (guided emission samples the proposal published by the previous
+    // build, not the one the previous pass trains)
+    const bool lead = c && c->lead_ok;
     int rc = check_own_pass(c, "sppm_pass");
     if (rc) return rc;
-    const int step = c->pass + 1;
+    const int step = c->pass + 1 + c->params.step_offset;
     hipStream_t st = c->stream;
     if (c->graph_exec) {
         HIPOK(hipGraphLaunch(c->graph_exec, st));
@@ -1203,7 +1216,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     c->t_gather.stop(st);
     if (c->guided) {
         c->t_guide.start(st);
-        do_guide(c, st);
+        do_guide(c, st, step, nullptr);
         c->t_guide.stop(st);
     }
     launch_add_i32(c->rb.pass_dev, 1, st);
@@ -1233,7 +1246,7 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
     int rc = do_gather(c, st);
     if (rc) return rc;
-    if (c->guided) do_guide(c, st);
+    if (c->guided) do_guide(c, st, 0, R.pass_dev);
     launch_add_i32(R.pass_dev, 1, st);
     return SPPM_OK;
 }
@@ -1247,6 +1260,8 @@ static int graph_ok(sppm_ctx* c, const sppm_batch* member_of = nullptr) {
     if (c->cam.sampler != SPPM_SAMPLER_UNIFORM)
         return fail(SPPM_E_STATE, "graph-captured passes need the UniformSample camera stream");
     if (c->params.record_neighbours) return fail(SPPM_E_STATE, "graph-captured passes need record_neighbours 0");
+    if (c->guided && c->gparams.train_interval > 1)
+        return fail(SPPM_E_STATE, "graph-captured passes train after every pass (guide train_interval 1)");
     return SPPM_OK;
 }
 
@@ -1469,6 +1484,17 @@ int sppm_ray_counts(sppm_ctx* c, int64_t* photon_rays, int64_t* eye_rays) {
     return SPPM_OK;
 }
 
+int sppm_tie_counts(sppm_ctx* c, int64_t* photon_ties, int64_t* eye_ties) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    Counters h{};
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if (photon_ties) *photon_ties = (int64_t)h.photon_ties;
+    if (eye_ties) *eye_ties = (int64_t)h.eye_ties;
+    return SPPM_OK;
+}
+
 int sppm_sync(sppm_ctx* c) {
     if (!c) return fail(SPPM_E_ARG, "NULL ctx");
     if (int rc = batch_wait(c)) return rc;
@@ -1505,6 +1531,7 @@ uint64_t setup_fingerprint(const sppm_ctx* c) {
     h = fnv1a(h, &c->params.first_radius, sizeof(float));
     h = fnv1a(h, &c->params.photons_per_pass, sizeof(int32_t));
     h = fnv1a(h, &c->params.eye_nee, sizeof(int32_t));
+    h = fnv1a(h, &c->params.step_offset, sizeof(int32_t));
     const int32_t g = c->guided ? 1 : 0;
     h = fnv1a(h, &g, sizeof g);
     if (c->guided) {
@@ -1514,6 +1541,7 @@ uint64_t setup_fingerprint(const sppm_ctx* c) {
         h = fnv1a(h, &q.train_steps, sizeof(int32_t));
         h = fnv1a(h, &q.history_decay, sizeof(float));
         h = fnv1a(h, &q.hidden, 3 * sizeof(int32_t));  // hidden, layers, fp16
+        h = fnv1a(h, &q.train_interval, sizeof(int32_t));
     }
     return h;
 }
@@ -1539,6 +1567,9 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
         v.push_back({g.m, sizeof(float) * g.nparams});
         v.push_back({g.v, sizeof(float) * g.nparams});
         v.push_back({g.hist, sizeof(float) * GUIDE_CELLS});
+        v.push_back({g.hpass, sizeof(float) * GUIDE_CELLS});
+        v.push_back({g.eprob, sizeof(float) * GUIDE_CELLS});
+        v.push_back({g.ecdf, sizeof(float) * GUIDE_CELLS});
         v.push_back({g.scal, sizeof(float) * 8});
     }
     return v;
@@ -1984,6 +2015,7 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!(p->mix_plain >= 0.25f && p->mix_plain <= 1.0f)) return fail(SPPM_E_ARG, "mix_plain must be in [0.25, 1]");
     if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
     if (!(p->history_decay >= 0.0f && p->history_decay < 1.0f)) return fail(SPPM_E_ARG, "history_decay must be in [0, 1)");
+    if (p->train_interval < 0) return fail(SPPM_E_ARG, "train_interval must be >= 0");
     if (c->params.record_neighbours == 1)
         return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0 or 2)");
     GuideNet net;
@@ -2016,17 +2048,18 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (guide_upload_params(c->guide, th.data(), c->stream)) return fail(SPPM_E_HIP, "guide upload");
     guide_clear_hist(c->guide, c->stream);
     guide_infer(c->guide, c->stream);
+    guide_publish(c->guide, c->stream);
     HIPOK(hipGetLastError());
     E.w = p->mix_plain;
-    E.prob = c->guide.prob;
-    E.cdf = c->guide.cdf;
+    E.prob = c->guide.eprob;
+    E.cdf = c->guide.ecdf;
     E.path_cell = c->rb.path_cell;
     c->emit = E;
     c->gparams = *p;
     c->guided = true;
     if (c->rb.ready && !c->rb.path_cell) {
         RenderBuffers& R = c->rb;
-        if ((rc = dalloc(&R.path_cell, R.N)) || (rc = dalloc(&R.pdep, R.photon_cap))) return rc;
+        if ((rc = dalloc(&R.path_cell, 2 * (size_t)R.N)) || (rc = dalloc(&R.pdep, R.photon_cap))) return rc;
         c->emit.path_cell = R.path_cell;
     }
     HIPOK(hipStreamSynchronize(c->stream));
@@ -2047,6 +2080,7 @@ int sppm_guide_weights(sppm_ctx* c, float* out, const float* in) {
     if (in) {
         if (guide_upload_params(c->guide, in, c->stream)) return fail(SPPM_E_HIP, "guide upload");
         guide_infer(c->guide, c->stream);
+        guide_publish(c->guide, c->stream);
     }
     if (out) HIPOK(hipMemcpyAsync(out, c->guide.theta, sizeof(float) * c->guide.nparams, hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
@@ -2079,6 +2113,7 @@ int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
         guide_adam(c->guide, lr, c->stream);
     }
     guide_infer(c->guide, c->stream);
+    guide_publish(c->guide, c->stream);
     HIPOK(hipGetLastError());
     HIPOK(hipStreamSynchronize(c->stream));
     return SPPM_OK;
@@ -2088,8 +2123,8 @@ int sppm_guide_tables(sppm_ctx* c, float* prob, float* cdf) {
     int rc = guide_ready(c);
     if (rc) return rc;
     HIPOK(hipStreamSynchronize(c->stream));
-    if (prob) HIPOK(hipMemcpy(prob, c->guide.prob, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
-    if (cdf) HIPOK(hipMemcpy(cdf, c->guide.cdf, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    if (prob) HIPOK(hipMemcpy(prob, c->guide.eprob, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    if (cdf) HIPOK(hipMemcpy(cdf, c->guide.ecdf, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
     return SPPM_OK;
 }
 

@@ -61,7 +61,8 @@ struct GuideEmit {
     float w;               // probability of the plain emission
     const float* prob;     // proposal over cells (unnormalised: total = cdf[last])
     const float* cdf;
-    int* path_cell;        // emission cell of each photon path (training record)
+    int* path_cell;        // emission cell of each photon path (training record), 2 x N:
+                           // the pass of step s writes block s & 1
     // textured light only
     int textured;
     const int* pt_cell;        // position cell of each table point (indexed like the light tables)
@@ -85,6 +86,10 @@ struct GuideBuffers {
     float* logits = nullptr;    // per cell
     float* prob = nullptr;      // softmax
     float* cdf = nullptr;       // inclusive prefix sum of prob
+    // the tables the photon pass samples: prob / cdf as they were when the
+    // build after the previous photon pass published them (guide_publish)
+    float* eprob = nullptr;
+    float* ecdf = nullptr;
     float* hist = nullptr;      // training histogram (decayed sum of the passes' deposits)
     float* hpass = nullptr;     // this pass's deposit histogram
     float* part = nullptr;      // per-block gradient partials: GUIDE_BLOCKS * (nparams + 1)
@@ -119,5 +124,8 @@ void guide_clear_hist(GuideBuffers& g, hipStream_t st);
 void guide_restore(GuideBuffers& g, hipStream_t st);
 // hist = decay * hist + hpass; hpass = 0
 void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st);
+// eprob / ecdf = prob / cdf (a copy kernel: the emission reads the copy while
+// the next proposal is trained)
+void guide_publish(GuideBuffers& g, hipStream_t st);
 
 }  // namespace sppm

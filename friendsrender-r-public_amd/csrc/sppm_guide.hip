@@ -587,6 +587,7 @@ int guide_alloc(GuideBuffers& g) {
         galloc(&g.grad, g.nparams) || galloc(&g.wfwd, guide_lw_size(n)) || galloc(&g.wbwd, guide_lw_size(n)) ||
         galloc(&g.acts, acts) || galloc(&g.dh, 2 * (size_t)GUIDE_CELLS * n.hidden) ||
         galloc(&g.logits, GUIDE_CELLS) || galloc(&g.prob, GUIDE_CELLS) || galloc(&g.cdf, GUIDE_CELLS) ||
+        galloc(&g.eprob, GUIDE_CELLS) || galloc(&g.ecdf, GUIDE_CELLS) ||
         galloc(&g.hist, GUIDE_CELLS) || galloc(&g.hpass, GUIDE_CELLS) ||
         galloc(&g.part, (size_t)GUIDE_BLOCKS * (g.nparams + 1)) || galloc(&g.part2, 4 * 1024) || galloc(&g.scal, 8)) {
         guide_free(g);
@@ -601,7 +602,8 @@ int guide_alloc(GuideBuffers& g) {
 
 void guide_free(GuideBuffers& g) {
     for (void* p : {(void*)g.theta, (void*)g.m, (void*)g.v, (void*)g.grad, (void*)g.wfwd, (void*)g.wbwd,
-                    (void*)g.acts, (void*)g.dh, (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.hist,
+                    (void*)g.acts, (void*)g.dh, (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.eprob,
+                    (void*)g.ecdf, (void*)g.hist,
                     (void*)g.hpass, (void*)g.part, (void*)g.part2, (void*)g.scal, (void*)g.pt_cell,
                     (void*)g.pcell_mass, (void*)g.pc_start, (void*)g.pc_pt, (void*)g.pc_cum})
         if (p) (void)hipFree(p);
@@ -718,6 +720,17 @@ void guide_clear_hist(GuideBuffers& g, hipStream_t st) {
 
 void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_fold, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, g.hpass, decay);
+}
+
+__global__ void k_guide_publish(const float* __restrict__ prob, const float* __restrict__ cdf,
+                                float* __restrict__ eprob, float* __restrict__ ecdf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= GUIDE_CELLS) return;
+    eprob[i] = prob[i];
+    ecdf[i] = cdf[i];
+}
+void guide_publish(GuideBuffers& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_publish, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.prob, g.cdf, g.eprob, g.ecdf);
 }
 
 }  // namespace sppm

@@ -31,6 +31,11 @@ struct Counters {
     unsigned long long candidates;  // photons the balanced gather tested (union boxes of its tiles)
     unsigned long long photon_rays; // closest-hit queries of the photon pass
     unsigned long long eye_rays;    // closest-hit queries of the eye pass (shadow-ray layers included)
+    // closest hits with another facet at exactly the same t (the rule "lowest
+    // facet id" and the reference's "last tested" may differ on them); each
+    // sits two words after its ray counter
+    unsigned long long photon_ties;
+    unsigned long long eye_ties;
 };
 
 // Camera words of one pass: K segments of L words (K * L = 6 * pixels);
@@ -47,15 +52,17 @@ struct TraceSpill {
 int trace_grid_blocks(int width);
 struct GuideEmit;
 // step: the pass step (SPPM.cpp:798), or, when step_dev is given, *step_dev + 1
-// rays: a 64-bit counter the launch adds its closest-hit queries to
+// rays: a 64-bit counter the launch adds its closest-hit queries to; the
+// exact-tie count goes to rays[2]
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
                    const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags,
                    unsigned long long* rays, hipStream_t st);
 void launch_add_i32(int* p, int v, hipStream_t st);
 void launch_zero_i32(int* p, int n, hipStream_t st);
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
-void launch_guide_deposit(const int* count, const int* off, const float* pdep, const int* path_cell, int N, float* hist,
-                          hipStream_t st);
+// step: the pass step, or *step_dev + 1 when step_dev is given (graph replays)
+void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell, int N,
+                          int step, const int* step_dev, float* hist, hipStream_t st);
 // nee: SPPM_NEE_* (the compiled-out next-event estimation of getHitpointInfo)
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                 const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,

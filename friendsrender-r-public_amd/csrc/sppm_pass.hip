@@ -338,7 +338,7 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
         q = G.w * p_plain + (1.0f - G.w) * p_mlp;
         rad0 = mk(G.rad[0], G.rad[1], G.rad[2]);
     }
-    G.path_cell[i] = cell;
+    G.path_cell[(size_t)(step & 1) * (size_t)N + i] = cell;
     const f3 rad = rad0 * smin(1.0f, cs);
     P.rad = rad;
     P.cur_org = lpos + Nn * SPPM_RAY_OFFSET;
@@ -361,12 +361,14 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
 // One iteration of the createPhotonMap loop body; false when the path ends.
 template <int W>
 __device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
-                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl, uint32_t& nray) {
+                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl, uint32_t& nray,
+                                              uint32_t& ntie) {
     if (P.loopcount++ > 32 || P.pidx >= 31) return false;
     if (P.T.x < HI_1EM8 || P.T.y < HI_1EM8 || P.T.z < HI_1EM8) return false;  // (double)T < 1e-8
     HitRec h;
     ++nray;
     if (!closest_hit<W>(S, P.cur_org, P.cur_dir, SPPM_RAY_OFFSET, ts, h, fl)) return false;
+    ntie += h.tie ? 1u : 0u;
     Geom g;
     calc_geometry(S, h, g);
     const f3 Nf = dot(g.N, P.cur_dir) > 0 ? -g.N : g.N;
@@ -452,7 +454,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t ovf = 0, fl = 0, nray = 0;
+    uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
     PhotonPath P;
     bool alive = false, exhausted = false;
     for (;;) {
@@ -477,7 +479,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
             if (exhausted) break;
             continue;
         }
-        if (alive && !photon_bounce<W>(S, slots, P, ts, &ovf, &fl, nray)) {
+        if (alive && !photon_bounce<W>(S, slots, P, ts, &ovf, &fl, nray, ntie)) {
             photon_finish(slots, P);
             alive = false;
         }
@@ -485,6 +487,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
     if (ovf) fl |= 2u;
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
+    add_wave_count(rays + 2, ntie);
 }
 
 // ---------------------------------------------------------------------------
@@ -689,7 +692,7 @@ __device__ __forceinline__ bool occluded_ray(const SceneDev& S, f3 o, f3 d, floa
 template <int W, bool NEE>
 __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, const HitDev& H, int step, int i,
                                          const uint32_t* __restrict__ words, const TraceStack& ts, uint32_t& fl,
-                                         bool keep, uint32_t& nray) {
+                                         bool keep, uint32_t& nray, uint32_t& ntie) {
     uint32_t ovf = 0;
     MTCursor m;
     mt_seed(m, (uint32_t)i + (uint32_t)C.W * (uint32_t)C.H * (uint32_t)step);
@@ -732,7 +735,9 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
         }
         HitRec h;
         ++nray;
-        if (!closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, ts, h, &fl)) {
+        const bool got = closest_hit<W>(S, cur_org, cur_dir, SPPM_RAY_OFFSET, ts, h, &fl);
+        ntie += h.tie ? 1u : 0u;
+        if (!got) {
             // escape: the environment's colour (SPPM.cpp:290-300)
             if (ibl) direct = direct + (ibl_color(S, cur_dir) * T) / PDF;
             if (type == 0) type = 2;
@@ -883,17 +888,18 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDe
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int NPIX = C.W * C.H;
-    uint32_t fl = 0, nray = 0;
+    uint32_t fl = 0, nray = 0, ntie = 0;
     for (;;) {
         int base = 0;
         if ((lane & 63) == 0) base = atomicAdd(queue, 64);
         base = __shfl(base, 0, 64);
         if (base >= NPIX) break;
         const int i = base + (lane & 63);
-        if (i < NPIX) eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray);
+        if (i < NPIX) eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray, ntie);
     }
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
+    add_wave_count(rays + 2, ntie);
 }
 
 // ---------------------------------------------------------------------------
@@ -1793,21 +1799,28 @@ __global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n)
     const int cnt = *n;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) p[k] = 0.0f;
 }
-__global__ void k_guide_deposit(const int* __restrict__ count, const int* __restrict__ off,
+// Per path: its flattened photons' deposits (offsets of this pass's build;
+// the path's last photon ends at the next path's offset, the last path's at
+// the stored count) into the histogram cell it was emitted from (the block of
+// path_cell the photon pass of this step wrote).  The slot counts are not
+// read: the next pass's photon pass may already be rewriting them.
+__global__ void k_guide_deposit(const int* __restrict__ off, const int* __restrict__ nstored,
                                 const float* __restrict__ pdep, const int* __restrict__ path_cell, int N,
-                                float* __restrict__ hist) {
+                                int step_arg, const int* __restrict__ step_dev, float* __restrict__ hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int c = count[i], o = off[i];
+    const int step = step_dev ? *step_dev + 1 : step_arg;
+    const int o = off[i], e = i + 1 < N ? off[i + 1] : *nstored;
     float d = 0.0f;
-    for (int k = 0; k < c; ++k) d += pdep[o + k];
-    if (d > 0.0f) atomicAdd(&hist[path_cell[i]], d);
+    for (int k = o; k < e; ++k) d += pdep[k];
+    if (d > 0.0f) atomicAdd(&hist[path_cell[(size_t)(step & 1) * (size_t)N + i]], d);
 }
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_f32_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
 }
-void launch_guide_deposit(const int* count, const int* off, const float* pdep, const int* path_cell, int N, float* hist,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, count, off, pdep, path_cell, N, hist);
+void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell, int N,
+                          int step, const int* step_dev, float* hist, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, off, nstored, pdep, path_cell, N,
+                       step, step_dev, hist);
 }
 }  // namespace sppm

@@ -31,6 +31,8 @@ struct HitRec {
     float t, u, v;
     int facet;  // -1: miss
     int slot;   // leaf-triangle index (sorted order)
+    bool tie;   // another facet is hit at exactly the same t: the reference keeps the
+                // last-tested one (bvh8.h:591), which depends on its tree; here the lowest id
 };
 
 __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int slot, f3 org, f3 dir, float& t,
@@ -103,6 +105,7 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
     int bfacet = 0x7fffffff;
     int bslot = -1;
     float bu = 0.0f, bv = 0.0f;
+    bool tie = false;
     const int NF = node_floats(W);
     int sp = 0;
     uint32_t ref = 0;  // root node
@@ -115,8 +118,11 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
                 float t, u, v;
                 if (tri_test(S.tri, s, org, dir, t, u, v)) {
                     int fid = __float_as_int(S.tri[3 * s].w);
-                    if (t > tmin && (t < best || (t == best && fid < bfacet))) {
-                        best = t; bu = u; bv = v; bfacet = fid; bslot = s;
+                    if (t > tmin) {
+                        tie = t == best ? (tie || fid != bfacet) : (t < best ? false : tie);
+                        if (t < best || (t == best && fid < bfacet)) {
+                            best = t; bu = u; bv = v; bfacet = fid; bslot = s;
+                        }
                     }
                 }
             }
@@ -162,9 +168,11 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
         h.facet = -1;
         h.slot = -1;
         h.t = 3.40282347e38f;
+        h.tie = false;
         return false;
     }
     h.t = best; h.u = bu; h.v = bv; h.facet = bfacet; h.slot = bslot;
+    h.tie = tie;
     return true;
 }
 

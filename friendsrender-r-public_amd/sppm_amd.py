@@ -33,6 +33,7 @@ EXPORTED = [
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
     "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
+    "sppm_tie_counts",
 ]
 TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
@@ -59,7 +60,7 @@ GUIDE_NPARAMS = 16705
 class GuideParams(C.Structure):
     _fields_ = [("enable", C.c_int32), ("mix_plain", C.c_float), ("learning_rate", C.c_float),
                 ("train_steps", C.c_int32), ("seed", C.c_uint32), ("history_decay", C.c_float),
-                ("hidden", C.c_int32), ("layers", C.c_int32), ("fp16", C.c_int32)]
+                ("hidden", C.c_int32), ("layers", C.c_int32), ("fp16", C.c_int32), ("train_interval", C.c_int32)]
 
 
 def guide_nparams(hidden=64, layers=4):
@@ -101,7 +102,8 @@ class CameraDesc(C.Structure):
 class Params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("first_radius", C.c_float), ("photons_per_pass", C.c_int32),
                 ("bvh_width", C.c_int32), ("block_size", C.c_int32), ("check_numeric", C.c_int32),
-                ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32), ("eye_nee", C.c_int32)]
+                ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32), ("eye_nee", C.c_int32),
+                ("step_offset", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -167,6 +169,8 @@ def load_library(path=LIB_PATH):
     lib.sppm_guide_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
     lib.sppm_guide_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.sppm_counters.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
+    lib.sppm_ray_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 2
+    lib.sppm_tie_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 2
     lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     return lib
 
@@ -248,14 +252,16 @@ RECORD_OFF, RECORD_LISTS, RECORD_PAIRS = 0, 1, 2  # sppm_params.record_neighbour
 
 
 def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False, nee=0,
-                  max_pairs=0):
+                  max_pairs=0, step_offset=0):
     """record: False/0 none, True/1 lists from the recording gather kernel, 2
-    pairs appended by the production gather (at most max_pairs per pass)."""
+    pairs appended by the production gather (at most max_pairs per pass).
+    step_offset: pass k runs the reference's step k + step_offset."""
     p = Params()
     p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
     p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)
     p.record_neighbours, p.max_neighbours = int(record), int(max_pairs)
     p.eye_nee = nee
+    p.step_offset = int(step_offset)
     return p
 
 
@@ -263,7 +269,7 @@ class Renderer:
     """One SPPM render on one GPU: scene + camera + params, then passes."""
 
     def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0, nee=NEE_OFF,
-                 max_pairs=0):
+                 max_pairs=0, step_offset=0):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_create(C.byref(h), device, None))
@@ -274,7 +280,8 @@ class Renderer:
         _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
         self._cam = camera_desc(cam)
         _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
-        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs)
+        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs,
+                                     step_offset=step_offset)
         _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
         _check(self.lib.sppm_begin(self.h))
         self.passes = 0
@@ -350,9 +357,9 @@ class Renderer:
 
     # ---- learned emission proposal --------------------------------------
     def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9,
-                  hidden=64, layers=4, fp16=False):
+                  hidden=64, layers=4, fp16=False, train_interval=1):
         g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay, hidden, layers,
-                        int(fp16))
+                        int(fp16), int(train_interval))
         _check(self.lib.sppm_guide_set(self.h, C.byref(g)))
         self.guide_np = guide_nparams(hidden, layers)
 
@@ -418,10 +425,12 @@ class Renderer:
     def counters(self):
         v = [C.c_int64() for _ in range(5)]
         _check(self.lib.sppm_counters(self.h, *[C.byref(x) for x in v]))
-        rays = [C.c_int64() for _ in range(2)]
+        rays = [C.c_int64() for _ in range(4)]
         _check(self.lib.sppm_ray_counts(self.h, C.byref(rays[0]), C.byref(rays[1])))
+        _check(self.lib.sppm_tie_counts(self.h, C.byref(rays[2]), C.byref(rays[3])))
         return dict(hit_points=v[0].value, neighbours=v[1].value, stored=v[2].value, candidates=v[3].value,
-                    passes=v[4].value, photon_rays=rays[0].value, eye_rays=rays[1].value)
+                    passes=v[4].value, photon_rays=rays[0].value, eye_rays=rays[1].value,
+                    photon_tie_rays=rays[2].value, eye_tie_rays=rays[3].value)
 
     def read_photons(self):
         n = C.c_int32()

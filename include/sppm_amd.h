@@ -34,7 +34,8 @@
 extern "C" {
 #endif
 
-#define SPPM_ABI_VERSION 3  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee */
+#define SPPM_ABI_VERSION 4  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
+                               4: sppm_guide_params.train_interval, sppm_params.step_offset */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -138,6 +139,9 @@ typedef struct sppm_params {
                                   photon) pairs, at most max_neighbours per pass (debug/parity) */
     int32_t max_neighbours;    /* capacity of the pair buffer in mode 2 (entries) */
     int32_t eye_nee;           /* SPPM_NEE_*: next-event estimation on eye paths */
+    int32_t step_offset;       /* sppm_pass runs step = passes done + 1 + step_offset (the reference's
+                                  step, SPPM.cpp:798, 836, which seeds every photon and eye path):
+                                  a second render with another offset draws independent streams */
 } sppm_params;
 
 /* Next-event estimation with MIS on eye paths: the two blocks of
@@ -285,6 +289,12 @@ int sppm_trace_occluded(sppm_ctx* ctx, int32_t n, const float* org, const float*
  * the eye passes (getHitpointInfo, SPPM.cpp:125, plus shadow-ray layers when
  * eye_nee is on). */
 int sppm_ray_counts(sppm_ctx* ctx, int64_t* photon_rays, int64_t* eye_rays);
+/* Of those, the closest hits where another facet is hit at exactly the same
+ * distance.  On them the reference keeps the last triangle its traversal
+ * tested (N_BVH::findClosestElement's `best >= t`, bvh8.h:591), which depends
+ * on its tree; this core keeps the lowest facet id, so these are the only
+ * rays where the two rules can pick different facets. */
+int sppm_tie_counts(sppm_ctx* ctx, int64_t* photon_ties, int64_t* eye_ties);
 int sppm_counters(sppm_ctx* ctx, int64_t* hit_points, int64_t* neighbours, int64_t* photons_stored,
                   int64_t* candidates, int64_t* passes);
 
@@ -307,7 +317,9 @@ int sppm_scene_stats(sppm_ctx* ctx, sppm_scene_info* info);
  * the GPU (MFMA in bf16 or fp16 with float32 accumulation, Adam)
  * from the photons' deposits on visible hit points: per cell, the sum over the
  * photons emitted from it of (hit points that gather the photon) x (photon
- * luminance / PDF).
+ * luminance / PDF).  The photon pass of a pass samples the proposal trained
+ * two passes before (published by the previous pass's build), so it can run
+ * beside the previous pass's gather and training.
  * Scenes: exactly one untextured rectangular light made of two facets. */
 #define SPPM_GUIDE_CELLS 262144
 #define SPPM_GUIDE_NPARAMS 16705 /* parameters of the default network (hidden 64, 4 layers) */
@@ -321,6 +333,8 @@ typedef struct sppm_guide_params {
     int32_t hidden;        /* units per hidden layer: 64 or 128 (0: 64) */
     int32_t layers;        /* hidden layers: 4 or 6 (0: 4) */
     int32_t fp16;          /* 1: fp16 MFMA, 0: bf16 */
+    int32_t train_interval; /* train after every k-th pass (0, 1: every pass); the deposits of the
+                               passes between gather into the histogram.  Captured passes need 0 or 1. */
 } sppm_guide_params;
 /* Parameter count: 64*hidden + hidden + (layers-1)*(hidden*hidden + hidden) + hidden + 1. */
 /* Enable (after sppm_scene_upload) or disable guided emission; re-initialises the weights. */
