@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=255)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["full", "box"], default="full")
+    ap.add_argument("--photon-kernel", choices=["megakernel", "wavefront"], default=None,
+                    help="photon pass kernels (default: the library's default)")
+    ap.add_argument("--wave-bounces", type=int, default=0, help="wavefront: bounces from ray queues (0 = default)")
+    ap.add_argument("--wave-lanes", type=int, default=0, help="wavefront: lanes per ray (0 = default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rmse", action="store_true",
@@ -87,6 +91,20 @@ def parse():
                     help="before the workload, run a device elementwise copy of MIB MiB (float32, torch) three "
                          "times: a kernel of known traffic for scaling the HBM byte counters of a profiler run")
     return ap.parse_args()
+
+
+def photon_kernel_args(args):
+    """Renderer keyword arguments choosing the photon pass kernels."""
+    import sppm_amd
+    kw = {}
+    if args.photon_kernel is not None:
+        kw["photon_kernel"] = (sppm_amd.PHOTON_WAVEFRONT if args.photon_kernel == "wavefront"
+                               else sppm_amd.PHOTON_MEGAKERNEL)
+    if args.wave_bounces:
+        kw["wave_bounces"] = args.wave_bounces
+    if args.wave_lanes:
+        kw["wave_lanes"] = args.wave_lanes
+    return kw
 
 
 def read_traffic(path):
@@ -221,37 +239,77 @@ def reference_image(args, w):
     return ref
 
 
-def steerable_share(w, passes=4):
-    """Of the light the photons bring into the camera's room (x < 1, the
-    occluded-light scene), the share carried by photon paths whose first
-    segment from the light crosses the door (their first stored photon lies
-    in that room): the part an emission proposal can steer.  Per path, c =
-    sum over its stored photons in the room of luminance(flux) / PDF;
-    returned: the steerable share of sum c (mean) and of sum c^2 (the
-    photon variance of a pass, paths being independent), plain emission,
-    fixed radius."""
+def emission_gain_bound(w, mixes=(0.5, 0.25), passes=4):
+    """The largest per-pass variance gain any emission proposal over the
+    262,144 emission cells can give, mixed with the plain emission at weight
+    w, on this workload.
+
+    A photon path's contribution c (the sum, over its stored photons in the
+    camera's room, x < 1 in the occluded-light scene, of luminance(flux) /
+    PDF) depends on its emission cell x only through the cell's plain
+    probability p_x and its second moment m_x = E[c^2 | x].  Emitting with
+    q = w p + (1 - w) g and weighting by p / q, the per-path second moment
+    is sum_x p_x^2 m_x / q_x; over all g it is smallest for q_x = max(w p_x,
+    t sqrt(p_x^2 m_x)) with t normalising q (the constrained optimum, found
+    by bisection).  Gain = (sum p m - mu^2) / (min sum p^2 m / q - mu^2).
+    p and m are measured on plain-emission passes (guided emission with
+    mix_plain = 1 draws plain photons and records each path's cell); the
+    mean share of paths whose first segment crosses the door (their first
+    stored photon lies in the room) is reported beside it."""
     import numpy as np
     import sppm_amd
     r = sppm_amd.Renderer(w["scene"], w["cam"], 1.0, w["r0"], w["N"], bvh_width=8)
-    tot = np.zeros(4)
+    r.guide_set(True, mix_plain=1.0, learning_rate=1e-3, train_steps=1, seed=1, history_decay=0.95, **w["net"])
+    ncell = sppm_amd.GUIDE_CELLS
+    cnt = np.zeros(ncell)
+    s1 = np.zeros(ncell)
+    s2 = np.zeros(ncell)
+    door = np.zeros(2)
     lum = np.array([0.2126, 0.7152, 0.0722])
-    for p in range(passes):
+    for _ in range(passes):
         r.pass_(stats=False)
         counts = r.read_path_counts()
+        cells = r.guide_path_cells()
         ph = r.read_photons()
         path = np.repeat(np.arange(len(counts)), counts)
         first = np.concatenate([[0], np.cumsum(counts)[:-1]])
-        steer_path = ph["pos"][first, 0] < 1.0
         inroom = ph["pos"][:, 0] < 1.0
         val = np.where(inroom, (ph["flux"].astype(np.float64) @ lum) / ph["pdf"].astype(np.float64), 0.0)
         c = np.bincount(path, weights=val, minlength=len(counts))
-        tot += [c[steer_path].sum(), c.sum(), (c[steer_path] ** 2).sum(), (c ** 2).sum()]
+        cnt += np.bincount(cells, minlength=ncell)
+        s1 += np.bincount(cells, weights=c, minlength=ncell)
+        s2 += np.bincount(cells, weights=c * c, minlength=ncell)
+        through = ph["pos"][first, 0] < 1.0
+        door += [c[through].sum(), c.sum()]
     r.close()
-    s, v = tot[0] / tot[1], tot[2] / tot[3]
-    return dict(mean_share=round(float(s), 4), variance_share=round(float(v), 4),
-                gain_cap_mix_0_5=round(0.5 / (1 - v), 3), gain_cap_mix_0_25=round(0.25 / (1 - v), 3),
-                passes=passes,
-                note="variance gain of any emission proposal mixed with weight w is at most w / (1 - variance_share)")
+    n = cnt.sum()
+    p = cnt / n
+    m = np.where(cnt > 0, s2 / np.maximum(cnt, 1), 0.0)
+    mu = s1.sum() / n
+    v_plain = (p * m).sum() - mu * mu
+    a = p * p * m
+    out = dict(paths=int(n), cells_hit=int((cnt > 0).sum()), door_mean_share=round(float(door[0] / door[1]), 4),
+               plain_variance_per_path=float(v_plain), passes=passes)
+    for wmix in mixes:
+        lo, hi = 0.0, 1.0
+        sa = np.sqrt(a)
+        while (np.maximum(wmix * p, hi * sa)).sum() < 1.0:
+            hi *= 2.0
+        for _ in range(100):
+            t = 0.5 * (lo + hi)
+            if np.maximum(wmix * p, t * sa).sum() > 1.0:
+                hi = t
+            else:
+                lo = t
+        q = np.maximum(wmix * p, hi * sa)
+        q /= q.sum()
+        ok = q > 0
+        v_best = (a[ok] / q[ok]).sum() - mu * mu
+        out[f"best_gain_mix_{wmix}"] = round(float(v_plain / v_best), 3)
+    out["note"] = ("per-pass variance gain of the best emission proposal over the cells, for each mixture weight "
+                   "(an upper bound for the learned one); time gain = variance gain x plain pass time / guided "
+                   "pass time")
+    return out
 
 
 def time_to_rmse(args, w, ref):
@@ -320,7 +378,7 @@ def rmse_mode(args):
         return
     res = time_to_rmse(args, w, ref)
     if args.rmse_scene == "occluded":
-        res["steerable"] = steerable_share(w)
+        res["emission_bound"] = emission_gain_bound(w)
     line = {"metric": res["metric"], "value": res["guided_s"], "unit": "s", "higher_is_better": False,
             "n_gpus": 1, "dtype": w["dtype"], "data": "synthetic (seeded procedural scene)",
             "config": {"workload": w["name"]}, "time_to_rmse": res}
@@ -511,8 +569,9 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     scene, cam, cfg = workload(args.workload)
+    pk = photon_kernel_args(args)
     log(f"scene ready: {cfg['name']}, {cfg['triangles']} triangles")
-    r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"])
+    r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"], **pk)
     for _ in range(args.warmup):
         r.pass_(stats=False)
     r.sync()
@@ -546,7 +605,7 @@ def main():
     # region, renders the same passes again with the split steps (one stream,
     # one step after another, HIP events around each) for the phases' own
     # times.
-    r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"])
+    r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"], **pk)
     for p in range(args.warmup + K):
         if p == args.warmup:
             r.sync()
@@ -589,7 +648,8 @@ def main():
         "config": {"workload": cfg["name"], "resolution": f"{cfg['width']}x{cfg['height']}",
                    "photons_per_pass": cfg["photons"], "triangles": cfg["triangles"], "bvh_width": cfg["bvh"],
                    "alpha": cfg["alpha"], "r0": round(cfg["r0"], 6), "camera": cam["type"] and "thin-lens" or "pinhole",
-                   "parallelism": "single GPU"},
+                   "photon_kernel": args.photon_kernel or "default", "wave_bounces": args.wave_bounces or "default",
+                   "wave_lanes": args.wave_lanes or "default", "parallelism": "single GPU"},
         "roofline": {"kernel": rk, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": int(algo[rk]), "dominant_phase": dom,
@@ -618,7 +678,7 @@ def main():
         w = rmse_workload(args)
         log(f"time to RMSE: {w['name']}, {w['scene'].nfacets} triangles, {w['res']}x{w['res']}")
         out["time_to_rmse"] = time_to_rmse(args, w, reference_image(args, w))
-        out["time_to_rmse"]["steerable"] = steerable_share(w)
+        out["time_to_rmse"]["emission_bound"] = emission_gain_bound(w)
     print(json.dumps(out), flush=True)
 
 

@@ -17,6 +17,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 #include "sppm_kernels.h"
@@ -467,6 +468,10 @@ int build_bvh(const float* pos, int n, int width, bool sah, hipStream_t st, BvhR
         snprintf(err, errlen, "empty scene");
         return -1;
     }
+    if (n > RADIX_SORT_MAX_N) {
+        snprintf(err, errlen, "scene of %d triangles: at most %d", n, RADIX_SORT_MAX_N);
+        return -1;
+    }
     HIPCHK(hipMalloc(&tbmin, sizeof(float4) * n));
     HIPCHK(hipMalloc(&tbmax, sizeof(float4) * n));
     HIPCHK(hipMalloc(&cb, sizeof(uint32_t) * 6));
@@ -546,7 +551,11 @@ int build_bvh(const float* pos, int n, int width, bool sah, hipStream_t st, BvhR
         hipLaunchKernelGGL(k_fill_u32, dim3((nint + TB - 1) / TB), dim3(TB), 0, st, (uint32_t*)flag, nint, 0u);
     } else {
         hipLaunchKernelGGL(k_morton, dim3(G), dim3(TB), 0, st, pos, n, cb, keys, vals);
-        radix_sort_pairs(keys, vals, tk, tv, hist, n, 30, st);
+        SortResult sr = radix_sort_pairs(keys, vals, tk, tv, hist, n, 30, st);
+        if (sr.keys != keys) {  // the sorted pairs are in the second buffers
+            std::swap(keys, tk);
+            std::swap(vals, tv);
+        }
         HIPCHK(hipGetLastError());
         if (n > 1) {
             hipLaunchKernelGGL(k_karras, dim3((n - 1 + TB - 1) / TB), dim3(TB), 0, st, keys, n, child, range, parent);

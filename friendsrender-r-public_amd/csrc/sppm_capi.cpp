@@ -26,6 +26,8 @@
 #include "sppm_kernels.h"
 #include "sppm_mtjump.h"
 #include "sppm_pass.h"
+#include "sppm_stripes.h"
+#include "sppm_wave.h"
 
 using namespace sppm;
 namespace H = sppm_host;
@@ -85,7 +87,7 @@ struct SceneBuffers {
     uint32_t* spill = nullptr;  // traversal stack entries beyond TRACE_LDS
     uint32_t* spill_eye = nullptr;  // the same for the eye pass, which runs beside the photon pass
     int spill_blocks = 0;
-    int* next_path = nullptr;   // [0] path counter of the photon pass, [1] pixel counter of the eye pass
+    int* next_path = nullptr;   // striped path counters of the photon pass, then pixel counters of the eye pass
     std::vector<DevLight> hlights;  // host copy of the area lights
     std::vector<float4> h_tuvw, h_tL;  // host copy of the textured lights' 1000-point tables
     float h_lpick_avg = 1.0f;           // TableSampler average of the light pick table
@@ -109,9 +111,9 @@ struct RenderBuffers {
     int* nstored = nullptr;   // device scalar
     int* scan_tmp = nullptr;
     int cell_cap = 0;
-    int* cell_count = nullptr;
+    int4* cell_gaps = nullptr;  // long runs of empty cells of the grid build
+    int* cell_ngap = nullptr;
     int* cell_start = nullptr;
-    int* cell_scan_tmp = nullptr;
     PhotonSortBuffers psort{};  // stable sort of the flattened photons by cell
     float4 *gpos = nullptr, *gflux = nullptr, *gdir = nullptr;
     int photon_cap = 0;
@@ -142,22 +144,35 @@ struct RenderBuffers {
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
+    // closest-hit ray and tie counts, striped (STRIPES lines of RAY_STRIPE
+    // words: photon rays, eye rays, photon ties, eye ties); readers add the
+    // lines' sums to the Counters record
+    unsigned long long* raystripes = nullptr;
     float4* image = nullptr;
     bool have_lists = false;
     // record_neighbours 2: (hit point, photon) pairs appended by the production gather
     int2* pairs = nullptr;
     unsigned long long* npairs = nullptr;
     long long pairs_cap = 0;
+    // wavefront photon pass
+    WaveBuffers wave{};
+    int wave_bounces = 0;      // 0: the megakernel photon pass
+    int wave_lanes = 0;        // lanes per ray of the trace kernel
+    int wave_blocks = 0;       // trace kernel grid
     void release() {
         dfree(pairs); dfree(npairs);
+        dfree(wave.path.rng); dfree(wave.path.org); dfree(wave.path.dir); dfree(wave.path.ndir);
+        dfree(wave.path.rad); dfree(wave.path.thr); dfree(wave.rays[0]); dfree(wave.rays[1]); dfree(wave.hits);
+        dfree(wave.qctl); dfree(wave.spill);
+        wave_bounces = 0;
         pairs_cap = 0;
         dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
-        dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_count); dfree(cell_start); dfree(cell_scan_tmp);
+        dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_gaps); dfree(cell_ngap); dfree(cell_start);
         dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.src); dfree(psort.hist);
         dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(image);
+        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
         ready = false;
     }
 };
@@ -720,7 +735,7 @@ int sppm_scene_upload(sppm_ctx* c, const sppm_scene_desc* d, int bvh_width_flags
         const size_t per_lane = S.stack_cap > TRACE_LDS ? (size_t)(S.stack_cap - TRACE_LDS) : 1;
         int rc;
         if ((rc = dalloc(&S.spill, per_lane * (size_t)S.spill_blocks * 64)) ||
-            (rc = dalloc(&S.spill_eye, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 2))) {
+            (rc = dalloc(&S.spill_eye, per_lane * (size_t)S.spill_blocks * 64)) || (rc = dalloc(&S.next_path, 2 * (size_t)STRIPE_INTS))) {
             S.release();
             return rc;
         }
@@ -776,11 +791,19 @@ int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
     if (!c || !p) return fail(SPPM_E_ARG, "NULL argument");
     if (int rc = not_in_batch(c, "sppm_params_set")) return rc;
     if (p->photons_per_pass <= 0) return fail(SPPM_E_ARG, "photons_per_pass must be > 0");
+    if ((long long)p->photons_per_pass * (NSLOT - 1) > RADIX_SORT_MAX_N)
+        return fail(SPPM_E_ARG, "photons_per_pass %d: at most %d (stored photons are sorted with 30-bit counts)",
+                    p->photons_per_pass, RADIX_SORT_MAX_N / (NSLOT - 1));
     if (!(p->alpha > 0.0f && p->alpha <= 1.0f)) return fail(SPPM_E_ARG, "alpha must be in (0, 1]");
     if (!(p->first_radius > 0.0f)) return fail(SPPM_E_ARG, "first_radius must be > 0");
     if (p->eye_nee < SPPM_NEE_OFF || p->eye_nee > SPPM_NEE_KEEP) return fail(SPPM_E_ARG, "eye_nee must be 0, 1 or 2");
     if (p->record_neighbours < 0 || p->record_neighbours > 2) return fail(SPPM_E_ARG, "record_neighbours must be 0, 1 or 2");
     if (p->step_offset < 0) return fail(SPPM_E_ARG, "step_offset must be >= 0");
+    if (p->photon_kernel != SPPM_PHOTON_WAVEFRONT && p->photon_kernel != SPPM_PHOTON_MEGAKERNEL)
+        return fail(SPPM_E_ARG, "photon_kernel must be SPPM_PHOTON_MEGAKERNEL (0) or SPPM_PHOTON_WAVEFRONT (1)");
+    if (p->wave_bounces < 0 || p->wave_bounces > 32) return fail(SPPM_E_ARG, "wave_bounces must be in [0, 32]");
+    if (p->wave_lanes != 0 && !wave_lanes_ok(p->bvh_width > 0 ? p->bvh_width : 8, p->wave_lanes))
+        return fail(SPPM_E_ARG, "wave_lanes must be 0, 2, 4 or 8 and at most the BVH width");
     if (p->record_neighbours == 2 && p->max_neighbours <= 0)
         return fail(SPPM_E_ARG, "record_neighbours 2 needs max_neighbours > 0 (pairs recorded per pass)");
     c->params = *p;
@@ -845,7 +868,7 @@ int sppm_begin(sppm_ctx* c) {
     const int N = c->params.photons_per_pass;
     const int W = c->cam.width, Hh = c->cam.height, np = W * Hh;
     R.N = N; R.W = W; R.H = Hh; R.np = np;
-    R.photon_cap = (int)std::min<long long>((long long)N * (NSLOT - 1), (1LL << 31) - 1);
+    R.photon_cap = N * (NSLOT - 1);  // checked against RADIX_SORT_MAX_N by sppm_params_set
     R.cell_cap = 1 << 22;
     const size_t NS = (size_t)NSLOT * N;
     int rc = 0;
@@ -853,8 +876,8 @@ int sppm_begin(sppm_ctx* c) {
     if ((rc = dalloc(&R.slots.pos, NS)) || (rc = dalloc(&R.slots.flux, NS)) || (rc = dalloc(&R.slots.dir, NS)) ||
         (rc = dalloc(&R.slots.valid, NS)) || (rc = dalloc(&R.slots.count, N)) || (rc = dalloc(&R.off, N)) ||
         (rc = dalloc(&R.nstored, 1)) || (rc = dalloc(&R.scan_tmp, scan_tmp_entries(N))) ||
-        (rc = dalloc(&R.cell_count, R.cell_cap + 1)) || (rc = dalloc(&R.cell_start, R.cell_cap + 1)) ||
-        (rc = dalloc(&R.cell_scan_tmp, scan_tmp_entries(R.cell_cap + 1))) || (rc = dalloc(&R.psort.keys, R.photon_cap)) ||
+        (rc = dalloc(&R.cell_gaps, cell_gap_entries(R.cell_cap))) || (rc = dalloc(&R.cell_ngap, 1)) ||
+        (rc = dalloc(&R.cell_start, R.cell_cap + 1)) || (rc = dalloc(&R.psort.keys, R.photon_cap)) ||
         (rc = dalloc(&R.psort.vals, R.photon_cap)) || (rc = dalloc(&R.psort.tmp_k, R.photon_cap)) ||
         (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.psort.src, R.photon_cap)) ||
         (rc = dalloc(&R.psort.hist, radix_hist_entries(R.photon_cap))) ||
@@ -867,6 +890,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
         (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_phi, MT_WORDS)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
+        (rc = dalloc(&R.raystripes, (size_t)STRIPES * RAY_STRIPE)) ||
         (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)gather_tile_count(W, Hh))) ||
         (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, gather_tile_count(W, Hh) + 1)) ||
         (rc = dalloc(&R.chunk_off, gather_tile_count(W, Hh) + 1)) ||
@@ -874,13 +898,42 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
-        (rc = dalloc(&R.groups.nvalid, 2)) || (rc = dalloc(&R.groups.stat, 3 * (size_t)gather_group_count(np))) ||
+        (rc = dalloc(&R.groups.nvalid, 1)) || (rc = dalloc(&R.groups.chunkq, STRIPE_INTS)) || (rc = dalloc(&R.groups.stat, 3 * (size_t)gather_group_count(np))) ||
         (rc = dalloc(&R.pass_dev, 1)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
     }
     R.hits.np = np;
+    if (c->params.photon_kernel == SPPM_PHOTON_WAVEFRONT) {
+        const int B = c->params.wave_bounces > 0 ? c->params.wave_bounces : WAVE_BOUNCES_DEFAULT;
+        const int width = c->scene.dev.width;
+        int lanes = c->params.wave_lanes > 0 ? c->params.wave_lanes : WAVE_LANES_DEFAULT;
+        if (lanes > width) lanes = width;
+        if (!wave_lanes_ok(width, lanes)) {
+            R.release();
+            return fail(SPPM_E_ARG, "wave_lanes %d with a BVH of width %d", lanes, width);
+        }
+        R.wave_lanes = lanes;
+        R.wave_blocks = wave_trace_blocks(width, lanes);
+        if (R.wave_blocks <= 0) {
+            R.release();
+            return fail(SPPM_E_HIP, "cannot query the device for the trace kernel grid");
+        }
+        R.wave.cap = c->scene.stack_cap;
+        WaveBuffers& w = R.wave;
+        const size_t n = (size_t)N;
+        if ((rc = dalloc(&w.path.rng, n)) || (rc = dalloc(&w.path.org, n)) || (rc = dalloc(&w.path.dir, n)) ||
+            (rc = dalloc(&w.path.ndir, n)) || (rc = dalloc(&w.path.rad, n)) || (rc = dalloc(&w.path.thr, n)) ||
+            (rc = dalloc(&w.rays[0], 2 * wave_queue_entries(N))) ||
+            (rc = dalloc(&w.rays[1], 2 * wave_queue_entries(N))) || (rc = dalloc(&w.hits, wave_queue_entries(N))) ||
+            (rc = dalloc(&w.qctl, wave_ctl_ints(B))) ||
+            (rc = dalloc(&w.spill, wave_spill_entries(lanes, R.wave_blocks, R.wave.cap)))) {
+            R.release();
+            return rc;
+        }
+        R.wave_bounces = B;
+    }
     if (c->params.record_neighbours == 2) {
         R.pairs_cap = c->params.max_neighbours;
         if ((rc = dalloc(&R.pairs, (size_t)R.pairs_cap)) || (rc = dalloc(&R.npairs, 2))) {
@@ -915,6 +968,7 @@ int sppm_begin(sppm_ctx* c) {
     {
         Counters z{};
         HIPOK(hipMemcpyAsync(R.ctr, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
+        HIPOK(hipMemsetAsync(R.raystripes, 0, sizeof(unsigned long long) * STRIPES * RAY_STRIPE, c->stream));
     }
     HIPOK(hipGetLastError());
     HIPOK(hipStreamSynchronize(c->stream));
@@ -973,22 +1027,45 @@ static TraceSpill trace_spill_eye(const sppm_ctx* c) {
     if (c->tuned_blocks > 0 && c->tuned_blocks < sp.blocks) sp.blocks = c->tuned_blocks;
     return sp;
 }
-// device addresses of the ray counters inside the Counters record
-static unsigned long long* photon_rays(RenderBuffers& R) {
-    return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, photon_rays));
-}
-static unsigned long long* eye_rays(RenderBuffers& R) {
-    return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R.ctr) + offsetof(Counters, eye_rays));
+// device addresses of the striped ray counters (ties two words further)
+static unsigned long long* photon_rays(RenderBuffers& R) { return R.raystripes; }
+static unsigned long long* eye_rays(RenderBuffers& R) { return R.raystripes + 1; }
+// the Counters record with the striped ray and tie counts added
+static int read_counters(sppm_ctx* c, Counters& h) {
+    RenderBuffers& R = c->rb;
+    std::vector<unsigned long long> rs((size_t)STRIPES * RAY_STRIPE);
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&h, R.ctr, sizeof h, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(rs.data(), R.raystripes, sizeof(unsigned long long) * rs.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < STRIPES; ++k) {
+        const unsigned long long* l = rs.data() + (size_t)k * RAY_STRIPE;
+        h.photon_rays += l[0];
+        h.eye_rays += l[1];
+        h.photon_ties += l[2];
+        h.eye_ties += l[3];
+    }
+    return SPPM_OK;
 }
 // beside_gather: the photon pass runs beside the previous pass's gather and
 // takes 9/16 of the path-kernel grid, so the gather's kernels find free wave
 // slots instead of waiting for the persistent photon waves to drain (full
 // grid 5.33 ms per pass, 9/16: 5.02 on the full-material workload).
-static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather = false) {
+static void photon_pass(sppm_ctx* c, int step, const int* step_dev, hipStream_t st, bool beside_gather) {
+    RenderBuffers& R = c->rb;
     TraceSpill sp = trace_spill(c);
     if (beside_gather) sp.blocks = sp.blocks * 9 / 16 > 0 ? sp.blocks * 9 / 16 : 1;
-    launch_photon(c->scene.dev, c->rb.slots, c->rb.N, step, nullptr, sp, c->scene.next_path,
-                  c->guided ? &c->emit : nullptr, c->rb.flags, photon_rays(c->rb), st);
+    if (R.wave_bounces > 0) {
+        int tb = R.wave_blocks;
+        if (beside_gather) tb = tb * 9 / 16 > 0 ? tb * 9 / 16 : 1;
+        launch_photon_wave(c->scene.dev, R.slots, R.N, step, step_dev, R.wave, R.wave_bounces, R.wave_lanes, tb, sp,
+                           c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
+    } else {
+        launch_photon(c->scene.dev, R.slots, R.N, step, step_dev, sp, c->scene.next_path,
+                      c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
+    }
+}
+static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather = false) {
+    photon_pass(c, step, nullptr, st, beside_gather);
 }
 // Training records of the pass (deposit histogram), and on training passes
 // (every train_interval-th step; every replay of a captured pass) Adam steps
@@ -1017,7 +1094,7 @@ static void do_build(sppm_ctx* c, hipStream_t st) {
     if (c->guided) guide_publish(c->guide, st);
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
     launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
-                      R.cell_count, R.cell_start, R.cell_scan_tmp, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
+                      R.cell_start, R.cell_gaps, R.cell_ngap, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
 }
 // Generates camera word blocks up to and including block k on stream2, in
 // stream order (each block continues the shared mt19937 stream).
@@ -1046,7 +1123,7 @@ static int do_eye(sppm_ctx* c, int step, hipStream_t st, Timer* t) {
     }
     if (t) t->start(st);
     launch_eye(c->scene.dev, c->camdev, c->rb.hits, step, nullptr, trace_spill_eye(c),
-               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->scene.next_path + 1,
+               c->rb.words + (size_t)b * 6 * (size_t)c->rb.np, c->rb.flags, eye_rays(c->rb), c->scene.next_path + STRIPE_INTS,
                c->params.eye_nee, st);
     if (t) t->stop(st);
     if (uni) {
@@ -1238,10 +1315,9 @@ static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     HIPOK(hipStreamWaitEvent(st2, c->ev_gfork, 0));
     launch_camera_words(R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words, st2);
     launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev, trace_spill_eye(c), R.words, R.flags, eye_rays(R),
-               c->scene.next_path + 1, c->params.eye_nee, st2);
+               c->scene.next_path + STRIPE_INTS, c->params.eye_nee, st2);
     HIPOK(hipEventRecord(c->ev_gjoin, st2));
-    launch_photon(c->scene.dev, R.slots, R.N, 0, R.pass_dev, trace_spill(c), c->scene.next_path,
-                  c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
+    photon_pass(c, 0, R.pass_dev, st, false);
     do_build(c, st);
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
     int rc = do_gather(c, st);
@@ -1463,8 +1539,7 @@ int sppm_counters(sppm_ctx* c, int64_t* hit_points, int64_t* neighbours, int64_t
     int rc = check_ready(c);
     if (rc) return rc;
     Counters h{};
-    HIPOK(hipStreamSynchronize(c->stream));
-    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if ((rc = read_counters(c, h))) return rc;
     if (hit_points) *hit_points = (int64_t)h.hit_points;
     if (neighbours) *neighbours = (int64_t)h.neighbours;
     if (photons_stored) *photons_stored = (int64_t)h.stored;
@@ -1477,8 +1552,7 @@ int sppm_ray_counts(sppm_ctx* c, int64_t* photon_rays, int64_t* eye_rays) {
     int rc = check_ready(c);
     if (rc) return rc;
     Counters h{};
-    HIPOK(hipStreamSynchronize(c->stream));
-    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if ((rc = read_counters(c, h))) return rc;
     if (photon_rays) *photon_rays = (int64_t)h.photon_rays;
     if (eye_rays) *eye_rays = (int64_t)h.eye_rays;
     return SPPM_OK;
@@ -1488,8 +1562,7 @@ int sppm_tie_counts(sppm_ctx* c, int64_t* photon_ties, int64_t* eye_ties) {
     int rc = check_ready(c);
     if (rc) return rc;
     Counters h{};
-    HIPOK(hipStreamSynchronize(c->stream));
-    HIPOK(hipMemcpy(&h, c->rb.ctr, sizeof h, hipMemcpyDeviceToHost));
+    if ((rc = read_counters(c, h))) return rc;
     if (photon_ties) *photon_ties = (int64_t)h.photon_ties;
     if (eye_ties) *eye_ties = (int64_t)h.eye_ties;
     return SPPM_OK;
@@ -1558,7 +1631,8 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
         {R.slots.pos, sizeof(float4) * NS}, {R.slots.flux, sizeof(float4) * NS}, {R.slots.dir, sizeof(float4) * NS},
         {R.slots.valid, NS}, {R.shared.rad, sizeof(float4) * R.np}, {R.shared.cnt, sizeof(int2) * R.np},
         {R.shared.direct, sizeof(float4) * R.np},
-        {R.rstat, 2 * sizeof(unsigned long long)}, {R.ctr, sizeof(Counters)}, {R.flags, sizeof(uint32_t)},
+        {R.rstat, 2 * sizeof(unsigned long long)}, {R.ctr, sizeof(Counters)},
+        {R.raystripes, sizeof(unsigned long long) * STRIPES * RAY_STRIPE}, {R.flags, sizeof(uint32_t)},
         {R.pass_dev, sizeof(int)}, {R.words, sizeof(uint32_t) * 2 * 6 * (size_t)R.np}};
     if (R.cam_states) v.push_back({R.cam_states, sizeof(uint32_t) * (size_t)R.cam_K * MT_WORDS});
     if (c->guided) {
@@ -1578,7 +1652,7 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
 static CkptHeader ckpt_header(sppm_ctx* c) {
     CkptHeader h{};
     memcpy(h.magic, "SPPMCKPT", 8);
-    h.version = 2;
+    h.version = 3;
     h.setup = setup_fingerprint(c);
     h.N = c->rb.N; h.W = c->rb.W; h.H = c->rb.H; h.np = c->rb.np; h.nslot = NSLOT;
     h.cam_K = c->rb.cam_K; h.cam_L = c->rb.cam_L;
@@ -1625,7 +1699,7 @@ int sppm_checkpoint_load(sppm_ctx* c, const void* buf, int64_t bytes) {
     CkptHeader h;
     memcpy(&h, buf, sizeof h);
     const CkptHeader mine = ckpt_header(c);
-    if (memcmp(h.magic, "SPPMCKPT", 8) || h.version != 2) return fail(SPPM_E_ARG, "not a checkpoint of this format");
+    if (memcmp(h.magic, "SPPMCKPT", 8) || h.version != 3) return fail(SPPM_E_ARG, "not a checkpoint of this format");
     if (h.N != mine.N || h.W != mine.W || h.H != mine.H || h.np != mine.np || h.nslot != mine.nslot ||
         h.cam_K != mine.cam_K || h.cam_L != mine.cam_L || h.guided != mine.guided || h.nparams != mine.nparams)
         return fail(SPPM_E_STATE, "checkpoint was taken with another resolution, photon count, camera sampler or "
@@ -1792,6 +1866,39 @@ int sppm_read_path_counts(sppm_ctx* c, int32_t* counts) {
     if (!counts) return fail(SPPM_E_ARG, "NULL output");
     HIPOK(hipStreamSynchronize(c->stream));
     HIPOK(hipMemcpy(counts, c->rb.slots.count, sizeof(int) * c->rb.N, hipMemcpyDeviceToHost));
+    return SPPM_OK;
+}
+
+int sppm_read_grid(sppm_ctx* c, int32_t* dims, float* geom, int64_t cell_cap, int32_t* cell_start,
+                   int64_t photon_cap, int32_t* order) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    RenderBuffers& R = c->rb;
+    GridParams g{};
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(&g, R.gp, sizeof g, hipMemcpyDeviceToHost));
+    if (dims) {
+        dims[0] = g.nx; dims[1] = g.ny; dims[2] = g.nz; dims[3] = g.ncells; dims[4] = g.nphotons;
+    }
+    if (geom) {
+        geom[0] = g.bmin[0]; geom[1] = g.bmin[1]; geom[2] = g.bmin[2]; geom[3] = g.cell;
+    }
+    if (cell_start) {
+        if (cell_cap < (int64_t)g.ncells + 1) return fail(SPPM_E_ARG, "capacity %lld < %d cell starts",
+                                                          (long long)cell_cap, g.ncells + 1);
+        HIPOK(hipMemcpy(cell_start, R.cell_start, sizeof(int) * ((size_t)g.ncells + 1), hipMemcpyDeviceToHost));
+    }
+    if (order) {
+        if (photon_cap < (int64_t)g.nphotons) return fail(SPPM_E_ARG, "capacity %lld < %d photons",
+                                                          (long long)photon_cap, g.nphotons);
+        std::vector<float4> d((size_t)g.nphotons);
+        if (g.nphotons) HIPOK(hipMemcpy(d.data(), R.gdir, sizeof(float4) * d.size(), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < d.size(); ++j) {
+            int32_t v;
+            memcpy(&v, &d[j].w, sizeof v);  // the record carries its flattened index in dir.w
+            order[j] = v;
+        }
+    }
     return SPPM_OK;
 }
 
@@ -2125,6 +2232,19 @@ int sppm_guide_tables(sppm_ctx* c, float* prob, float* cdf) {
     HIPOK(hipStreamSynchronize(c->stream));
     if (prob) HIPOK(hipMemcpy(prob, c->guide.eprob, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
     if (cdf) HIPOK(hipMemcpy(cdf, c->guide.ecdf, sizeof(float) * GUIDE_CELLS, hipMemcpyDeviceToHost));
+    return SPPM_OK;
+}
+
+int sppm_guide_path_cells(sppm_ctx* c, int32_t* cells) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    if (!cells) return fail(SPPM_E_ARG, "NULL output");
+    if (c->pass < 1) return fail(SPPM_E_STATE, "no pass rendered since sppm_begin");
+    // the emission of step s writes block s & 1; the last pass ran step pass + step_offset
+    const long long step = (long long)c->pass + c->params.step_offset;
+    HIPOK(hipStreamSynchronize(c->stream));
+    HIPOK(hipMemcpy(cells, c->rb.path_cell + (size_t)(step & 1) * (size_t)c->rb.N, sizeof(int) * (size_t)c->rb.N,
+                    hipMemcpyDeviceToHost));
     return SPPM_OK;
 }
 

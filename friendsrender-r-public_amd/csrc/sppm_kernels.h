@@ -8,10 +8,17 @@
 namespace sppm {
 
 // ---- sort / scan (sppm_sort.hip) ------------------------------------------
-// stable; n_dev (optional): sort only the first *n_dev <= n pairs
-void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* hist, int n,
-                      int bits, hipStream_t st, const int* n_dev = nullptr);
-size_t radix_hist_entries(int n);
+// Stable sort by the low `bits` bits; n_dev (optional): sort only the first
+// *n_dev <= n pairs.  n < 2^30.  The result is in keys/vals or in tmp_k/tmp_v
+// (an odd number of 8-bit digit passes): SortResult names which.
+struct SortResult {
+    uint32_t* keys;
+    uint32_t* vals;
+};
+SortResult radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* work, int n,
+                            int bits, hipStream_t st, const int* n_dev = nullptr);
+constexpr int RADIX_SORT_MAX_N = (1 << 30) - 1;
+size_t radix_hist_entries(int n);  // words of the sort's work area
 __global__ void k_tile_total_prep(int* tmp, int nt);
 __global__ void k_scan_small(uint32_t* data, int n);
 void exclusive_scan(const int* in, int* out, int n, int* total, int* tmp, hipStream_t st);

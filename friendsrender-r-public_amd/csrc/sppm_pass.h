@@ -81,10 +81,14 @@ struct PhotonSortBuffers {
     uint32_t *keys, *vals, *tmp_k, *tmp_v, *src, *hist;
 };
 constexpr int PHOTON_KEY_BITS = 22;  // cell keys are below the cell capacity 2^22
+// Photon grid: cell size, keys, stable sort by cell, cell starts
+// (cell_cap + 1 entries; gaps: cell_gap_entries(cell_cap); ngap: 1 int) and
+// the cell-sorted records.
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
-                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
-                       int* cell_start, int* scan_tmp, const PhotonSortBuffers& sb, int photon_cap, float4* gpos,
-                       float4* gflux, float4* gdir, hipStream_t st);
+                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap,
+                       int* cell_start, int4* gaps, int* ngap, const PhotonSortBuffers& sb, int photon_cap,
+                       float4* gpos, float4* gflux, float4* gdir, hipStream_t st);
+size_t cell_gap_entries(int cell_cap);
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,
                         hipStream_t st);
 void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int W, int Hh,
@@ -94,7 +98,8 @@ struct TileRange;
 // Sort buffers of the hit-point groups (np entries each; hist as radix_hist_entries(np)).
 struct GroupBuffers {
     uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
-    int* nvalid;  // [0] valid hit points, [1] chunk queue of the gather
+    int* nvalid;  // valid hit points
+    int* chunkq;  // striped chunk counters of the gather (STRIPE_INTS)
     unsigned long long* stat;  // 3 per group: radius sum, active hit points, neighbours
 };
 // Neighbour pairs (hit point, flattened photon index) that the production

@@ -7,9 +7,11 @@
 //                    a time, writes slots
 //   scan(count)      flatten offsets (SPPM.cpp:812-822 order)
 //   k_grid_params    cell size from the mean radius (device-side)
-//   k_photon_keys    per flattened photon: cell key + rank (atomic count)
-//   scan(cells)      cell starts
-//   k_photon_scatter cell-sorted photon records (kdtree::CreateTree replacement)
+//   k_photon_keys    per flattened photon: cell key, flattened index, slot
+//   radix sort       stable, by cell key
+//   k_cell_starts    cell starts from the sorted keys (k_fill_gaps: long
+//                    runs of empty cells)
+//   k_photon_sorted  cell-sorted photon records (kdtree::CreateTree replacement)
 //   k_eye<W>         createHitpoints: one lane per pixel
 //   k_hit_keys ... k_gather_finish, k_group_stats
 //                    applyHitPointInfo + progressive update over Morton
@@ -22,6 +24,8 @@
 #include "sppm_guide.h"
 #include "sppm_kernels.h"
 #include "sppm_pass.h"
+#include "sppm_photon.h"
+#include "sppm_stripes.h"
 #include "sppm_trace.h"
 
 namespace sppm {
@@ -117,331 +121,12 @@ __global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restr
     }
 }
 
-// ---------------------------------------------------------------------------
-// Alias-table draw: TableSampler::Sample(RandomMT&) (TableSampler.h:126-129)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int table_sample(const float* __restrict__ prob, const int* __restrict__ left,
-                                            const int* __restrict__ right, int n, MTCursor& m, uint32_t* ovf) {
-    double r2 = real1(m, ovf);
-    int bin = (int)((double)n * r2);
-    if (bin >= n) bin = n - 1;  // unreachable: r2 < 1
-    double r1 = real1(m, ovf);
-    int id = (r1 < (double)prob[bin]) ? left[bin] : right[bin];
-    return id < 0 ? left[bin] : id;
-}
-
-__device__ __forceinline__ bool near1(float s) { return fabs_(s - 1.0f) < HI_1EM5; }  // SPPM.h:27-29, (double)|s-1| < 0.00001
-
-// ---------------------------------------------------------------------------
-// createPhotonMap (SPPM.cpp:518-672), one lane per photon path
-// ---------------------------------------------------------------------------
-// One photon path in flight (createPhotonMap loop state, SPPM.cpp:495-680).
-struct PhotonPath {
-    MTCursor m;
-    f3 rad, T, cur_org, cur_dir, next_dir;
-    float cur_index, next_index, PDF;
-    int i, pidx, loopcount, depth;
-};
-
-// Adds a per-lane count to a 64-bit device counter, one atomic per wave.
-__device__ __forceinline__ void add_wave_count(unsigned long long* __restrict__ ctr, uint32_t v) {
-    unsigned long long x = v;
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd(ctr, x);
-}
-
-// LightSampler::SamplePointOnAreaLight (LightSampler.cpp:68-82): a point of
-// light L, its radiance and its area density.
-__device__ __forceinline__ void light_point(const SceneDev& S, const DevLight& L, MTCursor& m, uint32_t* ovf,
-                                            f3& lpos, f3& rad, float& pdf_area) {
-    f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
-    if (L.textured) {  // LightSampler.cpp:70-76
-        const int off = L.tab_off;
-        int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, m, ovf);
-        float4 w = S.ltab_uvw[off + k];
-        float4 Lk = S.ltab_L[off + k];
-        lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
-        rad = mk(Lk.x, Lk.y, Lk.z);
-        float gp = w.w / L.tab_avg / 1000.0f;
-        pdf_area = (gp * 1000.0f) / L.area;
-    } else {  // PointOnTriangle (Sampling.cpp:339-349)
-        float u = (float)real1(m, ovf);
-        float v = (float)real1(m, ovf);
-        if (v > u) { u *= 0.5f; v -= u; }
-        else { v *= 0.5f; u -= v; }
-        pdf_area = 2.0f / fabs_(norm(cross(p1 - p0, p2 - p0)));
-        lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
-        rad = mk(L.flux[0], L.flux[1], L.flux[2]) / (2.0f * SPPM_PI * L.area);
-    }
-}
-
-// LightSampler::GetAreaLightPDF (LightSampler.h:87): TableSampler::GetPDF of
-// the light pick table, f / average / N.
-__device__ __forceinline__ float light_pick_pdf(const SceneDev& S, const DevLight& L) {
-    return L.flux_norm / S.lpick_avg / (float)S.nlights;
-}
-
-// generateLightRay (SPPM.cpp:453-471) + the emission photon in slot 0
-// (SPPM.cpp:530-540); the first stored surface photon overwrites that slot
-// (SPPM.cpp:606), so writing it up front leaves the same slot contents.
-__device__ __forceinline__ void photon_start(const SceneDev& S, const SlotsDev& slots, int N, int step, int i,
-                                             PhotonPath& P, uint32_t* ovf) {
-    P.i = i;
-    mt_seed(P.m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
-    const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
-    const DevLight& L = S.lights[li];
-    f3 lpos, rad;
-    float pdf_area;
-    light_point(S, L, P.m, ovf, lpos, rad, pdf_area);
-    pdf_area *= light_pick_pdf(S, L);
-    const f3 lnorm = mk(L.normal[0], L.normal[1], L.normal[2]);
-    float pdf_dir = 1.0f;
-    f3 outdir = cos_sampling(P.m, ovf, lnorm, &pdf_dir);
-    rad = rad * smin(1.0f, smax(0.0f, dot(lnorm, outdir)));
-    P.rad = rad;
-    P.cur_org = lpos + lnorm * SPPM_RAY_OFFSET;
-    P.cur_dir = normalized(outdir);  // Ray ctor normalises
-    P.cur_index = 1.0f;
-    P.next_dir = P.cur_dir;
-    P.next_index = 1.0f;
-    P.PDF = pdf_area * pdf_dir;
-    P.T = mk(1.0f, 1.0f, 1.0f);
-    P.pidx = 0;
-    P.loopcount = 0;
-    P.depth = 0;
-    const f3 e_flux = (rad * P.T) * SPPM_PI;
-    slots.pos[i] = make_float4(P.cur_org.x, P.cur_org.y, P.cur_org.z, 1.0f);
-    slots.flux[i] = make_float4(e_flux.x, e_flux.y, e_flux.z, P.PDF);
-    slots.dir[i] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
-    slots.valid[i] = 1;
-}
-
-// Guided emission: draw from the mixture, evaluate the mixture density at the
-// drawn (position, direction) and use it as the photon's PDF; the photon
-// flux is L_e cos, as for plain emission, so flux / PDF stays an unbiased
-// weight whichever component drew the sample.
-//
-// Textured light: positions are the table points.  Point k has plain
-// probability pi_k and the plain PDF D_k * cos / pi, where D_k = pi_k * 1000 /
-// area is the reference's area density of the point (LightSampler.cpp:70-76);
-// the proposal picks a cell c, then a point of c's position cell with
-// probability pi_k / mass(c).  The PDF is D_k * (w cos / pi + (1 - w) P(c)
-// d(omega | c) / mass(c)), the mixture's probability of (k, omega) over pi_k
-// times D_k, which keeps the plain emission's expectation.
-__device__ __forceinline__ int cdf_search(const float* __restrict__ cdf, int lo, int hi, float x) {
-    while (lo < hi) {  // first entry > x in [lo, hi]
-        const int mid = (lo + hi) >> 1;
-        if (cdf[mid] > x) hi = mid; else lo = mid + 1;
-    }
-    return lo;
-}
-__device__ __forceinline__ void photon_start_guided(const SceneDev& S, const SlotsDev& slots, const GuideEmit& G,
-                                                    int N, int step, int i, PhotonPath& P, uint32_t* ovf) {
-    P.i = i;
-    mt_seed(P.m, (uint32_t)i + 135u + (uint32_t)step * (uint32_t)N);
-    const f3 P0 = mk(G.P0[0], G.P0[1], G.P0[2]), E1 = mk(G.e1[0], G.e1[1], G.e1[2]), E2 = mk(G.e2[0], G.e2[1], G.e2[2]);
-    const f3 Nn = mk(G.n[0], G.n[1], G.n[2]), T1 = mk(G.t1[0], G.t1[1], G.t1[2]), T2 = mk(G.t2[0], G.t2[1], G.t2[2]);
-    const float total = G.cdf[GUIDE_CELLS - 1];
-    f3 lpos, outdir, rad0;
-    int tj = -1, tl = 0;  // textured: table entry and light of the point
-    if (real1(P.m, ovf) < (double)G.w) {
-        // plain: light pick, then PointOnTriangle or a table point, cosine lobe (SPPM.cpp:453-468)
-        const int li = table_sample(S.lpick_prob, S.lpick_left, S.lpick_right, S.nlights, P.m, ovf);
-        const DevLight& L = S.lights[li];
-        f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
-        if (G.textured) {
-            const int off = L.tab_off;
-            const int k = table_sample(S.ltab_prob + off, S.ltab_left + off, S.ltab_right + off, 1000, P.m, ovf);
-            const float4 w = S.ltab_uvw[off + k];
-            lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
-            tj = off + k;
-            tl = li;
-        } else {
-            float u = (float)real1(P.m, ovf);
-            float v = (float)real1(P.m, ovf);
-            if (v > u) { u *= 0.5f; v -= u; }
-            else { v *= 0.5f; u -= v; }
-            lpos = (p0 * u + p1 * v) + p2 * (1.0f - u - v);
-        }
-        float pdf_dir = 1.0f;
-        outdir = normalized(cos_sampling(P.m, ovf, Nn, &pdf_dir));
-    } else {
-        // proposal: a cell by its CDF, then a position and a direction inside it
-        // x < total strictly, so the cell drawn has probability > 0 (masked cells have none)
-        const float x = fminf((float)real1(P.m, ovf) * total, __uint_as_float(__float_as_uint(total) - 1u));
-        const int c = cdf_search(G.cdf, 0, GUIDE_CELLS - 1, x);
-        const int it = c % GUIDE_GT, is = (c / GUIDE_GT) % GUIDE_GS, iv = (c / (GUIDE_GS * GUIDE_GT)) % GUIDE_GV,
-                  iu = c / (GUIDE_GV * GUIDE_GS * GUIDE_GT);
-        if (G.textured) {
-            // a table point of the position cell, by the plain probabilities
-            const int pc = iu * GUIDE_GV + iv;
-            const int a = G.pc_start[pc], e = G.pc_start[pc + 1];
-            const float x = (float)real1(P.m, ovf) * G.pcell_mass[pc];
-            const int q = e > a ? cdf_search(G.pc_cum, a, e - 1, x) : a;
-            const int2 pt = G.pc_pt[q];
-            const DevLight& L = S.lights[pt.x];
-            f3 p0 = mk(L.p[0], L.p[1], L.p[2]), p1 = mk(L.p[3], L.p[4], L.p[5]), p2 = mk(L.p[6], L.p[7], L.p[8]);
-            const float4 w = S.ltab_uvw[pt.y];
-            lpos = (p0 * w.x + p1 * w.y) + p2 * (1.0f - w.x - w.y);
-            tj = pt.y;
-            tl = pt.x;
-        } else {
-            const float u = ((float)iu + (float)real1(P.m, ovf)) / (float)GUIDE_GU;
-            const float v = ((float)iv + (float)real1(P.m, ovf)) / (float)GUIDE_GV;
-            lpos = (P0 + E1 * u) + E2 * v;
-        }
-        const float s = -1.0f + ((float)is + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GS);
-        const float t = -1.0f + ((float)it + (float)real1(P.m, ovf)) * (2.0f / (float)GUIDE_GT);
-        const float px = 0.5f * (s + t), py = 0.5f * (s - t);
-        const float pz = fmaxf(0.0f, 1.0f - fabs_(px) - fabs_(py));
-        outdir = normalized((T1 * px + T2 * py) + Nn * pz);
-    }
-    // mixture density at (lpos, outdir)
-    const float lx = dot(outdir, T1), ly = dot(outdir, T2), lz = dot(outdir, Nn);
-    const float cs = fmaxf(lz, 0.0f);
-    const float l1 = fabs_(lx) + fabs_(ly) + cs;
-    const float ox = lx / l1, oy = ly / l1, oz = cs / l1;  // octahedron point (|ox|+|oy|+oz = 1)
-    const float s = ox + oy, t = ox - oy;
-    const int is = min(max((int)((s + 1.0f) * (0.5f * (float)GUIDE_GS)), 0), GUIDE_GS - 1);
-    const int it = min(max((int)((t + 1.0f) * (0.5f * (float)GUIDE_GT)), 0), GUIDE_GT - 1);
-    int pcell;
-    if (G.textured) {
-        pcell = G.pt_cell[tj];
-    } else {
-        const f3 rel = lpos - P0;
-        const float u = fminf(fmaxf(dot(rel, E1) * G.inv_e1, 0.0f), 0.99999994f);
-        const float v = fminf(fmaxf(dot(rel, E2) * G.inv_e2, 0.0f), 0.99999994f);
-        const int iu = min((int)(u * (float)GUIDE_GU), GUIDE_GU - 1), iv = min((int)(v * (float)GUIDE_GV), GUIDE_GV - 1);
-        pcell = iu * GUIDE_GV + iv;
-    }
-    const int cell = (pcell * GUIDE_GS + is) * GUIDE_GT + it;
-    const float plen = sqrtf(ox * ox + oy * oy + oz * oz);
-    // direction density inside a cell: 1 / (cell area in (s, t)) * 2 |P|^3 (d omega = ds dt / (2 |P|^3))
-    const float dd = (float)(GUIDE_GS * GUIDE_GT / 4) * (2.0f * plen * plen * plen);
-    const float pc_prob = G.prob[cell] / total;
-    float q;
-    if (G.textured) {
-        const DevLight& L = S.lights[tl];
-        const float4 w = S.ltab_uvw[tj];
-        const float4 Lk = S.ltab_L[tj];
-        // D_k as the plain emission computes it (photon_start)
-        const float gp = w.w / L.tab_avg / 1000.0f;
-        const float Dk = ((gp * 1000.0f) / L.area) * (L.flux_norm / S.lpick_avg / (float)S.nlights);
-        const float mass = G.pcell_mass[pcell];
-        const float prop = mass > 0.0f ? pc_prob * dd / mass : 0.0f;
-        q = Dk * (G.w * cs / SPPM_PI + (1.0f - G.w) * prop);
-        rad0 = mk(Lk.x, Lk.y, Lk.z);
-    } else {
-        // cell density in (u, v, s, t) -> area x solid angle: / A
-        const float p_mlp = pc_prob * (float)(GUIDE_GU * GUIDE_GV) * G.inv_area * dd;
-        const float p_plain = G.inv_area * cs / SPPM_PI;
-        q = G.w * p_plain + (1.0f - G.w) * p_mlp;
-        rad0 = mk(G.rad[0], G.rad[1], G.rad[2]);
-    }
-    G.path_cell[(size_t)(step & 1) * (size_t)N + i] = cell;
-    const f3 rad = rad0 * smin(1.0f, cs);
-    P.rad = rad;
-    P.cur_org = lpos + Nn * SPPM_RAY_OFFSET;
-    P.cur_dir = outdir;
-    P.cur_index = 1.0f;
-    P.next_dir = P.cur_dir;
-    P.next_index = 1.0f;
-    P.PDF = q;
-    P.T = mk(1.0f, 1.0f, 1.0f);
-    P.pidx = 0;
-    P.loopcount = 0;
-    P.depth = 0;
-    const f3 e_flux = (rad * P.T) * SPPM_PI;
-    slots.pos[i] = make_float4(P.cur_org.x, P.cur_org.y, P.cur_org.z, 1.0f);
-    slots.flux[i] = make_float4(e_flux.x, e_flux.y, e_flux.z, P.PDF);
-    slots.dir[i] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
-    slots.valid[i] = 1;
-}
-
-// One iteration of the createPhotonMap loop body; false when the path ends.
-template <int W>
-__device__ __forceinline__ bool photon_bounce(const SceneDev& S, const SlotsDev& slots, PhotonPath& P,
-                                              const TraceStack& ts, uint32_t* ovf, uint32_t* fl, uint32_t& nray,
-                                              uint32_t& ntie) {
-    if (P.loopcount++ > 32 || P.pidx >= 31) return false;
-    if (P.T.x < HI_1EM8 || P.T.y < HI_1EM8 || P.T.z < HI_1EM8) return false;  // (double)T < 1e-8
-    HitRec h;
-    ++nray;
-    if (!closest_hit<W>(S, P.cur_org, P.cur_dir, SPPM_RAY_OFFSET, ts, h, fl)) return false;
-    ntie += h.tie ? 1u : 0u;
-    Geom g;
-    calc_geometry(S, h, g);
-    const f3 Nf = dot(g.N, P.cur_dir) > 0 ? -g.N : g.N;
-    f3 next_org = g.pos;
-    if (norm2(g.radiance) > LO_1EM5) return false;  // (double)norm2 > 0.00001
-    if (g.transl > 0.5f) {
-        P.T = P.T * g.transl;
-        P.cur_org = g.pos + P.cur_dir * SPPM_RAY_OFFSET;
-        return true;
-    }
-    if (P.depth > 16) return false;  // russianRulette_PhotonMapping (SPPM.h:33-52)
-    float a0 = g.albedo.x, a1 = g.albedo.y, a2 = g.albedo.z;
-    float pdf_rr = 0.98f * (a0 > a1 ? (a0 > a2 ? a0 : a2) : (a1 > a2 ? a1 : a2));
-    if (real1(P.m, ovf) > (double)pdf_rr) return false;
-    P.PDF *= pdf_rr;
-    float Ii, Io = 1.0f;
-    if (near1(P.cur_index)) { Ii = 1.0f; Io = g.index; }
-    else { Ii = P.cur_index; Io = 1.0f; }
-    FParam fp{P.cur_dir, P.next_dir, Nf, Ii, Io, g.rough, g.absorb, g.refl};
-    if (g.is_sppm) {
-        if (P.pidx >= NSLOT - 2) { *fl |= 8u; return false; }
-        f3 ph_flux = (P.rad * P.T) * g.albedo;
-        if (!(P.PDF > 0.0f) || !isfinite(ph_flux.x) || !isfinite(ph_flux.y) || !isfinite(ph_flux.z) ||
-            !isfinite(P.PDF))
-            *fl |= 16u;  // stored photon with pdf <= 0 or non-finite flux (propagated as the reference does)
-        const size_t o = (size_t)P.pidx * (size_t)slots.npaths + P.i;
-        slots.pos[o] = make_float4(g.pos.x, g.pos.y, g.pos.z, P.cur_index);
-        slots.flux[o] = make_float4(ph_flux.x, ph_flux.y, ph_flux.z, P.PDF);
-        slots.dir[o] = make_float4(P.cur_dir.x, P.cur_dir.y, P.cur_dir.z, 0.0f);
-        slots.valid[o] = 1;
-        ++P.pidx;
-    }
-    float pdf_s = 1.0f;
-    P.next_dir = surf_sample(g.kind, fp, P.m, ovf, &pdf_s);
-    if (!rutil_valid_dir(P.next_dir)) return false;
-    if (g.opaque) {
-        if (dot(P.next_dir, Nf) < HI_1EM6) return false;  // (double)dot < 1e-6
-    } else {
-        P.next_index = rutil_is_reflect(P.cur_dir, P.next_dir, Nf) ? P.cur_index : Io;
-    }
-    fp.out = P.next_dir;
-    pdf_s = surf_pdf(g.kind, fp);
-    f3 outN = dot(Nf, P.next_dir) > 0 ? Nf : -Nf;
-    next_org = next_org + outN * SPPM_RAY_OFFSET;
-    float brdf = surf_brdf(g.kind, fp);
-    float cs = smax(0.0f, smin(1.0f, dot(P.next_dir, outN)));
-    if (brdf < HI_1EM8) return false;  // (double)brdf < 1e-8
-    if (pdf_s < 0) return false;
-    P.T = P.T * ((g.albedo * cs) * brdf);
-    P.PDF *= pdf_s;
-    P.cur_org = next_org;
-    P.cur_dir = P.next_dir;
-    P.cur_index = P.next_index;
-    ++P.depth;
-    return true;
-}
-
-// End of a path: terminator at slot pidx + 1 (SPPM.cpp:672); slot pidx keeps
-// its content from an earlier pass and is flattened if valid (SPPM.cpp:812-822).
-__device__ __forceinline__ void photon_finish(const SlotsDev& slots, const PhotonPath& P) {
-    const size_t NP = (size_t)slots.npaths;
-    const int cnt = P.pidx == 0 ? 1 : P.pidx + (slots.valid[(size_t)P.pidx * NP + P.i] ? 1 : 0);
-    slots.valid[(size_t)(P.pidx + 1) * NP + P.i] = 0;
-    slots.count[P.i] = cnt;
-}
-
 // Persistent photon pass: each lane traces one path at a time; when at least
-// PHOTON_REGEN lanes of a wave have finished their paths, the wave takes the
-// next path indices from a global counter (one atomic per wave), so lanes do
-// not idle behind the longest path of their wave.  A path's result depends
+// PHOTON_REGEN lanes of a wave have finished their paths, the wave claims the
+// next path indices from the striped path counters (one atomic per claim),
+// so lanes do not idle behind the longest path of their wave.  A path's result depends
 // only on its index, not on the lane or order that traces it.  Every wave
 // exits once the counter has passed N and its lanes are done.
-constexpr int PHOTON_REGEN = 16;
 template <int W, bool GUIDED>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
                                                        const int* __restrict__ step_dev, int cap,
@@ -457,23 +142,23 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
     uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
     PhotonPath P;
     bool alive = false, exhausted = false;
+    StripeCursor cur = stripe_start();
     for (;;) {
         const unsigned long long idle = __ballot(!alive);
         const int nidle = __popcll(idle);
         if (!exhausted && nidle >= PHOTON_REGEN) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(next_path, nidle);
-            base = __shfl(base, leader, 64);
+            int first = 0;
+            const int got = stripe_claim(next_path, N, cur, nidle, &first);
+            if (got == 0) exhausted = true;
             if (!alive) {
-                const int i = base + __popcll(idle & below);
-                if (i < N) {
+                const int r = __popcll(idle & below);
+                if (r < got) {
+                    const int i = first + r;
                     if constexpr (GUIDED) photon_start_guided(S, slots, G, N, step, i, P, &ovf);
                     else photon_start(S, slots, N, step, i, P, &ovf);
                     alive = true;
                 }
             }
-            if (base + nidle >= N) exhausted = true;
         }
         if (__ballot(alive) == 0ull) {
             if (exhausted) break;
@@ -498,7 +183,8 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
 // ---------------------------------------------------------------------------
 __global__ void k_grid_params(GridParams* gp, const unsigned long long* __restrict__ rstat, float bmin_x, float bmin_y,
                               float bmin_z, float ext_x, float ext_y, float ext_z, int cell_cap,
-                              const int* __restrict__ nstored, Counters* __restrict__ ctr) {
+                              const int* __restrict__ nstored, Counters* __restrict__ ctr, int* __restrict__ ngap) {
+    *ngap = 0;
     // cell size = mean radius of the hit points after the previous gather;
     // any cell size gives the same neighbour sets (queries walk every cell
     // their box overlaps), this one keeps ~3 cells per query axis.
@@ -537,11 +223,9 @@ __global__ void k_zero_i32(int* __restrict__ p, int n) {
 }
 
 // Per path: the cell key, flattened index and slot address of each of its
-// flattened photons, and the cell counts (atomic adds; the totals do not
-// depend on their order).
+// flattened photons.
 __global__ void __launch_bounds__(256) k_photon_keys(SlotsDev slots, const int* __restrict__ off,
-                                                     const GridParams* __restrict__ gp, int* __restrict__ cell_count,
-                                                     PhotonSortBuffers sb) {
+                                                     const GridParams* __restrict__ gp, PhotonSortBuffers sb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots.npaths) return;
     const GridParams g = *gp;
@@ -555,23 +239,57 @@ __global__ void __launch_bounds__(256) k_photon_keys(SlotsDev slots, const int* 
         int cy = cell_coord(p.y, g.bmin[1], g.inv_cell, g.ny);
         int cz = cell_coord(p.z, g.bmin[2], g.inv_cell, g.nz);
         int key = (cz * g.ny + cy) * g.nx + cx;
-        atomicAdd(&cell_count[key], 1);
         sb.keys[base + s] = (uint32_t)key;
         sb.vals[base + s] = (uint32_t)(base + s);
         sb.src[base + s] = (uint32_t)o;
     }
 }
 
+// Cell starts from the cell-sorted keys: sorted position i starts every cell
+// in (keys[i-1], keys[i]] (position n the cells after the last key, up to
+// ncells).  Runs of up to CELL_RUN_INLINE cells are written by the position's
+// own thread, longer ones are queued for k_fill_gaps.  Every entry of
+// cell_start[0..ncells] is written exactly once.
+constexpr int CELL_RUN_INLINE = 64;
+__global__ void __launch_bounds__(256) k_cell_starts(const uint32_t* __restrict__ keys,
+                                                     const int* __restrict__ nstored,
+                                                     const GridParams* __restrict__ gp, int* __restrict__ cell_start,
+                                                     int4* __restrict__ gaps, int* __restrict__ ngap) {
+    const int n = *nstored;
+    const int nc = gp->ncells;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+        const int a = i > 0 ? (int)keys[i - 1] : -1;
+        const int b = i < n ? (int)keys[i] : nc;
+        if (b <= a) continue;
+        if (b - a <= CELL_RUN_INLINE) {
+            for (int c = a + 1; c <= b; ++c) cell_start[c] = i;
+        } else {
+            const int q = atomicAdd(ngap, 1);
+            gaps[q] = make_int4(a + 1, b, i, 0);
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k_fill_gaps(const int4* __restrict__ gaps, const int* __restrict__ ngap,
+                                                   int* __restrict__ cell_start) {
+    const int ng = *ngap;
+    for (int g = blockIdx.x; g < ng; g += gridDim.x) {
+        const int4 r = gaps[g];
+        for (int c = r.x + (int)threadIdx.x; c <= r.y; c += blockDim.x) cell_start[c] = r.z;
+    }
+}
+size_t cell_gap_entries(int cell_cap) { return (size_t)(cell_cap + 1) / CELL_RUN_INLINE + 2; }
+
 // Cell-sorted photon records: after the stable sort by cell key, position j
 // holds flattened photon vals[j], so photons of a cell are in flattened
 // (reference) order and every run lays them out the same way.  The record
 // carries its flattened index in dir.w.
 __global__ void __launch_bounds__(256) k_photon_sorted(SlotsDev slots, const int* __restrict__ nstored,
-                                                       PhotonSortBuffers sb, float4* __restrict__ gpos,
-                                                       float4* __restrict__ gflux, float4* __restrict__ gdir) {
+                                                       const uint32_t* __restrict__ order, PhotonSortBuffers sb,
+                                                       float4* __restrict__ gpos, float4* __restrict__ gflux,
+                                                       float4* __restrict__ gdir) {
     const int n = *nstored;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t flat = sb.vals[j];
+        const uint32_t flat = order[j];
         const size_t o = sb.src[flat];
         float4 d = slots.dir[o];
         d.w = __int_as_float((int)flat);
@@ -889,13 +607,12 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDe
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int NPIX = C.W * C.H;
     uint32_t fl = 0, nray = 0, ntie = 0;
+    StripeCursor cur = stripe_start();
     for (;;) {
-        int base = 0;
-        if ((lane & 63) == 0) base = atomicAdd(queue, 64);
-        base = __shfl(base, 0, 64);
-        if (base >= NPIX) break;
-        const int i = base + (lane & 63);
-        if (i < NPIX) eye_path<W, NEE>(S, C, H, step, i, words, ts, fl, keep != 0, nray, ntie);
+        int first = 0;
+        const int got = stripe_claim(queue, NPIX, cur, 64, &first);
+        if (got == 0) break;
+        if ((lane & 63) < got) eye_path<W, NEE>(S, C, H, step, first + (lane & 63), words, ts, fl, keep != 0, nray, ntie);
     }
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
@@ -1287,15 +1004,14 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
     const int CH = *chunkp;
     const int nvalid = *nvalidp;
     const GridParams g = *gp;
-    // chunks are claimed from a counter (zeroed before the launch); the
-    // counter only grows, so every wave ends once it passes the total.  The
-    // partial sums go to the chunk's own slot, so the result does not depend
-    // on which wave took which chunk.
+    // chunks are claimed through the striped counters (zeroed before the
+    // launch); every wave ends once every stripe is exhausted.  The partial
+    // sums go to the chunk's own slot, so the result does not depend on which
+    // wave took which chunk.
+    StripeCursor cur = stripe_start();
     for (;;) {
         int ch = 0;
-        if (lane == 0) ch = atomicAdd(queue, 1);
-        ch = __shfl(ch, 0, 64);
-        if (ch >= total) break;
+        if (stripe_claim(queue, total, cur, 1, &ch) == 0) break;
         // owning tile: last t with chunk_off[t] <= ch
         int lo = 0, hi = ntiles;
         while (hi - lo > 1) {
@@ -1634,7 +1350,7 @@ void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
                    const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags,
                    unsigned long long* rays, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, next_path, 1);
+    hipLaunchKernelGGL(k_zero_i32, dim3(STRIPE_INTS / 256), dim3(256), 0, st, next_path, STRIPE_INTS);
     dim3 g(sp.blocks), b(PATH_BLOCK);
     const GuideEmit G = guide ? *guide : GuideEmit{};
 #define SPPM_PHOTON(WW, GG) \
@@ -1656,7 +1372,7 @@ template <bool NEE>
 static void launch_eye_t(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                          const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays,
                          int* queue, int keep, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, queue, 1);
+    hipLaunchKernelGGL(k_zero_i32, dim3(STRIPE_INTS / 256), dim3(256), 0, st, queue, STRIPE_INTS);
     dim3 g(sp.blocks), b(PATH_BLOCK);
     if (S.width == 2)
         hipLaunchKernelGGL((k_eye<2, NEE>), g, b, 0, st, S, C, H, step, step_dev, sp.cap, sp.buf, words, flags, keep,
@@ -1708,20 +1424,21 @@ void launch_occluded(const SceneDev& S, int n, const float* org, const float* di
 }
 
 void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
-                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap, int* cell_count,
-                       int* cell_start, int* scan_tmp, const PhotonSortBuffers& sb, int photon_cap, float4* gpos,
-                       float4* gflux, float4* gdir, hipStream_t st) {
+                       const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap,
+                       int* cell_start, int4* gaps, int* ngap, const PhotonSortBuffers& sb, int photon_cap,
+                       float4* gpos, float4* gflux, float4* gdir, hipStream_t st) {
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rstat, bmin[0], bmin[1], bmin[2], ext[0],
-                       ext[1], ext[2], cell_cap, nstored, ctr);
-    hipLaunchKernelGGL(k_zero_i32, dim3((cell_cap + 1 + 255) / 256), dim3(256), 0, st, cell_count, cell_cap + 1);
+                       ext[1], ext[2], cell_cap, nstored, ctr, ngap);
     int gP = (slots.npaths + 255) / 256;
-    hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, cell_count, sb);
-    // cell_start = exclusive scan over cell_cap+1 counts (unused cells are zero)
-    exclusive_scan(cell_count, cell_start, cell_cap + 1, nullptr, scan_tmp, st);
+    hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, sb);
     // stable sort of the flattened photons by cell: cell keys are below
     // cell_cap = 2^22, the flattened order is kept inside a cell
-    radix_sort_pairs(sb.keys, sb.vals, sb.tmp_k, sb.tmp_v, sb.hist, photon_cap, PHOTON_KEY_BITS, st, nstored);
-    hipLaunchKernelGGL(k_photon_sorted, dim3(2048), dim3(256), 0, st, slots, nstored, sb, gpos, gflux, gdir);
+    const SortResult sr =
+        radix_sort_pairs(sb.keys, sb.vals, sb.tmp_k, sb.tmp_v, sb.hist, photon_cap, PHOTON_KEY_BITS, st, nstored);
+    hipLaunchKernelGGL(k_cell_starts, dim3(2048), dim3(256), 0, st, sr.keys, nstored, gp, cell_start, gaps, ngap);
+    hipLaunchKernelGGL(k_fill_gaps, dim3(1024), dim3(256), 0, st, gaps, ngap, cell_start);
+    hipLaunchKernelGGL(k_photon_sorted, dim3(2048), dim3(256), 0, st, slots, nstored, sr.vals, sb, gpos, gflux,
+                       gdir);
 }
 
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,
@@ -1745,20 +1462,21 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
                             const PairRecord& rec, hipStream_t st) {
     const int ntiles = gather_group_count(np);
     // hit-point groups: valid hit points in Morton order, 64 per group
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 2);  // valid count, chunk queue
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 1);  // valid count
+    hipLaunchKernelGGL(k_zero_i32, dim3(STRIPE_INTS / 256), dim3(256), 0, st, gb.chunkq, STRIPE_INTS);
     if (counts) hipLaunchKernelGGL(k_zero_i32, dim3((np + 255) / 256), dim3(256), 0, st, counts, np);
     hipLaunchKernelGGL(k_hit_keys, dim3((np + 255) / 256), dim3(256), 0, st, H, np, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], gb.keys, gb.vals);
-    radix_sort_pairs(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, gb.hist, np, 31, st);
-    hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st, gb.keys, np, gb.nvalid);
-    const uint32_t* order = gb.vals;
+    const SortResult sr = radix_sort_pairs(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, gb.hist, np, 31, st);
+    hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st, sr.keys, np, gb.nvalid);
+    const uint32_t* order = sr.vals;
     hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles);
     hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp, ctr);
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
     if (rec.pairs) launch_set_u64x2(rec.count, 0ull, 0ull, st);
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
-                       ntiles, chunk_off, chunkp, gb.nvalid + 1, partial, pdep, rec);
+                       ntiles, chunk_off, chunkp, gb.chunkq, partial, pdep, rec);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
                        partial, counts, gb.stat, flags);
     hipLaunchKernelGGL(k_group_stats, dim3(1), dim3(1024), 0, st, gb.stat, ntiles, rstat, ctr);

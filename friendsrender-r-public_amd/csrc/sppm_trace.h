@@ -7,7 +7,9 @@
 // facet id, so the result is independent of the tree shape and width.
 // Child boxes are padded by 0.001 like AABB::Expand (Intersect.cpp:556-575),
 // and the slab test is inclusive, so no box containing an accepted hit is
-// ever culled.
+// ever culled.  The slab distances are fma(bound, 1/d, -o/d): one FMA per
+// plane, whose rounding differs from (bound - o)/d by far less than the
+// padding (culling decides only which boxes are visited, never the hit).
 //
 // Traversal descends into the nearest intersected child and pushes the
 // others far-to-near.  The first TRACE_LDS stack entries of a lane live in
@@ -101,6 +103,7 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
     inv.x = fabs_(dir.x) < 1e-16f ? (dir.x > 0.0f ? BIG : -BIG) : 1.0f / dir.x;
     inv.y = fabs_(dir.y) < 1e-16f ? (dir.y > 0.0f ? BIG : -BIG) : 1.0f / dir.y;
     inv.z = fabs_(dir.z) < 1e-16f ? (dir.z > 0.0f ? BIG : -BIG) : 1.0f / dir.z;
+    const f3 oi = mk(org.x * inv.x, org.y * inv.y, org.z * inv.z);
     float best = 3.40282347e38f;  // FLT_MAX = Ray::m_MaxT
     int bfacet = 0x7fffffff;
     int bslot = -1;
@@ -133,9 +136,9 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
             const uint32_t* cref = reinterpret_cast<const uint32_t*>(nd + 6 * W);
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                float tx0 = (nd[0 * W + c] - org.x) * inv.x, tx1 = (nd[3 * W + c] - org.x) * inv.x;
-                float ty0 = (nd[1 * W + c] - org.y) * inv.y, ty1 = (nd[4 * W + c] - org.y) * inv.y;
-                float tz0 = (nd[2 * W + c] - org.z) * inv.z, tz1 = (nd[5 * W + c] - org.z) * inv.z;
+                float tx0 = fmaf(nd[0 * W + c], inv.x, -oi.x), tx1 = fmaf(nd[3 * W + c], inv.x, -oi.x);
+                float ty0 = fmaf(nd[1 * W + c], inv.y, -oi.y), ty1 = fmaf(nd[4 * W + c], inv.y, -oi.y);
+                float tz0 = fmaf(nd[2 * W + c], inv.z, -oi.z), tz1 = fmaf(nd[5 * W + c], inv.z, -oi.z);
                 float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
                 float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
                 uint32_t cr = cref[c];

@@ -33,7 +33,7 @@ EXPORTED = [
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
     "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
-    "sppm_tie_counts",
+    "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells",
 ]
 TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
@@ -103,7 +103,8 @@ class Params(C.Structure):
     _fields_ = [("alpha", C.c_float), ("first_radius", C.c_float), ("photons_per_pass", C.c_int32),
                 ("bvh_width", C.c_int32), ("block_size", C.c_int32), ("check_numeric", C.c_int32),
                 ("record_neighbours", C.c_int32), ("max_neighbours", C.c_int32), ("eye_nee", C.c_int32),
-                ("step_offset", C.c_int32)]
+                ("step_offset", C.c_int32), ("photon_kernel", C.c_int32), ("wave_bounces", C.c_int32),
+                ("wave_lanes", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -172,6 +173,9 @@ def load_library(path=LIB_PATH):
     lib.sppm_ray_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 2
     lib.sppm_tie_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 2
     lib.sppm_read_path_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.sppm_guide_path_cells.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.sppm_read_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int64,
+                                   C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32)]
     return lib
 
 
@@ -251,12 +255,17 @@ def camera_desc(cam):
 RECORD_OFF, RECORD_LISTS, RECORD_PAIRS = 0, 1, 2  # sppm_params.record_neighbours
 
 
+PHOTON_MEGAKERNEL, PHOTON_WAVEFRONT = 0, 1  # sppm_params.photon_kernel
+
+
 def params_struct(alpha, first_radius, photons, bvh_width=8, record=False, check_numeric=False, nee=0,
-                  max_pairs=0, step_offset=0):
+                  max_pairs=0, step_offset=0, photon_kernel=PHOTON_MEGAKERNEL, wave_bounces=0, wave_lanes=0):
     """record: False/0 none, True/1 lists from the recording gather kernel, 2
     pairs appended by the production gather (at most max_pairs per pass).
-    step_offset: pass k runs the reference's step k + step_offset."""
+    step_offset: pass k runs the reference's step k + step_offset.
+    photon_kernel / wave_bounces: how the photon pass runs (same photons)."""
     p = Params()
+    p.photon_kernel, p.wave_bounces, p.wave_lanes = int(photon_kernel), int(wave_bounces), int(wave_lanes)
     p.alpha, p.first_radius, p.photons_per_pass = alpha, first_radius, photons
     p.bvh_width, p.block_size, p.check_numeric = bvh_width, 0, int(check_numeric)
     p.record_neighbours, p.max_neighbours = int(record), int(max_pairs)
@@ -269,7 +278,7 @@ class Renderer:
     """One SPPM render on one GPU: scene + camera + params, then passes."""
 
     def __init__(self, scene, cam, alpha, first_radius, photons, bvh_width=8, record=False, device=0, nee=NEE_OFF,
-                 max_pairs=0, step_offset=0):
+                 max_pairs=0, step_offset=0, photon_kernel=PHOTON_MEGAKERNEL, wave_bounces=0, wave_lanes=0):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_create(C.byref(h), device, None))
@@ -281,7 +290,8 @@ class Renderer:
         self._cam = camera_desc(cam)
         _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
         self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs,
-                                     step_offset=step_offset)
+                                     step_offset=step_offset, photon_kernel=photon_kernel, wave_bounces=wave_bounces,
+                                     wave_lanes=wave_lanes)
         _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
         _check(self.lib.sppm_begin(self.h))
         self.passes = 0
@@ -403,6 +413,12 @@ class Renderer:
         _check(self.lib.sppm_guide_tables(self.h, _fp(prob), _fp(cdf)))
         return prob, cdf
 
+    def guide_path_cells(self):
+        """Emission cell of every photon path of the last guided pass."""
+        cells = np.zeros(self.N, np.int32)
+        _check(self.lib.sppm_guide_path_cells(self.h, _ip(cells)))
+        return cells
+
     def checkpoint(self):
         """Bytes of a checkpoint of the render state (see sppm_checkpoint_save)."""
         n = C.c_int64()
@@ -467,6 +483,19 @@ class Renderer:
         c = np.zeros(self.N, np.int32)
         _check(self.lib.sppm_read_path_counts(self.h, _ip(c)))
         return c
+
+    def read_grid(self):
+        """The photon grid of the last build: dims (nx, ny, nz, ncells,
+        nphotons), geometry (bmin, cell edge), cell starts and the flattened
+        photon index at each grid position."""
+        dims, geom = np.zeros(5, np.int32), np.zeros(4, np.float32)
+        _check(self.lib.sppm_read_grid(self.h, _ip(dims), _fp(geom), 0, None, 0, None))
+        start = np.zeros(int(dims[3]) + 1, np.int32)
+        order = np.zeros(max(int(dims[4]), 1), np.int32)
+        _check(self.lib.sppm_read_grid(self.h, None, None, start.size, _ip(start), order.size, _ip(order)))
+        return dict(nx=int(dims[0]), ny=int(dims[1]), nz=int(dims[2]), ncells=int(dims[3]),
+                    nphotons=int(dims[4]), bmin=geom[:3].copy(), cell=np.float32(geom[3]), start=start,
+                    order=order[:int(dims[4])])
 
     def read_counts(self):
         m = np.zeros(self.W * self.H, np.int32)

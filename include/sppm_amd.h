@@ -34,8 +34,9 @@
 extern "C" {
 #endif
 
-#define SPPM_ABI_VERSION 4  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
-                               4: sppm_guide_params.train_interval, sppm_params.step_offset */
+#define SPPM_ABI_VERSION 5  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
+                               4: sppm_guide_params.train_interval, sppm_params.step_offset;
+                               5: sppm_params.photon_kernel, wave_bounces, wave_lanes */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -142,7 +143,26 @@ typedef struct sppm_params {
     int32_t step_offset;       /* sppm_pass runs step = passes done + 1 + step_offset (the reference's
                                   step, SPPM.cpp:798, 836, which seeds every photon and eye path):
                                   a second render with another offset draws independent streams */
+    int32_t photon_kernel;     /* SPPM_PHOTON_*: how the photon pass (createPhotonMap) runs */
+    int32_t wave_bounces;      /* wavefront: bounces traced from ray queues before the last ones
+                                  finish one lane per path (0 = default) */
+    int32_t wave_lanes;        /* wavefront: lanes per ray of the closest-hit kernel, 2, 4 or 8 and
+                                  at most the BVH width (0 = default) */
 } sppm_params;
+
+/* Photon pass kernels (the stored photons are identical either way).
+ *   SPPM_PHOTON_MEGAKERNEL  (default, 0) one persistent kernel, one lane per
+ *                           path from emission to its end;
+ *   SPPM_PHOTON_WAVEFRONT   queues of rays between an emission kernel, a
+ *                           closest-hit kernel with wave_lanes lanes per ray
+ *                           (coalesced wide-node fetch; a ray's group takes the
+ *                           next ray as soon as its ray ends) and a shading
+ *                           kernel, wave_bounces times, then one lane per path.
+ *                           Faster per ray on full queues, but every bounce
+ *                           launch waits for its slowest ray: on the
+ *                           full-material workload a whole pass takes longer. */
+#define SPPM_PHOTON_MEGAKERNEL 0
+#define SPPM_PHOTON_WAVEFRONT 1
 
 /* Next-event estimation with MIS on eye paths: the two blocks of
  * getHitpointInfo that the reference compiles out with #if 0
@@ -247,6 +267,16 @@ int sppm_read_direct(sppm_ctx* ctx, float* hit_direct, float* shared_direct, int
  * surface photons stored + the stale slot the reference keeps, or 1 for the
  * emission photon (SPPM.cpp:530-540, 672, 812-822). */
 int sppm_read_path_counts(sppm_ctx* ctx, int32_t* counts);
+/* The photon grid of the last build, the gather structure that takes the place
+ * of kdtree<Photon>::CreateTree (R/kdtree.h:417-431); the reference keeps its
+ * tree in host memory and has no read-back of it.  dims = {nx, ny, nz,
+ * ncells, nphotons}, geom = {bmin x, y, z, cell edge}.  cell_start (ncells + 1
+ * entries, when not NULL): the photons of cell c = (z * ny + y) * nx + x sit at
+ * grid positions [cell_start[c], cell_start[c + 1]).  order (nphotons entries,
+ * when not NULL): the flattened photon index at each grid position (inside a
+ * cell in increasing order).  Capacities too small: SPPM_E_ARG. */
+int sppm_read_grid(sppm_ctx* ctx, int32_t* dims, float* geom, int64_t cell_cap, int32_t* cell_start,
+                   int64_t photon_cap, int32_t* order);
 /* Per-hit-point in-radius photon count M of the last gather. */
 int sppm_read_counts(sppm_ctx* ctx, int32_t* m);
 /* Neighbour lists of the last gather (requires params.record_neighbours):
@@ -352,6 +382,9 @@ int sppm_guide_read(sppm_ctx* ctx, float* prob, float* hist);
 /* The tables the next guided photon pass samples: the proposal per cell
  * (unnormalised) and its inclusive prefix sum (either may be NULL). */
 int sppm_guide_tables(sppm_ctx* ctx, float* prob, float* cdf);
+/* The emission cell (index into the 262,144 cells) of every photon path of
+ * the last guided pass, whichever mixture component drew it: N entries. */
+int sppm_guide_path_cells(sppm_ctx* ctx, int32_t* cells);
 
 /* ---- graph-captured passes, batches of scenes, autotuner --------------------
  * No counterpart in the reference (it renders one scene per process,
