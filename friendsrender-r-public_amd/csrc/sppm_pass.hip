@@ -1004,14 +1004,14 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
     const int CH = *chunkp;
     const int nvalid = *nvalidp;
     const GridParams g = *gp;
-    // chunks are claimed through the striped counters (zeroed before the
-    // launch); every wave ends once every stripe is exhausted.  The partial
+    // chunks are claimed round-robin through the striped counters (zeroed
+    // before the launch); every wave ends once every stripe is exhausted.  The partial
     // sums go to the chunk's own slot, so the result does not depend on which
     // wave took which chunk.
     StripeCursor cur = stripe_start();
     for (;;) {
         int ch = 0;
-        if (stripe_claim(queue, total, cur, 1, &ch) == 0) break;
+        if (!stripe_claim_rr(queue, total, cur, &ch)) break;
         // owning tile: last t with chunk_off[t] <= ch
         int lo = 0, hi = ntiles;
         while (hi - lo > 1) {

@@ -52,6 +52,27 @@ __device__ __forceinline__ int stripe_claim(int* __restrict__ ctr, int n, Stripe
     return 0;
 }
 
+// Items [0, n) dealt round-robin: item k * STRIPES + s belongs to stripe s,
+// claimed one at a time.  Waves on different stripes then claim neighbouring
+// items at about the same time, so the claims still follow the items' order
+// (work on items that share data stays together in the caches).  Returns
+// false when no work is left.
+__device__ __forceinline__ bool stripe_claim_rr(int* __restrict__ ctr, int n, StripeCursor& c, int* item) {
+    while (c.tried < STRIPES) {
+        int old = 0;
+        if ((threadIdx.x & 63) == 0) old = atomicAdd(&ctr[c.s * STRIPE_PAD], 1);
+        old = __shfl(old, 0, 64);
+        const long long it = (long long)old * STRIPES + c.s;
+        if (it < n) {
+            *item = (int)it;
+            return true;
+        }
+        c.s = c.s + 1 == STRIPES ? 0 : c.s + 1;
+        ++c.tried;
+    }
+    return false;
+}
+
 // A queue kept as STRIPES partitions of capacity P: partition s holds
 // entries [s * P, s * P + cnt[s * STRIPE_PAD]).  Claims up to `want`
 // consecutive entries through the partitions' claim counters fct.
