@@ -814,20 +814,22 @@ int sppm_params_set(sppm_ctx* c, const sppm_params* p) {
 
 // initializeSharedPoints (SPPM.cpp:356-373) + buffers of the pass loop
 // Camera stream segments: the block of a pass (6 words per pixel) is cut
-// into K segments of L words, K the largest divisor of the block up to 64.
+// into K segments of L words, K the largest divisor of the block up to 16.
 // Segment k starts k * L words into the stream; after a pass every segment
-// jumps over the K - 1 others.
+// jumps over the K - 1 others.  Every jump costs the same (~10^4 shifted
+// windows), so fewer, longer segments cost less in all; one wave emits a
+// segment, and the words are made a pass ahead of their use.
 static int init_camera_stream(sppm_ctx* c) {
     RenderBuffers& R = c->rb;
     const long long block = 6LL * R.np;
     int K = 1;
-    for (int k = 64; k >= 1; --k)
+    for (int k = 16; k >= 1; --k)
         if (block % k == 0) {
             K = k;
             break;
         }
     const long long L = block / K;
-    int rc = dalloc(&R.cam_states, (size_t)K * MT_WORDS);
+    int rc = dalloc(&R.cam_states, 2 * (size_t)K * MT_WORDS);  // the windows, then scratch for the kernels
     if (rc) return rc;
     std::vector<uint32_t> win((size_t)K * MT_WORDS);
     mt_window_seed((uint32_t)c->cam.sampler_id, win.data());

@@ -39,7 +39,8 @@ struct Counters {
 };
 
 // Camera words of one pass: K segments of L words (K * L = 6 * pixels);
-// states holds K windows of 624 raw words, phi the jump polynomial of (K-1)L.
+// states holds K windows of 624 raw words (then K more as scratch), phi the
+// jump polynomial of (K-1)L.
 void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long L, uint32_t* words, hipStream_t st);
 // Traversal stack spill area of one path kernel: blocks * 64 lanes,
 // cap - TRACE_LDS entries each.  Kernels that may run at the same time (the

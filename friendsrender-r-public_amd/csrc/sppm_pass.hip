@@ -1,8 +1,8 @@
 // sppm_pass.hip -- the per-pass kernels of the SPPM core on gfx950.
 //
 // One pass (SPPM.cpp:770-843):
-//   k_camera_words   shared UniformSample stream words for this pass (K
-//                    segments, each one workgroup, jump-ahead between passes)
+//   k_camw_emit/jump shared UniformSample stream words for this pass (K
+//                    segments, each one wave, jump-ahead between passes)
 //   k_photon<W>      createPhotonMap: persistent, one lane per photon path at
 //                    a time, writes slots
 //   scan(count)      flatten offsets (SPPM.cpp:812-822 order)
@@ -34,11 +34,16 @@ namespace sppm {
 // Shared camera sampler stream: UniformSample(ID) owns one mt19937 for the
 // whole render (CameraSampler.h:19-31); pixel i of a pass draws 3 doubles in
 // pixel order, 6 words per pixel.  The block of a pass (6 * pixels words) is
-// cut into K segments of L words.  Workgroup k holds the state window where
-// its segment starts, emits the segment's tempered words, and then jumps its
-// window over the other K-1 segments (t^((K-1)L) mod P applied as a sum of
-// shifted windows), so that it holds the start of its segment of the next
-// pass.  The jump polynomial and the starting windows come from the host.
+// cut into K segments of L words; segment k's state window (where it
+// starts) comes from the host for the first pass.
+//   k_camw_emit  one wave per segment emits the segment's tempered words and
+//                stores the window after it;
+//   k_camw_jump  one workgroup per segment jumps that window over the other
+//                K - 1 segments (t^((K-1)L) mod P applied as a sum of shifted
+//                windows): the start of its segment in the next pass.
+// Raw word x[m], m >= 624, is mix(x[m-624], x[m-623], x[m-227]), so 227
+// consecutive new words depend only on earlier ones, and a ring of 1024 raw
+// words holds everything either kernel reads.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t temper(uint32_t z) {
     z ^= (z >> 11);
@@ -51,74 +56,82 @@ __device__ __forceinline__ uint32_t mt_mix(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
     return c ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
 }
-constexpr int CAMW_THREADS = 1024;  // 16 waves: the jump's LDS reads are latency bound
-constexpr int CAMW_RING = 1024;                    // ring of raw words while emitting (>= 624 + 227)
-constexpr int CAMW_SEQ = 624 + 19937 - 1;          // raw words a jump reads
-constexpr int CAMW_DEG = 19937;
-// x[m] for m >= 624 depends on x[m-624], x[m-623], x[m-227]: 227 new words
-// at a time depend only on earlier ones.
-__global__ void __launch_bounds__(CAMW_THREADS) k_camera_words(uint32_t* __restrict__ states,
-                                                               const uint32_t* __restrict__ phi, long long L,
-                                                               uint32_t* __restrict__ words) {
+constexpr int CAMW_RING = 1024;       // raw words kept (>= 624 + 227, and >= 227 + 624 for the jump's reads)
+constexpr int CAMW_DEG = 19937;       // degree of the characteristic polynomial
+constexpr int CAMW_JUMP_THREADS = 640;  // 624 output words of the jumped window
+
+// One wave per segment: x[624 + q] -> out[q] for q < L, 227 words per step
+// in four rounds of 64 lanes.  A step reads only words of earlier steps, and
+// the wave's LDS accesses are executed in order, so no barrier is needed.
+__global__ void __launch_bounds__(64) k_camw_emit(const uint32_t* __restrict__ states, long long L,
+                                                  uint32_t* __restrict__ words, uint32_t* __restrict__ ends) {
     __shared__ uint32_t ring[CAMW_RING];
-    __shared__ uint32_t xs[CAMW_SEQ];
-    const int t = threadIdx.x;
-    uint32_t* st = states + (size_t)blockIdx.x * 624;
-    for (int k = t; k < 624; k += CAMW_THREADS) ring[k] = st[k];
-    __syncthreads();
+    const int lane = threadIdx.x;
+    const uint32_t* st = states + (size_t)blockIdx.x * 624;
+    for (int k = lane; k < 624; k += 64) ring[k] = st[k];
+    __builtin_amdgcn_wave_barrier();
     uint32_t* out = words + (size_t)blockIdx.x * (size_t)L;
-    // segment words: x[624 + q] -> out[q]; each step writes ring slots
-    // [m0, m0 + 227) mod 1024, older than any slot read in the step
-    for (long long m0 = 624; m0 < 624 + L; m0 += 227) {
-        const long long m = m0 + t;
-        if (t < 227 && m < 624 + L) {
-            const uint32_t x = mt_mix(ring[(m - 624) & (CAMW_RING - 1)], ring[(m - 623) & (CAMW_RING - 1)],
-                                      ring[(m - 227) & (CAMW_RING - 1)]);
-            ring[m & (CAMW_RING - 1)] = x;
-            out[m - 624] = temper(x);
-        }
-        __syncthreads();
-    }
-    // window after the segment: x[L .. L+623]; continue it linearly
-    for (int j = t; j < 624; j += CAMW_THREADS) xs[j] = ring[(L + j) & (CAMW_RING - 1)];
-    __syncthreads();
-    for (int m0 = 624; m0 < CAMW_SEQ; m0 += 227) {
-        const int m = m0 + t;
-        if (t < 227 && m < CAMW_SEQ) xs[m] = mt_mix(xs[m - 624], xs[m - 623], xs[m - 227]);
-        __syncthreads();
-    }
-    // jumped window: word j = XOR of xs[i + j] over the coefficients phi_i = 1;
-    // up to four coefficients per step keep four LDS reads in flight
-    if (t < 624) {
-        uint32_t acc = 0;
-        for (int c = 0; c < 624; ++c) {
-            uint32_t bits = phi[c];
-            if (c == 623) bits &= (1u << (CAMW_DEG - 623 * 32)) - 1u;  // degree < 19937
-            const uint32_t* row = xs + c * 32 + t;
-            while (bits) {
-                const int b0 = __builtin_ctz(bits);
-                bits &= bits - 1u;
-                uint32_t v = row[b0];
-                if (bits) {
-                    const int b1 = __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    v ^= row[b1];
-                }
-                if (bits) {
-                    const int b2 = __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    v ^= row[b2];
-                }
-                if (bits) {
-                    const int b3 = __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    v ^= row[b3];
-                }
-                acc ^= v;
+    const long long mend = 624 + L;
+    for (long long m0 = 624; m0 < mend; m0 += 227) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int idx = r * 64 + lane;
+            const long long m = m0 + idx;
+            if (idx < 227 && m < mend) {
+                const uint32_t x = mt_mix(ring[(m - 624) & (CAMW_RING - 1)], ring[(m - 623) & (CAMW_RING - 1)],
+                                          ring[(m - 227) & (CAMW_RING - 1)]);
+                ring[m & (CAMW_RING - 1)] = x;
+                out[m - 624] = temper(x);
             }
         }
-        st[t] = acc;
+        __builtin_amdgcn_wave_barrier();
     }
+    // the window after the segment: x[L .. L + 623]
+    uint32_t* e = ends + (size_t)blockIdx.x * 624;
+    for (int j = lane; j < 624; j += 64) e[j] = ring[(L + j) & (CAMW_RING - 1)];
+}
+
+// Jumped window: word t = XOR of y[i + t] over the coefficients phi_i = 1,
+// y[k] = x[L + k].  The coefficients are taken 227 at a time; before a block
+// [i0, i0 + 227) the words up to y[i0 + 226 + 623] are generated into the
+// ring (227 at a time, one barrier), whose oldest slot then holds y[i0 - 175]
+// or later, so the block's reads y[i0 ..] are intact.
+__global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump(const uint32_t* __restrict__ ends,
+                                                                 const uint32_t* __restrict__ phi,
+                                                                 uint32_t* __restrict__ states) {
+    __shared__ uint32_t ring[CAMW_RING];
+    const int t = threadIdx.x;
+    const uint32_t* e = ends + (size_t)blockIdx.x * 624;
+    if (t < 624) ring[t] = e[t];  // y[k] at ring[k & 1023]
+    __syncthreads();
+    uint32_t acc = 0;
+    int gen = 623;  // highest y generated
+    for (int i0 = 0; i0 < CAMW_DEG; i0 += 227) {
+        const int i1 = i0 + 227 < CAMW_DEG ? i0 + 227 : CAMW_DEG;
+        const int need = i1 - 1 + 623;
+        if (t < need - gen) {
+            const int k = gen + 1 + t;
+            ring[k & (CAMW_RING - 1)] = mt_mix(ring[(k - 624) & (CAMW_RING - 1)], ring[(k - 623) & (CAMW_RING - 1)],
+                                               ring[(k - 227) & (CAMW_RING - 1)]);
+        }
+        gen = need;
+        __syncthreads();
+        if (t < 624) {
+            for (int w = i0 >> 5; w <= (i1 - 1) >> 5; ++w) {
+                uint32_t bits = phi[w];
+                const int lo = i0 - w * 32, hi = i1 - w * 32;
+                if (lo > 0) bits &= ~((1u << lo) - 1u);
+                if (hi < 32) bits &= (1u << hi) - 1u;
+                while (bits) {
+                    const int b = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    acc ^= ring[(w * 32 + b + t) & (CAMW_RING - 1)];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t < 624) states[(size_t)blockIdx.x * 624 + t] = acc;
 }
 
 // Persistent photon pass: each lane traces one path at a time; when at least
@@ -1344,7 +1357,9 @@ int trace_grid_blocks(int width) {
 }
 
 void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long L, uint32_t* words, hipStream_t st) {
-    hipLaunchKernelGGL(k_camera_words, dim3(K), dim3(CAMW_THREADS), 0, st, states, phi, L, words);
+    uint32_t* ends = states + (size_t)K * 624;  // scratch: the windows after the segments
+    hipLaunchKernelGGL(k_camw_emit, dim3(K), dim3(64), 0, st, states, L, words, ends);
+    hipLaunchKernelGGL(k_camw_jump, dim3(K), dim3(CAMW_JUMP_THREADS), 0, st, ends, phi, states);
 }
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
