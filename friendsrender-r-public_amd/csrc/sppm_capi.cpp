@@ -874,6 +874,8 @@ int sppm_begin(sppm_ctx* c) {
     R.cell_cap = 1 << 22;
     const size_t NS = (size_t)NSLOT * N;
     int rc = 0;
+    // tiles of the recording gather (8x8 pixels) or hit-point groups of the production gather
+    const int ntile_max = std::max(gather_tile_count(W, Hh), gather_group_count(np));
     R.slots.npaths = N;
     if ((rc = dalloc(&R.slots.pos, NS)) || (rc = dalloc(&R.slots.flux, NS)) || (rc = dalloc(&R.slots.dir, NS)) ||
         (rc = dalloc(&R.slots.valid, NS)) || (rc = dalloc(&R.slots.count, N)) || (rc = dalloc(&R.off, N)) ||
@@ -893,11 +895,11 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_phi, MT_WORDS)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
         (rc = dalloc(&R.raystripes, (size_t)STRIPES * RAY_STRIPE)) ||
-        (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)gather_tile_count(W, Hh))) ||
-        (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, gather_tile_count(W, Hh) + 1)) ||
-        (rc = dalloc(&R.chunk_off, gather_tile_count(W, Hh) + 1)) ||
-        (rc = dalloc(&R.tile_scan_tmp, scan_tmp_entries(gather_tile_count(W, Hh) + 1))) ||
-        (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + gather_tile_count(W, Hh)))) ||
+        (rc = dalloc((char**)&R.tiles, gather_tile_bytes() * (size_t)ntile_max)) ||
+        (rc = dalloc(&R.chunkp, 1)) || (rc = dalloc(&R.nch, ntile_max + 1)) ||
+        (rc = dalloc(&R.chunk_off, ntile_max + 1)) ||
+        (rc = dalloc(&R.tile_scan_tmp, scan_tmp_entries(ntile_max + 1))) ||
+        (rc = dalloc(&R.partial, (size_t)64 * (GATHER_CHUNK_CAP + ntile_max))) ||
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
         (rc = dalloc(&R.groups.nvalid, 1)) || (rc = dalloc(&R.groups.chunkq, STRIPE_INTS)) || (rc = dalloc(&R.groups.stat, 3 * (size_t)gather_group_count(np))) ||

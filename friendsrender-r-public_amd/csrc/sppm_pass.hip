@@ -823,10 +823,19 @@ struct TileRange {
 };
 
 // Hit-point groups of the balanced gather: the valid hit points sorted by the
-// Morton code of their position (1024 steps per scene-box axis), 64 per
-// group.  World-space groups keep the union of their query boxes close to one
-// query box; screen tiles do not (a thin lens spreads a tile's hit points
-// over the circle of confusion).
+// Morton code of their position (1024 steps per scene-box axis),
+// GATHER_GROUP per group.  World-space groups keep the union of their query
+// boxes close to one query box; screen tiles do not (a thin lens spreads a
+// tile's hit points over the circle of confusion).  The smaller the group,
+// the closer its union box to each member's own box: a wave holds each hit
+// point of a group in GATHER_PHASES lanes, and each of those lanes tests
+// every GATHER_PHASES-th staged photon.
+#ifndef SPPM_GATHER_GROUP
+#define SPPM_GATHER_GROUP 32
+#endif
+constexpr int GATHER_GROUP = SPPM_GATHER_GROUP;
+constexpr int GATHER_PHASES = 64 / GATHER_GROUP;
+static_assert(GATHER_GROUP == 16 || GATHER_GROUP == 32 || GATHER_GROUP == 64, "group of 16, 32 or 64 hit points");
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v &= 0x3ffu;
     v = (v | (v << 16)) & 0x030000ffu;
@@ -920,7 +929,7 @@ __global__ void __launch_bounds__(64) k_gather_plan(HitDev H, SharedDev SH, Grid
                                                     const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
                                                     TileRange* __restrict__ tiles) {
     const int lane = threadIdx.x;
-    const int p = group_pixel(order, *nvalidp, blockIdx.x * 64 + lane);
+    const int p = lane < GATHER_GROUP ? group_pixel(order, *nvalidp, blockIdx.x * GATHER_GROUP + lane) : -1;
     const GridParams g = *gp;
     GatherLane L;
     bool active;
@@ -1013,6 +1022,8 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
     __shared__ unsigned char s_hit[64 * 64];
     __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
     const int lane = threadIdx.x;
+    const int hl = lane % GATHER_GROUP;   // the lane's hit point in the group
+    const int ph = lane / GATHER_GROUP;   // the lane tests staged photons ph, ph + GATHER_PHASES, ...
     const int total = chunk_off[ntiles];
     const int CH = *chunkp;
     const int nvalid = *nvalidp;
@@ -1033,7 +1044,7 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
         }
         const int t = lo;
         const TileRange T = tiles[t];
-        const int p = group_pixel(order, nvalid, t * 64 + lane);
+        const int p = group_pixel(order, nvalid, t * GATHER_GROUP + hl);
         GatherLane L;
         bool active;
         float4 sr;
@@ -1066,34 +1077,39 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                 int nh = 0;
                 if (pdep) {
                     // also count, per staged photon, the hit points that gather it
-                    for (int j = 0; j < n; ++j) {
-                        const bool hit = mine && in_query(L, s_pos[j]);
+                    // (photon jj + q is tested by the lanes of phase q)
+                    for (int jj = 0; jj < n; jj += GATHER_PHASES) {
+                        const int j = jj + ph;
+                        const bool hit = j < n && mine && in_query(L, s_pos[j]);
                         if (hit) {
                             s_hit[nh * 64 + lane] = (unsigned char)j;
                             ++nh;
                         }
                         const unsigned long long m = __ballot(hit);
-                        if (lane == 0) s_cnt[j] = __popcll(m);
+                        if (lane < GATHER_PHASES && jj + lane < n)
+                            s_cnt[jj + lane] = __popcll((m >> (lane * GATHER_GROUP)) &
+                                                        (GATHER_GROUP == 64 ? ~0ull : ((1ull << GATHER_GROUP) - 1ull)));
                     }
                 } else if (mine) {
-                    // four staged photons per step, no branches: the list slot nh is
-                    // written every time and kept only when the photon passes (nh <= j,
-                    // so the slot stays below 64)
-                    int j = 0;
-                    for (; j + 4 <= n; j += 4) {
-                        const float4 p0 = s_pos[j], p1 = s_pos[j + 1], p2 = s_pos[j + 2], p3 = s_pos[j + 3];
+                    // four of this lane's staged photons per step, no branches: the list
+                    // slot nh is written every time and kept only when the photon passes
+                    // (nh <= the photons tested so far, so the slot stays below 64)
+                    int j = ph;
+                    for (; j + 3 * GATHER_PHASES < n; j += 4 * GATHER_PHASES) {
+                        const float4 p0 = s_pos[j], p1 = s_pos[j + GATHER_PHASES];
+                        const float4 p2 = s_pos[j + 2 * GATHER_PHASES], p3 = s_pos[j + 3 * GATHER_PHASES];
                         const bool h0 = in_query_nb(L, p0), h1 = in_query_nb(L, p1);
                         const bool h2 = in_query_nb(L, p2), h3 = in_query_nb(L, p3);
                         s_hit[nh * 64 + lane] = (unsigned char)j;
                         nh += h0;
-                        s_hit[nh * 64 + lane] = (unsigned char)(j + 1);
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + GATHER_PHASES);
                         nh += h1;
-                        s_hit[nh * 64 + lane] = (unsigned char)(j + 2);
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + 2 * GATHER_PHASES);
                         nh += h2;
-                        s_hit[nh * 64 + lane] = (unsigned char)(j + 3);
+                        s_hit[nh * 64 + lane] = (unsigned char)(j + 3 * GATHER_PHASES);
                         nh += h3;
                     }
-                    for (; j < n; ++j) {
+                    for (; j < n; j += GATHER_PHASES) {
                         s_hit[nh * 64 + lane] = (unsigned char)j;
                         nh += in_query_nb(L, s_pos[j]);
                     }
@@ -1148,7 +1164,17 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                 __syncthreads();
             }
         }
-        partial[(size_t)ch * 64 + lane] = make_float4(L.add.x, L.add.y, L.add.z, __int_as_float(L.M));
+        // the hit point's sums over its lanes, in phase order (deterministic)
+        f3 tot = mk(0.0f, 0.0f, 0.0f);
+        int Mt = 0;
+#pragma unroll
+        for (int q = 0; q < GATHER_PHASES; ++q) {
+            const int src = hl + q * GATHER_GROUP;
+            tot = tot + mk(__shfl(L.add.x, src, 64), __shfl(L.add.y, src, 64), __shfl(L.add.z, src, 64));
+            Mt += __shfl(L.M, src, 64);
+        }
+        if (ph == 0)
+            partial[(size_t)ch * GATHER_GROUP + hl] = make_float4(tot.x, tot.y, tot.z, __int_as_float(Mt));
     }
 }
 
@@ -1160,7 +1186,7 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
                                                       uint32_t* __restrict__ flags) {
     const int lane = threadIdx.x;
     const int t = blockIdx.x;
-    const int p = group_pixel(order, *nvalidp, t * 64 + lane);
+    const int p = lane < GATHER_GROUP ? group_pixel(order, *nvalidp, t * GATHER_GROUP + lane) : -1;
     float4 hp = make_float4(0, 0, 0, 0), sr = make_float4(0, 0, 0, 0);
     if (p >= 0) {
         hp = H.pos[p];
@@ -1171,9 +1197,11 @@ __global__ void __launch_bounds__(64) k_gather_finish(HitDev H, SharedDev SH, co
     int M = 0;
 #pragma unroll 4
     for (int c = chunk_off[t]; c < chunk_off[t + 1]; ++c) {
-        float4 v = partial[(size_t)c * 64 + lane];
-        add = add + mk(v.x, v.y, v.z);
-        M += __float_as_int(v.w);
+        if (lane < GATHER_GROUP) {
+            float4 v = partial[(size_t)c * GATHER_GROUP + lane];
+            add = add + mk(v.x, v.y, v.z);
+            M += __float_as_int(v.w);
+        }
     }
     float rnew = sr.w;
     if (active && M > 0) {
@@ -1497,7 +1525,7 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
     hipLaunchKernelGGL(k_group_stats, dim3(1), dim3(1024), 0, st, gb.stat, ntiles, rstat, ctr);
 }
 int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); }
-int gather_group_count(int np) { return (np + 63) / 64; }
+int gather_group_count(int np) { return (np + GATHER_GROUP - 1) / GATHER_GROUP; }
 size_t gather_tile_bytes() { return sizeof(TileRange); }
 
 void launch_reconstruct(const SharedDev& SH, int np, long long total, int passes, float4* out, hipStream_t st) {

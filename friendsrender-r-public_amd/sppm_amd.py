@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsppm_amd.so")
+# the in-tree build; SPPM_AMD_LIB names another build of the same library
+LIB_PATH = os.environ.get("SPPM_AMD_LIB", os.path.join(HERE, "libsppm_amd.so"))
 
 MAT_DIFFUSE, MAT_METAL, MAT_ROUGH_CONDUCTOR, MAT_GGX_GLASS, MAT_GLASS = range(5)
 MAT_BIMETAL, MAT_DIFFUSE_TRANSMIT, MAT_GLOSS_TRANSMIT = 5, 6, 7
