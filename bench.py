@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=1, help="guided emission: Adam steps per training pass")
     ap.add_argument("--train-interval", type=int, default=1, help="guided emission: train after every k-th pass")
     ap.add_argument("--decay", type=float, default=0.95, help="guided emission: training histogram decay")
+    ap.add_argument("--guide-target", choices=["mean", "rms"], default="mean",
+                    help="guided emission: the proposal's training target (mean or root-mean-square deposit)")
     ap.add_argument("--max-passes", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=16)
     ap.add_argument("--target-rmse", type=float, default=0.01)
@@ -202,8 +204,10 @@ def rmse_workload(args):
 
 
 def guided_settings(args):
+    import sppm_amd
     return dict(mix_plain=args.mix, learning_rate=args.lr, train_steps=args.train_steps, seed=1,
-                history_decay=args.decay, train_interval=args.train_interval)
+                history_decay=args.decay, train_interval=args.train_interval,
+                target=sppm_amd.GUIDE_TARGET_RMS if args.guide_target == "rms" else sppm_amd.GUIDE_TARGET_MEAN)
 
 
 def linear_image(r, N, passes, res):
@@ -239,34 +243,68 @@ def reference_image(args, w):
     return ref
 
 
-def emission_gain_bound(w, mixes=(0.5, 0.25), passes=4):
-    """The largest per-pass variance gain any emission proposal over the
-    262,144 emission cells can give, mixed with the plain emission at weight
-    w, on this workload.
+def _fill_proposal(p, s, wmix):
+    """q_x = max(w p_x, t s_x), t by bisection so that q sums to 1."""
+    import numpy as np
+    lo, hi = 0.0, 1.0
+    while np.maximum(wmix * p, hi * s).sum() < 1.0:
+        hi *= 2.0
+    for _ in range(80):
+        t = 0.5 * (lo + hi)
+        if np.maximum(wmix * p, t * s).sum() > 1.0:
+            hi = t
+        else:
+            lo = t
+    q = np.maximum(wmix * p, hi * s)
+    return q / q.sum()
+
+
+def _pool_cells(x, blocks=(4, 4, 2, 2)):
+    """Sums over blocks of (u, v, s, t) emission cells, spread back over the cells."""
+    import numpy as np
+    a, b, c, d = blocks
+    g = x.reshape(32 // a, a, 32 // b, b, 16 // c, c, 16 // d, d).sum((1, 3, 5, 7))
+    for ax, k in enumerate(blocks):
+        g = np.repeat(g, k, ax)
+    return g.ravel()
+
+
+def emission_gain_bound(w, plain_ms, plain_gather_ms, mixes=(0.5, 0.25), passes=8):
+    """What any emission proposal over the 262,144 cells, mixed with the
+    plain emission at weight w, can gain on this workload: per-pass variance
+    and time.
 
     A photon path's contribution c (the sum, over its stored photons in the
     camera's room, x < 1 in the occluded-light scene, of luminance(flux) /
     PDF) depends on its emission cell x only through the cell's plain
     probability p_x and its second moment m_x = E[c^2 | x].  Emitting with
-    q = w p + (1 - w) g and weighting by p / q, the per-path second moment
-    is sum_x p_x^2 m_x / q_x; over all g it is smallest for q_x = max(w p_x,
-    t sqrt(p_x^2 m_x)) with t normalising q (the constrained optimum, found
-    by bisection).  Gain = (sum p m - mu^2) / (min sum p^2 m / q - mu^2).
-    p and m are measured on plain-emission passes (guided emission with
-    mix_plain = 1 draws plain photons and records each path's cell); the
-    mean share of paths whose first segment crosses the door (their first
-    stored photon lies in the room) is reported beside it."""
+    q = w p + (1 - w) g and weighting by p / q, the per-path second moment is
+    sum_x p_x^2 m_x / q_x; it is smallest for q_x = max(w p_x, t p_x
+    sqrt(m_x)).  The cost of a path is a + b r_x, r_x its mean number of
+    photons in the camera's room (the gather's work follows them): a and b
+    split the measured plain pass time (plain_ms) into its gather part
+    (plain_gather_ms, per room photon) and the rest (per path).  Time per
+    unit variance is smallest for q_x = max(w p_x, t p_x sqrt(m_x / (a + b
+    r_x + k))) (k by search).
+
+    p, m and r are measured on plain-emission passes (guided emission with
+    mix_plain = 1 draws plain photons and records each path's cell).  Fitted
+    and scored on the same passes the optimum overfits the noise of m (the
+    in-sample figure); the cross-validated figures fit on the even passes,
+    score on the odd ones and the other way round, with m and r pooled over
+    blocks of 4 x 4 position x 2 x 2 direction cells (the pooling that scores
+    best).  The mean share of paths whose first segment crosses the door
+    (their first stored photon lies in the room) is reported beside it."""
     import numpy as np
     import sppm_amd
     r = sppm_amd.Renderer(w["scene"], w["cam"], 1.0, w["r0"], w["N"], bvh_width=8)
     r.guide_set(True, mix_plain=1.0, learning_rate=1e-3, train_steps=1, seed=1, history_decay=0.95, **w["net"])
     ncell = sppm_amd.GUIDE_CELLS
-    cnt = np.zeros(ncell)
-    s1 = np.zeros(ncell)
-    s2 = np.zeros(ncell)
+    half = np.zeros((2, 4, ncell))  # count, sum c, sum c^2, sum of room photons
     door = np.zeros(2)
+    room_photons = 0
     lum = np.array([0.2126, 0.7152, 0.0722])
-    for _ in range(passes):
+    for k in range(passes):
         r.pass_(stats=False)
         counts = r.read_path_counts()
         cells = r.guide_path_cells()
@@ -276,39 +314,64 @@ def emission_gain_bound(w, mixes=(0.5, 0.25), passes=4):
         inroom = ph["pos"][:, 0] < 1.0
         val = np.where(inroom, (ph["flux"].astype(np.float64) @ lum) / ph["pdf"].astype(np.float64), 0.0)
         c = np.bincount(path, weights=val, minlength=len(counts))
-        cnt += np.bincount(cells, minlength=ncell)
-        s1 += np.bincount(cells, weights=c, minlength=ncell)
-        s2 += np.bincount(cells, weights=c * c, minlength=ncell)
+        nr = np.bincount(path, weights=inroom.astype(np.float64), minlength=len(counts))
+        h = half[k % 2]
+        h[0] += np.bincount(cells, minlength=ncell)
+        h[1] += np.bincount(cells, weights=c, minlength=ncell)
+        h[2] += np.bincount(cells, weights=c * c, minlength=ncell)
+        h[3] += np.bincount(cells, weights=nr, minlength=ncell)
         through = ph["pos"][first, 0] < 1.0
         door += [c[through].sum(), c.sum()]
+        room_photons += int(inroom.sum())
     r.close()
-    n = cnt.sum()
-    p = cnt / n
-    m = np.where(cnt > 0, s2 / np.maximum(cnt, 1), 0.0)
-    mu = s1.sum() / n
-    v_plain = (p * m).sum() - mu * mu
-    a = p * p * m
-    out = dict(paths=int(n), cells_hit=int((cnt > 0).sum()), door_mean_share=round(float(door[0] / door[1]), 4),
-               plain_variance_per_path=float(v_plain), passes=passes)
+    tot = half.sum(0)
+    n = tot[0].sum()
+    p = tot[0] / n
+    ok = p > 0
+    a_cost = (plain_ms - plain_gather_ms) / w["N"]
+    b_cost = plain_gather_ms / (room_photons / passes)
+
+    def cell_mean(h, k, blocks=None):
+        num, den = (h[k], h[0]) if blocks is None else (_pool_cells(h[k], blocks), _pool_cells(h[0], blocks))
+        return np.where(den > 0, num / np.maximum(den, 1), 0.0)
+
+    def score(q, h):
+        """Per-path variance and cost of proposal q on the paths of h."""
+        m, rr = cell_mean(h, 2), cell_mean(h, 3)
+        mu = h[1].sum() / h[0].sum()
+        return (p[ok] ** 2 * m[ok] / q[ok]).sum() - mu * mu, (q * (a_cost + b_cost * rr)).sum()
+
+    out = dict(paths=int(n), cells_hit=int(ok.sum()), door_mean_share=round(float(door[0] / door[1]), 4),
+               passes=passes, room_photons_per_pass=round(room_photons / passes),
+               cost_model_ms=dict(per_million_paths=round(a_cost * 1e6, 4),
+                                  per_thousand_room_photons=round(b_cost * 1e3, 5)))
     for wmix in mixes:
-        lo, hi = 0.0, 1.0
-        sa = np.sqrt(a)
-        while (np.maximum(wmix * p, hi * sa)).sum() < 1.0:
-            hi *= 2.0
-        for _ in range(100):
-            t = 0.5 * (lo + hi)
-            if np.maximum(wmix * p, t * sa).sum() > 1.0:
-                hi = t
-            else:
-                lo = t
-        q = np.maximum(wmix * p, hi * sa)
-        q /= q.sum()
-        ok = q > 0
-        v_best = (a[ok] / q[ok]).sum() - mu * mu
-        out[f"best_gain_mix_{wmix}"] = round(float(v_plain / v_best), 3)
-    out["note"] = ("per-pass variance gain of the best emission proposal over the cells, for each mixture weight "
-                   "(an upper bound for the learned one); time gain = variance gain x plain pass time / guided "
-                   "pass time")
+        m_all = cell_mean(tot, 2)
+        v_plain, c_plain = score(p, tot)
+        q = _fill_proposal(p, np.sqrt(p * p * m_all), wmix)
+        out[f"in_sample_variance_gain_mix_{wmix}"] = round(float(v_plain / score(q, tot)[0]), 3)
+        acc = []
+        for fit, ev in ((0, 1), (1, 0)):
+            m, rr = cell_mean(half[fit], 2, (4, 4, 2, 2)), cell_mean(half[fit], 3, (4, 4, 2, 2))
+            vp, cp = score(p, half[ev])
+            qv = _fill_proposal(p, np.sqrt(p * p * m), wmix)
+            vv, cv = score(qv, half[ev])
+            best = None
+            for kap in np.concatenate([[0.0], np.geomspace(1e-4, 1e2, 25) * a_cost]):
+                qc = _fill_proposal(p, np.sqrt(p * p * m / (a_cost + b_cost * rr + kap)), wmix)
+                vc, cc = score(qc, half[ev])
+                if best is None or vc * cc < best[0] * best[1]:
+                    best = (vc, cc)
+            acc.append((vp / vv, vp * cp / (vv * cv), vp / best[0], vp * cp / (best[0] * best[1])))
+        g = np.mean(acc, 0)
+        out[f"cv_variance_gain_mix_{wmix}"] = round(float(g[0]), 3)
+        out[f"cv_time_gain_of_variance_optimum_mix_{wmix}"] = round(float(g[1]), 3)
+        out[f"cv_cost_aware_variance_gain_mix_{wmix}"] = round(float(g[2]), 3)
+        out[f"cv_cost_aware_time_gain_mix_{wmix}"] = round(float(g[3]), 3)
+    out["note"] = ("gains of the best emission proposal over the cells against plain emission, per pass (variance) "
+                   "and per unit of variance (time, from the cost model; guided sampling and training overhead not "
+                   "counted): bounds for the learned proposal.  in_sample: fitted and scored on the same paths "
+                   "(overfits); cv: fitted and scored on disjoint passes")
     return out
 
 
@@ -378,7 +441,8 @@ def rmse_mode(args):
         return
     res = time_to_rmse(args, w, ref)
     if args.rmse_scene == "occluded":
-        res["emission_bound"] = emission_gain_bound(w)
+        res["emission_bound"] = emission_gain_bound(w, res["plain"]["ms_per_pass"],
+                                                    res["plain"]["phases_ms"]["gather"])
     line = {"metric": res["metric"], "value": res["guided_s"], "unit": "s", "higher_is_better": False,
             "n_gpus": 1, "dtype": w["dtype"], "data": "synthetic (seeded procedural scene)",
             "config": {"workload": w["name"]}, "time_to_rmse": res}
@@ -678,7 +742,8 @@ def main():
         w = rmse_workload(args)
         log(f"time to RMSE: {w['name']}, {w['scene'].nfacets} triangles, {w['res']}x{w['res']}")
         out["time_to_rmse"] = time_to_rmse(args, w, reference_image(args, w))
-        out["time_to_rmse"]["emission_bound"] = emission_gain_bound(w)
+        tr = out["time_to_rmse"]
+        tr["emission_bound"] = emission_gain_bound(w, tr["plain"]["ms_per_pass"], tr["plain"]["phases_ms"]["gather"])
     print(json.dumps(out), flush=True)
 
 

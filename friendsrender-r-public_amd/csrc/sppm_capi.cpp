@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -141,6 +142,7 @@ struct RenderBuffers {
     GroupBuffers groups{};
     int* pass_dev = nullptr;   // passes done (device copy of the pass counter, read by graph replays)
     int* path_cell = nullptr;  // guided emission: emission cell per path, 2 x N (block = step parity)
+    float* path_ratio = nullptr;  // guided emission: q / p_plain per path, like path_cell
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
@@ -172,7 +174,7 @@ struct RenderBuffers {
         dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(pdep); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
+        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(path_ratio); dfree(pdep); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
         ready = false;
     }
 };
@@ -946,11 +948,13 @@ int sppm_begin(sppm_ctx* c) {
         }
     }
     if (c->guided) {
-        if ((rc = dalloc(&R.path_cell, 2 * (size_t)N)) || (rc = dalloc(&R.pdep, R.photon_cap))) {
+        if ((rc = dalloc(&R.path_cell, 2 * (size_t)N)) || (rc = dalloc(&R.path_ratio, 2 * (size_t)N)) ||
+            (rc = dalloc(&R.pdep, R.photon_cap))) {
             R.release();
             return rc;
         }
         c->emit.path_cell = R.path_cell;
+        c->emit.path_ratio = c->gparams.target == SPPM_GUIDE_TARGET_RMS ? R.path_ratio : nullptr;
     }
     HIPOK(hipMemsetAsync(R.counts, 0, sizeof(int) * (size_t)(np + 1), c->stream));
     launch_init_slots(R.slots, c->stream);
@@ -1080,13 +1084,20 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather
 // proposal that drew it, so the estimate stays unbiased.
 static void do_guide(sppm_ctx* c, hipStream_t st, int step, const int* step_dev) {
     RenderBuffers& R = c->rb;
-    launch_guide_deposit(R.off, R.nstored, R.pdep, R.path_cell, R.N, step, step_dev, c->guide.hpass, st);
+    const bool rms = c->gparams.target == SPPM_GUIDE_TARGET_RMS;
+    launch_guide_deposit(R.off, R.nstored, R.pdep, R.path_cell, rms ? R.path_ratio : nullptr, R.N, step, step_dev,
+                         c->guide.hpass, st);
     const int every = c->gparams.train_interval > 1 ? c->gparams.train_interval : 1;
     if (!step_dev && step % every != 0) return;  // the deposits gather in the pass histogram until then
     guide_fold_hist(c->guide, c->gparams.history_decay, st);
-    guide_hist_sum(c->guide, st);
+    const float* w = c->guide.hist;
+    if (rms) {
+        guide_rms_target(c->guide, st);
+        w = c->guide.tgt;
+    }
+    guide_hist_sum(c->guide, w, st);
     for (int k = 0; k < c->gparams.train_steps; ++k) {
-        guide_gradient(c->guide, st);
+        guide_gradient(c->guide, w, st);
         guide_adam(c->guide, c->gparams.learning_rate, st);
     }
     guide_infer(c->guide, st);
@@ -2041,11 +2052,44 @@ static int guide_light_frame(sppm_ctx* c, GuideEmit& E) {
     return SPPM_OK;
 }
 
+// The plain emission's probability of each cell: the position cell's share of
+// the light (uniform area, or the table points' plain probabilities for a
+// textured light) times the cosine lobe's mass in the direction cell,
+// integral of cos / pi d(omega) = pz / (2 pi |P|^4) ds dt over the cell of the
+// hemi-octahedral map (32 x 32 midpoint samples per cell; the lobe's total is
+// normalised to 1).
+static void guide_plain_cells(const std::vector<float>& pos_mass, std::vector<float>& pc) {
+    std::vector<double> dir((size_t)GUIDE_GS * GUIDE_GT, 0.0);
+    const int Q = 32;
+    double tot = 0.0;
+    for (int is = 0; is < GUIDE_GS; ++is)
+        for (int it = 0; it < GUIDE_GT; ++it) {
+            double acc = 0.0;
+            for (int a = 0; a < Q; ++a)
+                for (int b = 0; b < Q; ++b) {
+                    const double s = -1.0 + (is + (a + 0.5) / Q) * (2.0 / GUIDE_GS);
+                    const double t = -1.0 + (it + (b + 0.5) / Q) * (2.0 / GUIDE_GT);
+                    const double px = 0.5 * (s + t), py = 0.5 * (s - t);
+                    const double pz = std::max(0.0, 1.0 - std::fabs(px) - std::fabs(py));
+                    const double P2 = px * px + py * py + pz * pz;
+                    acc += pz / (2.0 * 3.14159265358979323846 * P2 * P2);
+                }
+            dir[(size_t)is * GUIDE_GT + it] = acc * (2.0 / GUIDE_GS) * (2.0 / GUIDE_GT) / (Q * Q);
+            tot += dir[(size_t)is * GUIDE_GT + it];
+        }
+    double mtot = 0.0;
+    for (float m : pos_mass) mtot += m;
+    pc.assign(GUIDE_CELLS, 0.0f);
+    for (int p = 0; p < GUIDE_GU * GUIDE_GV; ++p)
+        for (int d = 0; d < GUIDE_GS * GUIDE_GT; ++d)
+            pc[(size_t)p * GUIDE_GS * GUIDE_GT + d] = (float)(pos_mass[p] / mtot * dir[d] / tot);
+}
+
 // Textured light: the table points grouped by position cell.  Point k of
 // facet f has plain probability pi = P(pick f) * value_k / (1000 * average)
 // (the pick and the point tables, LightSampler.cpp:68-76); its position cell
 // is fixed here once, so both mixture components use the same cell.
-static int guide_light_points(sppm_ctx* c, const GuideEmit& E) {
+static int guide_light_points(sppm_ctx* c, const GuideEmit& E, std::vector<float>& mass_out) {
     const auto& Ls = c->scene.hlights;
     const auto& S = c->scene;
     const int ncell = GUIDE_GU * GUIDE_GV;
@@ -2090,6 +2134,7 @@ static int guide_light_points(sppm_ctx* c, const GuideEmit& E) {
         mass[cc] = cum.empty() || cells[cc].empty() ? 0.0f : cum.back();
     }
     start[ncell] = (int)pts.size();
+    mass_out = mass;
     GuideBuffers& g = c->guide;
     dfree(g.pt_cell); dfree(g.pcell_mass); dfree(g.pc_start); dfree(g.pc_pt); dfree(g.pc_cum);
     g.pt_cell = nullptr; g.pcell_mass = nullptr; g.pc_start = nullptr; g.pc_pt = nullptr; g.pc_cum = nullptr;
@@ -2127,6 +2172,8 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (!(p->learning_rate > 0.0f) || p->train_steps < 0) return fail(SPPM_E_ARG, "bad learning rate / train steps");
     if (!(p->history_decay >= 0.0f && p->history_decay < 1.0f)) return fail(SPPM_E_ARG, "history_decay must be in [0, 1)");
     if (p->train_interval < 0) return fail(SPPM_E_ARG, "train_interval must be >= 0");
+    if (p->target != SPPM_GUIDE_TARGET_MEAN && p->target != SPPM_GUIDE_TARGET_RMS)
+        return fail(SPPM_E_ARG, "target must be SPPM_GUIDE_TARGET_MEAN (0) or SPPM_GUIDE_TARGET_RMS (1)");
     if (c->params.record_neighbours == 1)
         return fail(SPPM_E_STATE, "guided emission needs the production gather (record_neighbours 0 or 2)");
     GuideNet net;
@@ -2142,8 +2189,9 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
         guide_free(c->guide);
     c->guide.net = net;
     if (!c->guide.theta && guide_alloc(c->guide)) return fail(SPPM_E_NOMEM, "guide buffers");
+    std::vector<float> pos_mass((size_t)GUIDE_GU * GUIDE_GV, 1.0f);  // plain probability of each position cell
     if (E.textured) {
-        if ((rc = guide_light_points(c, E))) return rc;
+        if ((rc = guide_light_points(c, E, pos_mass))) return rc;
         E.pt_cell = c->guide.pt_cell;
         E.pcell_mass = c->guide.pcell_mass;
         E.pc_start = c->guide.pc_start;
@@ -2153,6 +2201,15 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
         GuideBuffers& g = c->guide;
         dfree(g.pt_cell); dfree(g.pcell_mass); dfree(g.pc_start); dfree(g.pc_pt); dfree(g.pc_cum);
         g.pt_cell = nullptr; g.pcell_mass = nullptr; g.pc_start = nullptr; g.pc_pt = nullptr; g.pc_cum = nullptr;
+    }
+    {
+        // the plain emission's probability of every cell (RMS target)
+        std::vector<float> pc;
+        guide_plain_cells(pos_mass, pc);
+        GuideBuffers& g = c->guide;
+        if ((!g.pcell && (rc = dalloc(&g.pcell, GUIDE_CELLS))) || (!g.tgt && (rc = dalloc(&g.tgt, GUIDE_CELLS))))
+            return rc;
+        HIPOK(hipMemcpy(g.pcell, pc.data(), sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice));
     }
     std::vector<float> th;
     guide_init_params(net, p->seed, th);
@@ -2170,9 +2227,12 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     c->guided = true;
     if (c->rb.ready && !c->rb.path_cell) {
         RenderBuffers& R = c->rb;
-        if ((rc = dalloc(&R.path_cell, 2 * (size_t)R.N)) || (rc = dalloc(&R.pdep, R.photon_cap))) return rc;
+        if ((rc = dalloc(&R.path_cell, 2 * (size_t)R.N)) || (rc = dalloc(&R.path_ratio, 2 * (size_t)R.N)) ||
+            (rc = dalloc(&R.pdep, R.photon_cap)))
+            return rc;
         c->emit.path_cell = R.path_cell;
     }
+    c->emit.path_ratio = p->target == SPPM_GUIDE_TARGET_RMS ? c->rb.path_ratio : nullptr;
     HIPOK(hipStreamSynchronize(c->stream));
     return SPPM_OK;
 }
@@ -2202,8 +2262,8 @@ int sppm_guide_gradient(sppm_ctx* c, const float* hist, float* loss, float* grad
     int rc = guide_ready(c);
     if (rc) return rc;
     if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
-    guide_hist_sum(c->guide, c->stream);
-    guide_gradient(c->guide, c->stream);
+    guide_hist_sum(c->guide, c->guide.hist, c->stream);
+    guide_gradient(c->guide, c->guide.hist, c->stream);
     float sc[3];
     HIPOK(hipMemcpyAsync(sc, c->guide.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
     if (grad) HIPOK(hipMemcpyAsync(grad, c->guide.grad, sizeof(float) * c->guide.nparams, hipMemcpyDeviceToHost, c->stream));
@@ -2218,9 +2278,9 @@ int sppm_guide_train(sppm_ctx* c, const float* hist, int steps) {
     if (steps < 0) return fail(SPPM_E_ARG, "steps < 0");
     if (hist) HIPOK(hipMemcpyAsync(c->guide.hist, hist, sizeof(float) * GUIDE_CELLS, hipMemcpyHostToDevice, c->stream));
     const float lr = c->gparams.learning_rate > 0.0f ? c->gparams.learning_rate : 1e-3f;
-    guide_hist_sum(c->guide, c->stream);
+    guide_hist_sum(c->guide, c->guide.hist, c->stream);
     for (int k = 0; k < steps; ++k) {
-        guide_gradient(c->guide, c->stream);
+        guide_gradient(c->guide, c->guide.hist, c->stream);
         guide_adam(c->guide, lr, c->stream);
     }
     guide_infer(c->guide, c->stream);

@@ -63,6 +63,8 @@ struct GuideEmit {
     const float* cdf;
     int* path_cell;        // emission cell of each photon path (training record), 2 x N:
                            // the pass of step s writes block s & 1
+    float* path_ratio;     // q / p_plain at each path's emission, laid out like path_cell
+                           // (NULL: not recorded)
     // textured light only
     int textured;
     const int* pt_cell;        // position cell of each table point (indexed like the light tables)
@@ -75,6 +77,8 @@ struct GuideEmit {
 struct GuideBuffers {
     GuideNet net;
     int nparams = 0;
+    float* pcell = nullptr;     // plain emission's probability of each cell
+    float* tgt = nullptr;       // training target per cell (SPPM_GUIDE_TARGET_RMS)
     float* theta = nullptr;     // master weights
     float* m = nullptr;         // Adam moments
     float* v = nullptr;
@@ -110,12 +114,13 @@ void guide_free(GuideBuffers& g);
 int guide_upload_params(GuideBuffers& g, const float* theta, hipStream_t st);
 // Forward over all cells: logits, log-sum-exp, probabilities and their CDF.
 void guide_infer(GuideBuffers& g, hipStream_t st);
-// Sum of the training histogram (scal[1]).
-void guide_hist_sum(GuideBuffers& g, hipStream_t st);
-// Gradient of the loss for the histogram in g.hist (needs the log-sum-exp of
-// the same weights from guide_infer and the histogram sum); writes g.grad and
-// the loss.
-void guide_gradient(GuideBuffers& g, hipStream_t st);
+// Sum of the training weights w (one per cell: g.hist, or g.tgt) into scal[1].
+void guide_hist_sum(GuideBuffers& g, const float* w, hipStream_t st);
+// Gradient of the loss for the weights w (needs the log-sum-exp of the same
+// weights from guide_infer and the weights' sum); writes g.grad and the loss.
+void guide_gradient(GuideBuffers& g, const float* w, hipStream_t st);
+// g.tgt = sqrt(g.pcell * g.hist) (SPPM_GUIDE_TARGET_RMS)
+void guide_rms_target(GuideBuffers& g, hipStream_t st);
 // Adam step with g.grad (skipped on the device when the histogram is empty).
 void guide_adam(GuideBuffers& g, float lr, hipStream_t st);
 void guide_clear_hist(GuideBuffers& g, hipStream_t st);

@@ -605,7 +605,8 @@ void guide_free(GuideBuffers& g) {
                     (void*)g.acts, (void*)g.dh, (void*)g.logits, (void*)g.prob, (void*)g.cdf, (void*)g.eprob,
                     (void*)g.ecdf, (void*)g.hist,
                     (void*)g.hpass, (void*)g.part, (void*)g.part2, (void*)g.scal, (void*)g.pt_cell,
-                    (void*)g.pcell_mass, (void*)g.pc_start, (void*)g.pc_pt, (void*)g.pc_cum})
+                    (void*)g.pcell_mass, (void*)g.pc_start, (void*)g.pc_pt, (void*)g.pc_cum, (void*)g.pcell,
+                    (void*)g.tgt})
         if (p) (void)hipFree(p);
     const GuideNet n = g.net;
     g = GuideBuffers();
@@ -644,22 +645,22 @@ struct NetLaunch {
             hipLaunchKernelGGL((k_mlp_forward<H, L, F16, false>), dim3(FWD_BLOCKS), dim3(64), 0, st, X, g.wfwd,
                                g.theta, g.acts, g.logits, g.part2);
     }
-    static void backward(GuideBuffers& g, hipStream_t st) {
+    static void backward(GuideBuffers& g, const float* w, hipStream_t st) {
         const size_t LS = (size_t)GUIDE_CELLS * H;
         const uint16_t* X = g_features[F16 ? 1 : 0];
         const int np = g.nparams, wo = guide_wout_off(NET);
         // layer L: dH_L from the logits; writes dH_{L-1} into dh[L & 1]
         hipLaunchKernelGGL((k_mlp_backward<H, H, F16, true, false>), dim3(GUIDE_BLOCKS), dim3(256), 0, st,
-                           g.acts + (L - 2) * LS, (const uint16_t*)nullptr, g.acts + (L - 1) * LS, g.logits, g.hist,
+                           g.acts + (L - 2) * LS, (const uint16_t*)nullptr, g.acts + (L - 1) * LS, g.logits, w,
                            g.scal, g.theta, g.wbwd + guide_lw_off(NET, L), g.dh + (L & 1) * LS, g.part, np,
                            guide_w_off(NET, L), guide_b_off(NET, L - 1), guide_b_off(NET, L), wo);
         for (int l = L - 1; l >= 2; --l)
             hipLaunchKernelGGL((k_mlp_backward<H, H, F16, false, false>), dim3(GUIDE_BLOCKS), dim3(256), 0, st,
                                g.acts + (l - 2) * LS, g.dh + ((l + 1) & 1) * LS, (const uint16_t*)nullptr, g.logits,
-                               g.hist, g.scal, g.theta, g.wbwd + guide_lw_off(NET, l), g.dh + (l & 1) * LS, g.part, np,
+                               w, g.scal, g.theta, g.wbwd + guide_lw_off(NET, l), g.dh + (l & 1) * LS, g.part, np,
                                guide_w_off(NET, l), guide_b_off(NET, l - 1), 0, wo);
         hipLaunchKernelGGL((k_mlp_backward<H, GUIDE_FEAT, F16, false, true>), dim3(GUIDE_BLOCKS), dim3(256), 0, st, X,
-                           g.dh + (2 & 1) * LS, (const uint16_t*)nullptr, g.logits, g.hist, g.scal, g.theta,
+                           g.dh + (2 & 1) * LS, (const uint16_t*)nullptr, g.logits, w, g.scal, g.theta,
                            g.wbwd + guide_lw_off(NET, 1), (uint16_t*)nullptr, g.part, np, guide_w_off(NET, 1), 0, 0,
                            wo);
     }
@@ -686,17 +687,17 @@ void guide_infer(GuideBuffers& g, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_cdf_offsets, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.cdf, g.part2);
 }
 
-void guide_hist_sum(GuideBuffers& g, hipStream_t st) {
+void guide_hist_sum(GuideBuffers& g, const float* w, hipStream_t st) {
     double* hpart = reinterpret_cast<double*>(g.part2 + 2048);  // after the forward partials (2 * 1024 floats)
-    hipLaunchKernelGGL(k_guide_hsum, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, g.hist, hpart);
+    hipLaunchKernelGGL(k_guide_hsum, dim3(GUIDE_CELLS / 1024), dim3(256), 0, st, w, hpart);
     hipLaunchKernelGGL(k_guide_hsum_total, dim3(1), dim3(256), 0, st, hpart, g.scal);
 }
 
-void guide_gradient(GuideBuffers& g, hipStream_t st) {
+void guide_gradient(GuideBuffers& g, const float* w, hipStream_t st) {
     // training forward (activations kept), its log-sum-exp, then the layers backwards
     with_net(g.net, [&](auto op) { op.forward(g, true, st); });
     hipLaunchKernelGGL(k_guide_lse, dim3(1), dim3(256), 0, st, g.part2, FWD_BLOCKS, g.scal);
-    with_net(g.net, [&](auto op) { op.backward(g, st); });
+    with_net(g.net, [&](auto op) { op.backward(g, w, st); });
     hipLaunchKernelGGL(k_guide_grad_reduce, dim3((g.nparams + 1 + 255) / 256), dim3(256), 0, st, g.part,
                        GUIDE_BLOCKS, g.nparams, g.scal, g.grad, g.scal);
 }
@@ -711,6 +712,19 @@ void guide_adam(GuideBuffers& g, float lr, hipStream_t st) {
 void guide_restore(GuideBuffers& g, hipStream_t st) {
     launch_lowp(g, st);
     guide_infer(g, st);
+}
+
+// tgt = sqrt(pcell * max(hist, 0)): the RMS target from the histogram of
+// deposit^2 x q / p
+__global__ void k_guide_rms_target(const float* __restrict__ hist, const float* __restrict__ pcell,
+                                   float* __restrict__ tgt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= GUIDE_CELLS) return;
+    const float h = hist[k];
+    tgt[k] = h > 0.0f ? sqrtf(pcell[k] * h) : 0.0f;
+}
+void guide_rms_target(GuideBuffers& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_rms_target, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, g.pcell, g.tgt);
 }
 
 void guide_clear_hist(GuideBuffers& g, hipStream_t st) {

@@ -62,8 +62,9 @@ void launch_add_i32(int* p, int v, hipStream_t st);
 void launch_zero_i32(int* p, int n, hipStream_t st);
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
 // step: the pass step, or *step_dev + 1 when step_dev is given (graph replays)
-void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell, int N,
-                          int step, const int* step_dev, float* hist, hipStream_t st);
+// ratio (optional): q / p_plain per path; with it the deposit is d^2 ratio
+void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell,
+                          const float* ratio, int N, int step, const int* step_dev, float* hist, hipStream_t st);
 // nee: SPPM_NEE_* (the compiled-out next-event estimation of getHitpointInfo)
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                 const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,

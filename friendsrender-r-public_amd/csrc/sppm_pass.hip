@@ -1565,23 +1565,28 @@ __global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n)
 // the stored count) into the histogram cell it was emitted from (the block of
 // path_cell the photon pass of this step wrote).  The slot counts are not
 // read: the next pass's photon pass may already be rewriting them.
+// ratio (optional, q / p_plain per path): deposit d^2 q / p instead of d (the
+// RMS target's histogram).
 __global__ void k_guide_deposit(const int* __restrict__ off, const int* __restrict__ nstored,
-                                const float* __restrict__ pdep, const int* __restrict__ path_cell, int N,
-                                int step_arg, const int* __restrict__ step_dev, float* __restrict__ hist) {
+                                const float* __restrict__ pdep, const int* __restrict__ path_cell,
+                                const float* __restrict__ ratio, int N, int step_arg,
+                                const int* __restrict__ step_dev, float* __restrict__ hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int step = step_dev ? *step_dev + 1 : step_arg;
     const int o = off[i], e = i + 1 < N ? off[i + 1] : *nstored;
     float d = 0.0f;
     for (int k = o; k < e; ++k) d += pdep[k];
-    if (d > 0.0f) atomicAdd(&hist[path_cell[(size_t)(step & 1) * (size_t)N + i]], d);
+    const size_t slot = (size_t)(step & 1) * (size_t)N + i;
+    if (ratio) d = d * d * ratio[slot];
+    if (d > 0.0f && d < 3.0e38f) atomicAdd(&hist[path_cell[slot]], d);
 }
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_f32_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
 }
-void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell, int N,
-                          int step, const int* step_dev, float* hist, hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, off, nstored, pdep, path_cell, N,
-                       step, step_dev, hist);
+void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell,
+                          const float* ratio, int N, int step, const int* step_dev, float* hist, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, off, nstored, pdep, path_cell, ratio,
+                       N, step, step_dev, hist);
 }
 }  // namespace sppm

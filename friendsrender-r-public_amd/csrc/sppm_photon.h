@@ -216,7 +216,7 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
     // direction density inside a cell: 1 / (cell area in (s, t)) * 2 |P|^3 (d omega = ds dt / (2 |P|^3))
     const float dd = (float)(GUIDE_GS * GUIDE_GT / 4) * (2.0f * plen * plen * plen);
     const float pc_prob = G.prob[cell] / total;
-    float q;
+    float q, p_plain;
     if (G.textured) {
         const DevLight& L = S.lights[tl];
         const float4 w = S.ltab_uvw[tj];
@@ -227,15 +227,17 @@ __device__ __forceinline__ void photon_start_guided(const SceneDev& S, const Slo
         const float mass = G.pcell_mass[pcell];
         const float prop = mass > 0.0f ? pc_prob * dd / mass : 0.0f;
         q = Dk * (G.w * cs / SPPM_PI + (1.0f - G.w) * prop);
+        p_plain = Dk * cs / SPPM_PI;
         rad0 = mk(Lk.x, Lk.y, Lk.z);
     } else {
         // cell density in (u, v, s, t) -> area x solid angle: / A
         const float p_mlp = pc_prob * (float)(GUIDE_GU * GUIDE_GV) * G.inv_area * dd;
-        const float p_plain = G.inv_area * cs / SPPM_PI;
+        p_plain = G.inv_area * cs / SPPM_PI;
         q = G.w * p_plain + (1.0f - G.w) * p_mlp;
         rad0 = mk(G.rad[0], G.rad[1], G.rad[2]);
     }
     G.path_cell[(size_t)(step & 1) * (size_t)N + i] = cell;
+    if (G.path_ratio) G.path_ratio[(size_t)(step & 1) * (size_t)N + i] = p_plain > 0.0f ? q / p_plain : 0.0f;
     const f3 rad = rad0 * smin(1.0f, cs);
     P.rad = rad;
     P.cur_org = lpos + Nn * SPPM_RAY_OFFSET;

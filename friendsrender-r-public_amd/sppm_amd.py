@@ -61,7 +61,11 @@ GUIDE_NPARAMS = 16705
 class GuideParams(C.Structure):
     _fields_ = [("enable", C.c_int32), ("mix_plain", C.c_float), ("learning_rate", C.c_float),
                 ("train_steps", C.c_int32), ("seed", C.c_uint32), ("history_decay", C.c_float),
-                ("hidden", C.c_int32), ("layers", C.c_int32), ("fp16", C.c_int32), ("train_interval", C.c_int32)]
+                ("hidden", C.c_int32), ("layers", C.c_int32), ("fp16", C.c_int32), ("train_interval", C.c_int32),
+                ("target", C.c_int32)]
+
+
+GUIDE_TARGET_MEAN, GUIDE_TARGET_RMS = 0, 1
 
 
 def guide_nparams(hidden=64, layers=4):
@@ -368,9 +372,9 @@ class Renderer:
 
     # ---- learned emission proposal --------------------------------------
     def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9,
-                  hidden=64, layers=4, fp16=False, train_interval=1):
+                  hidden=64, layers=4, fp16=False, train_interval=1, target=GUIDE_TARGET_MEAN):
         g = GuideParams(int(enable), mix_plain, learning_rate, train_steps, seed, history_decay, hidden, layers,
-                        int(fp16), int(train_interval))
+                        int(fp16), int(train_interval), int(target))
         _check(self.lib.sppm_guide_set(self.h, C.byref(g)))
         self.guide_np = guide_nparams(hidden, layers)
 

@@ -36,7 +36,8 @@ extern "C" {
 
 #define SPPM_ABI_VERSION 5  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
                                4: sppm_guide_params.train_interval, sppm_params.step_offset;
-                               5: sppm_params.photon_kernel, wave_bounces, wave_lanes */
+                               5: sppm_params.photon_kernel, wave_bounces, wave_lanes,
+                                  sppm_guide_params.target */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -365,7 +366,19 @@ typedef struct sppm_guide_params {
     int32_t fp16;          /* 1: fp16 MFMA, 0: bf16 */
     int32_t train_interval; /* train after every k-th pass (0, 1: every pass); the deposits of the
                                passes between gather into the histogram.  Captured passes need 0 or 1. */
+    int32_t target;        /* SPPM_GUIDE_TARGET_*: what the proposal is trained toward */
 } sppm_guide_params;
+/* Training targets of the proposal (the cross-entropy's weight per cell).
+ *   SPPM_GUIDE_TARGET_MEAN  the histogram of the paths' deposits (hit points
+ *                           reached x luminance / PDF): proportional to
+ *                           p_c E[c | cell], p_c the plain emission's cell
+ *                           probability and c a path's contribution;
+ *   SPPM_GUIDE_TARGET_RMS   sqrt(p_c S_c), S_c the histogram of deposit^2 x
+ *                           q / p (an estimate of p_c E_p[c^2 | cell]): the
+ *                           cell proposal that minimises the per-path variance
+ *                           (for emission inside a cell as the plain one). */
+#define SPPM_GUIDE_TARGET_MEAN 0
+#define SPPM_GUIDE_TARGET_RMS 1
 /* Parameter count: 64*hidden + hidden + (layers-1)*(hidden*hidden + hidden) + hidden + 1. */
 /* Enable (after sppm_scene_upload) or disable guided emission; re-initialises the weights. */
 int sppm_guide_set(sppm_ctx* ctx, const sppm_guide_params* params);

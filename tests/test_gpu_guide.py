@@ -136,14 +136,15 @@ def occluded_small():
     return scenes.occluded_scene(clutter_freq=6), scenes.occluded_camera(48, 48)
 
 
-def render(scene, cam, r0, photons, passes, guided, steps=4, seed=3):
+def render(scene, cam, r0, photons, passes, guided, steps=4, seed=3, target=sppm_amd.GUIDE_TARGET_MEAN):
     """alpha = 1: the radius stays r0, so the accumulated flux divided by the
     emitted photons estimates the same r0-smoothed radiance for any emission
     density (the reference's reconstruct also rescales by passes / NumGather,
     which depends on how often a pixel saw photons; it is left out here)."""
     r = sppm_amd.Renderer(scene, cam, 1.0, r0, photons, bvh_width=4)
     if guided:
-        r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=steps, seed=seed, history_decay=0.9)
+        r.guide_set(True, mix_plain=0.5, learning_rate=1e-3, train_steps=steps, seed=seed, history_decay=0.9,
+                    target=target)
     for _ in range(passes):
         r.pass_(stats=False)
     sh = r.read_shared()
@@ -153,13 +154,14 @@ def render(scene, cam, r0, photons, passes, guided, steps=4, seed=3):
     return est.reshape(cam["height"], cam["width"], 3)
 
 
-def test_guided_is_unbiased_at_fixed_radius():
+@pytest.mark.parametrize("target", [sppm_amd.GUIDE_TARGET_MEAN, sppm_amd.GUIDE_TARGET_RMS])
+def test_guided_is_unbiased_at_fixed_radius(target):
     scene, cam = occluded_small()
     r0 = 0.05
     plain = render(scene, cam, r0, 100000, 60, False)
-    guided = render(scene, cam, r0, 100000, 60, True)
+    guided = render(scene, cam, r0, 100000, 60, True, target=target)
     mp, mg = plain.mean(), guided.mean()
-    print("mean L plain %.5f guided %.5f" % (mp, mg), flush=True)
+    print("target %d: mean L plain %.5f guided %.5f" % (target, mp, mg), flush=True)
     assert abs(mg - mp) < 0.05 * mp
     # coarse blocks agree within noise
     bp = plain.reshape(6, 8, 6, 8, 3).mean((1, 3))
@@ -167,16 +169,17 @@ def test_guided_is_unbiased_at_fixed_radius():
     assert np.median(np.abs(bg - bp) / np.maximum(bp, 1e-6)) < 0.1
 
 
-def test_guided_textured_light_is_unbiased(full_scene_small):
+@pytest.mark.parametrize("target", [sppm_amd.GUIDE_TARGET_MEAN, sppm_amd.GUIDE_TARGET_RMS])
+def test_guided_textured_light_is_unbiased(full_scene_small, target):
     """Full-material scene, whose only light is a textured quad: the proposal
     picks table points inside its position cells.  Fixed radius, mean and
     coarse blocks of the smoothed radiance against plain emission."""
     cam = scenes.pinhole_camera(48, 48)
     r0 = 0.05
     plain = render(full_scene_small, cam, r0, 60000, 50, False)
-    guided = render(full_scene_small, cam, r0, 60000, 50, True)
+    guided = render(full_scene_small, cam, r0, 60000, 50, True, target=target)
     mp, mg = plain.mean(), guided.mean()
-    print("textured light: mean L plain %.5f guided %.5f" % (mp, mg), flush=True)
+    print("textured light, target %d: mean L plain %.5f guided %.5f" % (target, mp, mg), flush=True)
     assert np.isfinite(guided).all()
     assert abs(mg - mp) < 0.05 * mp
     bp = plain.reshape(6, 8, 6, 8, 3).mean((1, 3))
