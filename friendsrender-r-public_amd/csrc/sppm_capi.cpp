@@ -280,6 +280,8 @@ struct sppm_ctx {
     sppm_camera_desc cam{};
     bool have_cam = false;
     CamDev camdev{};
+    float* dia_prob = nullptr;  // thin-lens diaphragm table (CamDev::dia_*)
+    float4* dia_uv = nullptr;
     sppm_params params{};
     bool have_params = false;
     int pass = 0;
@@ -440,6 +442,8 @@ int sppm_destroy(sppm_ctx* c) {
     c->scene.release();
     for (Timer* t : {&c->t_photons, &c->t_build, &c->t_eye, &c->t_gather, &c->t_guide}) t->destroy();
     guide_free(c->guide);
+    dfree(c->dia_prob);
+    dfree(c->dia_uv);
     for (int b = 0; b < 2; ++b) {
         (void)hipEventDestroy(c->ev_words[b]);
         (void)hipEventDestroy(c->ev_eye[b]);
@@ -784,8 +788,64 @@ int sppm_camera_set(sppm_ctx* c, const sppm_camera_desc* cd) {
     C.F = cd->F;
     C.D = cd->dist_to_lens;
     C.sampler = cd->sampler;
+    C.dia_n = 0;
+    dfree(c->dia_prob);
+    dfree(c->dia_uv);
+    C.dia_prob = nullptr;
+    C.dia_uv = nullptr;
     c->have_cam = true;
     c->rb.ready = false;  // resolution may have changed
+    return SPPM_OK;
+}
+
+// ThinLensCamera(..., DiaphragmImageFileName) (Camera.h:136-158): the alias
+// table over the image's pixels, weight 1 - red, and the open-pixel share
+// that selects plain lens sampling (Camera.h:106-107).
+int sppm_camera_diaphragm(sppm_ctx* c, int32_t width, int32_t height, const float* rgba) {
+    if (!c) return fail(SPPM_E_ARG, "NULL ctx");
+    if (int rc = not_in_batch(c, "sppm_camera_diaphragm")) return rc;
+    if (!c->have_cam) return fail(SPPM_E_STATE, "no camera set (sppm_camera_set first)");
+    HIPOK(hipSetDevice(c->device));
+    CamDev& C = c->camdev;
+    C.dia_n = 0;
+    dfree(c->dia_prob);
+    dfree(c->dia_uv);
+    C.dia_prob = nullptr;
+    C.dia_uv = nullptr;
+    c->rb.ready = false;
+    if (!rgba) return SPPM_OK;
+    if (c->cam.type != SPPM_CAM_THINLENS) return fail(SPPM_E_ARG, "a diaphragm needs a thin-lens camera");
+    if (width <= 0 || height <= 0 || (long long)width * height > (1LL << 26))
+        return fail(SPPM_E_ARG, "diaphragm image %dx%d: sides > 0, at most 2^26 pixels", width, height);
+    const int n = width * height;
+    std::vector<float> L(n);
+    int open = 0;
+    for (int i = 0; i < n; ++i) {
+        const float r = rgba[4 * (size_t)i];
+        if ((double)r < 1e-6) ++open;
+        L[i] = (float)(1.0 - (double)r);
+    }
+    const float hole = (float)open / (float)n;
+    if ((double)hole > 0.999) return SPPM_OK;  // plain lens sampling
+    std::vector<float> prob;
+    std::vector<int> left, right;
+    float avg = 0.0f;
+    H::build_alias(L, prob, left, right, avg);
+    if (!(avg > 0.0f)) return fail(SPPM_E_ARG, "the diaphragm image has no open pixel (all red >= 1)");
+    auto lens_u = [&](int i) { return (float)(i % width) / (float)width; };
+    auto lens_v = [&](int i) { return (float)(1.0 - (double)((float)(i / width) / (float)height)); };
+    std::vector<float4> uv(n);
+    for (int k = 0; k < n; ++k) {
+        const int l = left[k], r = right[k] < 0 ? l : right[k];
+        uv[k] = make_float4(lens_u(l), lens_v(l), lens_u(r), lens_v(r));
+    }
+    int rc;
+    if ((rc = dalloc(&c->dia_prob, (size_t)n)) || (rc = dalloc(&c->dia_uv, (size_t)n))) return rc;
+    HIPOK(hipMemcpy(c->dia_prob, prob.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->dia_uv, uv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    C.dia_n = n;
+    C.dia_prob = c->dia_prob;
+    C.dia_uv = c->dia_uv;
     return SPPM_OK;
 }
 

@@ -95,6 +95,12 @@ struct CamDev {
     float PW, PH, LW, LH, F, D;
     int sampler;         // SPPM_SAMPLER_*
     float fixed_smp[3];  // Halton / Hammersley sample of this pass (index = step)
+    // thin-lens diaphragm (Camera.h:106-111): alias table over the aperture
+    // image's pixels, dia_n bins (0: plain lens sampling); bin k keeps its
+    // left-pick probability and the lens (u, v) of its left and right pixels
+    int dia_n;
+    const float* dia_prob;
+    const float4* dia_uv;
 };
 
 // ---- photon slots: std::vector<std::deque<Photon>> (SPPM.cpp:753) --------

@@ -345,8 +345,20 @@ __device__ inline void camera_ray(const CamDev& c, float u, float v, f3 smp, f3&
         org = img;
         dir = normalized(normalized(pos - img));
     } else {
+        float lu = smp.x, lv = smp.y;
+        if (c.dia_n > 0) {
+            // TableSampler::Sample(Vec2) (TableSampler.h:129-132): bin
+            // n * x (float product, truncated), left pixel when y < its
+            // probability.  x = 1 (a float-rounded canonical draw) would index
+            // past the table in the reference; it takes the last bin here.
+            const int k = min((int)((float)c.dia_n * smp.x), c.dia_n - 1);
+            const float4 uv = c.dia_uv[k];
+            const bool left = smp.y < c.dia_prob[k];
+            lu = left ? uv.x : uv.z;
+            lv = left ? uv.y : uv.w;
+        }
         float pdf_area = 1.0f / (c.LW * c.LH);
-        f3 lens = (pos + (e1 * (smp.x - 0.5f)) * c.LW) + (e2 * (smp.y - 0.5f)) * c.LH;
+        f3 lens = (pos + (e1 * (lu - 0.5f)) * c.LW) + (e2 * (lv - 0.5f)) * c.LH;
         f3 img = pos - cd * c.D;
         img = img + (e1 * (u - 0.5f)) * c.PW;
         img = img + (e2 * (v - 0.5f)) * c.PH;

@@ -165,8 +165,12 @@ ThinLensCamera::ThinLensCamera(int ScreenWidth, int ScreenHeight, const Vec3& Ca
                                float Fov, float F, float DistanceToLens, float AspectRatio,
                                iCameraSsmpler* CameraSampler, const char* DiaphragmImageFileName)
     : iCamera(ScreenWidth, ScreenHeight, CameraPos, Target, Fov, F, DistanceToLens, AspectRatio, CameraSampler) {
-    if (DiaphragmImageFileName != nullptr)
-        throw std::invalid_argument("ThinLensCamera: a diaphragm image is not supported by the SPPM pass core");
+    if (DiaphragmImageFileName != nullptr) {
+        if (!TextureUtility::LoadTextureFromFile(_DiaphragmImage, DiaphragmImageFileName))
+            throw std::invalid_argument(std::string("ThinLensCamera: cannot load diaphragm image ") +
+                                        DiaphragmImageFileName);
+        _HasDiaphragm = true;
+    }
 }
 
 // ---- Scene: OBJ / MTL ingestion (Scene.cpp:13-218) ------------------------------
@@ -531,6 +535,8 @@ void StochasticProgressivePhotonMapping::prepare(const Scene& scene, const iCame
     c.sampler_id = camera._CameraSampler->m_ID;
     c.sampler_nsample = camera._CameraSampler->m_NumSample;
     check(sppm_camera_set(I.ctx, &c), "sppm_camera_set");
+    if (const Texture2D* d = camera.Diaphragm())
+        check(sppm_camera_diaphragm(I.ctx, d->GetWidth(), d->GetHeight(), d->Data()), "sppm_camera_diaphragm");
     sppm_params p;
     memset(&p, 0, sizeof(p));
     p.alpha = m_Alpha;

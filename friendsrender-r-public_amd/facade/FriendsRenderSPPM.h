@@ -21,7 +21,8 @@
 //   * the environment light (IBL texture argument) lights escaping eye
 //     paths as the reference's does (SPHERE mapping, colour 0.4 c^2); like
 //     the reference, it emits no photons;
-//   * ThinLensCamera with a diaphragm image is not supported (throws);
+//   * a ThinLensCamera diaphragm image that cannot be read throws (the
+//     reference goes on with an empty image);
 //   * textures referenced by an MTL file are read from binary PPM (P6, 8-bit,
 //     stored as byte/255 like the reference's loader) or PFM (float) files;
 //   * surface kinds outside the five above make CreateScene return false.
@@ -173,6 +174,8 @@ struct iCamera {
           _CameraSampler(CameraSampler) {}
     virtual ~iCamera() {}
     virtual int Type() const = 0;
+    // the thin lens's diaphragm image (NULL: none)
+    virtual const Texture2D* Diaphragm() const { return nullptr; }
 };
 struct PinholeCamera : iCamera {
     using iCamera::iCamera;
@@ -183,6 +186,11 @@ struct ThinLensCamera : iCamera {
                    const PTUtility::Vec3& Target, float Fov, float F, float DistanceToLens, float AspectRatio,
                    iCameraSsmpler* CameraSampler, const char* DiaphragmImageFileName);
     int Type() const override;
+    const Texture2D* Diaphragm() const override { return _HasDiaphragm ? &_DiaphragmImage : nullptr; }
+
+private:
+    Texture2D _DiaphragmImage;
+    bool _HasDiaphragm = false;
 };
 
 // Scene (Scene.h:55-93): facets, attributes and textures on the host; the

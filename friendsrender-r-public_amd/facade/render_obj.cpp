@@ -16,6 +16,7 @@
 //   --bvh W               BVH width 2, 4 or 8
 //   --sampler-id ID       UniformSample seed ID
 //   --ibl FILE            environment map (binary PPM or PFM), Scene::CreateScene's IBL argument
+//   --diaphragm FILE      thin-lens diaphragm image (binary PPM or PFM), ThinLensCamera's last argument
 //   --nee M               eye-path next-event estimation: 0 off, 1 as written, 2 kept
 //   --dump-scene FILE     write the parsed facet records and attribute table
 //                         (no rendering, no GPU)
@@ -101,6 +102,7 @@ int main(int argc, char** argv) {
     float pos[3] = {0.5f, 0.5f, 2.2f}, tgt[3] = {0.5f, 0.5f, 0.5f};
     const char* dump = nullptr;
     const char* ibl = nullptr;
+    const char* diaphragm = nullptr;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
         auto need = [&](int n) {
@@ -123,6 +125,7 @@ int main(int argc, char** argv) {
         else if (a == "--radius") { need(1); radius = (float)atof(argv[++i]); }
         else if (a == "--bvh") { need(1); bvh = atoi(argv[++i]); }
         else if (a == "--ibl") { need(1); ibl = argv[++i]; }
+        else if (a == "--diaphragm") { need(1); diaphragm = argv[++i]; }
         else if (a == "--nee") { need(1); nee = atoi(argv[++i]); }
         else if (a == "--sampler-id") { need(1); sid = atoi(argv[++i]); }
         else if (a == "--dump-scene") { need(1); dump = argv[++i]; }
@@ -154,7 +157,7 @@ int main(int argc, char** argv) {
         render.SetBVHWidth(bvh);
         render.SetEyeNEE(nee);
         PinholeCamera pin(W, H, P, T, fov, 1.0f, 1.0f, aspect, &sampler);
-        ThinLensCamera lens(W, H, P, T, fov, F, A, aspect, &sampler, nullptr);
+        ThinLensCamera lens(W, H, P, T, fov, F, A, aspect, &sampler, diaphragm);
         const iCamera& cam = thin ? (const iCamera&)lens : (const iCamera&)pin;
         if (seconds >= 0)
             render.RenderImage_FixRenderTime(target, scene, cam, photons, seconds);

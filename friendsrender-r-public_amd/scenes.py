@@ -148,6 +148,38 @@ def thinlens_camera(width, height, aperture=0.02, focus=2.0, fov_deg=45.0, sampl
                 sampler_id=sampler_id, sampler_nsample=1)
 
 
+def aperture_image(width, height, kind="hexagon", seed=0):
+    """A thin-lens diaphragm image, (height, width, 4) float32 with red 0 where
+    the aperture is open and 1 where it is closed (ThinLensCamera weights each
+    pixel by 1 - red): "hexagon", a hexagonal opening with a soft edge, or
+    "noise", random red with a few fully open pixels.  Values are multiples
+    of 1/255, so an 8-bit PPM of the image loads back to the same floats."""
+    rng = np.random.default_rng(seed)
+    if kind == "hexagon":
+        y, x = np.mgrid[0:height, 0:width]
+        px = (x + 0.5) / width * 2.0 - 1.0
+        py = (y + 0.5) / height * 2.0 - 1.0
+        r = np.max([np.abs(px * np.cos(a) + py * np.sin(a)) for a in (0.0, np.pi / 3, 2 * np.pi / 3)], axis=0)
+        red = np.clip((r - 0.6) / 0.2, 0.0, 1.0)
+    elif kind == "noise":
+        red = rng.random((height, width))
+        red[rng.random((height, width)) < 0.1] = 0.0
+    else:
+        raise ValueError(kind)
+    img = np.zeros((height, width, 4), np.float32)
+    img[..., 0] = np.round(red * 255.0).astype(np.float32) / np.float32(255.0)
+    img[..., 3] = 1.0
+    return img
+
+
+def write_ppm(img, path):
+    """8-bit binary PPM (P6) of the RGB channels of a (height, width, 4) image in [0, 1]."""
+    h, w = img.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"P6\n%d %d\n255\n" % (w, h))
+        f.write(np.round(np.clip(img[..., :3], 0.0, 1.0) * 255.0).astype(np.uint8).tobytes())
+
+
 # ---- procedural textures ----------------------------------------------------
 def _value_noise(n, seed, octaves=3, base=8):
     rng = np.random.default_rng(seed)

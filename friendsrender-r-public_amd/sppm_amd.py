@@ -34,7 +34,7 @@ EXPORTED = [
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
     "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
-    "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells",
+    "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells", "sppm_camera_diaphragm",
 ]
 TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
@@ -136,6 +136,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_destroy.argtypes = [C.c_void_p]
     lib.sppm_scene_upload.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int]
     lib.sppm_camera_set.argtypes = [C.c_void_p, C.POINTER(CameraDesc)]
+    lib.sppm_camera_diaphragm.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
     lib.sppm_params_set.argtypes = [C.c_void_p, C.POINTER(Params)]
     lib.sppm_begin.argtypes = [C.c_void_p]
     lib.sppm_pass.argtypes = [C.c_void_p, C.POINTER(PassStats)]
@@ -257,6 +258,15 @@ def camera_desc(cam):
     return c
 
 
+def diaphragm_rgba(img):
+    """A thin-lens diaphragm image as (height, width, 4) float32 RGBA, pixel
+    (x, y) at [y, x] (storage order x + y * width); only red is read."""
+    a = np.ascontiguousarray(img, np.float32)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("diaphragm image must be (height, width, 4) RGBA")
+    return a
+
+
 RECORD_OFF, RECORD_LISTS, RECORD_PAIRS = 0, 1, 2  # sppm_params.record_neighbours
 
 
@@ -294,6 +304,9 @@ class Renderer:
         _check(self.lib.sppm_scene_upload(self.h, C.byref(d), bvh_width))
         self._cam = camera_desc(cam)
         _check(self.lib.sppm_camera_set(self.h, C.byref(self._cam)))
+        if cam.get("diaphragm") is not None:
+            self._dia = diaphragm_rgba(cam["diaphragm"])
+            _check(self.lib.sppm_camera_diaphragm(self.h, self._dia.shape[1], self._dia.shape[0], _fp(self._dia)))
         self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs,
                                      step_offset=step_offset, photon_kernel=photon_kernel, wave_bounces=wave_bounces,
                                      wave_lanes=wave_lanes)

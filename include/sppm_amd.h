@@ -108,7 +108,7 @@ typedef struct sppm_scene_desc {
 
 /* ---- camera ------------------------------------------------------------ */
 #define SPPM_CAM_PINHOLE 0    /* PinholeCamera  Camera.h:72-87 */
-#define SPPM_CAM_THINLENS 1   /* ThinLensCamera Camera.h:89-160 (no diaphragm) */
+#define SPPM_CAM_THINLENS 1   /* ThinLensCamera Camera.h:89-160 (diaphragm: sppm_camera_diaphragm) */
 
 #define SPPM_SAMPLER_UNIFORM 0    /* UniformSample(ID): one shared mt19937 stream */
 #define SPPM_SAMPLER_HALTON 1     /* HaltonSample */
@@ -217,7 +217,20 @@ int sppm_destroy(sppm_ctx* ctx);
  * Morton codes; either is collapsed to `bvh_width`-wide nodes. */
 #define SPPM_BVH_LBVH 0x100
 int sppm_scene_upload(sppm_ctx* ctx, const sppm_scene_desc* scene, int bvh_width);
+/* Clears the thin-lens diaphragm (sppm_camera_diaphragm); the render must be
+ * begun again (sppm_begin). */
 int sppm_camera_set(sppm_ctx* ctx, const sppm_camera_desc* cam);
+/* ThinLensCamera's diaphragm image (the DiaphragmImageFileName argument,
+ * Camera.h:136-158, and the lens sampling of Camera.h:106-111): the image the
+ * reference loads from that file, width x height RGBA float32 in the
+ * texture's storage order (pixel i = x + y * width).  Lens points are drawn
+ * with the camera sampler's first two numbers from an alias table over the
+ * pixels weighted by 1 - red; pixel i sits at lens (u, v) = ((i % width) /
+ * width, 1 - (i / width) / height).  When more than 99.9 % of the pixels are
+ * open (red < 1e-6) the lens keeps its plain sampling, as in the reference.
+ * rgba = NULL clears it.  Thin-lens cameras only; call after sppm_camera_set
+ * and before sppm_begin. */
+int sppm_camera_diaphragm(sppm_ctx* ctx, int32_t width, int32_t height, const float* rgba);
 int sppm_params_set(sppm_ctx* ctx, const sppm_params* params);
 /* initializeSharedPoints + photon slot allocation; resets the pass counter. */
 int sppm_begin(sppm_ctx* ctx);

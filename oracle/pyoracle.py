@@ -17,7 +17,8 @@ REF_PATH = os.path.join(HERE, "_ref", "libsppm_ref.so")
 _PKG = os.path.join(os.path.dirname(HERE), "friendsrender-r-public_amd")
 if _PKG not in sys.path:
     sys.path.insert(0, _PKG)
-from sppm_amd import CameraDesc, Params, SceneDesc, camera_desc, params_struct, scene_desc  # noqa: E402
+from sppm_amd import (CameraDesc, Params, SceneDesc, camera_desc, diaphragm_rgba, params_struct,  # noqa: E402
+                      scene_desc)
 
 F = C.POINTER(C.c_float)
 I = C.POINTER(C.c_int32)
@@ -67,6 +68,9 @@ def _load_oracle():
     L.orc_closest_hit_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, F, F, C.c_float, C.c_float, I, F, F, F]
     L.orc_occluded_batch.argtypes = [C.c_void_p, C.c_int, F, F, C.c_float, F, C.c_float, I]
     L.orc_camera_ray.argtypes = [C.POINTER(CameraDesc), C.c_float, C.c_float, F, F, F, F]
+    L.orc_camera_ray_diaphragm.argtypes = [C.POINTER(CameraDesc), C.c_int, C.c_int, F, C.c_float, C.c_float, F, F,
+                                           F, F]
+    L.orc_render_diaphragm.argtypes = [C.c_void_p, C.c_int, C.c_int, F]
     L.orc_render_create.argtypes = [C.c_void_p, C.POINTER(CameraDesc), C.POINTER(Params), C.c_int]
     L.orc_render_create.restype = C.c_void_p
     L.orc_render_destroy.argtypes = [C.c_void_p]
@@ -309,6 +313,10 @@ class OracleRender:
         self.p = params_struct(alpha, first_radius, photons, nee=nee)
         nt = nthreads or min(16, os.cpu_count() or 1)
         self.h = orc().orc_render_create(oscene.h, C.byref(self.cam), C.byref(self.p), nt)
+        if cam.get("diaphragm") is not None:
+            self._dia = diaphragm_rgba(cam["diaphragm"])
+            if orc().orc_render_diaphragm(self.h, self._dia.shape[1], self._dia.shape[0], fp(self._dia)):
+                raise ValueError("diaphragm image has no open pixel")
         self.W, self.H, self.N = cam["width"], cam["height"], photons
         self.passes = 0
         self.total_photons = 0
@@ -451,5 +459,11 @@ def camera_ray(cam, u, v, sample3):
     c = camera_desc(cam)
     s = np.ascontiguousarray(sample3, np.float32)
     org, d, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
-    orc().orc_camera_ray(C.byref(c), u, v, fp(s), fp(org), fp(d), fp(pdf))
+    if cam.get("diaphragm") is not None:
+        img = diaphragm_rgba(cam["diaphragm"])
+        if orc().orc_camera_ray_diaphragm(C.byref(c), img.shape[1], img.shape[0], fp(img), u, v, fp(s), fp(org),
+                                          fp(d), fp(pdf)):
+            raise ValueError("diaphragm image has no open pixel")
+    else:
+        orc().orc_camera_ray(C.byref(c), u, v, fp(s), fp(org), fp(d), fp(pdf))
     return org, d, pdf[0]

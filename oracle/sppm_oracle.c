@@ -1182,8 +1182,16 @@ typedef struct {
     int type, W, H;
     v3 pos, dir, e1, e2;
     float fov, PW, PH, LW, LH, aspect, F, D;
+    /* ThinLensCamera's m_Table over the diaphragm image (dn pixels; 0: plain
+     * lens sampling, m_HoleRatio > 0.999 or no image) */
+    int dn;
+    float *dprob, *du, *dv;
+    int32_t *dleft, *dright;
 } ocam;
 static void cam_setup(const sppm_camera_desc* c, ocam* o) {
+    o->dn = 0;
+    o->dprob = o->du = o->dv = NULL;
+    o->dleft = o->dright = NULL;
     o->type = c->type; o->W = c->width; o->H = c->height;
     o->pos = vload(c->pos);
     o->dir = vnormalized(vsub(vload(c->target), o->pos));
@@ -1211,8 +1219,21 @@ static void cam_ray(const ocam* o, float u, float v, v3 smp, v3* org, v3* dir, f
         *pdf = 1.0f;
         *org = img;
         *dir = vnormalized(vnormalized(vsub(lens, img)));
-    } else {  /* Camera.h:103-129 (no diaphragm: m_HoleRatio = 1) */
+    } else {  /* Camera.h:103-129 */
         v3 luv = smp;
+        if (o->dn > 0) {
+            /* LensUV = m_Table.Sample(Sample(index).xy()).UV (Camera.h:109-111):
+             * bin = m_NumElements (int) * x in float, truncated; left when
+             * y < probLeftSample (TableSampler.h:130-133).  A bin index of n
+             * (x rounded up to 1) or a right id of -1 (y = 1 on a bin of
+             * probability 1) are out of the table in the reference: the last
+             * bin and the left pixel are taken. */
+            int bin = (int)((float)o->dn * smp.x);
+            if (bin > o->dn - 1) bin = o->dn - 1;
+            int id = (smp.y < o->dprob[bin]) ? o->dleft[bin] : o->dright[bin];
+            if (id < 0) id = o->dleft[bin];
+            luv = V3(o->du[id], o->dv[id], 0.0f);
+        }
         float pdf_area = 1.0f / (o->LW * o->LH);
         v3 lens = vadd(vadd(o->pos, vscale(vscale(o->e1, luv.x - 0.5f), o->LW)),
                        vscale(vscale(o->e2, luv.y - 0.5f), o->LH));
@@ -1229,6 +1250,64 @@ static void cam_ray(const ocam* o, float u, float v, v3 smp, v3* org, v3* dir, f
         *org = lens;
         *dir = vnormalized(vnormalized(vsub(target, lens)));
     }
+}
+static void cam_diaphragm_free(ocam* o) {
+    free(o->dprob); free(o->du); free(o->dv); free(o->dleft); free(o->dright);
+    o->dprob = o->du = o->dv = NULL;
+    o->dleft = o->dright = NULL;
+    o->dn = 0;
+}
+/* ThinLensCamera(..., DiaphragmImageFileName) (Camera.h:136-158): per pixel
+ * i of the image, UV = ((i % W) / W, 1 - (i / W) / H) and L = 1 - red; a
+ * pixel is a hole when red < 1e-6; m_HoleRatio = holes / (W H) in float and
+ * more than 0.999 keeps plain lens sampling; else m_Table = alias table over
+ * L.  Returns 0, or -1 when the image has no weight (the reference's table
+ * would divide by a zero average). */
+static int cam_diaphragm(ocam* o, int w, int h, const float* rgba) {
+    cam_diaphragm_free(o);
+    if (!rgba || w <= 0 || h <= 0) return 0;
+    const int n = w * h;
+    float* L = (float*)malloc(sizeof(float) * (size_t)n);
+    o->du = (float*)malloc(sizeof(float) * (size_t)n);
+    o->dv = (float*)malloc(sizeof(float) * (size_t)n);
+    int holes = 0;
+    for (int i = 0; i < n; ++i) {
+        const float red = rgba[4 * (size_t)i];
+        if ((double)red < 1e-6) ++holes;
+        o->du[i] = (float)(i % w) / (float)w;
+        o->dv[i] = (float)(1.0 - (double)((float)(i / w) / (float)h));
+        L[i] = (float)(1.0 - (double)red);
+    }
+    const float ratio = (float)holes / (float)n;
+    if ((double)ratio > 0.999) {
+        free(L);
+        cam_diaphragm_free(o);
+        return 0;
+    }
+    o->dprob = (float*)malloc(sizeof(float) * (size_t)n);
+    o->dleft = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    o->dright = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    float avg = 0.0f;
+    orc_table_build(L, n, o->dprob, o->dleft, o->dright, &avg);
+    free(L);
+    if (!(avg > 0.0f)) {
+        cam_diaphragm_free(o);
+        return -1;
+    }
+    o->dn = n;
+    return 0;
+}
+int orc_camera_ray_diaphragm(const sppm_camera_desc* c, int dw, int dh, const float* drgba, float img_u,
+                             float img_v, const float* sample3, float* org, float* dir, float* pdf) {
+    ocam o;
+    cam_setup(c, &o);
+    if (cam_diaphragm(&o, dw, dh, drgba)) return -1;
+    v3 a, b;
+    cam_ray(&o, img_u, img_v, vload(sample3), &a, &b, pdf);
+    cam_diaphragm_free(&o);
+    vstore(org, a);
+    vstore(dir, b);
+    return 0;
 }
 void orc_camera_ray(const sppm_camera_desc* c, float img_u, float img_v, const float* sample3,
                     float* org, float* dir, float* pdf) {
@@ -1307,8 +1386,12 @@ orc_render* orc_render_create(const orc_scene* s, const sppm_camera_desc* cam, c
     if (cam->sampler == SPPM_SAMPLER_UNIFORM) orc_mt_seed(&r->camstream, (uint32_t)cam->sampler_id);
     return r;
 }
+int orc_render_diaphragm(orc_render* r, int w, int h, const float* rgba) {
+    return cam_diaphragm(&r->cam, w, h, rgba);
+}
 void orc_render_destroy(orc_render* r) {
     if (!r) return;
+    cam_diaphragm_free(&r->cam);
     free(r->slots); free(r->flat); free(r->hits); free(r->shared); free(r->counts);
     free(r->nb_off); free(r->nb_idx);
     free(r);

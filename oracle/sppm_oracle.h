@@ -22,6 +22,9 @@
  *   BSDFs ............................... closed forms, MC pdf normalisation,
  *                                         reflection/refraction laws
  *   camera .............................. closed form (central ray, pinhole geometry)
+ *   thin-lens diaphragm ................. one-open-pixel images land on that pixel's
+ *                                         lens UV; pick frequencies == 1 - red;
+ *                                         open images == plain lens sampling
  *   neighbour sets / update / reconstruct  brute force + hand-built recurrences
  *   BiMetal / DiffuseTransmit / GlossTransmit  closed forms, density normalisation,
  *                                         mirror symmetry, agreement with Metal/Roughconductor
@@ -117,12 +120,18 @@ void orc_occluded_batch(const orc_scene* s, int n, const float* org, const float
 /* One SampleCameraRay with a given sampler value (3 floats) -> org, dir, pdf */
 void orc_camera_ray(const sppm_camera_desc* c, float img_u, float img_v, const float* sample3,
                     float* org, float* dir, float* pdf);
+/* The same with ThinLensCamera's diaphragm image (dw x dh RGBA, NULL: none);
+ * -1 when the image has no open pixel */
+int orc_camera_ray_diaphragm(const sppm_camera_desc* c, int dw, int dh, const float* drgba, float img_u,
+                             float img_v, const float* sample3, float* org, float* dir, float* pdf);
 
 /* ---- the SPPM pass ------------------------------------------------------ */
 typedef struct orc_render orc_render;
 orc_render* orc_render_create(const orc_scene* s, const sppm_camera_desc* cam,
                               const sppm_params* p, int nthreads);
 void orc_render_destroy(orc_render* r);
+/* ThinLensCamera's diaphragm image for the render's camera (NULL: none) */
+int orc_render_diaphragm(orc_render* r, int w, int h, const float* rgba);
 /* createPhotonMap(step): returns processed photons */
 int orc_photon_pass(orc_render* r, int step);
 /* flatten slots (SPPM.cpp:812-822); returns stored count */

@@ -127,3 +127,36 @@ def test_facade_render_matches_capi_and_oracle(tmp_path, box_scene, env):
     ref = o.reconstruct(N * P, P).reshape(H, W, 4)[..., :3]
     assert rel_rmse(img, ref) <= 1e-4
     assert "passes 3" in p.stdout
+
+
+@pytest.mark.gpu
+def test_facade_thinlens_diaphragm_matches_capi_and_oracle(tmp_path, box_scene):
+    """ThinLensCamera(..., DiaphragmImageFileName) through the facade: the
+    image is read from a binary PPM and renders as the C ABI with the same
+    image and as the oracle."""
+    import pyoracle
+    W, H, N, P = 48, 48, 15000, 2
+    obj = scenes.write_obj(box_scene, str(tmp_path), "box")
+    img = scenes.aperture_image(24, 18, "hexagon")
+    dia = str(tmp_path / "aperture.ppm")
+    scenes.write_ppm(img, dia)
+    cam = dict(scenes.thinlens_camera(W, H, aperture=0.05, focus=1.5), diaphragm=img)
+    out = str(tmp_path / "img.pfm")
+    args = [RENDER_OBJ, obj, out, "--size", str(W), str(H), "--photons", str(N), "--passes", str(P),
+            "--thinlens", repr(cam["dist_to_lens"]), repr(cam["F"]), "--pos", *map(str, cam["pos"]),
+            "--target", *map(str, cam["target"]), "--fov", str(cam["fov_deg"]), "--alpha", "0.9", "--radius",
+            "0.015", "--bvh", "4", "--diaphragm", dia]
+    print("facade render:", " ".join(args[2:]), flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    got = read_pfm(out)
+    r = sppm_amd.Renderer(box_scene, cam, 0.9, 0.015, N, bvh_width=4)
+    for _ in range(P):
+        r.pass_(stats=False)
+    ref_gpu = r.reconstruct(N * P, P).reshape(H, W, 4)[..., :3]
+    r.close()
+    assert rel_rmse(got, ref_gpu) <= 1e-6
+    o = pyoracle.OracleRender(pyoracle.OracleScene(box_scene), cam, 0.9, 0.015, N)
+    for _ in range(P):
+        o.full_pass()
+    assert rel_rmse(got, o.reconstruct(N * P, P).reshape(H, W, 4)[..., :3]) <= 1e-4

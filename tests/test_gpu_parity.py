@@ -100,6 +100,17 @@ def test_pass_parity_full_materials_thinlens(full_scene_small):
     run_parity(full_scene_small, scenes.thinlens_camera(48, 48), 0.7, r0, 15000, 2, 8, "full thinlens")
 
 
+@pytest.mark.parametrize("kind,sampler", [("hexagon", sppm_amd.SAMPLER_UNIFORM), ("noise", sppm_amd.SAMPLER_UNIFORM),
+                                          ("hexagon", sppm_amd.SAMPLER_HALTON)])
+def test_pass_parity_thinlens_diaphragm(full_scene_small, kind, sampler):
+    """ThinLensCamera with a diaphragm image (Camera.h:106-111, 136-158): lens
+    points from the alias table over the aperture's pixels."""
+    r0 = 0.02 * full_scene_small.diagonal()
+    cam = dict(scenes.thinlens_camera(48, 48, aperture=0.05, focus=1.5), sampler=sampler,
+               diaphragm=scenes.aperture_image(40, 30, kind, seed=3))
+    run_parity(full_scene_small, cam, 0.7, r0, 15000, 2, 8, f"diaphragm {kind} sampler {sampler}")
+
+
 def test_pass_parity_balanced_gather_box(box_scene):
     # production gather (chunked, load-balanced): counts and update exact, flux within tolerance
     r0 = 0.02 * box_scene.diagonal()

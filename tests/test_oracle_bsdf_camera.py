@@ -184,6 +184,86 @@ def test_pinhole_camera_geometry():
     assert abs(ang - 22.5) < 1e-3
 
 
+def lens_uv(cam, o):
+    """Lens (u, v) of a thin-lens ray origin: o = pos + (u - 1/2) e1 LW + (v - 1/2) e2 LH
+    (Camera.h:34-53 for the basis and the lens size)."""
+    pos = np.asarray(cam["pos"], np.float64)
+    d = norm(np.subtract(cam["target"], cam["pos"]))
+    e1 = norm(np.cross([0.0, 1.0, 0.0], d))
+    e2 = norm(np.cross(d, e1))
+    half = np.float32(np.pi) * np.float32(cam["fov_deg"]) / np.float32(360.0)
+    LH = 2.0 * cam["dist_to_lens"] * math.tan(float(half))
+    LW = LH * cam["aspect"]
+    r = np.asarray(o, np.float64) - pos
+    return r @ e1 / LW + 0.5, r @ e2 / LH + 0.5
+
+
+def test_diaphragm_single_open_pixel_fixes_the_lens_point():
+    """ThinLensCamera with a diaphragm image (Camera.h:106-111, 136-158):
+    with one open pixel every lens sample lands on that pixel's lens point
+    ((i % W) / W, 1 - (i / W) / H) -- pins the pixel order and orientation."""
+    W, H = 5, 3
+    base = scenes.thinlens_camera(32, 32, aperture=0.05, focus=1.5)
+    rng = np.random.default_rng(8)
+    for i in (0, 4, 7, 10, 14):
+        img = np.ones((H, W, 4), np.float32)
+        img.reshape(-1, 4)[i, 0] = 0.0
+        cam = dict(base, diaphragm=img)
+        want = ((i % W) / W, 1.0 - (i // W) / H)
+        for _ in range(8):
+            o, d, pdf = po.camera_ray(cam, *rng.random(2), rng.random(3))
+            u, v = lens_uv(cam, o)
+            assert abs(u - want[0]) < 1e-4 and abs(v - want[1]) < 1e-4, (i, u, v, want)
+            assert pdf > 0
+
+
+def test_diaphragm_pick_frequencies_follow_one_minus_red():
+    """Lens pixels are picked with probability (1 - red) / sum (1 - red): a
+    deterministic grid of sampler pairs (x through every table bin, y over
+    [0, 1)) against the weights (alpha, green and blue are ignored)."""
+    W, H = 8, 4
+    rng = np.random.default_rng(9)
+    img = rng.random((H, W, 4)).astype(np.float32)
+    img[0, :3, 0] = 0.0  # holes
+    img[2, 5, 0] = 1.0   # closed
+    n = W * H
+    cam = dict(scenes.thinlens_camera(32, 32, aperture=0.05, focus=1.5), diaphragm=img)
+    B = 400
+    counts = np.zeros(n)
+    for a in range(2 * n):
+        for b in range(B):
+            o, _, _ = po.camera_ray(cam, 0.5, 0.5, [(a + 0.5) / (2 * n), (b + 0.5) / B, 0.0])
+            u, v = lens_uv(cam, o)
+            x, y = int(round(u * W)), int(round((1.0 - v) * H))
+            counts[y * W + x] += 1
+    freq = counts / counts.sum()
+    w = 1.0 - img[..., 0].reshape(-1).astype(np.float64)
+    assert freq[2 * W + 5] == 0.0
+    np.testing.assert_allclose(freq, w / w.sum(), atol=4.0 / (n * B))
+
+
+def test_diaphragm_open_image_keeps_plain_lens_sampling():
+    """m_HoleRatio > 0.999 (holes = pixels with red < 1e-6, ratio in float)
+    keeps the plain lens sampling: all open, and 999 of 1000 open (0.999f is
+    above 0.999); 998 of 1000 open uses the table."""
+    base = scenes.thinlens_camera(32, 32, aperture=0.05, focus=1.5)
+    rng = np.random.default_rng(10)
+    smp = [rng.random(3) for _ in range(6)]
+    plain = [po.camera_ray(base, 0.3, 0.6, s) for s in smp]
+    for closed, same in ((0, True), (1, True), (2, False)):
+        img = np.zeros((20, 50, 4), np.float32)
+        img.reshape(-1, 4)[:closed, 0] = 1.0
+        got = [po.camera_ray(dict(base, diaphragm=img), 0.3, 0.6, s) for s in smp]
+        eq = all(np.array_equal(g[0], p_[0]) and np.array_equal(g[1], p_[1]) for g, p_ in zip(got, plain))
+        assert eq == same, closed
+
+
+def test_diaphragm_without_open_pixel_is_refused():
+    img = np.ones((4, 4, 4), np.float32)
+    with pytest.raises(ValueError):
+        po.camera_ray(dict(scenes.thinlens_camera(32, 32), diaphragm=img), 0.5, 0.5, [0.5, 0.5, 0.5])
+
+
 def test_thinlens_focus_and_pdf():
     cam = scenes.thinlens_camera(64, 64, aperture=0.02, focus=2.0)
     rng = np.random.default_rng(6)
