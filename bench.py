@@ -77,6 +77,8 @@ def parse():
                          "depends on the photons gathered and a finite plain reference is not bias-converged")
     ap.add_argument("--batch", action="store_true",
                     help="batched mode: many uneven seeded scenes, one graph launch per pass over all of them")
+    ap.add_argument("--batch-mode", choices=["multi", "branches"], default="multi",
+                    help="batch mode: one launch per phase for all scenes (multi) or a graph branch per scene")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--caustic", action="store_true",
@@ -483,7 +485,7 @@ def batch_mode(args):
     for r in rs:
         sppm_amd._check(r.lib.sppm_begin(r.h))
         r.passes = r.total_photons = 0
-    batch = sppm_amd.Batch(rs)
+    batch = sppm_amd.Batch(rs, mode={"multi": sppm_amd.BATCH_MULTI, "branches": sppm_amd.BATCH_BRANCHES}[args.batch_mode])
     for _ in range(W):
         batch.pass_()
     batch.sync()
@@ -498,6 +500,7 @@ def batch_mode(args):
         widths[bw] = widths.get(bw, 0) + 1
     line = {
         "metric": "photons/s (batched plain SPPM, aggregate over scenes, one graph launch per pass)",
+        "batch_mode": args.batch_mode,
         "value": round(photons / el, 1), "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(el / K * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded procedural scenes)",

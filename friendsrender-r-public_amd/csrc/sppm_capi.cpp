@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <mutex>
 #include <stdexcept>
@@ -234,8 +235,14 @@ struct Timer {
 // each), forked from and joined into the batch's origin stream.  `done` is
 // recorded after every launch; member contexts wait for it before reading or
 // freeing what the graph uses.
+// One multi-scene launch: jobs [first, first + count) of its table, block
+// offsets at bstart + boff, `grid` blocks; key = BVH width * 2 + flag.
+struct BatchLaunch {
+    int first, count, key, boff, grid;
+};
 struct sppm_batch {
     int device = 0;
+    int mode = SPPM_BATCH_MULTI;
     std::vector<sppm_ctx*> ctx;
     std::vector<hipStream_t> s1, s2;
     std::vector<hipEvent_t> join;
@@ -245,6 +252,17 @@ struct sppm_batch {
     hipGraphExec_t exec = nullptr;
     bool launched = false;
     bool broken = false;  // a member was destroyed: the graph is gone
+    // multi-scene mode: the launches, their job tables and block offsets,
+    // the camera segments' offsets and the path kernels' spill area
+    std::vector<BatchLaunch> pl, el, gl;  // photon, eye and gather launches
+    int cap = 0, nseg = 0;
+    PhotonJob* pjobs = nullptr;
+    EyeJob* ejobs = nullptr;
+    GatherJob* gjobs = nullptr;
+    CamJob* cjobs = nullptr;
+    int* bstart = nullptr;
+    int* kstart = nullptr;
+    uint32_t* spill = nullptr;
 };
 
 struct sppm_ctx {
@@ -1537,6 +1555,253 @@ int sppm_batch_add(sppm_batch* b, sppm_ctx* c) {
     return SPPM_OK;
 }
 
+int sppm_batch_mode(sppm_batch* b, int mode) {
+    if (!b) return fail(SPPM_E_ARG, "NULL batch");
+    if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
+    if (mode != SPPM_BATCH_MULTI && mode != SPPM_BATCH_BRANCHES)
+        return fail(SPPM_E_ARG, "mode must be SPPM_BATCH_MULTI (0) or SPPM_BATCH_BRANCHES (1)");
+    b->mode = mode;
+    return SPPM_OK;
+}
+
+// Block offsets of one multi-scene launch: grid blocks dealt to the scenes in
+// proportion to their work (at least the scenes' order is kept; a scene may
+// get none and is then reached by waves that finished their own).
+static void deal_blocks(const std::vector<double>& work, int grid, std::vector<int>& out) {
+    double tot = 0.0;
+    for (double w : work) tot += w;
+    out.push_back(0);
+    double acc = 0.0;
+    for (size_t k = 0; k < work.size(); ++k) {
+        acc += work[k];
+        int v = tot > 0.0 ? (int)std::llround((double)grid * acc / tot) : (int)((k + 1) * grid / work.size());
+        v = std::min(std::max(v, out.back()), grid);
+        out.push_back(k + 1 == work.size() ? grid : v);
+    }
+}
+
+// The batch pass as one launch per phase for all scenes, pipelined over two
+// streams like the eager passes: stream X runs the photon passes (graph
+// "photons": per-scene counter clears, then one photon launch per group of
+// scenes sharing the kernel's template arguments -- BVH width, guided
+// emission; scenes on the wavefront kernels trace in their own branch) and
+// the grid builds (graph "build", per scene); stream Y runs the camera words
+// and the eye passes (graph "eye": two camera launches and one eye launch
+// per width / NEE group) and the gathers (graph "gather": per-scene
+// planning, one chunk launch, per-scene sums, training, pass counter).  A
+// pass's build waits for the previous pass's gather (it rewrites the grid
+// that gather reads) and its gather for its build, so the photon pass of
+// pass p + 1 runs beside the gather of pass p.  Stream X counts its passes
+// in its own device counters (the steps of the photon launches).
+static int capture_on(hipStream_t st, hipGraph_t* g, hipGraphExec_t* x, const std::function<int()>& body) {
+    HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = body();
+    hipGraph_t gg = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &gg);
+    if (rc) {
+        if (gg) (void)hipGraphDestroy(gg);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (hipGraphInstantiate(x, gg, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(gg);
+        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
+    }
+    *g = gg;
+    return SPPM_OK;
+}
+static int hip_step(hipError_t e, const char* what) {
+    return e == hipSuccess ? SPPM_OK : fail(SPPM_E_HIP, "batch: %s: %s", what, hipGetErrorString(e));
+}
+// Per-scene work as concurrent graph branches from stream st: body(k, branch
+// stream).  The scenes' setup kernels are small and latency bound; run side
+// by side they share the device.
+static int batch_branches(sppm_batch* b, hipStream_t st, const std::function<void(int, hipStream_t)>& body) {
+    const int n = (int)b->ctx.size();
+    int r = hip_step(hipEventRecord(b->fork, st), "fork");
+    for (int k = 0; k < n && !r; ++k) {
+        if ((r = hip_step(hipStreamWaitEvent(b->s1[k], b->fork, 0), "fork wait"))) break;
+        body(k, b->s1[k]);
+        r = hip_step(hipEventRecord(b->join[k], b->s1[k]), "branch end");
+    }
+    for (int k = 0; k < n && !r; ++k) r = hip_step(hipStreamWaitEvent(st, b->join[k], 0), "join");
+    return r;
+}
+
+// The batch pass as one launch per phase for all scenes, in one graph:
+//   per scene: clear the work counters (scenes on the wavefront photon
+//     kernels trace their photon pass here);
+//   one photon launch per group of scenes sharing the kernel's template
+//     arguments (BVH width, guided emission);
+//   per scene: the grid build;
+//   the camera words of all scenes (two launches), one eye launch per BVH
+//     width / NEE group;
+//   per scene: the gather's planning;
+//   one gather chunk launch;
+//   per scene: the gather's sums, direct light, training, pass counter.
+// The phases run one after the other: each of the big launches fills the
+// device by itself (run side by side, measured, they only slowed each other
+// and starved the small setup kernels).
+static int prepare_multi(sppm_batch* b) {
+    const int n = (int)b->ctx.size();
+    std::vector<PhotonJob> pj;
+    std::vector<EyeJob> ej;
+    std::vector<GatherJob> gj;
+    std::vector<CamJob> cj;
+    std::vector<int> pkey, ekey, kst{0};
+    int cap = 1;
+    int rc;
+    for (int k = 0; k < n; ++k) {
+        sppm_ctx* c = b->ctx[k];
+        RenderBuffers& R = c->rb;
+        cap = std::max(cap, c->scene.stack_cap);
+        if (R.wave_bounces <= 0) {
+            PhotonJob J{};
+            J.S = c->scene.dev;
+            J.slots = R.slots;
+            J.N = R.N;
+            J.step_dev = R.pass_dev;
+            J.next_path = c->scene.next_path;
+            if (c->guided) J.G = c->emit;
+            J.flags = R.flags;
+            J.rays = photon_rays(R);
+            pj.push_back(J);
+            pkey.push_back(c->scene.dev.width * 2 + (c->guided ? 1 : 0));
+        }
+        EyeJob E{};
+        E.S = c->scene.dev;
+        E.C = c->camdev;
+        E.H = R.hits;
+        E.keep = c->params.eye_nee == SPPM_NEE_KEEP ? 1 : 0;
+        E.step_dev = R.pass_dev;
+        E.words = R.words;
+        E.queue = c->scene.next_path + STRIPE_INTS;
+        E.flags = R.flags;
+        E.rays = eye_rays(R);
+        ej.push_back(E);
+        ekey.push_back(c->scene.dev.width * 2 + (c->params.eye_nee != SPPM_NEE_OFF ? 1 : 0));
+        GatherJob J{};
+        J.H = R.hits;
+        J.SH = R.shared;
+        J.G = grid_dev(c);
+        J.gp = R.gp;
+        J.order = gather_order(R.groups, R.np);
+        J.nvalid = R.groups.nvalid;
+        J.tiles = R.tiles;
+        J.ntiles = gather_group_count(R.np);
+        J.chunk_off = R.chunk_off;
+        J.chunkp = R.chunkp;
+        J.queue = R.groups.chunkq;
+        J.partial = R.partial;
+        J.pdep = c->guided ? R.pdep : nullptr;
+        J.rec = PairRecord{R.pairs, R.npairs, R.pairs_cap};
+        gj.push_back(J);
+        cj.push_back(CamJob{R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words});
+        kst.push_back(kst.back() + R.cam_K);
+    }
+    // jobs sorted by group (stable: scenes keep their order inside a group),
+    // grid blocks dealt to a group's scenes in proportion to their work
+    std::vector<int> bst;
+    auto plan = [&](const std::vector<int>& key, const std::vector<double>& work, int fixed_grid, std::vector<int>& o,
+                    std::vector<BatchLaunch>& L) {
+        o.resize(key.size());
+        L.clear();
+        for (size_t q = 0; q < o.size(); ++q) o[q] = (int)q;
+        std::stable_sort(o.begin(), o.end(), [&](int a, int c) { return key[a] < key[c]; });
+        for (size_t i = 0; i < o.size();) {
+            size_t j = i;
+            while (j < o.size() && key[o[j]] == key[o[i]]) ++j;
+            const int grid = fixed_grid > 0 ? fixed_grid : trace_grid_blocks(key[o[i]] / 2);
+            std::vector<double> w;
+            for (size_t q = i; q < j; ++q) w.push_back(work[o[q]]);
+            L.push_back(BatchLaunch{(int)i, (int)(j - i), key[o[i]], (int)bst.size(), grid});
+            deal_blocks(w, grid, bst);
+            i = j;
+        }
+    };
+    std::vector<double> pw, ew;
+    for (const PhotonJob& J : pj) pw.push_back((double)J.N);
+    for (int k = 0; k < n; ++k) ew.push_back((double)b->ctx[k]->rb.np);
+    std::vector<int> po, eo, go;
+    plan(pkey, pw, 0, po, b->pl);
+    plan(ekey, ew, 0, eo, b->el);
+    plan(std::vector<int>(n, 0), ew, gather_wave_count(), go, b->gl);
+    int grid_max = 1;
+    for (const BatchLaunch& l : b->pl) grid_max = std::max(grid_max, l.grid);
+    for (const BatchLaunch& l : b->el) grid_max = std::max(grid_max, l.grid);
+    b->cap = cap;
+    b->nseg = kst.back();
+    dfree(b->pjobs); dfree(b->ejobs); dfree(b->gjobs); dfree(b->cjobs); dfree(b->bstart); dfree(b->kstart);
+    dfree(b->spill);
+    // the photon and eye launches run one after the other: one spill area
+    const size_t per_lane = cap > TRACE_LDS ? (size_t)(cap - TRACE_LDS) : 1;
+    if ((rc = dalloc(&b->pjobs, std::max<size_t>(pj.size(), 1))) || (rc = dalloc(&b->ejobs, ej.size())) ||
+        (rc = dalloc(&b->gjobs, gj.size())) || (rc = dalloc(&b->cjobs, cj.size())) ||
+        (rc = dalloc(&b->bstart, bst.size())) || (rc = dalloc(&b->kstart, kst.size())) ||
+        (rc = dalloc(&b->spill, per_lane * (size_t)grid_max * 64)))
+        return rc;
+    std::vector<PhotonJob> pjs;
+    for (int q : po) pjs.push_back(pj[q]);
+    std::vector<EyeJob> ejs;
+    for (int q : eo) ejs.push_back(ej[q]);
+    std::vector<GatherJob> gjs;
+    for (int q : go) gjs.push_back(gj[q]);
+    if ((!pjs.empty() && hipMemcpy(b->pjobs, pjs.data(), sizeof(PhotonJob) * pjs.size(), hipMemcpyHostToDevice)) ||
+        hipMemcpy(b->ejobs, ejs.data(), sizeof(EyeJob) * ejs.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->gjobs, gjs.data(), sizeof(GatherJob) * gjs.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->cjobs, cj.data(), sizeof(CamJob) * cj.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->kstart, kst.data(), sizeof(int) * kst.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->bstart, bst.data(), sizeof(int) * bst.size(), hipMemcpyHostToDevice))
+        return fail(SPPM_E_HIP, "batch job tables upload");
+    return SPPM_OK;
+}
+static int enqueue_multi(sppm_batch* b, hipStream_t st) {
+    int r = batch_branches(b, st, [&](int k, hipStream_t bs) {
+        sppm_ctx* c = b->ctx[k];
+        if (c->rb.wave_bounces > 0) photon_pass(c, 0, c->rb.pass_dev, bs, false);
+        else launch_zero_i32(c->scene.next_path, STRIPE_INTS, bs);
+        launch_zero_i32(c->scene.next_path + STRIPE_INTS, STRIPE_INTS, bs);
+    });
+    if (r) return r;
+    for (const BatchLaunch& l : b->pl) {
+        TraceSpill sp{b->spill, l.grid, b->cap};
+        launch_photon_multi(l.key / 2, (l.key & 1) != 0, b->pjobs + l.first, l.count, b->bstart + l.boff, sp, st);
+    }
+    if ((r = batch_branches(b, st, [&](int k, hipStream_t bs) { do_build(b->ctx[k], bs); }))) return r;
+    launch_camera_words_multi(b->cjobs, b->kstart, (int)b->ctx.size(), b->nseg, st);
+    for (const BatchLaunch& l : b->el) {
+        TraceSpill sp{b->spill, l.grid, b->cap};
+        launch_eye_multi(l.key / 2, (l.key & 1) != 0, b->ejobs + l.first, l.count, b->bstart + l.boff, sp, st);
+    }
+    r = batch_branches(b, st, [&](int k, hipStream_t bs) {
+        sppm_ctx* c = b->ctx[k];
+        RenderBuffers& R = c->rb;
+        launch_set_u64x2(R.rstat, 0ull, 0ull, bs);
+        if (c->guided) launch_deposit_clear(R.pdep, R.nstored, bs);
+        const PairRecord rec{R.pairs, R.npairs, R.pairs_cap};
+        (void)launch_gather_plan(R.hits, R.shared, grid_dev(c), R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups,
+                                 R.tiles, GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.counts,
+                                 R.ctr, rec, bs);
+    });
+    if (r) return r;
+    for (const BatchLaunch& l : b->gl) launch_gather_chunks_multi(b->gjobs + l.first, l.count, b->bstart + l.boff, st);
+    return batch_branches(b, st, [&](int k, hipStream_t bs) {
+        sppm_ctx* c = b->ctx[k];
+        RenderBuffers& R = c->rb;
+        launch_gather_finish(R.hits, R.shared, R.np, gather_order(R.groups, R.np), R.groups, c->params.alpha,
+                             R.chunk_off, R.partial, R.counts, R.rstat, R.ctr, R.flags, bs);
+        if (c->scene.dev.ibl_tex >= 0 || c->params.eye_nee != SPPM_NEE_OFF)
+            launch_direct_light(R.hits, R.shared, R.np, bs);
+        if (c->guided) do_guide(c, bs, 0, R.pass_dev);
+        launch_add_i32(R.pass_dev, 1, bs);
+    });
+}
+static int capture_multi(sppm_batch* b) {
+    int rc = prepare_multi(b);
+    if (rc) return rc;
+    return capture_on(b->origin, &b->graph, &b->exec, [&]() { return enqueue_multi(b, b->origin); });
+}
+
 int sppm_batch_capture(sppm_batch* b) {
     if (!b || b->ctx.empty()) return fail(SPPM_E_ARG, "empty batch");
     if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
@@ -1547,6 +1812,7 @@ int sppm_batch_capture(sppm_batch* b) {
         if (rc) return rc;
         HIPOK(hipStreamSynchronize(c->stream));
     }
+    if (b->mode == SPPM_BATCH_MULTI) return capture_multi(b);
     HIPOK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
     int rc = SPPM_OK;
     if (hipEventRecord(b->fork, b->origin) != hipSuccess) rc = fail(SPPM_E_HIP, "fork record");
@@ -1602,6 +1868,8 @@ int sppm_batch_destroy(sppm_batch* b) {
         (void)hipStreamDestroy(b->s2[k]);
         (void)hipEventDestroy(b->join[k]);
     }
+    dfree(b->pjobs); dfree(b->ejobs); dfree(b->gjobs); dfree(b->cjobs); dfree(b->bstart); dfree(b->kstart);
+    dfree(b->spill);
     (void)hipEventDestroy(b->fork);
     (void)hipEventDestroy(b->done);
     (void)hipStreamDestroy(b->origin);

@@ -15,6 +15,8 @@ struct SortResult {
     uint32_t* keys;
     uint32_t* vals;
 };
+// The buffers radix_sort_pairs leaves its output in for these arguments (no launch).
+SortResult radix_sort_result(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, int n, int bits);
 SortResult radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, uint32_t* work, int n,
                             int bits, hipStream_t st, const int* n_dev = nullptr);
 constexpr int RADIX_SORT_MAX_N = (1 << 30) - 1;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sppm_guide.h"
 #include "sppm_layout.h"
 
 namespace sppm {
@@ -42,6 +43,16 @@ struct Counters {
 // states holds K windows of 624 raw words (then K more as scratch), phi the
 // jump polynomial of (K-1)L.
 void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long L, uint32_t* words, hipStream_t st);
+// The camera words of several streams in two launches: job s has K segments,
+// blocks kstart[s] .. kstart[s + 1] - 1 (nseg = kstart[njobs] in all).
+struct CamJob {
+    uint32_t* states;
+    const uint32_t* phi;
+    int K;
+    long long L;
+    uint32_t* words;
+};
+void launch_camera_words_multi(const CamJob* jobs, const int* kstart, int njobs, int nseg, hipStream_t st);
 // Traversal stack spill area of one path kernel: blocks * 64 lanes,
 // cap - TRACE_LDS entries each.  Kernels that may run at the same time (the
 // photon and eye passes) take separate areas.
@@ -117,10 +128,78 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
                             int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
                             const PairRecord& rec, hipStream_t st);
+// The same in three parts: planning (hit-point groups, tiles, chunk lists;
+// returns the group order), the chunk kernel, and the per-group sums.
+const uint32_t* launch_gather_plan(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp,
+                                   int np, const float* bmin, const float* ext, const GroupBuffers& gb,
+                                   TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp,
+                                   int* counts, Counters* ctr, const PairRecord& rec, hipStream_t st);
+void launch_gather_run(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
+                       const uint32_t* order, const GroupBuffers& gb, const TileRange* tiles, const int* chunkp,
+                       const int* chunk_off, float4* partial, float* pdep, const PairRecord& rec, hipStream_t st);
+void launch_gather_finish(const HitDev& H, const SharedDev& SH, int np, const uint32_t* order, const GroupBuffers& gb,
+                          float alpha, const int* chunk_off, const float4* partial, int* counts,
+                          unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st);
 int gather_group_count(int np);
 int gather_tile_count(int W, int Hh);
 size_t gather_tile_bytes();
 void launch_set_u64x2(unsigned long long* p, unsigned long long a, unsigned long long b, hipStream_t st);
+
+// ---- multi-scene launches ----------------------------------------------------
+// One launch runs the photon pass, the eye pass or the gather's chunk kernel
+// of every scene of a batch: the per-scene arguments of the one-scene
+// launches, one entry per scene in device memory.  Waves start on the scene
+// that owns their block (bstart: njobs + 1 block offsets, in proportion to
+// the scenes' work) and then help the others.  The scenes' work counters
+// (next_path, queue) are zeroed before the launch, as for the one-scene
+// kernels.  step_dev: the scene's pass counter (the step is *step_dev + 1),
+// or NULL for `step`.
+struct PhotonJob {
+    SceneDev S;
+    SlotsDev slots;
+    int N, step;
+    const int* step_dev;
+    int* next_path;
+    GuideEmit G;
+    uint32_t* flags;
+    unsigned long long* rays;
+};
+struct EyeJob {
+    SceneDev S;
+    CamDev C;
+    HitDev H;
+    int step, keep;
+    const int* step_dev;
+    const uint32_t* words;
+    int* queue;
+    uint32_t* flags;
+    unsigned long long* rays;
+};
+struct GatherJob {
+    HitDev H;
+    SharedDev SH;
+    GridDev G;
+    const GridParams* gp;
+    const uint32_t* order;
+    const int* nvalid;
+    const TileRange* tiles;
+    int ntiles;
+    const int* chunk_off;
+    const int* chunkp;
+    int* queue;
+    float4* partial;
+    float* pdep;
+    PairRecord rec;
+};
+// width: the scenes' common BVH width; sp: a spill area for sp.blocks blocks
+void launch_photon_multi(int width, bool guided, const PhotonJob* jobs, int njobs, const int* bstart,
+                         const TraceSpill& sp, hipStream_t st);
+void launch_eye_multi(int width, bool nee, const EyeJob* jobs, int njobs, const int* bstart, const TraceSpill& sp,
+                      hipStream_t st);
+void launch_gather_chunks_multi(const GatherJob* jobs, int njobs, const int* bstart, hipStream_t st);
+int gather_wave_count();
+// the group order launch_gather_plan returns for these buffers (no launch)
+const uint32_t* gather_order(const GroupBuffers& gb, int np);
 // Adds the hit points' DirectLight into the shared hit points (environment light only).
 void launch_direct_light(const HitDev& H, const SharedDev& SH, int np, hipStream_t st);
 void launch_reconstruct(const SharedDev& SH, int np, long long total, int passes, float4* out, hipStream_t st);

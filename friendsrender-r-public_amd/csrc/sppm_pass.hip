@@ -63,14 +63,15 @@ constexpr int CAMW_JUMP_THREADS = 640;  // 624 output words of the jumped window
 // One wave per segment: x[624 + q] -> out[q] for q < L, 227 words per step
 // in four rounds of 64 lanes.  A step reads only words of earlier steps, and
 // the wave's LDS accesses are executed in order, so no barrier is needed.
-__global__ void __launch_bounds__(64) k_camw_emit(const uint32_t* __restrict__ states, long long L,
-                                                  uint32_t* __restrict__ words, uint32_t* __restrict__ ends) {
-    __shared__ uint32_t ring[CAMW_RING];
+// segment seg of a stream: states, words and ends as for launch_camera_words
+__device__ __forceinline__ void camw_emit(const uint32_t* __restrict__ states, long long L,
+                                          uint32_t* __restrict__ words, uint32_t* __restrict__ ends, int seg,
+                                          uint32_t* ring) {
     const int lane = threadIdx.x;
-    const uint32_t* st = states + (size_t)blockIdx.x * 624;
+    const uint32_t* st = states + (size_t)seg * 624;
     for (int k = lane; k < 624; k += 64) ring[k] = st[k];
     __builtin_amdgcn_wave_barrier();
-    uint32_t* out = words + (size_t)blockIdx.x * (size_t)L;
+    uint32_t* out = words + (size_t)seg * (size_t)L;
     const long long mend = 624 + L;
     for (long long m0 = 624; m0 < mend; m0 += 227) {
 #pragma unroll
@@ -87,8 +88,13 @@ __global__ void __launch_bounds__(64) k_camw_emit(const uint32_t* __restrict__ s
         __builtin_amdgcn_wave_barrier();
     }
     // the window after the segment: x[L .. L + 623]
-    uint32_t* e = ends + (size_t)blockIdx.x * 624;
+    uint32_t* e = ends + (size_t)seg * 624;
     for (int j = lane; j < 624; j += 64) e[j] = ring[(L + j) & (CAMW_RING - 1)];
+}
+__global__ void __launch_bounds__(64) k_camw_emit(const uint32_t* __restrict__ states, long long L,
+                                                  uint32_t* __restrict__ words, uint32_t* __restrict__ ends) {
+    __shared__ uint32_t ring[CAMW_RING];
+    camw_emit(states, L, words, ends, (int)blockIdx.x, ring);
 }
 
 // Jumped window: word t = XOR of y[i + t] over the coefficients phi_i = 1,
@@ -96,12 +102,10 @@ __global__ void __launch_bounds__(64) k_camw_emit(const uint32_t* __restrict__ s
 // [i0, i0 + 227) the words up to y[i0 + 226 + 623] are generated into the
 // ring (227 at a time, one barrier), whose oldest slot then holds y[i0 - 175]
 // or later, so the block's reads y[i0 ..] are intact.
-__global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump(const uint32_t* __restrict__ ends,
-                                                                 const uint32_t* __restrict__ phi,
-                                                                 uint32_t* __restrict__ states) {
-    __shared__ uint32_t ring[CAMW_RING];
+__device__ __forceinline__ void camw_jump(const uint32_t* __restrict__ ends, const uint32_t* __restrict__ phi,
+                                          uint32_t* __restrict__ states, int seg, uint32_t* ring) {
     const int t = threadIdx.x;
-    const uint32_t* e = ends + (size_t)blockIdx.x * 624;
+    const uint32_t* e = ends + (size_t)seg * 624;
     if (t < 624) ring[t] = e[t];  // y[k] at ring[k & 1023]
     __syncthreads();
     uint32_t acc = 0;
@@ -131,7 +135,38 @@ __global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump(const uint32_t*
         }
         __syncthreads();
     }
-    if (t < 624) states[(size_t)blockIdx.x * 624 + t] = acc;
+    if (t < 624) states[(size_t)seg * 624 + t] = acc;
+}
+__global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump(const uint32_t* __restrict__ ends,
+                                                                 const uint32_t* __restrict__ phi,
+                                                                 uint32_t* __restrict__ states) {
+    __shared__ uint32_t ring[CAMW_RING];
+    camw_jump(ends, phi, states, (int)blockIdx.x, ring);
+}
+
+// The camera words of every scene of a batch: block b is segment b -
+// kstart[s] of scene s, kstart[s] <= b < kstart[s + 1].
+__device__ __forceinline__ int owner_of(const int* __restrict__ start, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__global__ void __launch_bounds__(64) k_camw_emit_multi(const CamJob* __restrict__ jobs,
+                                                        const int* __restrict__ kstart, int njobs) {
+    __shared__ uint32_t ring[CAMW_RING];
+    const int s = owner_of(kstart, njobs, (int)blockIdx.x);
+    const CamJob& J = jobs[s];
+    camw_emit(J.states, J.L, J.words, J.states + (size_t)J.K * 624, (int)blockIdx.x - kstart[s], ring);
+}
+__global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump_multi(const CamJob* __restrict__ jobs,
+                                                                       const int* __restrict__ kstart, int njobs) {
+    __shared__ uint32_t ring[CAMW_RING];
+    const int s = owner_of(kstart, njobs, (int)blockIdx.x);
+    const CamJob& J = jobs[s];
+    camw_jump(J.states + (size_t)J.K * 624, J.phi, J.states, (int)blockIdx.x - kstart[s], ring);
 }
 
 // Persistent photon pass: each lane traces one path at a time; when at least
@@ -140,19 +175,14 @@ __global__ void __launch_bounds__(CAMW_JUMP_THREADS) k_camw_jump(const uint32_t*
 // so lanes do not idle behind the longest path of their wave.  A path's result depends
 // only on its index, not on the lane or order that traces it.  Every wave
 // exits once the counter has passed N and its lanes are done.
+// The photon paths [0, N) of one scene, traced by the calling wave until the
+// scene's counter is exhausted and the wave's lanes are done.
 template <int W, bool GUIDED>
-__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
-                                                       const int* __restrict__ step_dev, int cap,
-                                                       uint32_t* __restrict__ spill, int* __restrict__ next_path,
-                                                       GuideEmit G, uint32_t* __restrict__ flags,
-                                                       unsigned long long* __restrict__ rays) {
-    // the pass step comes from the device pass counter when the pass is replayed from a graph
-    const int step = step_dev ? *step_dev + 1 : step_arg;
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+__device__ __forceinline__ void photon_paths(const SceneDev& S, const SlotsDev& slots, int N, int step,
+                                             int* __restrict__ next_path, const GuideEmit& G, const TraceStack& ts,
+                                             uint32_t& ovf, uint32_t& fl, uint32_t& nray, uint32_t& ntie) {
     const int lane = threadIdx.x;
-    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
     PhotonPath P;
     bool alive = false, exhausted = false;
     StripeCursor cur = stripe_start();
@@ -182,10 +212,71 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
             alive = false;
         }
     }
+}
+
+template <int W, bool GUIDED>
+__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(SceneDev S, SlotsDev slots, int N, int step_arg,
+                                                       const int* __restrict__ step_dev, int cap,
+                                                       uint32_t* __restrict__ spill, int* __restrict__ next_path,
+                                                       GuideEmit G, uint32_t* __restrict__ flags,
+                                                       unsigned long long* __restrict__ rays) {
+    // the pass step comes from the device pass counter when the pass is replayed from a graph
+    const int step = step_dev ? *step_dev + 1 : step_arg;
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
+    photon_paths<W, GUIDED>(S, slots, N, step, next_path, G, ts, ovf, fl, nray, ntie);
     if (ovf) fl |= 2u;
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
     add_wave_count(rays + 2, ntie);
+}
+
+// ---------------------------------------------------------------------------
+// Multi-scene launches: one launch runs a phase of the pass for every scene
+// of a batch.  Wave w starts on the scene that owns launch block w (blocks
+// are dealt to the scenes in proportion to their work, bstart[s] <=
+// blockIdx.x < bstart[s + 1]), works there until the scene's counters are
+// exhausted, then visits the other scenes in order.  A wave that finds every
+// stripe of a scene exhausted raises the scene's done flag (an unused word
+// of its counter area, zeroed with it), so later visits cost one load.
+// Every item's result depends only on the item, as in the one-scene
+// kernels, so a batch renders each scene exactly as its own passes would.
+// ---------------------------------------------------------------------------
+constexpr int STRIPE_DONE = 1;  // word of a striped counter area that flags it exhausted
+__device__ __forceinline__ int home_scene(const int* __restrict__ bstart, int n) {
+    return owner_of(bstart, n, (int)blockIdx.x);
+}
+__device__ __forceinline__ bool scene_done(const int* ctr) {
+    return __hip_atomic_load(ctr + STRIPE_DONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+__device__ __forceinline__ void scene_mark_done(int* ctr) {
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(ctr + STRIPE_DONE, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int W, bool GUIDED>
+__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon_multi(const PhotonJob* __restrict__ jobs,
+                                                                                int njobs,
+                                                                                const int* __restrict__ bstart, int cap,
+                                                                                uint32_t* __restrict__ spill) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    const int home = home_scene(bstart, njobs);
+    for (int k = 0; k < njobs; ++k) {
+        const int s = __builtin_amdgcn_readfirstlane(home + k < njobs ? home + k : home + k - njobs);
+        const PhotonJob& J = jobs[s];
+        if (scene_done(J.next_path)) continue;
+        const int step = J.step_dev ? *J.step_dev + 1 : J.step;
+        uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
+        photon_paths<W, GUIDED>(J.S, J.slots, J.N, step, J.next_path, J.G, ts, ovf, fl, nray, ntie);
+        scene_mark_done(J.next_path);
+        if (ovf) fl |= 2u;
+        if (fl) atomicOr(J.flags, fl);
+        add_wave_count(J.rays, nray);
+        add_wave_count(J.rays + 2, ntie);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -620,6 +711,22 @@ __device__ __forceinline__ void eye_path(const SceneDev& S, const CamDev& C, con
 // counter (zeroed before the launch), so waves whose pixels have long paths
 // (glass, mirrors) do not hold back the others; a pixel's result does not
 // depend on the wave that traces it.
+// The pixels of one scene, 64 at a time, until its counter is exhausted.
+template <int W, bool NEE>
+__device__ __forceinline__ void eye_pixels(const SceneDev& S, const CamDev& C, const HitDev& H, int step,
+                                           const uint32_t* __restrict__ words, int* __restrict__ queue, bool keep,
+                                           const TraceStack& ts, uint32_t& fl, uint32_t& nray, uint32_t& ntie) {
+    const int lane = threadIdx.x;
+    const int NPIX = C.W * C.H;
+    StripeCursor cur = stripe_start();
+    for (;;) {
+        int first = 0;
+        const int got = stripe_claim(queue, NPIX, cur, 64, &first);
+        if (got == 0) break;
+        if ((lane & 63) < got) eye_path<W, NEE>(S, C, H, step, first + (lane & 63), words, ts, fl, keep, nray, ntie);
+    }
+}
+
 template <int W, bool NEE>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDev S, CamDev C, HitDev H, int step_arg,
                                                     const int* __restrict__ step_dev, int cap,
@@ -630,18 +737,34 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDe
     __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
-    const int NPIX = C.W * C.H;
     uint32_t fl = 0, nray = 0, ntie = 0;
-    StripeCursor cur = stripe_start();
-    for (;;) {
-        int first = 0;
-        const int got = stripe_claim(queue, NPIX, cur, 64, &first);
-        if (got == 0) break;
-        if ((lane & 63) < got) eye_path<W, NEE>(S, C, H, step, first + (lane & 63), words, ts, fl, keep != 0, nray, ntie);
-    }
+    eye_pixels<W, NEE>(S, C, H, step, words, queue, keep != 0, ts, fl, nray, ntie);
     if (fl) atomicOr(flags, fl);
     add_wave_count(rays, nray);
     add_wave_count(rays + 2, ntie);
+}
+
+// The eye pass of every scene of a batch (see k_photon_multi).
+template <int W, bool NEE>
+__global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye_multi(const EyeJob* __restrict__ jobs, int njobs,
+                                                                             const int* __restrict__ bstart, int cap,
+                                                                             uint32_t* __restrict__ spill) {
+    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    const int lane = threadIdx.x;
+    const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
+    const int home = home_scene(bstart, njobs);
+    for (int k = 0; k < njobs; ++k) {
+        const int s = __builtin_amdgcn_readfirstlane(home + k < njobs ? home + k : home + k - njobs);
+        const EyeJob& J = jobs[s];
+        if (scene_done(J.queue)) continue;
+        const int step = J.step_dev ? *J.step_dev + 1 : J.step;
+        uint32_t fl = 0, nray = 0, ntie = 0;
+        eye_pixels<W, NEE>(J.S, J.C, J.H, step, J.words, J.queue, J.keep != 0, ts, fl, nray, ntie);
+        scene_mark_done(J.queue);
+        if (fl) atomicOr(J.flags, fl);
+        add_wave_count(J.rays, nray);
+        add_wave_count(J.rays + 2, ntie);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1023,16 +1146,26 @@ __device__ __forceinline__ bool in_query(const GatherLane& L, const float4 pp) {
 // keeps the lanes of a wave busy in the second loop instead of evaluating
 // one lane's BRDF per candidate; each lane still adds its neighbours in
 // ascending candidate order.
-__global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, GridDev G,
-                                                      const GridParams* __restrict__ gp,
-                                                      const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
-                                                      const TileRange* __restrict__ tiles, int ntiles,
-                                                      const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
-                                                      int* __restrict__ queue, float4* __restrict__ partial,
-                                                      float* __restrict__ pdep, PairRecord rec) {
-    __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
-    __shared__ unsigned char s_hit[64 * 64];
-    __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
+// LDS of the gather chunk loop (one wave per block)
+struct GatherLds {
+    float4 pos[64], flux[64], dir[64];
+    unsigned char hit[64 * 64];
+    int cnt[64];  // hit points of this group gathering each staged photon (guided emission)
+};
+
+// The chunks of one scene's gather, claimed until its counters are exhausted.
+__device__ __forceinline__ void gather_chunks(const HitDev& H, const SharedDev& SH, const GridDev& G,
+                                              const GridParams* __restrict__ gp, const uint32_t* __restrict__ order,
+                                              const int* __restrict__ nvalidp, const TileRange* __restrict__ tiles,
+                                              int ntiles, const int* __restrict__ chunk_off,
+                                              const int* __restrict__ chunkp, int* __restrict__ queue,
+                                              float4* __restrict__ partial, float* __restrict__ pdep,
+                                              const PairRecord& rec, GatherLds& lds) {
+    float4* const s_pos = lds.pos;
+    float4* const s_flux = lds.flux;
+    float4* const s_dir = lds.dir;
+    unsigned char* const s_hit = lds.hit;
+    int* const s_cnt = lds.cnt;
     const int lane = threadIdx.x;
     const int hl = lane % GATHER_GROUP;   // the lane's hit point in the group
     const int ph = lane / GATHER_GROUP;   // the lane tests staged photons ph, ph + GATHER_PHASES, ...
@@ -1187,6 +1320,32 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
         }
         if (ph == 0)
             partial[(size_t)ch * GATHER_GROUP + hl] = make_float4(tot.x, tot.y, tot.z, __int_as_float(Mt));
+    }
+}
+
+__global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, GridDev G,
+                                                      const GridParams* __restrict__ gp,
+                                                      const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
+                                                      const TileRange* __restrict__ tiles, int ntiles,
+                                                      const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
+                                                      int* __restrict__ queue, float4* __restrict__ partial,
+                                                      float* __restrict__ pdep, PairRecord rec) {
+    __shared__ GatherLds lds;
+    gather_chunks(H, SH, G, gp, order, nvalidp, tiles, ntiles, chunk_off, chunkp, queue, partial, pdep, rec, lds);
+}
+
+// The gather chunks of every scene of a batch (see k_photon_multi).
+__global__ void __launch_bounds__(64) k_gather_chunks_multi(const GatherJob* __restrict__ jobs, int njobs,
+                                                            const int* __restrict__ bstart) {
+    __shared__ GatherLds lds;
+    const int home = home_scene(bstart, njobs);
+    for (int k = 0; k < njobs; ++k) {
+        const int s = __builtin_amdgcn_readfirstlane(home + k < njobs ? home + k : home + k - njobs);
+        const GatherJob& J = jobs[s];
+        if (scene_done(J.queue)) continue;
+        gather_chunks(J.H, J.SH, J.G, J.gp, J.order, J.nvalid, J.tiles, J.ntiles, J.chunk_off, J.chunkp, J.queue,
+                      J.partial, J.pdep, J.rec, lds);
+        scene_mark_done(J.queue);
     }
 }
 
@@ -1401,6 +1560,10 @@ void launch_camera_words(uint32_t* states, const uint32_t* phi, int K, long long
     hipLaunchKernelGGL(k_camw_emit, dim3(K), dim3(64), 0, st, states, L, words, ends);
     hipLaunchKernelGGL(k_camw_jump, dim3(K), dim3(CAMW_JUMP_THREADS), 0, st, ends, phi, states);
 }
+void launch_camera_words_multi(const CamJob* jobs, const int* kstart, int njobs, int nseg, hipStream_t st) {
+    hipLaunchKernelGGL(k_camw_emit_multi, dim3(nseg), dim3(64), 0, st, jobs, kstart, njobs);
+    hipLaunchKernelGGL(k_camw_jump_multi, dim3(nseg), dim3(CAMW_JUMP_THREADS), 0, st, jobs, kstart, njobs);
+}
 
 void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, const int* step_dev,
                    const TraceSpill& sp, int* next_path, const GuideEmit* guide, uint32_t* flags,
@@ -1510,13 +1673,12 @@ void launch_gather(const HitDev& H, const SharedDev& SH, const GridDev& G, const
                        gp, W, Hh, alpha, mode, counts, list_off, lists, rstat, ctr, flags);
 }
 
-void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
-                            const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
-                            int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
-                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
-                            const PairRecord& rec, hipStream_t st) {
+const uint32_t* launch_gather_plan(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp,
+                                   int np, const float* bmin, const float* ext, const GroupBuffers& gb,
+                                   TileRange* tiles, int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp,
+                                   int* counts, Counters* ctr, const PairRecord& rec, hipStream_t st) {
     const int ntiles = gather_group_count(np);
-    // hit-point groups: valid hit points in Morton order, 64 per group
+    // hit-point groups: valid hit points in Morton order, GATHER_GROUP per group
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, gb.nvalid, 1);  // valid count
     hipLaunchKernelGGL(k_zero_i32, dim3(STRIPE_INTS / 256), dim3(256), 0, st, gb.chunkq, STRIPE_INTS);
     if (counts) hipLaunchKernelGGL(k_zero_i32, dim3((np + 255) / 256), dim3(256), 0, st, counts, np);
@@ -1524,17 +1686,75 @@ void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev&
                        ext[1], ext[2], gb.keys, gb.vals);
     const SortResult sr = radix_sort_pairs(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, gb.hist, np, 31, st);
     hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st, sr.keys, np, gb.nvalid);
-    const uint32_t* order = sr.vals;
+    const uint32_t* order = sr.vals;  // == gather_order(gb, np)
     hipLaunchKernelGGL(k_gather_plan, dim3(ntiles), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles);
     hipLaunchKernelGGL(k_chunk_size, dim3(1), dim3(256), 0, st, tiles, ntiles, cap, chunkp, ctr);
     hipLaunchKernelGGL(k_tile_chunks, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st, tiles, ntiles, chunkp, nch);
     exclusive_scan(nch, chunk_off, ntiles + 1, nullptr, scan_tmp, st);
     if (rec.pairs) launch_set_u64x2(rec.count, 0ull, 0ull, st);
+    return order;
+}
+void launch_gather_run(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
+                       const uint32_t* order, const GroupBuffers& gb, const TileRange* tiles, const int* chunkp,
+                       const int* chunk_off, float4* partial, float* pdep, const PairRecord& rec, hipStream_t st) {
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
-                       ntiles, chunk_off, chunkp, gb.chunkq, partial, pdep, rec);
+                       gather_group_count(np), chunk_off, chunkp, gb.chunkq, partial, pdep, rec);
+}
+void launch_gather_finish(const HitDev& H, const SharedDev& SH, int np, const uint32_t* order, const GroupBuffers& gb,
+                          float alpha, const int* chunk_off, const float4* partial, int* counts,
+                          unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st) {
+    const int ntiles = gather_group_count(np);
     hipLaunchKernelGGL(k_gather_finish, dim3(ntiles), dim3(64), 0, st, H, SH, order, gb.nvalid, alpha, chunk_off,
                        partial, counts, gb.stat, flags);
     hipLaunchKernelGGL(k_group_stats, dim3(1), dim3(1024), 0, st, gb.stat, ntiles, rstat, ctr);
+}
+void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
+                            const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
+                            int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
+                            const PairRecord& rec, hipStream_t st) {
+    const uint32_t* order = launch_gather_plan(H, SH, G, gp, np, bmin, ext, gb, tiles, cap, chunkp, nch, chunk_off,
+                                               scan_tmp, counts, ctr, rec, st);
+    launch_gather_run(H, SH, G, gp, np, order, gb, tiles, chunkp, chunk_off, partial, pdep, rec, st);
+    launch_gather_finish(H, SH, np, order, gb, alpha, chunk_off, partial, counts, rstat, ctr, flags, st);
+}
+
+void launch_photon_multi(int width, bool guided, const PhotonJob* jobs, int njobs, const int* bstart,
+                         const TraceSpill& sp, hipStream_t st) {
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+#define SPPM_PM(WW, GG) hipLaunchKernelGGL((k_photon_multi<WW, GG>), g, b, 0, st, jobs, njobs, bstart, sp.cap, sp.buf)
+    if (guided) {
+        if (width == 2) SPPM_PM(2, true);
+        else if (width == 4) SPPM_PM(4, true);
+        else SPPM_PM(8, true);
+    } else {
+        if (width == 2) SPPM_PM(2, false);
+        else if (width == 4) SPPM_PM(4, false);
+        else SPPM_PM(8, false);
+    }
+#undef SPPM_PM
+}
+void launch_eye_multi(int width, bool nee, const EyeJob* jobs, int njobs, const int* bstart, const TraceSpill& sp,
+                      hipStream_t st) {
+    dim3 g(sp.blocks), b(PATH_BLOCK);
+#define SPPM_EM(WW, NN) hipLaunchKernelGGL((k_eye_multi<WW, NN>), g, b, 0, st, jobs, njobs, bstart, sp.cap, sp.buf)
+    if (nee) {
+        if (width == 2) SPPM_EM(2, true);
+        else if (width == 4) SPPM_EM(4, true);
+        else SPPM_EM(8, true);
+    } else {
+        if (width == 2) SPPM_EM(2, false);
+        else if (width == 4) SPPM_EM(4, false);
+        else SPPM_EM(8, false);
+    }
+#undef SPPM_EM
+}
+void launch_gather_chunks_multi(const GatherJob* jobs, int njobs, const int* bstart, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_chunks_multi, dim3(GATHER_WAVES), dim3(64), 0, st, jobs, njobs, bstart);
+}
+int gather_wave_count() { return GATHER_WAVES; }
+const uint32_t* gather_order(const GroupBuffers& gb, int np) {
+    return radix_sort_result(gb.keys, gb.vals, gb.tmp_k, gb.tmp_v, np, 31).vals;
 }
 int gather_tile_count(int W, int Hh) { return ((W + 7) >> 3) * ((Hh + 7) >> 3); }
 int gather_group_count(int np) { return (np + GATHER_GROUP - 1) / GATHER_GROUP; }

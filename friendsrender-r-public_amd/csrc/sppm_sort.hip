@@ -242,6 +242,11 @@ SortResult radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uin
     res.vals = va;
     return res;
 }
+SortResult radix_sort_result(uint32_t* keys, uint32_t* vals, uint32_t* tmp_k, uint32_t* tmp_v, int n, int bits) {
+    const int passes = (bits + OS_BITS - 1) / OS_BITS;
+    if (n <= 1 || passes % 2 == 0) return SortResult{keys, vals};
+    return SortResult{tmp_k, tmp_v};
+}
 size_t radix_hist_entries(int n) { return OS_STATUS + (size_t)OS_MAX_PASSES * os_tiles(n < 1 ? 1 : n) * OS_RADIX; }
 
 // Single-block exclusive scan of up to a few hundred thousand uint32 entries.

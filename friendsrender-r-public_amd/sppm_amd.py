@@ -34,7 +34,7 @@ EXPORTED = [
     "sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy", "sppm_autotune",
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
     "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
-    "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells", "sppm_camera_diaphragm",
+    "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells", "sppm_camera_diaphragm", "sppm_batch_mode",
 ]
 TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
@@ -164,6 +164,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_graph_capture.argtypes = [C.c_void_p]
     lib.sppm_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
     lib.sppm_batch_add.argtypes = [C.c_void_p, C.c_void_p]
+    lib.sppm_batch_mode.argtypes = [C.c_void_p, C.c_int]
     for n in ("sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy"):
         getattr(lib, n).argtypes = [C.c_void_p]
     lib.sppm_autotune.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(TuneResult)]
@@ -570,11 +571,16 @@ class Renderer:
         return out
 
 
+BATCH_MULTI, BATCH_BRANCHES = 0, 1  # sppm_batch_mode
+
+
 class Batch:
     """One graph launch per pass over several Renderers (each begun, not yet
-    stepped); their passes run as concurrent branches of the graph."""
+    stepped).  mode BATCH_MULTI: the photon, eye and gather-chunk kernels of
+    all scenes run as one launch each; BATCH_BRANCHES: each scene's pass is
+    its own graph branch."""
 
-    def __init__(self, renderers, device=0):
+    def __init__(self, renderers, device=0, mode=BATCH_MULTI):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_batch_create(C.byref(h), device))
@@ -582,6 +588,7 @@ class Batch:
         self.renderers = list(renderers)
         for r in self.renderers:
             _check(self.lib.sppm_batch_add(self.h, r.h))
+        _check(self.lib.sppm_batch_mode(self.h, mode))
         _check(self.lib.sppm_batch_capture(self.h))
 
     def pass_(self):

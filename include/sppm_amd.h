@@ -425,6 +425,17 @@ int sppm_graph_capture(sppm_ctx* ctx);
 typedef struct sppm_batch sppm_batch;
 int sppm_batch_create(sppm_batch** out, int device);
 int sppm_batch_add(sppm_batch* batch, sppm_ctx* ctx);
+/* How sppm_batch_capture lays out the batch's pass (before capture):
+ *   SPPM_BATCH_MULTI     (default) one launch per phase for all contexts: the
+ *                        photon passes, the camera words, the eye passes and
+ *                        the gathers' chunk kernels of every scene run as one
+ *                        persistent launch each (per BVH width / guided / NEE
+ *                        group), the per-scene setup kernels as graph branches;
+ *   SPPM_BATCH_BRANCHES  each context's pass as its own graph branch.
+ * Either renders each scene exactly as its own passes would. */
+#define SPPM_BATCH_MULTI 0
+#define SPPM_BATCH_BRANCHES 1
+int sppm_batch_mode(sppm_batch* batch, int mode);
 int sppm_batch_capture(sppm_batch* batch);
 /* one pass of every context in the batch (one graph launch; asynchronous) */
 int sppm_batch_pass(sppm_batch* batch);

@@ -42,14 +42,15 @@ def test_graph_replay_matches_eager(box_scene):
     graph.close()
 
 
-def test_batch_matches_individual_renders(box_scene, full_scene_small):
+@pytest.mark.parametrize("mode", [sppm_amd.BATCH_MULTI, sppm_amd.BATCH_BRANCHES], ids=["multi", "branches"])
+def test_batch_matches_individual_renders(box_scene, full_scene_small, mode):
     occl = scenes.occluded_scene(clutter_freq=6)
     setups = [(box_scene, scenes.pinhole_camera(32, 32), 0.9, 0.015, 8000, 4),
               (full_scene_small, scenes.thinlens_camera(48, 32, aperture=0.02, focus=2.0), 0.7, 0.03, 12000, 8),
               (occl, scenes.occluded_camera(40, 40), 0.7, 0.05, 10000, 2)]
     solo = [sppm_amd.Renderer(s, c, a, r0, n, bvh_width=w) for s, c, a, r0, n, w in setups]
     inb = [sppm_amd.Renderer(s, c, a, r0, n, bvh_width=w) for s, c, a, r0, n, w in setups]
-    batch = sppm_amd.Batch(inb)
+    batch = sppm_amd.Batch(inb, mode=mode)
     for p in range(3):
         for r in solo:
             r.pass_(stats=False)
@@ -62,6 +63,72 @@ def test_batch_matches_individual_renders(box_scene, full_scene_small):
             assert_same_state(a, b)
     batch.close()
     for r in solo + inb:
+        r.close()
+
+
+def test_multi_scene_batch_with_mixed_setups(box_scene, full_scene_small):
+    """Twelve scenes in one multi-scene batch: BVH widths 2 / 4 / 8 (one
+    photon and eye launch per width), eye-path NEE (its own eye launch), the
+    wavefront photon kernel (traced in its scene's branch), a diaphragm
+    camera, work from 2k to 60k photons (waves move between scenes): every
+    scene's state after each pass equals its own eager passes."""
+    occl = scenes.occluded_scene(clutter_freq=6)
+    box_ibl = scenes.box_scene_ibl()
+    dia = dict(scenes.thinlens_camera(40, 40, aperture=0.05, focus=1.5), diaphragm=scenes.aperture_image(24, 18))
+    setups = []
+    for k in range(12):
+        scene, cam = [(box_scene, scenes.pinhole_camera(24 + 4 * k, 24)),
+                      (full_scene_small, scenes.thinlens_camera(32, 24 + 4 * k, aperture=0.02, focus=2.0)),
+                      (occl, scenes.occluded_camera(32, 32)),
+                      (box_ibl, dia)][k % 4]
+        kw = dict(bvh_width=(2, 4, 8)[k % 3])
+        if k == 5:
+            kw["nee"] = sppm_amd.NEE_KEEP
+        if k == 7:
+            kw["photon_kernel"] = sppm_amd.PHOTON_WAVEFRONT
+        setups.append((scene, cam, 0.8, 0.02 * scene.diagonal(), 2000 + 5000 * k, kw))
+    solo = [sppm_amd.Renderer(s, c, a, r0, n, **kw) for s, c, a, r0, n, kw in setups]
+    inb = [sppm_amd.Renderer(s, c, a, r0, n, **kw) for s, c, a, r0, n, kw in setups]
+    batch = sppm_amd.Batch(inb)
+    for p in range(3):
+        for r in solo:
+            r.pass_(stats=False)
+        batch.pass_()
+        batch.sync()
+        for r in solo:
+            r.sync()
+        print(f"mixed batch pass {p + 1}", flush=True)
+        for k, (a, b) in enumerate(zip(solo, inb)):
+            assert_same_state(a, b)
+            ca, cb = a.counters(), b.counters()
+            for key in ("stored", "neighbours", "photon_rays", "eye_rays"):
+                assert ca[key] == cb[key], (k, key, ca[key], cb[key])
+    batch.close()
+    for r in solo + inb:
+        r.close()
+
+
+def test_multi_scene_batch_guided_runs(box_scene):
+    """Guided scenes in a multi-scene batch: the guided photon launch is its
+    own group; the proposals train and the images are finite."""
+    rs = []
+    for k in range(3):
+        r = sppm_amd.Renderer(box_scene, scenes.pinhole_camera(32, 32), 0.9, 0.015, 10000 + 5000 * k, bvh_width=4)
+        if k != 1:
+            r.guide_set(True, mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=2 + k, history_decay=0.95)
+        rs.append(r)
+    batch = sppm_amd.Batch(rs)
+    for _ in range(3):
+        batch.pass_()
+    batch.sync()
+    for k, r in enumerate(rs):
+        if k != 1:
+            prob, hist = r.guide_read()
+            assert abs(prob.sum() - 1.0) < 1e-3 and hist.sum() > 0
+        img = r.reconstruct()
+        assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    batch.close()
+    for r in rs:
         r.close()
 
 
