@@ -467,7 +467,7 @@ def batch_mode(args):
         r = sppm_amd.Renderer(w["scene"], w["cam"], w["alpha"], w["r0"], w["photons"], bvh_width=8)
         if not args.no_autotune:
             res = r.autotune(w["scene"], passes=2)
-            tuned.append((res["bvh_width"], res["grid_blocks"]))
+            tuned.append(res)
         rs.append(r)
         if (k + 1) % 8 == 0:
             log(f"set up {k + 1}/{len(work)} scenes")
@@ -495,9 +495,17 @@ def batch_mode(args):
     batch.sync()
     el = time.perf_counter() - tb
     photons = sum(w["photons"] for w in work) * K
-    widths = {}
-    for bw, _ in tuned:
-        widths[bw] = widths.get(bw, 0) + 1
+    def tally(key):
+        out = {}
+        for t in tuned:
+            out[str(key(t))] = out.get(str(key(t)), 0) + 1
+        return out
+    choices = dict(bvh=tally(lambda t: ("lbvh" if t["bvh_builder"] else "sah") + str(t["bvh_width"])),
+                   grid=tally(lambda t: t["grid_blocks"]),
+                   photon_kernel=tally(lambda t: "wavefront" if t["photon_kernel"] else "megakernel"),
+                   photons_suggested_vs_set=tally(lambda t: "half" if t["photons_per_s"][0] == max(t["photons_per_s"])
+                                                  else ("same" if t["photons_per_s"][1] == max(t["photons_per_s"])
+                                                        else "twice")))
     line = {
         "metric": "photons/s (batched plain SPPM, aggregate over scenes, one graph launch per pass)",
         "batch_mode": args.batch_mode,
@@ -508,7 +516,7 @@ def batch_mode(args):
                    "triangles_min": min(tris), "triangles_max": max(tris),
                    "resolutions": sorted({w["cam"]["width"] for w in work}),
                    "photons_per_pass_total": sum(w["photons"] for w in work), "alpha": 0.7,
-                   "autotune": not args.no_autotune, "tuned_bvh_widths": widths},
+                   "autotune": not args.no_autotune, "tuned": choices if tuned else None},
         "sequential_eager_photons_per_s": round(photons * (W + K) / K / seq, 1),
         "batch_speedup_vs_sequential": round(seq * K / (W + K) / el, 3),
     }

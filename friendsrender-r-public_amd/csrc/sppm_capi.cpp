@@ -1467,6 +1467,22 @@ extern "C" {
 // x path-kernel grid {device-filling, half}, keeps the fastest, and leaves the
 // context re-begun with it.  The closest hit is width-independent, so the
 // choice changes time only.
+// ms per pass of `passes` passes after a warm-up pass, everything begun anew
+static int tune_time(sppm_ctx* c, int passes, hipEvent_t e0, hipEvent_t e1, float* ms) {
+    int rc;
+    // warm-up pass, finished on every stream before the timed passes start
+    // (the next photon pass may otherwise begin before e0)
+    if ((rc = sppm_begin(c)) || (rc = sppm_pass(c, nullptr)) || (rc = sppm_sync(c))) return rc;
+    c->lead_ok = false;
+    HIPOK(hipEventRecord(e0, c->stream));
+    for (int p = 0; p < passes; ++p)
+        if ((rc = sppm_pass(c, nullptr))) return rc;
+    HIPOK(hipEventRecord(e1, c->stream));
+    HIPOK(hipEventSynchronize(e1));
+    HIPOK(hipEventElapsedTime(ms, e0, e1));
+    *ms /= (float)passes;
+    return SPPM_OK;
+}
 int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tune_result* out) {
     if (!c || !scene || !out) return fail(SPPM_E_ARG, "NULL argument");
     if (passes < 1) return fail(SPPM_E_ARG, "passes < 1");
@@ -1474,46 +1490,80 @@ int sppm_autotune(sppm_ctx* c, const sppm_scene_desc* scene, int passes, sppm_tu
     if (int rc0 = not_in_batch(c, "sppm_autotune")) return rc0;
     c->lead_ok = false;
     memset(out, 0, sizeof *out);
-    const int widths[3] = {2, 4, 8};
+    const int widths[3] = {2, 4, 8}, builders[2] = {0, SPPM_BVH_LBVH};
     float best = 3.4e38f;
-    int bw = 8, bb = 0, k = 0, rc;
+    int bw = 8, bf = 0, rc = SPPM_OK;
     hipEvent_t e0, e1;
     HIPOK(hipEventCreate(&e0));
     HIPOK(hipEventCreate(&e1));
-    for (int w : widths) {
-        if ((rc = sppm_scene_upload(c, scene, w))) break;
-        const int full = c->scene.spill_blocks;
-        for (int half = 0; half < 2; ++half, ++k) {
-            c->tuned_blocks = half ? full / 2 : 0;
-            // warm-up pass, finished on every stream before the timed passes
-            // start (the next photon pass may otherwise begin before e0)
-            if ((rc = sppm_begin(c)) || (rc = sppm_pass(c, nullptr)) || (rc = sppm_sync(c))) break;
-            c->lead_ok = false;
-            HIPOK(hipEventRecord(e0, c->stream));
-            for (int p = 0; p < passes && !rc; ++p) rc = sppm_pass(c, nullptr);
-            if (rc) break;
-            HIPOK(hipEventRecord(e1, c->stream));
-            HIPOK(hipEventSynchronize(e1));
+    const int kernel0 = c->params.photon_kernel;
+    c->params.photon_kernel = SPPM_PHOTON_MEGAKERNEL;
+    // stage 1: builder x width, device-filling grid, megakernel
+    c->tuned_blocks = 0;
+    for (int b = 0; b < 2 && !rc; ++b)
+        for (int i = 0; i < 3 && !rc; ++i) {
             float ms = 0.0f;
-            HIPOK(hipEventElapsedTime(&ms, e0, e1));
-            ms /= (float)passes;
-            out->trial_ms[k] = ms;
+            if ((rc = sppm_scene_upload(c, scene, widths[i] | builders[b]))) break;
+            if ((rc = tune_time(c, passes, e0, e1, &ms))) break;
+            out->trial_ms[b * 3 + i] = ms;
             if (ms < best) {
                 best = ms;
-                bw = w;
-                bb = c->tuned_blocks;
+                bw = widths[i];
+                bf = builders[b];
             }
         }
-        if (rc) break;
+    // stage 2: grid; stage 3: photon kernel
+    int bb = 0, bk = SPPM_PHOTON_MEGAKERNEL;
+    if (!rc) rc = sppm_scene_upload(c, scene, bw | bf);
+    const int full = c->scene.spill_blocks;
+    float gbest = 3.4e38f;
+    for (int g = 0; g < 3 && !rc; ++g) {
+        float ms = 0.0f;
+        c->tuned_blocks = g == 0 ? 0 : (g == 1 ? full * 3 / 4 : full / 2);
+        if ((rc = tune_time(c, passes, e0, e1, &ms))) break;
+        out->grid_ms[g] = ms;
+        if (ms < gbest) {
+            gbest = ms;
+            bb = c->tuned_blocks;
+        }
     }
+    c->tuned_blocks = bb;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        float ms = 0.0f;
+        c->params.photon_kernel = k == 0 ? SPPM_PHOTON_MEGAKERNEL : SPPM_PHOTON_WAVEFRONT;
+        if ((rc = tune_time(c, passes, e0, e1, &ms))) break;
+        out->kernel_ms[k] = ms;
+        if (k == 0 || ms < out->kernel_ms[0]) bk = c->params.photon_kernel;
+    }
+    c->params.photon_kernel = bk;
+    best = out->kernel_ms[bk == SPPM_PHOTON_MEGAKERNEL ? 0 : 1];
+    // photons per pass: measured, reported, not applied
+    const int n0 = c->params.photons_per_pass;
+    const int cand[3] = {std::max(1, n0 / 2), n0,
+                         (int)std::min<long long>(2LL * n0, RADIX_SORT_MAX_N / (NSLOT - 1))};
+    float bestps = 0.0f;
+    for (int q = 0; q < 3 && !rc; ++q) {
+        float ms = 0.0f;
+        c->params.photons_per_pass = cand[q];
+        if ((rc = tune_time(c, passes, e0, e1, &ms))) break;
+        out->photons_per_s[q] = ms > 0.0f ? (float)cand[q] / (ms * 1e-3f) : 0.0f;
+        if (out->photons_per_s[q] > bestps) {
+            bestps = out->photons_per_s[q];
+            out->photons_suggested = cand[q];
+        }
+    }
+    c->params.photons_per_pass = n0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    if ((rc = sppm_scene_upload(c, scene, bw))) return rc;
-    c->tuned_blocks = bb;
+    if (rc) {
+        c->params.photon_kernel = kernel0;
+        return rc;
+    }
     c->params.bvh_width = bw;
     if ((rc = sppm_begin(c))) return rc;
     out->bvh_width = bw;
+    out->bvh_builder = bf;
+    out->photon_kernel = bk;
     out->grid_blocks = bb ? bb : c->scene.spill_blocks;
     out->ms_per_pass = best;
     return SPPM_OK;

@@ -52,7 +52,9 @@ def mt19937_jump(state, steps, lib=None):
 
 class TuneResult(C.Structure):
     _fields_ = [("bvh_width", C.c_int32), ("grid_blocks", C.c_int32), ("ms_per_pass", C.c_float),
-                ("trial_ms", C.c_float * 6)]
+                ("trial_ms", C.c_float * 6), ("bvh_builder", C.c_int32), ("photon_kernel", C.c_int32),
+                ("grid_ms", C.c_float * 3), ("kernel_ms", C.c_float * 2), ("photons_per_s", C.c_float * 3),
+                ("photons_suggested", C.c_int32)]
 GUIDE_CELLS = 262144
 BVH_LBVH = 0x100  # bvh_width flag: radix tree over Morton codes instead of binned SAH
 GUIDE_NPARAMS = 16705
@@ -382,7 +384,9 @@ class Renderer:
         self.passes = 0
         self.total_photons = 0
         return dict(bvh_width=res.bvh_width, grid_blocks=res.grid_blocks, ms_per_pass=res.ms_per_pass,
-                    trial_ms=list(res.trial_ms))
+                    trial_ms=list(res.trial_ms), bvh_builder=res.bvh_builder, photon_kernel=res.photon_kernel,
+                    grid_ms=list(res.grid_ms), kernel_ms=list(res.kernel_ms),
+                    photons_per_s=list(res.photons_per_s), photons_suggested=res.photons_suggested)
 
     # ---- learned emission proposal --------------------------------------
     def guide_set(self, enable=True, mix_plain=0.5, learning_rate=1e-3, train_steps=4, seed=1, history_decay=0.9,

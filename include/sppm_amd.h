@@ -34,10 +34,11 @@
 extern "C" {
 #endif
 
-#define SPPM_ABI_VERSION 5  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
+#define SPPM_ABI_VERSION 6  /* 2: sppm_guide_params gained hidden, layers, fp16; 3: sppm_params.eye_nee;
                                4: sppm_guide_params.train_interval, sppm_params.step_offset;
                                5: sppm_params.photon_kernel, wave_bounces, wave_lanes,
-                                  sppm_guide_params.target */
+                                  sppm_guide_params.target;
+                               6: sppm_tune_result gained bvh_builder ... photons_suggested */
 
 /* ---- status codes -------------------------------------------------- */
 #define SPPM_OK 0
@@ -443,15 +444,28 @@ int sppm_batch_sync(sppm_batch* batch);
 /* destroys the batch (not its contexts) */
 int sppm_batch_destroy(sppm_batch* batch);
 
-/* Autotuner: times `passes` passes of each BVH width {2, 4, 8} x path-kernel
- * grid {device-filling, half}, keeps the fastest (the context is left begun
- * with it).  Call after sppm_camera_set and sppm_params_set; `scene` is the
- * scene to (re)upload. */
+/* Autotuner: times `passes` passes (after one warm-up pass) of each
+ * configuration in stages and keeps the fastest of each stage (the context
+ * is left begun with the result):
+ *   1. closest-hit structure: builder {binned SAH, LBVH} x BVH width {2, 4, 8};
+ *   2. path-kernel grid {device-filling, 3/4, 1/2} (blocks of 64 lanes);
+ *   3. photon kernel {megakernel, wavefront}.
+ * None of these changes a pass's result.  Then it times photons/s at half,
+ * the same and twice photons_per_pass with the chosen configuration and
+ * reports the best as photons_suggested, without applying it (the photon
+ * count per pass is part of the estimate).  Call after sppm_camera_set and
+ * sppm_params_set; `scene` is the scene to (re)upload. */
 typedef struct sppm_tune_result {
     int32_t bvh_width;     /* chosen BVH width */
     int32_t grid_blocks;   /* chosen path-kernel grid (blocks of 64 lanes) */
     float ms_per_pass;     /* chosen configuration */
-    float trial_ms[6];     /* width 2, 4, 8 x grid full, half */
+    float trial_ms[6];     /* stage 1: width 2, 4, 8 with SAH, then with LBVH */
+    int32_t bvh_builder;   /* chosen builder: 0 binned SAH, SPPM_BVH_LBVH */
+    int32_t photon_kernel; /* chosen SPPM_PHOTON_* */
+    float grid_ms[3];      /* stage 2: grid full, 3/4, 1/2 */
+    float kernel_ms[2];    /* stage 3: megakernel, wavefront */
+    float photons_per_s[3];     /* half, same, twice photons_per_pass */
+    int32_t photons_suggested;  /* the fastest of those photon counts per pass */
 } sppm_tune_result;
 int sppm_autotune(sppm_ctx* ctx, const sppm_scene_desc* scene, int passes, sppm_tune_result* out);
 

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_error_paths_without_gpu():
     lib = sppm_amd.load_library()
-    assert lib.sppm_abi_version() == 5
+    assert lib.sppm_abi_version() == 6
     # NULL arguments are rejected before any device work
     assert lib.sppm_create(None, 0, None) == -1
     assert b"NULL" in lib.sppm_last_error()

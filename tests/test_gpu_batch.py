@@ -139,7 +139,15 @@ def test_autotune_keeps_results(full_scene_small):
     res = tuned.autotune(full_scene_small, passes=2)
     print("autotune", res, flush=True)
     assert res["bvh_width"] in (2, 4, 8) and res["grid_blocks"] > 0
-    assert all(t > 0 for t in res["trial_ms"]) and res["ms_per_pass"] == min(res["trial_ms"])
+    assert res["bvh_builder"] in (0, sppm_amd.BVH_LBVH)
+    assert res["photon_kernel"] in (sppm_amd.PHOTON_MEGAKERNEL, sppm_amd.PHOTON_WAVEFRONT)
+    assert all(t > 0 for t in res["trial_ms"] + res["grid_ms"] + res["kernel_ms"] + res["photons_per_s"])
+    # each stage keeps its fastest trial; the result is the last stage's choice
+    k = 3 * (res["bvh_builder"] == sppm_amd.BVH_LBVH) + (2, 4, 8).index(res["bvh_width"])
+    assert res["trial_ms"][k] == min(res["trial_ms"])
+    assert res["ms_per_pass"] == min(res["kernel_ms"])
+    assert res["photons_suggested"] in (7500, 15000, 30000)
+    assert res["photons_per_s"][(7500, 15000, 30000).index(res["photons_suggested"])] == max(res["photons_per_s"])
     for _ in range(2):
         ref.pass_(stats=False)
         tuned.pass_(stats=False)
