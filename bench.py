@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=255)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["full", "box"], default="full")
+    ap.add_argument("--bvh", type=str, default=None,
+                    help="closest-hit structure of the main workload: BVH width 2, 4 or 8, with an 'l' suffix "
+                         "for the LBVH builder (e.g. 4l); default 8 (binned SAH)")
     ap.add_argument("--photon-kernel", choices=["megakernel", "wavefront"], default=None,
                     help="photon pass kernels (default: the library's default)")
     ap.add_argument("--wave-bounces", type=int, default=0, help="wavefront: bounces from ray queues (0 = default)")
@@ -120,6 +123,25 @@ def read_traffic(path):
         if line.strip().startswith("bytes_per_launch:"):
             return int(float(line.split(":", 1)[1]))
     return None
+
+
+def read_valu(path):
+    """Per-launch VALU counts of the pass kernels from their committed counter
+    summary (lines 'lane_ops_per_launch.<kernel>: N' and
+    'issued_slots_per_launch.<kernel>: N'): {kernel: (lane_ops, slots)}."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        t = line.strip()
+        for key in ("lane_ops_per_launch.", "issued_slots_per_launch."):
+            if t.startswith(key):
+                k, v = t[len(key):].split(":", 1)
+                out.setdefault(k, [0.0, 0.0])[0 if key.startswith("lane") else 1] = float(v)
+    return out
+
+
+VALU_PEAK_T = 39.3  # MI355X vector ALUs: 256 CUs x 64 lanes x 2.4 GHz, T lane operations / s
 
 
 def log(msg):
@@ -644,6 +666,8 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     scene, cam, cfg = workload(args.workload)
+    if args.bvh:
+        cfg["bvh"] = int(args.bvh.rstrip("l")) | (sppm_amd.BVH_LBVH if args.bvh.endswith("l") else 0)
     pk = photon_kernel_args(args)
     log(f"scene ready: {cfg['name']}, {cfg['triangles']} triangles")
     r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"], **pk)
@@ -707,6 +731,22 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{rk}_{args.workload}.md")) \
         if args.workload == "full" else None
     value = cfg["photons"] * K / el
+    # the photon pass (the longest phase) is neither HBM nor MFMA bound: its
+    # VALU rate, from per-launch counter totals (committed for the BVH8
+    # megakernel; the photon pass's work does not depend on the radius)
+    valu = read_valu(os.path.join(ROOT, "profiles", "valu_pass_kernels_full_material.md")) \
+        if args.workload == "full" and cfg["bvh"] == 8 and args.photon_kernel in (None, "megakernel") else {}
+    valu_line = None
+    if "k_photon" in valu:
+        lane_ops, slots = valu["k_photon"]
+        t = ms_alone["photons"] * 1e-3
+        valu_line = {"kernel": "k_photon (photon pass alone)", "bound": "valu",
+                     "achieved": round(lane_ops / t / 1e12, 3), "peak": VALU_PEAK_T, "unit": "T lane-op/s",
+                     "frac": round(lane_ops / t / 1e12 / VALU_PEAK_T, 4),
+                     "issued_frac": round(slots / t / 1e12 / VALU_PEAK_T, 4),
+                     "lane_ops_per_launch": lane_ops, "issued_slots_per_launch": slots,
+                     "source": "profiles/valu_pass_kernels_full_material.md (counter totals per launch) / the "
+                               "photon phase's own duration (phases_alone_ms.photons)"}
     out = {
         "metric": "photons/s + seconds to RMSE<=0.01, guided vs plain SPPM, 1 MI355X",
         "value": round(value, 1),
@@ -721,7 +761,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": cfg["name"], "resolution": f"{cfg['width']}x{cfg['height']}",
-                   "photons_per_pass": cfg["photons"], "triangles": cfg["triangles"], "bvh_width": cfg["bvh"],
+                   "photons_per_pass": cfg["photons"], "triangles": cfg["triangles"], "bvh_width": cfg["bvh"] & 0xff,
+                   "bvh_builder": "lbvh" if cfg["bvh"] & 0x100 else "sah",
                    "alpha": cfg["alpha"], "r0": round(cfg["r0"], 6), "camera": cam["type"] and "thin-lens" or "pinhole",
                    "photon_kernel": args.photon_kernel or "default", "wave_bounces": args.wave_bounces or "default",
                    "wave_lanes": args.wave_lanes or "default", "parallelism": "single GPU"},
@@ -736,6 +777,7 @@ def main():
                            "duration": "gather phase alone: the same passes rendered again with the split steps"},
         "phases_ms": {k: round(v, 3) for k, v in ms.items()},
         "phases_alone_ms": {k: round(v, 3) for k, v in ms_alone.items()},
+        "roofline_valu": valu_line,
         "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
                      "gather_candidates": int(cands), "photon_rays": int(prays), "eye_rays": int(erays),
                      "hit_point_updates_per_s": round(hits / (ms_alone["gather"] * 1e-3), 1),

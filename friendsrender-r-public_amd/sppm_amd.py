@@ -310,7 +310,7 @@ class Renderer:
         if cam.get("diaphragm") is not None:
             self._dia = diaphragm_rgba(cam["diaphragm"])
             _check(self.lib.sppm_camera_diaphragm(self.h, self._dia.shape[1], self._dia.shape[0], _fp(self._dia)))
-        self._params = params_struct(alpha, first_radius, photons, bvh_width, record, nee=nee, max_pairs=max_pairs,
+        self._params = params_struct(alpha, first_radius, photons, bvh_width & 0xff, record, nee=nee, max_pairs=max_pairs,
                                      step_offset=step_offset, photon_kernel=photon_kernel, wave_bounces=wave_bounces,
                                      wave_lanes=wave_lanes)
         _check(self.lib.sppm_params_set(self.h, C.byref(self._params)))
