@@ -80,8 +80,9 @@ def parse():
                          "depends on the photons gathered and a finite plain reference is not bias-converged")
     ap.add_argument("--batch", action="store_true",
                     help="batched mode: many uneven seeded scenes, one graph launch per pass over all of them")
-    ap.add_argument("--batch-mode", choices=["multi", "branches"], default="multi",
-                    help="batch mode: one launch per phase for all scenes (multi) or a graph branch per scene")
+    ap.add_argument("--batch-mode", choices=["auto", "multi", "branches"], default="auto",
+                    help="batch mode: one launch per phase for all scenes (multi), a graph branch per scene "
+                         "(branches), or by the mean photons per pass (auto)")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--caustic", action="store_true",
@@ -142,6 +143,10 @@ def read_valu(path):
 
 
 VALU_PEAK_T = 39.3  # MI355X vector ALUs: 256 CUs x 64 lanes x 2.4 GHz, T lane operations / s
+
+
+BATCH_MODES = {"multi": 0, "branches": 1, "auto": 2}  # sppm_batch_mode
+BATCH_MODE_NAMES = {0: "multi", 1: "branches"}
 
 
 def log(msg):
@@ -507,7 +512,8 @@ def batch_mode(args):
     for r in rs:
         sppm_amd._check(r.lib.sppm_begin(r.h))
         r.passes = r.total_photons = 0
-    batch = sppm_amd.Batch(rs, mode={"multi": sppm_amd.BATCH_MULTI, "branches": sppm_amd.BATCH_BRANCHES}[args.batch_mode])
+    batch = sppm_amd.Batch(rs, mode=BATCH_MODES[args.batch_mode])
+    used_mode = BATCH_MODE_NAMES[batch.mode]
     for _ in range(W):
         batch.pass_()
     batch.sync()
@@ -530,7 +536,7 @@ def batch_mode(args):
                                                         else "twice")))
     line = {
         "metric": "photons/s (batched plain SPPM, aggregate over scenes, one graph launch per pass)",
-        "batch_mode": args.batch_mode,
+        "batch_mode": f"{args.batch_mode} ({used_mode})" if args.batch_mode == "auto" else used_mode,
         "value": round(photons / el, 1), "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(el / K * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
@@ -573,7 +579,8 @@ def caustic_mode(args):
                             hidden=128, layers=6, fp16=True)
             rs.append(r)
         log(f"{mode}: {len(rs)} contexts set up")
-        batch = sppm_amd.Batch(rs)
+        batch = sppm_amd.Batch(rs, mode=BATCH_MODES[args.batch_mode])
+        used_mode = BATCH_MODE_NAMES[batch.mode]
         for _ in range(W):
             batch.pass_()
         batch.sync()
@@ -597,6 +604,7 @@ def caustic_mode(args):
             r.close()
     line = {
         "metric": "photons/s (caustic workload, aggregate over cameras, one graph launch per pass)",
+        "batch_mode": f"{args.batch_mode} ({used_mode})" if args.batch_mode == "auto" else used_mode,
         "value": out["plain"]["photons_per_s"], "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": W,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (MLP fp16 MFMA)",
         "data": "synthetic (seeded procedural scene)",

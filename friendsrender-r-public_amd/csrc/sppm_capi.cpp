@@ -242,7 +242,7 @@ struct BatchLaunch {
 };
 struct sppm_batch {
     int device = 0;
-    int mode = SPPM_BATCH_MULTI;
+    int mode = SPPM_BATCH_AUTO;
     std::vector<sppm_ctx*> ctx;
     std::vector<hipStream_t> s1, s2;
     std::vector<hipEvent_t> join;
@@ -1608,8 +1608,8 @@ int sppm_batch_add(sppm_batch* b, sppm_ctx* c) {
 int sppm_batch_mode(sppm_batch* b, int mode) {
     if (!b) return fail(SPPM_E_ARG, "NULL batch");
     if (b->exec) return fail(SPPM_E_STATE, "batch already captured");
-    if (mode != SPPM_BATCH_MULTI && mode != SPPM_BATCH_BRANCHES)
-        return fail(SPPM_E_ARG, "mode must be SPPM_BATCH_MULTI (0) or SPPM_BATCH_BRANCHES (1)");
+    if (mode != SPPM_BATCH_MULTI && mode != SPPM_BATCH_BRANCHES && mode != SPPM_BATCH_AUTO)
+        return fail(SPPM_E_ARG, "mode must be SPPM_BATCH_MULTI (0), SPPM_BATCH_BRANCHES (1) or SPPM_BATCH_AUTO (2)");
     b->mode = mode;
     return SPPM_OK;
 }
@@ -1679,10 +1679,9 @@ static int batch_branches(sppm_batch* b, hipStream_t st, const std::function<voi
 }
 
 // The batch pass as one launch per phase for all scenes, in one graph:
-//   per scene: clear the work counters (scenes on the wavefront photon
-//     kernels trace their photon pass here);
+//   per scene: clear the work counters;
 //   one photon launch per group of scenes sharing the kernel's template
-//     arguments (BVH width, guided emission);
+//     arguments (BVH width, guided emission), the megakernel for every scene;
 //   per scene: the grid build;
 //   the camera words of all scenes (two launches), one eye launch per BVH
 //     width / NEE group;
@@ -1705,7 +1704,10 @@ static int prepare_multi(sppm_batch* b) {
         sppm_ctx* c = b->ctx[k];
         RenderBuffers& R = c->rb;
         cap = std::max(cap, c->scene.stack_cap);
-        if (R.wave_bounces <= 0) {
+        {
+            // every scene's photon pass joins the shared megakernel launch,
+            // whatever photon kernel the context chose for its own passes
+            // (the result is the same)
             PhotonJob J{};
             J.S = c->scene.dev;
             J.slots = R.slots;
@@ -1807,10 +1809,7 @@ static int prepare_multi(sppm_batch* b) {
 }
 static int enqueue_multi(sppm_batch* b, hipStream_t st) {
     int r = batch_branches(b, st, [&](int k, hipStream_t bs) {
-        sppm_ctx* c = b->ctx[k];
-        if (c->rb.wave_bounces > 0) photon_pass(c, 0, c->rb.pass_dev, bs, false);
-        else launch_zero_i32(c->scene.next_path, STRIPE_INTS, bs);
-        launch_zero_i32(c->scene.next_path + STRIPE_INTS, STRIPE_INTS, bs);
+        launch_zero_i32(b->ctx[k]->scene.next_path, 2 * STRIPE_INTS, bs);
     });
     if (r) return r;
     for (const BatchLaunch& l : b->pl) {
@@ -1862,6 +1861,12 @@ int sppm_batch_capture(sppm_batch* b) {
         if (rc) return rc;
         HIPOK(hipStreamSynchronize(c->stream));
     }
+    if (b->mode == SPPM_BATCH_AUTO) {
+        double photons = 0.0;
+        for (sppm_ctx* c : b->ctx) photons += (double)c->rb.N;
+        b->mode = photons < (double)SPPM_BATCH_AUTO_PHOTONS * (double)b->ctx.size() ? SPPM_BATCH_MULTI
+                                                                                       : SPPM_BATCH_BRANCHES;
+    }
     if (b->mode == SPPM_BATCH_MULTI) return capture_multi(b);
     HIPOK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
     int rc = SPPM_OK;
@@ -1886,6 +1891,12 @@ int sppm_batch_capture(sppm_batch* b) {
         return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
     }
     b->graph = g;
+    return SPPM_OK;
+}
+
+int sppm_batch_get_mode(sppm_batch* b, int32_t* mode) {
+    if (!b || !mode) return fail(SPPM_E_ARG, "NULL argument");
+    *mode = b->mode;
     return SPPM_OK;
 }
 

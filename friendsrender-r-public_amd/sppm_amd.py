@@ -35,6 +35,7 @@ EXPORTED = [
     "sppm_mt19937_jump", "sppm_checkpoint_size", "sppm_checkpoint_save", "sppm_checkpoint_load",
     "sppm_read_direct", "sppm_trace_occluded", "sppm_ray_counts", "sppm_guide_tables", "sppm_trace_closest_static",
     "sppm_tie_counts", "sppm_read_grid", "sppm_guide_path_cells", "sppm_camera_diaphragm", "sppm_batch_mode",
+    "sppm_batch_get_mode",
 ]
 TRACE_STATIC_MAX_RAYS = 65536  # SPPM_TRACE_STATIC_MAX_RAYS
 
@@ -167,6 +168,7 @@ def load_library(path=LIB_PATH):
     lib.sppm_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
     lib.sppm_batch_add.argtypes = [C.c_void_p, C.c_void_p]
     lib.sppm_batch_mode.argtypes = [C.c_void_p, C.c_int]
+    lib.sppm_batch_get_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     for n in ("sppm_batch_capture", "sppm_batch_pass", "sppm_batch_sync", "sppm_batch_destroy"):
         getattr(lib, n).argtypes = [C.c_void_p]
     lib.sppm_autotune.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(TuneResult)]
@@ -575,16 +577,16 @@ class Renderer:
         return out
 
 
-BATCH_MULTI, BATCH_BRANCHES = 0, 1  # sppm_batch_mode
+BATCH_MULTI, BATCH_BRANCHES, BATCH_AUTO = 0, 1, 2  # sppm_batch_mode
 
 
 class Batch:
     """One graph launch per pass over several Renderers (each begun, not yet
     stepped).  mode BATCH_MULTI: the photon, eye and gather-chunk kernels of
     all scenes run as one launch each; BATCH_BRANCHES: each scene's pass is
-    its own graph branch."""
+    its own graph branch; BATCH_AUTO picks one by the mean photons per pass."""
 
-    def __init__(self, renderers, device=0, mode=BATCH_MULTI):
+    def __init__(self, renderers, device=0, mode=BATCH_AUTO):
         self.lib = lib()
         h = C.c_void_p()
         _check(self.lib.sppm_batch_create(C.byref(h), device))
@@ -594,6 +596,9 @@ class Batch:
             _check(self.lib.sppm_batch_add(self.h, r.h))
         _check(self.lib.sppm_batch_mode(self.h, mode))
         _check(self.lib.sppm_batch_capture(self.h))
+        m = C.c_int32()
+        _check(self.lib.sppm_batch_get_mode(self.h, C.byref(m)))
+        self.mode = m.value  # the mode in use (what BATCH_AUTO chose)
 
     def pass_(self):
         _check(self.lib.sppm_batch_pass(self.h))

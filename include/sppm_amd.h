@@ -427,16 +427,26 @@ typedef struct sppm_batch sppm_batch;
 int sppm_batch_create(sppm_batch** out, int device);
 int sppm_batch_add(sppm_batch* batch, sppm_ctx* ctx);
 /* How sppm_batch_capture lays out the batch's pass (before capture):
- *   SPPM_BATCH_MULTI     (default) one launch per phase for all contexts: the
- *                        photon passes, the camera words, the eye passes and
- *                        the gathers' chunk kernels of every scene run as one
+ *   SPPM_BATCH_MULTI     one launch per phase for all contexts: the photon
+ *                        passes, the camera words, the eye passes and the
+ *                        gathers' chunk kernels of every scene run as one
  *                        persistent launch each (per BVH width / guided / NEE
  *                        group), the per-scene setup kernels as graph branches;
- *   SPPM_BATCH_BRANCHES  each context's pass as its own graph branch.
- * Either renders each scene exactly as its own passes would. */
+ *   SPPM_BATCH_BRANCHES  each context's pass as its own graph branch (the
+ *                        phases of different scenes overlap);
+ *   SPPM_BATCH_AUTO      (default) MULTI when the contexts average fewer than
+ *                        SPPM_BATCH_AUTO_PHOTONS photons per pass (many small
+ *                        passes, whose launch tails the shared launches
+ *                        remove), else BRANCHES (passes that fill the device
+ *                        by themselves overlap better).
+ * Every mode renders each scene exactly as its own passes would. */
 #define SPPM_BATCH_MULTI 0
 #define SPPM_BATCH_BRANCHES 1
+#define SPPM_BATCH_AUTO 2
+#define SPPM_BATCH_AUTO_PHOTONS 2000000
 int sppm_batch_mode(sppm_batch* batch, int mode);
+/* the mode in use (after sppm_batch_capture: the one SPPM_BATCH_AUTO chose) */
+int sppm_batch_get_mode(sppm_batch* batch, int32_t* mode);
 int sppm_batch_capture(sppm_batch* batch);
 /* one pass of every context in the batch (one graph launch; asynchronous) */
 int sppm_batch_pass(sppm_batch* batch);

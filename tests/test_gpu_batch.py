@@ -69,7 +69,8 @@ def test_batch_matches_individual_renders(box_scene, full_scene_small, mode):
 def test_multi_scene_batch_with_mixed_setups(box_scene, full_scene_small):
     """Twelve scenes in one multi-scene batch: BVH widths 2 / 4 / 8 (one
     photon and eye launch per width), eye-path NEE (its own eye launch), the
-    wavefront photon kernel (traced in its scene's branch), a diaphragm
+    wavefront photon kernel (its photon pass joins the shared megakernel
+    launch), a diaphragm
     camera, work from 2k to 60k photons (waves move between scenes): every
     scene's state after each pass equals its own eager passes."""
     occl = scenes.occluded_scene(clutter_freq=6)
@@ -89,7 +90,8 @@ def test_multi_scene_batch_with_mixed_setups(box_scene, full_scene_small):
         setups.append((scene, cam, 0.8, 0.02 * scene.diagonal(), 2000 + 5000 * k, kw))
     solo = [sppm_amd.Renderer(s, c, a, r0, n, **kw) for s, c, a, r0, n, kw in setups]
     inb = [sppm_amd.Renderer(s, c, a, r0, n, **kw) for s, c, a, r0, n, kw in setups]
-    batch = sppm_amd.Batch(inb)
+    batch = sppm_amd.Batch(inb)  # automatic mode: small passes -> multi-scene launches
+    assert batch.mode == sppm_amd.BATCH_MULTI
     for p in range(3):
         for r in solo:
             r.pass_(stats=False)
