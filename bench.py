@@ -65,12 +65,14 @@ def parse():
                     help="leave out the time-to-RMSE leg (guided vs plain on the occluded-light scene)")
     ap.add_argument("--mix", type=float, default=0.5, help="guided emission: probability of the plain emission")
     ap.add_argument("--lr", type=float, default=1e-3, help="guided emission: Adam step size")
-    ap.add_argument("--train-steps", type=int, default=1, help="guided emission: Adam steps per training pass")
-    ap.add_argument("--train-interval", type=int, default=1, help="guided emission: train after every k-th pass")
+    ap.add_argument("--train-steps", type=int, default=2, help="guided emission: Adam steps per training pass")
+    ap.add_argument("--train-interval", type=int, default=4, help="guided emission: train after every k-th pass")
     ap.add_argument("--decay", type=float, default=0.95, help="guided emission: training histogram decay")
-    ap.add_argument("--guide-target", choices=["mean", "rms"], default="mean",
+    ap.add_argument("--guide-target", choices=["mean", "rms"], default="rms",
                     help="guided emission: the proposal's training target (mean or root-mean-square deposit)")
     ap.add_argument("--max-passes", type=int, default=2000)
+    ap.add_argument("--no-bound", action="store_true",
+                    help="time-to-RMSE mode: leave out the emission-proposal bound")
     ap.add_argument("--check-every", type=int, default=16)
     ap.add_argument("--target-rmse", type=float, default=0.01)
     ap.add_argument("--resolution", type=int, default=1024)
@@ -469,7 +471,7 @@ def rmse_mode(args):
     if args.ref_save:
         return
     res = time_to_rmse(args, w, ref)
-    if args.rmse_scene == "occluded":
+    if args.rmse_scene == "occluded" and not args.no_bound:
         res["emission_bound"] = emission_gain_bound(w, res["plain"]["ms_per_pass"],
                                                     res["plain"]["phases_ms"]["gather"])
     line = {"metric": res["metric"], "value": res["guided_s"], "unit": "s", "higher_is_better": False,
