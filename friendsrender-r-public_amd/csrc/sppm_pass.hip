@@ -1146,26 +1146,19 @@ __device__ __forceinline__ bool in_query(const GatherLane& L, const float4 pp) {
 // keeps the lanes of a wave busy in the second loop instead of evaluating
 // one lane's BRDF per candidate; each lane still adds its neighbours in
 // ascending candidate order.
-// LDS of the gather chunk loop (one wave per block)
-struct GatherLds {
-    float4 pos[64], flux[64], dir[64];
-    unsigned char hit[64 * 64];
-    int cnt[64];  // hit points of this group gathering each staged photon (guided emission)
-};
-
 // The chunks of one scene's gather, claimed until its counters are exhausted.
-__device__ __forceinline__ void gather_chunks(const HitDev& H, const SharedDev& SH, const GridDev& G,
+__device__ __forceinline__ void gather_chunks(HitDev H, SharedDev SH, GridDev G,
                                               const GridParams* __restrict__ gp, const uint32_t* __restrict__ order,
                                               const int* __restrict__ nvalidp, const TileRange* __restrict__ tiles,
                                               int ntiles, const int* __restrict__ chunk_off,
                                               const int* __restrict__ chunkp, int* __restrict__ queue,
                                               float4* __restrict__ partial, float* __restrict__ pdep,
-                                              const PairRecord& rec, GatherLds& lds) {
-    float4* const s_pos = lds.pos;
-    float4* const s_flux = lds.flux;
-    float4* const s_dir = lds.dir;
-    unsigned char* const s_hit = lds.hit;
-    int* const s_cnt = lds.cnt;
+                                              PairRecord rec) {
+    // LDS of the loop (one wave per block; each kernel that inlines this
+    // function gets its own arrays)
+    __shared__ float4 s_pos[64], s_flux[64], s_dir[64];
+    __shared__ unsigned char s_hit[64 * 64];
+    __shared__ int s_cnt[64];  // hit points of this group gathering each staged photon (guided emission)
     const int lane = threadIdx.x;
     const int hl = lane % GATHER_GROUP;   // the lane's hit point in the group
     const int ph = lane / GATHER_GROUP;   // the lane tests staged photons ph, ph + GATHER_PHASES, ...
@@ -1323,6 +1316,10 @@ __device__ __forceinline__ void gather_chunks(const HitDev& H, const SharedDev& 
     }
 }
 
+// The kernels fit 128 VGPRs, 4 waves per SIMD (one register more drops them
+// to 3 waves and costs 10 % of the gather).  gather_chunks takes its
+// arguments by value: with references the one-scene kernel needed 129.
+constexpr int GATHER_WAVES_PER_SIMD = 4;
 __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, GridDev G,
                                                       const GridParams* __restrict__ gp,
                                                       const uint32_t* __restrict__ order, const int* __restrict__ nvalidp,
@@ -1330,21 +1327,19 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
                                                       int* __restrict__ queue, float4* __restrict__ partial,
                                                       float* __restrict__ pdep, PairRecord rec) {
-    __shared__ GatherLds lds;
-    gather_chunks(H, SH, G, gp, order, nvalidp, tiles, ntiles, chunk_off, chunkp, queue, partial, pdep, rec, lds);
+    gather_chunks(H, SH, G, gp, order, nvalidp, tiles, ntiles, chunk_off, chunkp, queue, partial, pdep, rec);
 }
 
 // The gather chunks of every scene of a batch (see k_photon_multi).
-__global__ void __launch_bounds__(64) k_gather_chunks_multi(const GatherJob* __restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(64, GATHER_WAVES_PER_SIMD) k_gather_chunks_multi(const GatherJob* __restrict__ jobs, int njobs,
                                                             const int* __restrict__ bstart) {
-    __shared__ GatherLds lds;
     const int home = home_scene(bstart, njobs);
     for (int k = 0; k < njobs; ++k) {
         const int s = __builtin_amdgcn_readfirstlane(home + k < njobs ? home + k : home + k - njobs);
         const GatherJob& J = jobs[s];
         if (scene_done(J.queue)) continue;
         gather_chunks(J.H, J.SH, J.G, J.gp, J.order, J.nvalid, J.tiles, J.ntiles, J.chunk_off, J.chunkp, J.queue,
-                      J.partial, J.pdep, J.rec, lds);
+                      J.partial, J.pdep, J.rec);
         scene_mark_done(J.queue);
     }
 }
