@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--guide-target", choices=["mean", "rms"], default="rms",
                     help="guided emission: the proposal's training target (mean or root-mean-square deposit)")
     ap.add_argument("--max-passes", type=int, default=2000)
+    ap.add_argument("--no-graph-leg", action="store_true",
+                    help="leave out the graph-replayed timing of the main workload")
     ap.add_argument("--no-bound", action="store_true",
                     help="time-to-RMSE mode: leave out the emission-proposal bound")
     ap.add_argument("--check-every", type=int, default=16)
@@ -728,6 +730,30 @@ def main():
     a1 = r.counters()
     ms_alone = {k: r.kernel_ms(k) for k in ("photons", "build", "eye", "gather")}
     r.close()
+    # A third leg: the same render with its passes captured as graphs
+    # (sppm_graph_capture) and replayed, with the same warm-up and timed
+    # passes; the graph-replayed passes overlap as the eager ones do.
+    graph_leg = None
+    if not args.no_graph_leg:
+        r = sppm_amd.Renderer(scene, cam, cfg["alpha"], cfg["r0"], cfg["photons"], bvh_width=cfg["bvh"], **pk)
+        r.graph_capture()
+        for _ in range(args.warmup):
+            r.pass_(stats=False)
+        r.sync()
+        tg = time.perf_counter()
+        for _ in range(K):
+            r.pass_(stats=False)
+        r.sync()
+        elg = time.perf_counter() - tg
+        g1 = r.counters()
+        r.close()
+        graph_leg = {"value": round(cfg["photons"] * K / elg, 1), "unit": "photons/s",
+                     "ms_per_step": round(elg / K * 1e3, 3),
+                     "same_counters_as_eager": all(g1[k] == c1[k] for k in ("hit_points", "neighbours", "stored",
+                                                                              "photon_rays", "eye_rays")),
+                     "note": "the same passes captured with sppm_graph_capture and replayed (photon pass beside "
+                             "the previous gather, eye pass beside the photon pass, each step a graph launch)"}
+        log(f"graph-replayed passes: {graph_leg['ms_per_step']} ms per pass")
     a_hits = (a1["hit_points"] - a0["hit_points"]) / K
     a_nbs = (a1["neighbours"] - a0["neighbours"]) / K
     a_gather = a_hits * GATHER_BYTES_PER_HIT + a_nbs * GATHER_BYTES_PER_NEIGHBOUR
@@ -788,6 +814,7 @@ def main():
         "phases_ms": {k: round(v, 3) for k, v in ms.items()},
         "phases_alone_ms": {k: round(v, 3) for k, v in ms_alone.items()},
         "roofline_valu": valu_line,
+        "graph_replay": graph_leg,
         "per_pass": {"hit_points": int(hits), "neighbours": int(nbs), "stored_photons": int(stored),
                      "gather_candidates": int(cands), "photon_rays": int(prays), "eye_rays": int(erays),
                      "hit_point_updates_per_s": round(hits / (ms_alone["gather"] * 1e-3), 1),

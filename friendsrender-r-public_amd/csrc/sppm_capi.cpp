@@ -141,7 +141,8 @@ struct RenderBuffers {
     int* tile_scan_tmp = nullptr;
     float4* partial = nullptr;
     GroupBuffers groups{};
-    int* pass_dev = nullptr;   // passes done (device copy of the pass counter, read by graph replays)
+    int* pass_dev = nullptr;   // passes done (device copy of the pass counter, read by graph replays);
+                               // [1], [2]: the same counted on the photon and eye streams of a replay
     int* path_cell = nullptr;  // guided emission: emission cell per path, 2 x N (block = step parity)
     float* path_ratio = nullptr;  // guided emission: q / p_plain per path, like path_cell
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
@@ -281,8 +282,17 @@ struct sppm_ctx {
     // lands in buffer k % 2 and is overwritten only after eye pass k has run.
     hipEvent_t ev_words[2] = {nullptr, nullptr}, ev_eye[2] = {nullptr, nullptr};
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;  // camera-word branch inside a captured pass
-    hipGraph_t graph = nullptr;                          // captured pass (sppm_graph_capture)
+    // Captured passes (sppm_graph_capture), replayed as the eager passes run:
+    // the photon pass on stream3 beside the previous gather, the eye pass on
+    // stream2 (word buffer b), then there the next pass's camera words (into
+    // buffer 1 - b, beside the gather), build and gather on the main stream.
+    // graph / graph_exec is the gather's graph; graph_passes counts the replays.
+    hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t g_photon = nullptr, g_build = nullptr, g_eye[2] = {nullptr, nullptr}, g_words[2] = {nullptr, nullptr};
+    hipGraphExec_t x_photon = nullptr, x_build = nullptr, x_eye[2] = {nullptr, nullptr},
+                   x_words[2] = {nullptr, nullptr};
+    long long graph_passes = 0;
     // The batch whose graph holds this context's buffers (sppm_batch_add), and
     // whether it has launched since: readbacks and synchronisation wait for
     // the batch's last launch, setup changes are refused while it holds them.
@@ -325,10 +335,17 @@ static int not_in_batch(sppm_ctx* c, const char* what) {
 }
 // A captured pass holds the scene, render and proposal buffers by value.
 static void drop_graph(sppm_ctx* c) {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
+    for (hipGraphExec_t* x :
+         {&c->graph_exec, &c->x_photon, &c->x_build, &c->x_eye[0], &c->x_eye[1], &c->x_words[0], &c->x_words[1]}) {
+        if (*x) (void)hipGraphExecDestroy(*x);
+        *x = nullptr;
+    }
+    for (hipGraph_t* g : {&c->graph, &c->g_photon, &c->g_build, &c->g_eye[0], &c->g_eye[1], &c->g_words[0],
+                          &c->g_words[1]}) {
+        if (*g) (void)hipGraphDestroy(*g);
+        *g = nullptr;
+    }
+    c->graph_passes = 0;
 }
 // FNV-1a over bytes (setup fingerprints of checkpoints).
 static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
@@ -983,7 +1000,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.groups.keys, np)) || (rc = dalloc(&R.groups.vals, np)) || (rc = dalloc(&R.groups.tmp_k, np)) ||
         (rc = dalloc(&R.groups.tmp_v, np)) || (rc = dalloc(&R.groups.hist, radix_hist_entries(np))) ||
         (rc = dalloc(&R.groups.nvalid, 1)) || (rc = dalloc(&R.groups.chunkq, STRIPE_INTS)) || (rc = dalloc(&R.groups.stat, 3 * (size_t)gather_group_count(np))) ||
-        (rc = dalloc(&R.pass_dev, 1)) ||
+        (rc = dalloc(&R.pass_dev, 3)) ||
         (rc = dalloc(&R.image, np))) {
         R.release();
         return rc;
@@ -1345,8 +1362,25 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     const int step = c->pass + 1 + c->params.step_offset;
     hipStream_t st = c->stream;
     if (c->graph_exec) {
+        // the eager pass's streams and waits, each of its steps a graph launch
+        const int b = (int)(c->graph_passes & 1);
+        HIPOK(hipEventRecord(c->ev_gfork, st));
+        HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+        if (c->graph_passes == 0) HIPOK(hipGraphLaunch(c->x_words[0], c->stream2));  // the first pass's words
+        HIPOK(hipGraphLaunch(c->x_eye[b], c->stream2));
+        HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
+        HIPOK(hipGraphLaunch(c->x_words[1 - b], c->stream2));  // the next pass's, beside the gather
+        HIPOK(hipStreamWaitEvent(c->stream3, lead ? c->ev_built : c->ev_gfork, 0));
+        HIPOK(hipGraphLaunch(c->x_photon, c->stream3));
+        HIPOK(hipEventRecord(c->ev_photons, c->stream3));
+        HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
+        HIPOK(hipGraphLaunch(c->x_build, st));
+        HIPOK(hipEventRecord(c->ev_built, st));
+        HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
         HIPOK(hipGraphLaunch(c->graph_exec, st));
+        c->graph_passes += 1;
         c->pass += 1;
+        c->lead_ok = true;
         return stats ? read_pass_stats(c, stats) : SPPM_OK;
     }
     // The eye pass runs on the side stream beside the photon pass and the
@@ -1396,12 +1430,32 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     return stats ? read_pass_stats(c, stats) : SPPM_OK;
 }
 
+// Captures what body() queues on st into *g and instantiates it as *x.
+static int capture_on(hipStream_t st, hipGraph_t* g, hipGraphExec_t* x, const std::function<int()>& body) {
+    HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = body();
+    hipGraph_t gg = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &gg);
+    if (rc) {
+        if (gg) (void)hipGraphDestroy(gg);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (hipGraphInstantiate(x, gg, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(gg);
+        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
+    }
+    *g = gg;
+    return SPPM_OK;
+}
+
 // ---- graph-captured passes -------------------------------------------------
-// One pass as a capturable sequence: the pass's camera words and the eye
-// pass on st2 (forked from st) while the photon pass and the build run on st,
-// then, joined, the gather, guided training and the device pass counter.  Every size and step is read
-// on the device, so a replay continues where the previous one ended; memory
-// is cleared by kernels, not by memset nodes.
+// One pass as a capturable sequence (a batch branch): the pass's camera
+// words and the eye pass on st2 (forked from st) while the photon pass and
+// the build run on st, then, joined, the gather, guided training and the
+// device pass counter.  Every size and step is read on the device, so a
+// replay continues where the previous one ended; memory is cleared by
+// kernels, not by memset nodes.
 static int enqueue_pass_graph(sppm_ctx* c, hipStream_t st, hipStream_t st2) {
     RenderBuffers& R = c->rb;
     HIPOK(hipEventRecord(c->ev_gfork, st));
@@ -1437,23 +1491,52 @@ static int graph_ok(sppm_ctx* c, const sppm_batch* member_of = nullptr) {
 int sppm_graph_capture(sppm_ctx* c) {
     int rc = graph_ok(c);
     if (rc) return rc;
-    HIPOK(hipStreamSynchronize(c->stream));
-    HIPOK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_pass_graph(c, c->stream, c->stream2);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
+    if ((rc = sppm_sync(c))) return rc;
+    RenderBuffers& R = c->rb;
+    // The photon and eye streams count their passes from where the main one
+    // is, one behind: their graphs add one first, so the counter's kernel
+    // runs before the pass kernel (after it, it would wait for free wave
+    // slots behind the gather's waves).
+    const int before = c->pass + c->params.step_offset - 1;  // the main counter reads pass + step_offset
+    HIPOK(hipMemcpy(R.pass_dev + 1, &before, sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.pass_dev + 2, &before, sizeof(int), hipMemcpyHostToDevice));
+    const size_t blk = 6 * (size_t)R.np;
+    if ((rc = capture_on(c->stream3, &c->g_photon, &c->x_photon, [&]() {
+             launch_add_i32(R.pass_dev + 1, 1, c->stream3);
+             photon_pass(c, 0, R.pass_dev + 1, c->stream3, true);
+             return SPPM_OK;
+         })) ||
+        (rc = capture_on(c->stream, &c->g_build, &c->x_build, [&]() {
+             do_build(c, c->stream);
+             return SPPM_OK;
+         })) ||
+        (rc = capture_on(c->stream, &c->graph, &c->graph_exec, [&]() {
+             int r = do_gather(c, c->stream);
+             if (!r && c->guided) do_guide(c, c->stream, 0, R.pass_dev);
+             launch_add_i32(R.pass_dev, 1, c->stream);
+             return r;
+         }))) {
+        drop_graph(c);
         return rc;
     }
-    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    hipGraphExec_t x = nullptr;
-    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = capture_on(c->stream2, &c->g_eye[b], &c->x_eye[b], [&]() {
+            launch_add_i32(R.pass_dev + 2, 1, c->stream2);
+            launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev + 2, trace_spill_eye(c), R.words + b * blk,
+                       R.flags, eye_rays(R), c->scene.next_path + STRIPE_INTS, c->params.eye_nee, c->stream2);
+            return SPPM_OK;
+        });
+        if (!rc)
+            rc = capture_on(c->stream2, &c->g_words[b], &c->x_words[b], [&]() {
+                launch_camera_words(R.cam_states, R.cam_phi, R.cam_K, R.cam_L, R.words + b * blk, c->stream2);
+                return SPPM_OK;
+            });
     }
-    c->graph = g;
-    c->graph_exec = x;
+    if (rc) {
+        drop_graph(c);
+        return rc;
+    }
+    c->graph_passes = 0;
     return SPPM_OK;
 }
 
@@ -1643,23 +1726,6 @@ static void deal_blocks(const std::vector<double>& work, int grid, std::vector<i
 // that gather reads) and its gather for its build, so the photon pass of
 // pass p + 1 runs beside the gather of pass p.  Stream X counts its passes
 // in its own device counters (the steps of the photon launches).
-static int capture_on(hipStream_t st, hipGraph_t* g, hipGraphExec_t* x, const std::function<int()>& body) {
-    HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = body();
-    hipGraph_t gg = nullptr;
-    const hipError_t e = hipStreamEndCapture(st, &gg);
-    if (rc) {
-        if (gg) (void)hipGraphDestroy(gg);
-        return rc;
-    }
-    if (e != hipSuccess) return fail(SPPM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    if (hipGraphInstantiate(x, gg, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(gg);
-        return fail(SPPM_E_HIP, "hipGraphInstantiate failed");
-    }
-    *g = gg;
-    return SPPM_OK;
-}
 static int hip_step(hipError_t e, const char* what) {
     return e == hipSuccess ? SPPM_OK : fail(SPPM_E_HIP, "batch: %s: %s", what, hipGetErrorString(e));
 }

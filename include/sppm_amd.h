@@ -416,9 +416,11 @@ int sppm_guide_path_cells(sppm_ctx* ctx, int32_t* cells);
 /* ---- graph-captured passes, batches of scenes, autotuner --------------------
  * No counterpart in the reference (it renders one scene per process,
  * main.cpp:117-167).  A captured pass replays every kernel of sppm_pass from
- * one hipGraph launch; a batch captures one pass of several contexts (each
- * set up with its own scene, camera and params and begun) into one graph whose
- * branches run concurrently.  Capture right after sppm_begin; needs the
+ * graphs of its steps (photon pass, eye pass, camera words, build, gather),
+ * launched on the same streams with the same events as an eager pass, so a
+ * replay's photon pass runs beside the previous gather; a batch captures one
+ * pass of several contexts (each set up with its own scene, camera and params
+ * and begun) into one graph whose branches run concurrently.  Capture right after sppm_begin; needs the
  * UniformSample camera stream and record_neighbours 0.  After capture,
  * sppm_pass replays the graph and the split pass steps are refused;
  * sppm_begin drops the graph. */

@@ -42,6 +42,28 @@ def test_graph_replay_matches_eager(box_scene):
     graph.close()
 
 
+def test_graph_replay_back_to_back_matches_eager(full_scene_small):
+    """Replays queued with no synchronisation between them: each photon pass
+    runs beside the previous gather and each eye pass reads the camera words
+    the previous eye graph made, alternating between the two word buffers.
+    A step offset moves every stream's step counter alike."""
+    cam = scenes.thinlens_camera(48, 32, aperture=0.02, focus=2.0)
+    eager = sppm_amd.Renderer(full_scene_small, cam, 0.7, 0.03, 30000, bvh_width=8, step_offset=5)
+    graph = sppm_amd.Renderer(full_scene_small, cam, 0.7, 0.03, 30000, bvh_width=8, step_offset=5)
+    graph.graph_capture()
+    for _ in range(7):
+        eager.pass_(stats=False)
+        graph.pass_(stats=False)
+    eager.sync()
+    graph.sync()
+    assert_same_state(eager, graph)
+    ce, cg = eager.counters(), graph.counters()
+    for k in ("hit_points", "neighbours", "stored", "photon_rays", "eye_rays"):
+        assert ce[k] == cg[k], k
+    eager.close()
+    graph.close()
+
+
 @pytest.mark.parametrize("mode", [sppm_amd.BATCH_MULTI, sppm_amd.BATCH_BRANCHES], ids=["multi", "branches"])
 def test_batch_matches_individual_renders(box_scene, full_scene_small, mode):
     occl = scenes.occluded_scene(clutter_freq=6)
