@@ -117,6 +117,7 @@ struct RenderBuffers {
     int* cell_ngap = nullptr;
     int* cell_start = nullptr;
     PhotonSortBuffers psort{};  // stable sort of the flattened photons by cell
+    StagedPhotons staged{};     // the pass's flattened photons, out of the slots
     float4 *gpos = nullptr, *gflux = nullptr, *gdir = nullptr;
     int photon_cap = 0;
     GridParams* gp = nullptr;
@@ -172,7 +173,8 @@ struct RenderBuffers {
         pairs_cap = 0;
         dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
         dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_gaps); dfree(cell_ngap); dfree(cell_start);
-        dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.src); dfree(psort.hist);
+        dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.hist);
+        dfree(staged.pos); dfree(staged.flux); dfree(staged.dir);
         dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
@@ -270,11 +272,16 @@ struct sppm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;
-    // The photon pass of a pass may start on stream3 as soon as the previous
-    // pass's build has read the photon slots (ev_built), beside that pass's
-    // gather; lead_ok says that nothing but sppm_pass has queued work since.
+    // Passes in a row run their photon passes on stream3: a plain pass's
+    // photon pass follows the previous pass's staging (ev_staged: its photons
+    // copied out of the slots), and then stages its own photons there once the
+    // previous build has read the last staged ones (ev_built); a guided
+    // pass's photon pass waits for the previous build (which publishes the
+    // proposal it samples) and is staged by its own build.  Either way it runs
+    // beside the previous pass's gather.  lead_ok says that nothing but
+    // sppm_pass has queued work since.
     hipStream_t stream3 = nullptr;
-    hipEvent_t ev_built = nullptr, ev_photons = nullptr;
+    hipEvent_t ev_built = nullptr, ev_photons = nullptr, ev_staged = nullptr;
     bool lead_ok = false;
     bool own_stream = false;
     // Camera sample words (UniformSample stream) are generated on stream2 one
@@ -289,8 +296,11 @@ struct sppm_ctx {
     // graph / graph_exec is the gather's graph; graph_passes counts the replays.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t g_photon = nullptr, g_build = nullptr, g_eye[2] = {nullptr, nullptr}, g_words[2] = {nullptr, nullptr};
-    hipGraphExec_t x_photon = nullptr, x_build = nullptr, x_eye[2] = {nullptr, nullptr},
+    // Plain passes stage their photons in a graph of their own (g_stage /
+    // x_stage) on stream3 after the photon pass; guided ones in the build's.
+    hipGraph_t g_photon = nullptr, g_stage = nullptr, g_build = nullptr, g_eye[2] = {nullptr, nullptr},
+               g_words[2] = {nullptr, nullptr};
+    hipGraphExec_t x_photon = nullptr, x_stage = nullptr, x_build = nullptr, x_eye[2] = {nullptr, nullptr},
                    x_words[2] = {nullptr, nullptr};
     long long graph_passes = 0;
     // The batch whose graph holds this context's buffers (sppm_batch_add), and
@@ -336,12 +346,13 @@ static int not_in_batch(sppm_ctx* c, const char* what) {
 // A captured pass holds the scene, render and proposal buffers by value.
 static void drop_graph(sppm_ctx* c) {
     for (hipGraphExec_t* x :
-         {&c->graph_exec, &c->x_photon, &c->x_build, &c->x_eye[0], &c->x_eye[1], &c->x_words[0], &c->x_words[1]}) {
+         {&c->graph_exec, &c->x_photon, &c->x_stage, &c->x_build, &c->x_eye[0], &c->x_eye[1], &c->x_words[0],
+          &c->x_words[1]}) {
         if (*x) (void)hipGraphExecDestroy(*x);
         *x = nullptr;
     }
-    for (hipGraph_t* g : {&c->graph, &c->g_photon, &c->g_build, &c->g_eye[0], &c->g_eye[1], &c->g_words[0],
-                          &c->g_words[1]}) {
+    for (hipGraph_t* g : {&c->graph, &c->g_photon, &c->g_stage, &c->g_build, &c->g_eye[0], &c->g_eye[1],
+                          &c->g_words[0], &c->g_words[1]}) {
         if (*g) (void)hipGraphDestroy(*g);
         *g = nullptr;
     }
@@ -439,6 +450,7 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
     HIPOK(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
     HIPOK(hipEventCreateWithFlags(&c->ev_built, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_photons, hipEventDisableTiming));
+    HIPOK(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
     for (int b = 0; b < 2; ++b) {
         HIPOK(hipEventCreateWithFlags(&c->ev_words[b], hipEventDisableTiming));
         HIPOK(hipEventCreateWithFlags(&c->ev_eye[b], hipEventDisableTiming));
@@ -487,6 +499,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipEventDestroy(c->ev_gjoin);
     (void)hipEventDestroy(c->ev_built);
     (void)hipEventDestroy(c->ev_photons);
+    (void)hipEventDestroy(c->ev_staged);
     drop_graph(c);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream3);
@@ -980,7 +993,8 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.cell_gaps, cell_gap_entries(R.cell_cap))) || (rc = dalloc(&R.cell_ngap, 1)) ||
         (rc = dalloc(&R.cell_start, R.cell_cap + 1)) || (rc = dalloc(&R.psort.keys, R.photon_cap)) ||
         (rc = dalloc(&R.psort.vals, R.photon_cap)) || (rc = dalloc(&R.psort.tmp_k, R.photon_cap)) ||
-        (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.psort.src, R.photon_cap)) ||
+        (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.staged.pos, R.photon_cap)) ||
+        (rc = dalloc(&R.staged.flux, R.photon_cap)) || (rc = dalloc(&R.staged.dir, R.photon_cap)) ||
         (rc = dalloc(&R.psort.hist, radix_hist_entries(R.photon_cap))) ||
         (rc = dalloc(&R.gpos, R.photon_cap)) || (rc = dalloc(&R.gflux, R.photon_cap)) ||
         (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
@@ -1197,14 +1211,25 @@ static void do_guide(sppm_ctx* c, hipStream_t st, int step, const int* step_dev)
     }
     guide_infer(c->guide, st);
 }
-static void do_build(sppm_ctx* c, hipStream_t st) {
+// The build's first step: the pass's flattened photons out of the slots
+// (after it the next photon pass may rewrite them).
+static void do_stage(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
     // the photon pass that read the emission tables is done: publish the
     // proposal trained last for the next photon pass
     if (c->guided) guide_publish(c->guide, st);
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
-    launch_build_grid(R.slots, R.off, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
+    launch_photon_stage(R.slots, R.off, R.staged, st);
+}
+// The rest of the build: grid, keys, sort and the cell-sorted records.
+static void do_build_grid(sppm_ctx* c, hipStream_t st) {
+    RenderBuffers& R = c->rb;
+    launch_build_grid(R.staged, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_start, R.cell_gaps, R.cell_ngap, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
+}
+static void do_build(sppm_ctx* c, hipStream_t st) {
+    do_stage(c, st);
+    do_build_grid(c, st);
 }
 // Generates camera word blocks up to and including block k on stream2, in
 // stream order (each block continues the shared mt19937 stream).
@@ -1370,8 +1395,14 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         HIPOK(hipGraphLaunch(c->x_eye[b], c->stream2));
         HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
         HIPOK(hipGraphLaunch(c->x_words[1 - b], c->stream2));  // the next pass's, beside the gather
-        HIPOK(hipStreamWaitEvent(c->stream3, lead ? c->ev_built : c->ev_gfork, 0));
+        const bool early = c->x_stage != nullptr;
+        HIPOK(hipStreamWaitEvent(c->stream3, !lead ? c->ev_gfork : early ? c->ev_staged : c->ev_built, 0));
         HIPOK(hipGraphLaunch(c->x_photon, c->stream3));
+        if (early) {
+            if (lead) HIPOK(hipStreamWaitEvent(c->stream3, c->ev_built, 0));
+            HIPOK(hipGraphLaunch(c->x_stage, c->stream3));
+            HIPOK(hipEventRecord(c->ev_staged, c->stream3));
+        }
         HIPOK(hipEventRecord(c->ev_photons, c->stream3));
         HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
         HIPOK(hipGraphLaunch(c->x_build, st));
@@ -1395,22 +1426,34 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
     rc = words_generate_upto(c, c->words_used);
     if (rc) return rc;
-    // The photon pass rewrites the photon slots, which the previous pass's
-    // build read; its gather reads only the grid built from them.
+    // The photon pass rewrites the photon slots, which the previous pass
+    // staged (plain) or built from (guided); the previous gather reads only
+    // the grid built from them.
+    const bool early = lead && !c->guided;
     hipStream_t sp = st;
     if (lead) {
         sp = c->stream3;
-        HIPOK(hipStreamWaitEvent(sp, c->ev_built, 0));
+        HIPOK(hipStreamWaitEvent(sp, c->guided ? c->ev_built : c->ev_staged, 0));
     }
     c->t_photons.start(sp);
     do_photons(c, step, sp, lead);
     c->t_photons.stop(sp);
+    if (early) {
+        // the previous build has read the staged photons and their offsets
+        HIPOK(hipStreamWaitEvent(sp, c->ev_built, 0));
+        do_stage(c, sp);
+        HIPOK(hipEventRecord(c->ev_staged, sp));
+    }
     if (lead) {
         HIPOK(hipEventRecord(c->ev_photons, sp));
         HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
     }
     c->t_build.start(st);
-    do_build(c, st);
+    if (!early) {
+        do_stage(c, st);
+        HIPOK(hipEventRecord(c->ev_staged, st));
+    }
+    do_build_grid(c, st);
     c->t_build.stop(st);
     HIPOK(hipEventRecord(c->ev_built, st));
     HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
@@ -1506,8 +1549,13 @@ int sppm_graph_capture(sppm_ctx* c) {
              photon_pass(c, 0, R.pass_dev + 1, c->stream3, true);
              return SPPM_OK;
          })) ||
+        (!c->guided && (rc = capture_on(c->stream3, &c->g_stage, &c->x_stage, [&]() {
+             do_stage(c, c->stream3);
+             return SPPM_OK;
+         }))) ||
         (rc = capture_on(c->stream, &c->g_build, &c->x_build, [&]() {
-             do_build(c, c->stream);
+             if (c->guided) do_stage(c, c->stream);
+             do_build_grid(c, c->stream);
              return SPPM_OK;
          })) ||
         (rc = capture_on(c->stream, &c->graph, &c->graph_exec, [&]() {

@@ -20,6 +20,9 @@ constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t EMPTY_CHILD = 0xffffffffu;
 constexpr int LEAF_MAX = 4;  // triangles per leaf range
 constexpr int TRACE_LDS = 16;  // traversal stack entries per lane kept in LDS (deeper ones spill)
+// LDS words of a 64-lane block's traversal stack: the node references, then
+// their boxes' entry distances
+constexpr int TRACE_STACK_WORDS = 2 * TRACE_LDS * 64;
 __host__ __device__ constexpr int node_floats(int W) { return ((7 * W + 3) / 4) * 4; }
 
 // Leaf triangle t (sorted order): 3 float4 = (v0.xyz, facet id bits),

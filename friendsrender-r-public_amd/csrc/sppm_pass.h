@@ -88,16 +88,25 @@ void launch_occluded(const SceneDev& S, int n, const float* org, const float* di
                      float transl,
                      const TraceSpill& sp, int* occluded, uint32_t* flags, hipStream_t st);
 // Sort buffers of the grid build, photon_cap entries each (hist:
-// radix_hist_entries(photon_cap)): cell key, flattened index and slot
-// address of every flattened photon.
+// radix_hist_entries(photon_cap)): cell key and flattened index of every
+// flattened photon.
 struct PhotonSortBuffers {
-    uint32_t *keys, *vals, *tmp_k, *tmp_v, *src, *hist;
+    uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
 };
+// The flattened photons of a pass in reference order, photon_cap entries
+// each (dir.w: the flattened index), copied out of the slots by the build's
+// first step so that the next photon pass may run while the rest of the
+// build reads them.
+struct StagedPhotons {
+    float4 *pos, *flux, *dir;
+};
+// off: exclusive scan of the slots' per-path counts
+void launch_photon_stage(const SlotsDev& slots, const int* off, const StagedPhotons& sp, hipStream_t st);
 constexpr int PHOTON_KEY_BITS = 22;  // cell keys are below the cell capacity 2^22
 // Photon grid: cell size, keys, stable sort by cell, cell starts
 // (cell_cap + 1 entries; gaps: cell_gap_entries(cell_cap); ngap: 1 int) and
 // the cell-sorted records.
-void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
+void launch_build_grid(const StagedPhotons& sp, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap,
                        int* cell_start, int4* gaps, int* ngap, const PhotonSortBuffers& sb, int photon_cap,
                        float4* gpos, float4* gflux, float4* gdir, hipStream_t st);

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon(Scen
                                                        unsigned long long* __restrict__ rays) {
     // the pass step comes from the device pass counter when the pass is replayed from a graph
     const int step = step_dev ? *step_dev + 1 : step_arg;
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t ovf = 0, fl = 0, nray = 0, ntie = 0;
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_photon_mult
                                                                                 int njobs,
                                                                                 const int* __restrict__ bstart, int cap,
                                                                                 uint32_t* __restrict__ spill) {
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int home = home_scene(bstart, njobs);
@@ -326,26 +326,38 @@ __global__ void k_zero_i32(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Per path: the cell key, flattened index and slot address of each of its
-// flattened photons.
-__global__ void __launch_bounds__(256) k_photon_keys(SlotsDev slots, const int* __restrict__ off,
-                                                     const GridParams* __restrict__ gp, PhotonSortBuffers sb) {
+// The pass's flattened photons (SPPM.cpp:812-822) copied out of the slots in
+// reference order, each record's flattened index in dir.w: the rest of the
+// build reads only these, so the next photon pass may rewrite the slots.
+__global__ void __launch_bounds__(256) k_photon_stage(SlotsDev slots, const int* __restrict__ off,
+                                                      StagedPhotons sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots.npaths) return;
-    const GridParams g = *gp;
     const int c = slots.count[i];
     const int base = off[i];
     const size_t NP = (size_t)slots.npaths;
     for (int s = 0; s < c; ++s) {
         const size_t o = (size_t)s * NP + i;
-        float4 p = slots.pos[o];
+        float4 d = slots.dir[o];
+        d.w = __int_as_float(base + s);
+        sp.pos[base + s] = slots.pos[o];
+        sp.flux[base + s] = slots.flux[o];
+        sp.dir[base + s] = d;
+    }
+}
+
+// Cell key and flattened index of every staged photon.
+__global__ void __launch_bounds__(256) k_photon_keys(const float4* __restrict__ pos, const int* __restrict__ nstored,
+                                                     const GridParams* __restrict__ gp, PhotonSortBuffers sb) {
+    const int n = *nstored;
+    const GridParams g = *gp;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const float4 p = pos[j];
         int cx = cell_coord(p.x, g.bmin[0], g.inv_cell, g.nx);
         int cy = cell_coord(p.y, g.bmin[1], g.inv_cell, g.ny);
         int cz = cell_coord(p.z, g.bmin[2], g.inv_cell, g.nz);
-        int key = (cz * g.ny + cy) * g.nx + cx;
-        sb.keys[base + s] = (uint32_t)key;
-        sb.vals[base + s] = (uint32_t)(base + s);
-        sb.src[base + s] = (uint32_t)o;
+        sb.keys[j] = (uint32_t)((cz * g.ny + cy) * g.nx + cx);
+        sb.vals[j] = (uint32_t)j;
     }
 }
 
@@ -387,19 +399,16 @@ size_t cell_gap_entries(int cell_cap) { return (size_t)(cell_cap + 1) / CELL_RUN
 // holds flattened photon vals[j], so photons of a cell are in flattened
 // (reference) order and every run lays them out the same way.  The record
 // carries its flattened index in dir.w.
-__global__ void __launch_bounds__(256) k_photon_sorted(SlotsDev slots, const int* __restrict__ nstored,
-                                                       const uint32_t* __restrict__ order, PhotonSortBuffers sb,
+__global__ void __launch_bounds__(256) k_photon_sorted(StagedPhotons sp, const int* __restrict__ nstored,
+                                                       const uint32_t* __restrict__ order,
                                                        float4* __restrict__ gpos, float4* __restrict__ gflux,
                                                        float4* __restrict__ gdir) {
     const int n = *nstored;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t flat = order[j];
-        const size_t o = sb.src[flat];
-        float4 d = slots.dir[o];
-        d.w = __int_as_float((int)flat);
-        gpos[j] = slots.pos[o];
-        gflux[j] = slots.flux[o];
-        gdir[j] = d;
+        gpos[j] = sp.pos[flat];
+        gflux[j] = sp.flux[flat];
+        gdir[j] = sp.dir[flat];
     }
 }
 
@@ -734,7 +743,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye(SceneDe
                                                     uint32_t* __restrict__ flags, int keep,
                                                     unsigned long long* __restrict__ rays, int* __restrict__ queue) {
     const int step = step_dev ? *step_dev + 1 : step_arg;
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t fl = 0, nray = 0, ntie = 0;
@@ -749,7 +758,7 @@ template <int W, bool NEE>
 __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_eye_multi(const EyeJob* __restrict__ jobs, int njobs,
                                                                              const int* __restrict__ bstart, int cap,
                                                                              uint32_t* __restrict__ spill) {
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const int home = home_scene(bstart, njobs);
@@ -1496,7 +1505,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_trace(SceneDev S, int n, const f
                                                       uint32_t* __restrict__ spill, int* __restrict__ facet,
                                                       float* __restrict__ t, float* __restrict__ u,
                                                       float* __restrict__ v, uint32_t* __restrict__ flags) {
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t fl = 0;
@@ -1521,7 +1530,7 @@ __global__ void __launch_bounds__(PATH_BLOCK) k_occluded(SceneDev S, int n, cons
                                                          const float* __restrict__ tmax, float transl, int cap,
                                                          uint32_t* __restrict__ spill,
                                                          int* __restrict__ occluded, uint32_t* __restrict__ flags) {
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     uint32_t fl = 0;
@@ -1636,22 +1645,24 @@ void launch_occluded(const SceneDev& S, int n, const float* org, const float* di
         hipLaunchKernelGGL(k_occluded<8>, g, b, 0, st, S, n, org, dir, tmin, tmax, transl, sp.cap, sp.buf, occluded, flags);
 }
 
-void launch_build_grid(const SlotsDev& slots, const int* off, const int* nstored, Counters* ctr, GridParams* gp,
+void launch_photon_stage(const SlotsDev& slots, const int* off, const StagedPhotons& sp, hipStream_t st) {
+    hipLaunchKernelGGL(k_photon_stage, dim3((slots.npaths + 255) / 256), dim3(256), 0, st, slots, off, sp);
+}
+
+void launch_build_grid(const StagedPhotons& sp, const int* nstored, Counters* ctr, GridParams* gp,
                        const unsigned long long* rstat, const float* bmin, const float* ext, int cell_cap,
                        int* cell_start, int4* gaps, int* ngap, const PhotonSortBuffers& sb, int photon_cap,
                        float4* gpos, float4* gflux, float4* gdir, hipStream_t st) {
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rstat, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], cell_cap, nstored, ctr, ngap);
-    int gP = (slots.npaths + 255) / 256;
-    hipLaunchKernelGGL(k_photon_keys, dim3(gP), dim3(256), 0, st, slots, off, gp, sb);
+    hipLaunchKernelGGL(k_photon_keys, dim3(2048), dim3(256), 0, st, sp.pos, nstored, gp, sb);
     // stable sort of the flattened photons by cell: cell keys are below
     // cell_cap = 2^22, the flattened order is kept inside a cell
     const SortResult sr =
         radix_sort_pairs(sb.keys, sb.vals, sb.tmp_k, sb.tmp_v, sb.hist, photon_cap, PHOTON_KEY_BITS, st, nstored);
     hipLaunchKernelGGL(k_cell_starts, dim3(2048), dim3(256), 0, st, sr.keys, nstored, gp, cell_start, gaps, ngap);
     hipLaunchKernelGGL(k_fill_gaps, dim3(1024), dim3(256), 0, st, gaps, ngap, cell_start);
-    hipLaunchKernelGGL(k_photon_sorted, dim3(2048), dim3(256), 0, st, slots, nstored, sr.vals, sb, gpos, gflux,
-                       gdir);
+    hipLaunchKernelGGL(k_photon_sorted, dim3(2048), dim3(256), 0, st, sp, nstored, sr.vals, gpos, gflux, gdir);
 }
 
 void launch_photon_flat(const SlotsDev& slots, const int* off, float4* fpos, float4* fflux, float4* fdir,

@@ -13,10 +13,12 @@
 //
 // Traversal descends into the nearest intersected child and pushes the
 // others far-to-near.  The first TRACE_LDS stack entries of a lane live in
-// LDS (entry s of lane t at lds[s * 64 + t], bank-conflict free); deeper
-// entries spill to a global area (entry s at glob[(s - TRACE_LDS) * gstride]),
-// so the LDS footprint, and with it the occupancy, does not depend on the
-// tree depth.  The host sizes the total capacity from the measured depth.
+// LDS (entry s of lane t at lds[s * 64 + t], bank-conflict free, with its
+// box's entry distance at lds[(TRACE_LDS + s) * 64 + t], so a pop skips
+// boxes that lie beyond the best hit found since the push); deeper entries
+// spill to a global area (entry s at glob[(s - TRACE_LDS) * gstride]), so the
+// LDS footprint, and with it the occupancy, does not depend on the tree
+// depth.  The host sizes the total capacity from the measured depth.
 #pragma once
 #include "sppm_device.h"
 
@@ -91,6 +93,22 @@ __device__ __forceinline__ void sort_children(float* d, uint32_t* r) {
     }
 }
 
+// Pops the next stack entry whose box may still hold an accepted hit: an
+// LDS entry keeps its box's entry distance and is skipped once that is
+// beyond best (the rule it was pushed under; a hit inside the box lies at or
+// beyond it); spilled entries are always taken.  EMPTY_CHILD when none is left.
+__device__ __forceinline__ uint32_t trace_pop(const TraceStack& ts, int& sp, float best) {
+    while (sp > 0) {
+        --sp;
+        if (sp >= ts.cap) return EMPTY_CHILD;  // overflowed stack: stop (flagged)
+        if (sp < TRACE_LDS) {
+            if (__uint_as_float(ts.lds[(TRACE_LDS + sp) * 64]) > best) continue;
+            return ts.lds[sp * 64];
+        }
+        return ts.glob[(size_t)(sp - TRACE_LDS) * ts.gstride];
+    }
+    return EMPTY_CHILD;
+}
 // Closest hit for a ray (origin, direction as stored in the Ray, MinT = tmin,
 // MaxT = FLT_MAX).  Returns false on a miss.  *flags bit 2 is set if the
 // stack capacity was exceeded (the host sizes it from the tree depth, so
@@ -152,7 +170,10 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
 #pragma unroll
                 for (int c = W - 1; c >= 1; --c) {
                     if (r[c] != EMPTY_CHILD) {
-                        if (sp < TRACE_LDS) ts.lds[sp * 64] = r[c];
+                        if (sp < TRACE_LDS) {
+                            ts.lds[sp * 64] = r[c];
+                            ts.lds[(TRACE_LDS + sp) * 64] = __float_as_uint(d[c]);
+                        }
                         else if (sp < ts.cap) ts.glob[(size_t)(sp - TRACE_LDS) * ts.gstride] = r[c];
                         else *flags |= 4u;
                         ++sp;
@@ -162,10 +183,8 @@ __device__ bool closest_hit(const SceneDev& S, f3 org, f3 dir, float tmin, const
                 continue;
             }
         }
-        if (sp == 0) break;
-        --sp;
-        if (sp >= ts.cap) break;  // overflowed stack: stop (flagged)
-        ref = sp < TRACE_LDS ? ts.lds[sp * 64] : ts.glob[(size_t)(sp - TRACE_LDS) * ts.gstride];
+        ref = trace_pop(ts, sp, best);
+        if (ref == EMPTY_CHILD) break;
     }
     if (bslot < 0 || !(best < 3.40282347e38f)) {
         h.facet = -1;

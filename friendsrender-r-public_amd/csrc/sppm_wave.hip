@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(PATH_BLOCK, PATH_WAVES_PER_SIMD) k_pw_resume(
     SceneDev S, SlotsDev slots, PathStateDev Q, const float4* __restrict__ rays, const int* __restrict__ qcnt,
     int* __restrict__ fetch, int Pq, int cap, uint32_t* __restrict__ spill, uint32_t* __restrict__ flags,
     unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t stk[TRACE_LDS * PATH_BLOCK];
+    __shared__ uint32_t stk[TRACE_STACK_WORDS];
     const int lane = threadIdx.x;
     const TraceStack ts{stk + lane, spill + (size_t)blockIdx.x * PATH_BLOCK + lane, (int)gridDim.x * PATH_BLOCK, cap};
     const unsigned long long below = (1ull << lane) - 1ull;
