@@ -121,7 +121,10 @@ struct RenderBuffers {
     float4 *gpos = nullptr, *gflux = nullptr, *gdir = nullptr;
     int photon_cap = 0;
     GridParams* gp = nullptr;
-    HitDev hits{};
+    HitDev hits{};       // the hit points of the current pass: hitbuf[pass % 2]
+    // Pass k's eye pass writes buffer k % 2, so in passes in a row it waits
+    // only for the gather two passes back, not for the previous one.
+    HitDev hitbuf[2]{};
     SharedDev shared{};
     int* counts = nullptr;
     int* list_off = nullptr;
@@ -176,7 +179,10 @@ struct RenderBuffers {
         dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.hist);
         dfree(staged.pos); dfree(staged.flux); dfree(staged.dir);
         dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
-        dfree(hits.pos); dfree(hits.nrm); dfree(hits.out); dfree(hits.weight); dfree(hits.mat); dfree(hits.absorb); dfree(hits.direct);
+        for (HitDev& h : hitbuf) {
+            dfree(h.pos); dfree(h.nrm); dfree(h.out); dfree(h.weight); dfree(h.mat); dfree(h.absorb); dfree(h.direct);
+        }
+        hits = HitDev{};
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
         dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(path_ratio); dfree(pdep); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
         ready = false;
@@ -282,6 +288,7 @@ struct sppm_ctx {
     // sppm_pass has queued work since.
     hipStream_t stream3 = nullptr;
     hipEvent_t ev_built = nullptr, ev_photons = nullptr, ev_staged = nullptr;
+    hipEvent_t ev_gathered[2] = {nullptr, nullptr};  // the gather of the last pass of each parity (its hit points)
     bool lead_ok = false;
     bool own_stream = false;
     // Camera sample words (UniformSample stream) are generated on stream2 one
@@ -291,11 +298,13 @@ struct sppm_ctx {
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;  // camera-word branch inside a captured pass
     // Captured passes (sppm_graph_capture), replayed as the eager passes run:
     // the photon pass on stream3 beside the previous gather, the eye pass on
-    // stream2 (word buffer b), then there the next pass's camera words (into
-    // buffer 1 - b, beside the gather), build and gather on the main stream.
-    // graph / graph_exec is the gather's graph; graph_passes counts the replays.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // stream2 (word and hit-point buffer b = pass % 2), then there the next
+    // pass's camera words (into buffer 1 - b, beside the gather), build and
+    // gather on the main stream.  graph / graph_exec is the gather of even
+    // passes (hit-point buffer 0), graph_odd / graph_exec_odd of odd ones;
+    // graph_passes counts the replays.
+    hipGraph_t graph = nullptr, graph_odd = nullptr;
+    hipGraphExec_t graph_exec = nullptr, graph_exec_odd = nullptr;
     // Plain passes stage their photons in a graph of their own (g_stage /
     // x_stage) on stream3 after the photon pass; guided ones in the build's.
     hipGraph_t g_photon = nullptr, g_stage = nullptr, g_build = nullptr, g_eye[2] = {nullptr, nullptr},
@@ -346,12 +355,12 @@ static int not_in_batch(sppm_ctx* c, const char* what) {
 // A captured pass holds the scene, render and proposal buffers by value.
 static void drop_graph(sppm_ctx* c) {
     for (hipGraphExec_t* x :
-         {&c->graph_exec, &c->x_photon, &c->x_stage, &c->x_build, &c->x_eye[0], &c->x_eye[1], &c->x_words[0],
-          &c->x_words[1]}) {
+         {&c->graph_exec, &c->graph_exec_odd, &c->x_photon, &c->x_stage, &c->x_build, &c->x_eye[0], &c->x_eye[1],
+          &c->x_words[0], &c->x_words[1]}) {
         if (*x) (void)hipGraphExecDestroy(*x);
         *x = nullptr;
     }
-    for (hipGraph_t* g : {&c->graph, &c->g_photon, &c->g_stage, &c->g_build, &c->g_eye[0], &c->g_eye[1],
+    for (hipGraph_t* g : {&c->graph, &c->graph_odd, &c->g_photon, &c->g_stage, &c->g_build, &c->g_eye[0], &c->g_eye[1],
                           &c->g_words[0], &c->g_words[1]}) {
         if (*g) (void)hipGraphDestroy(*g);
         *g = nullptr;
@@ -451,6 +460,10 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
     HIPOK(hipEventCreateWithFlags(&c->ev_built, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_photons, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
+    for (hipEvent_t& e : c->ev_gathered) {
+        HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPOK(hipEventRecord(e, c->stream));
+    }
     for (int b = 0; b < 2; ++b) {
         HIPOK(hipEventCreateWithFlags(&c->ev_words[b], hipEventDisableTiming));
         HIPOK(hipEventCreateWithFlags(&c->ev_eye[b], hipEventDisableTiming));
@@ -500,6 +513,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipEventDestroy(c->ev_built);
     (void)hipEventDestroy(c->ev_photons);
     (void)hipEventDestroy(c->ev_staged);
+    for (hipEvent_t e : c->ev_gathered) (void)hipEventDestroy(e);
     drop_graph(c);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream3);
@@ -997,11 +1011,9 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.staged.flux, R.photon_cap)) || (rc = dalloc(&R.staged.dir, R.photon_cap)) ||
         (rc = dalloc(&R.psort.hist, radix_hist_entries(R.photon_cap))) ||
         (rc = dalloc(&R.gpos, R.photon_cap)) || (rc = dalloc(&R.gflux, R.photon_cap)) ||
-        (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.hits.pos, np)) ||
-        (rc = dalloc(&R.hits.nrm, np)) || (rc = dalloc(&R.hits.out, np)) || (rc = dalloc(&R.hits.weight, np)) ||
-        (rc = dalloc(&R.hits.mat, np)) || (rc = dalloc(&R.hits.absorb, np)) || (rc = dalloc(&R.shared.rad, np)) ||
+        (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.shared.rad, np)) ||
         (rc = dalloc(&R.shared.cnt, np)) || (rc = dalloc(&R.shared.direct, np)) ||
-        (rc = dalloc(&R.hits.direct, np)) || (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
+        (rc = dalloc(&R.counts, np + 1)) || (rc = dalloc(&R.list_off, np + 1)) ||
         (rc = dalloc(&R.list_scan_tmp, scan_tmp_entries(np + 1))) ||
         (rc = dalloc(&R.words, 2 * 6 * (size_t)np)) || (rc = dalloc(&R.cam_phi, MT_WORDS)) ||
         (rc = dalloc(&R.rstat, 2)) || (rc = dalloc(&R.flags, 1)) || (rc = dalloc(&R.ctr, 1)) ||
@@ -1019,7 +1031,16 @@ int sppm_begin(sppm_ctx* c) {
         R.release();
         return rc;
     }
-    R.hits.np = np;
+    for (HitDev& h : R.hitbuf) {
+        if ((rc = dalloc(&h.pos, np)) || (rc = dalloc(&h.nrm, np)) || (rc = dalloc(&h.out, np)) ||
+            (rc = dalloc(&h.weight, np)) || (rc = dalloc(&h.mat, np)) || (rc = dalloc(&h.absorb, np)) ||
+            (rc = dalloc(&h.direct, np))) {
+            R.release();
+            return rc;
+        }
+        h.np = np;
+    }
+    R.hits = R.hitbuf[0];
     if (c->params.photon_kernel == SPPM_PHOTON_WAVEFRONT) {
         const int B = c->params.wave_bounces > 0 ? c->params.wave_bounces : WAVE_BOUNCES_DEFAULT;
         const int width = c->scene.dev.width;
@@ -1389,8 +1410,13 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     if (c->graph_exec) {
         // the eager pass's streams and waits, each of its steps a graph launch
         const int b = (int)(c->graph_passes & 1);
-        HIPOK(hipEventRecord(c->ev_gfork, st));
-        HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+        c->rb.hits = c->rb.hitbuf[b];
+        if (lead) {
+            HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gathered[b], 0));
+        } else {
+            HIPOK(hipEventRecord(c->ev_gfork, st));
+            HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+        }
         if (c->graph_passes == 0) HIPOK(hipGraphLaunch(c->x_words[0], c->stream2));  // the first pass's words
         HIPOK(hipGraphLaunch(c->x_eye[b], c->stream2));
         HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
@@ -1408,7 +1434,8 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         HIPOK(hipGraphLaunch(c->x_build, st));
         HIPOK(hipEventRecord(c->ev_built, st));
         HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
-        HIPOK(hipGraphLaunch(c->graph_exec, st));
+        HIPOK(hipGraphLaunch(b ? c->graph_exec_odd : c->graph_exec, st));
+        HIPOK(hipEventRecord(c->ev_gathered[b], st));
         c->graph_passes += 1;
         c->pass += 1;
         c->lead_ok = true;
@@ -1416,11 +1443,19 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     }
     // The eye pass runs on the side stream beside the photon pass and the
     // grid build (createHitpoints runs beside the kd-tree build in the
-    // reference too, SPPM.cpp:809-838): it waits for what the main stream has
-    // queued so far (the previous gather reads the hit points it rewrites),
-    // then this pass's camera words; the next pass's words follow it there.
-    HIPOK(hipEventRecord(c->ev_gfork, st));
-    HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+    // reference too, SPPM.cpp:809-838) into the hit-point buffer of this
+    // pass's parity: in passes in a row it waits for the gather that last
+    // read that buffer (two passes back), otherwise for what the main stream
+    // has queued so far; then for this pass's camera words.  The next pass's
+    // words follow it there.
+    const int hb = (int)(c->pass & 1);
+    c->rb.hits = c->rb.hitbuf[hb];
+    if (lead) {
+        HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gathered[hb], 0));
+    } else {
+        HIPOK(hipEventRecord(c->ev_gfork, st));
+        HIPOK(hipStreamWaitEvent(c->stream2, c->ev_gfork, 0));
+    }
     rc = do_eye(c, step, c->stream2, &c->t_eye);
     if (rc) return rc;
     HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
@@ -1467,6 +1502,7 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         c->t_guide.stop(st);
     }
     launch_add_i32(c->rb.pass_dev, 1, st);
+    HIPOK(hipEventRecord(c->ev_gathered[hb], st));
     HIPOK(hipGetLastError());
     c->pass += 1;
     c->lead_ok = true;
@@ -1557,20 +1593,29 @@ int sppm_graph_capture(sppm_ctx* c) {
              if (c->guided) do_stage(c, c->stream);
              do_build_grid(c, c->stream);
              return SPPM_OK;
-         })) ||
-        (rc = capture_on(c->stream, &c->graph, &c->graph_exec, [&]() {
-             int r = do_gather(c, c->stream);
-             if (!r && c->guided) do_guide(c, c->stream, 0, R.pass_dev);
-             launch_add_i32(R.pass_dev, 1, c->stream);
-             return r;
          }))) {
+        drop_graph(c);
+        return rc;
+    }
+    // even and odd passes: the gather of hit-point buffer b
+    for (int b = 0; b < 2 && !rc; ++b) {
+        R.hits = R.hitbuf[b];
+        rc = capture_on(c->stream, b ? &c->graph_odd : &c->graph, b ? &c->graph_exec_odd : &c->graph_exec, [&]() {
+            int r = do_gather(c, c->stream);
+            if (!r && c->guided) do_guide(c, c->stream, 0, R.pass_dev);
+            launch_add_i32(R.pass_dev, 1, c->stream);
+            return r;
+        });
+    }
+    R.hits = R.hitbuf[0];
+    if (rc) {
         drop_graph(c);
         return rc;
     }
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = capture_on(c->stream2, &c->g_eye[b], &c->x_eye[b], [&]() {
             launch_add_i32(R.pass_dev + 2, 1, c->stream2);
-            launch_eye(c->scene.dev, c->camdev, R.hits, 0, R.pass_dev + 2, trace_spill_eye(c), R.words + b * blk,
+            launch_eye(c->scene.dev, c->camdev, R.hitbuf[b], 0, R.pass_dev + 2, trace_spill_eye(c), R.words + b * blk,
                        R.flags, eye_rays(R), c->scene.next_path + STRIPE_INTS, c->params.eye_nee, c->stream2);
             return SPPM_OK;
         });
