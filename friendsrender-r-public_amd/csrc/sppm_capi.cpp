@@ -177,7 +177,7 @@ struct RenderBuffers {
         dfree(slots.pos); dfree(slots.flux); dfree(slots.dir); dfree(slots.valid); dfree(slots.count);
         dfree(off); dfree(nstored); dfree(scan_tmp); dfree(cell_gaps); dfree(cell_ngap); dfree(cell_start);
         dfree(psort.keys); dfree(psort.vals); dfree(psort.tmp_k); dfree(psort.tmp_v); dfree(psort.hist);
-        dfree(staged.pos); dfree(staged.flux); dfree(staged.dir);
+        dfree(staged.rec);
         dfree(gpos); dfree(gflux); dfree(gdir); dfree(gp);
         for (HitDev& h : hitbuf) {
             dfree(h.pos); dfree(h.nrm); dfree(h.out); dfree(h.weight); dfree(h.mat); dfree(h.absorb); dfree(h.direct);
@@ -1007,8 +1007,7 @@ int sppm_begin(sppm_ctx* c) {
         (rc = dalloc(&R.cell_gaps, cell_gap_entries(R.cell_cap))) || (rc = dalloc(&R.cell_ngap, 1)) ||
         (rc = dalloc(&R.cell_start, R.cell_cap + 1)) || (rc = dalloc(&R.psort.keys, R.photon_cap)) ||
         (rc = dalloc(&R.psort.vals, R.photon_cap)) || (rc = dalloc(&R.psort.tmp_k, R.photon_cap)) ||
-        (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.staged.pos, R.photon_cap)) ||
-        (rc = dalloc(&R.staged.flux, R.photon_cap)) || (rc = dalloc(&R.staged.dir, R.photon_cap)) ||
+        (rc = dalloc(&R.psort.tmp_v, R.photon_cap)) || (rc = dalloc(&R.staged.rec, 3 * (size_t)R.photon_cap)) ||
         (rc = dalloc(&R.psort.hist, radix_hist_entries(R.photon_cap))) ||
         (rc = dalloc(&R.gpos, R.photon_cap)) || (rc = dalloc(&R.gflux, R.photon_cap)) ||
         (rc = dalloc(&R.gdir, R.photon_cap)) || (rc = dalloc(&R.gp, 1)) || (rc = dalloc(&R.shared.rad, np)) ||
@@ -1186,8 +1185,9 @@ static int read_counters(sppm_ctx* c, Counters& h) {
 }
 // beside_gather: the photon pass runs beside the previous pass's gather and
 // takes 9/16 of the path-kernel grid, so the gather's kernels find free wave
-// slots instead of waiting for the persistent photon waves to drain (full
-// grid 5.33 ms per pass, 9/16: 5.02 on the full-material workload).
+// slots instead of waiting for the persistent photon waves to drain.  On the
+// full-material workload (1M photons, 1024^2) a pass takes 4.35 ms with
+// 9/16, 4.38-4.79 with the full grid, 4.36-4.83 with 12/16, 5.43 with 6/16.
 static void photon_pass(sppm_ctx* c, int step, const int* step_dev, hipStream_t st, bool beside_gather) {
     RenderBuffers& R = c->rb;
     TraceSpill sp = trace_spill(c);

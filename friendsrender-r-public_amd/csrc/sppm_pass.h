@@ -93,12 +93,14 @@ void launch_occluded(const SceneDev& S, int n, const float* org, const float* di
 struct PhotonSortBuffers {
     uint32_t *keys, *vals, *tmp_k, *tmp_v, *hist;
 };
-// The flattened photons of a pass in reference order, photon_cap entries
-// each (dir.w: the flattened index), copied out of the slots by the build's
-// first step so that the next photon pass may run while the rest of the
-// build reads them.
+// The flattened photons of a pass in reference order, copied out of the
+// slots by the build's first step so that the next photon pass may run while
+// the rest of the build reads them: photon j is the 48-byte record
+// rec[3j .. 3j+2] = (pos, flux, dir with the flattened index in dir.w), so
+// the cell-order copy reads one record per photon instead of three scattered
+// vectors (3 * photon_cap float4).
 struct StagedPhotons {
-    float4 *pos, *flux, *dir;
+    float4* rec;
 };
 // off: exclusive scan of the slots' per-path counts
 void launch_photon_stage(const SlotsDev& slots, const int* off, const StagedPhotons& sp, hipStream_t st);

@@ -340,19 +340,20 @@ __global__ void __launch_bounds__(256) k_photon_stage(SlotsDev slots, const int*
         const size_t o = (size_t)s * NP + i;
         float4 d = slots.dir[o];
         d.w = __int_as_float(base + s);
-        sp.pos[base + s] = slots.pos[o];
-        sp.flux[base + s] = slots.flux[o];
-        sp.dir[base + s] = d;
+        float4* r = sp.rec + 3 * (size_t)(base + s);
+        r[0] = slots.pos[o];
+        r[1] = slots.flux[o];
+        r[2] = d;
     }
 }
 
 // Cell key and flattened index of every staged photon.
-__global__ void __launch_bounds__(256) k_photon_keys(const float4* __restrict__ pos, const int* __restrict__ nstored,
+__global__ void __launch_bounds__(256) k_photon_keys(const float4* __restrict__ rec, const int* __restrict__ nstored,
                                                      const GridParams* __restrict__ gp, PhotonSortBuffers sb) {
     const int n = *nstored;
     const GridParams g = *gp;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const float4 p = pos[j];
+        const float4 p = rec[3 * (size_t)j];
         int cx = cell_coord(p.x, g.bmin[0], g.inv_cell, g.nx);
         int cy = cell_coord(p.y, g.bmin[1], g.inv_cell, g.ny);
         int cz = cell_coord(p.z, g.bmin[2], g.inv_cell, g.nz);
@@ -405,10 +406,10 @@ __global__ void __launch_bounds__(256) k_photon_sorted(StagedPhotons sp, const i
                                                        float4* __restrict__ gdir) {
     const int n = *nstored;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t flat = order[j];
-        gpos[j] = sp.pos[flat];
-        gflux[j] = sp.flux[flat];
-        gdir[j] = sp.dir[flat];
+        const float4* r = sp.rec + 3 * (size_t)order[j];
+        gpos[j] = r[0];
+        gflux[j] = r[1];
+        gdir[j] = r[2];
     }
 }
 
@@ -1655,7 +1656,7 @@ void launch_build_grid(const StagedPhotons& sp, const int* nstored, Counters* ct
                        float4* gpos, float4* gflux, float4* gdir, hipStream_t st) {
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1), 0, st, gp, rstat, bmin[0], bmin[1], bmin[2], ext[0],
                        ext[1], ext[2], cell_cap, nstored, ctr, ngap);
-    hipLaunchKernelGGL(k_photon_keys, dim3(2048), dim3(256), 0, st, sp.pos, nstored, gp, sb);
+    hipLaunchKernelGGL(k_photon_keys, dim3(2048), dim3(256), 0, st, sp.rec, nstored, gp, sb);
     // stable sort of the flattened photons by cell: cell keys are below
     // cell_cap = 2^22, the flattened order is kept inside a cell
     const SortResult sr =
