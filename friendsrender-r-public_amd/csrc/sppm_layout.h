@@ -14,11 +14,10 @@ namespace sppm {
 // Wide BVH node of width W (2, 4, 8): child boxes as 6 rows of W floats
 // (min x, min y, min z, max x, max y, max z), then W child references,
 // padded to a multiple of 16 bytes.  Child reference: >= 0 internal node,
-// LEAF_BIT | (start << 3) | (count - 1) leaf range in the leaf-triangle
-// array, EMPTY_CHILD empty slot (its box is inverted).
+// LEAF_BIT | (start << 3) | (count - 1) leaf range of 1-8 triangles in the
+// leaf-triangle array, EMPTY_CHILD empty slot (its box is inverted).
 constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t EMPTY_CHILD = 0xffffffffu;
-constexpr int LEAF_MAX = 4;  // triangles per leaf range
 constexpr int TRACE_LDS = 16;  // traversal stack entries per lane kept in LDS (deeper ones spill)
 // LDS words of a 64-lane block's traversal stack: the node references, then
 // their boxes' entry distances
