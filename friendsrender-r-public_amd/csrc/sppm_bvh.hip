@@ -168,8 +168,9 @@ __global__ void k_tri_reorder(const float* __restrict__ pos, const uint32_t* __r
 
 // ---- binned-SAH top-down builder ---------------------------------------------
 // One workgroup per open segment [lo, hi] of the primitive order: centroid
-// bounds, the axis of largest extent, 32 bins (counts and boxes), the split
-// with the lowest A_L N_L + A_R N_R, then a stable in-place partition.  The
+// bounds, 32 bins (counts and boxes) along each axis with a centroid extent,
+// the split with the lowest A_L N_L + A_R N_R over the three axes, then a
+// stable in-place partition.  The
 // segment's internal node id is its split position s (left = [lo, s]); every
 // position boundary is split exactly once, so ids are unique and the result
 // has the radix-tree format the refit and the wide collapse read: child
@@ -192,18 +193,21 @@ __global__ void __launch_bounds__(SAH_TB) k_sah_split(const SahSeg* __restrict__
                                                       int2* __restrict__ range, int* __restrict__ parent,
                                                       int* __restrict__ root) {
     __shared__ int cmin[3], cmax[3];
-    __shared__ int bcnt[SAH_BINS];
-    __shared__ int bmn[SAH_BINS][3], bmx[SAH_BINS][3];
+    __shared__ int bcnt[3][SAH_BINS];
+    __shared__ int bmn[3][SAH_BINS][3], bmx[3][SAH_BINS][3];
     __shared__ int s_split, s_axis;
-    __shared__ float s_lo, s_scale;
+    __shared__ float s_lo[3], s_scale[3];
+    __shared__ float s_cost[3];
+    __shared__ int s_best[3];
     __shared__ int wsum[SAH_TB / 64];
     const int t = threadIdx.x;
     const SahSeg sg = qin[blockIdx.x];
     const int lo = sg.lo, hi = sg.hi, cnt = hi - lo + 1;
     if (t < 3) { cmin[t] = 0x7fffffff; cmax[t] = (int)0x80000000; }
-    if (t < SAH_BINS) {
-        bcnt[t] = 0;
-        for (int a = 0; a < 3; ++a) { bmn[t][a] = 0x7fffffff; bmx[t][a] = (int)0x80000000; }
+    if (t < 3 * SAH_BINS) {
+        const int ax = t / SAH_BINS, k = t % SAH_BINS;
+        bcnt[ax][k] = 0;
+        for (int a = 0; a < 3; ++a) { bmn[ax][k][a] = 0x7fffffff; bmx[ax][k][a] = (int)0x80000000; }
     }
     __syncthreads();
     // centroid bounds
@@ -217,78 +221,85 @@ __global__ void __launch_bounds__(SAH_TB) k_sah_split(const SahSeg* __restrict__
         }
     }
     __syncthreads();
-    if (t == 0) {
-        int ax = 0;
-        float ext = -1.0f;
-        for (int k = 0; k < 3; ++k) {
-            const float e = fkey_inv(cmax[k]) - fkey_inv(cmin[k]);
-            if (e > ext) { ext = e; ax = k; }
-        }
-        s_axis = ax;
-        s_lo = fkey_inv(cmin[ax]);
-        s_scale = ext > 0.0f ? (float)SAH_BINS * (1.0f - 1e-6f) / ext : 0.0f;
+    if (t < 3) {
+        const float ext = fkey_inv(cmax[t]) - fkey_inv(cmin[t]);
+        s_lo[t] = fkey_inv(cmin[t]);
+        s_scale[t] = ext > 0.0f ? (float)SAH_BINS * (1.0f - 1e-6f) / ext : 0.0f;
     }
     __syncthreads();
-    const int ax = s_axis;
-    const float blo = s_lo, bs = s_scale;
-    auto bin_of = [&](uint32_t f) {
+    auto bin_of = [&](uint32_t f, int ax) {
         const float4 a = tbmin[f], b = tbmax[f];
         const float c = ax == 0 ? 0.5f * (a.x + b.x) : (ax == 1 ? 0.5f * (a.y + b.y) : 0.5f * (a.z + b.z));
-        int k = (int)((c - blo) * bs);
+        int k = (int)((c - s_lo[ax]) * s_scale[ax]);
         return k < 0 ? 0 : (k >= SAH_BINS ? SAH_BINS - 1 : k);
     };
-    if (bs > 0.0f) {
-        for (int p = lo + t; p <= hi; p += SAH_TB) {
-            const uint32_t f = ord[p];
-            const int k = bin_of(f);
-            const float4 a = tbmin[f], b = tbmax[f];
-            atomicAdd(&bcnt[k], 1);
-            atomicMin(&bmn[k][0], fkey(a.x)); atomicMin(&bmn[k][1], fkey(a.y)); atomicMin(&bmn[k][2], fkey(a.z));
-            atomicMax(&bmx[k][0], fkey(b.x)); atomicMax(&bmx[k][1], fkey(b.y)); atomicMax(&bmx[k][2], fkey(b.z));
+    // bins along every axis with a centroid extent
+    for (int p = lo + t; p <= hi; p += SAH_TB) {
+        const uint32_t f = ord[p];
+        const float4 a = tbmin[f], b = tbmax[f];
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!(s_scale[ax] > 0.0f)) continue;
+            const int k = bin_of(f, ax);
+            atomicAdd(&bcnt[ax][k], 1);
+            atomicMin(&bmn[ax][k][0], fkey(a.x)); atomicMin(&bmn[ax][k][1], fkey(a.y));
+            atomicMin(&bmn[ax][k][2], fkey(a.z));
+            atomicMax(&bmx[ax][k][0], fkey(b.x)); atomicMax(&bmx[ax][k][1], fkey(b.y));
+            atomicMax(&bmx[ax][k][2], fkey(b.z));
         }
     }
     __syncthreads();
-    if (t == 0) {
+    if (t < 3) {
+        // the SAH sweep of axis t: suffix areas / counts, then prefixes
+        const int ax = t;
         int best = -1;
-        if (bs > 0.0f) {
-            // suffix areas / counts, then the sweep
+        float bestc = 3.4e38f;
+        if (s_scale[ax] > 0.0f) {
             float ra[SAH_BINS];
             int rc[SAH_BINS];
             float mn[3] = {3e38f, 3e38f, 3e38f}, mx[3] = {-3e38f, -3e38f, -3e38f};
             int c = 0;
             for (int k = SAH_BINS - 1; k >= 0; --k) {
-                if (bcnt[k]) {
+                if (bcnt[ax][k]) {
                     for (int a = 0; a < 3; ++a) {
-                        mn[a] = fminf(mn[a], fkey_inv(bmn[k][a]));
-                        mx[a] = fmaxf(mx[a], fkey_inv(bmx[k][a]));
+                        mn[a] = fminf(mn[a], fkey_inv(bmn[ax][k][a]));
+                        mx[a] = fmaxf(mx[a], fkey_inv(bmx[ax][k][a]));
                     }
                 }
-                c += bcnt[k];
+                c += bcnt[ax][k];
                 const float x = mx[0] - mn[0], y = mx[1] - mn[1], z = mx[2] - mn[2];
                 ra[k] = c ? x * y + y * z + z * x : 0.0f;
                 rc[k] = c;
             }
             float lmn[3] = {3e38f, 3e38f, 3e38f}, lmx[3] = {-3e38f, -3e38f, -3e38f};
             int lc = 0;
-            float bestc = 3.4e38f;
             for (int k = 0; k < SAH_BINS - 1; ++k) {
-                if (bcnt[k]) {
+                if (bcnt[ax][k]) {
                     for (int a = 0; a < 3; ++a) {
-                        lmn[a] = fminf(lmn[a], fkey_inv(bmn[k][a]));
-                        lmx[a] = fmaxf(lmx[a], fkey_inv(bmx[k][a]));
+                        lmn[a] = fminf(lmn[a], fkey_inv(bmn[ax][k][a]));
+                        lmx[a] = fmaxf(lmx[a], fkey_inv(bmx[ax][k][a]));
                     }
                 }
-                lc += bcnt[k];
+                lc += bcnt[ax][k];
                 if (lc == 0 || rc[k + 1] == 0) continue;
                 const float x = lmx[0] - lmn[0], y = lmx[1] - lmn[1], z = lmx[2] - lmn[2];
                 const float cost = (x * y + y * z + z * x) * (float)lc + ra[k + 1] * (float)rc[k + 1];
                 if (cost < bestc) { bestc = cost; best = k; }
             }
         }
-        s_split = best;  // -1: split at the middle of the range
+        s_cost[ax] = bestc;
+        s_best[ax] = best;
     }
     __syncthreads();
-    const int split_bin = s_split;
+    if (t == 0) {
+        // the cheapest axis (ties: the lower axis); -1: split at the middle of the range
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (s_cost[k] < s_cost[ax]) ax = k;
+        s_axis = ax;
+        s_split = s_best[ax];
+    }
+    __syncthreads();
+    const int split_bin = s_split, ax = s_axis;
     int nleft;
     if (split_bin < 0) {
         nleft = cnt / 2;
@@ -297,12 +308,12 @@ __global__ void __launch_bounds__(SAH_TB) k_sah_split(const SahSeg* __restrict__
         const int lane = t & 63, w = t >> 6;
         int base_l = 0, base_r = 0;
         int total_left = 0;
-        for (int k = 0; k <= split_bin; ++k) total_left += bcnt[k];
+        for (int k = 0; k <= split_bin; ++k) total_left += bcnt[ax][k];
         for (int c0 = lo; c0 <= hi; c0 += SAH_TB) {
             const int p = c0 + t;
             const bool valid = p <= hi;
             const uint32_t f = valid ? ord[p] : 0u;
-            const bool left = valid && bin_of(f) <= split_bin;
+            const bool left = valid && bin_of(f, ax) <= split_bin;
             const unsigned long long bl = __ballot(left);
             const int rank_w = __popcll(bl & ((1ull << lane) - 1ull));
             if (lane == 0) wsum[w] = __popcll(bl);
