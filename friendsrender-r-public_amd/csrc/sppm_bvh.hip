@@ -175,7 +175,7 @@ __global__ void k_tri_reorder(const float* __restrict__ pos, const uint32_t* __r
 // position boundary is split exactly once, so ids are unique and the result
 // has the radix-tree format the refit and the wide collapse read: child
 // (left, right) with leaves n-1+k, range (lo, hi), parent.
-constexpr int SAH_BINS = 32;
+constexpr int SAH_BINS = 32;  // 16: as fast, 64: slower traversal (full-material scene, BVH8)
 constexpr int SAH_TB = 256;
 struct SahSeg {
     int lo, hi, parent, side;  // side 0: left child of parent, 1: right
@@ -397,13 +397,15 @@ void free_bvh(BvhResult* r) {
 // between its two binary children).  The wide tree then follows the
 // decisions from the root, breadth first.  Node boxes are the binary nodes'
 // refitted boxes, so every triangle stays inside the boxes above it.
-// C_TRI = 0.1 with leaves of up to 8 triangles measured fastest on the
-// full-material scene (BVH8: photon pass 2.88 ms against 3.01 with the
-// earlier greedy collapse, which opened the largest child until a node had
-// W; C_TRI 0.05 / 0.15 / 0.3 / 0.6 and 4-triangle leaves were slower).
+// C_TRI = 0.15 with leaves of up to 8 triangles measured fastest on the
+// full-material scene (BVH8, three-axis SAH: 4.06 ms per pass against 4.13
+// at 0.1 and 4.27 at 0.07; with the longest-axis SAH 0.05 / 0.3 / 0.6 and
+// 4-triangle leaves were slower, and the photon pass took 2.88 ms against
+// 3.01 with the earlier greedy collapse, which opened the largest child
+// until a node had W).
 namespace {
 constexpr float DP_C_NODE = 1.0f;
-constexpr float DP_C_TRI = 0.1f;
+constexpr float DP_C_TRI = 0.15f;
 constexpr int DP_LEAF_MAX = 8;  // a leaf reference holds count - 1 in 3 bits
 int collapse_dp(const std::vector<int2>& child, const std::vector<int2>& range, const std::vector<float4>& bmin,
                 const std::vector<float4>& bmax, int n, int root, int W, std::vector<float>& nodes) {
