@@ -150,6 +150,10 @@ struct RenderBuffers {
     int* path_cell = nullptr;  // guided emission: emission cell per path, 2 x N (block = step parity)
     float* path_ratio = nullptr;  // guided emission: q / p_plain per path, like path_cell
     float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
+    // guided emission: this pass's photon offsets and stored count (N + 1),
+    // copied by the build for the gather's deposits and the training records,
+    // which run while the next pass stages its photons into off / nstored
+    int* doff = nullptr;
     uint32_t* flags = nullptr;
     Counters* ctr = nullptr;
     // closest-hit ray and tie counts, striped (STRIPES lines of RAY_STRIPE
@@ -184,7 +188,7 @@ struct RenderBuffers {
         }
         hits = HitDev{};
         dfree(shared.rad); dfree(shared.cnt); dfree(shared.direct); dfree(counts); dfree(list_off); dfree(list_scan_tmp); dfree(lists);
-        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(path_ratio); dfree(pdep); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
+        dfree(words); dfree(cam_states); dfree(cam_phi); dfree(rstat); dfree(tiles); dfree(chunkp); dfree(nch); dfree(chunk_off); dfree(tile_scan_tmp); dfree(partial); dfree(groups.keys); dfree(groups.vals); dfree(groups.tmp_k); dfree(groups.tmp_v); dfree(groups.hist); dfree(groups.nvalid); dfree(groups.chunkq); dfree(groups.stat); dfree(pass_dev); dfree(path_cell); dfree(path_ratio); dfree(pdep); dfree(doff); dfree(flags); dfree(ctr); dfree(raystripes); dfree(image);
         ready = false;
     }
 };
@@ -287,7 +291,7 @@ struct sppm_ctx {
     // beside the previous pass's gather.  lead_ok says that nothing but
     // sppm_pass has queued work since.
     hipStream_t stream3 = nullptr;
-    hipEvent_t ev_built = nullptr, ev_photons = nullptr, ev_staged = nullptr;
+    hipEvent_t ev_built = nullptr, ev_photons = nullptr, ev_staged = nullptr, ev_published = nullptr;
     hipEvent_t ev_gathered[2] = {nullptr, nullptr};  // the gather of the last pass of each parity (its hit points)
     bool lead_ok = false;
     bool own_stream = false;
@@ -305,12 +309,13 @@ struct sppm_ctx {
     // graph_passes counts the replays.
     hipGraph_t graph = nullptr, graph_odd = nullptr;
     hipGraphExec_t graph_exec = nullptr, graph_exec_odd = nullptr;
-    // Plain passes stage their photons in a graph of their own (g_stage /
-    // x_stage) on stream3 after the photon pass; guided ones in the build's.
-    hipGraph_t g_photon = nullptr, g_stage = nullptr, g_build = nullptr, g_eye[2] = {nullptr, nullptr},
-               g_words[2] = {nullptr, nullptr};
-    hipGraphExec_t x_photon = nullptr, x_stage = nullptr, x_build = nullptr, x_eye[2] = {nullptr, nullptr},
-                   x_words[2] = {nullptr, nullptr};
+    // The photons are staged in a graph of their own (g_stage / x_stage) on
+    // stream3 after the photon pass; guided passes publish the proposal in
+    // one (g_publish / x_publish) on the main stream before the build.
+    hipGraph_t g_photon = nullptr, g_stage = nullptr, g_publish = nullptr, g_build = nullptr,
+               g_eye[2] = {nullptr, nullptr}, g_words[2] = {nullptr, nullptr};
+    hipGraphExec_t x_photon = nullptr, x_stage = nullptr, x_publish = nullptr, x_build = nullptr,
+                   x_eye[2] = {nullptr, nullptr}, x_words[2] = {nullptr, nullptr};
     long long graph_passes = 0;
     // The batch whose graph holds this context's buffers (sppm_batch_add), and
     // whether it has launched since: readbacks and synchronisation wait for
@@ -355,13 +360,13 @@ static int not_in_batch(sppm_ctx* c, const char* what) {
 // A captured pass holds the scene, render and proposal buffers by value.
 static void drop_graph(sppm_ctx* c) {
     for (hipGraphExec_t* x :
-         {&c->graph_exec, &c->graph_exec_odd, &c->x_photon, &c->x_stage, &c->x_build, &c->x_eye[0], &c->x_eye[1],
-          &c->x_words[0], &c->x_words[1]}) {
+         {&c->graph_exec, &c->graph_exec_odd, &c->x_photon, &c->x_stage, &c->x_publish, &c->x_build, &c->x_eye[0],
+          &c->x_eye[1], &c->x_words[0], &c->x_words[1]}) {
         if (*x) (void)hipGraphExecDestroy(*x);
         *x = nullptr;
     }
-    for (hipGraph_t* g : {&c->graph, &c->graph_odd, &c->g_photon, &c->g_stage, &c->g_build, &c->g_eye[0], &c->g_eye[1],
-                          &c->g_words[0], &c->g_words[1]}) {
+    for (hipGraph_t* g : {&c->graph, &c->graph_odd, &c->g_photon, &c->g_stage, &c->g_publish, &c->g_build,
+                          &c->g_eye[0], &c->g_eye[1], &c->g_words[0], &c->g_words[1]}) {
         if (*g) (void)hipGraphDestroy(*g);
         *g = nullptr;
     }
@@ -460,6 +465,7 @@ int sppm_create(sppm_ctx** out, int device, void* stream) {
     HIPOK(hipEventCreateWithFlags(&c->ev_built, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_photons, hipEventDisableTiming));
     HIPOK(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
+    HIPOK(hipEventCreateWithFlags(&c->ev_published, hipEventDisableTiming));
     for (hipEvent_t& e : c->ev_gathered) {
         HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPOK(hipEventRecord(e, c->stream));
@@ -513,6 +519,7 @@ int sppm_destroy(sppm_ctx* c) {
     (void)hipEventDestroy(c->ev_built);
     (void)hipEventDestroy(c->ev_photons);
     (void)hipEventDestroy(c->ev_staged);
+    (void)hipEventDestroy(c->ev_published);
     for (hipEvent_t e : c->ev_gathered) (void)hipEventDestroy(e);
     drop_graph(c);
     (void)hipStreamDestroy(c->stream2);
@@ -1078,7 +1085,7 @@ int sppm_begin(sppm_ctx* c) {
     }
     if (c->guided) {
         if ((rc = dalloc(&R.path_cell, 2 * (size_t)N)) || (rc = dalloc(&R.path_ratio, 2 * (size_t)N)) ||
-            (rc = dalloc(&R.pdep, R.photon_cap))) {
+            (rc = dalloc(&R.pdep, R.photon_cap)) || (rc = dalloc(&R.doff, (size_t)N + 1))) {
             R.release();
             return rc;
         }
@@ -1215,7 +1222,7 @@ static void do_photons(sppm_ctx* c, int step, hipStream_t st, bool beside_gather
 static void do_guide(sppm_ctx* c, hipStream_t st, int step, const int* step_dev) {
     RenderBuffers& R = c->rb;
     const bool rms = c->gparams.target == SPPM_GUIDE_TARGET_RMS;
-    launch_guide_deposit(R.off, R.nstored, R.pdep, R.path_cell, rms ? R.path_ratio : nullptr, R.N, step, step_dev,
+    launch_guide_deposit(R.doff, R.doff + R.N, R.pdep, R.path_cell, rms ? R.path_ratio : nullptr, R.N, step, step_dev,
                          c->guide.hpass, st);
     const int every = c->gparams.train_interval > 1 ? c->gparams.train_interval : 1;
     if (!step_dev && step % every != 0) return;  // the deposits gather in the pass histogram until then
@@ -1236,20 +1243,25 @@ static void do_guide(sppm_ctx* c, hipStream_t st, int step, const int* step_dev)
 // (after it the next photon pass may rewrite them).
 static void do_stage(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
-    // the photon pass that read the emission tables is done: publish the
-    // proposal trained last for the next photon pass
-    if (c->guided) guide_publish(c->guide, st);
     exclusive_scan(R.slots.count, R.off, R.N, R.nstored, R.scan_tmp, st);
     launch_photon_stage(R.slots, R.off, R.staged, st);
+}
+// Guided emission: once the photon pass that read the emission tables is
+// done, and before the next one, the proposal trained last (after the
+// previous pass's gather) becomes the next photon pass's.
+static void do_publish(sppm_ctx* c, hipStream_t st) {
+    if (c->guided) guide_publish(c->guide, st);
 }
 // The rest of the build: grid, keys, sort and the cell-sorted records.
 static void do_build_grid(sppm_ctx* c, hipStream_t st) {
     RenderBuffers& R = c->rb;
+    if (c->guided) launch_copy_offsets(R.off, R.nstored, R.N, R.doff, st);
     launch_build_grid(R.staged, R.nstored, R.ctr, R.gp, R.rstat, c->scene.bmin, c->scene.ext, R.cell_cap,
                       R.cell_start, R.cell_gaps, R.cell_ngap, R.psort, R.photon_cap, R.gpos, R.gflux, R.gdir, st);
 }
 static void do_build(sppm_ctx* c, hipStream_t st) {
     do_stage(c, st);
+    do_publish(c, st);
     do_build_grid(c, st);
 }
 // Generates camera word blocks up to and including block k on stream2, in
@@ -1312,7 +1324,7 @@ static int do_gather(sppm_ctx* c, hipStream_t st) {
         R.have_lists = true;
     } else {
         launch_set_u64x2(R.rstat, 0ull, 0ull, st);
-        if (c->guided) launch_deposit_clear(R.pdep, R.nstored, st);
+        if (c->guided) launch_deposit_clear(R.pdep, R.doff + R.N, st);
         const PairRecord rec{R.pairs, R.npairs, R.pairs_cap};
         launch_gather_balanced(R.hits, R.shared, G, R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups, alpha, R.tiles,
                                GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.partial, R.counts,
@@ -1421,16 +1433,18 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
         HIPOK(hipGraphLaunch(c->x_eye[b], c->stream2));
         HIPOK(hipEventRecord(c->ev_gjoin, c->stream2));
         HIPOK(hipGraphLaunch(c->x_words[1 - b], c->stream2));  // the next pass's, beside the gather
-        const bool early = c->x_stage != nullptr;
-        HIPOK(hipStreamWaitEvent(c->stream3, !lead ? c->ev_gfork : early ? c->ev_staged : c->ev_built, 0));
+        HIPOK(hipStreamWaitEvent(c->stream3,
+                                 !lead ? c->ev_gfork : c->guided ? c->ev_published : c->ev_staged, 0));
         HIPOK(hipGraphLaunch(c->x_photon, c->stream3));
-        if (early) {
-            if (lead) HIPOK(hipStreamWaitEvent(c->stream3, c->ev_built, 0));
-            HIPOK(hipGraphLaunch(c->x_stage, c->stream3));
-            HIPOK(hipEventRecord(c->ev_staged, c->stream3));
-        }
+        if (lead) HIPOK(hipStreamWaitEvent(c->stream3, c->ev_built, 0));
+        HIPOK(hipGraphLaunch(c->x_stage, c->stream3));
+        HIPOK(hipEventRecord(c->ev_staged, c->stream3));
         HIPOK(hipEventRecord(c->ev_photons, c->stream3));
         HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
+        if (c->x_publish) {
+            HIPOK(hipGraphLaunch(c->x_publish, st));
+            HIPOK(hipEventRecord(c->ev_published, st));
+        }
         HIPOK(hipGraphLaunch(c->x_build, st));
         HIPOK(hipEventRecord(c->ev_built, st));
         HIPOK(hipStreamWaitEvent(st, c->ev_gjoin, 0));
@@ -1462,31 +1476,35 @@ int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     rc = words_generate_upto(c, c->words_used);
     if (rc) return rc;
     // The photon pass rewrites the photon slots, which the previous pass
-    // staged (plain) or built from (guided); the previous gather reads only
-    // the grid built from them.
-    const bool early = lead && !c->guided;
+    // staged; the previous gather reads only the grid built from them.  A
+    // guided photon pass waits for the previous pass's publish instead: its
+    // emission tables, and the path-cell block it rewrites (read by the
+    // training records of the pass before).
     hipStream_t sp = st;
     if (lead) {
         sp = c->stream3;
-        HIPOK(hipStreamWaitEvent(sp, c->guided ? c->ev_built : c->ev_staged, 0));
+        HIPOK(hipStreamWaitEvent(sp, c->guided ? c->ev_published : c->ev_staged, 0));
     }
     c->t_photons.start(sp);
     do_photons(c, step, sp, lead);
     c->t_photons.stop(sp);
-    if (early) {
+    if (lead) {
         // the previous build has read the staged photons and their offsets
+        // (and copied them for the gather's deposits and training records)
         HIPOK(hipStreamWaitEvent(sp, c->ev_built, 0));
         do_stage(c, sp);
         HIPOK(hipEventRecord(c->ev_staged, sp));
-    }
-    if (lead) {
         HIPOK(hipEventRecord(c->ev_photons, sp));
         HIPOK(hipStreamWaitEvent(st, c->ev_photons, 0));
     }
     c->t_build.start(st);
-    if (!early) {
+    if (!lead) {
         do_stage(c, st);
         HIPOK(hipEventRecord(c->ev_staged, st));
+    }
+    if (c->guided) {
+        do_publish(c, st);
+        HIPOK(hipEventRecord(c->ev_published, st));
     }
     do_build_grid(c, st);
     c->t_build.stop(st);
@@ -1585,12 +1603,15 @@ int sppm_graph_capture(sppm_ctx* c) {
              photon_pass(c, 0, R.pass_dev + 1, c->stream3, true);
              return SPPM_OK;
          })) ||
-        (!c->guided && (rc = capture_on(c->stream3, &c->g_stage, &c->x_stage, [&]() {
+        (rc = capture_on(c->stream3, &c->g_stage, &c->x_stage, [&]() {
              do_stage(c, c->stream3);
+             return SPPM_OK;
+         })) ||
+        (c->guided && (rc = capture_on(c->stream, &c->g_publish, &c->x_publish, [&]() {
+             do_publish(c, c->stream);
              return SPPM_OK;
          }))) ||
         (rc = capture_on(c->stream, &c->g_build, &c->x_build, [&]() {
-             if (c->guided) do_stage(c, c->stream);
              do_build_grid(c, c->stream);
              return SPPM_OK;
          }))) {
@@ -1985,7 +2006,7 @@ static int enqueue_multi(sppm_batch* b, hipStream_t st) {
         sppm_ctx* c = b->ctx[k];
         RenderBuffers& R = c->rb;
         launch_set_u64x2(R.rstat, 0ull, 0ull, bs);
-        if (c->guided) launch_deposit_clear(R.pdep, R.nstored, bs);
+        if (c->guided) launch_deposit_clear(R.pdep, R.doff + R.N, bs);
         const PairRecord rec{R.pairs, R.npairs, R.pairs_cap};
         (void)launch_gather_plan(R.hits, R.shared, grid_dev(c), R.gp, R.np, c->scene.bmin, c->scene.ext, R.groups,
                                  R.tiles, GATHER_CHUNK_CAP, R.chunkp, R.nch, R.chunk_off, R.tile_scan_tmp, R.counts,
@@ -2776,7 +2797,7 @@ int sppm_guide_set(sppm_ctx* c, const sppm_guide_params* p) {
     if (c->rb.ready && !c->rb.path_cell) {
         RenderBuffers& R = c->rb;
         if ((rc = dalloc(&R.path_cell, 2 * (size_t)R.N)) || (rc = dalloc(&R.path_ratio, 2 * (size_t)R.N)) ||
-            (rc = dalloc(&R.pdep, R.photon_cap)))
+            (rc = dalloc(&R.pdep, R.photon_cap)) || (rc = dalloc(&R.doff, (size_t)R.N + 1)))
             return rc;
         c->emit.path_cell = R.path_cell;
     }

@@ -72,6 +72,8 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
 void launch_add_i32(int* p, int v, hipStream_t st);
 void launch_zero_i32(int* p, int n, hipStream_t st);
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
+// dst[0 .. N-1] = off, dst[N] = *nstored
+void launch_copy_offsets(const int* off, const int* nstored, int N, int* dst, hipStream_t st);
 // step: the pass step, or *step_dev + 1 when step_dev is given (graph replays)
 // ratio (optional): q / p_plain per path; with it the deposit is d^2 ratio
 void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell,

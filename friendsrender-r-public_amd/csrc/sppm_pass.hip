@@ -1130,22 +1130,14 @@ __global__ void k_tile_chunks(const TileRange* __restrict__ tiles, int ntiles, c
     nch[t] = (tiles[t].cand + c - 1) / c;
 }
 
-// Branch-free form of in_query (same two tests, evaluated without early exit).
+// The reference's two neighbour tests for one candidate, closed box
+// (kdtree.h:78-88) and the strict radius (SPPM.cpp:432), evaluated without
+// early exit.
 __device__ __forceinline__ bool in_query_nb(const GatherLane& L, const float4 pp) {
     const bool box = (L.bbmin.x <= pp.x) & (pp.x <= L.bbmax.x) & (L.bbmin.y <= pp.y) & (pp.y <= L.bbmax.y) &
                      (L.bbmin.z <= pp.z) & (pp.z <= L.bbmax.z);
     const f3 d = mk(pp.x, pp.y, pp.z) - L.c;
     return box & (norm2(d) < L.R2);
-}
-
-// The reference's two neighbour tests for one candidate: closed box
-// (kdtree.h:78-88) then the strict radius (SPPM.cpp:432).
-__device__ __forceinline__ bool in_query(const GatherLane& L, const float4 pp) {
-    if (L.bbmin.x > pp.x || L.bbmax.x < pp.x) return false;
-    if (L.bbmin.y > pp.y || L.bbmax.y < pp.y) return false;
-    if (L.bbmin.z > pp.z || L.bbmax.z < pp.z) return false;
-    const f3 d = mk(pp.x, pp.y, pp.z) - L.c;
-    return norm2(d) < L.R2;
 }
 
 // Persistent waves over the chunk list.  Per staged batch of 64 photons every
@@ -1220,25 +1212,11 @@ __device__ __forceinline__ void gather_chunks(HitDev H, SharedDev SH, GridDev G,
                     s_flux[lane] = G.flux[q];
                     s_dir[lane] = G.dir[q];
                 }
+                if (pdep) s_cnt[lane] = 0;
                 __syncthreads();
                 const int n = (int)((qa + s1 - base) < 64 ? (qa + s1 - base) : 64);
                 int nh = 0;
-                if (pdep) {
-                    // also count, per staged photon, the hit points that gather it
-                    // (photon jj + q is tested by the lanes of phase q)
-                    for (int jj = 0; jj < n; jj += GATHER_PHASES) {
-                        const int j = jj + ph;
-                        const bool hit = j < n && mine && in_query(L, s_pos[j]);
-                        if (hit) {
-                            s_hit[nh * 64 + lane] = (unsigned char)j;
-                            ++nh;
-                        }
-                        const unsigned long long m = __ballot(hit);
-                        if (lane < GATHER_PHASES && jj + lane < n)
-                            s_cnt[jj + lane] = __popcll((m >> (lane * GATHER_GROUP)) &
-                                                        (GATHER_GROUP == 64 ? ~0ull : ((1ull << GATHER_GROUP) - 1ull)));
-                    }
-                } else if (mine) {
+                if (mine) {
                     // four of this lane's staged photons per step, no branches: the list
                     // slot nh is written every time and kept only when the photon passes
                     // (nh <= the photons tested so far, so the slot stays below 64)
@@ -1299,6 +1277,8 @@ __device__ __forceinline__ void gather_chunks(HitDev H, SharedDev SH, GridDev G,
                         b = gather_brdf(L.kind, fp);
                     }
                     L.add = L.add + ((L.wgt * b) * mk(pf.x, pf.y, pf.z)) * __builtin_amdgcn_rcpf(pf.w);
+                    // guided emission: per staged photon, the hit points that gather it
+                    if (pdep) atomicAdd(&s_cnt[j], 1);
                 }
                 if (pdep) {
                     // training signal of the emission proposal: hit points reached x photon power
@@ -1819,6 +1799,17 @@ __global__ void k_guide_deposit(const int* __restrict__ off, const int* __restri
     const size_t slot = (size_t)(step & 1) * (size_t)N + i;
     if (ratio) d = d * d * ratio[slot];
     if (d > 0.0f && d < 3.0e38f) atomicAdd(&hist[path_cell[slot]], d);
+}
+// The offsets and stored count of this pass's photons (dst: N + 1 entries),
+// for readers that run while the next pass stages its own.
+__global__ void k_copy_offsets(const int* __restrict__ off, const int* __restrict__ nstored, int N,
+                               int* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) dst[i] = off[i];
+    if (i == N) dst[N] = *nstored;
+}
+void launch_copy_offsets(const int* off, const int* nstored, int N, int* dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_offsets, dim3((N + 1 + 255) / 256), dim3(256), 0, st, off, nstored, N, dst);
 }
 void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_f32_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
