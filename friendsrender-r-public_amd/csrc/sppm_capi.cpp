@@ -1412,8 +1412,8 @@ static int read_pass_stats(sppm_ctx* c, sppm_pass_stats* stats) {
 
 int sppm_pass(sppm_ctx* c, sppm_pass_stats* stats) {
     // passes in a row: this photon pass need not wait for the previous
-    // gather (guided emission samples the proposal published by the previous
-    // build, not the one the previous pass trains)
+    // build or gather (guided emission samples the proposal published right
+    // before the previous build, not the one the previous pass trains)
     const bool lead = c && c->lead_ok;
     int rc = check_own_pass(c, "sppm_pass");
     if (rc) return rc;
