@@ -53,6 +53,59 @@ def test_closest_hit_bit_exact(width, which, builder, box_scene, full_scene_smal
     np.testing.assert_array_equal(vg[hit], vo[hit])
 
 
+def small_scene(case):
+    """The light quad plus a few triangles: random ones (2, 3, 5, 9, 33 in
+    all), 64 coincident copies of one triangle (every centroid extent zero:
+    the SAH build splits at the middle), or a 4 x 4 grid of coplanar quads."""
+    attrs = [dict(name="white", kind=scenes.MAT_DIFFUSE, albedo=(0.7, 0.7, 0.7), is_sppm=1, if_opaque=1),
+             dict(name="light", kind=scenes.MAT_DIFFUSE, albedo=(0.78, 0.78, 0.78), radiance=(15.0, 15.0, 15.0),
+                  is_sppm=1, if_opaque=1)]
+    b = scenes._Builder()
+    scenes.light_quad(b, 1)
+    rng = np.random.default_rng(11)
+    z3 = [(0.0, 0.0, 1.0)] * 3
+    t3 = [(0.0, 0.0, 0.0)] * 3
+    if case == "coincident":
+        tri = rng.uniform(0.2, 0.8, size=(3, 3))
+        for _ in range(64):
+            b.tri(tri, z3, t3, 0)
+    elif case == "grid":
+        for i in range(4):
+            for j in range(4):
+                x0, z0 = 0.1 + 0.2 * i, 0.1 + 0.2 * j
+                b.quad((x0, 0.3, z0), (x0 + 0.2, 0.3, z0), (x0 + 0.2, 0.3, z0 + 0.2), (x0, 0.3, z0 + 0.2),
+                       (0, 1, 0), 0)
+    else:
+        for _ in range(int(case) - 2):
+            c = rng.uniform(0.1, 0.9, size=3)
+            b.tri(c + rng.uniform(-0.15, 0.15, size=(3, 3)), z3, t3, 0)
+    return b.scene(attrs)
+
+
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+@pytest.mark.parametrize("width", [2, 4, 8])
+@pytest.mark.parametrize("case", ["2", "3", "5", "9", "33", "coincident", "grid"])
+def test_closest_hit_small_and_degenerate_scenes(case, width, builder):
+    """The wide BVH (binary SAH or LBVH tree, collapsed by least SAH cost) on
+    scenes of a few triangles and on degenerate ones: closest hits bit-exact
+    against brute force."""
+    scene = small_scene(case)
+    say(f"closest hit, small scene {case} ({scene.nfacets} triangles), width={width} builder={builder}")
+    org, d = random_rays(scene, 4000, seed=len(case) * 13 + width)
+    os_ = pyoracle.OracleScene(scene)
+    fo, to, uo, vo = os_.closest_hit(org, d, mode=0)
+    flags = width | (sppm_amd.BVH_LBVH if builder == "lbvh" else 0)
+    r = sppm_amd.Renderer(scene, scenes.pinhole_camera(8, 8), 0.9, 0.015, 64, bvh_width=flags)
+    fg, tg, ug, vg = r.trace(org, d)
+    r.close()
+    np.testing.assert_array_equal(fg, fo)
+    hit = fo >= 0
+    assert hit.any()
+    np.testing.assert_array_equal(tg[hit], to[hit])
+    np.testing.assert_array_equal(ug[hit], uo[hit])
+    np.testing.assert_array_equal(vg[hit], vo[hit])
+
+
 def run_parity(scene, cam, alpha, r0, N, passes, width, label, record=True, nee=sppm_amd.NEE_OFF, guide=None,
                max_pairs=0):
     """record: True lists from the recording gather kernel, RECORD_PAIRS the
