@@ -1191,17 +1191,18 @@ static int read_counters(sppm_ctx* c, Counters& h) {
     return SPPM_OK;
 }
 // beside_gather: the photon pass runs beside the previous pass's gather and
-// takes 9/16 of the path-kernel grid, so the gather's kernels find free wave
+// takes 10/16 of the path-kernel grid, so the gather's kernels find free wave
 // slots instead of waiting for the persistent photon waves to drain.  On the
-// full-material workload (1M photons, 1024^2) a pass takes 4.35 ms with
-// 9/16, 4.38-4.79 with the full grid, 4.36-4.83 with 12/16, 5.43 with 6/16.
+// full-material workload (1M photons, 1024^2) a pass takes 3.95 ms with
+// 10/16, 4.03 with 9/16, 4.29 with 8/16, 4.04-4.58 with 12/16 (two steady
+// states) and 4.52 with the full grid, whose photon waves starve the gather.
 static void photon_pass(sppm_ctx* c, int step, const int* step_dev, hipStream_t st, bool beside_gather) {
     RenderBuffers& R = c->rb;
     TraceSpill sp = trace_spill(c);
-    if (beside_gather) sp.blocks = sp.blocks * 9 / 16 > 0 ? sp.blocks * 9 / 16 : 1;
+    if (beside_gather) sp.blocks = sp.blocks * 10 / 16 > 0 ? sp.blocks * 10 / 16 : 1;
     if (R.wave_bounces > 0) {
         int tb = R.wave_blocks;
-        if (beside_gather) tb = tb * 9 / 16 > 0 ? tb * 9 / 16 : 1;
+        if (beside_gather) tb = tb * 10 / 16 > 0 ? tb * 10 / 16 : 1;
         launch_photon_wave(c->scene.dev, R.slots, R.N, step, step_dev, R.wave, R.wave_bounces, R.wave_lanes, tb, sp,
                            c->guided ? &c->emit : nullptr, R.flags, photon_rays(R), st);
     } else {
