@@ -1194,8 +1194,9 @@ static int read_counters(sppm_ctx* c, Counters& h) {
 // takes 10/16 of the path-kernel grid, so the gather's kernels find free wave
 // slots instead of waiting for the persistent photon waves to drain.  On the
 // full-material workload (1M photons, 1024^2) a pass takes 3.95 ms with
-// 10/16, 4.03 with 9/16, 4.29 with 8/16, 4.04-4.58 with 12/16 (two steady
-// states) and 4.52 with the full grid, whose photon waves starve the gather.
+// 10/16, 4.03 with 9/16, 4.29 with 8/16, 3.94-4.47 with 11/16 and 4.04-4.58
+// with 12/16 (two steady states each), and 4.52 with the full grid, whose
+// photon waves starve the gather.
 static void photon_pass(sppm_ctx* c, int step, const int* step_dev, hipStream_t st, bool beside_gather) {
     RenderBuffers& R = c->rb;
     TraceSpill sp = trace_spill(c);
