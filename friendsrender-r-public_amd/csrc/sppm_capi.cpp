@@ -149,7 +149,7 @@ struct RenderBuffers {
                                // [1], [2]: the same counted on the photon and eye streams of a replay
     int* path_cell = nullptr;  // guided emission: emission cell per path, 2 x N (block = step parity)
     float* path_ratio = nullptr;  // guided emission: q / p_plain per path, like path_cell
-    float* pdep = nullptr;     // guided emission: deposited luminance per flattened photon
+    unsigned long long* pdep = nullptr;  // guided emission: deposited luminance per flattened photon (guide_fix)
     // guided emission: this pass's photon offsets and stored count (N + 1),
     // copied by the build for the gather's deposits and the training records,
     // which run while the next pass stages its photons into off / nstored
@@ -2227,7 +2227,7 @@ static std::vector<CkptSection> ckpt_sections(sppm_ctx* c) {
         v.push_back({g.m, sizeof(float) * g.nparams});
         v.push_back({g.v, sizeof(float) * g.nparams});
         v.push_back({g.hist, sizeof(float) * GUIDE_CELLS});
-        v.push_back({g.hpass, sizeof(float) * GUIDE_CELLS});
+        v.push_back({g.hpass, sizeof(unsigned long long) * GUIDE_CELLS});
         v.push_back({g.eprob, sizeof(float) * GUIDE_CELLS});
         v.push_back({g.ecdf, sizeof(float) * GUIDE_CELLS});
         v.push_back({g.scal, sizeof(float) * 8});

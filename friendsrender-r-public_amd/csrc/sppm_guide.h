@@ -74,6 +74,16 @@ struct GuideEmit {
     const float* pc_cum;       // inclusive cumulative plain probability within the cell
 };
 
+// The guided training signal is summed in 64-bit fixed point (integer
+// atomics), so its sums, and with them the trained proposal, do not depend on
+// the order in which waves add: 2^24 per unit, each addition clamped to 1e9.
+constexpr float GUIDE_FIX_SCALE = 16777216.0f;
+constexpr float GUIDE_FIX_INV = 5.9604644775390625e-08f;
+constexpr float GUIDE_FIX_MAX = 1.0e9f;
+__host__ __device__ inline unsigned long long guide_fix(float v) {
+    return (unsigned long long)((v < GUIDE_FIX_MAX ? v : GUIDE_FIX_MAX) * GUIDE_FIX_SCALE);
+}
+
 struct GuideBuffers {
     GuideNet net;
     int nparams = 0;
@@ -95,7 +105,7 @@ struct GuideBuffers {
     float* eprob = nullptr;
     float* ecdf = nullptr;
     float* hist = nullptr;      // training histogram (decayed sum of the passes' deposits)
-    float* hpass = nullptr;     // this pass's deposit histogram
+    unsigned long long* hpass = nullptr;  // deposits since the last fold (fixed point, guide_fix)
     float* part = nullptr;      // per-block gradient partials: GUIDE_BLOCKS * (nparams + 1)
     float* part2 = nullptr;     // per-block (max, sumexp), scan block sums, histogram partials
     float* scal = nullptr;      // [0] lse, [1] sum of hist, [2] loss, [3] Adam steps taken

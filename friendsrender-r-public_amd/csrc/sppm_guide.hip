@@ -531,11 +531,11 @@ __global__ void k_guide_adam(float* __restrict__ theta, float* __restrict__ m, f
     theta[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + 1e-8f);
 }
 
-__global__ void k_guide_fold(float* __restrict__ hist, float* __restrict__ hpass, float decay) {
+__global__ void k_guide_fold(float* __restrict__ hist, unsigned long long* __restrict__ hpass, float decay) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= GUIDE_CELLS) return;
-    hist[k] = decay * hist[k] + hpass[k];
-    hpass[k] = 0.0f;
+    hist[k] = decay * hist[k] + (float)hpass[k] * GUIDE_FIX_INV;
+    hpass[k] = 0ull;
 }
 
 __global__ void k_guide_tick(float* __restrict__ scal) {
@@ -729,7 +729,8 @@ void guide_rms_target(GuideBuffers& g, hipStream_t st) {
 
 void guide_clear_hist(GuideBuffers& g, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hist, GUIDE_CELLS);
-    hipLaunchKernelGGL(k_guide_zero, dim3(GUIDE_CELLS / 256), dim3(256), 0, st, g.hpass, GUIDE_CELLS);
+    hipLaunchKernelGGL(k_guide_zero, dim3(2 * GUIDE_CELLS / 256), dim3(256), 0, st, reinterpret_cast<float*>(g.hpass),
+                       2 * GUIDE_CELLS);
 }
 
 void guide_fold_hist(GuideBuffers& g, float decay, hipStream_t st) {

@@ -71,13 +71,15 @@ void launch_photon(const SceneDev& S, const SlotsDev& slots, int N, int step, co
                    unsigned long long* rays, hipStream_t st);
 void launch_add_i32(int* p, int v, hipStream_t st);
 void launch_zero_i32(int* p, int n, hipStream_t st);
-void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st);
+// pdep: per flattened photon, the luminance its gathering hit points received (fixed point, guide_fix)
+void launch_deposit_clear(unsigned long long* pdep, const int* nstored, hipStream_t st);
 // dst[0 .. N-1] = off, dst[N] = *nstored
 void launch_copy_offsets(const int* off, const int* nstored, int N, int* dst, hipStream_t st);
 // step: the pass step, or *step_dev + 1 when step_dev is given (graph replays)
 // ratio (optional): q / p_plain per path; with it the deposit is d^2 ratio
-void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell,
-                          const float* ratio, int N, int step, const int* step_dev, float* hist, hipStream_t st);
+void launch_guide_deposit(const int* off, const int* nstored, const unsigned long long* pdep, const int* path_cell,
+                          const float* ratio, int N, int step, const int* step_dev, unsigned long long* hist,
+                          hipStream_t st);
 // nee: SPPM_NEE_* (the compiled-out next-event estimation of getHitpointInfo)
 void launch_eye(const SceneDev& S, const CamDev& C, const HitDev& H, int step, const int* step_dev,
                 const TraceSpill& sp, const uint32_t* words, uint32_t* flags, unsigned long long* rays, int* queue,
@@ -139,7 +141,8 @@ struct PairRecord {
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
-                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags,
+                            unsigned long long* pdep,
                             const PairRecord& rec, hipStream_t st);
 // The same in three parts: planning (hit-point groups, tiles, chunk lists;
 // returns the group order), the chunk kernel, and the per-group sums.
@@ -149,7 +152,8 @@ const uint32_t* launch_gather_plan(const HitDev& H, const SharedDev& SH, const G
                                    int* counts, Counters* ctr, const PairRecord& rec, hipStream_t st);
 void launch_gather_run(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                        const uint32_t* order, const GroupBuffers& gb, const TileRange* tiles, const int* chunkp,
-                       const int* chunk_off, float4* partial, float* pdep, const PairRecord& rec, hipStream_t st);
+                       const int* chunk_off, float4* partial, unsigned long long* pdep, const PairRecord& rec,
+                       hipStream_t st);
 void launch_gather_finish(const HitDev& H, const SharedDev& SH, int np, const uint32_t* order, const GroupBuffers& gb,
                           float alpha, const int* chunk_off, const float4* partial, int* counts,
                           unsigned long long* rstat, Counters* ctr, uint32_t* flags, hipStream_t st);
@@ -201,7 +205,7 @@ struct GatherJob {
     const int* chunkp;
     int* queue;
     float4* partial;
-    float* pdep;
+    unsigned long long* pdep;
     PairRecord rec;
 };
 // width: the scenes' common BVH width; sp: a spill area for sp.blocks blocks

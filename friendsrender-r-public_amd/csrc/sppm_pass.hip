@@ -1154,7 +1154,7 @@ __device__ __forceinline__ void gather_chunks(HitDev H, SharedDev SH, GridDev G,
                                               const int* __restrict__ nvalidp, const TileRange* __restrict__ tiles,
                                               int ntiles, const int* __restrict__ chunk_off,
                                               const int* __restrict__ chunkp, int* __restrict__ queue,
-                                              float4* __restrict__ partial, float* __restrict__ pdep,
+                                              float4* __restrict__ partial, unsigned long long* __restrict__ pdep,
                                               PairRecord rec) {
     // LDS of the loop (one wave per block; each kernel that inlines this
     // function gets its own arrays)
@@ -1286,7 +1286,7 @@ __device__ __forceinline__ void gather_chunks(HitDev H, SharedDev SH, GridDev G,
                     if (lane < n && s_cnt[lane] > 0) {
                         const float4 pf = s_flux[lane];
                         const float d = (float)s_cnt[lane] * (0.2126f * pf.x + 0.7152f * pf.y + 0.0722f * pf.z) / pf.w;
-                        if (d > 0.0f && d < 3.0e38f) atomicAdd(&pdep[__float_as_int(s_dir[lane].w)], d);
+                        if (d > 0.0f && d < 3.0e38f) atomicAdd(&pdep[__float_as_int(s_dir[lane].w)], guide_fix(d));
                     }
                 }
                 __syncthreads();
@@ -1316,7 +1316,7 @@ __global__ void __launch_bounds__(64) k_gather_chunks(HitDev H, SharedDev SH, Gr
                                                       const TileRange* __restrict__ tiles, int ntiles,
                                                       const int* __restrict__ chunk_off, const int* __restrict__ chunkp,
                                                       int* __restrict__ queue, float4* __restrict__ partial,
-                                                      float* __restrict__ pdep, PairRecord rec) {
+                                                      unsigned long long* __restrict__ pdep, PairRecord rec) {
     gather_chunks(H, SH, G, gp, order, nvalidp, tiles, ntiles, chunk_off, chunkp, queue, partial, pdep, rec);
 }
 
@@ -1683,7 +1683,8 @@ const uint32_t* launch_gather_plan(const HitDev& H, const SharedDev& SH, const G
 }
 void launch_gather_run(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                        const uint32_t* order, const GroupBuffers& gb, const TileRange* tiles, const int* chunkp,
-                       const int* chunk_off, float4* partial, float* pdep, const PairRecord& rec, hipStream_t st) {
+                       const int* chunk_off, float4* partial, unsigned long long* pdep, const PairRecord& rec,
+                       hipStream_t st) {
     hipLaunchKernelGGL(k_gather_chunks, dim3(GATHER_WAVES), dim3(64), 0, st, H, SH, G, gp, order, gb.nvalid, tiles,
                        gather_group_count(np), chunk_off, chunkp, gb.chunkq, partial, pdep, rec);
 }
@@ -1698,7 +1699,8 @@ void launch_gather_finish(const HitDev& H, const SharedDev& SH, int np, const ui
 void launch_gather_balanced(const HitDev& H, const SharedDev& SH, const GridDev& G, const GridParams* gp, int np,
                             const float* bmin, const float* ext, const GroupBuffers& gb, float alpha, TileRange* tiles,
                             int cap, int* chunkp, int* nch, int* chunk_off, int* scan_tmp, float4* partial,
-                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags, float* pdep,
+                            int* counts, unsigned long long* rstat, Counters* ctr, uint32_t* flags,
+                            unsigned long long* pdep,
                             const PairRecord& rec, hipStream_t st) {
     const uint32_t* order = launch_gather_plan(H, SH, G, gp, np, bmin, ext, gb, tiles, cap, chunkp, nch, chunk_off,
                                                scan_tmp, counts, ctr, rec, st);
@@ -1775,10 +1777,6 @@ namespace sppm {
 // Deposit per flattened photon -> per path -> the histogram of
 // the paths' emission cells (float atomics: the summation order, and with it
 // the last bits of the histogram, vary from run to run).
-__global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n) {
-    const int cnt = *n;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) p[k] = 0.0f;
-}
 // Per path: its flattened photons' deposits (offsets of this pass's build;
 // the path's last photon ends at the next path's offset, the last path's at
 // the stored count) into the histogram cell it was emitted from (the block of
@@ -1787,18 +1785,23 @@ __global__ void k_zero_f32_dev(float* __restrict__ p, const int* __restrict__ n)
 // ratio (optional, q / p_plain per path): deposit d^2 q / p instead of d (the
 // RMS target's histogram).
 __global__ void k_guide_deposit(const int* __restrict__ off, const int* __restrict__ nstored,
-                                const float* __restrict__ pdep, const int* __restrict__ path_cell,
+                                const unsigned long long* __restrict__ pdep, const int* __restrict__ path_cell,
                                 const float* __restrict__ ratio, int N, int step_arg,
-                                const int* __restrict__ step_dev, float* __restrict__ hist) {
+                                const int* __restrict__ step_dev, unsigned long long* __restrict__ hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int step = step_dev ? *step_dev + 1 : step_arg;
     const int o = off[i], e = i + 1 < N ? off[i + 1] : *nstored;
-    float d = 0.0f;
-    for (int k = o; k < e; ++k) d += pdep[k];
+    unsigned long long sum = 0;
+    for (int k = o; k < e; ++k) sum += pdep[k];
+    float d = (float)sum * GUIDE_FIX_INV;
     const size_t slot = (size_t)(step & 1) * (size_t)N + i;
     if (ratio) d = d * d * ratio[slot];
-    if (d > 0.0f && d < 3.0e38f) atomicAdd(&hist[path_cell[slot]], d);
+    if (d > 0.0f && d < 3.0e38f) atomicAdd(&hist[path_cell[slot]], guide_fix(d));
+}
+__global__ void k_zero_u64_dev(unsigned long long* __restrict__ p, const int* __restrict__ n) {
+    const int cnt = *n;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) p[k] = 0ull;
 }
 // The offsets and stored count of this pass's photons (dst: N + 1 entries),
 // for readers that run while the next pass stages its own.
@@ -1811,11 +1814,12 @@ __global__ void k_copy_offsets(const int* __restrict__ off, const int* __restric
 void launch_copy_offsets(const int* off, const int* nstored, int N, int* dst, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_offsets, dim3((N + 1 + 255) / 256), dim3(256), 0, st, off, nstored, N, dst);
 }
-void launch_deposit_clear(float* pdep, const int* nstored, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_f32_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
+void launch_deposit_clear(unsigned long long* pdep, const int* nstored, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64_dev, dim3(2048), dim3(256), 0, st, pdep, nstored);
 }
-void launch_guide_deposit(const int* off, const int* nstored, const float* pdep, const int* path_cell,
-                          const float* ratio, int N, int step, const int* step_dev, float* hist, hipStream_t st) {
+void launch_guide_deposit(const int* off, const int* nstored, const unsigned long long* pdep, const int* path_cell,
+                          const float* ratio, int N, int step, const int* step_dev, unsigned long long* hist,
+                          hipStream_t st) {
     hipLaunchKernelGGL(k_guide_deposit, dim3((N + 255) / 256), dim3(256), 0, st, off, nstored, pdep, path_cell, ratio,
                        N, step, step_dev, hist);
 }

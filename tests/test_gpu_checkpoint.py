@@ -86,6 +86,42 @@ def test_resume_restores_the_proposal(box_scene):
     r2.close()
 
 
+def test_guided_resume_equals_uninterrupted(box_scene):
+    """Guided renders are bit-reproducible (fixed-point training sums), so a
+    render checkpointed after 3 passes and resumed for 4 more equals 7
+    straight passes: image, proposal and histogram, with deposits pending
+    between two trainings at the checkpoint (training every 2nd pass)."""
+    cam = scenes.pinhole_camera(48, 40)
+    guide = dict(mix_plain=0.5, learning_rate=1e-2, train_steps=1, seed=4, history_decay=0.95, train_interval=2,
+                 target=sppm_amd.GUIDE_TARGET_RMS)
+
+    def new():
+        r = sppm_amd.Renderer(box_scene, cam, 0.7, 0.03, 20000, bvh_width=4)
+        r.guide_set(True, **guide)
+        return r
+    straight, first = new(), new()
+    for _ in range(7):
+        straight.pass_(stats=False)
+    for _ in range(3):
+        first.pass_(stats=False)
+    ckpt = first.checkpoint()
+    first.close()
+    resumed = new()
+    resumed.restore(ckpt)
+    for _ in range(4):
+        resumed.pass_(stats=False)
+    straight.sync()
+    resumed.sync()
+    same_render(straight, resumed)
+    ps, hs = straight.guide_read()
+    pr, hr = resumed.guide_read()
+    np.testing.assert_array_equal(pr, ps)
+    np.testing.assert_array_equal(hr, hs)
+    np.testing.assert_array_equal(resumed.guide_weights(), straight.guide_weights())
+    straight.close()
+    resumed.close()
+
+
 def test_checkpoint_of_another_setup_is_refused(box_scene):
     a = sppm_amd.Renderer(box_scene, scenes.pinhole_camera(32, 32), 0.9, 0.015, 5000, bvh_width=4)
     a.pass_(stats=False)
